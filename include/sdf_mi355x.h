@@ -1,0 +1,132 @@
+/*
+ * sdf_mi355x.h -- C ABI of the MI355X signed-distance-field library.
+ *
+ * Drop-in boundary for the SDF hot path of jezekon/stl2sdf.jl.  The Julia
+ * package has no FFI of its own; each entry point below replaces one Julia
+ * function of the reference and keeps its argument meaning and array
+ * layouts, so a ccall binding (or any other FFI) can sit directly behind the
+ * reference's function names.
+ *
+ * Conventions (identical to the reference's in-memory layouts):
+ *   X        3 x nnp doubles, column-major (x,y,z of vertex 1, then vertex 2 ...)
+ *            (TriangularMesh.X, src/MeshGrid/MeshInformations.jl:58)
+ *   IEN      3 x nel int64, column-major, 1-based vertex indices
+ *            (TriangularMesh.IEN, MeshInformations.jl:59)
+ *   grid     aabb_min[3], cell size h, N[3] cells per axis; (N+1) points per
+ *            axis, linear index i + (N1+1)*(j + (N2+1)*k), 0-based here
+ *            (Grid, src/MeshGrid/Grid.jl:1-33, 82-96)
+ *   points   3 x npts doubles or NULL; NULL means the grid's own lattice
+ *            (generateGridPoints).  A points array equal to the lattice is
+ *            detected and takes the lattice path.
+ *   signs    inside = +1, outside = -1 (RaycastingSignDetection.jl:285)
+ *
+ * Every call returns an sdf_status: 0 on success, < 0 on error, with a
+ * message from sdf_last_error() (thread-local).  Nothing aborts or throws
+ * across the ABI.  Host-pointer entry points are synchronous: outputs are
+ * written before they return and no host pointer is retained.  Entry points
+ * with a _device suffix take device pointers, are asynchronous on the
+ * context's stream, and need sdf_ctx_synchronize() before the caller reads.
+ * A context serialises its own calls; distinct contexts may be used from
+ * different threads.
+ */
+#ifndef SDF_MI355X_H
+#define SDF_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t sdf_status;
+typedef struct sdf_ctx sdf_ctx;
+
+enum {
+  SDF_OK = 0,
+  SDF_ERR_ARG = -1,      /* invalid argument (sizes, indices, NULL) */
+  SDF_ERR_HIP = -2,      /* HIP runtime failure (no device, OOM, ...) */
+  SDF_ERR_LIMIT = -3,    /* an internal capacity was exceeded */
+  SDF_ERR_INTERNAL = -4
+};
+
+/* ABI version (major*100 + minor). */
+int32_t sdf_abi_version(void);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *sdf_last_error(void);
+
+/* Context: one HIP device, one stream, cached device buffers.
+ * No reference counterpart (the reference has no device state). */
+sdf_status sdf_ctx_create(int32_t device, sdf_ctx **out);
+sdf_status sdf_ctx_destroy(sdf_ctx *ctx);
+/* Run on a caller-owned hipStream_t (NULL: back to the context's own). */
+sdf_status sdf_ctx_set_stream(sdf_ctx *ctx, void *hip_stream);
+sdf_status sdf_ctx_synchronize(sdf_ctx *ctx);
+/* Stage timers (hipEvents) of the last call, when enabled: names are
+ * written as one ';'-separated string, milliseconds into ms[0..n). */
+sdf_status sdf_ctx_enable_timing(sdf_ctx *ctx, int32_t on);
+sdf_status sdf_ctx_stage_times(sdf_ctx *ctx, char *names, int32_t names_cap,
+                               double *ms, int32_t cap, int32_t *n);
+/* Counters of the last call: [0] points left undecided by the axis rays,
+ * [1] triangles listed in distance tiles, [2] axis-line hits,
+ * [3] device bytes held by the context. */
+sdf_status sdf_ctx_counters(sdf_ctx *ctx, int64_t *out, int32_t cap,
+                            int32_t *n);
+
+/* Grid(X_min, X_max, N_max, margin) -- src/MeshGrid/Grid.jl:9-33. */
+sdf_status sdf_grid_setup(const double xmin[3], const double xmax[3],
+                          int64_t n_max, int64_t margin, double aabb_min[3],
+                          double aabb_max[3], int64_t N[3], double *cell_size,
+                          int64_t *ngp);
+
+/* evalDistancesOnTriMesh(mesh, grid, points) -> dist
+ * -- src/SignedDistances/dfOnTriangularMesh.jl:19-209.
+ * dist_out[npts]: exact point-triangle distance clamped at h (min(d, h)),
+ * FP32 arithmetic on lattice-relative coordinates, FP64 output.  The
+ * reference's closest-point matrix xp is not produced. */
+sdf_status sdf_eval_distances(sdf_ctx *ctx, const double *X, int64_t nnp,
+                              const int64_t *IEN, int64_t nel,
+                              const double aabb_min[3], double h,
+                              const int64_t N[3], const double *points,
+                              int64_t npts, double *dist_out);
+
+/* raycast_sign_detection(mesh, grid, points; min_rays, max_rays,
+ * confidence_threshold, use_winding_fallback) -> (signs, confidences)
+ * -- src/SignedDistances/RaycastingSignDetection.jl:337-404.
+ * Bit-identical decisions: FP64 Moller-Trumbore in the reference's operation
+ * order, eps 1e-8, sorted-t dedup, 6 -> 30 adaptive vote, winding fallback. */
+sdf_status sdf_raycast_signs(sdf_ctx *ctx, const double *X, int64_t nnp,
+                             const int64_t *IEN, int64_t nel,
+                             const double aabb_min[3], double h,
+                             const int64_t N[3], const double *points,
+                             int64_t npts, int32_t min_rays, int32_t max_rays,
+                             double confidence_threshold,
+                             int32_t use_winding_fallback, double *signs_out,
+                             double *conf_out);
+
+/* The stl_to_sdf distance + sign + combine steps (src/stl_to_sdf.jl:101-119):
+ * sdf_out[ngp] = evalDistancesOnTriMesh(...) .* raycast_sign_detection(...)
+ * with default ray options, over the grid's lattice. */
+sdf_status sdf_signed_distance(sdf_ctx *ctx, const double *X, int64_t nnp,
+                               const int64_t *IEN, int64_t nel,
+                               const double aabb_min[3], double h,
+                               const int64_t N[3], double *sdf_out);
+
+/* Device-resident form of the same steps: the mesh is given as device
+ * pointers (same layouts) and is kept by the context until the next
+ * sdf_mesh_set_device call; sdf_signed_distance_device writes ngp doubles
+ * to a device buffer, asynchronously on the context stream. */
+sdf_status sdf_mesh_set_device(sdf_ctx *ctx, const double *d_X, int64_t nnp,
+                               const int64_t *d_IEN, int64_t nel);
+sdf_status sdf_signed_distance_device(sdf_ctx *ctx, const double aabb_min[3],
+                                      double h, const int64_t N[3],
+                                      double *d_sdf_out);
+
+/* The 36 prioritised ray directions (generate_ray_directions_prioritized,
+ * RaycastingSignDetection.jl:162-227), 3 doubles each. */
+sdf_status sdf_ray_directions(double *dirs_out /* 108 */);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SDF_MI355X_H */

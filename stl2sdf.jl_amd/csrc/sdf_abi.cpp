@@ -1,0 +1,562 @@
+// C ABI of the MI355X SDF library (declared in include/sdf_mi355x.h).
+//
+// Orchestration of one call on the lattice path:
+//   gather   1-based IEN + X -> per-triangle FP64 vertices
+//   bin      triangles -> 8x4x2 point tiles within h; axis-line hit lists
+//   fused    per tile: exact distance (LDS-staged triangles) + six axis rays
+//            + combine; undecided points appended to a pending list
+//   resolve  adaptive vote (rays 7..30) and winding fallback for pending
+#include <math.h>
+#include <string.h>
+
+#include <exception>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "sdf_internal.h"
+#include "sdf_mi355x.h"
+
+// Host arithmetic that restates reference formulas (grid setup, ray
+// directions) must round every operation separately.
+#pragma clang fp contract(off)
+
+using sdf::Error;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <typename F>
+sdf_status guarded(F &&f) {
+  try {
+    f();
+    g_last_error.clear();
+    return SDF_OK;
+  } catch (const Error &e) {
+    g_last_error = e.msg;
+    return e.code;
+  } catch (const std::exception &e) {
+    g_last_error = std::string("internal error: ") + e.what();
+    return SDF_ERR_INTERNAL;
+  } catch (...) {
+    g_last_error = "internal error";
+    return SDF_ERR_INTERNAL;
+  }
+}
+
+[[noreturn]] void fail(int code, const std::string &msg) {
+  throw Error{code, msg};
+}
+
+// generate_ray_directions_prioritized (RaycastingSignDetection.jl:162-227):
+// 6 axes, 12 face diagonals, 8 body diagonals, each v * inv(norm(v)); then
+// 10 golden-angle directions [r cos(theta), r sin(theta), z], not normalised.
+void ray_directions(double *d) {
+  static const double raw[26][3] = {
+      {1, 0, 0},   {0, 1, 0},   {0, 0, 1},   {-1, 0, 0},  {0, -1, 0},
+      {0, 0, -1},  {1, 1, 0},   {1, -1, 0},  {-1, 1, 0},  {-1, -1, 0},
+      {1, 0, 1},   {1, 0, -1},  {-1, 0, 1},  {-1, 0, -1}, {0, 1, 1},
+      {0, 1, -1},  {0, -1, 1},  {0, -1, -1}, {1, 1, 1},   {1, 1, -1},
+      {1, -1, 1},  {1, -1, -1}, {-1, 1, 1},  {-1, 1, -1}, {-1, -1, 1},
+      {-1, -1, -1}};
+  for (int i = 0; i < 26; i++) {
+    double s = raw[i][0] * raw[i][0];
+    s = s + raw[i][1] * raw[i][1];
+    s = s + raw[i][2] * raw[i][2];
+    double inv = 1.0 / sqrt(s);
+    for (int c = 0; c < 3; c++) d[3 * i + c] = raw[i][c] * inv;
+  }
+  const double phi = (1.0 + sqrt(5.0)) / 2.0;
+  const double two_pi = 6.283185307179586;  // Float64(2pi)
+  for (int i = 1; i <= 10; i++) {
+    double x = (double)i * phi;
+    double theta = two_pi * fmod(x, 1.0);
+    double z = 1.0 - 2.0 * ((double)i / 10.0);
+    double r = sqrt(1.0 - z * z);
+    double *o = d + 3 * (25 + i);
+    o[0] = r * cos(theta);
+    o[1] = r * sin(theta);
+    o[2] = z;
+  }
+}
+
+}  // namespace
+
+struct sdf_ctx {
+  int device = 0;
+  hipStream_t own = nullptr;
+  hipStream_t stream = nullptr;
+  std::mutex mu;
+  sdf::Arena ar;
+  // device mesh
+  sdf::Tri64 *tri = nullptr;
+  int64_t nel = 0, nnp = 0;
+  bool mesh_ready = false;
+  // small device words: [0] error flags, [1] pending count
+  uint32_t *d_words = nullptr;
+  // timing
+  bool timing = false;
+  std::vector<hipEvent_t> ev;
+  std::vector<std::string> ev_name;
+  int nev = 0;
+  int64_t counters[4] = {0, 0, 0, 0};
+
+  void mark(const char *name) {
+    if (!timing) return;
+    if (nev >= (int)ev.size()) {
+      hipEvent_t e;
+      SDF_HIP(hipEventCreate(&e));
+      ev.push_back(e);
+      ev_name.push_back("");
+    }
+    SDF_HIP(hipEventRecord(ev[nev], stream));
+    ev_name[nev] = name;
+    nev++;
+  }
+};
+
+namespace {
+
+void use_device(sdf_ctx *c) { SDF_HIP(hipSetDevice(c->device)); }
+
+sdf::GridP make_grid(const double aabb_min[3], double h, const int64_t N[3]) {
+  if (!aabb_min || !N) fail(SDF_ERR_ARG, "grid: NULL aabb_min or N");
+  if (!(h > 0.0) || !isfinite(h)) fail(SDF_ERR_ARG, "grid: cell size must be > 0");
+  for (int d = 0; d < 3; d++) {
+    if (N[d] < 0 || N[d] + 1 > (int64_t)0x7FFFFFFF)
+      fail(SDF_ERR_ARG, "grid: N out of range");
+    if (!isfinite(aabb_min[d])) fail(SDF_ERR_ARG, "grid: aabb_min not finite");
+  }
+  sdf::GridP g;
+  g.ox = aabb_min[0];
+  g.oy = aabb_min[1];
+  g.oz = aabb_min[2];
+  g.h = h;
+  g.nx = (int32_t)(N[0] + 1);
+  g.ny = (int32_t)(N[1] + 1);
+  g.nz = (int32_t)(N[2] + 1);
+  g.pad_ = 0;
+  return g;
+}
+
+int64_t grid_points(const sdf::GridP &g) {
+  return (int64_t)g.nx * g.ny * g.nz;
+}
+
+uint32_t read_word(sdf_ctx *c, int i) {
+  uint32_t v = 0;
+  SDF_HIP(hipMemcpyAsync(&v, c->d_words + i, sizeof(v), hipMemcpyDeviceToHost,
+                         c->stream));
+  SDF_HIP(hipStreamSynchronize(c->stream));
+  return v;
+}
+
+void reset_words(sdf_ctx *c) { sdf::launch_fill_u32(c->stream, c->d_words, 4, 0u); }
+
+void check_err_flags(sdf_ctx *c) {
+  uint32_t e = read_word(c, 0);
+  if (e & sdf::kErrBadIndex)
+    fail(SDF_ERR_ARG, "IEN holds a vertex index outside 1..nnp");
+  if (e & sdf::kErrHitOverflow)
+    fail(SDF_ERR_LIMIT, "a ray crossed more triangles than the hit buffer holds");
+  if (e & sdf::kErrStackOverflow)
+    fail(SDF_ERR_LIMIT, "BVH traversal stack overflow");
+  if (e & sdf::kErrPendingOverflow)
+    fail(SDF_ERR_INTERNAL, "pending list overflow");
+}
+
+// Mesh from device X / IEN (layouts of TriangularMesh) into per-triangle
+// FP64 vertices held by the context.
+void set_mesh_device(sdf_ctx *c, const double *dX, int64_t nnp,
+                     const int64_t *dIEN, int64_t nel) {
+  if (nel < 0 || nnp < 0) fail(SDF_ERR_ARG, "negative mesh size");
+  if (nel > 0 && (!dX || !dIEN)) fail(SDF_ERR_ARG, "NULL mesh arrays");
+  if (nel > 0x7FFFFFFFll) fail(SDF_ERR_ARG, "too many triangles");
+  c->tri = c->ar.get<sdf::Tri64>("tri", nel + 1);
+  c->nel = nel;
+  c->nnp = nnp;
+  c->mark("gather");
+  sdf::launch_gather_tris(c->stream, dX, nnp, dIEN, nel, c->tri, c->d_words);
+  c->mesh_ready = true;
+}
+
+void set_mesh_host(sdf_ctx *c, const double *X, int64_t nnp, const int64_t *IEN,
+                   int64_t nel) {
+  if (nel < 0 || nnp < 0) fail(SDF_ERR_ARG, "negative mesh size");
+  if (nel > 0 && (!X || !IEN)) fail(SDF_ERR_ARG, "NULL mesh arrays");
+  double *dX = c->ar.get<double>("in_X", 3 * nnp + 3);
+  int64_t *dI = c->ar.get<int64_t>("in_IEN", 3 * nel + 3);
+  if (nnp > 0)
+    SDF_HIP(hipMemcpyAsync(dX, X, (size_t)(3 * nnp) * sizeof(double),
+                           hipMemcpyHostToDevice, c->stream));
+  if (nel > 0)
+    SDF_HIP(hipMemcpyAsync(dI, IEN, (size_t)(3 * nel) * sizeof(int64_t),
+                           hipMemcpyHostToDevice, c->stream));
+  set_mesh_device(c, dX, nnp, dI, nel);
+}
+
+sdf::RayParams ray_params(int32_t min_rays, int32_t max_rays, double thr,
+                          int32_t use_winding) {
+  if (min_rays < 1 || min_rays > sdf::kNumDirs)
+    fail(SDF_ERR_ARG, "min_rays must be in 1..36");
+  if (max_rays < 0) fail(SDF_ERR_ARG, "max_rays must be >= 0");
+  sdf::RayParams rp;
+  ray_directions(rp.dirs);
+  rp.min_rays = min_rays;
+  rp.max_rays = max_rays;
+  rp.conf_thr = thr;
+  rp.use_winding = use_winding ? 1 : 0;
+  return rp;
+}
+
+// Lattice path: binning, fused kernel, adaptive finisher.
+void run_lattice(sdf_ctx *c, const sdf::GridP &g, bool do_dist, bool do_sign,
+                 const sdf::RayParams &rp, sdf::FusedOut out) {
+  if (!c->mesh_ready) fail(SDF_ERR_ARG, "no mesh");
+  int64_t npts = grid_points(g);
+  sdf::Bins bins;
+  c->mark("bin_tiles");
+  if (do_dist) sdf::build_tile_bins(c->stream, c->ar, c->tri, c->nel, g, g.h, bins);
+  c->mark("bin_lines");
+  if (do_sign) sdf::build_line_hits(c->stream, c->ar, c->tri, c->nel, g, bins);
+  uint32_t cap = (uint32_t)(npts < (1 << 22) ? (npts > 0 ? npts : 1) : (1 << 22));
+  sdf::Pending *pend = c->ar.get<sdf::Pending>("pend", cap);
+  c->mark("fused");
+  sdf::launch_fused_lattice(c->stream, g, c->tri, bins, do_dist, do_sign, out,
+                            pend, c->d_words + 1, cap, c->d_words);
+  uint32_t np = 0;
+  if (do_sign) {
+    np = read_word(c, 1);
+    if (np > cap) {
+      // Rerun with a list large enough for every undecided point.  Any
+      // other flag (bad index) is reported first.
+      uint32_t e = read_word(c, 0) & ~sdf::kErrPendingOverflow;
+      if (e) check_err_flags(c);
+      cap = np;
+      pend = c->ar.get<sdf::Pending>("pend", cap);
+      sdf::launch_fill_u32(c->stream, c->d_words, 4, 0u);
+      sdf::launch_fused_lattice(c->stream, g, c->tri, bins, do_dist, do_sign,
+                                out, pend, c->d_words + 1, cap, c->d_words);
+      np = read_word(c, 1);
+    }
+  }
+  c->mark("resolve");
+  if (np > 0)
+    sdf::launch_resolve_brute(c->stream, g, nullptr, c->tri, c->nel, rp, pend,
+                              np, out, c->d_words);
+  c->mark("end");
+  c->counters[0] = np;
+  c->counters[1] = bins.ntile_pairs;
+  c->counters[2] = bins.nline_hits;
+  c->counters[3] = (int64_t)c->ar.bytes();
+  check_err_flags(c);
+}
+
+bool is_lattice(const sdf::GridP &g, const double *pts, int64_t npts) {
+  if (!pts) return true;
+  if (npts != grid_points(g)) return false;
+  int64_t a = 0;
+  for (int64_t k = 0; k < g.nz; k++) {
+    double z = sdf::lattice_coord(g.oz, g.h, k);
+    for (int64_t j = 0; j < g.ny; j++) {
+      double y = sdf::lattice_coord(g.oy, g.h, j);
+      for (int64_t i = 0; i < g.nx; i++, a++) {
+        const double *p = pts + 3 * a;
+        if (p[1] != y || p[2] != z || p[0] != sdf::lattice_coord(g.ox, g.h, i))
+          return false;
+      }
+    }
+  }
+  return true;
+}
+
+void begin_call(sdf_ctx *c) {
+  use_device(c);
+  c->nev = 0;
+  reset_words(c);
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t sdf_abi_version(void) { return 100; }
+
+const char *sdf_last_error(void) { return g_last_error.c_str(); }
+
+sdf_status sdf_ctx_create(int32_t device, sdf_ctx **out) {
+  return guarded([&] {
+    if (!out) fail(SDF_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    int n = 0;
+    SDF_HIP(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n)
+      fail(SDF_ERR_ARG, "device " + std::to_string(device) + " not present (" +
+                            std::to_string(n) + " devices)");
+    sdf_ctx *c = new sdf_ctx();
+    c->device = device;
+    try {
+      SDF_HIP(hipSetDevice(device));
+      SDF_HIP(hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking));
+      c->stream = c->own;
+      SDF_HIP(hipMalloc(&c->d_words, 64));
+    } catch (...) {
+      delete c;
+      throw;
+    }
+    *out = c;
+  });
+}
+
+sdf_status sdf_ctx_destroy(sdf_ctx *c) {
+  return guarded([&] {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c->ar.release();
+    for (auto e : c->ev) (void)hipEventDestroy(e);
+    if (c->d_words) (void)hipFree(c->d_words);
+    if (c->own) (void)hipStreamDestroy(c->own);
+    delete c;
+  });
+}
+
+sdf_status sdf_ctx_set_stream(sdf_ctx *c, void *s) {
+  return guarded([&] {
+    if (!c) fail(SDF_ERR_ARG, "ctx is NULL");
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->stream = s ? (hipStream_t)s : c->own;
+  });
+}
+
+sdf_status sdf_ctx_synchronize(sdf_ctx *c) {
+  return guarded([&] {
+    if (!c) fail(SDF_ERR_ARG, "ctx is NULL");
+    use_device(c);
+    SDF_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+sdf_status sdf_ctx_enable_timing(sdf_ctx *c, int32_t on) {
+  return guarded([&] {
+    if (!c) fail(SDF_ERR_ARG, "ctx is NULL");
+    c->timing = on != 0;
+  });
+}
+
+sdf_status sdf_ctx_stage_times(sdf_ctx *c, char *names, int32_t names_cap,
+                               double *ms, int32_t cap, int32_t *n) {
+  return guarded([&] {
+    if (!c) fail(SDF_ERR_ARG, "ctx is NULL");
+    use_device(c);
+    std::string all;
+    int w = 0;
+    if (c->nev > 0) SDF_HIP(hipEventSynchronize(c->ev[c->nev - 1]));
+    for (int i = 0; i + 1 < c->nev; i++) {
+      float t = 0.f;
+      SDF_HIP(hipEventElapsedTime(&t, c->ev[i], c->ev[i + 1]));
+      if (w < cap && ms) ms[w] = t;
+      if (!all.empty()) all += ";";
+      all += c->ev_name[i];
+      w++;
+    }
+    if (names && names_cap > 0) {
+      size_t m = all.size() < (size_t)(names_cap - 1) ? all.size() : (size_t)(names_cap - 1);
+      memcpy(names, all.data(), m);
+      names[m] = 0;
+    }
+    if (n) *n = w;
+  });
+}
+
+sdf_status sdf_ctx_counters(sdf_ctx *c, int64_t *out, int32_t cap, int32_t *n) {
+  return guarded([&] {
+    if (!c) fail(SDF_ERR_ARG, "ctx is NULL");
+    int w = 0;
+    for (int i = 0; i < 4 && i < cap; i++, w++) out[i] = c->counters[i];
+    if (n) *n = w;
+  });
+}
+
+sdf_status sdf_grid_setup(const double xmin[3], const double xmax[3],
+                          int64_t n_max, int64_t margin, double aabb_min[3],
+                          double aabb_max[3], int64_t N[3], double *cell_size,
+                          int64_t *ngp) {
+  return guarded([&] {
+    if (!xmin || !xmax || !aabb_min || !aabb_max || !N || !cell_size || !ngp)
+      fail(SDF_ERR_ARG, "NULL argument");
+    if (n_max < 1) fail(SDF_ERR_ARG, "N_max must be >= 1");
+    // Grid.jl:15-29, operation for operation
+    double ext = xmax[0] - xmin[0];
+    for (int d = 1; d < 3; d++) {
+      double e = xmax[d] - xmin[d];
+      if (e > ext) ext = e;
+    }
+    double h = ext / (double)n_max;
+    double mh = (double)margin * h;
+    int64_t prod = 1;
+    for (int d = 0; d < 3; d++) {
+      double lo = xmin[d] - mh;
+      double hi = xmax[d] + mh;
+      double cells = ceil((hi - lo) / h);
+      N[d] = (int64_t)cells;
+      double nh = (double)N[d] * h;
+      aabb_min[d] = lo;
+      aabb_max[d] = lo + nh;
+      prod *= N[d] + 1;
+    }
+    *cell_size = h;
+    *ngp = prod;
+  });
+}
+
+sdf_status sdf_ray_directions(double *dirs_out) {
+  return guarded([&] {
+    if (!dirs_out) fail(SDF_ERR_ARG, "NULL");
+    ray_directions(dirs_out);
+  });
+}
+
+sdf_status sdf_eval_distances(sdf_ctx *c, const double *X, int64_t nnp,
+                              const int64_t *IEN, int64_t nel,
+                              const double aabb_min[3], double h,
+                              const int64_t N[3], const double *points,
+                              int64_t npts, double *dist_out) {
+  return guarded([&] {
+    if (!c) fail(SDF_ERR_ARG, "ctx is NULL");
+    std::lock_guard<std::mutex> lk(c->mu);
+    sdf::GridP g = make_grid(aabb_min, h, N);
+    if (!points) npts = grid_points(g);
+    if (npts < 0) fail(SDF_ERR_ARG, "npts < 0");
+    if (npts > 0 && !dist_out) fail(SDF_ERR_ARG, "dist_out is NULL");
+    begin_call(c);
+    set_mesh_host(c, X, nnp, IEN, nel);
+    double *d_dist = c->ar.get<double>("out_dist", npts + 1);
+    if (is_lattice(g, points, npts)) {
+      sdf::FusedOut out;
+      out.dist = d_dist;
+      sdf::RayParams rp = ray_params(6, 30, 0.6, 1);
+      run_lattice(c, g, true, false, rp, out);
+    } else {
+      double *d_pts = c->ar.get<double>("in_pts", 3 * npts + 3);
+      SDF_HIP(hipMemcpyAsync(d_pts, points, (size_t)(3 * npts) * sizeof(double),
+                             hipMemcpyHostToDevice, c->stream));
+      c->mark("dist_points");
+      sdf::launch_dist_points_brute(c->stream, d_pts, npts, c->tri, c->nel, h,
+                                    d_dist);
+      c->mark("end");
+      check_err_flags(c);
+    }
+    if (npts > 0)
+      SDF_HIP(hipMemcpyAsync(dist_out, d_dist, (size_t)npts * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
+    SDF_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+sdf_status sdf_raycast_signs(sdf_ctx *c, const double *X, int64_t nnp,
+                             const int64_t *IEN, int64_t nel,
+                             const double aabb_min[3], double h,
+                             const int64_t N[3], const double *points,
+                             int64_t npts, int32_t min_rays, int32_t max_rays,
+                             double conf_thr, int32_t use_winding,
+                             double *signs_out, double *conf_out) {
+  return guarded([&] {
+    if (!c) fail(SDF_ERR_ARG, "ctx is NULL");
+    std::lock_guard<std::mutex> lk(c->mu);
+    sdf::GridP g = make_grid(aabb_min, h, N);
+    if (!points) npts = grid_points(g);
+    if (npts < 0) fail(SDF_ERR_ARG, "npts < 0");
+    if (npts > 0 && (!signs_out || !conf_out))
+      fail(SDF_ERR_ARG, "signs_out / conf_out is NULL");
+    sdf::RayParams rp = ray_params(min_rays, max_rays, conf_thr, use_winding);
+    begin_call(c);
+    set_mesh_host(c, X, nnp, IEN, nel);
+    double *d_s = c->ar.get<double>("out_signs", npts + 1);
+    double *d_c = c->ar.get<double>("out_conf", npts + 1);
+    sdf::FusedOut out;
+    out.signs = d_s;
+    out.conf = d_c;
+    bool lattice = is_lattice(g, points, npts);
+    if (lattice && min_rays == 6 && conf_thr <= 1.0) {
+      run_lattice(c, g, false, true, rp, out);
+    } else {
+      const double *d_pts = nullptr;
+      if (!lattice) {
+        double *p = c->ar.get<double>("in_pts", 3 * npts + 3);
+        SDF_HIP(hipMemcpyAsync(p, points, (size_t)(3 * npts) * sizeof(double),
+                               hipMemcpyHostToDevice, c->stream));
+        d_pts = p;
+      }
+      sdf::Pending *pend = c->ar.get<sdf::Pending>("pend_all", npts + 1);
+      sdf::launch_init_pending_all(c->stream, pend, npts);
+      c->mark("resolve");
+      for (int64_t b = 0; b < npts; b += 0x40000000ll) {
+        uint32_t m = (uint32_t)(npts - b < 0x40000000ll ? npts - b : 0x40000000ll);
+        sdf::launch_resolve_brute(c->stream, g, d_pts, c->tri, c->nel, rp,
+                                  pend + b, m, out, c->d_words);
+      }
+      c->mark("end");
+      check_err_flags(c);
+    }
+    if (npts > 0) {
+      SDF_HIP(hipMemcpyAsync(signs_out, d_s, (size_t)npts * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
+      SDF_HIP(hipMemcpyAsync(conf_out, d_c, (size_t)npts * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
+    }
+    SDF_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+sdf_status sdf_signed_distance(sdf_ctx *c, const double *X, int64_t nnp,
+                               const int64_t *IEN, int64_t nel,
+                               const double aabb_min[3], double h,
+                               const int64_t N[3], double *sdf_out) {
+  return guarded([&] {
+    if (!c) fail(SDF_ERR_ARG, "ctx is NULL");
+    std::lock_guard<std::mutex> lk(c->mu);
+    sdf::GridP g = make_grid(aabb_min, h, N);
+    int64_t npts = grid_points(g);
+    if (npts > 0 && !sdf_out) fail(SDF_ERR_ARG, "sdf_out is NULL");
+    begin_call(c);
+    set_mesh_host(c, X, nnp, IEN, nel);
+    double *d_out = c->ar.get<double>("out_sdf", npts + 1);
+    sdf::FusedOut out;
+    out.sdf = d_out;
+    run_lattice(c, g, true, true, ray_params(6, 30, 0.6, 1), out);
+    if (npts > 0)
+      SDF_HIP(hipMemcpyAsync(sdf_out, d_out, (size_t)npts * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
+    SDF_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+sdf_status sdf_mesh_set_device(sdf_ctx *c, const double *d_X, int64_t nnp,
+                               const int64_t *d_IEN, int64_t nel) {
+  return guarded([&] {
+    if (!c) fail(SDF_ERR_ARG, "ctx is NULL");
+    std::lock_guard<std::mutex> lk(c->mu);
+    begin_call(c);
+    set_mesh_device(c, d_X, nnp, d_IEN, nel);
+    check_err_flags(c);
+  });
+}
+
+sdf_status sdf_signed_distance_device(sdf_ctx *c, const double aabb_min[3],
+                                      double h, const int64_t N[3],
+                                      double *d_sdf_out) {
+  return guarded([&] {
+    if (!c) fail(SDF_ERR_ARG, "ctx is NULL");
+    std::lock_guard<std::mutex> lk(c->mu);
+    sdf::GridP g = make_grid(aabb_min, h, N);
+    if (grid_points(g) > 0 && !d_sdf_out) fail(SDF_ERR_ARG, "d_sdf_out is NULL");
+    begin_call(c);
+    sdf::FusedOut out;
+    out.sdf = d_sdf_out;
+    run_lattice(c, g, true, true, ray_params(6, 30, 0.6, 1), out);
+  });
+}
+
+}  // extern "C"

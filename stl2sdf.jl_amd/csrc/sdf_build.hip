@@ -1,0 +1,488 @@
+// Build-side kernels: triangle gather, exclusive scan, triangle binning into
+// distance tiles, and axis-line hit lists for the sign vote.
+//
+// Binning replaces the reference's per-point KD-tree candidate search
+// (dfOnTriangularMesh.jl:27-35,79-94) for lattice points: every triangle is
+// listed in each 8x4x2 point tile that lies within the clamp radius h of it,
+// so a tile's list holds every triangle that can change min(d, h) for any of
+// its points.  Axis-line hit lists replace the per-point BVH ray traversal
+// (RaycastingSignDetection.jl:114-136) for the six axis rays.
+#include "sdf_internal.h"
+
+namespace sdf {
+
+namespace {
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanBlock = kScanThreads * kScanItems;
+
+__global__ void fill_u32_kernel(uint32_t *p, int64_t n, uint32_t v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+__global__ void fill_u64_kernel(uint64_t *p, int64_t n, uint64_t v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+__global__ void fill_f64_kernel(double *p, int64_t n, double v) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+inline int grid_for(int64_t n, int threads) {
+  int64_t b = (n + threads - 1) / threads;
+  if (b < 1) b = 1;
+  if (b > 65536) b = 65536;
+  return (int)b;
+}
+
+// One thread per triangle: 1-based IEN -> 9 doubles.  Out-of-range indices
+// raise kErrBadIndex and read vertex 1 instead (never out of bounds).
+__global__ void gather_tris_kernel(const double *__restrict__ X, int64_t nnp,
+                                   const int64_t *__restrict__ IEN, int64_t nel,
+                                   Tri64 *__restrict__ tri, uint32_t *err) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nel;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    Tri64 r;
+    for (int c = 0; c < 3; c++) {
+      int64_t id = IEN[3 * t + c];
+      if (id < 1 || id > nnp) {
+        atomicOr(err, kErrBadIndex);
+        id = 1;
+      }
+      r.v[3 * c + 0] = X[3 * (id - 1) + 0];
+      r.v[3 * c + 1] = X[3 * (id - 1) + 1];
+      r.v[3 * c + 2] = X[3 * (id - 1) + 2];
+    }
+    tri[t] = r;
+  }
+}
+
+// Block-level exclusive scan.  in has n_in entries (missing entries are 0),
+// out gets n_out entries; bsum[b] = block total.
+template <typename TIn>
+__global__ __launch_bounds__(kScanThreads) void scan_block_kernel(
+    const TIn *__restrict__ in, int64_t n_in, uint64_t *__restrict__ out,
+    int64_t n_out, uint64_t *__restrict__ bsum) {
+  __shared__ uint64_t buf[2][kScanThreads];
+  const int tid = threadIdx.x;
+  const int64_t base = (int64_t)blockIdx.x * kScanBlock + (int64_t)tid * kScanItems;
+  uint64_t v[kScanItems];
+  uint64_t tot = 0;
+#pragma unroll
+  for (int q = 0; q < kScanItems; q++) {
+    int64_t i = base + q;
+    uint64_t x = (i < n_in) ? (uint64_t)in[i] : 0ull;
+    v[q] = tot;
+    tot += x;
+  }
+  buf[0][tid] = tot;
+  __syncthreads();
+  int src = 0;
+  for (int off = 1; off < kScanThreads; off <<= 1) {
+    uint64_t x = buf[src][tid];
+    if (tid >= off) x += buf[src][tid - off];
+    buf[src ^ 1][tid] = x;
+    src ^= 1;
+    __syncthreads();
+  }
+  uint64_t incl = buf[src][tid];
+  uint64_t excl = incl - tot;
+#pragma unroll
+  for (int q = 0; q < kScanItems; q++) {
+    int64_t i = base + q;
+    if (i < n_out) out[i] = excl + v[q];
+  }
+  if (tid == kScanThreads - 1) bsum[blockIdx.x] = incl;
+}
+
+__global__ void scan_add_kernel(uint64_t *__restrict__ out, int64_t n,
+                                const uint64_t *__restrict__ boff) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] += boff[i / kScanBlock];
+}
+
+template <typename TIn>
+void scan_rec(hipStream_t s, Arena &ar, const TIn *in, int64_t n_in,
+              uint64_t *out, int64_t n_out, int level) {
+  int64_t nb = (n_out + kScanBlock - 1) / kScanBlock;
+  if (nb < 1) nb = 1;
+  uint64_t *bsum = ar.get<uint64_t>("scan_bsum" + std::to_string(level), nb + 1);
+  hipLaunchKernelGGL(scan_block_kernel<TIn>, dim3((unsigned)nb),
+                     dim3(kScanThreads), 0, s, in, n_in, out, n_out, bsum);
+  SDF_HIP(hipGetLastError());
+  if (nb > 1) {
+    uint64_t *boff = ar.get<uint64_t>("scan_boff" + std::to_string(level), nb + 1);
+    scan_rec<uint64_t>(s, ar, bsum, nb, boff, nb, level + 1);
+    hipLaunchKernelGGL(scan_add_kernel, dim3(grid_for(n_out, 256)), dim3(256),
+                       0, s, out, n_out, boff);
+    SDF_HIP(hipGetLastError());
+  }
+}
+
+__device__ inline int64_t upper_bound_u64(const uint64_t *a, int64_t n,
+                                          uint64_t x) {
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    int64_t mid = (lo + hi) >> 1;
+    if (a[mid] <= x)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+__device__ inline void tri_minmax(const Tri64 &t, double *mn, double *mx) {
+  for (int d = 0; d < 3; d++) {
+    double a = t.v[d], b = t.v[3 + d], c = t.v[6 + d];
+    mn[d] = fmin(a, fmin(b, c));
+    mx[d] = fmax(a, fmax(b, c));
+  }
+}
+
+// Inclusive lattice index range [lo, hi] of points within [a, b] on one axis.
+__device__ inline void point_range(double a, double b, double o, double h,
+                                   int32_t n, int32_t *lo, int32_t *hi) {
+  double fa = ceil((a - o) / h - 1e-7);
+  double fb = floor((b - o) / h + 1e-7);
+  if (fa < 0.0) fa = 0.0;
+  if (fb > (double)(n - 1)) fb = (double)(n - 1);
+  if (fa > fb || !(fa == fa) || !(fb == fb)) {
+    *lo = 1;
+    *hi = 0;
+    return;
+  }
+  *lo = (int32_t)fa;
+  *hi = (int32_t)fb;
+}
+
+struct TileRange {
+  int32_t lo[3], hi[3];  // tile index range (inclusive)
+  int64_t count;
+};
+
+__device__ inline TileRange tri_tile_range(const Tri64 &t, const GridP &g,
+                                           double R) {
+  double mn[3], mx[3];
+  tri_minmax(t, mn, mx);
+  const double o[3] = {g.ox, g.oy, g.oz};
+  const int32_t n[3] = {g.nx, g.ny, g.nz};
+  const int32_t tsz[3] = {kTileX, kTileY, kTileZ};
+  TileRange r;
+  r.count = 1;
+  for (int d = 0; d < 3; d++) {
+    int32_t plo, phi;
+    point_range(mn[d] - R, mx[d] + R, o[d], g.h, n[d], &plo, &phi);
+    if (plo > phi) {
+      r.count = 0;
+      r.lo[d] = 1;
+      r.hi[d] = 0;
+      continue;
+    }
+    r.lo[d] = plo / tsz[d];
+    r.hi[d] = phi / tsz[d];
+    r.count *= (int64_t)(r.hi[d] - r.lo[d] + 1);
+  }
+  return r;
+}
+
+// Does the tile's point box, grown by R on every side, meet the triangle's
+// plane?  A point within R of the triangle has its nearest triangle point
+// inside that grown box, so rejecting on this test never drops a needed
+// triangle.
+__device__ inline bool tile_plane_test(const Tri64 &t, const GridP &g,
+                                       const TileDims &td, int32_t tx,
+                                       int32_t ty, int32_t tz, double R) {
+  double e1[3] = {t.v[3] - t.v[0], t.v[4] - t.v[1], t.v[5] - t.v[2]};
+  double e2[3] = {t.v[6] - t.v[0], t.v[7] - t.v[1], t.v[8] - t.v[2]};
+  double n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2],
+                 e1[0] * e2[1] - e1[1] * e2[0]};
+  double nn = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  if (!(nn > 0.0)) return true;
+  int32_t i0 = tx * kTileX, j0 = ty * kTileY, k0 = tz * kTileZ;
+  int32_t i1 = min(i0 + kTileX - 1, g.nx - 1), j1 = min(j0 + kTileY - 1, g.ny - 1),
+          k1 = min(k0 + kTileZ - 1, g.nz - 1);
+  double lo[3] = {lattice_coord(g.ox, g.h, i0), lattice_coord(g.oy, g.h, j0),
+                  lattice_coord(g.oz, g.h, k0)};
+  double hi[3] = {lattice_coord(g.ox, g.h, i1), lattice_coord(g.oy, g.h, j1),
+                  lattice_coord(g.oz, g.h, k1)};
+  double proj = 0.0, dist = 0.0, mag = 0.0;
+  for (int d = 0; d < 3; d++) {
+    double c = 0.5 * (lo[d] + hi[d]);
+    double e = 0.5 * (hi[d] - lo[d]) + R;
+    proj += fabs(n[d]) * e;
+    dist += n[d] * (c - t.v[d]);
+    mag += fabs(n[d]) * (fabs(c) + fabs(t.v[d]));
+  }
+  return fabs(dist) <= proj * (1.0 + 1e-9) + 1e-12 * mag;
+}
+
+__global__ void tile_count_tri_kernel(const Tri64 *__restrict__ tri,
+                                      int64_t nel, GridP g, double R,
+                                      uint32_t *__restrict__ cnt) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nel;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    TileRange r = tri_tile_range(tri[t], g, R);
+    cnt[t] = (uint32_t)(r.count > 0xFFFFFFFFll ? 0xFFFFFFFFll : r.count);
+  }
+}
+
+// mode 0: count per tile; mode 1: fill tile lists.
+__global__ void tile_pairs_kernel(const Tri64 *__restrict__ tri, int64_t nel,
+                                  GridP g, TileDims td, double R,
+                                  const uint64_t *__restrict__ tri_off,
+                                  uint64_t total, int mode,
+                                  uint32_t *__restrict__ tile_cnt,
+                                  const uint64_t *__restrict__ tile_off,
+                                  uint32_t *__restrict__ cursor,
+                                  uint32_t *__restrict__ tile_tris) {
+  for (uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; p < total;
+       p += (uint64_t)gridDim.x * blockDim.x) {
+    int64_t t = upper_bound_u64(tri_off, nel + 1, p) - 1;
+    uint64_t local = p - tri_off[t];
+    Tri64 tr = tri[t];
+    TileRange r = tri_tile_range(tr, g, R);
+    int64_t wx = r.hi[0] - r.lo[0] + 1, wy = r.hi[1] - r.lo[1] + 1;
+    int32_t tx = r.lo[0] + (int32_t)(local % wx);
+    int32_t ty = r.lo[1] + (int32_t)((local / wx) % wy);
+    int32_t tz = r.lo[2] + (int32_t)(local / (wx * wy));
+    if (!tile_plane_test(tr, g, td, tx, ty, tz, R)) continue;
+    int64_t tile = ((int64_t)tz * td.ty + ty) * td.tx + tx;
+    if (mode == 0) {
+      atomicAdd(&tile_cnt[tile], 1u);
+    } else {
+      uint32_t slot = atomicAdd(&cursor[tile], 1u);
+      tile_tris[tile_off[tile] + slot] = (uint32_t)t;
+    }
+  }
+}
+
+// ---- axis-line hits ----
+
+struct LineRange {
+  int32_t lo[3][2], hi[3][2];  // per axis: the two transverse index ranges
+  int64_t cnt[3];
+};
+
+__device__ inline LineRange tri_line_range(const Tri64 &t, const GridP &g) {
+  double mn[3], mx[3];
+  tri_minmax(t, mn, mx);
+  const double o[3] = {g.ox, g.oy, g.oz};
+  const int32_t n[3] = {g.nx, g.ny, g.nz};
+  int32_t plo[3], phi[3];
+  for (int d = 0; d < 3; d++) {
+    // pad by a small fraction of a cell; the exact test decides
+    double pad = 1e-6 * g.h;
+    point_range(mn[d] - pad, mx[d] + pad, o[d], g.h, n[d], &plo[d], &phi[d]);
+  }
+  LineRange r;
+  for (int ax = 0; ax < 3; ax++) {
+    int b = ax == 0 ? 1 : 0;
+    int c = ax == 2 ? 1 : 2;
+    r.lo[ax][0] = plo[b];
+    r.hi[ax][0] = phi[b];
+    r.lo[ax][1] = plo[c];
+    r.hi[ax][1] = phi[c];
+    int64_t w0 = phi[b] - plo[b] + 1, w1 = phi[c] - plo[c] + 1;
+    r.cnt[ax] = (w0 > 0 && w1 > 0) ? w0 * w1 : 0;
+  }
+  return r;
+}
+
+__global__ void line_count_tri_kernel(const Tri64 *__restrict__ tri,
+                                      int64_t nel, GridP g,
+                                      uint32_t *__restrict__ cnt) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nel;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    LineRange r = tri_line_range(tri[t], g);
+    int64_t c = r.cnt[0] + r.cnt[1] + r.cnt[2];
+    cnt[t] = (uint32_t)(c > 0xFFFFFFFFll ? 0xFFFFFFFFll : c);
+  }
+}
+
+__global__ void line_pairs_kernel(const Tri64 *__restrict__ tri, int64_t nel,
+                                  GridP g, const uint64_t *__restrict__ tri_off,
+                                  uint64_t total, int64_t base1, int64_t base2,
+                                  int mode, uint32_t *__restrict__ line_cnt,
+                                  const uint64_t *__restrict__ line_off,
+                                  uint32_t *__restrict__ cursor,
+                                  LineHit *__restrict__ hits) {
+  for (uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; p < total;
+       p += (uint64_t)gridDim.x * blockDim.x) {
+    int64_t t = upper_bound_u64(tri_off, nel + 1, p) - 1;
+    uint64_t local = p - tri_off[t];
+    Tri64 tr = tri[t];
+    LineRange r = tri_line_range(tr, g);
+    int ax = 0;
+    while (ax < 2 && local >= (uint64_t)r.cnt[ax]) {
+      local -= r.cnt[ax];
+      ax++;
+    }
+    int64_t w0 = r.hi[ax][0] - r.lo[ax][0] + 1;
+    int32_t a = r.lo[ax][0] + (int32_t)(local % w0);
+    int32_t b = r.lo[ax][1] + (int32_t)(local / w0);
+    // origin: the lattice point with index 0 along the axis
+    double o[3];
+    int64_t line;
+    if (ax == 0) {  // lines over (j, k)
+      o[0] = lattice_coord(g.ox, g.h, 0);
+      o[1] = lattice_coord(g.oy, g.h, a);
+      o[2] = lattice_coord(g.oz, g.h, b);
+      line = (int64_t)b * g.ny + a;
+    } else if (ax == 1) {  // lines over (i, k)
+      o[0] = lattice_coord(g.ox, g.h, a);
+      o[1] = lattice_coord(g.oy, g.h, 0);
+      o[2] = lattice_coord(g.oz, g.h, b);
+      line = base1 + (int64_t)b * g.nx + a;
+    } else {  // lines over (i, j)
+      o[0] = lattice_coord(g.ox, g.h, a);
+      o[1] = lattice_coord(g.oy, g.h, b);
+      o[2] = lattice_coord(g.oz, g.h, 0);
+      line = base2 + (int64_t)b * g.nx + a;
+    }
+    LineHit rec;
+    if (!line_hit_test(o, ax, tr.v, tr.v + 3, tr.v + 6,
+                       mode == 0 ? nullptr : &rec))
+      continue;
+    if (mode == 0) {
+      atomicAdd(&line_cnt[line], 1u);
+    } else {
+      uint32_t slot = atomicAdd(&cursor[line], 1u);
+      hits[line_off[line] + slot] = rec;
+    }
+  }
+}
+
+uint64_t read_u64(hipStream_t s, const uint64_t *d) {
+  uint64_t v = 0;
+  SDF_HIP(hipMemcpyAsync(&v, d, sizeof(v), hipMemcpyDeviceToHost, s));
+  SDF_HIP(hipStreamSynchronize(s));
+  return v;
+}
+
+}  // namespace
+
+void launch_fill_u32(hipStream_t s, uint32_t *p, int64_t n, uint32_t v) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s,
+                     p, n, v);
+  SDF_HIP(hipGetLastError());
+}
+void launch_fill_u64(hipStream_t s, uint64_t *p, int64_t n, uint64_t v) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(fill_u64_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s,
+                     p, n, v);
+  SDF_HIP(hipGetLastError());
+}
+void launch_fill_f64(hipStream_t s, double *p, int64_t n, double v) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(fill_f64_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s,
+                     p, n, v);
+  SDF_HIP(hipGetLastError());
+}
+
+void launch_gather_tris(hipStream_t s, const double *X, int64_t nnp,
+                        const int64_t *IEN, int64_t nel, Tri64 *tri,
+                        uint32_t *err) {
+  if (nel <= 0) return;
+  hipLaunchKernelGGL(gather_tris_kernel, dim3(grid_for(nel, 256)), dim3(256),
+                     0, s, X, nnp, IEN, nel, tri, err);
+  SDF_HIP(hipGetLastError());
+}
+
+void scan_counts(hipStream_t s, Arena &ar, const uint32_t *in, uint64_t *out,
+                 int64_t n) {
+  scan_rec<uint32_t>(s, ar, in, n, out, n + 1, 0);
+}
+
+void build_tile_bins(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
+                     const GridP &g, double radius, Bins &b) {
+  TileDims td = tile_dims(g);
+  double R = radius * (1.0 + 1e-6);
+  uint32_t *tcnt = ar.get<uint32_t>("bin_tri_cnt", nel + 1);
+  uint64_t *toff = ar.get<uint64_t>("bin_tri_off", nel + 2);
+  uint32_t *tile_cnt = ar.get<uint32_t>("bin_tile_cnt", td.ntiles + 1);
+  uint64_t *tile_off = ar.get<uint64_t>("bin_tile_off", td.ntiles + 2);
+  launch_fill_u32(s, tile_cnt, td.ntiles + 1, 0u);
+  uint64_t total = 0;
+  if (nel > 0) {
+    hipLaunchKernelGGL(tile_count_tri_kernel, dim3(grid_for(nel, 256)),
+                       dim3(256), 0, s, tri, nel, g, R, tcnt);
+    SDF_HIP(hipGetLastError());
+    scan_counts(s, ar, tcnt, toff, nel);
+    total = read_u64(s, toff + nel);
+  }
+  if (total > 0) {
+    hipLaunchKernelGGL(tile_pairs_kernel, dim3(grid_for((int64_t)total, 256)),
+                       dim3(256), 0, s, tri, nel, g, td, R, toff, total, 0,
+                       tile_cnt, (const uint64_t *)nullptr, (uint32_t *)nullptr,
+                       (uint32_t *)nullptr);
+    SDF_HIP(hipGetLastError());
+  }
+  scan_counts(s, ar, tile_cnt, tile_off, td.ntiles);
+  uint64_t npairs = read_u64(s, tile_off + td.ntiles);
+  uint32_t *tile_tris = ar.get<uint32_t>("bin_tile_tris", npairs + 1);
+  if (npairs > 0) {
+    uint32_t *cursor = tile_cnt;  // reuse as cursor
+    launch_fill_u32(s, cursor, td.ntiles + 1, 0u);
+    hipLaunchKernelGGL(tile_pairs_kernel, dim3(grid_for((int64_t)total, 256)),
+                       dim3(256), 0, s, tri, nel, g, td, R, toff, total, 1,
+                       (uint32_t *)nullptr, tile_off, cursor, tile_tris);
+    SDF_HIP(hipGetLastError());
+  }
+  b.tile_off = tile_off;
+  b.tile_tris = tile_tris;
+  b.ntile_pairs = (int64_t)npairs;
+}
+
+void build_line_hits(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
+                     const GridP &g, Bins &b) {
+  int64_t nlx = (int64_t)g.ny * g.nz, nly = (int64_t)g.nx * g.nz,
+          nlz = (int64_t)g.nx * g.ny;
+  int64_t nlines = nlx + nly + nlz;
+  b.line_base[0] = 0;
+  b.line_base[1] = nlx;
+  b.line_base[2] = nlx + nly;
+  b.line_base[3] = nlines;
+  uint32_t *tcnt = ar.get<uint32_t>("line_tri_cnt", nel + 1);
+  uint64_t *toff = ar.get<uint64_t>("line_tri_off", nel + 2);
+  uint32_t *lcnt = ar.get<uint32_t>("line_cnt", nlines + 1);
+  uint64_t *loff = ar.get<uint64_t>("line_off", nlines + 2);
+  launch_fill_u32(s, lcnt, nlines + 1, 0u);
+  uint64_t total = 0;
+  if (nel > 0) {
+    hipLaunchKernelGGL(line_count_tri_kernel, dim3(grid_for(nel, 256)),
+                       dim3(256), 0, s, tri, nel, g, tcnt);
+    SDF_HIP(hipGetLastError());
+    scan_counts(s, ar, tcnt, toff, nel);
+    total = read_u64(s, toff + nel);
+  }
+  if (total > 0) {
+    hipLaunchKernelGGL(line_pairs_kernel, dim3(grid_for((int64_t)total, 256)),
+                       dim3(256), 0, s, tri, nel, g, toff, total, nlx,
+                       nlx + nly, 0, lcnt, (const uint64_t *)nullptr,
+                       (uint32_t *)nullptr, (LineHit *)nullptr);
+    SDF_HIP(hipGetLastError());
+  }
+  scan_counts(s, ar, lcnt, loff, nlines);
+  uint64_t nhits = read_u64(s, loff + nlines);
+  LineHit *hits = ar.get<LineHit>("line_hits", nhits + 1);
+  if (nhits > 0) {
+    launch_fill_u32(s, lcnt, nlines + 1, 0u);
+    hipLaunchKernelGGL(line_pairs_kernel, dim3(grid_for((int64_t)total, 256)),
+                       dim3(256), 0, s, tri, nel, g, toff, total, nlx,
+                       nlx + nly, 1, (uint32_t *)nullptr, loff, lcnt, hits);
+    SDF_HIP(hipGetLastError());
+  }
+  b.line_off = loff;
+  b.line_hits = hits;
+  b.nline_hits = (int64_t)nhits;
+}
+
+}  // namespace sdf

@@ -1,0 +1,257 @@
+// Shared device/host helpers for the MI355X SDF kernels.
+//
+// Exactness rules used throughout:
+//  * The sign predicates (Moller-Trumbore, RaycastingSignDetection.jl:64-106)
+//    are evaluated in FP64 with the reference's operation order and with FP
+//    contraction switched off, so every product and sum is rounded exactly
+//    like the reference's separate operations.
+//  * Lattice coordinates are AABB_min[d] + h * i in FP64 without contraction
+//    (src/MeshGrid/Grid.jl:90).
+//  * Distances are computed in FP32 on coordinates taken relative to a
+//    nearby lattice point (the difference itself formed in FP64), which keeps
+//    the absolute error far below 1e-6 x bbox diagonal.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sdf {
+
+constexpr double kRayEps = 1e-8;  // RaycastingSignDetection.jl:118,132
+constexpr int kNumDirs = 36;      // RaycastingSignDetection.jl:162-227
+
+// Lattice description: points per axis (N+1), origin, spacing.
+struct GridP {
+  double ox, oy, oz;
+  double h;
+  int32_t nx, ny, nz;
+  int32_t pad_;
+};
+
+// Distance tiles: 8 x 4 x 2 lattice points = one 64-lane wavefront.
+constexpr int kTileX = 8, kTileY = 4, kTileZ = 2;
+constexpr int kTilePts = kTileX * kTileY * kTileZ;
+
+struct TileDims {
+  int32_t tx, ty, tz;  // tiles per axis
+  int64_t ntiles;
+};
+
+__host__ __device__ inline TileDims tile_dims(const GridP &g) {
+  TileDims t;
+  t.tx = (g.nx + kTileX - 1) / kTileX;
+  t.ty = (g.ny + kTileY - 1) / kTileY;
+  t.tz = (g.nz + kTileZ - 1) / kTileZ;
+  t.ntiles = (int64_t)t.tx * t.ty * t.tz;
+  return t;
+}
+
+// Lattice coordinate AABB_min[d] + h * i (Grid.jl:90), no contraction.
+__host__ __device__ inline double lattice_coord(double o, double h, int64_t i) {
+#pragma clang fp contract(off)
+  double hi = h * (double)i;
+  return o + hi;
+}
+
+struct Vec3d {
+  double x, y, z;
+};
+
+__host__ __device__ inline Vec3d lattice_point(const GridP &g, int64_t i,
+                                               int64_t j, int64_t k) {
+  Vec3d p;
+  p.x = lattice_coord(g.ox, g.h, i);
+  p.y = lattice_coord(g.oy, g.h, j);
+  p.z = lattice_coord(g.oz, g.h, k);
+  return p;
+}
+
+// ---------------------------------------------------------------------
+// Reference-order FP64 vector helpers (no contraction).
+// dot(a,b) = ((a1*b1 + a2*b2) + a3*b3); cross as LinearAlgebra.cross.
+// ---------------------------------------------------------------------
+__host__ __device__ inline double dot3_ref(const double *a, const double *b) {
+#pragma clang fp contract(off)
+  double s = a[0] * b[0];
+  s = s + a[1] * b[1];
+  s = s + a[2] * b[2];
+  return s;
+}
+
+__host__ __device__ inline void cross3_ref(const double *a, const double *b,
+                                           double *c) {
+#pragma clang fp contract(off)
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// Moller-Trumbore exactly as RaycastingSignDetection.jl:64-106 with eps=1e-8.
+// Returns true and t on a hit with t > eps.
+__host__ __device__ inline bool mt_ref(const double *o, const double *d,
+                                       const double *v1, const double *v2,
+                                       const double *v3, double *t_out) {
+#pragma clang fp contract(off)
+  double e1[3] = {v2[0] - v1[0], v2[1] - v1[1], v2[2] - v1[2]};
+  double e2[3] = {v3[0] - v1[0], v3[1] - v1[1], v3[2] - v1[2]};
+  double hv[3];
+  cross3_ref(d, e2, hv);
+  double a = dot3_ref(e1, hv);
+  if (fabs(a) < kRayEps) return false;
+  double f = 1.0 / a;
+  double s[3] = {o[0] - v1[0], o[1] - v1[1], o[2] - v1[2]};
+  double u = f * dot3_ref(s, hv);
+  if (u < 0.0 || u > 1.0) return false;
+  double q[3];
+  cross3_ref(s, e1, q);
+  double v = f * dot3_ref(d, q);
+  if (v < 0.0 || u + v > 1.0) return false;
+  double t = f * dot3_ref(e2, q);
+  if (!(t > kRayEps)) return false;
+  *t_out = t;
+  return true;
+}
+
+// Axis-line hit record: everything MT needs to evaluate t for any origin on
+// the line (v1, e1, e2 and f = 1/a for the +axis direction).
+struct LineHit {
+  double v1[3];
+  double e1[3];
+  double e2[3];
+  double f;
+};
+
+// u/v/|a| part of MT for the +axis direction on an axis-aligned grid line.
+// For d = +e_axis those quantities do not depend on the origin's coordinate
+// along the axis (the along-axis component only enters through products with
+// an exact zero), so one test per (line, triangle) decides the hit for every
+// lattice point of the line, and for the -axis ray as well (a, f negate
+// exactly; u, v are bitwise equal; t negates exactly).
+__host__ __device__ inline bool line_hit_test(const double *o, int axis,
+                                              const double *v1,
+                                              const double *v2,
+                                              const double *v3, LineHit *rec) {
+#pragma clang fp contract(off)
+  double d[3] = {0.0, 0.0, 0.0};
+  d[axis] = 1.0;
+  double e1[3] = {v2[0] - v1[0], v2[1] - v1[1], v2[2] - v1[2]};
+  double e2[3] = {v3[0] - v1[0], v3[1] - v1[1], v3[2] - v1[2]};
+  double hv[3];
+  cross3_ref(d, e2, hv);
+  double a = dot3_ref(e1, hv);
+  if (fabs(a) < kRayEps) return false;
+  double f = 1.0 / a;
+  double s[3] = {o[0] - v1[0], o[1] - v1[1], o[2] - v1[2]};
+  double u = f * dot3_ref(s, hv);
+  if (u < 0.0 || u > 1.0) return false;
+  double q[3];
+  cross3_ref(s, e1, q);
+  double v = f * dot3_ref(d, q);
+  if (v < 0.0 || u + v > 1.0) return false;
+  if (rec) {
+    for (int c = 0; c < 3; c++) {
+      rec->v1[c] = v1[c];
+      rec->e1[c] = e1[c];
+      rec->e2[c] = e2[c];
+    }
+    rec->f = f;
+  }
+  return true;
+}
+
+// t for origin o on the +axis ray of a recorded line hit (reference order).
+__host__ __device__ inline double line_hit_t(const LineHit &r, const double *o) {
+#pragma clang fp contract(off)
+  double s[3] = {o[0] - r.v1[0], o[1] - r.v1[1], o[2] - r.v1[2]};
+  double q[3];
+  cross3_ref(s, r.e1, q);
+  return r.f * dot3_ref(r.e2, q);
+}
+
+// ---------------------------------------------------------------------
+// Point-triangle squared distance (closest point by Voronoi regions,
+// Ericson, Real-Time Collision Detection 5.1.5).  T = float or double.
+// a, b, c and p share one local origin.
+// ---------------------------------------------------------------------
+template <typename T>
+__host__ __device__ inline T tri_dist2(T px, T py, T pz, T ax, T ay, T az, T bx,
+                                       T by, T bz, T cx, T cy, T cz) {
+  T abx = bx - ax, aby = by - ay, abz = bz - az;
+  T acx = cx - ax, acy = cy - ay, acz = cz - az;
+  T apx = px - ax, apy = py - ay, apz = pz - az;
+  T d1 = abx * apx + aby * apy + abz * apz;
+  T d2 = acx * apx + acy * apy + acz * apz;
+  T qx, qy, qz;
+  if (d1 <= T(0) && d2 <= T(0)) {
+    qx = ax;
+    qy = ay;
+    qz = az;
+  } else {
+    T bpx = px - bx, bpy = py - by, bpz = pz - bz;
+    T d3 = abx * bpx + aby * bpy + abz * bpz;
+    T d4 = acx * bpx + acy * bpy + acz * bpz;
+    T cpx = px - cx, cpy = py - cy, cpz = pz - cz;
+    T d5 = abx * cpx + aby * cpy + abz * cpz;
+    T d6 = acx * cpx + acy * cpy + acz * cpz;
+    T vc = d1 * d4 - d3 * d2;
+    T vb = d5 * d2 - d1 * d6;
+    T va = d3 * d6 - d5 * d4;
+    if (d3 >= T(0) && d4 <= d3) {
+      qx = bx;
+      qy = by;
+      qz = bz;
+    } else if (vc <= T(0) && d1 >= T(0) && d3 <= T(0)) {
+      T v = d1 / (d1 - d3);
+      qx = ax + v * abx;
+      qy = ay + v * aby;
+      qz = az + v * abz;
+    } else if (d6 >= T(0) && d5 <= d6) {
+      qx = cx;
+      qy = cy;
+      qz = cz;
+    } else if (vb <= T(0) && d2 >= T(0) && d6 <= T(0)) {
+      T w = d2 / (d2 - d6);
+      qx = ax + w * acx;
+      qy = ay + w * acy;
+      qz = az + w * acz;
+    } else if (va <= T(0) && (d4 - d3) >= T(0) && (d5 - d6) >= T(0)) {
+      T w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+      qx = bx + w * (cx - bx);
+      qy = by + w * (cy - by);
+      qz = bz + w * (cz - bz);
+    } else {
+      T den = va + vb + vc;
+      if (!(den > T(0) && den - den == T(0))) {
+        // degenerate triangle: nearest of the three edges
+        T best = T(1e30f);
+        const T P[3][3] = {{ax, ay, az}, {bx, by, bz}, {cx, cy, cz}};
+        for (int e = 0; e < 3; e++) {
+          const T *s0 = P[e], *s1 = P[(e + 1) % 3];
+          T ex = s1[0] - s0[0], ey = s1[1] - s0[1], ez = s1[2] - s0[2];
+          T wx = px - s0[0], wy = py - s0[1], wz = pz - s0[2];
+          T l2 = ex * ex + ey * ey + ez * ez;
+          T t = l2 > T(0) ? (wx * ex + wy * ey + wz * ez) / l2 : T(0);
+          t = t < T(0) ? T(0) : (t > T(1) ? T(1) : t);
+          T rx = wx - t * ex, ry = wy - t * ey, rz = wz - t * ez;
+          T dd = rx * rx + ry * ry + rz * rz;
+          best = dd < best ? dd : best;
+        }
+        return best;
+      }
+      T inv = T(1) / den;
+      T v = vb * inv, w = vc * inv;
+      qx = ax + abx * v + acx * w;
+      qy = ay + aby * v + acy * w;
+      qz = az + abz * v + acz * w;
+    }
+  }
+  T rx = px - qx, ry = py - qy, rz = pz - qz;
+  return rx * rx + ry * ry + rz * rz;
+}
+
+// Triangle vertices in FP64, 9 doubles per triangle (v1, v2, v3).
+struct Tri64 {
+  double v[9];
+};
+
+}  // namespace sdf

@@ -1,0 +1,151 @@
+// Host-side interface between the C ABI (sdf_abi.cpp) and the kernel
+// translation units.  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <map>
+#include <string>
+
+#include "sdf_common.h"
+
+namespace sdf {
+
+// Thrown inside the library only; the ABI layer converts it to a status.
+struct Error {
+  int code;
+  std::string msg;
+};
+
+#define SDF_HIP(call)                                                      \
+  do {                                                                     \
+    hipError_t e_ = (call);                                                \
+    if (e_ != hipSuccess)                                                  \
+      throw ::sdf::Error{-2, std::string("HIP error in ") + #call + ": " + \
+                                 hipGetErrorString(e_)};                   \
+  } while (0)
+
+// Grow-only named device buffers owned by a context.
+class Arena {
+ public:
+  ~Arena() { release(); }
+  void *get(const std::string &name, size_t bytes) {
+    Buf &b = bufs_[name];
+    if (b.bytes < bytes) {
+      if (b.p) SDF_HIP(hipFree(b.p));
+      b.p = nullptr;
+      b.bytes = 0;
+      size_t want = bytes < 256 ? 256 : bytes;
+      SDF_HIP(hipMalloc(&b.p, want));
+      b.bytes = want;
+    }
+    return b.p;
+  }
+  template <typename T>
+  T *get(const std::string &name, size_t count) {
+    return static_cast<T *>(get(name, count * sizeof(T)));
+  }
+  void release() {
+    for (auto &kv : bufs_)
+      if (kv.second.p) (void)hipFree(kv.second.p);
+    bufs_.clear();
+  }
+  size_t bytes() const {
+    size_t s = 0;
+    for (auto &kv : bufs_) s += kv.second.bytes;
+    return s;
+  }
+
+ private:
+  struct Buf {
+    void *p = nullptr;
+    size_t bytes = 0;
+  };
+  std::map<std::string, Buf> bufs_;
+};
+
+// Per-point state that the fused kernel could not settle with the six axis
+// rays; finished by the adaptive-vote kernel.
+struct Pending {
+  int64_t idx;
+  double dist;
+  int32_t inside;
+  int32_t used;
+};
+
+// Error flags written by kernels (bit mask).
+enum : uint32_t {
+  kErrBadIndex = 1u,       // IEN entry outside 1..nnp
+  kErrPendingOverflow = 2u,
+  kErrHitOverflow = 4u,    // one ray crossed more triangles than the buffer
+  kErrStackOverflow = 8u,
+};
+
+struct Timer {
+  hipEvent_t a = nullptr, b = nullptr;
+};
+
+// ---------------- kernel launchers (implemented in .hip files) -----------
+// build side
+void launch_fill_u32(hipStream_t s, uint32_t *p, int64_t n, uint32_t v);
+void launch_fill_u64(hipStream_t s, uint64_t *p, int64_t n, uint64_t v);
+void launch_fill_f64(hipStream_t s, double *p, int64_t n, double v);
+void launch_gather_tris(hipStream_t s, const double *X, int64_t nnp,
+                        const int64_t *IEN, int64_t nel, Tri64 *tri,
+                        uint32_t *err);
+// exclusive scan of n uint32 counts into n+1 uint64 offsets (out[n] = total)
+void scan_counts(hipStream_t s, Arena &ar, const uint32_t *in, uint64_t *out,
+                 int64_t n);
+
+struct Bins {
+  // tile binning (distance)
+  uint64_t *tile_off = nullptr;  // ntiles + 1
+  uint32_t *tile_tris = nullptr;
+  // axis-line hits (signs)
+  uint64_t *line_off = nullptr;  // nlines + 1
+  LineHit *line_hits = nullptr;
+  int64_t line_base[4] = {0, 0, 0, 0};
+  int64_t ntile_pairs = 0, nline_hits = 0;
+};
+
+void build_tile_bins(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
+                     const GridP &g, double radius, Bins &b);
+void build_line_hits(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
+                     const GridP &g, Bins &b);
+
+// query side
+struct FusedOut {
+  double *sdf = nullptr;    // dist * sign (nullable)
+  double *dist = nullptr;   // unsigned distance (nullable)
+  double *signs = nullptr;  // +-1 (nullable)
+  double *conf = nullptr;   // confidence (nullable)
+};
+
+void launch_fused_lattice(hipStream_t s, const GridP &g, const Tri64 *tri,
+                          const Bins &b, bool do_dist, bool do_sign,
+                          FusedOut out, Pending *pend, uint32_t *npend,
+                          uint32_t pend_cap, uint32_t *err);
+
+struct RayParams {
+  double dirs[kNumDirs * 3];
+  int32_t min_rays, max_rays;
+  double conf_thr;
+  int32_t use_winding;
+};
+
+// Finish the adaptive vote for pending items (brute-force ray casting over
+// all triangles, one workgroup per item). pts == nullptr: lattice points.
+void launch_resolve_brute(hipStream_t s, const GridP &g, const double *pts,
+                          const Tri64 *tri, int64_t nel, const RayParams &rp,
+                          const Pending *pend, uint32_t npend, FusedOut out,
+                          uint32_t *err);
+
+// Exact unsigned distance for arbitrary points, brute force.
+void launch_dist_points_brute(hipStream_t s, const double *pts, int64_t npts,
+                              const Tri64 *tri, int64_t nel, double h,
+                              double *dist);
+
+void launch_init_pending_all(hipStream_t s, Pending *pend, int64_t npts);
+
+}  // namespace sdf

@@ -1,0 +1,450 @@
+// Query kernels: the fused lattice kernel (distance + six axis rays + combine)
+// and the adaptive-vote finisher for points the axis rays leave undecided.
+//
+// Reference path replaced (jezekon/stl2sdf.jl):
+//   evalDistancesOnTriMesh      src/SignedDistances/dfOnTriangularMesh.jl:19-209
+//   raycast_sign_detection      src/SignedDistances/RaycastingSignDetection.jl:337-404
+//   adaptive_point_in_mesh_raycast / check_ray_inside_bvh  (:234-288, :114-154)
+//   generalized_winding_number  (:295-328)
+//   sdf_dists = dists .* signs  src/stl_to_sdf.jl:119
+#include "sdf_internal.h"
+
+namespace sdf {
+
+namespace {
+
+constexpr int kHitCap = 2048;  // per-ray hit buffer of the finisher (LDS)
+
+__device__ inline void cswap(double &a, double &b) {
+  double lo = fmin(a, b), hi = fmax(a, b);
+  a = lo;
+  b = hi;
+}
+
+// Unique-count of a sorted run of values, reference rule
+// (RaycastingSignDetection.jl:143-150): 1 + #{consecutive gaps > eps}.
+// Returns the parity for the +axis ray (values t > eps) and for the -axis
+// ray (values t < -eps, i.e. -t > eps).
+__device__ inline void axis_pair_parity(const LineHit *__restrict__ hits,
+                                        uint64_t hb, uint64_t he,
+                                        const double *o, int *pos_par,
+                                        int *neg_par) {
+  const int64_t n = (int64_t)(he - hb);
+  if (n <= 0) {
+    *pos_par = 0;
+    *neg_par = 0;
+    return;
+  }
+  const double INF = __builtin_huge_val();
+  if (n <= 8) {
+    double t[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      double x = INF;
+      if (q < n) x = line_hit_t(hits[hb + q], o);
+      t[q] = (x == x) ? x : INF;
+    }
+    // Batcher odd-even merge sort, 8 inputs
+    cswap(t[0], t[1]); cswap(t[2], t[3]); cswap(t[4], t[5]); cswap(t[6], t[7]);
+    cswap(t[0], t[2]); cswap(t[1], t[3]); cswap(t[4], t[6]); cswap(t[5], t[7]);
+    cswap(t[1], t[2]); cswap(t[5], t[6]);
+    cswap(t[0], t[4]); cswap(t[1], t[5]); cswap(t[2], t[6]); cswap(t[3], t[7]);
+    cswap(t[2], t[4]); cswap(t[3], t[5]);
+    cswap(t[1], t[2]); cswap(t[3], t[4]); cswap(t[5], t[6]);
+    int cp = 0, cn = 0;
+    bool hp = false, hn = false;
+    double pp = 0.0, pn = 0.0;
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      double x = t[q];
+      if (x > kRayEps && x < INF) {
+        if (!hp) {
+          cp = 1;
+          hp = true;
+        } else if (x - pp > kRayEps) {
+          cp++;
+        }
+        pp = x;
+      }
+      if (x < -kRayEps) {
+        if (!hn) {
+          cn = 1;
+          hn = true;
+        } else if (x - pn > kRayEps) {
+          cn++;
+        }
+        pn = x;
+      }
+    }
+    *pos_par = cp & 1;
+    *neg_par = cn & 1;
+    return;
+  }
+  // Long lists: per element, find its strict predecessor among the accepted
+  // values (recomputing t), without storing the list.
+  int cp = 0, cn = 0;
+  for (int64_t a = 0; a < n; a++) {
+    double x = line_hit_t(hits[hb + a], o);
+    bool posx = x > kRayEps, negx = x < -kRayEps;
+    if (!posx && !negx) continue;
+    bool dup = false, have = false;
+    double pred = 0.0;
+    for (int64_t b = 0; b < n; b++) {
+      if (b == a) continue;
+      double y = line_hit_t(hits[hb + b], o);
+      bool same_side = posx ? (y > kRayEps) : (y < -kRayEps);
+      if (!same_side) continue;
+      if (y == x && b < a) dup = true;
+      if (y < x && (!have || y > pred)) {
+        pred = y;
+        have = true;
+      }
+    }
+    if (dup) continue;
+    if (!have || x - pred > kRayEps) {
+      if (posx)
+        cp++;
+      else
+        cn++;
+    }
+  }
+  *pos_par = cp & 1;
+  *neg_par = cn & 1;
+}
+
+__global__ __launch_bounds__(kTilePts) void fused_lattice_kernel(
+    GridP g, TileDims td, const Tri64 *__restrict__ tri,
+    const uint64_t *__restrict__ tile_off,
+    const uint32_t *__restrict__ tile_tris,
+    const uint64_t *__restrict__ line_off, const LineHit *__restrict__ hits,
+    int64_t base1, int64_t base2, int do_dist, int do_sign, FusedOut out,
+    Pending *__restrict__ pend, uint32_t *npend, uint32_t cap, uint32_t *err) {
+  __shared__ float s_tri[9][kTilePts];
+  const int64_t tile = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int32_t tx = (int32_t)(tile % td.tx);
+  const int32_t ty = (int32_t)((tile / td.tx) % td.ty);
+  const int32_t tz = (int32_t)(tile / ((int64_t)td.tx * td.ty));
+  const int32_t i0 = tx * kTileX, j0 = ty * kTileY, k0 = tz * kTileZ;
+  const int32_t i = i0 + (lane & (kTileX - 1));
+  const int32_t j = j0 + ((lane / kTileX) & (kTileY - 1));
+  const int32_t k = k0 + lane / (kTileX * kTileY);
+  const bool valid = i < g.nx && j < g.ny && k < g.nz;
+  const Vec3d p = lattice_point(g, i, j, k);
+
+  double dist = g.h;
+  if (do_dist) {
+    const Vec3d T0 = lattice_point(g, i0, j0, k0);
+    const float px = (float)(p.x - T0.x), py = (float)(p.y - T0.y),
+                pz = (float)(p.z - T0.z);
+    float best = __builtin_huge_valf();
+    const uint64_t b = tile_off[tile], e = tile_off[tile + 1];
+    for (uint64_t c = b; c < e; c += kTilePts) {
+      const int n = (int)min((uint64_t)kTilePts, e - c);
+      if (lane < n) {
+        const Tri64 t = tri[tile_tris[c + lane]];
+#pragma unroll
+        for (int v = 0; v < 3; v++) {
+          s_tri[3 * v + 0][lane] = (float)(t.v[3 * v + 0] - T0.x);
+          s_tri[3 * v + 1][lane] = (float)(t.v[3 * v + 1] - T0.y);
+          s_tri[3 * v + 2][lane] = (float)(t.v[3 * v + 2] - T0.z);
+        }
+      }
+      __syncthreads();
+      for (int q = 0; q < n; q++) {
+        float d2 = tri_dist2<float>(px, py, pz, s_tri[0][q], s_tri[1][q],
+                                    s_tri[2][q], s_tri[3][q], s_tri[4][q],
+                                    s_tri[5][q], s_tri[6][q], s_tri[7][q],
+                                    s_tri[8][q]);
+        best = fminf(best, d2);
+      }
+      __syncthreads();
+    }
+    if (best < __builtin_huge_valf()) {
+      double d = sqrt((double)best);
+      dist = d < g.h ? d : g.h;
+    }
+  }
+  if (!valid) return;
+  const int64_t idx = ((int64_t)k * g.ny + j) * g.nx + i;
+
+  double sgn = 1.0, conf = 1.0;
+  bool pending = false;
+  if (do_sign) {
+    const double o[3] = {p.x, p.y, p.z};
+    int inside = 0, a, bpar;
+    int64_t L = (int64_t)k * g.ny + j;
+    axis_pair_parity(hits, line_off[L], line_off[L + 1], o, &a, &bpar);
+    inside += a + bpar;
+    L = base1 + (int64_t)k * g.nx + i;
+    axis_pair_parity(hits, line_off[L], line_off[L + 1], o, &a, &bpar);
+    inside += a + bpar;
+    L = base2 + (int64_t)j * g.nx + i;
+    axis_pair_parity(hits, line_off[L], line_off[L + 1], o, &a, &bpar);
+    inside += a + bpar;
+    if (inside == 6) {
+      sgn = 1.0;
+    } else if (inside == 0) {
+      sgn = -1.0;
+    } else {
+      pending = true;
+      uint32_t slot = atomicAdd(npend, 1u);
+      if (slot < cap) {
+        Pending pr;
+        pr.idx = idx;
+        pr.dist = dist;
+        pr.inside = inside;
+        pr.used = 6;
+        pend[slot] = pr;
+      } else {
+        atomicOr(err, kErrPendingOverflow);
+      }
+    }
+  }
+  if (out.dist) out.dist[idx] = dist;
+  if (!pending) {
+    if (out.sdf) out.sdf[idx] = dist * sgn;
+    if (out.signs) out.signs[idx] = sgn;
+    if (out.conf) out.conf[idx] = conf;
+  }
+}
+
+// Parity of one ray over all triangles, one workgroup (reference:
+// check_ray_inside_bvh with a brute-force contact list).
+__device__ int block_ray_parity(const double *o, const double *d,
+                                const Tri64 *__restrict__ tri, int64_t nel,
+                                double *s_t, int *s_n, int *s_cnt,
+                                uint32_t *err) {
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    *s_n = 0;
+    *s_cnt = 0;
+  }
+  __syncthreads();
+  for (int64_t el = tid; el < nel; el += blockDim.x) {
+    const Tri64 t = tri[el];
+    double tv;
+    if (mt_ref(o, d, t.v, t.v + 3, t.v + 6, &tv)) {
+      int slot = atomicAdd(s_n, 1);
+      if (slot < kHitCap) s_t[slot] = tv;
+    }
+  }
+  __syncthreads();
+  int n = *s_n;
+  if (n > kHitCap) {
+    if (tid == 0) atomicOr(err, kErrHitOverflow);
+    n = kHitCap;
+  }
+  for (int a = tid; a < n; a += blockDim.x) {
+    double x = s_t[a];
+    bool dup = false, have = false;
+    double pred = 0.0;
+    for (int b = 0; b < n; b++) {
+      double y = s_t[b];
+      if (b != a && y == x && b < a) dup = true;
+      if (y < x && (!have || y > pred)) {
+        pred = y;
+        have = true;
+      }
+    }
+    if (!dup && (!have || x - pred > kRayEps)) atomicAdd(s_cnt, 1);
+  }
+  __syncthreads();
+  int c = *s_cnt;
+  __syncthreads();
+  return c & 1;
+}
+
+// Solid angle sum (generalized_winding_number, :295-328), block reduction.
+__device__ inline void normalize_ref(const double *a, double *o) {
+#pragma clang fp contract(off)
+  double s = a[0] * a[0];
+  s = s + a[1] * a[1];
+  s = s + a[2] * a[2];
+  double n = sqrt(s);
+  if (n == 0.0) {
+    o[0] = o[1] = o[2] = __builtin_nan("");
+    return;
+  }
+  double inv = 1.0 / n;
+  o[0] = a[0] * inv;
+  o[1] = a[1] * inv;
+  o[2] = a[2] * inv;
+}
+
+__device__ double block_winding_sign(const double *p,
+                                     const Tri64 *__restrict__ tri,
+                                     int64_t nel, double *s_red) {
+#pragma clang fp contract(off)
+  const int tid = threadIdx.x;
+  double w = 0.0;
+  for (int64_t el = tid; el < nel; el += blockDim.x) {
+    const Tri64 t = tri[el];
+    double r[3], a[3], b[3], c[3], bc[3];
+    for (int q = 0; q < 3; q++) r[q] = t.v[q] - p[q];
+    normalize_ref(r, a);
+    for (int q = 0; q < 3; q++) r[q] = t.v[3 + q] - p[q];
+    normalize_ref(r, b);
+    for (int q = 0; q < 3; q++) r[q] = t.v[6 + q] - p[q];
+    normalize_ref(r, c);
+    cross3_ref(b, c, bc);
+    double num = dot3_ref(a, bc);
+    double den = ((1.0 + dot3_ref(a, b)) + dot3_ref(b, c)) + dot3_ref(c, a);
+    if (fabs(den) > 1e-10) {
+      double sa = 2.0 * atan2(fabs(num), den);
+      if (num < 0) sa = -sa;
+      w += sa;
+    }
+  }
+  s_red[tid] = w;
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if (tid < off) s_red[tid] += s_red[tid + off];
+    __syncthreads();
+  }
+  double tot = s_red[0] / 12.566370614359172;
+  __syncthreads();
+  return fabs(tot) > 0.5 ? 1.0 : -1.0;
+}
+
+__global__ __launch_bounds__(256) void resolve_brute_kernel(
+    GridP g, const double *__restrict__ pts, const Tri64 *__restrict__ tri,
+    int64_t nel, RayParams rp, const Pending *__restrict__ pend,
+    uint32_t npend, FusedOut out, uint32_t *err) {
+  __shared__ double s_t[kHitCap];
+  __shared__ double s_red[256];
+  __shared__ int s_n, s_cnt;
+  for (uint32_t it = blockIdx.x; it < npend; it += gridDim.x) {
+    const Pending P = pend[it];
+    double o[3];
+    if (pts) {
+      o[0] = pts[3 * P.idx + 0];
+      o[1] = pts[3 * P.idx + 1];
+      o[2] = pts[3 * P.idx + 2];
+    } else {
+      int64_t nxy = (int64_t)g.nx * g.ny;
+      int64_t kk = P.idx / nxy, rem = P.idx % nxy;
+      Vec3d q = lattice_point(g, rem % g.nx, rem / g.nx, kk);
+      o[0] = q.x;
+      o[1] = q.y;
+      o[2] = q.z;
+    }
+    int inside = P.inside, total = P.used, used = P.used;
+    // adaptive_point_in_mesh_raycast (:234-288)
+    for (int r = used; r < rp.min_rays; r++) {
+      if (block_ray_parity(o, rp.dirs + 3 * r, tri, nel, s_t, &s_n, &s_cnt,
+                           err))
+        inside++;
+      total++;
+    }
+    if (used < rp.min_rays) used = rp.min_rays;
+    double ratio = total > 0 ? (double)inside / (double)total : 0.0;
+    double conf = ratio > 1.0 - ratio ? ratio : 1.0 - ratio;
+    while (conf < 0.85 && used < rp.max_rays) {
+      int batch = rp.max_rays - used < 4 ? rp.max_rays - used : 4;
+      for (int r = used; r < used + batch; r++) {
+        if (r < kNumDirs) {
+          if (block_ray_parity(o, rp.dirs + 3 * r, tri, nel, s_t, &s_n,
+                               &s_cnt, err))
+            inside++;
+          total++;
+        }
+      }
+      used += batch;
+      ratio = (double)inside / (double)total;
+      conf = ratio > 1.0 - ratio ? ratio : 1.0 - ratio;
+      if (conf > 0.95) break;
+    }
+    double sgn = ratio > 0.5 ? 1.0 : -1.0;
+    if (rp.use_winding && conf < rp.conf_thr) {
+      sgn = block_winding_sign(o, tri, nel, s_red);
+      conf = 0.8;
+    }
+    if (threadIdx.x == 0) {
+      if (out.sdf) out.sdf[P.idx] = P.dist * sgn;
+      if (out.signs) out.signs[P.idx] = sgn;
+      if (out.conf) out.conf[P.idx] = conf;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void dist_points_brute_kernel(
+    const double *__restrict__ pts, int64_t npts,
+    const Tri64 *__restrict__ tri, int64_t nel, double h,
+    double *__restrict__ dist) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= npts) return;
+  const double px = pts[3 * i], py = pts[3 * i + 1], pz = pts[3 * i + 2];
+  double best = __builtin_huge_val();
+  for (int64_t el = 0; el < nel; el++) {
+    const Tri64 t = tri[el];
+    double d2 = tri_dist2<double>(0.0, 0.0, 0.0, t.v[0] - px, t.v[1] - py,
+                                  t.v[2] - pz, t.v[3] - px, t.v[4] - py,
+                                  t.v[5] - pz, t.v[6] - px, t.v[7] - py,
+                                  t.v[8] - pz);
+    best = fmin(best, d2);
+  }
+  double d = sqrt(best);
+  dist[i] = d < h ? d : h;
+}
+
+__global__ void init_pending_all_kernel(Pending *pend, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    Pending p;
+    p.idx = i;
+    p.dist = 0.0;
+    p.inside = 0;
+    p.used = 0;
+    pend[i] = p;
+  }
+}
+
+}  // namespace
+
+void launch_fused_lattice(hipStream_t s, const GridP &g, const Tri64 *tri,
+                          const Bins &b, bool do_dist, bool do_sign,
+                          FusedOut out, Pending *pend, uint32_t *npend,
+                          uint32_t pend_cap, uint32_t *err) {
+  TileDims td = tile_dims(g);
+  if (td.ntiles <= 0) return;
+  hipLaunchKernelGGL(fused_lattice_kernel, dim3((unsigned)td.ntiles),
+                     dim3(kTilePts), 0, s, g, td, tri, b.tile_off, b.tile_tris,
+                     b.line_off, b.line_hits, b.line_base[1], b.line_base[2],
+                     do_dist ? 1 : 0, do_sign ? 1 : 0, out, pend, npend,
+                     pend_cap, err);
+  SDF_HIP(hipGetLastError());
+}
+
+void launch_resolve_brute(hipStream_t s, const GridP &g, const double *pts,
+                          const Tri64 *tri, int64_t nel, const RayParams &rp,
+                          const Pending *pend, uint32_t npend, FusedOut out,
+                          uint32_t *err) {
+  if (npend == 0) return;
+  unsigned blocks = npend < 65536u ? npend : 65536u;
+  hipLaunchKernelGGL(resolve_brute_kernel, dim3(blocks), dim3(256), 0, s, g,
+                     pts, tri, nel, rp, pend, npend, out, err);
+  SDF_HIP(hipGetLastError());
+}
+
+void launch_dist_points_brute(hipStream_t s, const double *pts, int64_t npts,
+                              const Tri64 *tri, int64_t nel, double h,
+                              double *dist) {
+  if (npts <= 0) return;
+  hipLaunchKernelGGL(dist_points_brute_kernel,
+                     dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, s,
+                     pts, npts, tri, nel, h, dist);
+  SDF_HIP(hipGetLastError());
+}
+
+void launch_init_pending_all(hipStream_t s, Pending *pend, int64_t n) {
+  if (n <= 0) return;
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(init_pending_all_kernel, dim3((unsigned)blocks), dim3(256),
+                     0, s, pend, n);
+  SDF_HIP(hipGetLastError());
+}
+
+}  // namespace sdf

@@ -1,0 +1,93 @@
+"""SignedDistances entry points on the MI355X library
+(src/SignedDistances/SignedDistances.jl:3-4).
+
+Same names, arguments and results as the reference; the work runs in
+libsdf_mi355x.so.  Julia's ``remove_sdf_artifacts!`` is spelled
+``remove_sdf_artifacts_`` here and mutates its argument the same way.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .MeshGrid import generateGridPoints  # noqa: F401  (re-exported for callers)
+
+
+def _mesh_args(mesh):
+    X = np.asfortranarray(mesh.X, dtype=np.float64)
+    IEN = np.asfortranarray(mesh.IEN, dtype=np.int64)
+    return X, IEN
+
+
+def _grid_args(grid):
+    return _lib.f64_3(grid.AABB_min), float(grid.cell_size), _lib.i64_3(grid.N)
+
+
+def evalDistancesOnTriMesh(mesh, grid, points=None, device=0):
+    """evalDistancesOnTriMesh(mesh, grid, points) -- dfOnTriangularMesh.jl:19-209.
+
+    Returns (dist, xp).  dist[i] = min(exact distance, grid.cell_size); xp is
+    not produced by the GPU path and is returned as None.  points=None means
+    the grid's lattice (the usual generateGridPoints(grid) input, which is
+    also recognised when passed explicitly)."""
+    X, IEN = _mesh_args(mesh)
+    amin, h, N = _grid_args(grid)
+    if points is None:
+        npts = int(grid.ngp)
+        pp = None
+    else:
+        P = _lib.as_points(points)
+        npts = P.shape[1]
+        pp = _lib.dptr(P)
+    dist = np.empty(npts, dtype=np.float64)
+    L = _lib.lib()
+    _lib.check(L.sdf_eval_distances(
+        _lib.ctx(device), _lib.dptr(X), X.shape[1], _lib.i64ptr(IEN), IEN.shape[1],
+        _lib.dptr(amin), h, _lib.i64ptr(N), pp, npts, _lib.dptr(dist)))
+    return dist, None
+
+
+def raycast_sign_detection(mesh, grid, points=None, min_rays=6, max_rays=30,
+                           confidence_threshold=0.6, use_winding_fallback=True,
+                           device=0):
+    """raycast_sign_detection -- RaycastingSignDetection.jl:337-404.
+
+    Returns (signs, confidences); inside = +1."""
+    X, IEN = _mesh_args(mesh)
+    amin, h, N = _grid_args(grid)
+    if points is None:
+        npts = int(grid.ngp)
+        pp = None
+    else:
+        P = _lib.as_points(points)
+        npts = P.shape[1]
+        pp = _lib.dptr(P)
+    signs = np.empty(npts, dtype=np.float64)
+    conf = np.empty(npts, dtype=np.float64)
+    L = _lib.lib()
+    _lib.check(L.sdf_raycast_signs(
+        _lib.ctx(device), _lib.dptr(X), X.shape[1], _lib.i64ptr(IEN), IEN.shape[1],
+        _lib.dptr(amin), h, _lib.i64ptr(N), pp, npts, int(min_rays), int(max_rays),
+        float(confidence_threshold), 1 if use_winding_fallback else 0,
+        _lib.dptr(signs), _lib.dptr(conf)))
+    return signs, conf
+
+
+def signed_distance(mesh, grid, device=0):
+    """dists .* signs over the grid lattice in one library call
+    (stl_to_sdf.jl:101-119 with the default ray options)."""
+    X, IEN = _mesh_args(mesh)
+    amin, h, N = _grid_args(grid)
+    out = np.empty(int(grid.ngp), dtype=np.float64)
+    L = _lib.lib()
+    _lib.check(L.sdf_signed_distance(
+        _lib.ctx(device), _lib.dptr(X), X.shape[1], _lib.i64ptr(IEN), IEN.shape[1],
+        _lib.dptr(amin), h, _lib.i64ptr(N), _lib.dptr(out)))
+    return out
+
+
+def ray_directions():
+    """The 36 prioritised directions (RaycastingSignDetection.jl:162-227)."""
+    d = np.zeros(108, dtype=np.float64)
+    _lib.check(_lib.lib().sdf_ray_directions(_lib.dptr(d)))
+    return d.reshape(36, 3)

@@ -1,0 +1,121 @@
+"""Parity of the HIP path against the CPU oracle (GPU tests).
+
+Signs must be bit-identical to the oracle (which follows the reference's
+Moller-Trumbore operation order); distances must lie within
+1e-6 x bbox diagonal of the exact FP64 min(d, h).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle
+from stl2sdf import MeshGrid, SignedDistances, _lib
+from tests import _cases
+
+pytestmark = pytest.mark.gpu
+
+DIST_TOL = 1e-6  # x bbox diagonal
+
+
+def _check(mesh, grid, points=None):
+    P = MeshGrid.generateGridPoints(grid) if points is None else points
+    print(f"case: nel={mesh.nel} ngp={P.shape[1]} N={list(grid.N)}", flush=True)
+    d_gpu, _ = SignedDistances.evalDistancesOnTriMesh(mesh, grid, points)
+    s_gpu, c_gpu = SignedDistances.raycast_sign_detection(mesh, grid, points)
+    d_ref = oracle.dist_exact(mesh.X, mesh.IEN, P, grid.cell_size)
+    s_ref, c_ref = oracle.signs(mesh.X, mesh.IEN, P)
+    tol = DIST_TOL * _cases.diag(mesh)
+    err = np.abs(d_gpu - d_ref)
+    assert err.max() <= tol, f"max distance error {err.max():.3e} > {tol:.3e}"
+    mism = np.flatnonzero(s_gpu != s_ref)
+    assert mism.size == 0, f"{mism.size} sign mismatches, first at {mism[:5]}"
+    assert np.array_equal(c_gpu, c_ref), "confidence mismatch"
+    return d_gpu, s_gpu
+
+
+def test_icosphere_config1():
+    mesh, grid = _cases.icosphere_case()
+    assert grid.ngp == 91125
+    _check(mesh, grid)
+
+
+def test_icosphere_margin3_lattice_points_passed():
+    # stl_to_sdf's margin-3 grid, lattice handed in explicitly
+    mesh, grid = _cases.icosphere_case(margin=3)
+    P = MeshGrid.generateGridPoints(grid)
+    _check(mesh, grid, P)
+
+
+def test_artifacts_stl():
+    mesh, grid = _cases.artifacts_case(0.5)
+    assert list(grid.N) == [126, 46, 14] and grid.ngp == 89535
+    _check(mesh, grid)
+
+
+def test_star_surface():
+    mesh, grid = _cases.star_case()
+    _check(mesh, grid)
+
+
+def test_open_mesh_adaptive_and_winding():
+    mesh, grid = _cases.open_case()
+    P = MeshGrid.generateGridPoints(grid)
+    _, _, rays, wind = oracle.signs(mesh.X, mesh.IEN, P, diagnostics=True)
+    assert (rays > 6).sum() > 100 and wind.sum() > 0  # the case exercises both
+    _check(mesh, grid)
+
+
+def test_general_points():
+    mesh, grid = _cases.icosphere_case(subdiv=3, n_max=20)
+    rng = np.random.default_rng(5)
+    P = np.asfortranarray(rng.uniform(-1.3, 1.3, size=(3, 3000)))
+    _check(mesh, grid, P)
+
+
+def test_min_rays_other_than_six():
+    mesh, grid = _cases.open_case()
+    P = MeshGrid.generateGridPoints(grid)
+    s, c = SignedDistances.raycast_sign_detection(mesh, grid, None, min_rays=4, max_rays=20,
+                                                  confidence_threshold=0.7)
+    s_ref, c_ref = oracle.signs(mesh.X, mesh.IEN, P, min_rays=4, max_rays=20, conf_thr=0.7)
+    assert np.array_equal(s, s_ref) and np.array_equal(c, c_ref)
+
+
+def test_no_winding_fallback():
+    mesh, grid = _cases.open_case()
+    P = MeshGrid.generateGridPoints(grid)
+    s, c = SignedDistances.raycast_sign_detection(mesh, grid, None, use_winding_fallback=False)
+    s_ref, c_ref = oracle.signs(mesh.X, mesh.IEN, P, use_winding=False)
+    assert np.array_equal(s, s_ref) and np.array_equal(c, c_ref)
+    assert (c < 0.6).any()
+
+
+def test_fused_equals_product():
+    mesh, grid = _cases.star_case(n=40, n_max=30)
+    d, _ = SignedDistances.evalDistancesOnTriMesh(mesh, grid)
+    s, _ = SignedDistances.raycast_sign_detection(mesh, grid)
+    sdf = SignedDistances.signed_distance(mesh, grid)
+    assert np.array_equal(sdf, d * s)
+
+
+def test_bad_index_raises():
+    mesh, grid = _cases.icosphere_case(subdiv=1, n_max=8)
+    bad = MeshGrid.TriangularMesh(mesh.X, mesh.IEN.copy())
+    bad.IEN[1, 3] = mesh.nnp + 1
+    with pytest.raises(_lib.SdfError):
+        SignedDistances.evalDistancesOnTriMesh(bad, grid)
+
+
+def test_empty_mesh():
+    mesh, grid = _cases.icosphere_case(subdiv=1, n_max=8)
+    empty = MeshGrid.TriangularMesh(mesh.X, np.zeros((3, 0), np.int64))
+    d, _ = SignedDistances.evalDistancesOnTriMesh(empty, grid)
+    s, c = SignedDistances.raycast_sign_detection(empty, grid)
+    assert np.all(d == grid.cell_size)
+    assert np.all(s == -1.0) and np.all(c == 1.0)
+
+
+def test_repeat_calls_stable():
+    mesh, grid = _cases.star_case(n=40, n_max=30)
+    a = SignedDistances.signed_distance(mesh, grid)
+    b = SignedDistances.signed_distance(mesh, grid)
+    assert np.array_equal(a, b)
