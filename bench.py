@@ -69,24 +69,30 @@ def cpu_baseline(mesh, grid, target_s, threads):
         P[2] = grid.AABB_min[2] + h * k.astype(np.float64)
         return P
 
-    # calibrate on a small sample, then one timed run of ~target_s seconds
-    n = 4096
-    while True:
+    def run(n):
         P = sample(n)
         t0 = time.perf_counter()
         oracle.dist_reference(mesh.X, mesh.IEN, P, h, threads)
         oracle.signs(mesh.X, mesh.IEN, P, threads=threads)
-        dt = time.perf_counter() - t0
-        if dt > 1.0 or n >= (1 << 22):
-            break
-        n *= 4
-    n_run = int(min(max(n * target_s / max(dt, 1e-3), 4096), 1 << 24))
-    P = sample(n_run)
-    t0 = time.perf_counter()
-    oracle.dist_reference(mesh.X, mesh.IEN, P, h, threads)
-    oracle.signs(mesh.X, mesh.IEN, P, threads=threads)
-    dt = time.perf_counter() - t0
-    return n_run / dt, n_run, dt
+        return time.perf_counter() - t0
+
+    # Each oracle call rebuilds its trees (a fixed cost that a full grid
+    # amortises), so time two sample sizes and take the per-point slope.
+    n1 = 4096
+    t1 = run(n1)
+    while t1 < 1.0 and n1 < (1 << 20):
+        n1 *= 2
+        t1 = run(n1)
+    build_s = None
+    n2 = n1
+    t2 = t1
+    while t2 - t1 < target_s and n2 < (1 << 24):
+        n2 = int(n2 * 4)
+        t2 = run(n2)
+    slope = (t2 - t1) / max(n2 - n1, 1)
+    build_s = max(t1 - slope * n1, 0.0)
+    full_s = build_s + slope * grid.ngp
+    return grid.ngp / full_s, (n1, t1, n2, t2, build_s, full_s)
 
 
 def main():
@@ -180,11 +186,13 @@ def main():
     if not args.no_cpu_baseline:
         threads = min(16, len(os.sched_getaffinity(0)))
         log(f"[bench] cpu baseline with {threads} threads")
-        v, n_s, t_s = cpu_baseline(mesh, grid, args.cpu_seconds, threads)
-        res["cpu_baseline"] = {"value": v, "unit": "signed grid points/s", "cores": threads,
-                               "kind": "port",
-                               "sample": f"{n_s} uniformly random lattice points of the same grid "
-                                         f"({t_s:.1f} s), reference-faithful distance + BVH ray vote"}
+        v, (n1, t1, n2, t2, b_s, f_s) = cpu_baseline(mesh, grid, args.cpu_seconds, threads)
+        res["cpu_baseline"] = {
+            "value": v, "unit": "signed grid points/s", "cores": threads, "kind": "port",
+            "sample": (f"oracle (reference-faithful distance + bounding-sphere BVH adaptive ray "
+                       f"vote, FP64) on {n1} and {n2} uniformly random lattice points of the same "
+                       f"grid ({t1:.1f} s, {t2:.1f} s); tree builds {b_s:.1f} s; full grid "
+                       f"extrapolated to {f_s:.0f} s")}
         res["gpu_over_cpu"] = value / v
     print(json.dumps(res))
 

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "sdf_internal.h"
+#include "sdf_lbvh.h"
 #include "sdf_mi355x.h"
 
 // Host arithmetic that restates reference formulas (grid setup, ray
@@ -93,6 +94,8 @@ struct sdf_ctx {
   sdf::Tri64 *tri = nullptr;
   int64_t nel = 0, nnp = 0;
   bool mesh_ready = false;
+  sdf::Bvh bvh;
+  bool bvh_ready = false;
   // small device words: [0] error flags, [1] pending count
   uint32_t *d_words = nullptr;
   // timing
@@ -179,6 +182,44 @@ void set_mesh_device(sdf_ctx *c, const double *dX, int64_t nnp,
   c->mark("gather");
   sdf::launch_gather_tris(c->stream, dX, nnp, dIEN, nel, c->tri, c->d_words);
   c->mesh_ready = true;
+  c->bvh_ready = false;
+}
+
+const sdf::Bvh &need_bvh(sdf_ctx *c) {
+  if (!c->bvh_ready) {
+    c->mark("lbvh");
+    c->bvh = sdf::build_lbvh(c->stream, c->ar, c->tri, c->nel);
+    c->bvh_ready = true;
+  }
+  return c->bvh;
+}
+
+// Adaptive vote for a list of items starting at ray r0 (all rays below r0
+// already counted in the items' state).  Items that need more than the
+// first min_rays rays are carried to a second pass.
+uint32_t resolve_items(sdf_ctx *c, const sdf::GridP &g, const double *d_pts,
+                       const sdf::RayParams &rp, sdf::Pending *items,
+                       uint32_t n, int r0, sdf::FusedOut out) {
+  if (n == 0) return 0;
+  const sdf::Bvh &bvh = need_bvh(c);
+  c->mark("resolve");
+  uint32_t *cnt = c->ar.get<uint32_t>("res_counts", 8);
+  uint32_t nnext = 0, nwind = 0, nwind2 = 0;
+  int rmax = rp.max_rays < sdf::kNumDirs ? rp.max_rays : sdf::kNumDirs;
+  if (r0 < rp.min_rays) {
+    sdf::Pending *next = c->ar.get<sdf::Pending>("res_next", n);
+    sdf::resolve_pending(c->stream, c->ar, g, d_pts, c->tri, c->nel, bvh, rp,
+                         items, n, r0, rp.min_rays, 0, out, next, cnt,
+                         c->d_words, &nnext, &nwind);
+    sdf::resolve_pending(c->stream, c->ar, g, d_pts, c->tri, c->nel, bvh, rp,
+                         next, nnext, rp.min_rays, rmax, 1, out, nullptr, cnt,
+                         c->d_words, &nnext, &nwind2);
+  } else {
+    sdf::resolve_pending(c->stream, c->ar, g, d_pts, c->tri, c->nel, bvh, rp,
+                         items, n, r0, rmax, 1, out, nullptr, cnt, c->d_words,
+                         &nnext, &nwind);
+  }
+  return nwind + nwind2;
 }
 
 void set_mesh_host(sdf_ctx *c, const double *X, int64_t nnp, const int64_t *IEN,
@@ -241,10 +282,7 @@ void run_lattice(sdf_ctx *c, const sdf::GridP &g, bool do_dist, bool do_sign,
       np = read_word(c, 1);
     }
   }
-  c->mark("resolve");
-  if (np > 0)
-    sdf::launch_resolve_brute(c->stream, g, nullptr, c->tri, c->nel, rp, pend,
-                              np, out, c->d_words);
+  if (np > 0) resolve_items(c, g, nullptr, rp, pend, np, 6, out);
   c->mark("end");
   c->counters[0] = np;
   c->counters[1] = bins.ntile_pairs;
@@ -442,9 +480,12 @@ sdf_status sdf_eval_distances(sdf_ctx *c, const double *X, int64_t nnp,
       double *d_pts = c->ar.get<double>("in_pts", 3 * npts + 3);
       SDF_HIP(hipMemcpyAsync(d_pts, points, (size_t)(3 * npts) * sizeof(double),
                              hipMemcpyHostToDevice, c->stream));
+      const sdf::Bvh &bvh = need_bvh(c);
       c->mark("dist_points");
-      sdf::launch_dist_points_brute(c->stream, d_pts, npts, c->tri, c->nel, h,
-                                    d_dist);
+      float slack = 1e-6f * (float)(fabs(aabb_min[0]) + fabs(aabb_min[1]) +
+                                    fabs(aabb_min[2]) + (double)(N[0] + N[1] + N[2]) * h + 1e-30);
+      sdf::launch_dist_points_bvh(c->stream, d_pts, npts, c->tri, bvh, h, slack,
+                                  d_dist, c->d_words);
       c->mark("end");
       check_err_flags(c);
     }
@@ -489,14 +530,10 @@ sdf_status sdf_raycast_signs(sdf_ctx *c, const double *X, int64_t nnp,
                                hipMemcpyHostToDevice, c->stream));
         d_pts = p;
       }
+      if (npts > 0xFFFFFFFFll) fail(SDF_ERR_ARG, "too many points");
       sdf::Pending *pend = c->ar.get<sdf::Pending>("pend_all", npts + 1);
       sdf::launch_init_pending_all(c->stream, pend, npts);
-      c->mark("resolve");
-      for (int64_t b = 0; b < npts; b += 0x40000000ll) {
-        uint32_t m = (uint32_t)(npts - b < 0x40000000ll ? npts - b : 0x40000000ll);
-        sdf::launch_resolve_brute(c->stream, g, d_pts, c->tri, c->nel, rp,
-                                  pend + b, m, out, c->d_words);
-      }
+      resolve_items(c, g, d_pts, rp, pend, (uint32_t)npts, 0, out);
       c->mark("end");
       check_err_flags(c);
     }

@@ -134,17 +134,21 @@ struct RayParams {
   int32_t use_winding;
 };
 
-// Finish the adaptive vote for pending items (brute-force ray casting over
-// all triangles, one workgroup per item). pts == nullptr: lattice points.
-void launch_resolve_brute(hipStream_t s, const GridP &g, const double *pts,
-                          const Tri64 *tri, int64_t nel, const RayParams &rp,
-                          const Pending *pend, uint32_t npend, FusedOut out,
-                          uint32_t *err);
+struct Bvh;
 
-// Exact unsigned distance for arbitrary points, brute force.
-void launch_dist_points_brute(hipStream_t s, const double *pts, int64_t npts,
-                              const Tri64 *tri, int64_t nel, double h,
-                              double *dist);
+// Adaptive vote for pending items: rays [r0, r1) via the LBVH, then the
+// vote replay (final_phase = 0: items needing more rays go to `next`), then
+// the winding fallback.  pts == nullptr: lattice points of g.
+void resolve_pending(hipStream_t s, Arena &ar, const GridP &g, const double *pts,
+                     const Tri64 *tri, int64_t nel, const Bvh &bvh,
+                     const RayParams &rp, const Pending *items, uint32_t nitems,
+                     int r0, int r1, int final_phase, FusedOut out,
+                     Pending *next, uint32_t *d_counts, uint32_t *err,
+                     uint32_t *n_next_host, uint32_t *n_wind_host);
+
+void launch_dist_points_bvh(hipStream_t s, const double *pts, int64_t npts,
+                            const Tri64 *tri, const Bvh &bvh, double h,
+                            float slack, double *dist, uint32_t *err);
 
 void launch_init_pending_all(hipStream_t s, Pending *pend, int64_t npts);
 

@@ -1,0 +1,220 @@
+// LBVH layout and device traversal routines (ray parity, nearest triangle).
+#pragma once
+
+#include "sdf_internal.h"
+
+namespace sdf {
+
+// Internal node: both children's boxes (lo xyz, hi xyz) so one 64-byte
+// fetch tests two children.  child >= 0: internal node; child < 0: leaf
+// -(child+1) in sorted order.
+struct BvhNode {
+  float box[2][6];
+  int32_t child[2];
+  int32_t pad_[2];
+};
+
+struct Bvh {
+  int64_t n = 0;               // leaves (triangles)
+  BvhNode *nodes = nullptr;    // n-1 internal nodes, root = 0
+  float *leaf_box = nullptr;   // 6 floats per triangle (by triangle id)
+  uint32_t *leaf_tri = nullptr;  // sorted leaf -> triangle id
+};
+
+Bvh build_lbvh(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t n);
+void radix_sort_u32_pairs(hipStream_t s, Arena &ar, uint32_t *keys,
+                          uint32_t *vals, int64_t n);
+
+constexpr int kStackDepth = 64;
+
+// Slab test in FP64 on a box padded by a relative 1e-7 (conservative: a
+// triangle the FP64 predicates hit is never pruned).  Only t >= 0 matters.
+__device__ inline bool ray_box(const double *o, const double *d,
+                               const double *inv, const float *b) {
+  double tmin = 0.0, tmax = __builtin_huge_val();
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    double lo = (double)b[a], hi = (double)b[3 + a];
+    double pad = 1e-7 * (fabs(lo) + fabs(hi)) + 1e-30;
+    lo -= pad;
+    hi += pad;
+    if (d[a] == 0.0) {
+      if (o[a] < lo || o[a] > hi) return false;
+      continue;
+    }
+    double t1 = (lo - o[a]) * inv[a], t2 = (hi - o[a]) * inv[a];
+    double tn = fmin(t1, t2), tf = fmax(t1, t2);
+    tmin = fmax(tmin, tn);
+    tmax = fmin(tmax, tf);
+  }
+  return tmin <= tmax;
+}
+
+// Parity of one ray (check_ray_inside_bvh, RaycastingSignDetection.jl:
+// 114-154) by LBVH traversal.  Hits are kept in a private list of kMaxHits
+// values; returns -1 when the list or the stack would overflow (the caller
+// then decides the ray another way).
+constexpr int kMaxHits = 32;
+
+__device__ inline int ray_parity_bvh(const Bvh &bvh, const Tri64 *__restrict__ tri,
+                                     const double *o, const double *d) {
+  double inv[3];
+  for (int a = 0; a < 3; a++) inv[a] = d[a] != 0.0 ? 1.0 / d[a] : 0.0;
+  double hits[kMaxHits];
+  int nh = 0;
+  auto leaf = [&](uint32_t t) -> bool {
+    const Tri64 r = tri[t];
+    double tv;
+    if (mt_ref(o, d, r.v, r.v + 3, r.v + 6, &tv)) {
+      if (nh >= kMaxHits) return false;
+      hits[nh++] = tv;
+    }
+    return true;
+  };
+  if (bvh.n == 0) return 0;
+  if (bvh.n == 1) {
+    if (ray_box(o, d, inv, bvh.leaf_box + 6 * (int64_t)bvh.leaf_tri[0]))
+      if (!leaf(bvh.leaf_tri[0])) return -1;
+  } else {
+    int32_t stack[kStackDepth];
+    int sp = 0;
+    int32_t node = 0;
+    while (true) {
+      const BvhNode nd = bvh.nodes[node];
+      bool h0 = ray_box(o, d, inv, nd.box[0]);
+      bool h1 = ray_box(o, d, inv, nd.box[1]);
+      int32_t next = -0x7FFFFFFF;
+      for (int c = 0; c < 2; c++) {
+        if (!(c == 0 ? h0 : h1)) continue;
+        int32_t ch = nd.child[c];
+        if (ch < 0) {
+          if (!leaf(bvh.leaf_tri[-(ch + 1)])) return -1;
+        } else if (next == -0x7FFFFFFF) {
+          next = ch;
+        } else {
+          if (sp >= kStackDepth) return -1;
+          stack[sp++] = ch;
+        }
+      }
+      if (next != -0x7FFFFFFF) {
+        node = next;
+      } else if (sp > 0) {
+        node = stack[--sp];
+      } else {
+        break;
+      }
+    }
+  }
+  // unique count with the reference's rule (sorted, gap > eps)
+  for (int a = 1; a < nh; a++) {
+    double x = hits[a];
+    int b = a - 1;
+    while (b >= 0 && hits[b] > x) {
+      hits[b + 1] = hits[b];
+      b--;
+    }
+    hits[b + 1] = x;
+  }
+  if (nh == 0) return 0;
+  int u = 1;
+  for (int a = 1; a < nh; a++)
+    if (fabs(hits[a] - hits[a - 1]) > kRayEps) u++;
+  return u & 1;
+}
+
+// Squared distance from p (FP32, absolute) to a box.
+__device__ inline float box_dist2(const float *p, const float *b) {
+  float s = 0.f;
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    float e = fmaxf(fmaxf(b[a] - p[a], p[a] - b[3 + a]), 0.f);
+    s += e * e;
+  }
+  return s;
+}
+
+// Nearest triangle within radius r of p: returns min(exact distance, r)
+// (FP32 arithmetic on FP64 differences).  Sets *found when a triangle lies
+// within r.  Box pruning is padded by `slack` (absolute) to stay
+// conservative under FP32 rounding of p.
+__device__ inline double nearest_bvh(const Bvh &bvh, const Tri64 *__restrict__ tri,
+                                     const double *p, double r, float slack,
+                                     bool *found, bool *overflow) {
+  float pf[3] = {(float)p[0], (float)p[1], (float)p[2]};
+  float best = (float)r;  // current search radius
+  float best2 = best * best;
+  bool any = false;
+  auto leaf = [&](uint32_t t) {
+    const Tri64 q = tri[t];
+    float d2 = tri_dist2<float>(
+        0.f, 0.f, 0.f, (float)(q.v[0] - p[0]), (float)(q.v[1] - p[1]),
+        (float)(q.v[2] - p[2]), (float)(q.v[3] - p[0]), (float)(q.v[4] - p[1]),
+        (float)(q.v[5] - p[2]), (float)(q.v[6] - p[0]), (float)(q.v[7] - p[1]),
+        (float)(q.v[8] - p[2]));
+    if (d2 < best2) {
+      best2 = d2;
+      any = true;
+    }
+  };
+  *overflow = false;
+  if (bvh.n == 0) {
+    *found = false;
+    return r;
+  }
+  if (bvh.n == 1) {
+    leaf(bvh.leaf_tri[0]);
+  } else {
+    int32_t stack[kStackDepth];
+    int sp = 0;
+    int32_t node = 0;
+    while (true) {
+      const BvhNode nd = bvh.nodes[node];
+      float lim = sqrtf(best2) + slack;
+      float lim2 = lim * lim;
+      float d0 = box_dist2(pf, nd.box[0]);
+      float d1 = box_dist2(pf, nd.box[1]);
+      int32_t c0 = nd.child[0], c1 = nd.child[1];
+      if (d1 < d0) {  // visit the nearer child first
+        float tf = d0;
+        d0 = d1;
+        d1 = tf;
+        int32_t tc = c0;
+        c0 = c1;
+        c1 = tc;
+      }
+      int32_t next = -0x7FFFFFFF;
+      if (d0 <= lim2) {
+        if (c0 < 0)
+          leaf(bvh.leaf_tri[-(c0 + 1)]);
+        else
+          next = c0;
+      }
+      if (d1 <= lim2) {
+        if (c1 < 0) {
+          leaf(bvh.leaf_tri[-(c1 + 1)]);
+        } else if (next == -0x7FFFFFFF) {
+          next = c1;
+        } else {
+          if (sp >= kStackDepth) {
+            *overflow = true;
+            break;
+          }
+          stack[sp++] = c1;
+        }
+      }
+      if (next != -0x7FFFFFFF) {
+        node = next;
+      } else if (sp > 0) {
+        node = stack[--sp];
+      } else {
+        break;
+      }
+    }
+  }
+  *found = any;
+  if (!any) return r;
+  double dd = sqrt((double)best2);
+  return dd < r ? dd : r;
+}
+
+}  // namespace sdf

@@ -1,0 +1,336 @@
+// Linear BVH over triangles: 30-bit Morton codes of the centroids, an LDS
+// radix sort (4 passes of 8 bits, stable), the Karras (2012) hierarchy and a
+// bottom-up atomic refit of FP32 boxes rounded outward.
+//
+// Replaces the reference's ImplicitBVH bounding-sphere tree
+// (RaycastingSignDetection.jl:35-56) for ray casting and the triangle-centre
+// KD-tree (dfOnTriangularMesh.jl:27-35) for distance queries at arbitrary
+// points.  Boxes enclose their triangles exactly (outward rounding), so
+// pruning never drops a triangle that the FP64 predicates would accept.
+#include "sdf_internal.h"
+#include "sdf_lbvh.h"
+
+namespace sdf {
+
+namespace {
+
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 8;
+constexpr int kSortTile = kSortThreads * kSortItems;
+constexpr int kRadixBits = 8;
+constexpr int kRadix = 1 << kRadixBits;
+
+inline int blocks_for(int64_t n, int t) {
+  int64_t b = (n + t - 1) / t;
+  if (b < 1) b = 1;
+  if (b > 65536) b = 65536;
+  return (int)b;
+}
+
+__device__ inline uint32_t float_order(float f) {
+  uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ inline float order_float(uint32_t u) {
+  return __uint_as_float((u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u);
+}
+
+// Triangle boxes (outward-rounded FP32) and the centroid bounds.
+__global__ void leaf_boxes_kernel(const Tri64 *__restrict__ tri, int64_t n,
+                                  float *__restrict__ box,
+                                  uint32_t *__restrict__ cbounds) {
+  float cmin[3] = {3.4e38f, 3.4e38f, 3.4e38f};
+  float cmax[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const Tri64 r = tri[t];
+    for (int d = 0; d < 3; d++) {
+      double a = r.v[d], b = r.v[3 + d], c = r.v[6 + d];
+      double lo = fmin(a, fmin(b, c)), hi = fmax(a, fmax(b, c));
+      box[6 * t + d] = __double2float_rd(lo);
+      box[6 * t + 3 + d] = __double2float_ru(hi);
+      float cc = (float)((a + b + c) / 3.0);
+      cmin[d] = fminf(cmin[d], cc);
+      cmax[d] = fmaxf(cmax[d], cc);
+    }
+  }
+  for (int d = 0; d < 3; d++) {
+    atomicMin(&cbounds[d], float_order(cmin[d]));
+    atomicMax(&cbounds[3 + d], float_order(cmax[d]));
+  }
+}
+
+__device__ inline uint32_t expand10(uint32_t v) {
+  v = (v * 0x00010001u) & 0xFF0000FFu;
+  v = (v * 0x00000101u) & 0x0F00F00Fu;
+  v = (v * 0x00000011u) & 0xC30C30C3u;
+  v = (v * 0x00000005u) & 0x49249249u;
+  return v;
+}
+
+__global__ void morton_kernel(const float *__restrict__ box, int64_t n,
+                              const uint32_t *__restrict__ cbounds,
+                              uint32_t *__restrict__ keys,
+                              uint32_t *__restrict__ vals) {
+  float lo[3], ext[3];
+  for (int d = 0; d < 3; d++) {
+    lo[d] = order_float(cbounds[d]);
+    float hi = order_float(cbounds[3 + d]);
+    ext[d] = hi > lo[d] ? hi - lo[d] : 1.0f;
+  }
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    uint32_t q[3];
+    for (int d = 0; d < 3; d++) {
+      float c = 0.5f * (box[6 * t + d] + box[6 * t + 3 + d]);
+      float x = (c - lo[d]) / ext[d] * 1024.0f;
+      x = fminf(fmaxf(x, 0.0f), 1023.0f);
+      q[d] = (uint32_t)x;
+    }
+    keys[t] = (expand10(q[0]) << 2) | (expand10(q[1]) << 1) | expand10(q[2]);
+    vals[t] = (uint32_t)t;
+  }
+}
+
+// ---- radix sort (LSD, stable) ----
+
+__global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(
+    const uint32_t *__restrict__ keys, int64_t n, int shift, int64_t nblocks,
+    uint32_t *__restrict__ hist) {
+  __shared__ uint32_t h[kRadix];
+  for (int i = threadIdx.x; i < kRadix; i += blockDim.x) h[i] = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * kSortTile;
+  for (int r = 0; r < kSortItems; r++) {
+    int64_t i = base + (int64_t)r * kSortThreads + threadIdx.x;
+    if (i < n) atomicAdd(&h[(keys[i] >> shift) & (kRadix - 1)], 1u);
+  }
+  __syncthreads();
+  for (int d = threadIdx.x; d < kRadix; d += blockDim.x)
+    hist[(int64_t)d * nblocks + blockIdx.x] = h[d];
+}
+
+// Items are taken round-major (index = base + r*256 + thread), so the
+// rank of an item among equal digits is: digits seen in earlier rounds +
+// earlier waves of this round + earlier lanes of this wave.
+__global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
+    const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
+    uint32_t *__restrict__ kout, uint32_t *__restrict__ vout, int64_t n,
+    int shift, int64_t nblocks, const uint64_t *__restrict__ off) {
+  constexpr int kWaves = kSortThreads / 64;
+  __shared__ uint32_t running[kRadix];
+  __shared__ uint32_t wave_cnt[kWaves][kRadix];
+  __shared__ uint64_t base_off[kRadix];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int d = tid; d < kRadix; d += blockDim.x) {
+    running[d] = 0;
+    base_off[d] = off[(int64_t)d * nblocks + blockIdx.x];
+  }
+  const int64_t base = (int64_t)blockIdx.x * kSortTile;
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (int r = 0; r < kSortItems; r++) {
+    for (int q = tid; q < kWaves * kRadix; q += blockDim.x)
+      (&wave_cnt[0][0])[q] = 0;
+    __syncthreads();
+    const int64_t i = base + (int64_t)r * kSortThreads + tid;
+    const bool valid = i < n;
+    uint32_t key = valid ? kin[i] : 0u;
+    uint32_t val = valid ? vin[i] : 0u;
+    uint32_t dig = (key >> shift) & (kRadix - 1);
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < kRadixBits; b++) {
+      uint64_t bal = __ballot((dig >> b) & 1u);
+      peers &= ((dig >> b) & 1u) ? bal : ~bal;
+    }
+    const uint32_t wrank = (uint32_t)__popcll(peers & below);
+    if (valid && wrank == 0) wave_cnt[wave][dig] = (uint32_t)__popcll(peers);
+    __syncthreads();
+    if (valid) {
+      uint32_t rank = running[dig] + wrank;
+      for (int w = 0; w < wave; w++) rank += wave_cnt[w][dig];
+      const uint64_t pos = base_off[dig] + rank;
+      kout[pos] = key;
+      vout[pos] = val;
+    }
+    __syncthreads();
+    for (int d = tid; d < kRadix; d += blockDim.x) {
+      uint32_t s = 0;
+      for (int w = 0; w < kWaves; w++) s += wave_cnt[w][d];
+      running[d] += s;
+    }
+    __syncthreads();
+  }
+}
+
+// ---- Karras hierarchy ----
+
+__device__ inline int delta(const uint32_t *__restrict__ codes, int64_t n,
+                            int64_t i, int64_t j) {
+  if (j < 0 || j >= n) return -1;
+  uint32_t a = codes[i], b = codes[j];
+  if (a == b) return 32 + __clz((uint32_t)(i ^ j));
+  return __clz(a ^ b);
+}
+
+__global__ void karras_kernel(const uint32_t *__restrict__ codes, int64_t n,
+                              BvhNode *__restrict__ nodes,
+                              int32_t *__restrict__ parent_int,
+                              int32_t *__restrict__ parent_leaf) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n - 1;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int d = (delta(codes, n, i, i + 1) - delta(codes, n, i, i - 1)) >= 0 ? 1 : -1;
+    const int dmin = delta(codes, n, i, i - d);
+    int64_t lmax = 2;
+    while (delta(codes, n, i, i + lmax * d) > dmin) lmax *= 2;
+    int64_t l = 0;
+    for (int64_t t = lmax / 2; t >= 1; t /= 2)
+      if (delta(codes, n, i, i + (l + t) * d) > dmin) l += t;
+    const int64_t j = i + l * d;
+    const int dnode = delta(codes, n, i, j);
+    int64_t s = 0, step = l;
+    do {
+      step = (step + 1) >> 1;
+      if (delta(codes, n, i, i + (s + step) * d) > dnode) s += step;
+    } while (step > 1);
+    const int64_t gamma = i + s * d + (d < 0 ? -1 : 0);
+    const int64_t lo = i < j ? i : j, hi = i < j ? j : i;
+    int32_t left, right;
+    if (lo == gamma) {
+      left = -(int32_t)(gamma + 1);
+      parent_leaf[gamma] = (int32_t)i;
+    } else {
+      left = (int32_t)gamma;
+      parent_int[gamma] = (int32_t)i;
+    }
+    if (hi == gamma + 1) {
+      right = -(int32_t)(gamma + 2);
+      parent_leaf[gamma + 1] = (int32_t)i;
+    } else {
+      right = (int32_t)(gamma + 1);
+      parent_int[gamma + 1] = (int32_t)i;
+    }
+    nodes[i].child[0] = left;
+    nodes[i].child[1] = right;
+  }
+}
+
+__device__ inline void load_box(int32_t c, const float *__restrict__ leaf_box,
+                                const uint32_t *__restrict__ leaf_tri,
+                                const float *int_box, float *b) {
+  if (c < 0) {
+    const uint32_t t = leaf_tri[-(c + 1)];
+    for (int q = 0; q < 6; q++) b[q] = leaf_box[6 * (int64_t)t + q];
+  } else {
+    for (int q = 0; q < 6; q++) b[q] = int_box[6 * (int64_t)c + q];
+  }
+}
+
+// Bottom-up refit: the second thread to reach a node (atomic counter with
+// acquire-release ordering at device scope) has both child boxes ready.
+__global__ void refit_kernel(int64_t n, const float *__restrict__ leaf_box,
+                             const uint32_t *__restrict__ leaf_tri,
+                             const int32_t *__restrict__ parent_int,
+                             const int32_t *__restrict__ parent_leaf,
+                             uint32_t *__restrict__ visit, float *int_box,
+                             BvhNode *__restrict__ nodes) {
+  for (int64_t leaf = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; leaf < n;
+       leaf += (int64_t)gridDim.x * blockDim.x) {
+    int32_t node = parent_leaf[leaf];
+    while (node >= 0) {
+      uint32_t old = __hip_atomic_fetch_add(&visit[node], 1u, __ATOMIC_ACQ_REL,
+                                            __HIP_MEMORY_SCOPE_AGENT);
+      if (old == 0) break;
+      float b0[6], b1[6], u[6];
+      load_box(nodes[node].child[0], leaf_box, leaf_tri, int_box, b0);
+      load_box(nodes[node].child[1], leaf_box, leaf_tri, int_box, b1);
+      for (int q = 0; q < 3; q++) {
+        u[q] = fminf(b0[q], b1[q]);
+        u[3 + q] = fmaxf(b0[3 + q], b1[3 + q]);
+      }
+      for (int q = 0; q < 6; q++) {
+        nodes[node].box[0][q] = b0[q];
+        nodes[node].box[1][q] = b1[q];
+        int_box[6 * (int64_t)node + q] = u[q];
+      }
+      __threadfence();
+      node = parent_int[node];
+    }
+  }
+}
+
+void radix_sort_pairs(hipStream_t s, Arena &ar, uint32_t *keys, uint32_t *vals,
+                      int64_t n) {
+  if (n <= 1) return;
+  int64_t nblocks = (n + kSortTile - 1) / kSortTile;
+  uint32_t *k2 = ar.get<uint32_t>("sort_k2", n);
+  uint32_t *v2 = ar.get<uint32_t>("sort_v2", n);
+  uint32_t *hist = ar.get<uint32_t>("sort_hist", kRadix * nblocks + 1);
+  uint64_t *off = ar.get<uint64_t>("sort_off", kRadix * nblocks + 2);
+  uint32_t *ka = keys, *va = vals, *kb = k2, *vb = v2;
+  for (int shift = 0; shift < 32; shift += kRadixBits) {
+    hipLaunchKernelGGL(radix_hist_kernel, dim3((unsigned)nblocks),
+                       dim3(kSortThreads), 0, s, ka, n, shift, nblocks, hist);
+    SDF_HIP(hipGetLastError());
+    scan_counts(s, ar, hist, off, kRadix * nblocks);
+    hipLaunchKernelGGL(radix_scatter_kernel, dim3((unsigned)nblocks),
+                       dim3(kSortThreads), 0, s, ka, va, kb, vb, n, shift,
+                       nblocks, off);
+    SDF_HIP(hipGetLastError());
+    uint32_t *t = ka;
+    ka = kb;
+    kb = t;
+    t = va;
+    va = vb;
+    vb = t;
+  }
+  // 4 passes: the result is back in keys/vals
+}
+
+}  // namespace
+
+void radix_sort_u32_pairs(hipStream_t s, Arena &ar, uint32_t *keys,
+                          uint32_t *vals, int64_t n) {
+  radix_sort_pairs(s, ar, keys, vals, n);
+}
+
+Bvh build_lbvh(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t n) {
+  Bvh b;
+  b.n = n;
+  if (n <= 0) return b;
+  b.leaf_box = ar.get<float>("bvh_leaf_box", 6 * n);
+  uint32_t *cb = ar.get<uint32_t>("bvh_cbounds", 8);
+  uint32_t init[8] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u, 0u, 0u};
+  launch_fill_u32(s, cb, 3, init[0]);
+  launch_fill_u32(s, cb + 3, 3, 0u);
+  hipLaunchKernelGGL(leaf_boxes_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,
+                     s, tri, n, b.leaf_box, cb);
+  SDF_HIP(hipGetLastError());
+  uint32_t *codes = ar.get<uint32_t>("bvh_codes", n);
+  uint32_t *leaf_tri = ar.get<uint32_t>("bvh_leaf_tri", n);
+  hipLaunchKernelGGL(morton_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s,
+                     b.leaf_box, n, cb, codes, leaf_tri);
+  SDF_HIP(hipGetLastError());
+  radix_sort_pairs(s, ar, codes, leaf_tri, n);
+  b.leaf_tri = leaf_tri;
+  if (n == 1) {
+    b.nodes = nullptr;
+    return b;
+  }
+  b.nodes = ar.get<BvhNode>("bvh_nodes", n - 1);
+  int32_t *pint = ar.get<int32_t>("bvh_parent_int", n - 1);
+  int32_t *pleaf = ar.get<int32_t>("bvh_parent_leaf", n);
+  uint32_t *visit = ar.get<uint32_t>("bvh_visit", n - 1);
+  float *ibox = ar.get<float>("bvh_int_box", 6 * (n - 1));
+  launch_fill_u32(s, (uint32_t *)pint, 1, 0xFFFFFFFFu);  // root has no parent
+  launch_fill_u32(s, visit, n - 1, 0u);
+  hipLaunchKernelGGL(karras_kernel, dim3(blocks_for(n - 1, 256)), dim3(256), 0,
+                     s, codes, n, b.nodes, pint, pleaf);
+  SDF_HIP(hipGetLastError());
+  hipLaunchKernelGGL(refit_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, n,
+                     b.leaf_box, leaf_tri, pint, pleaf, visit, ibox, b.nodes);
+  SDF_HIP(hipGetLastError());
+  return b;
+}
+
+}  // namespace sdf

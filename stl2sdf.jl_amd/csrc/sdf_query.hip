@@ -8,6 +8,7 @@
 //   generalized_winding_number  (:295-328)
 //   sdf_dists = dists .* signs  src/stl_to_sdf.jl:119
 #include "sdf_internal.h"
+#include "sdf_lbvh.h"
 
 namespace sdf {
 
@@ -307,46 +308,106 @@ __device__ double block_winding_sign(const double *p,
   return fabs(tot) > 0.5 ? 1.0 : -1.0;
 }
 
-__global__ __launch_bounds__(256) void resolve_brute_kernel(
-    GridP g, const double *__restrict__ pts, const Tri64 *__restrict__ tri,
-    int64_t nel, RayParams rp, const Pending *__restrict__ pend,
-    uint32_t npend, FusedOut out, uint32_t *err) {
-  __shared__ double s_t[kHitCap];
-  __shared__ double s_red[256];
-  __shared__ int s_n, s_cnt;
-  for (uint32_t it = blockIdx.x; it < npend; it += gridDim.x) {
-    const Pending P = pend[it];
+
+__device__ inline void item_point(const GridP &g, const double *pts, int64_t idx,
+                                  double *o) {
+  if (pts) {
+    o[0] = pts[3 * idx + 0];
+    o[1] = pts[3 * idx + 1];
+    o[2] = pts[3 * idx + 2];
+  } else {
+    int64_t nxy = (int64_t)g.nx * g.ny;
+    int64_t kk = idx / nxy, rem = idx % nxy;
+    Vec3d q = lattice_point(g, rem % g.nx, rem / g.nx, kk);
+    o[0] = q.x;
+    o[1] = q.y;
+    o[2] = q.z;
+  }
+}
+
+// One thread per (item, ray r in [r0, r1)): parity by LBVH traversal, set
+// as bit r of the item's mask.  Rays whose hit list or stack would overflow
+// go to a fallback list that the block-level brute-force kernel decides.
+__global__ void ray_bits_kernel(GridP g, const double *__restrict__ pts,
+                                const Tri64 *__restrict__ tri, Bvh bvh,
+                                RayParams rp, const Pending *__restrict__ items,
+                                uint32_t nitems, int r0, int r1,
+                                unsigned long long *__restrict__ bits,
+                                uint32_t *__restrict__ fb, uint32_t *nfb,
+                                uint32_t fb_cap, uint32_t *err) {
+  const int nr = r1 - r0;
+  const int64_t total = (int64_t)nitems * nr;
+  for (int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; w < total;
+       w += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t it = (uint32_t)(w / nr);
+    const int r = r0 + (int)(w % nr);
     double o[3];
-    if (pts) {
-      o[0] = pts[3 * P.idx + 0];
-      o[1] = pts[3 * P.idx + 1];
-      o[2] = pts[3 * P.idx + 2];
-    } else {
-      int64_t nxy = (int64_t)g.nx * g.ny;
-      int64_t kk = P.idx / nxy, rem = P.idx % nxy;
-      Vec3d q = lattice_point(g, rem % g.nx, rem / g.nx, kk);
-      o[0] = q.x;
-      o[1] = q.y;
-      o[2] = q.z;
+    item_point(g, pts, items[it].idx, o);
+    int par = ray_parity_bvh(bvh, tri, o, rp.dirs + 3 * r);
+    if (par < 0) {
+      uint32_t slot = atomicAdd(nfb, 1u);
+      if (slot < fb_cap)
+        fb[slot] = it * 64u + (uint32_t)r;
+      else
+        atomicOr(err, kErrHitOverflow);
+    } else if (par) {
+      atomicOr(&bits[it], 1ull << r);
     }
+  }
+}
+
+__global__ __launch_bounds__(256) void ray_bits_brute_kernel(
+    GridP g, const double *__restrict__ pts, const Tri64 *__restrict__ tri,
+    int64_t nel, RayParams rp, const Pending *__restrict__ items,
+    const uint32_t *__restrict__ fb, uint32_t nfb,
+    unsigned long long *__restrict__ bits, uint32_t *err) {
+  __shared__ double s_t[kHitCap];
+  __shared__ int s_n, s_cnt;
+  for (uint32_t q = blockIdx.x; q < nfb; q += gridDim.x) {
+    const uint32_t it = fb[q] / 64u;
+    const int r = (int)(fb[q] % 64u);
+    double o[3];
+    item_point(g, pts, items[it].idx, o);
+    int par = block_ray_parity(o, rp.dirs + 3 * r, tri, nel, s_t, &s_n, &s_cnt, err);
+    if (threadIdx.x == 0 && par) atomicOr(&bits[it], 1ull << r);
+    __syncthreads();
+  }
+}
+
+// Replay of adaptive_point_in_mesh_raycast (:234-288) on precomputed ray
+// parities.  final_phase == 0: only rays below min_rays are available;
+// items that need the batch loop are appended to `next` with their state.
+__global__ void vote_kernel(RayParams rp, const Pending *__restrict__ items,
+                            uint32_t nitems,
+                            const unsigned long long *__restrict__ bits,
+                            int final_phase, FusedOut out,
+                            Pending *__restrict__ next, uint32_t *nnext,
+                            uint32_t *__restrict__ wind, uint32_t *nwind,
+                            double *__restrict__ wconf) {
+  for (uint32_t it = blockIdx.x * blockDim.x + threadIdx.x; it < nitems;
+       it += gridDim.x * blockDim.x) {
+    Pending P = items[it];
+    const unsigned long long m = bits[it];
     int inside = P.inside, total = P.used, used = P.used;
-    // adaptive_point_in_mesh_raycast (:234-288)
     for (int r = used; r < rp.min_rays; r++) {
-      if (block_ray_parity(o, rp.dirs + 3 * r, tri, nel, s_t, &s_n, &s_cnt,
-                           err))
-        inside++;
+      if ((m >> r) & 1ull) inside++;
       total++;
     }
     if (used < rp.min_rays) used = rp.min_rays;
     double ratio = total > 0 ? (double)inside / (double)total : 0.0;
     double conf = ratio > 1.0 - ratio ? ratio : 1.0 - ratio;
+    if (!final_phase && conf < 0.85 && used < rp.max_rays) {
+      P.inside = inside;
+      P.used = used;
+      uint32_t slot = atomicAdd(nnext, 1u);
+      next[slot] = P;
+      continue;
+    }
     while (conf < 0.85 && used < rp.max_rays) {
       int batch = rp.max_rays - used < 4 ? rp.max_rays - used : 4;
       for (int r = used; r < used + batch; r++) {
         if (r < kNumDirs) {
-          if (block_ray_parity(o, rp.dirs + 3 * r, tri, nel, s_t, &s_n,
-                               &s_cnt, err))
-            inside++;
+          if ((m >> r) & 1ull) inside++;
           total++;
         }
       }
@@ -357,36 +418,47 @@ __global__ __launch_bounds__(256) void resolve_brute_kernel(
     }
     double sgn = ratio > 0.5 ? 1.0 : -1.0;
     if (rp.use_winding && conf < rp.conf_thr) {
-      sgn = block_winding_sign(o, tri, nel, s_red);
-      conf = 0.8;
+      uint32_t slot = atomicAdd(nwind, 1u);
+      wind[slot] = it;
+      continue;
     }
+    if (out.sdf) out.sdf[P.idx] = P.dist * sgn;
+    if (out.signs) out.signs[P.idx] = sgn;
+    if (out.conf) out.conf[P.idx] = conf;
+  }
+}
+
+__global__ __launch_bounds__(256) void winding_kernel(
+    GridP g, const double *__restrict__ pts, const Tri64 *__restrict__ tri,
+    int64_t nel, const Pending *__restrict__ items,
+    const uint32_t *__restrict__ wind, uint32_t nwind, FusedOut out) {
+  __shared__ double s_red[256];
+  for (uint32_t q = blockIdx.x; q < nwind; q += gridDim.x) {
+    const Pending P = items[wind[q]];
+    double o[3];
+    item_point(g, pts, P.idx, o);
+    double sgn = block_winding_sign(o, tri, nel, s_red);
     if (threadIdx.x == 0) {
       if (out.sdf) out.sdf[P.idx] = P.dist * sgn;
       if (out.signs) out.signs[P.idx] = sgn;
-      if (out.conf) out.conf[P.idx] = conf;
+      if (out.conf) out.conf[P.idx] = 0.8;
     }
     __syncthreads();
   }
 }
 
-__global__ __launch_bounds__(256) void dist_points_brute_kernel(
-    const double *__restrict__ pts, int64_t npts,
-    const Tri64 *__restrict__ tri, int64_t nel, double h,
-    double *__restrict__ dist) {
-  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= npts) return;
-  const double px = pts[3 * i], py = pts[3 * i + 1], pz = pts[3 * i + 2];
-  double best = __builtin_huge_val();
-  for (int64_t el = 0; el < nel; el++) {
-    const Tri64 t = tri[el];
-    double d2 = tri_dist2<double>(0.0, 0.0, 0.0, t.v[0] - px, t.v[1] - py,
-                                  t.v[2] - pz, t.v[3] - px, t.v[4] - py,
-                                  t.v[5] - pz, t.v[6] - px, t.v[7] - py,
-                                  t.v[8] - pz);
-    best = fmin(best, d2);
+__global__ void dist_points_bvh_kernel(const double *__restrict__ pts,
+                                       int64_t npts, const Tri64 *__restrict__ tri,
+                                       Bvh bvh, double h, float slack,
+                                       double *__restrict__ dist, uint32_t *err) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < npts;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double p[3] = {pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};
+    bool found, ovf;
+    double d = nearest_bvh(bvh, tri, p, h, slack, &found, &ovf);
+    if (ovf) atomicOr(err, kErrStackOverflow);
+    dist[i] = d;
   }
-  double d = sqrt(best);
-  dist[i] = d < h ? d : h;
 }
 
 __global__ void init_pending_all_kernel(Pending *pend, int64_t n) {
@@ -417,24 +489,74 @@ void launch_fused_lattice(hipStream_t s, const GridP &g, const Tri64 *tri,
   SDF_HIP(hipGetLastError());
 }
 
-void launch_resolve_brute(hipStream_t s, const GridP &g, const double *pts,
-                          const Tri64 *tri, int64_t nel, const RayParams &rp,
-                          const Pending *pend, uint32_t npend, FusedOut out,
-                          uint32_t *err) {
-  if (npend == 0) return;
-  unsigned blocks = npend < 65536u ? npend : 65536u;
-  hipLaunchKernelGGL(resolve_brute_kernel, dim3(blocks), dim3(256), 0, s, g,
-                     pts, tri, nel, rp, pend, npend, out, err);
-  SDF_HIP(hipGetLastError());
+
+static int blocks_for(int64_t n, int t) {
+  int64_t b = (n + t - 1) / t;
+  if (b < 1) b = 1;
+  if (b > 65536) b = 65536;
+  return (int)b;
 }
 
-void launch_dist_points_brute(hipStream_t s, const double *pts, int64_t npts,
-                              const Tri64 *tri, int64_t nel, double h,
-                              double *dist) {
+// Decide pending items with the adaptive vote: rays [r0, r1) by LBVH
+// traversal, overflowing rays by brute force, then the vote replay.
+void resolve_pending(hipStream_t s, Arena &ar, const GridP &g, const double *pts,
+                     const Tri64 *tri, int64_t nel, const Bvh &bvh,
+                     const RayParams &rp, const Pending *items, uint32_t nitems,
+                     int r0, int r1, int final_phase, FusedOut out,
+                     Pending *next, uint32_t *d_counts, uint32_t *err,
+                     uint32_t *n_next_host, uint32_t *n_wind_host) {
+  // d_counts: [0] fallback count, [1] next count, [2] winding count
+  if (nitems == 0) {
+    *n_next_host = 0;
+    *n_wind_host = 0;
+    return;
+  }
+  if (r1 > kNumDirs) r1 = kNumDirs;
+  unsigned long long *bits = ar.get<unsigned long long>("res_bits", nitems);
+  launch_fill_u64(s, (uint64_t *)bits, nitems, 0ull);
+  launch_fill_u32(s, d_counts, 4, 0u);
+  uint32_t fb_cap = nitems * 4u + 1024u;
+  uint32_t *fb = ar.get<uint32_t>("res_fb", fb_cap);
+  if (r1 > r0) {
+    int64_t work = (int64_t)nitems * (r1 - r0);
+    hipLaunchKernelGGL(ray_bits_kernel, dim3(blocks_for(work, 64)), dim3(64), 0,
+                       s, g, pts, tri, bvh, rp, items, nitems, r0, r1, bits, fb,
+                       d_counts, fb_cap, err);
+    SDF_HIP(hipGetLastError());
+    uint32_t nfb = 0;
+    SDF_HIP(hipMemcpyAsync(&nfb, d_counts, 4, hipMemcpyDeviceToHost, s));
+    SDF_HIP(hipStreamSynchronize(s));
+    if (nfb > fb_cap) nfb = fb_cap;
+    if (nfb > 0) {
+      hipLaunchKernelGGL(ray_bits_brute_kernel, dim3(nfb < 65536u ? nfb : 65536u),
+                         dim3(256), 0, s, g, pts, tri, nel, rp, items, fb, nfb,
+                         bits, err);
+      SDF_HIP(hipGetLastError());
+    }
+  }
+  uint32_t *wind = ar.get<uint32_t>("res_wind", nitems);
+  hipLaunchKernelGGL(vote_kernel, dim3(blocks_for(nitems, 256)), dim3(256), 0, s,
+                     rp, items, nitems, bits, final_phase, out, next,
+                     d_counts + 1, wind, d_counts + 2, (double *)nullptr);
+  SDF_HIP(hipGetLastError());
+  uint32_t cnt[4] = {0, 0, 0, 0};
+  SDF_HIP(hipMemcpyAsync(cnt, d_counts, 16, hipMemcpyDeviceToHost, s));
+  SDF_HIP(hipStreamSynchronize(s));
+  if (cnt[2] > 0) {
+    hipLaunchKernelGGL(winding_kernel, dim3(cnt[2] < 65536u ? cnt[2] : 65536u),
+                       dim3(256), 0, s, g, pts, tri, nel, items, wind, cnt[2], out);
+    SDF_HIP(hipGetLastError());
+  }
+  *n_next_host = cnt[1];
+  *n_wind_host = cnt[2];
+}
+
+void launch_dist_points_bvh(hipStream_t s, const double *pts, int64_t npts,
+                            const Tri64 *tri, const Bvh &bvh, double h,
+                            float slack, double *dist, uint32_t *err) {
   if (npts <= 0) return;
-  hipLaunchKernelGGL(dist_points_brute_kernel,
-                     dim3((unsigned)((npts + 255) / 256)), dim3(256), 0, s,
-                     pts, npts, tri, nel, h, dist);
+  hipLaunchKernelGGL(dist_points_bvh_kernel, dim3(blocks_for(npts, 256)),
+                     dim3(256), 0, s, pts, npts, tri, bvh, h, slack, dist, err);
   SDF_HIP(hipGetLastError());
 }
 

@@ -119,3 +119,27 @@ def test_repeat_calls_stable():
     a = SignedDistances.signed_distance(mesh, grid)
     b = SignedDistances.signed_distance(mesh, grid)
     assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("name,case", [("c1_icosphere.npz", "icosphere"),
+                                       ("artifacts_05.npz", "artifacts")])
+def test_against_golden_fixtures(name, case):
+    import os
+    g = np.load(os.path.join(_cases.GOLDEN, name))
+    mesh, grid = _cases.icosphere_case() if case == "icosphere" else _cases.artifacts_case(0.5)
+    s, c = SignedDistances.raycast_sign_detection(mesh, grid)
+    assert np.array_equal(np.packbits(s > 0), g["signs"])
+    idx = np.flatnonzero(c != 1.0)
+    assert np.array_equal(idx, g["conf_idx"]) and np.array_equal(c[idx], g["conf_val"])
+    d, _ = SignedDistances.evalDistancesOnTriMesh(mesh, grid)
+    assert np.max(np.abs(d - g["dist"])) <= DIST_TOL * _cases.diag(mesh) + 1e-7 * grid.cell_size
+
+
+def test_general_points_many_pending_star():
+    # off-lattice points near the surface of a 7k-triangle star: LBVH rays
+    mesh, grid = _cases.star_case()
+    rng = np.random.default_rng(11)
+    P = MeshGrid.generateGridPoints(grid)
+    sel = rng.choice(P.shape[1], 4000, replace=False)
+    Q = np.asfortranarray(P[:, sel] + rng.normal(scale=grid.cell_size * 0.3, size=(3, 4000)))
+    _check(mesh, grid, Q)
