@@ -1,6 +1,6 @@
 // Linear BVH over triangles: 30-bit Morton codes of the centroids, an LDS
-// radix sort (4 passes of 8 bits, stable), the Karras (2012) hierarchy and a
-// bottom-up atomic refit of FP32 boxes rounded outward.
+// radix sort (4 passes of 8 bits, stable), the Karras (2012) hierarchy and
+// FP32 node boxes rounded outward, refit from a segment tree over the leaves.
 //
 // Replaces the reference's ImplicitBVH bounding-sphere tree
 // (RaycastingSignDetection.jl:35-56) for ray casting and the triangle-centre
@@ -35,12 +35,14 @@ __device__ inline float order_float(uint32_t u) {
   return __uint_as_float((u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u);
 }
 
-// Triangle boxes (outward-rounded FP32) and the centroid bounds.
-__global__ void leaf_boxes_kernel(const Tri64 *__restrict__ tri, int64_t n,
-                                  float *__restrict__ box,
-                                  uint32_t *__restrict__ cbounds) {
-  float cmin[3] = {3.4e38f, 3.4e38f, 3.4e38f};
-  float cmax[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+// Triangle boxes (outward-rounded FP32) and the centroid bounds (reduced per
+// block in LDS, then one atomic per block and bound).
+__global__ __launch_bounds__(256) void leaf_boxes_kernel(
+    const Tri64 *__restrict__ tri, int64_t n, float *__restrict__ box,
+    uint32_t *__restrict__ cbounds) {
+  __shared__ uint32_t red[6][256];
+  uint32_t cmin[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
+  uint32_t cmax[3] = {0u, 0u, 0u};
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
        t += (int64_t)gridDim.x * blockDim.x) {
     const Tri64 r = tri[t];
@@ -49,15 +51,27 @@ __global__ void leaf_boxes_kernel(const Tri64 *__restrict__ tri, int64_t n,
       double lo = fmin(a, fmin(b, c)), hi = fmax(a, fmax(b, c));
       box[6 * t + d] = __double2float_rd(lo);
       box[6 * t + 3 + d] = __double2float_ru(hi);
-      float cc = (float)((a + b + c) / 3.0);
-      cmin[d] = fminf(cmin[d], cc);
-      cmax[d] = fmaxf(cmax[d], cc);
+      uint32_t cc = float_order((float)((a + b + c) / 3.0));
+      cmin[d] = min(cmin[d], cc);
+      cmax[d] = max(cmax[d], cc);
     }
   }
+  const int tid = threadIdx.x;
   for (int d = 0; d < 3; d++) {
-    atomicMin(&cbounds[d], float_order(cmin[d]));
-    atomicMax(&cbounds[3 + d], float_order(cmax[d]));
+    red[d][tid] = cmin[d];
+    red[3 + d][tid] = cmax[d];
   }
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (tid < off)
+      for (int d = 0; d < 3; d++) {
+        red[d][tid] = min(red[d][tid], red[d][tid + off]);
+        red[3 + d][tid] = max(red[3 + d][tid], red[3 + d][tid + off]);
+      }
+    __syncthreads();
+  }
+  if (tid < 3) atomicMin(&cbounds[tid], red[tid][0]);
+  else if (tid < 6) atomicMax(&cbounds[tid], red[tid][0]);
 }
 
 __device__ inline uint32_t expand10(uint32_t v) {
@@ -175,8 +189,7 @@ __device__ inline int delta(const uint32_t *__restrict__ codes, int64_t n,
 
 __global__ void karras_kernel(const uint32_t *__restrict__ codes, int64_t n,
                               BvhNode *__restrict__ nodes,
-                              int32_t *__restrict__ parent_int,
-                              int32_t *__restrict__ parent_leaf) {
+                              int32_t *__restrict__ range) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n - 1;
        i += (int64_t)gridDim.x * blockDim.x) {
     const int d = (delta(codes, n, i, i + 1) - delta(codes, n, i, i - 1)) >= 0 ? 1 : -1;
@@ -195,66 +208,92 @@ __global__ void karras_kernel(const uint32_t *__restrict__ codes, int64_t n,
     } while (step > 1);
     const int64_t gamma = i + s * d + (d < 0 ? -1 : 0);
     const int64_t lo = i < j ? i : j, hi = i < j ? j : i;
-    int32_t left, right;
-    if (lo == gamma) {
-      left = -(int32_t)(gamma + 1);
-      parent_leaf[gamma] = (int32_t)i;
-    } else {
-      left = (int32_t)gamma;
-      parent_int[gamma] = (int32_t)i;
-    }
-    if (hi == gamma + 1) {
-      right = -(int32_t)(gamma + 2);
-      parent_leaf[gamma + 1] = (int32_t)i;
-    } else {
-      right = (int32_t)(gamma + 1);
-      parent_int[gamma + 1] = (int32_t)i;
-    }
-    nodes[i].child[0] = left;
-    nodes[i].child[1] = right;
+    nodes[i].child[0] = (lo == gamma) ? -(int32_t)(gamma + 1) : (int32_t)gamma;
+    nodes[i].child[1] = (hi == gamma + 1) ? -(int32_t)(gamma + 2) : (int32_t)(gamma + 1);
+    range[2 * i] = (int32_t)lo;
+    range[2 * i + 1] = (int32_t)hi;
   }
 }
 
-__device__ inline void load_box(int32_t c, const float *__restrict__ leaf_box,
-                                const uint32_t *__restrict__ leaf_tri,
-                                const float *int_box, float *b) {
-  if (c < 0) {
-    const uint32_t t = leaf_tri[-(c + 1)];
-    for (int q = 0; q < 6; q++) b[q] = leaf_box[6 * (int64_t)t + q];
-  } else {
-    for (int q = 0; q < 6; q++) b[q] = int_box[6 * (int64_t)c + q];
+// Box refit without inter-workgroup synchronisation.  Karras node i covers
+// the sorted leaf range [lo_i, hi_i]; its box is the union of the leaf boxes
+// of that range, taken from a segment tree over the sorted leaves (level L
+// entry k covers leaves [k 2^L, (k+1) 2^L)), O(log n) boxes per node.
+// Unions are exact, so the result equals a bottom-up refit.
+__global__ void seg_leaf_kernel(const float *__restrict__ leaf_box,
+                                const uint32_t *__restrict__ leaf_tri, int64_t n,
+                                float *__restrict__ seg0) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t t = leaf_tri[i];
+    for (int q = 0; q < 6; q++) seg0[6 * i + q] = leaf_box[6 * t + q];
   }
 }
 
-// Bottom-up refit: the second thread to reach a node (atomic counter with
-// acquire-release ordering at device scope) has both child boxes ready.
-__global__ void refit_kernel(int64_t n, const float *__restrict__ leaf_box,
-                             const uint32_t *__restrict__ leaf_tri,
-                             const int32_t *__restrict__ parent_int,
-                             const int32_t *__restrict__ parent_leaf,
-                             uint32_t *__restrict__ visit, float *int_box,
-                             BvhNode *__restrict__ nodes) {
-  for (int64_t leaf = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; leaf < n;
-       leaf += (int64_t)gridDim.x * blockDim.x) {
-    int32_t node = parent_leaf[leaf];
-    while (node >= 0) {
-      uint32_t old = __hip_atomic_fetch_add(&visit[node], 1u, __ATOMIC_ACQ_REL,
-                                            __HIP_MEMORY_SCOPE_AGENT);
-      if (old == 0) break;
-      float b0[6], b1[6], u[6];
-      load_box(nodes[node].child[0], leaf_box, leaf_tri, int_box, b0);
-      load_box(nodes[node].child[1], leaf_box, leaf_tri, int_box, b1);
+__global__ void seg_level_kernel(const float *__restrict__ below, int64_t nb,
+                                 float *__restrict__ level, int64_t m) {
+  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < m;
+       k += (int64_t)gridDim.x * blockDim.x) {
+    float b[6];
+    for (int q = 0; q < 6; q++) b[q] = below[12 * k + q];
+    if (2 * k + 1 < nb) {
       for (int q = 0; q < 3; q++) {
-        u[q] = fminf(b0[q], b1[q]);
-        u[3 + q] = fmaxf(b0[3 + q], b1[3 + q]);
+        b[q] = fminf(b[q], below[12 * k + 6 + q]);
+        b[3 + q] = fmaxf(b[3 + q], below[12 * k + 9 + q]);
       }
-      for (int q = 0; q < 6; q++) {
-        nodes[node].box[0][q] = b0[q];
-        nodes[node].box[1][q] = b1[q];
-        int_box[6 * (int64_t)node + q] = u[q];
+    }
+    for (int q = 0; q < 6; q++) level[6 * k + q] = b[q];
+  }
+}
+
+struct SegLevels {
+  int64_t off[64];  // float offset of each level inside the segment array
+  int nlev;
+};
+
+__device__ inline void seg_query(const float *__restrict__ seg, const SegLevels &L,
+                                 int64_t lo, int64_t hi, float *b) {
+  for (int q = 0; q < 3; q++) {
+    b[q] = 3.4e38f;
+    b[3 + q] = -3.4e38f;
+  }
+  int64_t l = lo, r = hi + 1;
+  for (int lev = 0; l < r && lev < L.nlev; lev++) {
+    const float *S = seg + L.off[lev];
+    if (l & 1) {
+      for (int q = 0; q < 3; q++) {
+        b[q] = fminf(b[q], S[6 * l + q]);
+        b[3 + q] = fmaxf(b[3 + q], S[6 * l + 3 + q]);
       }
-      __threadfence();
-      node = parent_int[node];
+      l++;
+    }
+    if (r & 1) {
+      r--;
+      for (int q = 0; q < 3; q++) {
+        b[q] = fminf(b[q], S[6 * r + q]);
+        b[3 + q] = fmaxf(b[3 + q], S[6 * r + 3 + q]);
+      }
+    }
+    l >>= 1;
+    r >>= 1;
+  }
+}
+
+__global__ void node_boxes_kernel(const float *__restrict__ seg, SegLevels L,
+                                  const int32_t *__restrict__ range, int64_t n,
+                                  BvhNode *__restrict__ nodes) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n - 1;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    for (int c = 0; c < 2; c++) {
+      const int32_t ch = nodes[i].child[c];
+      float b[6];
+      if (ch < 0) {
+        const int64_t leaf = -(int64_t)(ch + 1);
+        for (int q = 0; q < 6; q++) b[q] = seg[6 * leaf + q];
+      } else {
+        seg_query(seg, L, range[2 * ch], range[2 * ch + 1], b);
+      }
+      for (int q = 0; q < 6; q++) nodes[i].box[c][q] = b[q];
     }
   }
 }
@@ -300,8 +339,7 @@ Bvh build_lbvh(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t n) {
   if (n <= 0) return b;
   b.leaf_box = ar.get<float>("bvh_leaf_box", 6 * n);
   uint32_t *cb = ar.get<uint32_t>("bvh_cbounds", 8);
-  uint32_t init[8] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u, 0u, 0u};
-  launch_fill_u32(s, cb, 3, init[0]);
+  launch_fill_u32(s, cb, 3, 0xFFFFFFFFu);
   launch_fill_u32(s, cb + 3, 3, 0u);
   hipLaunchKernelGGL(leaf_boxes_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,
                      s, tri, n, b.leaf_box, cb);
@@ -318,17 +356,33 @@ Bvh build_lbvh(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t n) {
     return b;
   }
   b.nodes = ar.get<BvhNode>("bvh_nodes", n - 1);
-  int32_t *pint = ar.get<int32_t>("bvh_parent_int", n - 1);
-  int32_t *pleaf = ar.get<int32_t>("bvh_parent_leaf", n);
-  uint32_t *visit = ar.get<uint32_t>("bvh_visit", n - 1);
-  float *ibox = ar.get<float>("bvh_int_box", 6 * (n - 1));
-  launch_fill_u32(s, (uint32_t *)pint, 1, 0xFFFFFFFFu);  // root has no parent
-  launch_fill_u32(s, visit, n - 1, 0u);
+  int32_t *range = ar.get<int32_t>("bvh_range", 2 * (n - 1));
   hipLaunchKernelGGL(karras_kernel, dim3(blocks_for(n - 1, 256)), dim3(256), 0,
-                     s, codes, n, b.nodes, pint, pleaf);
+                     s, codes, n, b.nodes, range);
   SDF_HIP(hipGetLastError());
-  hipLaunchKernelGGL(refit_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, n,
-                     b.leaf_box, leaf_tri, pint, pleaf, visit, ibox, b.nodes);
+  SegLevels L;
+  int64_t total = 0, m = n;
+  L.nlev = 0;
+  while (true) {
+    L.off[L.nlev++] = 6 * total;
+    total += m;
+    if (m == 1 || L.nlev == 64) break;
+    m = (m + 1) / 2;
+  }
+  float *seg = ar.get<float>("bvh_seg", 6 * total);
+  hipLaunchKernelGGL(seg_leaf_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s,
+                     b.leaf_box, leaf_tri, n, seg);
+  SDF_HIP(hipGetLastError());
+  m = n;
+  for (int lev = 1; lev < L.nlev; lev++) {
+    int64_t mm = (m + 1) / 2;
+    hipLaunchKernelGGL(seg_level_kernel, dim3(blocks_for(mm, 256)), dim3(256), 0,
+                       s, seg + L.off[lev - 1], m, seg + L.off[lev], mm);
+    SDF_HIP(hipGetLastError());
+    m = mm;
+  }
+  hipLaunchKernelGGL(node_boxes_kernel, dim3(blocks_for(n - 1, 256)), dim3(256),
+                     0, s, seg, L, range, n, b.nodes);
   SDF_HIP(hipGetLastError());
   return b;
 }

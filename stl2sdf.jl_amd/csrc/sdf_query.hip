@@ -273,42 +273,6 @@ __device__ inline void normalize_ref(const double *a, double *o) {
   o[2] = a[2] * inv;
 }
 
-__device__ double block_winding_sign(const double *p,
-                                     const Tri64 *__restrict__ tri,
-                                     int64_t nel, double *s_red) {
-#pragma clang fp contract(off)
-  const int tid = threadIdx.x;
-  double w = 0.0;
-  for (int64_t el = tid; el < nel; el += blockDim.x) {
-    const Tri64 t = tri[el];
-    double r[3], a[3], b[3], c[3], bc[3];
-    for (int q = 0; q < 3; q++) r[q] = t.v[q] - p[q];
-    normalize_ref(r, a);
-    for (int q = 0; q < 3; q++) r[q] = t.v[3 + q] - p[q];
-    normalize_ref(r, b);
-    for (int q = 0; q < 3; q++) r[q] = t.v[6 + q] - p[q];
-    normalize_ref(r, c);
-    cross3_ref(b, c, bc);
-    double num = dot3_ref(a, bc);
-    double den = ((1.0 + dot3_ref(a, b)) + dot3_ref(b, c)) + dot3_ref(c, a);
-    if (fabs(den) > 1e-10) {
-      double sa = 2.0 * atan2(fabs(num), den);
-      if (num < 0) sa = -sa;
-      w += sa;
-    }
-  }
-  s_red[tid] = w;
-  __syncthreads();
-  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
-    if (tid < off) s_red[tid] += s_red[tid + off];
-    __syncthreads();
-  }
-  double tot = s_red[0] / 12.566370614359172;
-  __syncthreads();
-  return fabs(tot) > 0.5 ? 1.0 : -1.0;
-}
-
-
 __device__ inline void item_point(const GridP &g, const double *pts, int64_t idx,
                                   double *o) {
   if (pts) {
@@ -428,22 +392,70 @@ __global__ void vote_kernel(RayParams rp, const Pending *__restrict__ items,
   }
 }
 
-__global__ __launch_bounds__(256) void winding_kernel(
+// generalized_winding_number (:295-328) split over many workgroups per
+// point: each block sums the solid angles of one triangle chunk (fixed
+// reduction tree), the final pass adds the chunk sums in chunk order, so the
+// result is deterministic.
+__global__ __launch_bounds__(256) void winding_partial_kernel(
     GridP g, const double *__restrict__ pts, const Tri64 *__restrict__ tri,
     int64_t nel, const Pending *__restrict__ items,
-    const uint32_t *__restrict__ wind, uint32_t nwind, FusedOut out) {
+    const uint32_t *__restrict__ wind, uint32_t nwind, int chunks,
+    int64_t chunk, double *__restrict__ partial) {
+#pragma clang fp contract(off)
   __shared__ double s_red[256];
-  for (uint32_t q = blockIdx.x; q < nwind; q += gridDim.x) {
-    const Pending P = items[wind[q]];
+  const int64_t nblk = (int64_t)nwind * chunks;
+  for (int64_t q = blockIdx.x; q < nblk; q += gridDim.x) {
+    const uint32_t w = (uint32_t)(q / chunks);
+    const int64_t c = q % chunks;
+    const Pending P = items[wind[w]];
     double o[3];
     item_point(g, pts, P.idx, o);
-    double sgn = block_winding_sign(o, tri, nel, s_red);
-    if (threadIdx.x == 0) {
-      if (out.sdf) out.sdf[P.idx] = P.dist * sgn;
-      if (out.signs) out.signs[P.idx] = sgn;
-      if (out.conf) out.conf[P.idx] = 0.8;
+    const int64_t e0 = c * chunk, e1 = min(nel, e0 + chunk);
+    double acc = 0.0;
+    for (int64_t el = e0 + threadIdx.x; el < e1; el += blockDim.x) {
+      const Tri64 t = tri[el];
+      double r[3], a[3], b[3], cc[3], bc[3];
+      for (int k = 0; k < 3; k++) r[k] = t.v[k] - o[k];
+      normalize_ref(r, a);
+      for (int k = 0; k < 3; k++) r[k] = t.v[3 + k] - o[k];
+      normalize_ref(r, b);
+      for (int k = 0; k < 3; k++) r[k] = t.v[6 + k] - o[k];
+      normalize_ref(r, cc);
+      cross3_ref(b, cc, bc);
+      double num = dot3_ref(a, bc);
+      double den = ((1.0 + dot3_ref(a, b)) + dot3_ref(b, cc)) + dot3_ref(cc, a);
+      if (fabs(den) > 1e-10) {
+        double sa = 2.0 * atan2(fabs(num), den);
+        if (num < 0) sa = -sa;
+        acc += sa;
+      }
     }
+    s_red[threadIdx.x] = acc;
     __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+      if ((int)threadIdx.x < off) s_red[threadIdx.x] += s_red[threadIdx.x + off];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[q] = s_red[0];
+    __syncthreads();
+  }
+}
+
+__global__ void winding_final_kernel(const Pending *__restrict__ items,
+                                     const uint32_t *__restrict__ wind,
+                                     uint32_t nwind, int chunks,
+                                     const double *__restrict__ partial,
+                                     FusedOut out) {
+  for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < nwind;
+       w += gridDim.x * blockDim.x) {
+    double tot = 0.0;
+    for (int c = 0; c < chunks; c++) tot += partial[(int64_t)w * chunks + c];
+    tot = tot / 12.566370614359172;  // Float64(4pi)
+    const double sgn = fabs(tot) > 0.5 ? 1.0 : -1.0;
+    const Pending P = items[wind[w]];
+    if (out.sdf) out.sdf[P.idx] = P.dist * sgn;
+    if (out.signs) out.signs[P.idx] = sgn;
+    if (out.conf) out.conf[P.idx] = 0.8;
   }
 }
 
@@ -543,8 +555,22 @@ void resolve_pending(hipStream_t s, Arena &ar, const GridP &g, const double *pts
   SDF_HIP(hipMemcpyAsync(cnt, d_counts, 16, hipMemcpyDeviceToHost, s));
   SDF_HIP(hipStreamSynchronize(s));
   if (cnt[2] > 0) {
-    hipLaunchKernelGGL(winding_kernel, dim3(cnt[2] < 65536u ? cnt[2] : 65536u),
-                       dim3(256), 0, s, g, pts, tri, nel, items, wind, cnt[2], out);
+    int64_t chunk = 4096;
+    int chunks = (int)((nel + chunk - 1) / chunk);
+    if (chunks < 1) chunks = 1;
+    if (chunks > 1024) {
+      chunks = 1024;
+      chunk = (nel + chunks - 1) / chunks;
+    }
+    double *partial = ar.get<double>("res_wpartial", (size_t)cnt[2] * chunks);
+    int64_t nblk = (int64_t)cnt[2] * chunks;
+    hipLaunchKernelGGL(winding_partial_kernel,
+                       dim3((unsigned)(nblk < 1048576 ? nblk : 1048576)), dim3(256),
+                       0, s, g, pts, tri, nel, items, wind, cnt[2], chunks, chunk,
+                       partial);
+    SDF_HIP(hipGetLastError());
+    hipLaunchKernelGGL(winding_final_kernel, dim3(blocks_for(cnt[2], 256)),
+                       dim3(256), 0, s, items, wind, cnt[2], chunks, partial, out);
     SDF_HIP(hipGetLastError());
   }
   *n_next_host = cnt[1];
