@@ -122,6 +122,47 @@ sdf_status sdf_signed_distance_device(sdf_ctx *ctx, const double aabb_min[3],
                                       double h, const int64_t N[3],
                                       double *d_sdf_out);
 
+/* remove_sdf_artifacts!(sdf, grid; threshold, min_component_ratio)
+ * -- src/SignedDistances/SdfArtifactRemoval.jl:134-248.
+ * sdf_inout[(N+1) product] is modified in place: interior nodes
+ * (sdf >= threshold) of every 6-connected component other than the largest
+ * whose size is below max(1, round(ratio * largest)) (round half to even)
+ * become -abs(sdf).  *nflipped receives the count. */
+sdf_status sdf_remove_artifacts(sdf_ctx *ctx, double *sdf_inout,
+                                const int64_t N[3], double threshold,
+                                double min_component_ratio, int64_t *nflipped);
+/* Same on a device buffer (synchronous: *nflipped is set on return). */
+sdf_status sdf_remove_artifacts_device(sdf_ctx *ctx, double *d_sdf,
+                                       const int64_t N[3], double threshold,
+                                       double min_component_ratio,
+                                       int64_t *nflipped);
+
+/* analyze_sdf_components(sdf, grid; threshold) -> Dict(root => size)
+ * -- SdfArtifactRemoval.jl:259-318.  Roots are canonical: the smallest
+ * 0-based linear index of each component.  roots_out/sizes_out receive up to
+ * cap entries sorted by root; *ncomp is the total.  labels_out (nullable,
+ * one int64 per node) receives each node's root or -1. */
+sdf_status sdf_analyze_components(sdf_ctx *ctx, const double *sdf,
+                                  const int64_t N[3], double threshold,
+                                  int64_t *labels_out, int64_t *roots_out,
+                                  int64_t *sizes_out, int64_t cap,
+                                  int64_t *ncomp);
+
+/* RBFs_smoothing(dist, grid, is_interpolation, smooth, threshold)
+ * -- src/SdfSmoothing/RBFs4Smoothing.jl:275-320.
+ * dist: (N+1) product doubles; fine_out: prod(N*smooth + 1) floats (i
+ * fastest).  Kernel exp(-(r/h)^2) truncated where it drops to threshold;
+ * interpolation solves K w = Float32.(dist) until the FP64 residual satisfies
+ * ||b - K w|| <= reltol ||b|| (reltol <= 0: sqrt(eps(Float32)), the
+ * reference's cg default); approximation uses w = Float32.(dist).
+ * iters_out / rel_residual_out (nullable): CG iterations and final relative
+ * residual. */
+sdf_status sdf_rbf_smoothing(sdf_ctx *ctx, const double *dist,
+                             const int64_t N[3], double h,
+                             int32_t is_interpolation, int32_t smooth,
+                             double threshold, double reltol, float *fine_out,
+                             int32_t *iters_out, double *rel_residual_out);
+
 /* The 36 prioritised ray directions (generate_ray_directions_prioritized,
  * RaycastingSignDetection.jl:162-227), 3 doubles each. */
 sdf_status sdf_ray_directions(double *dirs_out /* 108 */);

@@ -9,6 +9,7 @@
 #include <math.h>
 #include <string.h>
 
+#include <algorithm>
 #include <exception>
 #include <mutex>
 #include <string>
@@ -446,6 +447,172 @@ sdf_status sdf_grid_setup(const double xmin[3], const double xmax[3],
     }
     *cell_size = h;
     *ngp = prod;
+  });
+}
+
+namespace {
+
+sdf::GridP lattice_only(const int64_t N[3]) {
+  const double o[3] = {0.0, 0.0, 0.0};
+  return make_grid(o, 1.0, N);
+}
+
+int64_t remove_artifacts_dev(sdf_ctx *c, double *d_sdf, const sdf::GridP &g,
+                             double thr, double ratio) {
+  if (grid_points(g) > 0x7FFFFFFFll)
+    fail(SDF_ERR_LIMIT, "artifact removal supports < 2^31 grid points");
+  c->mark("ccl");
+  sdf::CclResult r = sdf::run_ccl(c->stream, c->ar, g, d_sdf, thr);
+  unsigned long long w[2] = {0, 0};
+  SDF_HIP(hipMemcpyAsync(w, r.words, sizeof(w), hipMemcpyDeviceToHost, c->stream));
+  SDF_HIP(hipStreamSynchronize(c->stream));
+  if (w[1] == 0) return 0;  // no interior nodes (SdfArtifactRemoval.jl:152-155)
+  const uint32_t largest_size = (uint32_t)(w[0] >> 32);
+  const int32_t largest = (int32_t)(0xFFFFFFFFu - (uint32_t)(w[0] & 0xFFFFFFFFull));
+  // max(1, round(Int, ratio * largest)) with round-half-even (:209)
+  double x = nearbyint(ratio * (double)largest_size);
+  uint32_t min_size = x < 1.0 ? 1u : (x > 4.0e9 ? 0xFFFFFFFFu : (uint32_t)x);
+  c->mark("flip");
+  sdf::ccl_flip(c->stream, r, largest, min_size, d_sdf);
+  unsigned long long f = 0;
+  SDF_HIP(hipMemcpyAsync(&f, r.words + 2, sizeof(f), hipMemcpyDeviceToHost, c->stream));
+  SDF_HIP(hipStreamSynchronize(c->stream));
+  c->mark("end");
+  return (int64_t)f;
+}
+
+}  // namespace
+
+sdf_status sdf_remove_artifacts(sdf_ctx *c, double *sdf_inout, const int64_t N[3],
+                                double thr, double ratio, int64_t *nflipped) {
+  return guarded([&] {
+    if (!c) fail(SDF_ERR_ARG, "ctx is NULL");
+    std::lock_guard<std::mutex> lk(c->mu);
+    sdf::GridP g = lattice_only(N);
+    const int64_t n = grid_points(g);
+    if (n > 0 && !sdf_inout) fail(SDF_ERR_ARG, "sdf is NULL");
+    begin_call(c);
+    double *d = c->ar.get<double>("art_sdf", n + 1);
+    if (n > 0)
+      SDF_HIP(hipMemcpyAsync(d, sdf_inout, (size_t)n * sizeof(double),
+                             hipMemcpyHostToDevice, c->stream));
+    int64_t f = remove_artifacts_dev(c, d, g, thr, ratio);
+    if (f > 0)
+      SDF_HIP(hipMemcpyAsync(sdf_inout, d, (size_t)n * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
+    SDF_HIP(hipStreamSynchronize(c->stream));
+    if (nflipped) *nflipped = f;
+  });
+}
+
+sdf_status sdf_remove_artifacts_device(sdf_ctx *c, double *d_sdf, const int64_t N[3],
+                                       double thr, double ratio, int64_t *nflipped) {
+  return guarded([&] {
+    if (!c) fail(SDF_ERR_ARG, "ctx is NULL");
+    std::lock_guard<std::mutex> lk(c->mu);
+    sdf::GridP g = lattice_only(N);
+    if (grid_points(g) > 0 && !d_sdf) fail(SDF_ERR_ARG, "sdf is NULL");
+    begin_call(c);
+    int64_t f = remove_artifacts_dev(c, d_sdf, g, thr, ratio);
+    if (nflipped) *nflipped = f;
+  });
+}
+
+sdf_status sdf_analyze_components(sdf_ctx *c, const double *sdf, const int64_t N[3],
+                                  double thr, int64_t *labels_out, int64_t *roots_out,
+                                  int64_t *sizes_out, int64_t cap, int64_t *ncomp) {
+  return guarded([&] {
+    if (!c) fail(SDF_ERR_ARG, "ctx is NULL");
+    std::lock_guard<std::mutex> lk(c->mu);
+    sdf::GridP g = lattice_only(N);
+    const int64_t n = grid_points(g);
+    if (n > 0x7FFFFFFFll) fail(SDF_ERR_LIMIT, "component analysis supports < 2^31 grid points");
+    if (n > 0 && !sdf) fail(SDF_ERR_ARG, "sdf is NULL");
+    if (cap < 0 || (cap > 0 && (!roots_out || !sizes_out)))
+      fail(SDF_ERR_ARG, "roots_out / sizes_out is NULL");
+    begin_call(c);
+    double *d = c->ar.get<double>("art_sdf", n + 1);
+    if (n > 0)
+      SDF_HIP(hipMemcpyAsync(d, sdf, (size_t)n * sizeof(double), hipMemcpyHostToDevice,
+                             c->stream));
+    sdf::CclResult r = sdf::run_ccl(c->stream, c->ar, g, d, thr);
+    unsigned long long w[2] = {0, 0};
+    SDF_HIP(hipMemcpyAsync(w, r.words, sizeof(w), hipMemcpyDeviceToHost, c->stream));
+    SDF_HIP(hipStreamSynchronize(c->stream));
+    const uint32_t nc = (uint32_t)w[1];
+    int64_t *droots = c->ar.get<int64_t>("cc_roots", (size_t)nc + 1);
+    int64_t *dsizes = c->ar.get<int64_t>("cc_sizes", (size_t)nc + 1);
+    uint32_t *dn = c->ar.get<uint32_t>("cc_n", 1);
+    sdf::launch_fill_u32(c->stream, dn, 1, 0u);
+    sdf::ccl_roots(c->stream, r, droots, dsizes, dn, nc);
+    std::vector<int64_t> hr(nc), hs(nc);
+    if (nc > 0) {
+      SDF_HIP(hipMemcpyAsync(hr.data(), droots, nc * sizeof(int64_t), hipMemcpyDeviceToHost,
+                             c->stream));
+      SDF_HIP(hipMemcpyAsync(hs.data(), dsizes, nc * sizeof(int64_t), hipMemcpyDeviceToHost,
+                             c->stream));
+    }
+    if (labels_out && n > 0) {
+      int64_t *dl = c->ar.get<int64_t>("cc_labels", n);
+      sdf::ccl_labels(c->stream, r, dl);
+      SDF_HIP(hipMemcpyAsync(labels_out, dl, (size_t)n * sizeof(int64_t),
+                             hipMemcpyDeviceToHost, c->stream));
+    }
+    SDF_HIP(hipStreamSynchronize(c->stream));
+    std::vector<size_t> ord(nc);
+    for (size_t i = 0; i < nc; i++) ord[i] = i;
+    std::sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return hr[a] < hr[b]; });
+    for (int64_t i = 0; i < cap && i < (int64_t)nc; i++) {
+      roots_out[i] = hr[ord[i]];
+      sizes_out[i] = hs[ord[i]];
+    }
+    if (ncomp) *ncomp = nc;
+  });
+}
+
+sdf_status sdf_rbf_smoothing(sdf_ctx *c, const double *dist, const int64_t N[3],
+                             double h, int32_t is_interp, int32_t smooth,
+                             double threshold, double reltol, float *fine_out,
+                             int32_t *iters_out, double *rel_out) {
+  return guarded([&] {
+    if (!c) fail(SDF_ERR_ARG, "ctx is NULL");
+    std::lock_guard<std::mutex> lk(c->mu);
+    sdf::GridP g = lattice_only(N);
+    (void)h;  // the kernel is expressed in cells: sigma = h
+    if (smooth < 1 || smooth > 8) fail(SDF_ERR_ARG, "smooth must be in 1..8");
+    if (!(threshold > 0.0 && threshold < 1.0)) fail(SDF_ERR_ARG, "threshold must be in (0, 1)");
+    const int64_t n = grid_points(g);
+    const int64_t nf = ((int64_t)(g.nx - 1) * smooth + 1) * ((int64_t)(g.ny - 1) * smooth + 1) *
+                       ((int64_t)(g.nz - 1) * smooth + 1);
+    if (n > 0 && (!dist || !fine_out)) fail(SDF_ERR_ARG, "NULL array");
+    if (reltol <= 0.0) reltol = 3.4526698300124393e-4;  // sqrt(eps(Float32))
+    begin_call(c);
+    double *dd = c->ar.get<double>("rbf_in", n + 1);
+    float *b = c->ar.get<float>("rbf_b", n + 1);
+    float *w = c->ar.get<float>("rbf_w", n + 1);
+    float *fine = c->ar.get<float>("rbf_fine", nf + 1);
+    if (n > 0)
+      SDF_HIP(hipMemcpyAsync(dd, dist, (size_t)n * sizeof(double), hipMemcpyHostToDevice,
+                             c->stream));
+    c->mark("rbf_prepare");
+    sdf::rbf_prepare(c->stream, c->ar, dd, n, b);
+    int iters = 0;
+    double rel = 0.0;
+    c->mark("rbf_weights");
+    if (is_interp) {
+      iters = sdf::rbf_weights(c->stream, c->ar, g, b, threshold, reltol, 50, 2000, w, &rel);
+    } else {
+      SDF_HIP(hipMemcpyAsync(w, b, (size_t)n * sizeof(float), hipMemcpyDeviceToDevice,
+                             c->stream));
+    }
+    c->mark("rbf_fine");
+    sdf::rbf_fine(c->stream, c->ar, g, smooth, threshold, w, fine);
+    c->mark("end");
+    SDF_HIP(hipMemcpyAsync(fine_out, fine, (size_t)nf * sizeof(float), hipMemcpyDeviceToHost,
+                           c->stream));
+    SDF_HIP(hipStreamSynchronize(c->stream));
+    if (iters_out) *iters_out = iters;
+    if (rel_out) *rel_out = rel;
   });
 }
 

@@ -152,4 +152,27 @@ void launch_dist_points_bvh(hipStream_t s, const double *pts, int64_t npts,
 
 void launch_init_pending_all(hipStream_t s, Pending *pend, int64_t npts);
 
+// connected components (sdf_ccl.hip)
+struct CclResult {
+  int64_t n = 0;
+  int32_t *lab = nullptr;   // canonical root (smallest index) or -1
+  uint32_t *size = nullptr; // component size, indexed by root
+  unsigned long long *words = nullptr;  // [0] largest packed, [1] ncomp, [2] flipped
+};
+CclResult run_ccl(hipStream_t s, Arena &ar, const GridP &g, const double *sdf,
+                  double thr);
+void ccl_flip(hipStream_t s, const CclResult &r, int32_t largest,
+              uint32_t min_size, double *sdf);
+void ccl_roots(hipStream_t s, const CclResult &r, int64_t *roots, int64_t *sizes,
+               uint32_t *nroots, uint32_t cap);
+void ccl_labels(hipStream_t s, const CclResult &r, int64_t *out);
+
+// RBF smoothing (sdf_rbf.hip)
+void rbf_prepare(hipStream_t s, Arena &ar, const double *d, int64_t n, float *f32);
+int rbf_weights(hipStream_t s, Arena &ar, const GridP &g, const float *b,
+                double threshold, double reltol, int max_outer, int max_inner,
+                float *w_out, double *final_rel);
+void rbf_fine(hipStream_t s, Arena &ar, const GridP &g, int smooth,
+              double threshold, const float *w, float *fine);
+
 }  // namespace sdf

@@ -86,6 +86,56 @@ def signed_distance(mesh, grid, device=0):
     return out
 
 
+def _check_len(sdf_values, grid):
+    v = np.asarray(sdf_values)
+    if v.ndim != 1 or v.shape[0] != grid.ngp:
+        # SdfArtifactRemoval.jl:144-146
+        raise ValueError(f"SDF values length ({v.size}) doesn't match grid points ({grid.ngp})")
+
+
+def remove_sdf_artifacts_(sdf_values, grid, threshold=0.0, min_component_ratio=0.01, device=0):
+    """remove_sdf_artifacts!(sdf_values, grid; threshold, min_component_ratio)
+    -- SdfArtifactRemoval.jl:134-248.  Modifies ``sdf_values`` (float64,
+    contiguous) in place and returns the number of flipped nodes."""
+    _check_len(sdf_values, grid)
+    if not (isinstance(sdf_values, np.ndarray) and sdf_values.dtype == np.float64
+            and sdf_values.flags.c_contiguous):
+        raise TypeError("sdf_values must be a contiguous float64 numpy array (modified in place)")
+    n = C.c_int64(0)
+    _lib.check(_lib.lib().sdf_remove_artifacts(
+        _lib.ctx(device), _lib.dptr(sdf_values), _lib.i64ptr(_lib.i64_3(grid.N)),
+        float(threshold), float(min_component_ratio), C.byref(n)))
+    return int(n.value)
+
+
+def analyze_sdf_components(sdf_values, grid, threshold=0.0, device=0, labels=False):
+    """analyze_sdf_components(sdf_values, grid; threshold) -> {root: size}
+    -- SdfArtifactRemoval.jl:259-318.  Roots are canonical: the smallest
+    1-based linear index of each component (the reference's roots depend on
+    its union order).  labels=True also returns the per-node 1-based root
+    (0 for exterior nodes)."""
+    _check_len(sdf_values, grid)
+    v = np.ascontiguousarray(sdf_values, dtype=np.float64)
+    N = _lib.i64_3(grid.N)
+    L = _lib.lib()
+    c = _lib.ctx(device)
+    ncomp = C.c_int64(0)
+    lab = np.empty(v.size, np.int64) if labels else None
+    _lib.check(L.sdf_analyze_components(c, _lib.dptr(v), _lib.i64ptr(N), float(threshold),
+                                        _lib.i64ptr(lab) if labels else None, None, None, 0,
+                                        C.byref(ncomp)))
+    k = int(ncomp.value)
+    roots = np.empty(max(k, 1), np.int64)
+    sizes = np.empty(max(k, 1), np.int64)
+    _lib.check(L.sdf_analyze_components(c, _lib.dptr(v), _lib.i64ptr(N), float(threshold), None,
+                                        _lib.i64ptr(roots), _lib.i64ptr(sizes), k,
+                                        C.byref(ncomp)))
+    d = {int(r) + 1: int(s) for r, s in zip(roots[:k], sizes[:k])}
+    if labels:
+        return d, np.where(lab >= 0, lab + 1, 0)
+    return d
+
+
 def ray_directions():
     """The 36 prioritised directions (RaycastingSignDetection.jl:162-227)."""
     d = np.zeros(108, dtype=np.float64)

@@ -44,6 +44,13 @@ SIGNATURES = {
     "sdf_mesh_set_device": (C.c_int32, [_vp, _vp, C.c_int64, _vp, C.c_int64]),
     "sdf_signed_distance_device": (C.c_int32, [_vp, _dp, C.c_double, _i64p, _vp]),
     "sdf_ray_directions": (C.c_int32, [_dp]),
+    "sdf_remove_artifacts": (C.c_int32, [_vp, _dp, _i64p, C.c_double, C.c_double, _i64p]),
+    "sdf_remove_artifacts_device": (C.c_int32, [_vp, _vp, _i64p, C.c_double, C.c_double,
+                                                _i64p]),
+    "sdf_analyze_components": (C.c_int32, [_vp, _dp, _i64p, C.c_double, _i64p, _i64p, _i64p,
+                                           C.c_int64, _i64p]),
+    "sdf_rbf_smoothing": (C.c_int32, [_vp, _dp, _i64p, C.c_double, C.c_int32, C.c_int32,
+                                      C.c_double, C.c_double, _fp, _i32p, _dp]),
 }
 
 
