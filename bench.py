@@ -103,7 +103,19 @@ def main():
     ap.add_argument("--workload", default="star512")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
+                    help="only copy an MB-sized device buffer 3 times (counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        a = torch.empty(args.copy_probe * (1 << 20) // 4, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1.0)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": a.numel() * 4}))
+        return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(each GPU would take its own z-slab of the grid with no exchange).",

@@ -259,14 +259,25 @@ void run_lattice(sdf_ctx *c, const sdf::GridP &g, bool do_dist, bool do_sign,
   int64_t npts = grid_points(g);
   sdf::Bins bins;
   c->mark("bin_tiles");
-  if (do_dist) sdf::build_tile_bins(c->stream, c->ar, c->tri, c->nel, g, g.h, bins);
+  float *d2buf = nullptr;
+  if (do_dist) {
+    sdf::build_tile_bins(c->stream, c->ar, c->tri, c->nel, g, g.h, bins);
+    sdf::build_dist_chunks(c->stream, c->ar, g, bins);
+  }
   c->mark("bin_lines");
   if (do_sign) sdf::build_line_hits(c->stream, c->ar, c->tri, c->nel, g, bins);
+  if (do_dist) {
+    c->mark("dist");
+    d2buf = c->ar.get<float>("d2buf", (size_t)bins.nchunks * sdf::kTilePts + 1);
+    sdf::launch_fill_u32(c->stream, (uint32_t *)d2buf,
+                         (int64_t)bins.nchunks * sdf::kTilePts, 0x7F800000u);
+    sdf::launch_dist_chunks(c->stream, g, c->tri, bins, d2buf);
+  }
   uint32_t cap = (uint32_t)(npts < (1 << 22) ? (npts > 0 ? npts : 1) : (1 << 22));
   sdf::Pending *pend = c->ar.get<sdf::Pending>("pend", cap);
   c->mark("fused");
-  sdf::launch_fused_lattice(c->stream, g, c->tri, bins, do_dist, do_sign, out,
-                            pend, c->d_words + 1, cap, c->d_words);
+  sdf::launch_fused_lattice(c->stream, g, c->tri, bins, d2buf, do_sign, out, pend,
+                            c->d_words + 1, cap, c->d_words);
   uint32_t np = 0;
   if (do_sign) {
     np = read_word(c, 1);
@@ -278,8 +289,8 @@ void run_lattice(sdf_ctx *c, const sdf::GridP &g, bool do_dist, bool do_sign,
       cap = np;
       pend = c->ar.get<sdf::Pending>("pend", cap);
       sdf::launch_fill_u32(c->stream, c->d_words, 4, 0u);
-      sdf::launch_fused_lattice(c->stream, g, c->tri, bins, do_dist, do_sign,
-                                out, pend, c->d_words + 1, cap, c->d_words);
+      sdf::launch_fused_lattice(c->stream, g, c->tri, bins, d2buf, do_sign, out, pend,
+                                c->d_words + 1, cap, c->d_words);
       np = read_word(c, 1);
     }
   }

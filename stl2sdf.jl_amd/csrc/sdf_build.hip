@@ -359,6 +359,146 @@ __global__ void line_pairs_kernel(const Tri64 *__restrict__ tri, int64_t nel,
   }
 }
 
+__global__ void tile_chunks_kernel(const uint64_t *__restrict__ tile_off,
+                                   int64_t ntiles, uint32_t *__restrict__ nchunk) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < ntiles;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t c = tile_off[t + 1] - tile_off[t];
+    nchunk[t] = (uint32_t)((c + kTilePts - 1) / kTilePts);
+  }
+}
+
+__global__ void chunk_items_kernel(const uint64_t *__restrict__ chunk_off,
+                                   int64_t ntiles, uint32_t *__restrict__ items) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < ntiles;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    for (uint64_t q = chunk_off[t]; q < chunk_off[t + 1]; q++) items[q] = (uint32_t)t;
+  }
+}
+
+// Fast-path summaries of each axis line (one thread per line).
+//
+// Error model (FP64 without contraction, unit roundoff u = 2^-53): for a hit
+// with record (v1, e1, e2, f) on a line along axis a, the reference's
+// t(x) = f * dot(e2, cross(o - v1, e1)) at the point with coordinate x is
+// compared with c - x, where c = o0_a + t(o0) is formed once at the line's
+// first point o0.  In exact arithmetic on the FP64 inputs, t is affine in x
+// with slope -f * a_exact = -1 + beta, |beta| <= 2u (M_a |f| + 1), M_a the
+// two products forming a.  The evaluation error of t is at most
+// 8u (|f| M_t + |t|), M_t = sum_i |e2_i| (S_j |e1_k| + S_k |e1_j|) with S the
+// largest |o - v1| components on the line.  Hence
+//   |t(x) - (c - x)| <= beta L + 2 E + u (|c| + |t0| + 2L) =: delta / 2
+// (L the line length); delta stores twice that, rounded up to FP32.  Two
+// consecutive sorted crossings c_k, c_k+1 belong to one cluster when
+// c_k+1 - c_k < eps - 2 Dmax and start a new one when the gap exceeds
+// eps + 2 Dmax (Dmax the line's largest delta); any gap in between makes
+// the whole line take the exact path.
+__global__ void line_prep_kernel(GridP g, const uint64_t *__restrict__ line_off,
+                                 const LineHit *__restrict__ hits, int64_t base1,
+                                 int64_t base2, int64_t nlines,
+                                 LineFast *__restrict__ fast,
+                                 uint32_t *__restrict__ meta) {
+#pragma clang fp contract(off)
+  const double u = 1.1102230246251565e-16;
+  for (int64_t L = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; L < nlines;
+       L += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t hb = line_off[L], he = line_off[L + 1];
+    const int64_t n = (int64_t)(he - hb);
+    if (n == 0) {
+      meta[L] = 0u;
+      continue;
+    }
+    if (n > kLineFastMax) {
+      meta[L] = kLineExact;
+      continue;
+    }
+    int ax;
+    int64_t a, b;
+    if (L < base1) {
+      ax = 0;
+      a = L % g.ny;
+      b = L / g.ny;
+    } else if (L < base2) {
+      ax = 1;
+      a = (L - base1) % g.nx;
+      b = (L - base1) / g.nx;
+    } else {
+      ax = 2;
+      a = (L - base2) % g.nx;
+      b = (L - base2) / g.nx;
+    }
+    double o0[3], on[3];
+    const int32_t nax = ax == 0 ? g.nx : (ax == 1 ? g.ny : g.nz);
+    if (ax == 0) {
+      o0[1] = on[1] = lattice_coord(g.oy, g.h, a);
+      o0[2] = on[2] = lattice_coord(g.oz, g.h, b);
+      o0[0] = lattice_coord(g.ox, g.h, 0);
+      on[0] = lattice_coord(g.ox, g.h, nax - 1);
+    } else if (ax == 1) {
+      o0[0] = on[0] = lattice_coord(g.ox, g.h, a);
+      o0[2] = on[2] = lattice_coord(g.oz, g.h, b);
+      o0[1] = lattice_coord(g.oy, g.h, 0);
+      on[1] = lattice_coord(g.oy, g.h, nax - 1);
+    } else {
+      o0[0] = on[0] = lattice_coord(g.ox, g.h, a);
+      o0[1] = on[1] = lattice_coord(g.oy, g.h, b);
+      o0[2] = lattice_coord(g.oz, g.h, 0);
+      on[2] = lattice_coord(g.oz, g.h, nax - 1);
+    }
+    const double len = on[ax] - o0[ax];
+    const int j = (ax + 1) % 3, k = (ax + 2) % 3;
+    double dmax = 0.0;
+    for (int64_t q = 0; q < n; q++) {
+      const LineHit r = hits[hb + q];
+      const double t0 = line_hit_t(r, o0);
+      const double c = o0[ax] + t0;
+      double S[3];
+      for (int d = 0; d < 3; d++) S[d] = fabs(o0[d] - r.v1[d]);
+      S[ax] = fmax(fabs(o0[ax] - r.v1[ax]), fabs(on[ax] - r.v1[ax]));
+      double Mt = 0.0;
+      for (int i = 0; i < 3; i++) {
+        const int i1 = (i + 1) % 3, i2 = (i + 2) % 3;
+        Mt += fabs(r.e2[i]) * (S[i1] * fabs(r.e1[i2]) + S[i2] * fabs(r.e1[i1]));
+      }
+      const double af = fabs(r.f);
+      const double Ma = fabs(r.e1[j] * r.e2[k]) + fabs(r.e1[k] * r.e2[j]);
+      const double beta = 2.0 * u * (Ma * af + 1.0);
+      const double E = 8.0 * u * (af * Mt + fabs(t0) + 2.0 * len);
+      double delta = 2.0 * (beta * len + 2.0 * E + u * (fabs(c) + fabs(t0) + 2.0 * len)) +
+                     1e-300;
+      delta *= 1.0 + 1e-6;
+      LineFast lf;
+      lf.c = c;
+      lf.delta = __double2float_ru(delta);
+      lf.cid = 0;
+      lf.pad_ = 0;
+      // insertion into the sorted prefix (global memory, small n)
+      int64_t p = q;
+      while (p > 0 && fast[hb + p - 1].c > c) {
+        fast[hb + p] = fast[hb + p - 1];
+        p--;
+      }
+      fast[hb + p] = lf;
+      dmax = fmax(dmax, (double)lf.delta);
+    }
+    uint32_t cl = 0;
+    bool ok = true;
+    double prev = fast[hb].c;
+    for (int64_t q = 1; q < n && ok; q++) {
+      const double c = fast[hb + q].c;
+      const double gap = c - prev;
+      const double m = 2.0 * dmax + 4.0 * u * fabs(gap) + 1e-300;
+      if (gap - kRayEps > m)
+        cl++;
+      else if (!(gap - kRayEps < -m))
+        ok = false;
+      fast[hb + q].cid = (uint16_t)cl;
+      prev = c;
+    }
+    meta[L] = ok ? (cl + 1u) : kLineExact;
+  }
+}
+
 uint64_t read_u64(hipStream_t s, const uint64_t *d) {
   uint64_t v = 0;
   SDF_HIP(hipMemcpyAsync(&v, d, sizeof(v), hipMemcpyDeviceToHost, s));
@@ -441,6 +581,26 @@ void build_tile_bins(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
   b.ntile_pairs = (int64_t)npairs;
 }
 
+void build_dist_chunks(hipStream_t s, Arena &ar, const GridP &g, Bins &b) {
+  TileDims td = tile_dims(g);
+  uint32_t *nch = ar.get<uint32_t>("chunk_cnt", td.ntiles + 1);
+  uint64_t *coff = ar.get<uint64_t>("chunk_off", td.ntiles + 2);
+  hipLaunchKernelGGL(tile_chunks_kernel, dim3(grid_for(td.ntiles, 256)), dim3(256),
+                     0, s, (const uint64_t *)b.tile_off, td.ntiles, nch);
+  SDF_HIP(hipGetLastError());
+  scan_counts(s, ar, nch, coff, td.ntiles);
+  const uint64_t n = read_u64(s, coff + td.ntiles);
+  uint32_t *items = ar.get<uint32_t>("chunk_tile", n + 1);
+  if (n > 0) {
+    hipLaunchKernelGGL(chunk_items_kernel, dim3(grid_for(td.ntiles, 256)), dim3(256),
+                       0, s, (const uint64_t *)coff, td.ntiles, items);
+    SDF_HIP(hipGetLastError());
+  }
+  b.chunk_off = coff;
+  b.chunk_tile = items;
+  b.nchunks = (int64_t)n;
+}
+
 void build_line_hits(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
                      const GridP &g, Bins &b) {
   int64_t nlx = (int64_t)g.ny * g.nz, nly = (int64_t)g.nx * g.nz,
@@ -480,8 +640,16 @@ void build_line_hits(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
                        nlx + nly, 1, (uint32_t *)nullptr, loff, lcnt, hits);
     SDF_HIP(hipGetLastError());
   }
+  LineFast *fast = ar.get<LineFast>("line_fast", nhits + 1);
+  uint32_t *meta = ar.get<uint32_t>("line_meta", nlines + 1);
+  hipLaunchKernelGGL(line_prep_kernel, dim3(grid_for(nlines, 256)), dim3(256), 0, s, g,
+                     (const uint64_t *)loff, (const LineHit *)hits, nlx, nlx + nly,
+                     nlines, fast, meta);
+  SDF_HIP(hipGetLastError());
   b.line_off = loff;
   b.line_hits = hits;
+  b.line_fast = fast;
+  b.line_meta = meta;
   b.nline_hits = (int64_t)nhits;
 }
 

@@ -159,6 +159,20 @@ __host__ __device__ inline bool line_hit_test(const double *o, int axis,
   return true;
 }
 
+// Certified summary of one hit for the per-point fast path: the crossing
+// coordinate c along the line (c - x approximates the reference's t for the
+// point at coordinate x), a rigorous bound delta on |t(x) - (c - x)| over the
+// whole line, and the hit's cluster index among the line's sorted hits
+// (clusters: maximal runs whose consecutive gaps are <= eps).
+struct LineFast {
+  double c;
+  float delta;
+  uint16_t cid;
+  uint16_t pad_;
+};
+constexpr uint32_t kLineExact = 0x80000000u;  // line meta flag: no fast path
+constexpr int kLineFastMax = 64;             // lines with more hits: exact path
+
 // t for origin o on the +axis ray of a recorded line hit (reference order).
 __host__ __device__ inline double line_hit_t(const LineHit &r, const double *o) {
 #pragma clang fp contract(off)
@@ -253,5 +267,90 @@ __host__ __device__ inline T tri_dist2(T px, T py, T pz, T ax, T ay, T az, T bx,
 struct Tri64 {
   double v[9];
 };
+
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2).  Map launch index b of B so that XCD x receives the contiguous range
+// [x*B/8, (x+1)*B/8) of work items: neighbouring tiles then share one L2.
+constexpr int kNumXcd = 8;
+__device__ inline int64_t xcd_remap(int64_t b, int64_t B) {
+  const int64_t per = B / kNumXcd, rem = B % kNumXcd;
+  const int64_t x = b % kNumXcd, local = b / kNumXcd;
+  return x * per + (x < rem ? x : rem) + local;
+}
+
+// Per-triangle data for the branch-free point-triangle distance, relative
+// to a tile origin (FP32).  Fields, each 3 floats unless noted: a, b, c,
+// ab, bc, ca, il (1/|edge|^2, 0 for a zero-length edge), n = ab x (c - a),
+// inn (1 float: 1/|n|^2, 0 when degenerate), mab = n x ab, mbc, mca.
+constexpr int kTriPreFloats = 34;
+
+__device__ inline void tri_pre_make(const float *a, const float *b,
+                                    const float *c, float *o) {
+  float ab[3], bc[3], ca[3], ac[3], n[3];
+  for (int k = 0; k < 3; k++) {
+    ab[k] = b[k] - a[k];
+    bc[k] = c[k] - b[k];
+    ca[k] = a[k] - c[k];
+    ac[k] = c[k] - a[k];
+  }
+  n[0] = ab[1] * ac[2] - ab[2] * ac[1];
+  n[1] = ab[2] * ac[0] - ab[0] * ac[2];
+  n[2] = ab[0] * ac[1] - ab[1] * ac[0];
+  const float nn = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+  const float *E[3] = {ab, bc, ca};
+  for (int k = 0; k < 3; k++) {
+    o[k] = a[k];
+    o[3 + k] = b[k];
+    o[6 + k] = c[k];
+    o[9 + k] = ab[k];
+    o[12 + k] = bc[k];
+    o[15 + k] = ca[k];
+    o[21 + k] = n[k];
+  }
+  for (int e = 0; e < 3; e++) {
+    const float l2 = E[e][0] * E[e][0] + E[e][1] * E[e][1] + E[e][2] * E[e][2];
+    o[18 + e] = l2 > 0.f ? 1.0f / l2 : 0.f;
+    float m[3] = {n[1] * E[e][2] - n[2] * E[e][1], n[2] * E[e][0] - n[0] * E[e][2],
+                  n[0] * E[e][1] - n[1] * E[e][0]};
+    for (int k = 0; k < 3; k++) o[25 + 3 * e + k] = m[k];
+  }
+  o[24] = nn > 0.f ? 1.0f / nn : 0.f;
+}
+
+// Squared distance from p to the triangle: the closest point lies in the
+// face interior (all three edge functions (p - v) . (n x edge) >= 0, then
+// the plane distance) or on one of the three edges (clamped projection);
+// the minimum of those four candidates, without branches.  T(i) reads
+// field i of the precomputed record.
+template <typename Get>
+__device__ inline float tri_pre_dist2(float px, float py, float pz, Get T) {
+  const float pax = px - T(0), pay = py - T(1), paz = pz - T(2);
+  const float pbx = px - T(3), pby = py - T(4), pbz = pz - T(5);
+  const float pcx = px - T(6), pcy = py - T(7), pcz = pz - T(8);
+  const float wc = pax * T(25) + pay * T(26) + paz * T(27);
+  const float wa = pbx * T(28) + pby * T(29) + pbz * T(30);
+  const float wb = pcx * T(31) + pcy * T(32) + pcz * T(33);
+  const float pn = pax * T(21) + pay * T(22) + paz * T(23);
+  const float inn = T(24);
+  const bool inside = (wa >= 0.f) & (wb >= 0.f) & (wc >= 0.f) & (inn > 0.f);
+  const float face = inside ? pn * pn * inn : __builtin_huge_valf();
+  float t, rx, ry, rz;
+  t = fminf(fmaxf((pax * T(9) + pay * T(10) + paz * T(11)) * T(18), 0.f), 1.f);
+  rx = pax - t * T(9);
+  ry = pay - t * T(10);
+  rz = paz - t * T(11);
+  const float d0 = rx * rx + ry * ry + rz * rz;
+  t = fminf(fmaxf((pbx * T(12) + pby * T(13) + pbz * T(14)) * T(19), 0.f), 1.f);
+  rx = pbx - t * T(12);
+  ry = pby - t * T(13);
+  rz = pbz - t * T(14);
+  const float d1 = rx * rx + ry * ry + rz * rz;
+  t = fminf(fmaxf((pcx * T(15) + pcy * T(16) + pcz * T(17)) * T(20), 0.f), 1.f);
+  rx = pcx - t * T(15);
+  ry = pcy - t * T(16);
+  rz = pcz - t * T(17);
+  const float d2 = rx * rx + ry * ry + rz * rz;
+  return fminf(fminf(face, d0), fminf(d1, d2));
+}
 
 }  // namespace sdf

@@ -106,13 +106,20 @@ struct Bins {
   uint64_t *line_off = nullptr;  // nlines + 1
   LineHit *line_hits = nullptr;
   int64_t line_base[4] = {0, 0, 0, 0};
+  LineFast *line_fast = nullptr;   // per hit, sorted by c within each line
+  uint32_t *line_meta = nullptr;   // per line: cluster count | kLineExact
   int64_t ntile_pairs = 0, nline_hits = 0;
+  // distance work items: each non-empty tile's list in chunks of 64
+  uint64_t *chunk_off = nullptr;  // ntiles + 1 (first work item of a tile)
+  uint32_t *chunk_tile = nullptr;  // work item -> tile
+  int64_t nchunks = 0;
 };
 
 void build_tile_bins(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
                      const GridP &g, double radius, Bins &b);
 void build_line_hits(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
                      const GridP &g, Bins &b);
+void build_dist_chunks(hipStream_t s, Arena &ar, const GridP &g, Bins &b);
 
 // query side
 struct FusedOut {
@@ -122,8 +129,13 @@ struct FusedOut {
   double *conf = nullptr;   // confidence (nullable)
 };
 
+// Distance work items -> squared distances per point of non-empty tiles
+// (d2buf: 64 floats per work item slot, indexed by the tile's first item).
+void launch_dist_chunks(hipStream_t s, const GridP &g, const Tri64 *tri,
+                        const Bins &b, float *d2buf);
+// Per tile: distance from d2buf (or h), six axis rays, combine, outputs.
 void launch_fused_lattice(hipStream_t s, const GridP &g, const Tri64 *tri,
-                          const Bins &b, bool do_dist, bool do_sign,
+                          const Bins &b, const float *d2buf, bool do_sign,
                           FusedOut out, Pending *pend, uint32_t *npend,
                           uint32_t pend_cap, uint32_t *err);
 

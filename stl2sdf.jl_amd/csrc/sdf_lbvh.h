@@ -50,78 +50,6 @@ __device__ inline bool ray_box(const double *o, const double *d,
   return tmin <= tmax;
 }
 
-// Parity of one ray (check_ray_inside_bvh, RaycastingSignDetection.jl:
-// 114-154) by LBVH traversal.  Hits are kept in a private list of kMaxHits
-// values; returns -1 when the list or the stack would overflow (the caller
-// then decides the ray another way).
-constexpr int kMaxHits = 32;
-
-__device__ inline int ray_parity_bvh(const Bvh &bvh, const Tri64 *__restrict__ tri,
-                                     const double *o, const double *d) {
-  double inv[3];
-  for (int a = 0; a < 3; a++) inv[a] = d[a] != 0.0 ? 1.0 / d[a] : 0.0;
-  double hits[kMaxHits];
-  int nh = 0;
-  auto leaf = [&](uint32_t t) -> bool {
-    const Tri64 r = tri[t];
-    double tv;
-    if (mt_ref(o, d, r.v, r.v + 3, r.v + 6, &tv)) {
-      if (nh >= kMaxHits) return false;
-      hits[nh++] = tv;
-    }
-    return true;
-  };
-  if (bvh.n == 0) return 0;
-  if (bvh.n == 1) {
-    if (ray_box(o, d, inv, bvh.leaf_box + 6 * (int64_t)bvh.leaf_tri[0]))
-      if (!leaf(bvh.leaf_tri[0])) return -1;
-  } else {
-    int32_t stack[kStackDepth];
-    int sp = 0;
-    int32_t node = 0;
-    while (true) {
-      const BvhNode nd = bvh.nodes[node];
-      bool h0 = ray_box(o, d, inv, nd.box[0]);
-      bool h1 = ray_box(o, d, inv, nd.box[1]);
-      int32_t next = -0x7FFFFFFF;
-      for (int c = 0; c < 2; c++) {
-        if (!(c == 0 ? h0 : h1)) continue;
-        int32_t ch = nd.child[c];
-        if (ch < 0) {
-          if (!leaf(bvh.leaf_tri[-(ch + 1)])) return -1;
-        } else if (next == -0x7FFFFFFF) {
-          next = ch;
-        } else {
-          if (sp >= kStackDepth) return -1;
-          stack[sp++] = ch;
-        }
-      }
-      if (next != -0x7FFFFFFF) {
-        node = next;
-      } else if (sp > 0) {
-        node = stack[--sp];
-      } else {
-        break;
-      }
-    }
-  }
-  // unique count with the reference's rule (sorted, gap > eps)
-  for (int a = 1; a < nh; a++) {
-    double x = hits[a];
-    int b = a - 1;
-    while (b >= 0 && hits[b] > x) {
-      hits[b + 1] = hits[b];
-      b--;
-    }
-    hits[b + 1] = x;
-  }
-  if (nh == 0) return 0;
-  int u = 1;
-  for (int a = 1; a < nh; a++)
-    if (fabs(hits[a] - hits[a - 1]) > kRayEps) u++;
-  return u & 1;
-}
-
 // Squared distance from p (FP32, absolute) to a box.
 __device__ inline float box_dist2(const float *p, const float *b) {
   float s = 0.f;

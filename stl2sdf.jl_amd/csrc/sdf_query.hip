@@ -113,76 +113,147 @@ __device__ inline void axis_pair_parity(const LineHit *__restrict__ hits,
   *neg_par = cn & 1;
 }
 
-__global__ __launch_bounds__(kTilePts) void fused_lattice_kernel(
+// Distance work item = (tile, chunk of <= 64 listed triangles).  The chunk
+// is staged into LDS as tile-relative FP32 records (each lane converts one
+// triangle from FP64: v - tile_origin is formed in FP64, so the FP32 values
+// are small and accurate), then every lane takes the minimum over the chunk
+// for its point.  Tiles with several chunks combine by atomicMin on the
+// float bits (non-negative floats order like their bit patterns).
+// Fast path for one axis line: classify every hit of the line for the point
+// at coordinate x from its certified crossing (c - x vs +-eps, margin delta)
+// and count clusters.  Returns false when any decision falls inside a margin
+// or the line is flagged exact; the caller then evaluates the reference's
+// arithmetic.  +dir accepts t > eps (a suffix in crossing order), -dir
+// accepts t < -eps (a prefix).
+__device__ inline bool axis_pair_parity_fast(const LineFast *__restrict__ fast,
+                                             uint64_t hb, uint64_t he, uint32_t meta,
+                                             double x, int *pos_par, int *neg_par) {
+  if (meta & kLineExact) return false;
+  const int ncl = (int)(meta & 0xFFFFu);
+  int first_pos = -1, last_neg = -1;
+  for (uint64_t q = hb; q < he; q++) {
+    const LineFast r = fast[q];
+    const double A = r.c - x, D = (double)r.delta;
+    if (A > kRayEps + D) {
+      if (first_pos < 0) first_pos = r.cid;
+    } else if (!(A < kRayEps - D)) {
+      return false;
+    }
+    if (A < -kRayEps - D) {
+      last_neg = r.cid;
+    } else if (!(A > -kRayEps + D)) {
+      return false;
+    }
+  }
+  *pos_par = first_pos >= 0 ? ((ncl - first_pos) & 1) : 0;
+  *neg_par = last_neg >= 0 ? ((last_neg + 1) & 1) : 0;
+  return true;
+}
+
+__global__ __launch_bounds__(kTilePts) void dist_chunks_kernel(
     GridP g, TileDims td, const Tri64 *__restrict__ tri,
-    const uint64_t *__restrict__ tile_off,
-    const uint32_t *__restrict__ tile_tris,
-    const uint64_t *__restrict__ line_off, const LineHit *__restrict__ hits,
-    int64_t base1, int64_t base2, int do_dist, int do_sign, FusedOut out,
-    Pending *__restrict__ pend, uint32_t *npend, uint32_t cap, uint32_t *err) {
-  __shared__ float s_tri[9][kTilePts];
-  const int64_t tile = blockIdx.x;
+    const uint64_t *__restrict__ tile_off, const uint32_t *__restrict__ tile_tris,
+    const uint64_t *__restrict__ chunk_off, const uint32_t *__restrict__ chunk_tile,
+    int64_t nchunks, float *__restrict__ d2buf) {
+  __shared__ float s_pre[kTriPreFloats][kTilePts];
+  const int64_t item = xcd_remap(blockIdx.x, gridDim.x);
+  if (item >= nchunks) return;
   const int lane = threadIdx.x;
+  const int64_t tile = chunk_tile[item];
+  const uint64_t first = chunk_off[tile];
+  const int64_t q = item - (int64_t)first;
   const int32_t tx = (int32_t)(tile % td.tx);
   const int32_t ty = (int32_t)((tile / td.tx) % td.ty);
   const int32_t tz = (int32_t)(tile / ((int64_t)td.tx * td.ty));
   const int32_t i0 = tx * kTileX, j0 = ty * kTileY, k0 = tz * kTileZ;
+  const Vec3d T0 = lattice_point(g, i0, j0, k0);
+  const uint64_t b = tile_off[tile] + (uint64_t)q * kTilePts;
+  const uint64_t e = min(b + (uint64_t)kTilePts, tile_off[tile + 1]);
+  const int n = (int)(e - b);
+  if (lane < n) {
+    const Tri64 t = tri[tile_tris[b + lane]];
+    float v[9];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      v[3 * k + 0] = (float)(t.v[3 * k + 0] - T0.x);
+      v[3 * k + 1] = (float)(t.v[3 * k + 1] - T0.y);
+      v[3 * k + 2] = (float)(t.v[3 * k + 2] - T0.z);
+    }
+    float rec[kTriPreFloats];
+    tri_pre_make(v, v + 3, v + 6, rec);
+#pragma unroll
+    for (int f = 0; f < kTriPreFloats; f++) s_pre[f][lane] = rec[f];
+  }
+  __syncthreads();
+  // this lane's point, relative to the tile origin (exact small multiples of h)
   const int32_t i = i0 + (lane & (kTileX - 1));
   const int32_t j = j0 + ((lane / kTileX) & (kTileY - 1));
   const int32_t k = k0 + lane / (kTileX * kTileY);
-  const bool valid = i < g.nx && j < g.ny && k < g.nz;
   const Vec3d p = lattice_point(g, i, j, k);
+  const float px = (float)(p.x - T0.x), py = (float)(p.y - T0.y), pz = (float)(p.z - T0.z);
+  float best = __builtin_huge_valf();
+  for (int c = 0; c < n; c++) {
+    const float d2 = tri_pre_dist2(px, py, pz, [&](int f) { return s_pre[f][c]; });
+    best = fminf(best, d2);
+  }
+  float *slot = d2buf + first * kTilePts + lane;
+  if (chunk_off[tile + 1] - first == 1)
+    *slot = best;
+  else
+    atomicMin((unsigned int *)slot, __float_as_uint(best));
+}
+
+// Per tile (one wavefront): distance from the chunk results (or h for tiles
+// with no triangle within h), the six axis-ray votes from the line-hit
+// lists, the combine step, and the outputs.  Points whose six votes are not
+// unanimous go to the pending list for the adaptive vote.
+__global__ __launch_bounds__(kTilePts) void fused_lattice_kernel(
+    GridP g, TileDims td, const uint64_t *__restrict__ chunk_off,
+    const float *__restrict__ d2buf, const uint64_t *__restrict__ line_off,
+    const LineHit *__restrict__ hits, const LineFast *__restrict__ fast,
+    const uint32_t *__restrict__ meta, int64_t base1, int64_t base2, int do_sign,
+    FusedOut out, Pending *__restrict__ pend, uint32_t *npend, uint32_t cap,
+    uint32_t *err) {
+  const int64_t tile = xcd_remap(blockIdx.x, gridDim.x);
+  const int lane = threadIdx.x;
+  const int32_t tx = (int32_t)(tile % td.tx);
+  const int32_t ty = (int32_t)((tile / td.tx) % td.ty);
+  const int32_t tz = (int32_t)(tile / ((int64_t)td.tx * td.ty));
+  const int32_t i = tx * kTileX + (lane & (kTileX - 1));
+  const int32_t j = ty * kTileY + ((lane / kTileX) & (kTileY - 1));
+  const int32_t k = tz * kTileZ + lane / (kTileX * kTileY);
+  if (i >= g.nx || j >= g.ny || k >= g.nz) return;
+  const int64_t idx = ((int64_t)k * g.ny + j) * g.nx + i;
 
   double dist = g.h;
-  if (do_dist) {
-    const Vec3d T0 = lattice_point(g, i0, j0, k0);
-    const float px = (float)(p.x - T0.x), py = (float)(p.y - T0.y),
-                pz = (float)(p.z - T0.z);
-    float best = __builtin_huge_valf();
-    const uint64_t b = tile_off[tile], e = tile_off[tile + 1];
-    for (uint64_t c = b; c < e; c += kTilePts) {
-      const int n = (int)min((uint64_t)kTilePts, e - c);
-      if (lane < n) {
-        const Tri64 t = tri[tile_tris[c + lane]];
-#pragma unroll
-        for (int v = 0; v < 3; v++) {
-          s_tri[3 * v + 0][lane] = (float)(t.v[3 * v + 0] - T0.x);
-          s_tri[3 * v + 1][lane] = (float)(t.v[3 * v + 1] - T0.y);
-          s_tri[3 * v + 2][lane] = (float)(t.v[3 * v + 2] - T0.z);
-        }
+  if (d2buf) {
+    const uint64_t c0 = chunk_off[tile];
+    if (chunk_off[tile + 1] > c0) {
+      const float best = d2buf[c0 * kTilePts + lane];
+      if (best < __builtin_huge_valf()) {
+        const double d = sqrt((double)best);
+        dist = d < g.h ? d : g.h;
       }
-      __syncthreads();
-      for (int q = 0; q < n; q++) {
-        float d2 = tri_dist2<float>(px, py, pz, s_tri[0][q], s_tri[1][q],
-                                    s_tri[2][q], s_tri[3][q], s_tri[4][q],
-                                    s_tri[5][q], s_tri[6][q], s_tri[7][q],
-                                    s_tri[8][q]);
-        best = fminf(best, d2);
-      }
-      __syncthreads();
-    }
-    if (best < __builtin_huge_valf()) {
-      double d = sqrt((double)best);
-      dist = d < g.h ? d : g.h;
     }
   }
-  if (!valid) return;
-  const int64_t idx = ((int64_t)k * g.ny + j) * g.nx + i;
 
   double sgn = 1.0, conf = 1.0;
   bool pending = false;
   if (do_sign) {
+    const Vec3d p = lattice_point(g, i, j, k);
     const double o[3] = {p.x, p.y, p.z};
-    int inside = 0, a, bpar;
-    int64_t L = (int64_t)k * g.ny + j;
-    axis_pair_parity(hits, line_off[L], line_off[L + 1], o, &a, &bpar);
-    inside += a + bpar;
-    L = base1 + (int64_t)k * g.nx + i;
-    axis_pair_parity(hits, line_off[L], line_off[L + 1], o, &a, &bpar);
-    inside += a + bpar;
-    L = base2 + (int64_t)j * g.nx + i;
-    axis_pair_parity(hits, line_off[L], line_off[L + 1], o, &a, &bpar);
-    inside += a + bpar;
+    int inside = 0;
+    const int64_t Ls[3] = {(int64_t)k * g.ny + j, base1 + (int64_t)k * g.nx + i,
+                           base2 + (int64_t)j * g.nx + i};
+#pragma unroll
+    for (int ax = 0; ax < 3; ax++) {
+      const int64_t L = Ls[ax];
+      const uint64_t hb = line_off[L], he = line_off[L + 1];
+      int a, bpar;
+      if (!axis_pair_parity_fast(fast, hb, he, meta[L], o[ax], &a, &bpar))
+        axis_pair_parity(hits, hb, he, o, &a, &bpar);
+      inside += a + bpar;
+    }
     if (inside == 6) {
       sgn = 1.0;
     } else if (inside == 0) {
@@ -289,34 +360,132 @@ __device__ inline void item_point(const GridP &g, const double *pts, int64_t idx
   }
 }
 
-// One thread per (item, ray r in [r0, r1)): parity by LBVH traversal, set
-// as bit r of the item's mask.  Rays whose hit list or stack would overflow
-// go to a fallback list that the block-level brute-force kernel decides.
-__global__ void ray_bits_kernel(GridP g, const double *__restrict__ pts,
-                                const Tri64 *__restrict__ tri, Bvh bvh,
-                                RayParams rp, const Pending *__restrict__ items,
-                                uint32_t nitems, int r0, int r1,
-                                unsigned long long *__restrict__ bits,
-                                uint32_t *__restrict__ fb, uint32_t *nfb,
-                                uint32_t fb_cap, uint32_t *err) {
+// Wavefront-cooperative ray parity: one wavefront per (item, ray).  The 64
+// lanes pop up to 64 nodes at a time from an LDS stack, test both children
+// (FP64 slab test on boxes rounded outward), run the reference's FP64
+// Moller-Trumbore on leaf triangles and push internal children with a
+// ballot prefix sum.  Hits go to an LDS list; the unique count follows the
+// reference rule (sorted, consecutive gap > eps) computed as: a value counts
+// when it is the smallest or its strict predecessor lies more than eps
+// below, equal values counting once.  Stack or hit-list overflow sends the
+// ray to the brute-force fallback.
+constexpr int kWaveStack = 2048;
+constexpr int kWaveHits = 1024;
+
+__global__ __launch_bounds__(64) void ray_wave_kernel(
+    GridP g, const double *__restrict__ pts, const Tri64 *__restrict__ tri, Bvh bvh,
+    RayParams rp, const Pending *__restrict__ items, uint32_t nitems, int r0, int r1,
+    unsigned long long *__restrict__ bits, uint32_t *__restrict__ fb, uint32_t *nfb,
+    uint32_t fb_cap, uint32_t *err) {
+  __shared__ int32_t stack[kWaveStack];
+  __shared__ double hit_t[kWaveHits];
+  __shared__ int s_top, s_nh, s_ovf, s_cnt;
+  const int lane = threadIdx.x;
+  const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   const int nr = r1 - r0;
   const int64_t total = (int64_t)nitems * nr;
-  for (int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; w < total;
-       w += (int64_t)gridDim.x * blockDim.x) {
+  for (int64_t w = blockIdx.x; w < total; w += gridDim.x) {
     const uint32_t it = (uint32_t)(w / nr);
     const int r = r0 + (int)(w % nr);
     double o[3];
     item_point(g, pts, items[it].idx, o);
-    int par = ray_parity_bvh(bvh, tri, o, rp.dirs + 3 * r);
-    if (par < 0) {
-      uint32_t slot = atomicAdd(nfb, 1u);
-      if (slot < fb_cap)
-        fb[slot] = it * 64u + (uint32_t)r;
-      else
-        atomicOr(err, kErrHitOverflow);
-    } else if (par) {
-      atomicOr(&bits[it], 1ull << r);
+    const double *d = rp.dirs + 3 * r;
+    double inv[3];
+    for (int a = 0; a < 3; a++) inv[a] = d[a] != 0.0 ? 1.0 / d[a] : 0.0;
+    if (lane == 0) {
+      s_top = 0;
+      s_nh = 0;
+      s_ovf = 0;
+      s_cnt = 0;
+      if (bvh.n == 1) {
+        const uint32_t t = bvh.leaf_tri[0];
+        const Tri64 q = tri[t];
+        double tv;
+        if (mt_ref(o, d, q.v, q.v + 3, q.v + 6, &tv)) {
+          hit_t[0] = tv;
+          s_nh = 1;
+        }
+      } else if (bvh.n > 1) {
+        stack[0] = 0;
+        s_top = 1;
+      }
     }
+    __syncthreads();
+    while (true) {
+      const int top = s_top;
+      if (top == 0 || s_ovf) break;
+      const int take = top < 64 ? top : 64;
+      const int base = top - take;
+      const int32_t node = lane < take ? stack[base + lane] : -1;
+      __syncthreads();
+      int32_t push0 = -1, push1 = -1;
+      if (node >= 0) {
+        const BvhNode nd = bvh.nodes[node];
+        for (int c = 0; c < 2; c++) {
+          if (!ray_box(o, d, inv, nd.box[c])) continue;
+          const int32_t ch = nd.child[c];
+          if (ch < 0) {
+            const Tri64 q = tri[bvh.leaf_tri[-(ch + 1)]];
+            double tv;
+            if (mt_ref(o, d, q.v, q.v + 3, q.v + 6, &tv)) {
+              const int slot = atomicAdd(&s_nh, 1);
+              if (slot < kWaveHits)
+                hit_t[slot] = tv;
+              else
+                s_ovf = 1;
+            }
+          } else if (push0 < 0) {
+            push0 = ch;
+          } else {
+            push1 = ch;
+          }
+        }
+      }
+      const uint64_t b1 = __ballot(push0 >= 0), b2 = __ballot(push1 >= 0);
+      const int pos = base + __popcll(b1 & below) + __popcll(b2 & below);
+      const int ntop = base + __popcll(b1) + __popcll(b2);
+      if (ntop > kWaveStack) {
+        if (lane == 0) s_ovf = 1;
+      } else {
+        int q = pos;
+        if (push0 >= 0) stack[q++] = push0;
+        if (push1 >= 0) stack[q] = push1;
+      }
+      __syncthreads();
+      if (lane == 0) s_top = ntop > kWaveStack ? 0 : ntop;
+      __syncthreads();
+    }
+    const int n = s_nh;
+    const bool ovf = s_ovf != 0;
+    if (!ovf) {
+      for (int a = lane; a < n; a += 64) {
+        const double x = hit_t[a];
+        bool dup = false, have = false;
+        double pred = 0.0;
+        for (int b = 0; b < n; b++) {
+          const double y = hit_t[b];
+          if (y == x && b < a) dup = true;
+          if (y < x && (!have || y > pred)) {
+            pred = y;
+            have = true;
+          }
+        }
+        if (!dup && (!have || x - pred > kRayEps)) atomicAdd(&s_cnt, 1);
+      }
+    }
+    __syncthreads();
+    if (lane == 0) {
+      if (ovf) {
+        const uint32_t slot = atomicAdd(nfb, 1u);
+        if (slot < fb_cap)
+          fb[slot] = it * 64u + (uint32_t)r;
+        else
+          atomicOr(err, kErrHitOverflow);
+      } else if (s_cnt & 1) {
+        atomicOr(&bits[it], 1ull << r);
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -487,20 +656,31 @@ __global__ void init_pending_all_kernel(Pending *pend, int64_t n) {
 
 }  // namespace
 
-void launch_fused_lattice(hipStream_t s, const GridP &g, const Tri64 *tri,
-                          const Bins &b, bool do_dist, bool do_sign,
-                          FusedOut out, Pending *pend, uint32_t *npend,
-                          uint32_t pend_cap, uint32_t *err) {
+void launch_dist_chunks(hipStream_t s, const GridP &g, const Tri64 *tri,
+                        const Bins &b, float *d2buf) {
+  if (b.nchunks <= 0) return;
   TileDims td = tile_dims(g);
-  if (td.ntiles <= 0) return;
-  hipLaunchKernelGGL(fused_lattice_kernel, dim3((unsigned)td.ntiles),
-                     dim3(kTilePts), 0, s, g, td, tri, b.tile_off, b.tile_tris,
-                     b.line_off, b.line_hits, b.line_base[1], b.line_base[2],
-                     do_dist ? 1 : 0, do_sign ? 1 : 0, out, pend, npend,
-                     pend_cap, err);
+  hipLaunchKernelGGL(dist_chunks_kernel, dim3((unsigned)b.nchunks), dim3(kTilePts), 0,
+                     s, g, td, tri, b.tile_off, b.tile_tris, b.chunk_off, b.chunk_tile,
+                     b.nchunks, d2buf);
   SDF_HIP(hipGetLastError());
 }
 
+void launch_fused_lattice(hipStream_t s, const GridP &g, const Tri64 *tri,
+                          const Bins &b, const float *d2buf, bool do_sign,
+                          FusedOut out, Pending *pend, uint32_t *npend,
+                          uint32_t pend_cap, uint32_t *err) {
+  (void)tri;
+  TileDims td = tile_dims(g);
+  if (td.ntiles <= 0) return;
+  hipLaunchKernelGGL(fused_lattice_kernel, dim3((unsigned)td.ntiles),
+                     dim3(kTilePts), 0, s, g, td, (const uint64_t *)b.chunk_off,
+                     d2buf, (const uint64_t *)b.line_off, (const LineHit *)b.line_hits,
+                     (const LineFast *)b.line_fast, (const uint32_t *)b.line_meta,
+                     b.line_base[1], b.line_base[2], do_sign ? 1 : 0, out, pend, npend,
+                     pend_cap, err);
+  SDF_HIP(hipGetLastError());
+}
 
 static int blocks_for(int64_t n, int t) {
   int64_t b = (n + t - 1) / t;
@@ -530,10 +710,10 @@ void resolve_pending(hipStream_t s, Arena &ar, const GridP &g, const double *pts
   uint32_t fb_cap = nitems * 4u + 1024u;
   uint32_t *fb = ar.get<uint32_t>("res_fb", fb_cap);
   if (r1 > r0) {
-    int64_t work = (int64_t)nitems * (r1 - r0);
-    hipLaunchKernelGGL(ray_bits_kernel, dim3(blocks_for(work, 64)), dim3(64), 0,
-                       s, g, pts, tri, bvh, rp, items, nitems, r0, r1, bits, fb,
-                       d_counts, fb_cap, err);
+    const int64_t work = (int64_t)nitems * (r1 - r0);
+    const unsigned nblk = (unsigned)(work < (1 << 20) ? work : (1 << 20));
+    hipLaunchKernelGGL(ray_wave_kernel, dim3(nblk), dim3(64), 0, s, g, pts, tri, bvh,
+                       rp, items, nitems, r0, r1, bits, fb, d_counts, fb_cap, err);
     SDF_HIP(hipGetLastError());
     uint32_t nfb = 0;
     SDF_HIP(hipMemcpyAsync(&nfb, d_counts, 4, hipMemcpyDeviceToHost, s));
