@@ -8,6 +8,7 @@
 // its points.  Axis-line hit lists replace the per-point BVH ray traversal
 // (RaycastingSignDetection.jl:114-136) for the six axis rays.
 #include "sdf_internal.h"
+#include "sdf_lbvh.h"
 
 namespace sdf {
 
@@ -125,19 +126,6 @@ void scan_rec(hipStream_t s, Arena &ar, const TIn *in, int64_t n_in,
   }
 }
 
-__device__ inline int64_t upper_bound_u64(const uint64_t *a, int64_t n,
-                                          uint64_t x) {
-  int64_t lo = 0, hi = n;
-  while (lo < hi) {
-    int64_t mid = (lo + hi) >> 1;
-    if (a[mid] <= x)
-      lo = mid + 1;
-    else
-      hi = mid;
-  }
-  return lo;
-}
-
 __device__ inline void tri_minmax(const Tri64 &t, double *mn, double *mx) {
   for (int d = 0; d < 3; d++) {
     double a = t.v[d], b = t.v[3 + d], c = t.v[6 + d];
@@ -223,43 +211,121 @@ __device__ inline bool tile_plane_test(const Tri64 &t, const GridP &g,
   return fabs(dist) <= proj * (1.0 + 1e-9) + 1e-12 * mag;
 }
 
-__global__ void tile_count_tri_kernel(const Tri64 *__restrict__ tri,
-                                      int64_t nel, GridP g, double R,
-                                      uint32_t *__restrict__ cnt) {
+constexpr int64_t kSmallMax = 256;  // candidates looped by one thread
+
+// Atomic-free tile binning: per triangle, count the tiles that pass the
+// plane test, scan, emit (tile, triangle) pairs at the triangle's offset,
+// then sort the pairs by tile (stable LDS radix sort).  Tile lists come out
+// in triangle order, independent of scheduling.  Triangles with more than
+// kSmallMax candidate tiles are counted and emitted by one workgroup each.
+__global__ void tile_count_small_kernel(const Tri64 *__restrict__ tri, int64_t nel,
+                                        GridP g, TileDims td, double R,
+                                        uint32_t *__restrict__ cnt,
+                                        uint32_t *__restrict__ big, uint32_t *nbig) {
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nel;
        t += (int64_t)gridDim.x * blockDim.x) {
-    TileRange r = tri_tile_range(tri[t], g, R);
-    cnt[t] = (uint32_t)(r.count > 0xFFFFFFFFll ? 0xFFFFFFFFll : r.count);
+    const Tri64 tr = tri[t];
+    const TileRange r = tri_tile_range(tr, g, R);
+    uint32_t c = 0;
+    if (r.count > kSmallMax) {
+      big[atomicAdd(nbig, 1u)] = (uint32_t)t;
+    } else if (r.count > 0) {
+      for (int32_t tz = r.lo[2]; tz <= r.hi[2]; tz++)
+        for (int32_t ty = r.lo[1]; ty <= r.hi[1]; ty++)
+          for (int32_t tx = r.lo[0]; tx <= r.hi[0]; tx++)
+            c += tile_plane_test(tr, g, td, tx, ty, tz, R) ? 1u : 0u;
+    }
+    cnt[t] = c;
   }
 }
 
-// mode 0: count per tile; mode 1: fill tile lists.
-__global__ void tile_pairs_kernel(const Tri64 *__restrict__ tri, int64_t nel,
-                                  GridP g, TileDims td, double R,
-                                  const uint64_t *__restrict__ tri_off,
-                                  uint64_t total, int mode,
-                                  uint32_t *__restrict__ tile_cnt,
-                                  const uint64_t *__restrict__ tile_off,
-                                  uint32_t *__restrict__ cursor,
-                                  uint32_t *__restrict__ tile_tris) {
-  for (uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; p < total;
-       p += (uint64_t)gridDim.x * blockDim.x) {
-    int64_t t = upper_bound_u64(tri_off, nel + 1, p) - 1;
-    uint64_t local = p - tri_off[t];
-    Tri64 tr = tri[t];
-    TileRange r = tri_tile_range(tr, g, R);
-    int64_t wx = r.hi[0] - r.lo[0] + 1, wy = r.hi[1] - r.lo[1] + 1;
-    int32_t tx = r.lo[0] + (int32_t)(local % wx);
-    int32_t ty = r.lo[1] + (int32_t)((local / wx) % wy);
-    int32_t tz = r.lo[2] + (int32_t)(local / (wx * wy));
-    if (!tile_plane_test(tr, g, td, tx, ty, tz, R)) continue;
-    int64_t tile = ((int64_t)tz * td.ty + ty) * td.tx + tx;
-    if (mode == 0) {
-      atomicAdd(&tile_cnt[tile], 1u);
-    } else {
-      uint32_t slot = atomicAdd(&cursor[tile], 1u);
-      tile_tris[tile_off[tile] + slot] = (uint32_t)t;
+__global__ void tile_emit_small_kernel(const Tri64 *__restrict__ tri, int64_t nel,
+                                       GridP g, TileDims td, double R,
+                                       const uint64_t *__restrict__ toff,
+                                       uint32_t *__restrict__ keys,
+                                       uint32_t *__restrict__ vals) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nel;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const Tri64 tr = tri[t];
+    const TileRange r = tri_tile_range(tr, g, R);
+    if (r.count == 0 || r.count > kSmallMax) continue;
+    uint64_t pos = toff[t];
+    for (int32_t tz = r.lo[2]; tz <= r.hi[2]; tz++)
+      for (int32_t ty = r.lo[1]; ty <= r.hi[1]; ty++)
+        for (int32_t tx = r.lo[0]; tx <= r.hi[0]; tx++)
+          if (tile_plane_test(tr, g, td, tx, ty, tz, R)) {
+            keys[pos] = (uint32_t)(((int64_t)tz * td.ty + ty) * td.tx + tx);
+            vals[pos] = (uint32_t)t;
+            pos++;
+          }
+  }
+}
+
+// One workgroup (256 threads) per large triangle; mode 0 counts, mode 1
+// emits in candidate order (block prefix sum over the pass flags).
+__global__ __launch_bounds__(256) void tile_big_kernel(
+    const Tri64 *__restrict__ tri, GridP g, TileDims td, double R, int mode,
+    const uint32_t *__restrict__ big, const uint32_t *nbig,
+    uint32_t *__restrict__ cnt, const uint64_t *__restrict__ toff,
+    uint32_t *__restrict__ keys, uint32_t *__restrict__ vals) {
+  __shared__ uint32_t s_scan[256];
+  __shared__ uint32_t s_base;
+  const uint32_t nb = *nbig;
+  for (uint32_t q = blockIdx.x; q < nb; q += gridDim.x) {
+    const uint32_t t = big[q];
+    const Tri64 tr = tri[t];
+    const TileRange r = tri_tile_range(tr, g, R);
+    const int64_t wx = r.hi[0] - r.lo[0] + 1, wy = r.hi[1] - r.lo[1] + 1;
+    if (threadIdx.x == 0) s_base = 0;
+    __syncthreads();
+    for (int64_t c0 = 0; c0 < r.count; c0 += blockDim.x) {
+      const int64_t c = c0 + threadIdx.x;
+      int32_t tx = 0, ty = 0, tz = 0;
+      uint32_t pass = 0;
+      if (c < r.count) {
+        tx = r.lo[0] + (int32_t)(c % wx);
+        ty = r.lo[1] + (int32_t)((c / wx) % wy);
+        tz = r.lo[2] + (int32_t)(c / (wx * wy));
+        pass = tile_plane_test(tr, g, td, tx, ty, tz, R) ? 1u : 0u;
+      }
+      s_scan[threadIdx.x] = pass;
+      __syncthreads();
+      for (int off = 1; off < 256; off <<= 1) {
+        uint32_t v = threadIdx.x >= (unsigned)off ? s_scan[threadIdx.x - off] : 0u;
+        __syncthreads();
+        s_scan[threadIdx.x] += v;
+        __syncthreads();
+      }
+      const uint32_t incl = s_scan[threadIdx.x];
+      if (mode == 1 && pass) {
+        const uint64_t pos = toff[t] + s_base + incl - 1;
+        keys[pos] = (uint32_t)(((int64_t)tz * td.ty + ty) * td.tx + tx);
+        vals[pos] = t;
+      }
+      __syncthreads();
+      if (threadIdx.x == 255) s_base += incl;
+      __syncthreads();
     }
+    if (mode == 0 && threadIdx.x == 0) cnt[t] = s_base;
+    __syncthreads();
+  }
+}
+
+// CSR offsets from the sorted tile keys: off[t] = first index whose key is
+// >= t (binary search per tile; no serial runs over empty stretches).
+__global__ void tile_offsets_kernel(const uint32_t *__restrict__ keys, int64_t n,
+                                    int64_t ntiles, uint64_t *__restrict__ off) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t <= ntiles;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      if ((int64_t)keys[mid] < t)
+        lo = mid + 1;
+      else
+        hi = mid;
+    }
+    off[t] = (uint64_t)lo;
   }
 }
 
@@ -295,66 +361,88 @@ __device__ inline LineRange tri_line_range(const Tri64 &t, const GridP &g) {
   return r;
 }
 
-__global__ void line_count_tri_kernel(const Tri64 *__restrict__ tri,
-                                      int64_t nel, GridP g,
-                                      uint32_t *__restrict__ cnt) {
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nel;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    LineRange r = tri_line_range(tri[t], g);
-    int64_t c = r.cnt[0] + r.cnt[1] + r.cnt[2];
-    cnt[t] = (uint32_t)(c > 0xFFFFFFFFll ? 0xFFFFFFFFll : c);
+__device__ inline void line_visit(const Tri64 &tr, const GridP &g, int ax,
+                                  int32_t a, int32_t b, int64_t base1,
+                                  int64_t base2, int mode,
+                                  uint32_t *__restrict__ cnt,
+                                  const uint64_t *__restrict__ off,
+                                  LineHit *__restrict__ hits) {
+  // origin: the lattice point with index 0 along the axis
+  double o[3];
+  int64_t line;
+  if (ax == 0) {  // lines over (j, k)
+    o[0] = lattice_coord(g.ox, g.h, 0);
+    o[1] = lattice_coord(g.oy, g.h, a);
+    o[2] = lattice_coord(g.oz, g.h, b);
+    line = (int64_t)b * g.ny + a;
+  } else if (ax == 1) {  // lines over (i, k)
+    o[0] = lattice_coord(g.ox, g.h, a);
+    o[1] = lattice_coord(g.oy, g.h, 0);
+    o[2] = lattice_coord(g.oz, g.h, b);
+    line = base1 + (int64_t)b * g.nx + a;
+  } else {  // lines over (i, j)
+    o[0] = lattice_coord(g.ox, g.h, a);
+    o[1] = lattice_coord(g.oy, g.h, b);
+    o[2] = lattice_coord(g.oz, g.h, 0);
+    line = base2 + (int64_t)b * g.nx + a;
+  }
+  LineHit rec;
+  if (!line_hit_test(o, ax, tr.v, tr.v + 3, tr.v + 6, mode == 0 ? nullptr : &rec))
+    return;
+  if (mode == 0) {
+    atomicAdd(&cnt[line], 1u);
+  } else {
+    const uint32_t slot = atomicAdd(&cnt[line], 1u);  // cnt is the cursor here
+    hits[off[line] + slot] = rec;
   }
 }
 
-__global__ void line_pairs_kernel(const Tri64 *__restrict__ tri, int64_t nel,
-                                  GridP g, const uint64_t *__restrict__ tri_off,
-                                  uint64_t total, int64_t base1, int64_t base2,
-                                  int mode, uint32_t *__restrict__ line_cnt,
-                                  const uint64_t *__restrict__ line_off,
-                                  uint32_t *__restrict__ cursor,
-                                  LineHit *__restrict__ hits) {
-  for (uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; p < total;
-       p += (uint64_t)gridDim.x * blockDim.x) {
-    int64_t t = upper_bound_u64(tri_off, nel + 1, p) - 1;
-    uint64_t local = p - tri_off[t];
-    Tri64 tr = tri[t];
-    LineRange r = tri_line_range(tr, g);
-    int ax = 0;
-    while (ax < 2 && local >= (uint64_t)r.cnt[ax]) {
-      local -= r.cnt[ax];
-      ax++;
-    }
-    int64_t w0 = r.hi[ax][0] - r.lo[ax][0] + 1;
-    int32_t a = r.lo[ax][0] + (int32_t)(local % w0);
-    int32_t b = r.lo[ax][1] + (int32_t)(local / w0);
-    // origin: the lattice point with index 0 along the axis
-    double o[3];
-    int64_t line;
-    if (ax == 0) {  // lines over (j, k)
-      o[0] = lattice_coord(g.ox, g.h, 0);
-      o[1] = lattice_coord(g.oy, g.h, a);
-      o[2] = lattice_coord(g.oz, g.h, b);
-      line = (int64_t)b * g.ny + a;
-    } else if (ax == 1) {  // lines over (i, k)
-      o[0] = lattice_coord(g.ox, g.h, a);
-      o[1] = lattice_coord(g.oy, g.h, 0);
-      o[2] = lattice_coord(g.oz, g.h, b);
-      line = base1 + (int64_t)b * g.nx + a;
-    } else {  // lines over (i, j)
-      o[0] = lattice_coord(g.ox, g.h, a);
-      o[1] = lattice_coord(g.oy, g.h, b);
-      o[2] = lattice_coord(g.oz, g.h, 0);
-      line = base2 + (int64_t)b * g.nx + a;
-    }
-    LineHit rec;
-    if (!line_hit_test(o, ax, tr.v, tr.v + 3, tr.v + 6,
-                       mode == 0 ? nullptr : &rec))
+__global__ void line_bin_small_kernel(const Tri64 *__restrict__ tri, int64_t nel,
+                                      GridP g, int64_t base1, int64_t base2,
+                                      int mode, uint32_t *__restrict__ cnt,
+                                      const uint64_t *__restrict__ off,
+                                      LineHit *__restrict__ hits,
+                                      uint32_t *__restrict__ big, uint32_t *nbig) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nel;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const Tri64 tr = tri[t];
+    const LineRange r = tri_line_range(tr, g);
+    const int64_t total = r.cnt[0] + r.cnt[1] + r.cnt[2];
+    if (total == 0) continue;
+    if (total > kSmallMax) {
+      if (mode == 0) big[atomicAdd(nbig, 1u)] = (uint32_t)t;
       continue;
-    if (mode == 0) {
-      atomicAdd(&line_cnt[line], 1u);
-    } else {
-      uint32_t slot = atomicAdd(&cursor[line], 1u);
-      hits[line_off[line] + slot] = rec;
+    }
+    for (int ax = 0; ax < 3; ax++)
+      for (int32_t b = r.lo[ax][1]; b <= r.hi[ax][1]; b++)
+        for (int32_t a = r.lo[ax][0]; a <= r.hi[ax][0]; a++)
+          line_visit(tr, g, ax, a, b, base1, base2, mode, cnt, off, hits);
+  }
+}
+
+__global__ void line_bin_big_kernel(const Tri64 *__restrict__ tri, GridP g,
+                                    int64_t base1, int64_t base2, int mode,
+                                    uint32_t *__restrict__ cnt,
+                                    const uint64_t *__restrict__ off,
+                                    LineHit *__restrict__ hits,
+                                    const uint32_t *__restrict__ big,
+                                    const uint32_t *nbig) {
+  const uint32_t nb = *nbig;
+  for (uint32_t q = blockIdx.x; q < nb; q += gridDim.x) {
+    const Tri64 tr = tri[big[q]];
+    const LineRange r = tri_line_range(tr, g);
+    const int64_t total = r.cnt[0] + r.cnt[1] + r.cnt[2];
+    for (int64_t c = threadIdx.x; c < total; c += blockDim.x) {
+      int64_t local = c;
+      int ax = 0;
+      while (ax < 2 && local >= r.cnt[ax]) {
+        local -= r.cnt[ax];
+        ax++;
+      }
+      const int64_t w0 = r.hi[ax][0] - r.lo[ax][0] + 1;
+      line_visit(tr, g, ax, r.lo[ax][0] + (int32_t)(local % w0),
+                 r.lo[ax][1] + (int32_t)(local / w0), base1, base2, mode, cnt, off,
+                 hits);
     }
   }
 }
@@ -397,19 +485,31 @@ __global__ void line_prep_kernel(GridP g, const uint64_t *__restrict__ line_off,
                                  const LineHit *__restrict__ hits, int64_t base1,
                                  int64_t base2, int64_t nlines,
                                  LineFast *__restrict__ fast,
-                                 uint32_t *__restrict__ meta) {
+                                 uint32_t *__restrict__ meta,
+                                 LineInline *__restrict__ inl) {
 #pragma clang fp contract(off)
   const double u = 1.1102230246251565e-16;
   for (int64_t L = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; L < nlines;
        L += (int64_t)gridDim.x * blockDim.x) {
     const uint64_t hb = line_off[L], he = line_off[L + 1];
     const int64_t n = (int64_t)(he - hb);
+    LineInline li;
+    for (int q = 0; q < kInlineHits; q++) {
+      li.c[q] = 0.0;
+      li.delta[q] = 0.f;
+      li.cid[q] = 0;
+    }
+    li.pad_[0] = li.pad_[1] = 0u;
     if (n == 0) {
       meta[L] = 0u;
+      li.info = 0u;
+      inl[L] = li;
       continue;
     }
     if (n > kLineFastMax) {
       meta[L] = kLineExact;
+      li.info = kLineExact | 255u;
+      inl[L] = li;
       continue;
     }
     int ax;
@@ -496,6 +596,16 @@ __global__ void line_prep_kernel(GridP g, const uint64_t *__restrict__ line_off,
       prev = c;
     }
     meta[L] = ok ? (cl + 1u) : kLineExact;
+    li.info = (uint32_t)(n < 255 ? n : 255) | ((cl + 1u) & 0xFFu) << 8 | (ok ? 0u : kLineExact);
+    if (n <= kInlineHits && ok) {
+      for (int64_t q = 0; q < n; q++) {
+        const LineFast f = fast[hb + q];
+        li.c[q] = f.c;
+        li.delta[q] = f.delta;
+        li.cid[q] = (uint8_t)f.cid;
+      }
+    }
+    inl[L] = li;
   }
 }
 
@@ -544,37 +654,41 @@ void scan_counts(hipStream_t s, Arena &ar, const uint32_t *in, uint64_t *out,
 void build_tile_bins(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
                      const GridP &g, double radius, Bins &b) {
   TileDims td = tile_dims(g);
-  double R = radius * (1.0 + 1e-6);
+  const double R = radius * (1.0 + 1e-6);
   uint32_t *tcnt = ar.get<uint32_t>("bin_tri_cnt", nel + 1);
   uint64_t *toff = ar.get<uint64_t>("bin_tri_off", nel + 2);
-  uint32_t *tile_cnt = ar.get<uint32_t>("bin_tile_cnt", td.ntiles + 1);
+  uint32_t *big = ar.get<uint32_t>("bin_big", nel + 1);
+  uint32_t *nbig = ar.get<uint32_t>("bin_nbig", 1);
   uint64_t *tile_off = ar.get<uint64_t>("bin_tile_off", td.ntiles + 2);
-  launch_fill_u32(s, tile_cnt, td.ntiles + 1, 0u);
-  uint64_t total = 0;
+  launch_fill_u32(s, nbig, 1, 0u);
+  uint64_t npairs = 0;
   if (nel > 0) {
-    hipLaunchKernelGGL(tile_count_tri_kernel, dim3(grid_for(nel, 256)),
-                       dim3(256), 0, s, tri, nel, g, R, tcnt);
+    hipLaunchKernelGGL(tile_count_small_kernel, dim3(grid_for(nel, 256)), dim3(256), 0, s,
+                       tri, nel, g, td, R, tcnt, big, nbig);
+    hipLaunchKernelGGL(tile_big_kernel, dim3(1024), dim3(256), 0, s, tri, g, td, R, 0,
+                       (const uint32_t *)big, (const uint32_t *)nbig, tcnt,
+                       (const uint64_t *)nullptr, (uint32_t *)nullptr, (uint32_t *)nullptr);
     SDF_HIP(hipGetLastError());
     scan_counts(s, ar, tcnt, toff, nel);
-    total = read_u64(s, toff + nel);
+    npairs = read_u64(s, toff + nel);
   }
-  if (total > 0) {
-    hipLaunchKernelGGL(tile_pairs_kernel, dim3(grid_for((int64_t)total, 256)),
-                       dim3(256), 0, s, tri, nel, g, td, R, toff, total, 0,
-                       tile_cnt, (const uint64_t *)nullptr, (uint32_t *)nullptr,
-                       (uint32_t *)nullptr);
-    SDF_HIP(hipGetLastError());
-  }
-  scan_counts(s, ar, tile_cnt, tile_off, td.ntiles);
-  uint64_t npairs = read_u64(s, tile_off + td.ntiles);
+  uint32_t *keys = ar.get<uint32_t>("bin_tile_keys", npairs + 1);
   uint32_t *tile_tris = ar.get<uint32_t>("bin_tile_tris", npairs + 1);
   if (npairs > 0) {
-    uint32_t *cursor = tile_cnt;  // reuse as cursor
-    launch_fill_u32(s, cursor, td.ntiles + 1, 0u);
-    hipLaunchKernelGGL(tile_pairs_kernel, dim3(grid_for((int64_t)total, 256)),
-                       dim3(256), 0, s, tri, nel, g, td, R, toff, total, 1,
-                       (uint32_t *)nullptr, tile_off, cursor, tile_tris);
+    hipLaunchKernelGGL(tile_emit_small_kernel, dim3(grid_for(nel, 256)), dim3(256), 0, s,
+                       tri, nel, g, td, R, (const uint64_t *)toff, keys, tile_tris);
+    hipLaunchKernelGGL(tile_big_kernel, dim3(1024), dim3(256), 0, s, tri, g, td, R, 1,
+                       (const uint32_t *)big, (const uint32_t *)nbig, tcnt,
+                       (const uint64_t *)toff, keys, tile_tris);
     SDF_HIP(hipGetLastError());
+    int bits = 1;
+    while (bits < 32 && ((int64_t)1 << bits) <= td.ntiles) bits++;
+    radix_sort_u32_pairs(s, ar, keys, tile_tris, (int64_t)npairs, bits);
+    hipLaunchKernelGGL(tile_offsets_kernel, dim3(grid_for(td.ntiles + 1, 256)), dim3(256),
+                       0, s, (const uint32_t *)keys, (int64_t)npairs, td.ntiles, tile_off);
+    SDF_HIP(hipGetLastError());
+  } else {
+    launch_fill_u64(s, tile_off, td.ntiles + 1, 0ull);
   }
   b.tile_off = tile_off;
   b.tile_tris = tile_tris;
@@ -610,46 +724,46 @@ void build_line_hits(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
   b.line_base[1] = nlx;
   b.line_base[2] = nlx + nly;
   b.line_base[3] = nlines;
-  uint32_t *tcnt = ar.get<uint32_t>("line_tri_cnt", nel + 1);
-  uint64_t *toff = ar.get<uint64_t>("line_tri_off", nel + 2);
   uint32_t *lcnt = ar.get<uint32_t>("line_cnt", nlines + 1);
   uint64_t *loff = ar.get<uint64_t>("line_off", nlines + 2);
+  uint32_t *big = ar.get<uint32_t>("line_big", nel + 1);
+  uint32_t *nbig = ar.get<uint32_t>("line_nbig", 1);
   launch_fill_u32(s, lcnt, nlines + 1, 0u);
-  uint64_t total = 0;
+  launch_fill_u32(s, nbig, 1, 0u);
   if (nel > 0) {
-    hipLaunchKernelGGL(line_count_tri_kernel, dim3(grid_for(nel, 256)),
-                       dim3(256), 0, s, tri, nel, g, tcnt);
-    SDF_HIP(hipGetLastError());
-    scan_counts(s, ar, tcnt, toff, nel);
-    total = read_u64(s, toff + nel);
-  }
-  if (total > 0) {
-    hipLaunchKernelGGL(line_pairs_kernel, dim3(grid_for((int64_t)total, 256)),
-                       dim3(256), 0, s, tri, nel, g, toff, total, nlx,
-                       nlx + nly, 0, lcnt, (const uint64_t *)nullptr,
-                       (uint32_t *)nullptr, (LineHit *)nullptr);
+    hipLaunchKernelGGL(line_bin_small_kernel, dim3(grid_for(nel, 256)), dim3(256), 0, s,
+                       tri, nel, g, nlx, nlx + nly, 0, lcnt, (const uint64_t *)nullptr,
+                       (LineHit *)nullptr, big, nbig);
+    hipLaunchKernelGGL(line_bin_big_kernel, dim3(1024), dim3(256), 0, s, tri, g, nlx,
+                       nlx + nly, 0, lcnt, (const uint64_t *)nullptr, (LineHit *)nullptr,
+                       (const uint32_t *)big, (const uint32_t *)nbig);
     SDF_HIP(hipGetLastError());
   }
   scan_counts(s, ar, lcnt, loff, nlines);
-  uint64_t nhits = read_u64(s, loff + nlines);
+  const uint64_t nhits = read_u64(s, loff + nlines);
   LineHit *hits = ar.get<LineHit>("line_hits", nhits + 1);
   if (nhits > 0) {
-    launch_fill_u32(s, lcnt, nlines + 1, 0u);
-    hipLaunchKernelGGL(line_pairs_kernel, dim3(grid_for((int64_t)total, 256)),
-                       dim3(256), 0, s, tri, nel, g, toff, total, nlx,
-                       nlx + nly, 1, (uint32_t *)nullptr, loff, lcnt, hits);
+    launch_fill_u32(s, lcnt, nlines + 1, 0u);  // reused as the cursor
+    hipLaunchKernelGGL(line_bin_small_kernel, dim3(grid_for(nel, 256)), dim3(256), 0, s,
+                       tri, nel, g, nlx, nlx + nly, 1, lcnt, (const uint64_t *)loff, hits,
+                       big, nbig);
+    hipLaunchKernelGGL(line_bin_big_kernel, dim3(1024), dim3(256), 0, s, tri, g, nlx,
+                       nlx + nly, 1, lcnt, (const uint64_t *)loff, hits,
+                       (const uint32_t *)big, (const uint32_t *)nbig);
     SDF_HIP(hipGetLastError());
   }
   LineFast *fast = ar.get<LineFast>("line_fast", nhits + 1);
   uint32_t *meta = ar.get<uint32_t>("line_meta", nlines + 1);
+  LineInline *inl = ar.get<LineInline>("line_inline", nlines + 1);
   hipLaunchKernelGGL(line_prep_kernel, dim3(grid_for(nlines, 256)), dim3(256), 0, s, g,
                      (const uint64_t *)loff, (const LineHit *)hits, nlx, nlx + nly,
-                     nlines, fast, meta);
+                     nlines, fast, meta, inl);
   SDF_HIP(hipGetLastError());
   b.line_off = loff;
   b.line_hits = hits;
   b.line_fast = fast;
   b.line_meta = meta;
+  b.line_inline = inl;
   b.nline_hits = (int64_t)nhits;
 }
 

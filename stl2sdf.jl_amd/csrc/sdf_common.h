@@ -173,6 +173,18 @@ struct LineFast {
 constexpr uint32_t kLineExact = 0x80000000u;  // line meta flag: no fast path
 constexpr int kLineFastMax = 64;             // lines with more hits: exact path
 
+// Inline copy of a short line's summary (up to 4 hits) indexed directly by
+// line id: one 64-byte load instead of offsets + records.  info: bits 0-7
+// hit count (255 = more than 254), 8-15 cluster count, bit 31 kLineExact.
+constexpr int kInlineHits = 4;
+struct __attribute__((aligned(16))) LineInline {
+  double c[kInlineHits];
+  float delta[kInlineHits];
+  uint8_t cid[kInlineHits];
+  uint32_t info;
+  uint32_t pad_[2];
+};
+
 // t for origin o on the +axis ray of a recorded line hit (reference order).
 __host__ __device__ inline double line_hit_t(const LineHit &r, const double *o) {
 #pragma clang fp contract(off)
@@ -276,6 +288,38 @@ __device__ inline int64_t xcd_remap(int64_t b, int64_t B) {
   const int64_t per = B / kNumXcd, rem = B % kNumXcd;
   const int64_t x = b % kNumXcd, local = b / kNumXcd;
   return x * per + (x < rem ? x : rem) + local;
+}
+
+// Cache-blocked tile order: bricks of 16x16x16 tiles (bricks x-fastest),
+// Morton order inside a brick.  Neighbouring launch indices then touch
+// neighbouring tiles in all three directions, so the axis lines a brick
+// needs (about 1 MB of hit data) stay in one XCD's L2.  Returns false for
+// the padding positions of partial bricks.
+constexpr int kBrickBits = 4;  // 16 tiles per brick edge
+__host__ __device__ inline int64_t brick_launch_size(const TileDims &td) {
+  const int64_t e = 1 << kBrickBits;
+  return ((td.tx + e - 1) / e) * ((td.ty + e - 1) / e) * ((td.tz + e - 1) / e) *
+         (e * e * e);
+}
+__device__ inline bool brick_tile(int64_t m, const TileDims &td, int32_t *tx,
+                                  int32_t *ty, int32_t *tz) {
+  const int32_t e = 1 << kBrickBits;
+  const int64_t brick = m >> (3 * kBrickBits);
+  const uint32_t w = (uint32_t)(m & ((1 << (3 * kBrickBits)) - 1));
+  const int32_t nbx = (td.tx + e - 1) / e, nby = (td.ty + e - 1) / e;
+  const int32_t bx = (int32_t)(brick % nbx), by = (int32_t)((brick / nbx) % nby),
+                bz = (int32_t)(brick / ((int64_t)nbx * nby));
+  int32_t lx = 0, ly = 0, lz = 0;
+#pragma unroll
+  for (int b = 0; b < kBrickBits; b++) {
+    lx |= ((w >> (3 * b + 0)) & 1) << b;
+    ly |= ((w >> (3 * b + 1)) & 1) << b;
+    lz |= ((w >> (3 * b + 2)) & 1) << b;
+  }
+  *tx = bx * e + lx;
+  *ty = by * e + ly;
+  *tz = bz * e + lz;
+  return *tx < td.tx && *ty < td.ty && *tz < td.tz;
 }
 
 // Per-triangle data for the branch-free point-triangle distance, relative

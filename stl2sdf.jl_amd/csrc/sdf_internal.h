@@ -108,6 +108,7 @@ struct Bins {
   int64_t line_base[4] = {0, 0, 0, 0};
   LineFast *line_fast = nullptr;   // per hit, sorted by c within each line
   uint32_t *line_meta = nullptr;   // per line: cluster count | kLineExact
+  LineInline *line_inline = nullptr;  // per line: inline summary (<= 4 hits)
   int64_t ntile_pairs = 0, nline_hits = 0;
   // distance work items: each non-empty tile's list in chunks of 64
   uint64_t *chunk_off = nullptr;  // ntiles + 1 (first work item of a tile)

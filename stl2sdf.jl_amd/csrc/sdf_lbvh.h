@@ -22,8 +22,9 @@ struct Bvh {
 };
 
 Bvh build_lbvh(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t n);
+// Stable LSD sort of (key, value) pairs by the low key_bits bits.
 void radix_sort_u32_pairs(hipStream_t s, Arena &ar, uint32_t *keys,
-                          uint32_t *vals, int64_t n);
+                          uint32_t *vals, int64_t n, int key_bits);
 
 constexpr int kStackDepth = 64;
 

@@ -299,7 +299,7 @@ __global__ void node_boxes_kernel(const float *__restrict__ seg, SegLevels L,
 }
 
 void radix_sort_pairs(hipStream_t s, Arena &ar, uint32_t *keys, uint32_t *vals,
-                      int64_t n) {
+                      int64_t n, int key_bits) {
   if (n <= 1) return;
   int64_t nblocks = (n + kSortTile - 1) / kSortTile;
   uint32_t *k2 = ar.get<uint32_t>("sort_k2", n);
@@ -307,7 +307,8 @@ void radix_sort_pairs(hipStream_t s, Arena &ar, uint32_t *keys, uint32_t *vals,
   uint32_t *hist = ar.get<uint32_t>("sort_hist", kRadix * nblocks + 1);
   uint64_t *off = ar.get<uint64_t>("sort_off", kRadix * nblocks + 2);
   uint32_t *ka = keys, *va = vals, *kb = k2, *vb = v2;
-  for (int shift = 0; shift < 32; shift += kRadixBits) {
+  int passes = 0;
+  for (int shift = 0; shift < key_bits; shift += kRadixBits, passes++) {
     hipLaunchKernelGGL(radix_hist_kernel, dim3((unsigned)nblocks),
                        dim3(kSortThreads), 0, s, ka, n, shift, nblocks, hist);
     SDF_HIP(hipGetLastError());
@@ -323,14 +324,19 @@ void radix_sort_pairs(hipStream_t s, Arena &ar, uint32_t *keys, uint32_t *vals,
     va = vb;
     vb = t;
   }
-  // 4 passes: the result is back in keys/vals
+  if (passes & 1) {  // odd pass count: the result sits in the scratch pair
+    SDF_HIP(hipMemcpyAsync(keys, ka, (size_t)n * sizeof(uint32_t),
+                           hipMemcpyDeviceToDevice, s));
+    SDF_HIP(hipMemcpyAsync(vals, va, (size_t)n * sizeof(uint32_t),
+                           hipMemcpyDeviceToDevice, s));
+  }
 }
 
 }  // namespace
 
 void radix_sort_u32_pairs(hipStream_t s, Arena &ar, uint32_t *keys,
-                          uint32_t *vals, int64_t n) {
-  radix_sort_pairs(s, ar, keys, vals, n);
+                          uint32_t *vals, int64_t n, int key_bits) {
+  radix_sort_pairs(s, ar, keys, vals, n, key_bits);
 }
 
 Bvh build_lbvh(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t n) {
@@ -349,7 +355,7 @@ Bvh build_lbvh(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t n) {
   hipLaunchKernelGGL(morton_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s,
                      b.leaf_box, n, cb, codes, leaf_tri);
   SDF_HIP(hipGetLastError());
-  radix_sort_pairs(s, ar, codes, leaf_tri, n);
+  radix_sort_pairs(s, ar, codes, leaf_tri, n, 30);
   b.leaf_tri = leaf_tri;
   if (n == 1) {
     b.nodes = nullptr;

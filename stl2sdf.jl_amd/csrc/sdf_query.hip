@@ -150,6 +150,52 @@ __device__ inline bool axis_pair_parity_fast(const LineFast *__restrict__ fast,
   return true;
 }
 
+// Fast path on a line's inline summary for NQ points at coordinates xs
+// (same rules as axis_pair_parity_fast).  ok[q] turns false for a point
+// whose decisions fall inside a margin; the whole call returns false when
+// the summary is not usable (more than kInlineHits hits, or exact line).
+template <int NQ>
+__device__ inline bool inline_parity(const LineInline &r, const double *xs,
+                                     int *pos, int *neg, bool *ok) {
+  const uint32_t n = r.info & 0xFFu;
+  if ((r.info & kLineExact) || n > (uint32_t)kInlineHits) return false;
+  const int ncl = (int)((r.info >> 8) & 0xFFu);
+  int first_pos[NQ], last_neg[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    first_pos[q] = -1;
+    last_neg[q] = -1;
+    ok[q] = true;
+  }
+#pragma unroll
+  for (int h2 = 0; h2 < kInlineHits; h2++) {
+    if (h2 < (int)n) {
+      const double D = (double)r.delta[h2];
+      const int cid = r.cid[h2];
+#pragma unroll
+      for (int q = 0; q < NQ; q++) {
+        const double A = r.c[h2] - xs[q];
+        if (A > kRayEps + D) {
+          if (first_pos[q] < 0) first_pos[q] = cid;
+        } else if (!(A < kRayEps - D)) {
+          ok[q] = false;
+        }
+        if (A < -kRayEps - D) {
+          last_neg[q] = cid;
+        } else if (!(A > -kRayEps + D)) {
+          ok[q] = false;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    pos[q] = first_pos[q] >= 0 ? ((ncl - first_pos[q]) & 1) : 0;
+    neg[q] = last_neg[q] >= 0 ? ((last_neg[q] + 1) & 1) : 0;
+  }
+  return true;
+}
+
 __global__ __launch_bounds__(kTilePts) void dist_chunks_kernel(
     GridP g, TileDims td, const Tri64 *__restrict__ tri,
     const uint64_t *__restrict__ tile_off, const uint32_t *__restrict__ tile_tris,
@@ -203,81 +249,142 @@ __global__ __launch_bounds__(kTilePts) void dist_chunks_kernel(
     atomicMin((unsigned int *)slot, __float_as_uint(best));
 }
 
-// Per tile (one wavefront): distance from the chunk results (or h for tiles
-// with no triangle within h), the six axis-ray votes from the line-hit
-// lists, the combine step, and the outputs.  Points whose six votes are not
-// unanimous go to the pending list for the adaptive vote.
-__global__ __launch_bounds__(kTilePts) void fused_lattice_kernel(
-    GridP g, TileDims td, const uint64_t *__restrict__ chunk_off,
+// Sign / combine kernel.  One wavefront covers 8 x 8 x 4 lattice points:
+// lane (i, j) takes the 4 points k0..k0+3 of its z-line, so the z-line's hit
+// summary is read once and classifies all 4 points; the x- and y-lines of
+// the 4 points are independent loads.  Per point: distance from the chunk
+// results (h when its distance tile lists no triangle), the six axis-ray
+// votes (certified fast path, else the reference's arithmetic), combine,
+// outputs; points whose votes are not unanimous go to the pending list.
+constexpr int kSignZ = 4;
+
+struct SignDims {
+  int32_t sx, sy, sz;
+};
+
+__host__ __device__ inline SignDims sign_dims(const GridP &g) {
+  SignDims d;
+  d.sx = (g.nx + 7) / 8;
+  d.sy = (g.ny + 7) / 8;
+  d.sz = (g.nz + kSignZ - 1) / kSignZ;
+  return d;
+}
+
+__global__ __launch_bounds__(64) void fused_lattice_kernel(
+    GridP g, TileDims td, SignDims sd, const uint64_t *__restrict__ chunk_off,
     const float *__restrict__ d2buf, const uint64_t *__restrict__ line_off,
     const LineHit *__restrict__ hits, const LineFast *__restrict__ fast,
-    const uint32_t *__restrict__ meta, int64_t base1, int64_t base2, int do_sign,
-    FusedOut out, Pending *__restrict__ pend, uint32_t *npend, uint32_t cap,
-    uint32_t *err) {
-  const int64_t tile = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t *__restrict__ meta, const LineInline *__restrict__ inl,
+    int64_t base1, int64_t base2, int do_sign, FusedOut out,
+    Pending *__restrict__ pend, uint32_t *npend, uint32_t cap, uint32_t *err) {
+  TileDims bd;
+  bd.tx = sd.sx;
+  bd.ty = sd.sy;
+  bd.tz = sd.sz;
+  bd.ntiles = (int64_t)sd.sx * sd.sy * sd.sz;
+  int32_t bx, by, bz;
+  if (!brick_tile(xcd_remap(blockIdx.x, gridDim.x), bd, &bx, &by, &bz)) return;
   const int lane = threadIdx.x;
-  const int32_t tx = (int32_t)(tile % td.tx);
-  const int32_t ty = (int32_t)((tile / td.tx) % td.ty);
-  const int32_t tz = (int32_t)(tile / ((int64_t)td.tx * td.ty));
-  const int32_t i = tx * kTileX + (lane & (kTileX - 1));
-  const int32_t j = ty * kTileY + ((lane / kTileX) & (kTileY - 1));
-  const int32_t k = tz * kTileZ + lane / (kTileX * kTileY);
-  if (i >= g.nx || j >= g.ny || k >= g.nz) return;
-  const int64_t idx = ((int64_t)k * g.ny + j) * g.nx + i;
-
-  double dist = g.h;
-  if (d2buf) {
-    const uint64_t c0 = chunk_off[tile];
-    if (chunk_off[tile + 1] > c0) {
-      const float best = d2buf[c0 * kTilePts + lane];
-      if (best < __builtin_huge_valf()) {
-        const double d = sqrt((double)best);
-        dist = d < g.h ? d : g.h;
-      }
-    }
-  }
-
-  double sgn = 1.0, conf = 1.0;
-  bool pending = false;
-  if (do_sign) {
-    const Vec3d p = lattice_point(g, i, j, k);
-    const double o[3] = {p.x, p.y, p.z};
-    int inside = 0;
-    const int64_t Ls[3] = {(int64_t)k * g.ny + j, base1 + (int64_t)k * g.nx + i,
-                           base2 + (int64_t)j * g.nx + i};
+  const int32_t i = bx * 8 + (lane & 7);
+  const int32_t j = by * 8 + (lane >> 3);
+  const int32_t k0 = bz * kSignZ;
+  if (i >= g.nx || j >= g.ny) return;
+  const int nk = min(kSignZ, g.nz - k0);
+  const double x = lattice_coord(g.ox, g.h, i), y = lattice_coord(g.oy, g.h, j);
+  double zq[kSignZ];
 #pragma unroll
-    for (int ax = 0; ax < 3; ax++) {
-      const int64_t L = Ls[ax];
-      const uint64_t hb = line_off[L], he = line_off[L + 1];
-      int a, bpar;
-      if (!axis_pair_parity_fast(fast, hb, he, meta[L], o[ax], &a, &bpar))
-        axis_pair_parity(hits, hb, he, o, &a, &bpar);
-      inside += a + bpar;
+  for (int q = 0; q < kSignZ; q++) zq[q] = lattice_coord(g.oz, g.h, k0 + q);
+
+  int zpos[kSignZ], zneg[kSignZ];
+  bool zok[kSignZ];
+  if (do_sign) {
+    // z-line (i, j): one inline record classifies all 4 points
+    const int64_t Lz = base2 + (int64_t)j * g.nx + i;
+    const LineInline rz = inl[Lz];
+    if (!inline_parity<kSignZ>(rz, zq, zpos, zneg, zok)) {
+      const uint64_t hb = line_off[Lz], he = line_off[Lz + 1];
+#pragma unroll
+      for (int q = 0; q < kSignZ; q++) {
+        zok[q] = q < nk ? axis_pair_parity_fast(fast, hb, he, meta[Lz], zq[q], &zpos[q],
+                                                &zneg[q])
+                        : true;
+      }
     }
-    if (inside == 6) {
-      sgn = 1.0;
-    } else if (inside == 0) {
-      sgn = -1.0;
-    } else {
-      pending = true;
-      uint32_t slot = atomicAdd(npend, 1u);
-      if (slot < cap) {
-        Pending pr;
-        pr.idx = idx;
-        pr.dist = dist;
-        pr.inside = inside;
-        pr.used = 6;
-        pend[slot] = pr;
-      } else {
-        atomicOr(err, kErrPendingOverflow);
+#pragma unroll
+    for (int q = 0; q < kSignZ; q++) {
+      if (!zok[q] && q < nk) {
+        const double o[3] = {x, y, zq[q]};
+        axis_pair_parity(hits, line_off[Lz], line_off[Lz + 1], o, &zpos[q], &zneg[q]);
       }
     }
   }
-  if (out.dist) out.dist[idx] = dist;
-  if (!pending) {
-    if (out.sdf) out.sdf[idx] = dist * sgn;
-    if (out.signs) out.signs[idx] = sgn;
-    if (out.conf) out.conf[idx] = conf;
+
+#pragma unroll
+  for (int q = 0; q < kSignZ; q++) {
+    if (q >= nk) break;
+    const int32_t k = k0 + q;
+    const int64_t idx = ((int64_t)k * g.ny + j) * g.nx + i;
+    double dist = g.h;
+    if (d2buf) {
+      const int64_t tile = ((int64_t)(k / kTileZ) * td.ty + j / kTileY) * td.tx + i / kTileX;
+      const uint64_t c0 = chunk_off[tile];
+      if (chunk_off[tile + 1] > c0) {
+        const int tl = (i & (kTileX - 1)) + kTileX * ((j & (kTileY - 1)) + kTileY * (k & (kTileZ - 1)));
+        const float best = d2buf[c0 * kTilePts + tl];
+        if (best < __builtin_huge_valf()) {
+          const double d = sqrt((double)best);
+          dist = d < g.h ? d : g.h;
+        }
+      }
+    }
+    double sgn = 1.0;
+    bool pending = false;
+    if (do_sign) {
+      const double o[3] = {x, y, zq[q]};
+      int inside = zpos[q] + zneg[q];
+      int a, bpar;
+      bool okq;
+      const int64_t Lx = (int64_t)k * g.ny + j;
+      const int64_t Ly = base1 + (int64_t)k * g.nx + i;
+      const LineInline rx = inl[Lx];
+      const LineInline ry = inl[Ly];
+      if (!inline_parity<1>(rx, &x, &a, &bpar, &okq) || !okq) {
+        const uint64_t hb = line_off[Lx], he = line_off[Lx + 1];
+        if (!axis_pair_parity_fast(fast, hb, he, meta[Lx], x, &a, &bpar))
+          axis_pair_parity(hits, hb, he, o, &a, &bpar);
+      }
+      inside += a + bpar;
+      if (!inline_parity<1>(ry, &y, &a, &bpar, &okq) || !okq) {
+        const uint64_t hb = line_off[Ly], he = line_off[Ly + 1];
+        if (!axis_pair_parity_fast(fast, hb, he, meta[Ly], y, &a, &bpar))
+          axis_pair_parity(hits, hb, he, o, &a, &bpar);
+      }
+      inside += a + bpar;
+      if (inside == 6) {
+        sgn = 1.0;
+      } else if (inside == 0) {
+        sgn = -1.0;
+      } else {
+        pending = true;
+        const uint32_t slot = atomicAdd(npend, 1u);
+        if (slot < cap) {
+          Pending pr;
+          pr.idx = idx;
+          pr.dist = dist;
+          pr.inside = inside;
+          pr.used = 6;
+          pend[slot] = pr;
+        } else {
+          atomicOr(err, kErrPendingOverflow);
+        }
+      }
+    }
+    if (out.dist) out.dist[idx] = dist;
+    if (!pending) {
+      if (out.sdf) out.sdf[idx] = dist * sgn;
+      if (out.signs) out.signs[idx] = sgn;
+      if (out.conf) out.conf[idx] = 1.0;
+    }
   }
 }
 
@@ -673,10 +780,17 @@ void launch_fused_lattice(hipStream_t s, const GridP &g, const Tri64 *tri,
   (void)tri;
   TileDims td = tile_dims(g);
   if (td.ntiles <= 0) return;
-  hipLaunchKernelGGL(fused_lattice_kernel, dim3((unsigned)td.ntiles),
-                     dim3(kTilePts), 0, s, g, td, (const uint64_t *)b.chunk_off,
+  const SignDims sd = sign_dims(g);
+  TileDims bd;
+  bd.tx = sd.sx;
+  bd.ty = sd.sy;
+  bd.tz = sd.sz;
+  bd.ntiles = (int64_t)sd.sx * sd.sy * sd.sz;
+  hipLaunchKernelGGL(fused_lattice_kernel, dim3((unsigned)brick_launch_size(bd)),
+                     dim3(64), 0, s, g, td, sd, (const uint64_t *)b.chunk_off,
                      d2buf, (const uint64_t *)b.line_off, (const LineHit *)b.line_hits,
                      (const LineFast *)b.line_fast, (const uint32_t *)b.line_meta,
+                     (const LineInline *)b.line_inline,
                      b.line_base[1], b.line_base[2], do_sign ? 1 : 0, out, pend, npend,
                      pend_cap, err);
   SDF_HIP(hipGetLastError());

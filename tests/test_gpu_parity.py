@@ -143,3 +143,16 @@ def test_general_points_many_pending_star():
     sel = rng.choice(P.shape[1], 4000, replace=False)
     Q = np.asfortranarray(P[:, sel] + rng.normal(scale=grid.cell_size * 0.3, size=(3, 4000)))
     _check(mesh, grid, Q)
+
+
+def test_large_triangles_cube_fine_grid():
+    # 12 triangles spanning thousands of tiles and lines (the per-workgroup
+    # binning path); lattice planes pass within 1e-17 of the faces, so many
+    # ray decisions sit at the eps boundaries and take the exact path
+    v = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0],
+                  [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], float)
+    f = np.array([[0, 2, 1], [0, 3, 2], [4, 5, 6], [4, 6, 7], [0, 1, 5], [0, 5, 4],
+                  [2, 3, 7], [2, 7, 6], [1, 2, 6], [1, 6, 5], [0, 4, 7], [0, 7, 3]])
+    mesh = MeshGrid.TriangularMesh(np.asfortranarray(v.T), np.asfortranarray((f + 1).T))
+    grid = MeshGrid.Grid(np.zeros(3), np.ones(3), 98, 3)
+    _check(mesh, grid)
