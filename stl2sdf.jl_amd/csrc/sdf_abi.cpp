@@ -274,29 +274,37 @@ void run_lattice(sdf_ctx *c, const sdf::GridP &g, bool do_dist, bool do_sign,
     sdf::launch_dist_chunks(c->stream, g, c->tri, bins, d2buf);
   }
   uint32_t cap = (uint32_t)(npts < (1 << 22) ? (npts > 0 ? npts : 1) : (1 << 22));
+  uint32_t cap0 = (uint32_t)(npts < (1 << 20) ? (npts > 0 ? npts : 1) : (1 << 20));
   sdf::Pending *pend = c->ar.get<sdf::Pending>("pend", cap);
+  sdf::Pending *pend0 = c->ar.get<sdf::Pending>("pend0", cap0);
   c->mark("fused");
   sdf::launch_fused_lattice(c->stream, g, c->tri, bins, d2buf, do_sign, out, pend,
-                            c->d_words + 1, cap, c->d_words);
-  uint32_t np = 0;
+                            c->d_words + 1, cap, pend0, c->d_words + 2, cap0, c->d_words);
+  uint32_t np = 0, np0 = 0;
   if (do_sign) {
     np = read_word(c, 1);
-    if (np > cap) {
-      // Rerun with a list large enough for every undecided point.  Any
+    np0 = read_word(c, 2);
+    if (np > cap || np0 > cap0) {
+      // Rerun with lists large enough for every undecided point.  Any
       // other flag (bad index) is reported first.
       uint32_t e = read_word(c, 0) & ~sdf::kErrPendingOverflow;
       if (e) check_err_flags(c);
-      cap = np;
+      if (np > cap) cap = np;
+      if (np0 > cap0) cap0 = np0;
       pend = c->ar.get<sdf::Pending>("pend", cap);
+      pend0 = c->ar.get<sdf::Pending>("pend0", cap0);
       sdf::launch_fill_u32(c->stream, c->d_words, 4, 0u);
       sdf::launch_fused_lattice(c->stream, g, c->tri, bins, d2buf, do_sign, out, pend,
-                                c->d_words + 1, cap, c->d_words);
+                                c->d_words + 1, cap, pend0, c->d_words + 2, cap0,
+                                c->d_words);
       np = read_word(c, 1);
+      np0 = read_word(c, 2);
     }
   }
   if (np > 0) resolve_items(c, g, nullptr, rp, pend, np, 6, out);
+  if (np0 > 0) resolve_items(c, g, nullptr, rp, pend0, np0, 0, out);
+  c->counters[0] = (int64_t)np + np0;
   c->mark("end");
-  c->counters[0] = np;
   c->counters[1] = bins.ntile_pairs;
   c->counters[2] = bins.nline_hits;
   c->counters[3] = (int64_t)c->ar.bytes();

@@ -135,10 +135,13 @@ struct FusedOut {
 void launch_dist_chunks(hipStream_t s, const GridP &g, const Tri64 *tri,
                         const Bins &b, float *d2buf);
 // Per tile: distance from d2buf (or h), six axis rays, combine, outputs.
+// Points with a vote inside its rounding margin go to pend0 (restart from
+// ray 1 on the LBVH); non-unanimous votes go to pend (continue at ray 7).
 void launch_fused_lattice(hipStream_t s, const GridP &g, const Tri64 *tri,
                           const Bins &b, const float *d2buf, bool do_sign,
                           FusedOut out, Pending *pend, uint32_t *npend,
-                          uint32_t pend_cap, uint32_t *err);
+                          uint32_t pend_cap, Pending *pend0, uint32_t *npend0,
+                          uint32_t pend0_cap, uint32_t *err);
 
 struct RayParams {
   double dirs[kNumDirs * 3];

@@ -16,109 +16,6 @@ namespace {
 
 constexpr int kHitCap = 2048;  // per-ray hit buffer of the finisher (LDS)
 
-__device__ inline void cswap(double &a, double &b) {
-  double lo = fmin(a, b), hi = fmax(a, b);
-  a = lo;
-  b = hi;
-}
-
-// Unique-count of a sorted run of values, reference rule
-// (RaycastingSignDetection.jl:143-150): 1 + #{consecutive gaps > eps}.
-// Returns the parity for the +axis ray (values t > eps) and for the -axis
-// ray (values t < -eps, i.e. -t > eps).
-__device__ inline void axis_pair_parity(const LineHit *__restrict__ hits,
-                                        uint64_t hb, uint64_t he,
-                                        const double *o, int *pos_par,
-                                        int *neg_par) {
-  const int64_t n = (int64_t)(he - hb);
-  if (n <= 0) {
-    *pos_par = 0;
-    *neg_par = 0;
-    return;
-  }
-  const double INF = __builtin_huge_val();
-  if (n <= 8) {
-    double t[8];
-#pragma unroll
-    for (int q = 0; q < 8; q++) {
-      double x = INF;
-      if (q < n) x = line_hit_t(hits[hb + q], o);
-      t[q] = (x == x) ? x : INF;
-    }
-    // Batcher odd-even merge sort, 8 inputs
-    cswap(t[0], t[1]); cswap(t[2], t[3]); cswap(t[4], t[5]); cswap(t[6], t[7]);
-    cswap(t[0], t[2]); cswap(t[1], t[3]); cswap(t[4], t[6]); cswap(t[5], t[7]);
-    cswap(t[1], t[2]); cswap(t[5], t[6]);
-    cswap(t[0], t[4]); cswap(t[1], t[5]); cswap(t[2], t[6]); cswap(t[3], t[7]);
-    cswap(t[2], t[4]); cswap(t[3], t[5]);
-    cswap(t[1], t[2]); cswap(t[3], t[4]); cswap(t[5], t[6]);
-    int cp = 0, cn = 0;
-    bool hp = false, hn = false;
-    double pp = 0.0, pn = 0.0;
-#pragma unroll
-    for (int q = 0; q < 8; q++) {
-      double x = t[q];
-      if (x > kRayEps && x < INF) {
-        if (!hp) {
-          cp = 1;
-          hp = true;
-        } else if (x - pp > kRayEps) {
-          cp++;
-        }
-        pp = x;
-      }
-      if (x < -kRayEps) {
-        if (!hn) {
-          cn = 1;
-          hn = true;
-        } else if (x - pn > kRayEps) {
-          cn++;
-        }
-        pn = x;
-      }
-    }
-    *pos_par = cp & 1;
-    *neg_par = cn & 1;
-    return;
-  }
-  // Long lists: per element, find its strict predecessor among the accepted
-  // values (recomputing t), without storing the list.
-  int cp = 0, cn = 0;
-  for (int64_t a = 0; a < n; a++) {
-    double x = line_hit_t(hits[hb + a], o);
-    bool posx = x > kRayEps, negx = x < -kRayEps;
-    if (!posx && !negx) continue;
-    bool dup = false, have = false;
-    double pred = 0.0;
-    for (int64_t b = 0; b < n; b++) {
-      if (b == a) continue;
-      double y = line_hit_t(hits[hb + b], o);
-      bool same_side = posx ? (y > kRayEps) : (y < -kRayEps);
-      if (!same_side) continue;
-      if (y == x && b < a) dup = true;
-      if (y < x && (!have || y > pred)) {
-        pred = y;
-        have = true;
-      }
-    }
-    if (dup) continue;
-    if (!have || x - pred > kRayEps) {
-      if (posx)
-        cp++;
-      else
-        cn++;
-    }
-  }
-  *pos_par = cp & 1;
-  *neg_par = cn & 1;
-}
-
-// Distance work item = (tile, chunk of <= 64 listed triangles).  The chunk
-// is staged into LDS as tile-relative FP32 records (each lane converts one
-// triangle from FP64: v - tile_origin is formed in FP64, so the FP32 values
-// are small and accurate), then every lane takes the minimum over the chunk
-// for its point.  Tiles with several chunks combine by atomicMin on the
-// float bits (non-negative floats order like their bit patterns).
 // Fast path for one axis line: classify every hit of the line for the point
 // at coordinate x from its certified crossing (c - x vs +-eps, margin delta)
 // and count clusters.  Returns false when any decision falls inside a margin
@@ -276,7 +173,8 @@ __global__ __launch_bounds__(64) void fused_lattice_kernel(
     const LineHit *__restrict__ hits, const LineFast *__restrict__ fast,
     const uint32_t *__restrict__ meta, const LineInline *__restrict__ inl,
     int64_t base1, int64_t base2, int do_sign, FusedOut out,
-    Pending *__restrict__ pend, uint32_t *npend, uint32_t cap, uint32_t *err) {
+    Pending *__restrict__ pend, uint32_t *npend, uint32_t cap,
+    Pending *__restrict__ pend0, uint32_t *npend0, uint32_t cap0, uint32_t *err) {
   TileDims bd;
   bd.tx = sd.sx;
   bd.ty = sd.sy;
@@ -303,19 +201,11 @@ __global__ __launch_bounds__(64) void fused_lattice_kernel(
     const LineInline rz = inl[Lz];
     if (!inline_parity<kSignZ>(rz, zq, zpos, zneg, zok)) {
       const uint64_t hb = line_off[Lz], he = line_off[Lz + 1];
+      const uint32_t mz = meta[Lz];
 #pragma unroll
-      for (int q = 0; q < kSignZ; q++) {
-        zok[q] = q < nk ? axis_pair_parity_fast(fast, hb, he, meta[Lz], zq[q], &zpos[q],
-                                                &zneg[q])
+      for (int q = 0; q < kSignZ; q++)
+        zok[q] = q < nk ? axis_pair_parity_fast(fast, hb, he, mz, zq[q], &zpos[q], &zneg[q])
                         : true;
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < kSignZ; q++) {
-      if (!zok[q] && q < nk) {
-        const double o[3] = {x, y, zq[q]};
-        axis_pair_parity(hits, line_off[Lz], line_off[Lz + 1], o, &zpos[q], &zneg[q]);
-      }
     }
   }
 
@@ -343,24 +233,36 @@ __global__ __launch_bounds__(64) void fused_lattice_kernel(
       const double o[3] = {x, y, zq[q]};
       int inside = zpos[q] + zneg[q];
       int a, bpar;
-      bool okq;
+      bool okq, amb = !zok[q];
       const int64_t Lx = (int64_t)k * g.ny + j;
       const int64_t Ly = base1 + (int64_t)k * g.nx + i;
       const LineInline rx = inl[Lx];
       const LineInline ry = inl[Ly];
-      if (!inline_parity<1>(rx, &x, &a, &bpar, &okq) || !okq) {
-        const uint64_t hb = line_off[Lx], he = line_off[Lx + 1];
-        if (!axis_pair_parity_fast(fast, hb, he, meta[Lx], x, &a, &bpar))
-          axis_pair_parity(hits, hb, he, o, &a, &bpar);
-      }
+      if (!inline_parity<1>(rx, &x, &a, &bpar, &okq) || !okq)
+        amb |= !axis_pair_parity_fast(fast, line_off[Lx], line_off[Lx + 1], meta[Lx], x, &a,
+                                      &bpar);
       inside += a + bpar;
-      if (!inline_parity<1>(ry, &y, &a, &bpar, &okq) || !okq) {
-        const uint64_t hb = line_off[Ly], he = line_off[Ly + 1];
-        if (!axis_pair_parity_fast(fast, hb, he, meta[Ly], y, &a, &bpar))
-          axis_pair_parity(hits, hb, he, o, &a, &bpar);
-      }
+      if (!inline_parity<1>(ry, &y, &a, &bpar, &okq) || !okq)
+        amb |= !axis_pair_parity_fast(fast, line_off[Ly], line_off[Ly + 1], meta[Ly], y, &a,
+                                      &bpar);
       inside += a + bpar;
-      if (inside == 6) {
+      (void)o;
+      if (amb) {
+        // a decision sat inside its rounding margin: the point restarts the
+        // vote from ray 1 with the reference's arithmetic (LBVH path)
+        pending = true;
+        const uint32_t slot = atomicAdd(npend0, 1u);
+        if (slot < cap0) {
+          Pending pr;
+          pr.idx = idx;
+          pr.dist = dist;
+          pr.inside = 0;
+          pr.used = 0;
+          pend0[slot] = pr;
+        } else {
+          atomicOr(err, kErrPendingOverflow);
+        }
+      } else if (inside == 6) {
         sgn = 1.0;
       } else if (inside == 0) {
         sgn = -1.0;
@@ -776,7 +678,8 @@ void launch_dist_chunks(hipStream_t s, const GridP &g, const Tri64 *tri,
 void launch_fused_lattice(hipStream_t s, const GridP &g, const Tri64 *tri,
                           const Bins &b, const float *d2buf, bool do_sign,
                           FusedOut out, Pending *pend, uint32_t *npend,
-                          uint32_t pend_cap, uint32_t *err) {
+                          uint32_t pend_cap, Pending *pend0, uint32_t *npend0,
+                          uint32_t pend0_cap, uint32_t *err) {
   (void)tri;
   TileDims td = tile_dims(g);
   if (td.ntiles <= 0) return;
@@ -792,7 +695,7 @@ void launch_fused_lattice(hipStream_t s, const GridP &g, const Tri64 *tri,
                      (const LineFast *)b.line_fast, (const uint32_t *)b.line_meta,
                      (const LineInline *)b.line_inline,
                      b.line_base[1], b.line_base[2], do_sign ? 1 : 0, out, pend, npend,
-                     pend_cap, err);
+                     pend_cap, pend0, npend0, pend0_cap, err);
   SDF_HIP(hipGetLastError());
 }
 
