@@ -89,6 +89,10 @@ struct sdf_ctx {
   int device = 0;
   hipStream_t own = nullptr;
   hipStream_t stream = nullptr;
+  // helper streams for independent build chains (line hits, LBVH); default
+  // priority, joined to the main stream with events
+  hipStream_t aux[2] = {nullptr, nullptr};
+  hipEvent_t join[3] = {nullptr, nullptr, nullptr};
   std::mutex mu;
   sdf::Arena ar;
   // device mesh
@@ -195,6 +199,16 @@ const sdf::Bvh &need_bvh(sdf_ctx *c) {
   return c->bvh;
 }
 
+// Build the LBVH on helper stream q (joined later through join[2]).
+void bvh_async(sdf_ctx *c, int q) {
+  if (c->bvh_ready) return;
+  c->ar.scope = "bvh/";
+  c->bvh = sdf::build_lbvh(c->aux[q], c->ar, c->tri, c->nel);
+  c->ar.scope.clear();
+  c->bvh_ready = true;
+  SDF_HIP(hipEventRecord(c->join[2], c->aux[q]));
+}
+
 // Adaptive vote for a list of items starting at ray r0 (all rays below r0
 // already counted in the items' state).  Items that need more than the
 // first min_rays rays are carried to a second pass.
@@ -258,14 +272,34 @@ void run_lattice(sdf_ctx *c, const sdf::GridP &g, bool do_dist, bool do_sign,
   if (!c->mesh_ready) fail(SDF_ERR_ARG, "no mesh");
   int64_t npts = grid_points(g);
   sdf::Bins bins;
+  // three independent chains: LBVH (helper stream 1, no host syncs), line
+  // hits (helper stream 0), tile bins + distances (main stream)
+  SDF_HIP(hipEventRecord(c->join[0], c->stream));
+  for (int q = 0; q < 2; q++) SDF_HIP(hipStreamWaitEvent(c->aux[q], c->join[0], 0));
+  const bool bvh_was_ready = c->bvh_ready;
+  // counting phases first (no host waits; the critical tile chain leads),
+  // then the LBVH, then size + fill each chain
+  sdf::LineBuild lb;
+  sdf::TileBuild tb;
   c->mark("bin_tiles");
+  if (do_dist) sdf::tile_bins_begin(c->stream, c->ar, c->tri, c->nel, g, g.h, tb);
+  if (do_sign) {
+    c->ar.scope = "line/";
+    sdf::line_hits_begin(c->aux[0], c->ar, c->tri, c->nel, g, lb, bins);
+    c->ar.scope.clear();
+    bvh_async(c, 1);
+  }
+  if (do_sign) {
+    c->ar.scope = "line/";
+    sdf::line_hits_finish(c->aux[0], c->ar, c->tri, c->nel, g, lb, bins);
+    c->ar.scope.clear();
+    SDF_HIP(hipEventRecord(c->join[1], c->aux[0]));
+  }
   float *d2buf = nullptr;
   if (do_dist) {
-    sdf::build_tile_bins(c->stream, c->ar, c->tri, c->nel, g, g.h, bins);
+    sdf::tile_bins_finish(c->stream, c->ar, c->tri, c->nel, g, tb, bins);
     sdf::build_dist_chunks(c->stream, c->ar, g, bins);
   }
-  c->mark("bin_lines");
-  if (do_sign) sdf::build_line_hits(c->stream, c->ar, c->tri, c->nel, g, bins);
   if (do_dist) {
     c->mark("dist");
     d2buf = c->ar.get<float>("d2buf", (size_t)bins.nchunks * sdf::kTilePts + 1);
@@ -273,6 +307,8 @@ void run_lattice(sdf_ctx *c, const sdf::GridP &g, bool do_dist, bool do_sign,
                          (int64_t)bins.nchunks * sdf::kTilePts, 0x7F800000u);
     sdf::launch_dist_chunks(c->stream, g, c->tri, bins, d2buf);
   }
+  c->mark("bin_lines_wait");
+  if (do_sign) SDF_HIP(hipStreamWaitEvent(c->stream, c->join[1], 0));
   uint32_t cap = (uint32_t)(npts < (1 << 22) ? (npts > 0 ? npts : 1) : (1 << 22));
   uint32_t cap0 = (uint32_t)(npts < (1 << 20) ? (npts > 0 ? npts : 1) : (1 << 20));
   sdf::Pending *pend = c->ar.get<sdf::Pending>("pend", cap);
@@ -300,6 +336,10 @@ void run_lattice(sdf_ctx *c, const sdf::GridP &g, bool do_dist, bool do_sign,
       np = read_word(c, 1);
       np0 = read_word(c, 2);
     }
+  }
+  if (do_sign && !bvh_was_ready) {
+    c->mark("lbvh_wait");
+    SDF_HIP(hipStreamWaitEvent(c->stream, c->join[2], 0));
   }
   if (np > 0) resolve_items(c, g, nullptr, rp, pend, np, 6, out);
   if (np0 > 0) resolve_items(c, g, nullptr, rp, pend0, np0, 0, out);
@@ -358,6 +398,10 @@ sdf_status sdf_ctx_create(int32_t device, sdf_ctx **out) {
       SDF_HIP(hipSetDevice(device));
       SDF_HIP(hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking));
       c->stream = c->own;
+      for (int q = 0; q < 2; q++)
+        SDF_HIP(hipStreamCreateWithFlags(&c->aux[q], hipStreamNonBlocking));
+      for (int q = 0; q < 3; q++)
+        SDF_HIP(hipEventCreateWithFlags(&c->join[q], hipEventDisableTiming));
       SDF_HIP(hipMalloc(&c->d_words, 64));
     } catch (...) {
       delete c;
@@ -374,6 +418,13 @@ sdf_status sdf_ctx_destroy(sdf_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     c->ar.release();
     for (auto e : c->ev) (void)hipEventDestroy(e);
+    for (int q = 0; q < 2; q++)
+      if (c->aux[q]) {
+        (void)hipStreamSynchronize(c->aux[q]);
+        (void)hipStreamDestroy(c->aux[q]);
+      }
+    for (int q = 0; q < 3; q++)
+      if (c->join[q]) (void)hipEventDestroy(c->join[q]);
     if (c->d_words) (void)hipFree(c->d_words);
     if (c->own) (void)hipStreamDestroy(c->own);
     delete c;

@@ -113,12 +113,12 @@ void scan_rec(hipStream_t s, Arena &ar, const TIn *in, int64_t n_in,
               uint64_t *out, int64_t n_out, int level) {
   int64_t nb = (n_out + kScanBlock - 1) / kScanBlock;
   if (nb < 1) nb = 1;
-  uint64_t *bsum = ar.get<uint64_t>("scan_bsum" + std::to_string(level), nb + 1);
+  uint64_t *bsum = ar.get<uint64_t>(ar.tmp("scan_bsum" + std::to_string(level)), nb + 1);
   hipLaunchKernelGGL(scan_block_kernel<TIn>, dim3((unsigned)nb),
                      dim3(kScanThreads), 0, s, in, n_in, out, n_out, bsum);
   SDF_HIP(hipGetLastError());
   if (nb > 1) {
-    uint64_t *boff = ar.get<uint64_t>("scan_boff" + std::to_string(level), nb + 1);
+    uint64_t *boff = ar.get<uint64_t>(ar.tmp("scan_boff" + std::to_string(level)), nb + 1);
     scan_rec<uint64_t>(s, ar, bsum, nb, boff, nb, level + 1);
     hipLaunchKernelGGL(scan_add_kernel, dim3(grid_for(n_out, 256)), dim3(256),
                        0, s, out, n_out, boff);
@@ -651,35 +651,39 @@ void scan_counts(hipStream_t s, Arena &ar, const uint32_t *in, uint64_t *out,
   scan_rec<uint32_t>(s, ar, in, n, out, n + 1, 0);
 }
 
-void build_tile_bins(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
-                     const GridP &g, double radius, Bins &b) {
+void tile_bins_begin(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
+                     const GridP &g, double radius, TileBuild &tb) {
   TileDims td = tile_dims(g);
-  const double R = radius * (1.0 + 1e-6);
-  uint32_t *tcnt = ar.get<uint32_t>("bin_tri_cnt", nel + 1);
-  uint64_t *toff = ar.get<uint64_t>("bin_tri_off", nel + 2);
-  uint32_t *big = ar.get<uint32_t>("bin_big", nel + 1);
-  uint32_t *nbig = ar.get<uint32_t>("bin_nbig", 1);
-  uint64_t *tile_off = ar.get<uint64_t>("bin_tile_off", td.ntiles + 2);
-  launch_fill_u32(s, nbig, 1, 0u);
-  uint64_t npairs = 0;
+  tb.R = radius * (1.0 + 1e-6);
+  tb.tcnt = ar.get<uint32_t>("bin_tri_cnt", nel + 1);
+  tb.toff = ar.get<uint64_t>("bin_tri_off", nel + 2);
+  tb.big = ar.get<uint32_t>("bin_big", nel + 1);
+  tb.nbig = ar.get<uint32_t>("bin_nbig", 1);
+  launch_fill_u32(s, tb.nbig, 1, 0u);
   if (nel > 0) {
     hipLaunchKernelGGL(tile_count_small_kernel, dim3(grid_for(nel, 256)), dim3(256), 0, s,
-                       tri, nel, g, td, R, tcnt, big, nbig);
-    hipLaunchKernelGGL(tile_big_kernel, dim3(1024), dim3(256), 0, s, tri, g, td, R, 0,
-                       (const uint32_t *)big, (const uint32_t *)nbig, tcnt,
+                       tri, nel, g, td, tb.R, tb.tcnt, tb.big, tb.nbig);
+    hipLaunchKernelGGL(tile_big_kernel, dim3(1024), dim3(256), 0, s, tri, g, td, tb.R, 0,
+                       (const uint32_t *)tb.big, (const uint32_t *)tb.nbig, tb.tcnt,
                        (const uint64_t *)nullptr, (uint32_t *)nullptr, (uint32_t *)nullptr);
     SDF_HIP(hipGetLastError());
-    scan_counts(s, ar, tcnt, toff, nel);
-    npairs = read_u64(s, toff + nel);
+    scan_counts(s, ar, tb.tcnt, tb.toff, nel);
   }
+}
+
+void tile_bins_finish(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
+                      const GridP &g, TileBuild &tb, Bins &b) {
+  TileDims td = tile_dims(g);
+  uint64_t *tile_off = ar.get<uint64_t>("bin_tile_off", td.ntiles + 2);
+  const uint64_t npairs = nel > 0 ? read_u64(s, tb.toff + nel) : 0;
   uint32_t *keys = ar.get<uint32_t>("bin_tile_keys", npairs + 1);
   uint32_t *tile_tris = ar.get<uint32_t>("bin_tile_tris", npairs + 1);
   if (npairs > 0) {
     hipLaunchKernelGGL(tile_emit_small_kernel, dim3(grid_for(nel, 256)), dim3(256), 0, s,
-                       tri, nel, g, td, R, (const uint64_t *)toff, keys, tile_tris);
-    hipLaunchKernelGGL(tile_big_kernel, dim3(1024), dim3(256), 0, s, tri, g, td, R, 1,
-                       (const uint32_t *)big, (const uint32_t *)nbig, tcnt,
-                       (const uint64_t *)toff, keys, tile_tris);
+                       tri, nel, g, td, tb.R, (const uint64_t *)tb.toff, keys, tile_tris);
+    hipLaunchKernelGGL(tile_big_kernel, dim3(1024), dim3(256), 0, s, tri, g, td, tb.R, 1,
+                       (const uint32_t *)tb.big, (const uint32_t *)tb.nbig, tb.tcnt,
+                       (const uint64_t *)tb.toff, keys, tile_tris);
     SDF_HIP(hipGetLastError());
     int bits = 1;
     while (bits < 32 && ((int64_t)1 << bits) <= td.ntiles) bits++;
@@ -693,6 +697,13 @@ void build_tile_bins(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
   b.tile_off = tile_off;
   b.tile_tris = tile_tris;
   b.ntile_pairs = (int64_t)npairs;
+}
+
+void build_tile_bins(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
+                     const GridP &g, double radius, Bins &b) {
+  TileBuild tb;
+  tile_bins_begin(s, ar, tri, nel, g, radius, tb);
+  tile_bins_finish(s, ar, tri, nel, g, tb, b);
 }
 
 void build_dist_chunks(hipStream_t s, Arena &ar, const GridP &g, Bins &b) {
@@ -715,56 +726,69 @@ void build_dist_chunks(hipStream_t s, Arena &ar, const GridP &g, Bins &b) {
   b.nchunks = (int64_t)n;
 }
 
-void build_line_hits(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
-                     const GridP &g, Bins &b) {
-  int64_t nlx = (int64_t)g.ny * g.nz, nly = (int64_t)g.nx * g.nz,
-          nlz = (int64_t)g.nx * g.ny;
-  int64_t nlines = nlx + nly + nlz;
+void line_hits_begin(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
+                     const GridP &g, LineBuild &lb, Bins &b) {
+  lb.nlx = (int64_t)g.ny * g.nz;
+  lb.nly = (int64_t)g.nx * g.nz;
+  const int64_t nlz = (int64_t)g.nx * g.ny;
+  lb.nlines = lb.nlx + lb.nly + nlz;
   b.line_base[0] = 0;
-  b.line_base[1] = nlx;
-  b.line_base[2] = nlx + nly;
-  b.line_base[3] = nlines;
-  uint32_t *lcnt = ar.get<uint32_t>("line_cnt", nlines + 1);
-  uint64_t *loff = ar.get<uint64_t>("line_off", nlines + 2);
-  uint32_t *big = ar.get<uint32_t>("line_big", nel + 1);
-  uint32_t *nbig = ar.get<uint32_t>("line_nbig", 1);
-  launch_fill_u32(s, lcnt, nlines + 1, 0u);
-  launch_fill_u32(s, nbig, 1, 0u);
+  b.line_base[1] = lb.nlx;
+  b.line_base[2] = lb.nlx + lb.nly;
+  b.line_base[3] = lb.nlines;
+  lb.lcnt = ar.get<uint32_t>("line_cnt", lb.nlines + 1);
+  lb.loff = ar.get<uint64_t>("line_off", lb.nlines + 2);
+  lb.big = ar.get<uint32_t>("line_big", nel + 1);
+  lb.nbig = ar.get<uint32_t>("line_nbig", 1);
+  launch_fill_u32(s, lb.lcnt, lb.nlines + 1, 0u);
+  launch_fill_u32(s, lb.nbig, 1, 0u);
   if (nel > 0) {
     hipLaunchKernelGGL(line_bin_small_kernel, dim3(grid_for(nel, 256)), dim3(256), 0, s,
-                       tri, nel, g, nlx, nlx + nly, 0, lcnt, (const uint64_t *)nullptr,
-                       (LineHit *)nullptr, big, nbig);
-    hipLaunchKernelGGL(line_bin_big_kernel, dim3(1024), dim3(256), 0, s, tri, g, nlx,
-                       nlx + nly, 0, lcnt, (const uint64_t *)nullptr, (LineHit *)nullptr,
-                       (const uint32_t *)big, (const uint32_t *)nbig);
+                       tri, nel, g, lb.nlx, lb.nlx + lb.nly, 0, lb.lcnt,
+                       (const uint64_t *)nullptr, (LineHit *)nullptr, lb.big, lb.nbig);
+    hipLaunchKernelGGL(line_bin_big_kernel, dim3(1024), dim3(256), 0, s, tri, g, lb.nlx,
+                       lb.nlx + lb.nly, 0, lb.lcnt, (const uint64_t *)nullptr,
+                       (LineHit *)nullptr, (const uint32_t *)lb.big,
+                       (const uint32_t *)lb.nbig);
     SDF_HIP(hipGetLastError());
   }
-  scan_counts(s, ar, lcnt, loff, nlines);
-  const uint64_t nhits = read_u64(s, loff + nlines);
+  scan_counts(s, ar, lb.lcnt, lb.loff, lb.nlines);
+}
+
+void line_hits_finish(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
+                      const GridP &g, LineBuild &lb, Bins &b) {
+  const uint64_t nhits = read_u64(s, lb.loff + lb.nlines);
   LineHit *hits = ar.get<LineHit>("line_hits", nhits + 1);
   if (nhits > 0) {
-    launch_fill_u32(s, lcnt, nlines + 1, 0u);  // reused as the cursor
+    launch_fill_u32(s, lb.lcnt, lb.nlines + 1, 0u);  // reused as the cursor
     hipLaunchKernelGGL(line_bin_small_kernel, dim3(grid_for(nel, 256)), dim3(256), 0, s,
-                       tri, nel, g, nlx, nlx + nly, 1, lcnt, (const uint64_t *)loff, hits,
-                       big, nbig);
-    hipLaunchKernelGGL(line_bin_big_kernel, dim3(1024), dim3(256), 0, s, tri, g, nlx,
-                       nlx + nly, 1, lcnt, (const uint64_t *)loff, hits,
-                       (const uint32_t *)big, (const uint32_t *)nbig);
+                       tri, nel, g, lb.nlx, lb.nlx + lb.nly, 1, lb.lcnt,
+                       (const uint64_t *)lb.loff, hits, lb.big, lb.nbig);
+    hipLaunchKernelGGL(line_bin_big_kernel, dim3(1024), dim3(256), 0, s, tri, g, lb.nlx,
+                       lb.nlx + lb.nly, 1, lb.lcnt, (const uint64_t *)lb.loff, hits,
+                       (const uint32_t *)lb.big, (const uint32_t *)lb.nbig);
     SDF_HIP(hipGetLastError());
   }
   LineFast *fast = ar.get<LineFast>("line_fast", nhits + 1);
-  uint32_t *meta = ar.get<uint32_t>("line_meta", nlines + 1);
-  LineInline *inl = ar.get<LineInline>("line_inline", nlines + 1);
-  hipLaunchKernelGGL(line_prep_kernel, dim3(grid_for(nlines, 256)), dim3(256), 0, s, g,
-                     (const uint64_t *)loff, (const LineHit *)hits, nlx, nlx + nly,
-                     nlines, fast, meta, inl);
+  uint32_t *meta = ar.get<uint32_t>("line_meta", lb.nlines + 1);
+  LineInline *inl = ar.get<LineInline>("line_inline", lb.nlines + 1);
+  hipLaunchKernelGGL(line_prep_kernel, dim3(grid_for(lb.nlines, 256)), dim3(256), 0, s, g,
+                     (const uint64_t *)lb.loff, (const LineHit *)hits, lb.nlx,
+                     lb.nlx + lb.nly, lb.nlines, fast, meta, inl);
   SDF_HIP(hipGetLastError());
-  b.line_off = loff;
+  b.line_off = lb.loff;
   b.line_hits = hits;
   b.line_fast = fast;
   b.line_meta = meta;
   b.line_inline = inl;
   b.nline_hits = (int64_t)nhits;
+}
+
+void build_line_hits(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
+                     const GridP &g, Bins &b) {
+  LineBuild lb;
+  line_hits_begin(s, ar, tri, nel, g, lb, b);
+  line_hits_finish(s, ar, tri, nel, g, lb, b);
 }
 
 }  // namespace sdf

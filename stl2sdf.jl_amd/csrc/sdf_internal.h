@@ -56,6 +56,10 @@ class Arena {
     for (auto &kv : bufs_) s += kv.second.bytes;
     return s;
   }
+  // Prefix for scratch names (scan / sort temporaries) so that work issued
+  // on concurrent streams uses distinct buffers.
+  std::string scope;
+  std::string tmp(const std::string &name) const { return scope + name; }
 
  private:
   struct Buf {
@@ -118,6 +122,26 @@ struct Bins {
 
 void build_tile_bins(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
                      const GridP &g, double radius, Bins &b);
+// Two-phase forms: *_begin issues the counting kernels and scans (no host
+// wait); *_finish reads the total, sizes the lists and fills them.
+struct TileBuild {
+  double R = 0.0;
+  uint32_t *tcnt = nullptr, *big = nullptr, *nbig = nullptr;
+  uint64_t *toff = nullptr;
+};
+struct LineBuild {
+  int64_t nlx = 0, nly = 0, nlines = 0;
+  uint32_t *lcnt = nullptr, *big = nullptr, *nbig = nullptr;
+  uint64_t *loff = nullptr;
+};
+void tile_bins_begin(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
+                     const GridP &g, double radius, TileBuild &tb);
+void tile_bins_finish(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
+                      const GridP &g, TileBuild &tb, Bins &b);
+void line_hits_begin(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
+                     const GridP &g, LineBuild &lb, Bins &b);
+void line_hits_finish(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
+                      const GridP &g, LineBuild &lb, Bins &b);
 void build_line_hits(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
                      const GridP &g, Bins &b);
 void build_dist_chunks(hipStream_t s, Arena &ar, const GridP &g, Bins &b);

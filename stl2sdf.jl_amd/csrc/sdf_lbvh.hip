@@ -302,10 +302,10 @@ void radix_sort_pairs(hipStream_t s, Arena &ar, uint32_t *keys, uint32_t *vals,
                       int64_t n, int key_bits) {
   if (n <= 1) return;
   int64_t nblocks = (n + kSortTile - 1) / kSortTile;
-  uint32_t *k2 = ar.get<uint32_t>("sort_k2", n);
-  uint32_t *v2 = ar.get<uint32_t>("sort_v2", n);
-  uint32_t *hist = ar.get<uint32_t>("sort_hist", kRadix * nblocks + 1);
-  uint64_t *off = ar.get<uint64_t>("sort_off", kRadix * nblocks + 2);
+  uint32_t *k2 = ar.get<uint32_t>(ar.tmp("sort_k2"), n);
+  uint32_t *v2 = ar.get<uint32_t>(ar.tmp("sort_v2"), n);
+  uint32_t *hist = ar.get<uint32_t>(ar.tmp("sort_hist"), kRadix * nblocks + 1);
+  uint64_t *off = ar.get<uint64_t>(ar.tmp("sort_off"), kRadix * nblocks + 2);
   uint32_t *ka = keys, *va = vals, *kb = k2, *vb = v2;
   int passes = 0;
   for (int shift = 0; shift < key_bits; shift += kRadixBits, passes++) {
