@@ -9,8 +9,11 @@ binning, fused distance + axis-ray kernel, adaptive vote for undecided
 points, written as a float64 SDF array in HBM.
 
 The JSON line carries the stage split, the roofline of the dominant (fused)
-kernel from HIP events on its stream, and the CPU oracle timed on a bounded
-random sample of the same grid points on the host cores.
+kernel from HIP events on its stream, the learned-radius path on the same
+workload (network fitted once on a 128-point coarse lattice, not timed; each
+timed step as above with the radius-limited distance loop; its output is
+checked bit-for-bit against the plain path's), and the CPU oracle timed on a
+bounded random sample of the same grid points on the host cores.
 """
 import argparse
 import ctypes as C
@@ -103,6 +106,8 @@ def main():
     ap.add_argument("--workload", default="star512")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--fit-steps", type=int, default=1500)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
                     help="only copy an MB-sized device buffer 3 times (counter calibration)")
     args = ap.parse_args()
@@ -166,6 +171,58 @@ def main():
     cnt = _lib.counters(ctx)
     log(f"[bench] timed {args.steps} steps in {dt:.3f} s")
 
+    learned = None
+    if not args.no_learned:
+        import torch as _t
+        fst = np.zeros(9)
+        step()
+        _lib.check(L.sdf_learned_fit(ctx, _lib.dptr(amin), h, _lib.i64ptr(N), 128,
+                                     args.fit_steps, 4096, C.c_float(2e-3), 1, 0.99,
+                                     _lib.dptr(fst), 9))
+        fit = dict(zip(("loss", "mae_band_hc", "delta_hc", "samples", "samples_band",
+                        "train_ms", "coarse_ms", "hc", "steps"), fst.tolist()))
+        log(f"[bench] learned fit: {fit}")
+        out_l = torch.empty_like(out)
+
+        def lstep():
+            _lib.check(L.sdf_mesh_set_device(ctx, C.c_void_p(dX.data_ptr()), mesh.nnp,
+                                             C.c_void_p(dI.data_ptr()), mesh.nel))
+            _lib.check(L.sdf_learned_signed_distance_device(
+                ctx, _lib.dptr(amin), h, _lib.i64ptr(N), C.c_void_p(out_l.data_ptr())))
+
+        for w in range(args.warmup):
+            lstep()
+            torch.cuda.synchronize()
+        _lib.check(L.sdf_ctx_enable_timing(ctx, 1))
+        lstages = {}
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            lstep()
+            for k, v in _lib.stage_times(ctx).items():
+                lstages.setdefault(k, []).append(v)
+        torch.cuda.synchronize()
+        ldt = time.perf_counter() - t0
+        _lib.check(L.sdf_ctx_enable_timing(ctx, 0))
+        lst = np.zeros(7)
+        nst = C.c_int32()
+        _lib.check(L.sdf_learned_stats(ctx, _lib.dptr(lst), 7, C.byref(nst)))
+        step()
+        torch.cuda.synchronize()
+        identical = bool(_t.equal(out, out_l))
+        lval = grid.ngp / (ldt / args.steps)
+        learned = {
+            "value": lval, "ms_per_step": ldt / args.steps * 1e3,
+            "vs_plain": lval / (grid.ngp / (dt / args.steps)),
+            "identical_to_plain": identical,
+            "stages_ms": {k: float(np.mean(v)) for k, v in lstages.items()},
+            "fit": fit,
+            "run": dict(zip(("fallback_nodes", "pairs_evaluated", "pairs_plain",
+                             "node_pairs_radius", "node_pairs_h", "delta", "hc"),
+                            lst.tolist())),
+        }
+        log(f"[bench] learned: {lval:.4g} pts/s, identical={identical}")
+
     ms_step = dt / args.steps * 1e3
     value = grid.ngp / (dt / args.steps)
     stage_ms = {k: float(np.mean(v)) for k, v in stages.items()}
@@ -173,7 +230,7 @@ def main():
     alg_bytes = 8.0 * grid.ngp + 24.0 * mesh.nnp + 24.0 * mesh.nel
     achieved = alg_bytes / (fused_ms * 1e-3) / 1e9
     res = {
-        "metric": "signed grid pts/sec at 512^3, 1M tris (plain GPU path)",
+        "metric": "signed grid pts/sec at 512^3, 1M tris: learned-radius vs plain GPU vs CPU ref",
         "value": value,
         "unit": "signed grid points/s",
         "n_gpus": 1,
@@ -195,6 +252,10 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "fused_lattice_kernel", "alg_bytes_per_launch": alg_bytes},
     }
+    res["paths"] = {"value": "plain GPU path (the drop-in default)",
+                    "learned_radius": "same workload with the learned search radius"}
+    if learned is not None:
+        res["learned_radius"] = learned
     if not args.no_cpu_baseline:
         threads = min(16, len(os.sched_getaffinity(0)))
         log(f"[bench] cpu baseline with {threads} threads")

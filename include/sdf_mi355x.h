@@ -121,6 +121,9 @@ sdf_status sdf_mesh_set_device(sdf_ctx *ctx, const double *d_X, int64_t nnp,
 sdf_status sdf_signed_distance_device(sdf_ctx *ctx, const double aabb_min[3],
                                       double h, const int64_t N[3],
                                       double *d_sdf_out);
+/* Host-array form of sdf_mesh_set_device (copies X / IEN, synchronous). */
+sdf_status sdf_mesh_set(sdf_ctx *ctx, const double *X, int64_t nnp, const int64_t *IEN,
+                        int64_t nel);
 
 /* remove_sdf_artifacts!(sdf, grid; threshold, min_component_ratio)
  * -- src/SignedDistances/SdfArtifactRemoval.jl:134-248.
@@ -166,6 +169,65 @@ sdf_status sdf_rbf_smoothing(sdf_ctx *ctx, const double *dist,
 /* The 36 prioritised ray directions (generate_ray_directions_prioritized,
  * RaycastingSignDetection.jl:162-227), 3 doubles each. */
 sdf_status sdf_ray_directions(double *dirs_out /* 108 */);
+
+/* ------------------------------------------------------------------
+ * Learned search radius.  No counterpart in the reference: a per-mesh
+ * coordinate MLP (16 Fourier frequencies, three hidden layers of 128, ReLU,
+ * BF16 weights and activations with FP32 accumulation) proposes a search
+ * radius for each fine-grid node; the distance loop evaluates only triangles
+ * within it and re-runs a node's tile over all triangles when none is found,
+ * so every output equals the one of sdf_signed_distance (nodes whose radius
+ * held no triangle are settled by exact LBVH queries with the same
+ * arithmetic).
+ *
+ * Parameter vector (SDF_MLP_NPARAM floats, weights out x in, row-major):
+ *   W1 @0, b1 @16384, W2 @16512, b2 @32896, W3 @33024, b3 @49408,
+ *   w4 @49536, b4 @49664.
+ * Inputs u are normalised coordinates (3 floats per point); features are
+ * u, 0, then sin/cos(pi 2^(0.4k) u_a) for k < 16, zero padded to 128. */
+#define SDF_MLP_NPARAM 49665
+
+sdf_status sdf_mlp_set_params(sdf_ctx *ctx, const float *params);
+sdf_status sdf_mlp_get_params(sdf_ctx *ctx, float *params_out);
+/* y_out[i] = network(u[3i..3i+2]) */
+sdf_status sdf_mlp_forward(sdf_ctx *ctx, const float *u, int64_t n, float *y_out);
+/* Mean squared error over (u, t) and its gradient (SDF_MLP_NPARAM floats). */
+sdf_status sdf_mlp_loss_grad(sdf_ctx *ctx, const float *u, const float *t, int64_t n,
+                             float *grad_out, double *loss_out);
+/* `steps` Adam steps (beta 0.9 / 0.999, eps 1e-8) on batches of `batch`
+ * samples, learning rate decayed by a cosine from lr to lr/20; seed 0 takes
+ * the samples in order, otherwise reshuffles every epoch.  loss_hist
+ * (nullable) receives each step's batch loss. */
+sdf_status sdf_mlp_train(sdf_ctx *ctx, const float *u, const float *t, int64_t n,
+                         int32_t steps, int32_t batch, float lr, uint64_t seed,
+                         float *loss_hist);
+/* Fit the network to the context's mesh for the fine grid (aabb_min, h, N):
+ * exact distances on a coarse lattice over the same box with n_coarse points
+ * on the longest axis (spacing hc), clamped at hc, are the targets (d/hc);
+ * training set = every node with d < hc plus about as many farther nodes.
+ * The radius offset delta is the given quantile (<= 0: 0.99) of
+ * (target - output) over the nodes with d < hc; a fine node's radius is
+ * min(max((output + delta) * hc, h / 100), h).  stats_out (nullable, up to
+ * 9): final loss, mean absolute error (units of hc), delta, samples, samples
+ * with d < hc, training ms, coarse-distance ms, hc, steps. */
+sdf_status sdf_learned_fit(sdf_ctx *ctx, const double aabb_min[3], double h,
+                           const int64_t N[3], int32_t n_coarse, int32_t steps,
+                           int32_t batch, float lr, uint64_t seed, double quantile,
+                           double *stats_out, int32_t stats_cap);
+/* sdf_signed_distance / sdf_signed_distance_device with the learned radius
+ * (identical outputs). */
+sdf_status sdf_learned_signed_distance(sdf_ctx *ctx, const double *X, int64_t nnp,
+                                       const int64_t *IEN, int64_t nel,
+                                       const double aabb_min[3], double h,
+                                       const int64_t N[3], double *sdf_out);
+sdf_status sdf_learned_signed_distance_device(sdf_ctx *ctx, const double aabb_min[3],
+                                              double h, const int64_t N[3],
+                                              double *d_sdf_out);
+/* Counters of the last learned call (synchronises), up to 7: nodes whose
+ * radius held no triangle (settled by exact LBVH queries), evaluated (tile,
+ * triangle) pairs, pairs the plain loop evaluates, (node, triangle) pairs
+ * within the learned radius and within h (box/plane lower bound), delta, hc. */
+sdf_status sdf_learned_stats(sdf_ctx *ctx, double *out, int32_t cap, int32_t *n);
 
 #ifdef __cplusplus
 }

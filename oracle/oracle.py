@@ -60,6 +60,14 @@ def lib():
                                    C.c_int]
         L.orc_read_stl_binary.argtypes = [C.c_char_p, _dp, _i64p, _i64p, _i64p]
         L.orc_read_stl_binary.restype = C.c_int
+        L.orc_bf16.argtypes = [C.c_float]
+        L.orc_bf16.restype = C.c_float
+        L.orc_mlp_freqs.argtypes = [_fp]
+        L.orc_mlp_features.argtypes = [_fp, C.c_int, _dp]
+        L.orc_mlp_forward.argtypes = [_dp, _fp, C.c_int64, C.c_int, _dp]
+        L.orc_mlp_loss_grad.argtypes = [_dp, _fp, _fp, C.c_int64, C.c_int, _dp]
+        L.orc_mlp_loss_grad.restype = C.c_double
+        L.orc_mlp_adam.argtypes = [_fp, _fp, _fp, _fp, C.c_int64, C.c_int, C.c_float]
         _lib = L
     return _lib
 
@@ -199,3 +207,62 @@ def read_stl_binary(path):
     if rc != 0:
         raise IOError(f"orc_read_stl_binary failed: {rc}")
     return X, IEN
+
+
+# ---------------------------------------------------------------- MLP
+MLP_NPARAM = 49665
+MLP_EXACT, MLP_BF16 = 0, 1
+MLP_OFFSETS = {"W1": 0, "b1": 16384, "W2": 16512, "b2": 32896, "W3": 33024,
+               "b3": 49408, "w4": 49536, "b4": 49664}
+
+
+def bf16(x):
+    return float(lib().orc_bf16(float(x)))
+
+
+def mlp_freqs():
+    f = np.empty(16, np.float32)
+    lib().orc_mlp_freqs(f.ctypes.data_as(_fp))
+    return f
+
+
+def _u(u):
+    u = np.ascontiguousarray(np.asarray(u, np.float32).reshape(-1, 3))
+    return u, u.shape[0]
+
+
+def mlp_features(u, mode=MLP_BF16):
+    u = np.ascontiguousarray(np.asarray(u, np.float32).reshape(3))
+    phi = np.empty(128)
+    lib().orc_mlp_features(u.ctypes.data_as(_fp), int(mode), _d(phi))
+    return phi
+
+
+def mlp_forward(params, u, mode=MLP_BF16):
+    """Network output for points u (n x 3 normalised coordinates)."""
+    p = np.ascontiguousarray(params, np.float64)
+    u, n = _u(u)
+    y = np.empty(n)
+    lib().orc_mlp_forward(_d(p), u.ctypes.data_as(_fp), n, int(mode), _d(y))
+    return y
+
+
+def mlp_loss_grad(params, u, t, mode=MLP_BF16):
+    p = np.ascontiguousarray(params, np.float64)
+    u, n = _u(u)
+    t = np.ascontiguousarray(t, np.float32)
+    g = np.empty(MLP_NPARAM)
+    loss = lib().orc_mlp_loss_grad(_d(p), u.ctypes.data_as(_fp), t.ctypes.data_as(_fp), n,
+                                   int(mode), _d(g))
+    return loss, g
+
+
+def mlp_adam(params, m, v, grad, t, lr):
+    """One FP32 Adam step; returns new (params, m, v)."""
+    p = np.array(params, np.float32)
+    m = np.array(m, np.float32)
+    v = np.array(v, np.float32)
+    g = np.ascontiguousarray(grad, np.float32)
+    lib().orc_mlp_adam(p.ctypes.data_as(_fp), m.ctypes.data_as(_fp), v.ctypes.data_as(_fp),
+                       g.ctypes.data_as(_fp), p.size, int(t), float(lr))
+    return p, m, v

@@ -10,6 +10,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
 #include <exception>
 #include <mutex>
 #include <string>
@@ -18,6 +19,7 @@
 #include "sdf_internal.h"
 #include "sdf_lbvh.h"
 #include "sdf_mi355x.h"
+#include "sdf_mlp.h"
 
 // Host arithmetic that restates reference formulas (grid setup, ray
 // directions) must round every operation separately.
@@ -28,6 +30,9 @@ using sdf::Error;
 namespace {
 
 thread_local std::string g_last_error;
+
+// device counters of a learned-radius run (layout: launch_dist_chunks_learned)
+constexpr int kLearnedStats = 776;
 
 template <typename F>
 sdf_status guarded(F &&f) {
@@ -109,6 +114,15 @@ struct sdf_ctx {
   std::vector<std::string> ev_name;
   int nev = 0;
   int64_t counters[4] = {0, 0, 0, 0};
+  // learned search radius
+  sdf::MlpDev mlp;
+  bool mlp_alloc = false;
+  bool mlp_ready = false;     // parameters present (fitted or set)
+  sdf::MlpNorm mlp_norm{};    // point normalisation of the fit
+  double mlp_hc = 0.0;        // radius = (y + delta) * hc
+  double mlp_delta = 0.0;
+  unsigned long long *lstats = nullptr;  // counters of the last learned run
+  int64_t lstat_pairs = 0;
 
   void mark(const char *name) {
     if (!timing) return;
@@ -268,7 +282,7 @@ sdf::RayParams ray_params(int32_t min_rays, int32_t max_rays, double thr,
 
 // Lattice path: binning, fused kernel, adaptive finisher.
 void run_lattice(sdf_ctx *c, const sdf::GridP &g, bool do_dist, bool do_sign,
-                 const sdf::RayParams &rp, sdf::FusedOut out) {
+                 const sdf::RayParams &rp, sdf::FusedOut out, bool learned = false) {
   if (!c->mesh_ready) fail(SDF_ERR_ARG, "no mesh");
   int64_t npts = grid_points(g);
   sdf::Bins bins;
@@ -305,7 +319,31 @@ void run_lattice(sdf_ctx *c, const sdf::GridP &g, bool do_dist, bool do_sign,
     d2buf = c->ar.get<float>("d2buf", (size_t)bins.nchunks * sdf::kTilePts + 1);
     sdf::launch_fill_u32(c->stream, (uint32_t *)d2buf,
                          (int64_t)bins.nchunks * sdf::kTilePts, 0x7F800000u);
-    sdf::launch_dist_chunks(c->stream, g, c->tri, bins, d2buf);
+    if (learned) {
+      // network radius per tile point, then the radius-limited distance
+      // loop with its exact re-run of tiles whose radius held no triangle
+      float *r2 = c->ar.get<float>("l_r2", (size_t)bins.nchunks * sdf::kTilePts + 1);
+      uint64_t *list = c->ar.get<uint64_t>("l_list", (size_t)bins.nchunks * sdf::kTilePts + 1);
+      c->lstats = c->ar.get<unsigned long long>("l_stats", kLearnedStats);
+      sdf::launch_fill_u64(c->stream, (uint64_t *)c->lstats, kLearnedStats, 0ull);
+      c->mark("radius");
+      sdf::mlp_tile_radius(c->stream, c->mlp, c->mlp_norm, g, bins, (float)c->mlp_hc,
+                           (float)c->mlp_delta, r2);
+      c->mark("dist_learned");
+      sdf::launch_dist_chunks_learned(c->stream, g, c->tri, bins, r2, d2buf, list, c->lstats);
+      // nodes whose radius held no triangle: exact LBVH queries (the LBVH
+      // is built on a helper stream for the sign stage anyway)
+      if (!bvh_was_ready) bvh_async(c, 1);
+      SDF_HIP(hipStreamWaitEvent(c->stream, c->join[2], 0));
+      c->mark("fallback");
+      const float slack = 1e-6f * (float)(fabs(g.ox) + fabs(g.oy) + fabs(g.oz) +
+                                          (double)(g.nx + g.ny + g.nz) * g.h + 1e-30);
+      sdf::launch_learned_fallback(c->stream, g, c->tri, c->bvh, bins, list, c->lstats, slack,
+                                   d2buf, c->d_words);
+      c->lstat_pairs = bins.ntile_pairs;
+    } else {
+      sdf::launch_dist_chunks(c->stream, g, c->tri, bins, d2buf);
+    }
   }
   c->mark("bin_lines_wait");
   if (do_sign) SDF_HIP(hipStreamWaitEvent(c->stream, c->join[1], 0));
@@ -373,6 +411,125 @@ void begin_call(sdf_ctx *c) {
   use_device(c);
   c->nev = 0;
   reset_words(c);
+}
+
+
+// ---------------------------------------------------------------- learned radius
+void need_mlp(sdf_ctx *c) {
+  if (!c->mlp_alloc) {
+    sdf::mlp_alloc(c->ar, c->mlp);
+    c->mlp_alloc = true;
+  }
+}
+
+// SplitMix64: deterministic stream for initialisation and shuffles.
+struct SplitMix {
+  uint64_t s;
+  uint64_t next() {
+    uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  }
+  double uniform() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }
+};
+
+// He-uniform weights (fan-in 100 active features for layer 1, 128 after),
+// zero hidden biases, output bias 0.5.
+void mlp_init_params(uint64_t seed, std::vector<float> &p) {
+  p.assign(sdf::kMlpParams, 0.f);
+  SplitMix r{seed};
+  const int offs[3] = {sdf::kOffW1, sdf::kOffW2, sdf::kOffW3};
+  const double fan[3] = {100.0, 128.0, 128.0};
+  for (int L = 0; L < 3; L++) {
+    const double a = sqrt(6.0 / fan[L]);
+    for (int i = 0; i < 16384; i++) p[offs[L] + i] = (float)((2.0 * r.uniform() - 1.0) * a);
+  }
+  const double a4 = sqrt(3.0 / 128.0);
+  for (int i = 0; i < 128; i++) p[sdf::kOffW4 + i] = (float)((2.0 * r.uniform() - 1.0) * a4);
+  p[sdf::kOffB4] = 0.5f;
+}
+
+void mlp_upload(sdf_ctx *c, const float *params) {
+  need_mlp(c);
+  SDF_HIP(hipMemcpyAsync(c->mlp.params, params, sizeof(float) * sdf::kMlpParams,
+                         hipMemcpyHostToDevice, c->stream));
+  sdf::launch_fill_u32(c->stream, (uint32_t *)c->mlp.m, sdf::kMlpParams, 0u);
+  sdf::launch_fill_u32(c->stream, (uint32_t *)c->mlp.v, sdf::kMlpParams, 0u);
+  c->mlp.step = 0;
+  sdf::mlp_pack(c->stream, c->mlp);
+  c->mlp_ready = true;
+}
+
+// Batch order for `steps` steps of `batch` samples out of n: concatenated
+// Fisher-Yates shuffles (seed != 0) or the samples in order (seed == 0).
+std::vector<uint32_t> batch_order(int64_t n, int64_t steps, int64_t batch, uint64_t seed) {
+  std::vector<uint32_t> perm((size_t)n), out((size_t)(steps * batch));
+  for (int64_t i = 0; i < n; i++) perm[i] = (uint32_t)i;
+  SplitMix r{seed ^ 0xA5A5A5A5DEADBEEFull};
+  int64_t pos = n;  // exhausted: reshuffle
+  for (int64_t k = 0; k < steps * batch; k++) {
+    if (pos == n) {
+      if (seed != 0)
+        for (int64_t i = n - 1; i > 0; i--) {
+          const int64_t j = (int64_t)(r.next() % (uint64_t)(i + 1));
+          std::swap(perm[i], perm[j]);
+        }
+      pos = 0;
+    }
+    out[k] = perm[pos++];
+  }
+  return out;
+}
+
+// Adam over `steps` batches with a cosine learning-rate decay to lr/20.
+// Returns the mean loss of the last min(20, steps) steps.
+double mlp_train(sdf_ctx *c, const float4 *d_samples, int64_t n, int32_t steps, int32_t batch,
+                 float lr, uint64_t seed, float *loss_hist_host) {
+  if (steps < 1 || batch < 1) fail(SDF_ERR_ARG, "steps and batch must be >= 1");
+  if (lr <= 0.f) fail(SDF_ERR_ARG, "lr must be > 0");
+  if ((int64_t)batch > n) batch = (int32_t)n;
+  std::vector<uint32_t> order = batch_order(n, steps, batch, seed);
+  uint32_t *d_idx = c->ar.get<uint32_t>("mlp_idx", order.size());
+  SDF_HIP(hipMemcpyAsync(d_idx, order.data(), order.size() * sizeof(uint32_t),
+                         hipMemcpyHostToDevice, c->stream));
+  float *d_loss = c->ar.get<float>("mlp_loss", steps);
+  for (int32_t t = 0; t < steps; t++) {
+    const double frac = steps > 1 ? (double)t / (double)(steps - 1) : 0.0;
+    const float lrt = (float)(lr * (0.05 + 0.95 * 0.5 * (1.0 + cos(3.14159265358979323846 * frac))));
+    sdf::mlp_step(c->stream, c->ar, c->mlp, d_samples, d_idx + (int64_t)t * batch, batch,
+                  nullptr, d_loss + t, lrt);
+  }
+  std::vector<float> hist((size_t)steps);
+  SDF_HIP(hipMemcpyAsync(hist.data(), d_loss, sizeof(float) * steps, hipMemcpyDeviceToHost,
+                         c->stream));
+  SDF_HIP(hipStreamSynchronize(c->stream));
+  if (loss_hist_host) memcpy(loss_hist_host, hist.data(), sizeof(float) * steps);
+  const int k = steps < 20 ? steps : 20;
+  double m = 0.0;
+  for (int i = steps - k; i < steps; i++) m += hist[i];
+  return m / k;
+}
+
+float4 *upload_samples(sdf_ctx *c, const float *u, const float *t, int64_t n) {
+  std::vector<float> hs((size_t)(4 * n));
+  for (int64_t i = 0; i < n; i++) {
+    hs[4 * i] = u[3 * i];
+    hs[4 * i + 1] = u[3 * i + 1];
+    hs[4 * i + 2] = u[3 * i + 2];
+    hs[4 * i + 3] = t ? t[i] : 0.f;
+  }
+  float4 *d = c->ar.get<float4>("mlp_user_samples", n + 1);
+  SDF_HIP(hipMemcpyAsync(d, hs.data(), hs.size() * sizeof(float), hipMemcpyHostToDevice,
+                         c->stream));
+  SDF_HIP(hipStreamSynchronize(c->stream));
+  return d;
+}
+
+double now_ms() {
+  return std::chrono::duration<double, std::milli>(
+             std::chrono::steady_clock::now().time_since_epoch())
+      .count();
 }
 
 }  // namespace
@@ -830,6 +987,238 @@ sdf_status sdf_signed_distance_device(sdf_ctx *c, const double aabb_min[3],
     sdf::FusedOut out;
     out.sdf = d_sdf_out;
     run_lattice(c, g, true, true, ray_params(6, 30, 0.6, 1), out);
+  });
+}
+
+
+// ---------------------------------------------------------------- learned radius
+sdf_status sdf_mlp_set_params(sdf_ctx *c, const float *params) {
+  return guarded([&] {
+    if (!c || !params) fail(SDF_ERR_ARG, "NULL argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    use_device(c);
+    mlp_upload(c, params);
+    SDF_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+sdf_status sdf_mlp_get_params(sdf_ctx *c, float *params_out) {
+  return guarded([&] {
+    if (!c || !params_out) fail(SDF_ERR_ARG, "NULL argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (!c->mlp_ready) fail(SDF_ERR_ARG, "no network parameters (fit or set them first)");
+    use_device(c);
+    SDF_HIP(hipMemcpyAsync(params_out, c->mlp.params, sizeof(float) * sdf::kMlpParams,
+                           hipMemcpyDeviceToHost, c->stream));
+    SDF_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+sdf_status sdf_mlp_forward(sdf_ctx *c, const float *u, int64_t n, float *y_out) {
+  return guarded([&] {
+    if (!c) fail(SDF_ERR_ARG, "ctx is NULL");
+    if (n < 0) fail(SDF_ERR_ARG, "n < 0");
+    if (n > 0 && (!u || !y_out)) fail(SDF_ERR_ARG, "NULL argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (!c->mlp_ready) fail(SDF_ERR_ARG, "no network parameters (fit or set them first)");
+    use_device(c);
+    if (n == 0) return;
+    float *du = c->ar.get<float>("mlp_fwd_u", 3 * n);
+    float *dy = c->ar.get<float>("mlp_fwd_y", n);
+    SDF_HIP(hipMemcpyAsync(du, u, sizeof(float) * 3 * n, hipMemcpyHostToDevice, c->stream));
+    sdf::mlp_forward(c->stream, c->mlp, du, 3, n, dy);
+    SDF_HIP(hipMemcpyAsync(y_out, dy, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+    SDF_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+sdf_status sdf_mlp_loss_grad(sdf_ctx *c, const float *u, const float *t, int64_t n,
+                             float *grad_out, double *loss_out) {
+  return guarded([&] {
+    if (!c) fail(SDF_ERR_ARG, "ctx is NULL");
+    if (n < 1 || !u || !t) fail(SDF_ERR_ARG, "need n >= 1 samples");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (!c->mlp_ready) fail(SDF_ERR_ARG, "no network parameters (fit or set them first)");
+    use_device(c);
+    float4 *ds = upload_samples(c, u, t, n);
+    float *dg = c->ar.get<float>("mlp_grad", sdf::kMlpParams);
+    float *dl = c->ar.get<float>("mlp_loss1", 1);
+    sdf::mlp_step(c->stream, c->ar, c->mlp, ds, nullptr, n, dg, dl, 0.f);
+    float l = 0.f;
+    if (grad_out)
+      SDF_HIP(hipMemcpyAsync(grad_out, dg, sizeof(float) * sdf::kMlpParams,
+                             hipMemcpyDeviceToHost, c->stream));
+    SDF_HIP(hipMemcpyAsync(&l, dl, sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    SDF_HIP(hipStreamSynchronize(c->stream));
+    if (loss_out) *loss_out = l;
+  });
+}
+
+sdf_status sdf_mlp_train(sdf_ctx *c, const float *u, const float *t, int64_t n, int32_t steps,
+                         int32_t batch, float lr, uint64_t seed, float *loss_hist) {
+  return guarded([&] {
+    if (!c) fail(SDF_ERR_ARG, "ctx is NULL");
+    if (n < 1 || !u || !t) fail(SDF_ERR_ARG, "need n >= 1 samples");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (!c->mlp_ready) fail(SDF_ERR_ARG, "no network parameters (fit or set them first)");
+    use_device(c);
+    float4 *ds = upload_samples(c, u, t, n);
+    mlp_train(c, ds, n, steps, batch, lr, seed, loss_hist);
+  });
+}
+
+sdf_status sdf_learned_fit(sdf_ctx *c, const double aabb_min[3], double h, const int64_t N[3],
+                           int32_t n_coarse, int32_t steps, int32_t batch, float lr,
+                           uint64_t seed, double quantile, double *stats_out,
+                           int32_t stats_cap) {
+  return guarded([&] {
+    if (!c) fail(SDF_ERR_ARG, "ctx is NULL");
+    if (n_coarse < 4) fail(SDF_ERR_ARG, "n_coarse must be >= 4");
+    if (quantile <= 0.0) quantile = 0.99;
+    if (quantile > 1.0) fail(SDF_ERR_ARG, "quantile must be in (0, 1]");
+    std::lock_guard<std::mutex> lk(c->mu);
+    sdf::GridP g = make_grid(aabb_min, h, N);
+    begin_call(c);
+    if (!c->mesh_ready) fail(SDF_ERR_ARG, "no mesh (set one first)");
+    // coarse lattice over the same box, n_coarse points on the longest axis
+    const int64_t maxN = std::max(N[0], std::max(N[1], N[2]));
+    if (maxN < 1) fail(SDF_ERR_ARG, "empty grid");
+    const double hc = h * (double)maxN / (double)(n_coarse - 1);
+    int64_t Nc[3];
+    for (int a = 0; a < 3; a++) {
+      Nc[a] = (int64_t)ceil((double)N[a] * h / hc - 1e-9);
+      if (Nc[a] < 1) Nc[a] = 1;
+    }
+    sdf::GridP gc = make_grid(aabb_min, hc, Nc);
+    const double t0 = now_ms();
+    double *dc = c->ar.get<double>("fit_dc", grid_points(gc) + 1);
+    sdf::FusedOut o;
+    o.dist = dc;
+    run_lattice(c, gc, true, false, ray_params(6, 30, 0.6, 1), o);
+    SDF_HIP(hipStreamSynchronize(c->stream));
+    const double t1 = now_ms();
+    sdf::MlpNorm nm;
+    nm.c[0] = g.ox + 0.5 * h * (double)N[0];
+    nm.c[1] = g.oy + 0.5 * h * (double)N[1];
+    nm.c[2] = g.oz + 0.5 * h * (double)N[2];
+    nm.s = 2.0 / (h * (double)maxN);
+    float4 *smp = nullptr;
+    int64_t nband = 0;
+    const int64_t ns = sdf::mlp_fit_samples(c->stream, c->ar, gc, nm, dc, hc, &smp, &nband);
+    if (ns < 1) fail(SDF_ERR_ARG, "no training samples");
+    std::vector<float> p;
+    mlp_init_params(seed, p);
+    mlp_upload(c, p.data());
+    const double loss = mlp_train(c, smp, ns, steps, batch, lr, seed, nullptr);
+    // calibration: delta = `quantile` of (target - output) over the band
+    float *dy = c->ar.get<float>("fit_y", ns);
+    sdf::mlp_forward(c->stream, c->mlp, (const float *)smp, 4, ns, dy);
+    std::vector<float> y((size_t)ns), hs((size_t)(4 * ns));
+    SDF_HIP(hipMemcpyAsync(y.data(), dy, sizeof(float) * ns, hipMemcpyDeviceToHost, c->stream));
+    SDF_HIP(hipMemcpyAsync(hs.data(), smp, sizeof(float) * 4 * ns, hipMemcpyDeviceToHost,
+                           c->stream));
+    SDF_HIP(hipStreamSynchronize(c->stream));
+    const double t2 = now_ms();
+    std::vector<double> res;
+    double mae = 0.0;
+    for (int64_t i = 0; i < ns; i++)
+      if (hs[4 * i + 3] < 1.f) {
+        const double e = (double)hs[4 * i + 3] - (double)y[i];
+        res.push_back(e);
+        mae += fabs(e);
+      }
+    double delta = 0.0;
+    if (!res.empty()) {
+      mae /= (double)res.size();
+      size_t k = (size_t)(quantile * (double)(res.size() - 1));
+      std::nth_element(res.begin(), res.begin() + k, res.end());
+      delta = res[k];
+    }
+    c->mlp_norm = nm;
+    c->mlp_hc = hc;
+    c->mlp_delta = delta;
+    const double st[9] = {loss, mae, delta, (double)ns, (double)nband, t2 - t1, t1 - t0, hc,
+                          (double)steps};
+    for (int i = 0; i < 9 && i < stats_cap && stats_out; i++) stats_out[i] = st[i];
+  });
+}
+
+sdf_status sdf_learned_signed_distance_device(sdf_ctx *c, const double aabb_min[3], double h,
+                                              const int64_t N[3], double *d_sdf_out) {
+  return guarded([&] {
+    if (!c) fail(SDF_ERR_ARG, "ctx is NULL");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (!c->mlp_ready) fail(SDF_ERR_ARG, "no network parameters (fit or set them first)");
+    sdf::GridP g = make_grid(aabb_min, h, N);
+    if (grid_points(g) > 0 && !d_sdf_out) fail(SDF_ERR_ARG, "d_sdf_out is NULL");
+    begin_call(c);
+    sdf::FusedOut out;
+    out.sdf = d_sdf_out;
+    run_lattice(c, g, true, true, ray_params(6, 30, 0.6, 1), out, true);
+  });
+}
+
+sdf_status sdf_learned_signed_distance(sdf_ctx *c, const double *X, int64_t nnp,
+                                       const int64_t *IEN, int64_t nel,
+                                       const double aabb_min[3], double h, const int64_t N[3],
+                                       double *sdf_out) {
+  return guarded([&] {
+    if (!c) fail(SDF_ERR_ARG, "ctx is NULL");
+    std::lock_guard<std::mutex> lk(c->mu);
+    if (!c->mlp_ready) fail(SDF_ERR_ARG, "no network parameters (fit or set them first)");
+    sdf::GridP g = make_grid(aabb_min, h, N);
+    int64_t npts = grid_points(g);
+    if (npts > 0 && !sdf_out) fail(SDF_ERR_ARG, "sdf_out is NULL");
+    begin_call(c);
+    set_mesh_host(c, X, nnp, IEN, nel);
+    double *d_out = c->ar.get<double>("out_sdf", npts + 1);
+    sdf::FusedOut out;
+    out.sdf = d_out;
+    run_lattice(c, g, true, true, ray_params(6, 30, 0.6, 1), out, true);
+    if (npts > 0)
+      SDF_HIP(hipMemcpyAsync(sdf_out, d_out, (size_t)npts * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
+    SDF_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+sdf_status sdf_learned_stats(sdf_ctx *c, double *out, int32_t cap, int32_t *n) {
+  return guarded([&] {
+    if (!c) fail(SDF_ERR_ARG, "ctx is NULL");
+    std::lock_guard<std::mutex> lk(c->mu);
+    use_device(c);
+    double st[8] = {0, 0, (double)c->lstat_pairs, 0, 0, c->mlp_delta, c->mlp_hc, 0};
+    if (c->lstats) {
+      std::vector<unsigned long long> h(kLearnedStats);
+      SDF_HIP(hipMemcpyAsync(h.data(), c->lstats, sizeof(unsigned long long) * kLearnedStats,
+                             hipMemcpyDeviceToHost, c->stream));
+      SDF_HIP(hipStreamSynchronize(c->stream));
+      unsigned long long v = 0, lr = 0, lh = 0;
+      for (int i = 0; i < 256; i++) {
+        v += h[8 + i];
+        lr += h[264 + i];
+        lh += h[520 + i];
+      }
+      st[0] = (double)h[0];
+      st[1] = (double)v;
+      st[3] = (double)lr;
+      st[4] = (double)lh;
+    }
+    int w = 0;
+    for (; w < 7 && w < cap && out; w++) out[w] = st[w];
+    if (n) *n = w;
+  });
+}
+
+sdf_status sdf_mesh_set(sdf_ctx *c, const double *X, int64_t nnp, const int64_t *IEN,
+                        int64_t nel) {
+  return guarded([&] {
+    if (!c) fail(SDF_ERR_ARG, "ctx is NULL");
+    std::lock_guard<std::mutex> lk(c->mu);
+    begin_call(c);
+    set_mesh_host(c, X, nnp, IEN, nel);
+    SDF_HIP(hipStreamSynchronize(c->stream));
+    check_err_flags(c);
   });
 }
 

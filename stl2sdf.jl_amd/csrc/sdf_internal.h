@@ -157,7 +157,22 @@ struct FusedOut {
 // Distance work items -> squared distances per point of non-empty tiles
 // (d2buf: 64 floats per work item slot, indexed by the tile's first item).
 void launch_dist_chunks(hipStream_t s, const GridP &g, const Tri64 *tri,
-                        const Bins &b, float *d2buf);
+                        const Bins &b, float *d2buf, const uint32_t *only = nullptr);
+// Distance work items with a learned per-point squared radius r2buf (slots
+// as d2buf); nodes whose radius held no triangle are appended to `list`.
+// stats: [0] fallback nodes, [8..263] evaluated (tile, triangle) pairs,
+// [264..519] (node, triangle) pairs within the learned radius (lower bound),
+// [520..775] the same within h.
+void launch_dist_chunks_learned(hipStream_t s, const GridP &g, const Tri64 *tri,
+                                const Bins &b, const float *r2buf, float *d2buf,
+                                uint64_t *list, unsigned long long *stats);
+struct Bvh;
+// Exact distances of the listed nodes (count in stats[0]) over the LBVH with
+// the tile loop's arithmetic; overwrites their d2buf slots.
+void launch_learned_fallback(hipStream_t s, const GridP &g, const Tri64 *tri, const Bvh &bvh,
+                             const Bins &b, const uint64_t *list,
+                             const unsigned long long *stats, float slack, float *d2buf,
+                             uint32_t *err);
 // Per tile: distance from d2buf (or h), six axis rays, combine, outputs.
 // Points with a vote inside its rounding margin go to pend0 (restart from
 // ray 1 on the LBVH); non-unanimous votes go to pend (continue at ray 7).

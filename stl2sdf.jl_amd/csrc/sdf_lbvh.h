@@ -146,4 +146,65 @@ __device__ inline double nearest_bvh(const Bvh &bvh, const Tri64 *__restrict__ t
   return dd < r ? dd : r;
 }
 
+
+// Smallest leaf value over the triangles whose boxes lie within
+// min(sqrt(best), rcap) + slack of p (absolute FP32 coordinates); leaf(t)
+// returns a squared distance computed by the caller's own arithmetic.
+// Returns +inf when no leaf was evaluated.
+template <typename Leaf>
+__device__ inline float nearest_bvh_with(const Bvh &bvh, const float *pf, float rcap,
+                                         float slack, Leaf leaf, bool *overflow) {
+  float best2 = __builtin_huge_valf();
+  *overflow = false;
+  if (bvh.n == 0) return best2;
+  if (bvh.n == 1) return leaf(bvh.leaf_tri[0]);
+  int32_t stack[kStackDepth];
+  int sp = 0;
+  int32_t node = 0;
+  while (true) {
+    const BvhNode nd = bvh.nodes[node];
+    const float lim = fminf(sqrtf(best2), rcap) + slack;
+    const float lim2 = lim * lim;
+    float d0 = box_dist2(pf, nd.box[0]);
+    float d1 = box_dist2(pf, nd.box[1]);
+    int32_t c0 = nd.child[0], c1 = nd.child[1];
+    if (d1 < d0) {
+      const float tf = d0;
+      d0 = d1;
+      d1 = tf;
+      const int32_t tc = c0;
+      c0 = c1;
+      c1 = tc;
+    }
+    int32_t next = -0x7FFFFFFF;
+    if (d0 <= lim2) {
+      if (c0 < 0)
+        best2 = fminf(best2, leaf(bvh.leaf_tri[-(c0 + 1)]));
+      else
+        next = c0;
+    }
+    if (d1 <= lim2) {
+      if (c1 < 0) {
+        best2 = fminf(best2, leaf(bvh.leaf_tri[-(c1 + 1)]));
+      } else if (next == -0x7FFFFFFF) {
+        next = c1;
+      } else {
+        if (sp >= kStackDepth) {
+          *overflow = true;
+          break;
+        }
+        stack[sp++] = c1;
+      }
+    }
+    if (next != -0x7FFFFFFF) {
+      node = next;
+    } else if (sp > 0) {
+      node = stack[--sp];
+    } else {
+      break;
+    }
+  }
+  return best2;
+}
+
 }  // namespace sdf

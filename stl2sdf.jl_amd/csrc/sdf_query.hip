@@ -97,13 +97,14 @@ __global__ __launch_bounds__(kTilePts) void dist_chunks_kernel(
     GridP g, TileDims td, const Tri64 *__restrict__ tri,
     const uint64_t *__restrict__ tile_off, const uint32_t *__restrict__ tile_tris,
     const uint64_t *__restrict__ chunk_off, const uint32_t *__restrict__ chunk_tile,
-    int64_t nchunks, float *__restrict__ d2buf) {
+    int64_t nchunks, const uint32_t *__restrict__ only, float *__restrict__ d2buf) {
   __shared__ float s_pre[kTriPreFloats][kTilePts];
   const int64_t item = xcd_remap(blockIdx.x, gridDim.x);
   if (item >= nchunks) return;
   const int lane = threadIdx.x;
   const int64_t tile = chunk_tile[item];
   const uint64_t first = chunk_off[tile];
+  if (only && only[first] == 0) return;  // re-run restricted to flagged tiles
   const int64_t q = item - (int64_t)first;
   const int32_t tx = (int32_t)(tile % td.tx);
   const int32_t ty = (int32_t)((tile / td.tx) % td.ty);
@@ -144,6 +145,166 @@ __global__ __launch_bounds__(kTilePts) void dist_chunks_kernel(
     *slot = best;
   else
     atomicMin((unsigned int *)slot, __float_as_uint(best));
+}
+
+// Learned-radius variant of dist_chunks_kernel.  Lane p carries a squared
+// search radius R2(p) <= h^2 proposed by the network; a triangle is
+// evaluated only when some lane's lower bound (the larger of the squared
+// distances to the triangle's box and to its plane) is within R2 inflated by
+// 2^-10 (which covers the rounding of the bound against the exact
+// evaluation).  A lane that ends with best >= R2 < h^2 has no triangle within
+// its radius and is settled afterwards by an exact re-run of its tile; every
+// other lane's best is the same value the full loop gives.  stats[b & 255]
+// collects evaluated (tile, triangle) pairs per block slot.
+__global__ __launch_bounds__(kTilePts) void dist_chunks_learned_kernel(
+    GridP g, TileDims td, const Tri64 *__restrict__ tri,
+    const uint64_t *__restrict__ tile_off, const uint32_t *__restrict__ tile_tris,
+    const uint64_t *__restrict__ chunk_off, const uint32_t *__restrict__ chunk_tile,
+    int64_t nchunks, const float *__restrict__ r2buf, float *__restrict__ d2buf,
+    unsigned long long *__restrict__ stats) {
+  __shared__ float s_pre[kTriPreFloats][kTilePts];
+  __shared__ float s_box[6][kTilePts];
+  const int64_t item = xcd_remap(blockIdx.x, gridDim.x);
+  if (item >= nchunks) return;
+  const int lane = threadIdx.x;
+  const int64_t tile = chunk_tile[item];
+  const uint64_t first = chunk_off[tile];
+  const int64_t q = item - (int64_t)first;
+  const int32_t tx = (int32_t)(tile % td.tx);
+  const int32_t ty = (int32_t)((tile / td.tx) % td.ty);
+  const int32_t tz = (int32_t)(tile / ((int64_t)td.tx * td.ty));
+  const int32_t i0 = tx * kTileX, j0 = ty * kTileY, k0 = tz * kTileZ;
+  const Vec3d T0 = lattice_point(g, i0, j0, k0);
+  const uint64_t b = tile_off[tile] + (uint64_t)q * kTilePts;
+  const uint64_t e = min(b + (uint64_t)kTilePts, tile_off[tile + 1]);
+  const int n = (int)(e - b);
+  if (lane < n) {
+    const Tri64 t = tri[tile_tris[b + lane]];
+    float v[9];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      v[3 * k + 0] = (float)(t.v[3 * k + 0] - T0.x);
+      v[3 * k + 1] = (float)(t.v[3 * k + 1] - T0.y);
+      v[3 * k + 2] = (float)(t.v[3 * k + 2] - T0.z);
+    }
+    float rec[kTriPreFloats];
+    tri_pre_make(v, v + 3, v + 6, rec);
+#pragma unroll
+    for (int f = 0; f < kTriPreFloats; f++) s_pre[f][lane] = rec[f];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      s_box[a][lane] = fminf(fminf(v[a], v[3 + a]), v[6 + a]);
+      s_box[3 + a][lane] = fmaxf(fmaxf(v[a], v[3 + a]), v[6 + a]);
+    }
+  }
+  __syncthreads();
+  const int32_t i = i0 + (lane & (kTileX - 1));
+  const int32_t j = j0 + ((lane / kTileX) & (kTileY - 1));
+  const int32_t k = k0 + lane / (kTileX * kTileY);
+  const Vec3d p = lattice_point(g, i, j, k);
+  const float px = (float)(p.x - T0.x), py = (float)(p.y - T0.y), pz = (float)(p.z - T0.z);
+  const float R2 = r2buf[first * kTilePts + lane] * (1.0f + 0x1p-10f);
+  const float hf = (float)g.h;
+  const float H2 = hf * hf * (1.0f + 0x1p-10f);
+  float best = __builtin_huge_valf();
+  int visited = 0, lanes_r = 0, lanes_h = 0;
+  for (int c = 0; c < n; c++) {
+    const float ex = fmaxf(fmaxf(s_box[0][c] - px, px - s_box[3][c]), 0.f);
+    const float ey = fmaxf(fmaxf(s_box[1][c] - py, py - s_box[4][c]), 0.f);
+    const float ez = fmaxf(fmaxf(s_box[2][c] - pz, pz - s_box[5][c]), 0.f);
+    const float box2 = ex * ex + ey * ey + ez * ez;
+    const float pn = (px - s_pre[0][c]) * s_pre[21][c] + (py - s_pre[1][c]) * s_pre[22][c] +
+                     (pz - s_pre[2][c]) * s_pre[23][c];
+    const float lb2 = fmaxf(box2, pn * pn * s_pre[24][c]);
+    const unsigned long long need = __ballot(lb2 <= R2);
+    lanes_r += __popcll(need);
+    lanes_h += __popcll(__ballot(lb2 <= H2));
+    if (need) {
+      const float d2 = tri_pre_dist2(px, py, pz, [&](int f) { return s_pre[f][c]; });
+      best = fminf(best, d2);
+      visited++;
+    }
+  }
+  float *slot = d2buf + first * kTilePts + lane;
+  if (chunk_off[tile + 1] - first == 1)
+    *slot = best;
+  else
+    atomicMin((unsigned int *)slot, __float_as_uint(best));
+  if (lane == 0 && stats) {
+    atomicAdd(stats + 8 + (blockIdx.x & 255), (unsigned long long)visited);
+    atomicAdd(stats + 264 + (blockIdx.x & 255), (unsigned long long)lanes_r);
+    atomicAdd(stats + 520 + (blockIdx.x & 255), (unsigned long long)lanes_h);
+  }
+}
+
+// Appends every lane whose learned radius R2 < h^2 enclosed no triangle to
+// the fallback list (packed first_item * 64 + lane); counts them in stats[0].
+__global__ __launch_bounds__(256) void learned_check_kernel(
+    const uint64_t *__restrict__ chunk_off, const uint32_t *__restrict__ chunk_tile,
+    int64_t nchunks, const float *__restrict__ r2buf, const float *__restrict__ d2buf,
+    float h2, uint64_t *__restrict__ list, unsigned long long *__restrict__ stats) {
+  const int64_t item = blockIdx.x * 4LL + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (item >= nchunks) return;
+  const uint64_t first = chunk_off[chunk_tile[item]];
+  if ((uint64_t)item != first) return;
+  const float R2 = r2buf[first * kTilePts + lane];
+  const float d2 = d2buf[first * kTilePts + lane];
+  const bool fb = d2 >= R2 && R2 < h2;
+  const unsigned long long m = __ballot(fb);
+  if (!m) return;
+  unsigned long long base = 0;
+  if (lane == 0) base = atomicAdd(stats + 0, (unsigned long long)__popcll(m));
+  base = __shfl(base, 0);
+  if (fb) {
+    const unsigned long long below = m & ((1ull << lane) - 1ull);
+    list[base + __popcll(below)] = first * kTilePts + (uint64_t)lane;
+  }
+}
+
+// Exact distance of each fallback node over the LBVH with the tile kernel's
+// arithmetic (records and point relative to the tile origin), so the value
+// equals the full tile loop's: the nearest triangle within h lies in both
+// candidate sets and is evaluated the same way.  Overwrites the node's slot.
+__global__ __launch_bounds__(256) void learned_fallback_kernel(
+    GridP g, TileDims td, const Tri64 *__restrict__ tri, Bvh bvh,
+    const uint32_t *__restrict__ chunk_tile, const uint64_t *__restrict__ list,
+    const unsigned long long *__restrict__ count, float slack, float *__restrict__ d2buf,
+    uint32_t *__restrict__ err) {
+  const int64_t n = (int64_t)*count;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const uint64_t code = list[i];
+    const uint64_t first = code / kTilePts;
+    const int lane = (int)(code % kTilePts);
+    const int64_t tile = chunk_tile[first];
+    const int32_t tx = (int32_t)(tile % td.tx);
+    const int32_t ty = (int32_t)((tile / td.tx) % td.ty);
+    const int32_t tz = (int32_t)(tile / ((int64_t)td.tx * td.ty));
+    const int32_t i0 = tx * kTileX, j0 = ty * kTileY, k0 = tz * kTileZ;
+    const Vec3d T0 = lattice_point(g, i0, j0, k0);
+    const Vec3d p = lattice_point(g, i0 + (lane & (kTileX - 1)), j0 + ((lane / kTileX) & (kTileY - 1)),
+                                  k0 + lane / (kTileX * kTileY));
+    const float px = (float)(p.x - T0.x), py = (float)(p.y - T0.y), pz = (float)(p.z - T0.z);
+    const float pa[3] = {(float)p.x, (float)p.y, (float)p.z};
+    auto leaf = [&](uint32_t t) {
+      const Tri64 q = tri[t];
+      float v[9];
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        v[3 * k + 0] = (float)(q.v[3 * k + 0] - T0.x);
+        v[3 * k + 1] = (float)(q.v[3 * k + 1] - T0.y);
+        v[3 * k + 2] = (float)(q.v[3 * k + 2] - T0.z);
+      }
+      float rec[kTriPreFloats];
+      tri_pre_make(v, v + 3, v + 6, rec);
+      return tri_pre_dist2(px, py, pz, [&](int f) { return rec[f]; });
+    };
+    bool ovf = false;
+    const float best = nearest_bvh_with(bvh, pa, (float)g.h * (1.0f + 0x1p-10f), slack, leaf, &ovf);
+    if (ovf) atomicOr(err, kErrStackOverflow);
+    d2buf[code] = best;
+  }
 }
 
 // Sign / combine kernel.  One wavefront covers 8 x 8 x 4 lattice points:
@@ -666,12 +827,37 @@ __global__ void init_pending_all_kernel(Pending *pend, int64_t n) {
 }  // namespace
 
 void launch_dist_chunks(hipStream_t s, const GridP &g, const Tri64 *tri,
-                        const Bins &b, float *d2buf) {
+                        const Bins &b, float *d2buf, const uint32_t *only) {
   if (b.nchunks <= 0) return;
   TileDims td = tile_dims(g);
   hipLaunchKernelGGL(dist_chunks_kernel, dim3((unsigned)b.nchunks), dim3(kTilePts), 0,
                      s, g, td, tri, b.tile_off, b.tile_tris, b.chunk_off, b.chunk_tile,
-                     b.nchunks, d2buf);
+                     b.nchunks, only, d2buf);
+  SDF_HIP(hipGetLastError());
+}
+
+void launch_dist_chunks_learned(hipStream_t s, const GridP &g, const Tri64 *tri,
+                                const Bins &b, const float *r2buf, float *d2buf,
+                                uint64_t *list, unsigned long long *stats) {
+  if (b.nchunks <= 0) return;
+  TileDims td = tile_dims(g);
+  hipLaunchKernelGGL(dist_chunks_learned_kernel, dim3((unsigned)b.nchunks), dim3(kTilePts), 0,
+                     s, g, td, tri, b.tile_off, b.tile_tris, b.chunk_off, b.chunk_tile,
+                     b.nchunks, r2buf, d2buf, stats);
+  const float hf = (float)g.h;
+  hipLaunchKernelGGL(learned_check_kernel, dim3((unsigned)((b.nchunks + 3) / 4)), dim3(256),
+                     0, s, b.chunk_off, b.chunk_tile, b.nchunks, r2buf,
+                     (const float *)d2buf, hf * hf, list, stats);
+  SDF_HIP(hipGetLastError());
+}
+
+void launch_learned_fallback(hipStream_t s, const GridP &g, const Tri64 *tri, const Bvh &bvh,
+                             const Bins &b, const uint64_t *list,
+                             const unsigned long long *stats, float slack, float *d2buf,
+                             uint32_t *err) {
+  if (b.nchunks <= 0) return;
+  hipLaunchKernelGGL(learned_fallback_kernel, dim3(256), dim3(256), 0, s, g, tile_dims(g), tri,
+                     bvh, b.chunk_tile, list, stats, slack, d2buf, err);
   SDF_HIP(hipGetLastError());
 }
 

@@ -44,6 +44,20 @@ SIGNATURES = {
     "sdf_mesh_set_device": (C.c_int32, [_vp, _vp, C.c_int64, _vp, C.c_int64]),
     "sdf_signed_distance_device": (C.c_int32, [_vp, _dp, C.c_double, _i64p, _vp]),
     "sdf_ray_directions": (C.c_int32, [_dp]),
+    "sdf_mesh_set": (C.c_int32, [_vp, _dp, C.c_int64, _i64p, C.c_int64]),
+    "sdf_mlp_set_params": (C.c_int32, [_vp, _fp]),
+    "sdf_mlp_get_params": (C.c_int32, [_vp, _fp]),
+    "sdf_mlp_forward": (C.c_int32, [_vp, _fp, C.c_int64, _fp]),
+    "sdf_mlp_loss_grad": (C.c_int32, [_vp, _fp, _fp, C.c_int64, _fp, _dp]),
+    "sdf_mlp_train": (C.c_int32, [_vp, _fp, _fp, C.c_int64, C.c_int32, C.c_int32, C.c_float,
+                                  C.c_uint64, _fp]),
+    "sdf_learned_fit": (C.c_int32, [_vp, _dp, C.c_double, _i64p, C.c_int32, C.c_int32,
+                                    C.c_int32, C.c_float, C.c_uint64, C.c_double, _dp,
+                                    C.c_int32]),
+    "sdf_learned_signed_distance": (C.c_int32, [_vp, _dp, C.c_int64, _i64p, C.c_int64, _dp,
+                                                C.c_double, _i64p, _dp]),
+    "sdf_learned_signed_distance_device": (C.c_int32, [_vp, _dp, C.c_double, _i64p, _vp]),
+    "sdf_learned_stats": (C.c_int32, [_vp, _dp, C.c_int32, _i32p]),
     "sdf_remove_artifacts": (C.c_int32, [_vp, _dp, _i64p, C.c_double, C.c_double, _i64p]),
     "sdf_remove_artifacts_device": (C.c_int32, [_vp, _vp, _i64p, C.c_double, C.c_double,
                                                 _i64p]),
