@@ -106,8 +106,18 @@ struct sdf_ctx {
   bool mesh_ready = false;
   sdf::Bvh bvh;
   bool bvh_ready = false;
-  // small device words: [0] error flags, [1] pending count
+  // small device words: [0] error flags, [1] pending count, [2] pending
+  // count of ambiguous axis decisions, [4..19] resolve-phase counters
   uint32_t *d_words = nullptr;
+  // Capacities of the lattice path's lists, from earlier calls (grow-only;
+  // < 0: size exactly, reading the counts back).  With all set, a call
+  // issues no host waits until its final status read.
+  struct Caps {
+    int64_t pairs = -1, hits = -1, chunks = -1;
+    uint32_t pend = 0, pend0 = 0, wind = 1024;
+  } caps;
+  uint64_t *d_status = nullptr;  // sizes of the last lattice call (16 words)
+  bool capturing = false;        // inside stream capture: no host waits allowed
   // timing
   bool timing = false;
   std::vector<hipEvent_t> ev;
@@ -174,7 +184,7 @@ uint32_t read_word(sdf_ctx *c, int i) {
   return v;
 }
 
-void reset_words(sdf_ctx *c) { sdf::launch_fill_u32(c->stream, c->d_words, 4, 0u); }
+void reset_words(sdf_ctx *c) { sdf::launch_fill_u32(c->stream, c->d_words, 32, 0u); }
 
 void check_err_flags(sdf_ctx *c) {
   uint32_t e = read_word(c, 0);
@@ -223,32 +233,29 @@ void bvh_async(sdf_ctx *c, int q) {
   SDF_HIP(hipEventRecord(c->join[2], c->aux[q]));
 }
 
-// Adaptive vote for a list of items starting at ray r0 (all rays below r0
-// already counted in the items' state).  Items that need more than the
-// first min_rays rays are carried to a second pass.
-uint32_t resolve_items(sdf_ctx *c, const sdf::GridP &g, const double *d_pts,
-                       const sdf::RayParams &rp, sdf::Pending *items,
-                       uint32_t n, int r0, sdf::FusedOut out) {
-  if (n == 0) return 0;
-  const sdf::Bvh &bvh = need_bvh(c);
-  c->mark("resolve");
-  uint32_t *cnt = c->ar.get<uint32_t>("res_counts", 8);
-  uint32_t nnext = 0, nwind = 0, nwind2 = 0;
-  int rmax = rp.max_rays < sdf::kNumDirs ? rp.max_rays : sdf::kNumDirs;
+// Adaptive vote for the *n_dev items of a list (capacity cap) starting at
+// ray r0 (all rays below r0 already counted in the items' state).  Items
+// that need more than the first min_rays rays are carried to a second pass.
+// cnt: 8 device words (two phases x 4); winding counts land in cnt[2], cnt[6].
+void resolve_items(sdf_ctx *c, const sdf::GridP &g, const double *d_pts,
+                   const sdf::RayParams &rp, sdf::Pending *items, const uint32_t *n_dev,
+                   uint32_t cap, int r0, sdf::FusedOut out, uint32_t *cnt,
+                   const char *next_name) {
+  if (cap == 0) return;
+  const sdf::Bvh &bvh = c->bvh;
+  const int rmax = rp.max_rays < sdf::kNumDirs ? rp.max_rays : sdf::kNumDirs;
   if (r0 < rp.min_rays) {
-    sdf::Pending *next = c->ar.get<sdf::Pending>("res_next", n);
-    sdf::resolve_pending(c->stream, c->ar, g, d_pts, c->tri, c->nel, bvh, rp,
-                         items, n, r0, rp.min_rays, 0, out, next, cnt,
-                         c->d_words, &nnext, &nwind);
-    sdf::resolve_pending(c->stream, c->ar, g, d_pts, c->tri, c->nel, bvh, rp,
-                         next, nnext, rp.min_rays, rmax, 1, out, nullptr, cnt,
-                         c->d_words, &nnext, &nwind2);
+    sdf::Pending *next = c->ar.get<sdf::Pending>(next_name, cap);
+    sdf::resolve_pending(c->stream, c->ar, g, d_pts, c->tri, c->nel, bvh, rp, items, n_dev,
+                         cap, r0, rp.min_rays, 0, out, next, cnt, c->caps.wind, c->d_words);
+    sdf::resolve_pending(c->stream, c->ar, g, d_pts, c->tri, c->nel, bvh, rp, next,
+                         cnt + 1, cap, rp.min_rays, rmax, 1, out, nullptr, cnt + 4,
+                         c->caps.wind, c->d_words);
   } else {
-    sdf::resolve_pending(c->stream, c->ar, g, d_pts, c->tri, c->nel, bvh, rp,
-                         items, n, r0, rmax, 1, out, nullptr, cnt, c->d_words,
-                         &nnext, &nwind);
+    sdf::launch_fill_u32(c->stream, cnt + 4, 4, 0u);
+    sdf::resolve_pending(c->stream, c->ar, g, d_pts, c->tri, c->nel, bvh, rp, items, n_dev,
+                         cap, r0, rmax, 1, out, nullptr, cnt, c->caps.wind, c->d_words);
   }
-  return nwind + nwind2;
 }
 
 void set_mesh_host(sdf_ctx *c, const double *X, int64_t nnp, const int64_t *IEN,
@@ -280,48 +287,65 @@ sdf::RayParams ray_params(int32_t min_rays, int32_t max_rays, double thr,
   return rp;
 }
 
-// Lattice path: binning, fused kernel, adaptive finisher.
-void run_lattice(sdf_ctx *c, const sdf::GridP &g, bool do_dist, bool do_sign,
-                 const sdf::RayParams &rp, sdf::FusedOut out, bool learned = false) {
-  if (!c->mesh_ready) fail(SDF_ERR_ARG, "no mesh");
-  int64_t npts = grid_points(g);
+// Lattice path: binning, fused kernel, adaptive finisher, as one pass with
+// no host waits once the capacities are known (the first call on a context
+// sizes its lists exactly).  The pass ends with the sizes gathered into
+// d_status; run_lattice reads them once and repeats the pass with larger
+// capacities when a list overflowed.
+struct LatticeRun {
+  sdf::GridP g;
+  bool do_dist = false, do_sign = false, learned = false;
+  sdf::RayParams rp;
+  sdf::FusedOut out;
   sdf::Bins bins;
-  // three independent chains: LBVH (helper stream 1, no host syncs), line
-  // hits (helper stream 0), tile bins + distances (main stream)
+  float *d2buf = nullptr;
+  bool bvh_was_ready = false;
+  uint32_t cap = 0, cap0 = 0;
+};
+
+enum : int {
+  kStErr = 0, kStPend, kStPend0, kStPairs, kStHits, kStChunks, kStWindA1, kStWindA2,
+  kStWindB1, kStWindB2, kStCount
+};
+
+void lattice_pass(sdf_ctx *c, LatticeRun &L) {
+  const sdf::GridP &g = L.g;
+  const int64_t npts = grid_points(g);
+  sdf::Bins &bins = L.bins;
+  bins = sdf::Bins();
+  if (c->capturing && ((L.do_dist && (c->caps.pairs < 0 || c->caps.chunks < 0)) ||
+                       (L.do_sign && c->caps.hits < 0)))
+    fail(SDF_ERR_INTERNAL, "capture needs known capacities");
+  // three independent chains: tile bins + distances (main stream), line
+  // hits (helper stream 0), LBVH (helper stream 1)
   SDF_HIP(hipEventRecord(c->join[0], c->stream));
   for (int q = 0; q < 2; q++) SDF_HIP(hipStreamWaitEvent(c->aux[q], c->join[0], 0));
-  const bool bvh_was_ready = c->bvh_ready;
-  // counting phases first (no host waits; the critical tile chain leads),
-  // then the LBVH, then size + fill each chain
+  L.bvh_was_ready = c->bvh_ready;
   sdf::LineBuild lb;
   sdf::TileBuild tb;
   c->mark("bin_tiles");
-  if (do_dist) sdf::tile_bins_begin(c->stream, c->ar, c->tri, c->nel, g, g.h, tb);
-  if (do_sign) {
+  if (L.do_dist) sdf::tile_bins_begin(c->stream, c->ar, c->tri, c->nel, g, g.h, tb);
+  if (L.do_sign) {
     c->ar.scope = "line/";
     sdf::line_hits_begin(c->aux[0], c->ar, c->tri, c->nel, g, lb, bins);
     c->ar.scope.clear();
     bvh_async(c, 1);
-  }
-  if (do_sign) {
     c->ar.scope = "line/";
-    sdf::line_hits_finish(c->aux[0], c->ar, c->tri, c->nel, g, lb, bins);
+    sdf::line_hits_finish(c->aux[0], c->ar, c->tri, c->nel, g, lb, bins, c->caps.hits);
     c->ar.scope.clear();
     SDF_HIP(hipEventRecord(c->join[1], c->aux[0]));
   }
-  float *d2buf = nullptr;
-  if (do_dist) {
-    sdf::tile_bins_finish(c->stream, c->ar, c->tri, c->nel, g, tb, bins);
-    sdf::build_dist_chunks(c->stream, c->ar, g, bins);
-  }
-  if (do_dist) {
+  L.d2buf = nullptr;
+  if (L.do_dist) {
+    sdf::tile_bins_finish(c->stream, c->ar, c->tri, c->nel, g, tb, bins, c->caps.pairs);
+    sdf::build_dist_chunks(c->stream, c->ar, g, bins, c->caps.chunks);
     c->mark("dist");
-    d2buf = c->ar.get<float>("d2buf", (size_t)bins.nchunks * sdf::kTilePts + 1);
-    sdf::launch_fill_u32(c->stream, (uint32_t *)d2buf,
+    L.d2buf = c->ar.get<float>("d2buf", (size_t)bins.nchunks * sdf::kTilePts + 1);
+    sdf::launch_fill_u32(c->stream, (uint32_t *)L.d2buf,
                          (int64_t)bins.nchunks * sdf::kTilePts, 0x7F800000u);
-    if (learned) {
+    if (L.learned) {
       // network radius per tile point, then the radius-limited distance
-      // loop with its exact re-run of tiles whose radius held no triangle
+      // loop; nodes whose radius held no triangle get exact LBVH queries
       float *r2 = c->ar.get<float>("l_r2", (size_t)bins.nchunks * sdf::kTilePts + 1);
       uint64_t *list = c->ar.get<uint64_t>("l_list", (size_t)bins.nchunks * sdf::kTilePts + 1);
       c->lstats = c->ar.get<unsigned long long>("l_stats", kLearnedStats);
@@ -330,63 +354,132 @@ void run_lattice(sdf_ctx *c, const sdf::GridP &g, bool do_dist, bool do_sign,
       sdf::mlp_tile_radius(c->stream, c->mlp, c->mlp_norm, g, bins, (float)c->mlp_hc,
                            (float)c->mlp_delta, r2);
       c->mark("dist_learned");
-      sdf::launch_dist_chunks_learned(c->stream, g, c->tri, bins, r2, d2buf, list, c->lstats);
-      // nodes whose radius held no triangle: exact LBVH queries (the LBVH
-      // is built on a helper stream for the sign stage anyway)
-      if (!bvh_was_ready) bvh_async(c, 1);
-      SDF_HIP(hipStreamWaitEvent(c->stream, c->join[2], 0));
+      sdf::launch_dist_chunks_learned(c->stream, g, c->tri, bins, r2, L.d2buf, list, c->lstats);
+      if (!L.bvh_was_ready) {
+        bvh_async(c, 1);
+        SDF_HIP(hipStreamWaitEvent(c->stream, c->join[2], 0));
+      }
       c->mark("fallback");
       const float slack = 1e-6f * (float)(fabs(g.ox) + fabs(g.oy) + fabs(g.oz) +
                                           (double)(g.nx + g.ny + g.nz) * g.h + 1e-30);
       sdf::launch_learned_fallback(c->stream, g, c->tri, c->bvh, bins, list, c->lstats, slack,
-                                   d2buf, c->d_words);
+                                   L.d2buf, c->d_words);
       c->lstat_pairs = bins.ntile_pairs;
     } else {
-      sdf::launch_dist_chunks(c->stream, g, c->tri, bins, d2buf);
+      sdf::launch_dist_chunks(c->stream, g, c->tri, bins, L.d2buf);
     }
   }
   c->mark("bin_lines_wait");
-  if (do_sign) SDF_HIP(hipStreamWaitEvent(c->stream, c->join[1], 0));
-  uint32_t cap = (uint32_t)(npts < (1 << 22) ? (npts > 0 ? npts : 1) : (1 << 22));
-  uint32_t cap0 = (uint32_t)(npts < (1 << 20) ? (npts > 0 ? npts : 1) : (1 << 20));
-  sdf::Pending *pend = c->ar.get<sdf::Pending>("pend", cap);
-  sdf::Pending *pend0 = c->ar.get<sdf::Pending>("pend0", cap0);
+  if (L.do_sign) SDF_HIP(hipStreamWaitEvent(c->stream, c->join[1], 0));
+  L.cap = c->caps.pend ? c->caps.pend
+                       : (uint32_t)(npts < (1 << 22) ? (npts > 0 ? npts : 1) : (1 << 22));
+  L.cap0 = c->caps.pend0 ? c->caps.pend0
+                         : (uint32_t)(npts < (1 << 20) ? (npts > 0 ? npts : 1) : (1 << 20));
+  sdf::Pending *pend = c->ar.get<sdf::Pending>("pend", L.cap);
+  sdf::Pending *pend0 = c->ar.get<sdf::Pending>("pend0", L.cap0);
   c->mark("fused");
-  sdf::launch_fused_lattice(c->stream, g, c->tri, bins, d2buf, do_sign, out, pend,
-                            c->d_words + 1, cap, pend0, c->d_words + 2, cap0, c->d_words);
-  uint32_t np = 0, np0 = 0;
-  if (do_sign) {
-    np = read_word(c, 1);
-    np0 = read_word(c, 2);
-    if (np > cap || np0 > cap0) {
-      // Rerun with lists large enough for every undecided point.  Any
-      // other flag (bad index) is reported first.
-      uint32_t e = read_word(c, 0) & ~sdf::kErrPendingOverflow;
-      if (e) check_err_flags(c);
-      if (np > cap) cap = np;
-      if (np0 > cap0) cap0 = np0;
-      pend = c->ar.get<sdf::Pending>("pend", cap);
-      pend0 = c->ar.get<sdf::Pending>("pend0", cap0);
-      sdf::launch_fill_u32(c->stream, c->d_words, 4, 0u);
-      sdf::launch_fused_lattice(c->stream, g, c->tri, bins, d2buf, do_sign, out, pend,
-                                c->d_words + 1, cap, pend0, c->d_words + 2, cap0,
-                                c->d_words);
-      np = read_word(c, 1);
-      np0 = read_word(c, 2);
+  sdf::launch_fused_lattice(c->stream, g, c->tri, bins, L.d2buf, L.do_sign, L.out, pend,
+                            c->d_words + 1, L.cap, pend0, c->d_words + 2, L.cap0, c->d_words);
+  uint32_t *cnt = c->d_words + 4;
+  if (L.do_sign) {
+    if (!L.bvh_was_ready) {
+      c->mark("lbvh_wait");
+      SDF_HIP(hipStreamWaitEvent(c->stream, c->join[2], 0));
     }
+    c->mark("resolve");
+    resolve_items(c, g, nullptr, L.rp, pend, c->d_words + 1, L.cap, 6, L.out, cnt, "res_nextA");
+    resolve_items(c, g, nullptr, L.rp, pend0, c->d_words + 2, L.cap0, 0, L.out, cnt + 8,
+                  "res_nextB");
   }
-  if (do_sign && !bvh_was_ready) {
-    c->mark("lbvh_wait");
-    SDF_HIP(hipStreamWaitEvent(c->stream, c->join[2], 0));
-  }
-  if (np > 0) resolve_items(c, g, nullptr, rp, pend, np, 6, out);
-  if (np0 > 0) resolve_items(c, g, nullptr, rp, pend0, np0, 0, out);
-  c->counters[0] = (int64_t)np + np0;
   c->mark("end");
-  c->counters[1] = bins.ntile_pairs;
-  c->counters[2] = bins.nline_hits;
-  c->counters[3] = (int64_t)c->ar.bytes();
-  check_err_flags(c);
+  sdf::StatusSrc src;
+  src.n = kStCount;
+  src.set(kStErr, c->d_words, false);
+  src.set(kStPend, c->d_words + 1, false);
+  src.set(kStPend0, c->d_words + 2, false);
+  src.set(kStPairs, bins.npairs_dev, true);
+  src.set(kStHits, bins.nhits_dev, true);
+  src.set(kStChunks, bins.nchunks_dev, true);
+  if (L.do_sign) {
+    src.set(kStWindA1, cnt + 2, false);
+    src.set(kStWindA2, cnt + 6, false);
+    src.set(kStWindB1, cnt + 10, false);
+    src.set(kStWindB2, cnt + 14, false);
+  }
+  sdf::launch_status(c->stream, src, c->d_status);
+}
+
+// Reads the status of the last pass; grows the capacities that overflowed
+// (true: the pass must be repeated).  Reports errors otherwise.
+bool lattice_settle(sdf_ctx *c, LatticeRun &L, const uint64_t *st) {
+  bool again = false;
+  auto grow64 = [&](int64_t &cap, uint64_t need, bool used) {
+    if (!used) return;
+    const int64_t want = (int64_t)(need + need / 8 + 1024);
+    if (cap >= 0 && (int64_t)need > cap) again = true;
+    if (cap < want && ((int64_t)need > cap || cap < 0)) cap = want;
+  };
+  grow64(c->caps.pairs, st[kStPairs], L.do_dist && c->nel > 0);
+  grow64(c->caps.chunks, st[kStChunks], L.do_dist);
+  grow64(c->caps.hits, st[kStHits], L.do_sign);
+  if (st[kStPend] > L.cap) {
+    c->caps.pend = (uint32_t)std::min<uint64_t>(st[kStPend] + st[kStPend] / 8 + 1024, 0xFFFFFFF0u);
+    again = true;
+  }
+  if (st[kStPend0] > L.cap0) {
+    c->caps.pend0 = (uint32_t)std::min<uint64_t>(st[kStPend0] + st[kStPend0] / 8 + 1024, 0xFFFFFFF0u);
+    again = true;
+  }
+  uint64_t w = std::max(std::max(st[kStWindA1], st[kStWindA2]),
+                        std::max(st[kStWindB1], st[kStWindB2]));
+  if (w > c->caps.wind) {
+    c->caps.wind = (uint32_t)std::min<uint64_t>(w + w / 8 + 64, 0xFFFFFFF0u);
+    again = true;
+  }
+  const uint32_t e = (uint32_t)st[kStErr] & ~(uint32_t)sdf::kErrPendingOverflow;
+  if (e & sdf::kErrBadIndex) fail(SDF_ERR_ARG, "IEN holds a vertex index outside 1..nnp");
+  if (again) return true;
+  if (e & sdf::kErrHitOverflow)
+    fail(SDF_ERR_LIMIT, "a ray crossed more triangles than the hit buffer holds");
+  if (e & sdf::kErrStackOverflow) fail(SDF_ERR_LIMIT, "BVH traversal stack overflow");
+  if (st[kStErr] & sdf::kErrPendingOverflow) fail(SDF_ERR_INTERNAL, "pending list overflow");
+  return false;
+}
+
+void read_status(sdf_ctx *c, uint64_t *st) {
+  SDF_HIP(hipMemcpyAsync(st, c->d_status, 16 * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                         c->stream));
+  SDF_HIP(hipStreamSynchronize(c->stream));
+}
+
+void run_lattice(sdf_ctx *c, const sdf::GridP &g, bool do_dist, bool do_sign,
+                 const sdf::RayParams &rp, sdf::FusedOut out, bool learned = false) {
+  if (!c->mesh_ready) fail(SDF_ERR_ARG, "no mesh");
+  LatticeRun L;
+  L.g = g;
+  L.do_dist = do_dist;
+  L.do_sign = do_sign;
+  L.learned = learned;
+  L.rp = rp;
+  L.out = out;
+  for (int attempt = 0;; attempt++) {
+    if (attempt > 0) {
+      reset_words(c);
+      c->nev = 0;
+    }
+    lattice_pass(c, L);
+    uint64_t st[16];
+    read_status(c, st);
+    if (!lattice_settle(c, L, st)) {
+      c->counters[0] = (int64_t)(st[kStPend] + st[kStPend0]);
+      c->counters[1] = (int64_t)st[kStPairs];
+      c->counters[2] = (int64_t)st[kStHits];
+      c->counters[3] = (int64_t)c->ar.bytes();
+      c->lstat_pairs = (int64_t)st[kStPairs];
+      return;
+    }
+    if (attempt >= 3) fail(SDF_ERR_INTERNAL, "list capacities did not settle");
+  }
 }
 
 bool is_lattice(const sdf::GridP &g, const double *pts, int64_t npts) {
@@ -559,7 +652,8 @@ sdf_status sdf_ctx_create(int32_t device, sdf_ctx **out) {
         SDF_HIP(hipStreamCreateWithFlags(&c->aux[q], hipStreamNonBlocking));
       for (int q = 0; q < 3; q++)
         SDF_HIP(hipEventCreateWithFlags(&c->join[q], hipEventDisableTiming));
-      SDF_HIP(hipMalloc(&c->d_words, 64));
+      SDF_HIP(hipMalloc(&c->d_words, 256));
+      SDF_HIP(hipMalloc(&c->d_status, 16 * sizeof(uint64_t)));
     } catch (...) {
       delete c;
       throw;
@@ -583,6 +677,7 @@ sdf_status sdf_ctx_destroy(sdf_ctx *c) {
     for (int q = 0; q < 3; q++)
       if (c->join[q]) (void)hipEventDestroy(c->join[q]);
     if (c->d_words) (void)hipFree(c->d_words);
+    if (c->d_status) (void)hipFree(c->d_status);
     if (c->own) (void)hipStreamDestroy(c->own);
     delete c;
   });
@@ -926,9 +1021,23 @@ sdf_status sdf_raycast_signs(sdf_ctx *c, const double *X, int64_t nnp,
       }
       if (npts > 0xFFFFFFFFll) fail(SDF_ERR_ARG, "too many points");
       sdf::Pending *pend = c->ar.get<sdf::Pending>("pend_all", npts + 1);
-      sdf::launch_init_pending_all(c->stream, pend, npts);
-      resolve_items(c, g, d_pts, rp, pend, (uint32_t)npts, 0, out);
-      c->mark("end");
+      need_bvh(c);
+      for (int attempt = 0;; attempt++) {
+        if (attempt > 0) reset_words(c);
+        sdf::launch_init_pending_all(c->stream, pend, npts);
+        uint32_t *nd = c->d_words + 3;
+        sdf::launch_fill_u32(c->stream, nd, 1, (uint32_t)npts);
+        uint32_t *cnt = c->d_words + 4;
+        c->mark("resolve");
+        resolve_items(c, g, d_pts, rp, pend, nd, (uint32_t)npts, 0, out, cnt, "res_nextA");
+        c->mark("end");
+        uint32_t hc[8];
+        SDF_HIP(hipMemcpyAsync(hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, c->stream));
+        SDF_HIP(hipStreamSynchronize(c->stream));
+        const uint32_t w = std::max(hc[2], hc[6]);
+        if (w <= c->caps.wind || attempt >= 2) break;
+        c->caps.wind = w + w / 8 + 64;
+      }
       check_err_flags(c);
     }
     if (npts > 0) {

@@ -18,6 +18,15 @@ constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanBlock = kScanThreads * kScanItems;
 
+__global__ void status_kernel(StatusSrc src, uint64_t *out) {
+  const int i = threadIdx.x;
+  if (i >= 16) return;
+  uint64_t v = 0;
+  if (i < src.n && src.p[i])
+    v = src.is64[i] ? *(const uint64_t *)src.p[i] : (uint64_t) * (const uint32_t *)src.p[i];
+  out[i] = v;
+}
+
 __global__ void fill_u32_kernel(uint32_t *p, int64_t n, uint32_t v) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
@@ -243,7 +252,7 @@ __global__ void tile_emit_small_kernel(const Tri64 *__restrict__ tri, int64_t ne
                                        GridP g, TileDims td, double R,
                                        const uint64_t *__restrict__ toff,
                                        uint32_t *__restrict__ keys,
-                                       uint32_t *__restrict__ vals) {
+                                       uint32_t *__restrict__ vals, uint64_t cap) {
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nel;
        t += (int64_t)gridDim.x * blockDim.x) {
     const Tri64 tr = tri[t];
@@ -254,8 +263,10 @@ __global__ void tile_emit_small_kernel(const Tri64 *__restrict__ tri, int64_t ne
       for (int32_t ty = r.lo[1]; ty <= r.hi[1]; ty++)
         for (int32_t tx = r.lo[0]; tx <= r.hi[0]; tx++)
           if (tile_plane_test(tr, g, td, tx, ty, tz, R)) {
-            keys[pos] = (uint32_t)(((int64_t)tz * td.ty + ty) * td.tx + tx);
-            vals[pos] = (uint32_t)t;
+            if (pos < cap) {  // beyond: overflow, reported by the sizes
+              keys[pos] = (uint32_t)(((int64_t)tz * td.ty + ty) * td.tx + tx);
+              vals[pos] = (uint32_t)t;
+            }
             pos++;
           }
   }
@@ -267,7 +278,7 @@ __global__ __launch_bounds__(256) void tile_big_kernel(
     const Tri64 *__restrict__ tri, GridP g, TileDims td, double R, int mode,
     const uint32_t *__restrict__ big, const uint32_t *nbig,
     uint32_t *__restrict__ cnt, const uint64_t *__restrict__ toff,
-    uint32_t *__restrict__ keys, uint32_t *__restrict__ vals) {
+    uint32_t *__restrict__ keys, uint32_t *__restrict__ vals, uint64_t cap) {
   __shared__ uint32_t s_scan[256];
   __shared__ uint32_t s_base;
   const uint32_t nb = *nbig;
@@ -299,8 +310,10 @@ __global__ __launch_bounds__(256) void tile_big_kernel(
       const uint32_t incl = s_scan[threadIdx.x];
       if (mode == 1 && pass) {
         const uint64_t pos = toff[t] + s_base + incl - 1;
-        keys[pos] = (uint32_t)(((int64_t)tz * td.ty + ty) * td.tx + tx);
-        vals[pos] = t;
+        if (pos < cap) {
+          keys[pos] = (uint32_t)(((int64_t)tz * td.ty + ty) * td.tx + tx);
+          vals[pos] = t;
+        }
       }
       __syncthreads();
       if (threadIdx.x == 255) s_base += incl;
@@ -314,7 +327,9 @@ __global__ __launch_bounds__(256) void tile_big_kernel(
 // CSR offsets from the sorted tile keys: off[t] = first index whose key is
 // >= t (binary search per tile; no serial runs over empty stretches).
 __global__ void tile_offsets_kernel(const uint32_t *__restrict__ keys, int64_t n,
-                                    int64_t ntiles, uint64_t *__restrict__ off) {
+                                    const uint64_t *__restrict__ n_dev, int64_t ntiles,
+                                    uint64_t *__restrict__ off) {
+  if (n_dev && (int64_t)*n_dev < n) n = (int64_t)*n_dev;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t <= ntiles;
        t += (int64_t)gridDim.x * blockDim.x) {
     int64_t lo = 0, hi = n;
@@ -366,7 +381,7 @@ __device__ inline void line_visit(const Tri64 &tr, const GridP &g, int ax,
                                   int64_t base2, int mode,
                                   uint32_t *__restrict__ cnt,
                                   const uint64_t *__restrict__ off,
-                                  LineHit *__restrict__ hits) {
+                                  LineHit *__restrict__ hits, uint64_t cap) {
   // origin: the lattice point with index 0 along the axis
   double o[3];
   int64_t line;
@@ -393,7 +408,8 @@ __device__ inline void line_visit(const Tri64 &tr, const GridP &g, int ax,
     atomicAdd(&cnt[line], 1u);
   } else {
     const uint32_t slot = atomicAdd(&cnt[line], 1u);  // cnt is the cursor here
-    hits[off[line] + slot] = rec;
+    const uint64_t pos = off[line] + slot;
+    if (pos < cap) hits[pos] = rec;  // beyond: overflow, reported by the sizes
   }
 }
 
@@ -401,7 +417,7 @@ __global__ void line_bin_small_kernel(const Tri64 *__restrict__ tri, int64_t nel
                                       GridP g, int64_t base1, int64_t base2,
                                       int mode, uint32_t *__restrict__ cnt,
                                       const uint64_t *__restrict__ off,
-                                      LineHit *__restrict__ hits,
+                                      LineHit *__restrict__ hits, uint64_t cap,
                                       uint32_t *__restrict__ big, uint32_t *nbig) {
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nel;
        t += (int64_t)gridDim.x * blockDim.x) {
@@ -416,7 +432,7 @@ __global__ void line_bin_small_kernel(const Tri64 *__restrict__ tri, int64_t nel
     for (int ax = 0; ax < 3; ax++)
       for (int32_t b = r.lo[ax][1]; b <= r.hi[ax][1]; b++)
         for (int32_t a = r.lo[ax][0]; a <= r.hi[ax][0]; a++)
-          line_visit(tr, g, ax, a, b, base1, base2, mode, cnt, off, hits);
+          line_visit(tr, g, ax, a, b, base1, base2, mode, cnt, off, hits, cap);
   }
 }
 
@@ -424,7 +440,7 @@ __global__ void line_bin_big_kernel(const Tri64 *__restrict__ tri, GridP g,
                                     int64_t base1, int64_t base2, int mode,
                                     uint32_t *__restrict__ cnt,
                                     const uint64_t *__restrict__ off,
-                                    LineHit *__restrict__ hits,
+                                    LineHit *__restrict__ hits, uint64_t cap,
                                     const uint32_t *__restrict__ big,
                                     const uint32_t *nbig) {
   const uint32_t nb = *nbig;
@@ -442,7 +458,7 @@ __global__ void line_bin_big_kernel(const Tri64 *__restrict__ tri, GridP g,
       const int64_t w0 = r.hi[ax][0] - r.lo[ax][0] + 1;
       line_visit(tr, g, ax, r.lo[ax][0] + (int32_t)(local % w0),
                  r.lo[ax][1] + (int32_t)(local / w0), base1, base2, mode, cnt, off,
-                 hits);
+                 hits, cap);
     }
   }
 }
@@ -457,10 +473,11 @@ __global__ void tile_chunks_kernel(const uint64_t *__restrict__ tile_off,
 }
 
 __global__ void chunk_items_kernel(const uint64_t *__restrict__ chunk_off,
-                                   int64_t ntiles, uint32_t *__restrict__ items) {
+                                   int64_t ntiles, uint32_t *__restrict__ items,
+                                   uint64_t cap) {
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < ntiles;
        t += (int64_t)gridDim.x * blockDim.x) {
-    for (uint64_t q = chunk_off[t]; q < chunk_off[t + 1]; q++) items[q] = (uint32_t)t;
+    for (uint64_t q = chunk_off[t]; q < chunk_off[t + 1] && q < cap; q++) items[q] = (uint32_t)t;
   }
 }
 
@@ -482,8 +499,8 @@ __global__ void chunk_items_kernel(const uint64_t *__restrict__ chunk_off,
 // eps + 2 Dmax (Dmax the line's largest delta); any gap in between makes
 // the whole line take the exact path.
 __global__ void line_prep_kernel(GridP g, const uint64_t *__restrict__ line_off,
-                                 const LineHit *__restrict__ hits, int64_t base1,
-                                 int64_t base2, int64_t nlines,
+                                 const LineHit *__restrict__ hits, uint64_t cap,
+                                 int64_t base1, int64_t base2, int64_t nlines,
                                  LineFast *__restrict__ fast,
                                  uint32_t *__restrict__ meta,
                                  LineInline *__restrict__ inl) {
@@ -506,7 +523,7 @@ __global__ void line_prep_kernel(GridP g, const uint64_t *__restrict__ line_off,
       inl[L] = li;
       continue;
     }
-    if (n > kLineFastMax) {
+    if (n > kLineFastMax || he > cap) {  // he > cap: overflow, the call is re-run
       meta[L] = kLineExact;
       li.info = kLineExact | 255u;
       inl[L] = li;
@@ -618,6 +635,11 @@ uint64_t read_u64(hipStream_t s, const uint64_t *d) {
 
 }  // namespace
 
+void launch_status(hipStream_t s, const StatusSrc &src, uint64_t *out) {
+  hipLaunchKernelGGL(status_kernel, dim3(1), dim3(64), 0, s, src, out);
+  SDF_HIP(hipGetLastError());
+}
+
 void launch_fill_u32(hipStream_t s, uint32_t *p, int64_t n, uint32_t v) {
   if (n <= 0) return;
   hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s,
@@ -665,38 +687,42 @@ void tile_bins_begin(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
                        tri, nel, g, td, tb.R, tb.tcnt, tb.big, tb.nbig);
     hipLaunchKernelGGL(tile_big_kernel, dim3(1024), dim3(256), 0, s, tri, g, td, tb.R, 0,
                        (const uint32_t *)tb.big, (const uint32_t *)tb.nbig, tb.tcnt,
-                       (const uint64_t *)nullptr, (uint32_t *)nullptr, (uint32_t *)nullptr);
+                       (const uint64_t *)nullptr, (uint32_t *)nullptr, (uint32_t *)nullptr,
+                       (uint64_t)0);
     SDF_HIP(hipGetLastError());
     scan_counts(s, ar, tb.tcnt, tb.toff, nel);
   }
 }
 
 void tile_bins_finish(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
-                      const GridP &g, TileBuild &tb, Bins &b) {
+                      const GridP &g, TileBuild &tb, Bins &b, int64_t cap) {
   TileDims td = tile_dims(g);
   uint64_t *tile_off = ar.get<uint64_t>("bin_tile_off", td.ntiles + 2);
-  const uint64_t npairs = nel > 0 ? read_u64(s, tb.toff + nel) : 0;
-  uint32_t *keys = ar.get<uint32_t>("bin_tile_keys", npairs + 1);
-  uint32_t *tile_tris = ar.get<uint32_t>("bin_tile_tris", npairs + 1);
-  if (npairs > 0) {
+  const uint64_t *n_dev = nel > 0 ? tb.toff + nel : nullptr;
+  if (cap < 0) cap = nel > 0 ? (int64_t)read_u64(s, tb.toff + nel) : 0;
+  uint32_t *keys = ar.get<uint32_t>("bin_tile_keys", cap + 1);
+  uint32_t *tile_tris = ar.get<uint32_t>("bin_tile_tris", cap + 1);
+  if (cap > 0 && nel > 0) {
     hipLaunchKernelGGL(tile_emit_small_kernel, dim3(grid_for(nel, 256)), dim3(256), 0, s,
-                       tri, nel, g, td, tb.R, (const uint64_t *)tb.toff, keys, tile_tris);
+                       tri, nel, g, td, tb.R, (const uint64_t *)tb.toff, keys, tile_tris,
+                       (uint64_t)cap);
     hipLaunchKernelGGL(tile_big_kernel, dim3(1024), dim3(256), 0, s, tri, g, td, tb.R, 1,
                        (const uint32_t *)tb.big, (const uint32_t *)tb.nbig, tb.tcnt,
-                       (const uint64_t *)tb.toff, keys, tile_tris);
+                       (const uint64_t *)tb.toff, keys, tile_tris, (uint64_t)cap);
     SDF_HIP(hipGetLastError());
     int bits = 1;
     while (bits < 32 && ((int64_t)1 << bits) <= td.ntiles) bits++;
-    radix_sort_u32_pairs(s, ar, keys, tile_tris, (int64_t)npairs, bits);
+    radix_sort_u32_pairs(s, ar, keys, tile_tris, cap, bits, n_dev);
     hipLaunchKernelGGL(tile_offsets_kernel, dim3(grid_for(td.ntiles + 1, 256)), dim3(256),
-                       0, s, (const uint32_t *)keys, (int64_t)npairs, td.ntiles, tile_off);
+                       0, s, (const uint32_t *)keys, cap, n_dev, td.ntiles, tile_off);
     SDF_HIP(hipGetLastError());
   } else {
     launch_fill_u64(s, tile_off, td.ntiles + 1, 0ull);
   }
   b.tile_off = tile_off;
   b.tile_tris = tile_tris;
-  b.ntile_pairs = (int64_t)npairs;
+  b.ntile_pairs = cap;
+  b.npairs_dev = n_dev;
 }
 
 void build_tile_bins(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
@@ -706,7 +732,7 @@ void build_tile_bins(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
   tile_bins_finish(s, ar, tri, nel, g, tb, b);
 }
 
-void build_dist_chunks(hipStream_t s, Arena &ar, const GridP &g, Bins &b) {
+void build_dist_chunks(hipStream_t s, Arena &ar, const GridP &g, Bins &b, int64_t cap) {
   TileDims td = tile_dims(g);
   uint32_t *nch = ar.get<uint32_t>("chunk_cnt", td.ntiles + 1);
   uint64_t *coff = ar.get<uint64_t>("chunk_off", td.ntiles + 2);
@@ -714,16 +740,17 @@ void build_dist_chunks(hipStream_t s, Arena &ar, const GridP &g, Bins &b) {
                      0, s, (const uint64_t *)b.tile_off, td.ntiles, nch);
   SDF_HIP(hipGetLastError());
   scan_counts(s, ar, nch, coff, td.ntiles);
-  const uint64_t n = read_u64(s, coff + td.ntiles);
-  uint32_t *items = ar.get<uint32_t>("chunk_tile", n + 1);
-  if (n > 0) {
+  if (cap < 0) cap = (int64_t)read_u64(s, coff + td.ntiles);
+  uint32_t *items = ar.get<uint32_t>("chunk_tile", cap + 1);
+  if (cap > 0) {
     hipLaunchKernelGGL(chunk_items_kernel, dim3(grid_for(td.ntiles, 256)), dim3(256),
-                       0, s, (const uint64_t *)coff, td.ntiles, items);
+                       0, s, (const uint64_t *)coff, td.ntiles, items, (uint64_t)cap);
     SDF_HIP(hipGetLastError());
   }
   b.chunk_off = coff;
   b.chunk_tile = items;
-  b.nchunks = (int64_t)n;
+  b.nchunks = cap;
+  b.nchunks_dev = coff + td.ntiles;
 }
 
 void line_hits_begin(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
@@ -745,10 +772,11 @@ void line_hits_begin(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
   if (nel > 0) {
     hipLaunchKernelGGL(line_bin_small_kernel, dim3(grid_for(nel, 256)), dim3(256), 0, s,
                        tri, nel, g, lb.nlx, lb.nlx + lb.nly, 0, lb.lcnt,
-                       (const uint64_t *)nullptr, (LineHit *)nullptr, lb.big, lb.nbig);
+                       (const uint64_t *)nullptr, (LineHit *)nullptr, (uint64_t)0, lb.big,
+                       lb.nbig);
     hipLaunchKernelGGL(line_bin_big_kernel, dim3(1024), dim3(256), 0, s, tri, g, lb.nlx,
                        lb.nlx + lb.nly, 0, lb.lcnt, (const uint64_t *)nullptr,
-                       (LineHit *)nullptr, (const uint32_t *)lb.big,
+                       (LineHit *)nullptr, (uint64_t)0, (const uint32_t *)lb.big,
                        (const uint32_t *)lb.nbig);
     SDF_HIP(hipGetLastError());
   }
@@ -756,24 +784,25 @@ void line_hits_begin(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
 }
 
 void line_hits_finish(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
-                      const GridP &g, LineBuild &lb, Bins &b) {
-  const uint64_t nhits = read_u64(s, lb.loff + lb.nlines);
-  LineHit *hits = ar.get<LineHit>("line_hits", nhits + 1);
-  if (nhits > 0) {
+                      const GridP &g, LineBuild &lb, Bins &b, int64_t cap) {
+  const uint64_t *n_dev = lb.loff + lb.nlines;
+  if (cap < 0) cap = (int64_t)read_u64(s, lb.loff + lb.nlines);
+  LineHit *hits = ar.get<LineHit>("line_hits", cap + 1);
+  if (cap > 0 && nel > 0) {
     launch_fill_u32(s, lb.lcnt, lb.nlines + 1, 0u);  // reused as the cursor
     hipLaunchKernelGGL(line_bin_small_kernel, dim3(grid_for(nel, 256)), dim3(256), 0, s,
                        tri, nel, g, lb.nlx, lb.nlx + lb.nly, 1, lb.lcnt,
-                       (const uint64_t *)lb.loff, hits, lb.big, lb.nbig);
+                       (const uint64_t *)lb.loff, hits, (uint64_t)cap, lb.big, lb.nbig);
     hipLaunchKernelGGL(line_bin_big_kernel, dim3(1024), dim3(256), 0, s, tri, g, lb.nlx,
                        lb.nlx + lb.nly, 1, lb.lcnt, (const uint64_t *)lb.loff, hits,
-                       (const uint32_t *)lb.big, (const uint32_t *)lb.nbig);
+                       (uint64_t)cap, (const uint32_t *)lb.big, (const uint32_t *)lb.nbig);
     SDF_HIP(hipGetLastError());
   }
-  LineFast *fast = ar.get<LineFast>("line_fast", nhits + 1);
+  LineFast *fast = ar.get<LineFast>("line_fast", cap + 1);
   uint32_t *meta = ar.get<uint32_t>("line_meta", lb.nlines + 1);
   LineInline *inl = ar.get<LineInline>("line_inline", lb.nlines + 1);
   hipLaunchKernelGGL(line_prep_kernel, dim3(grid_for(lb.nlines, 256)), dim3(256), 0, s, g,
-                     (const uint64_t *)lb.loff, (const LineHit *)hits, lb.nlx,
+                     (const uint64_t *)lb.loff, (const LineHit *)hits, (uint64_t)cap, lb.nlx,
                      lb.nlx + lb.nly, lb.nlines, fast, meta, inl);
   SDF_HIP(hipGetLastError());
   b.line_off = lb.loff;
@@ -781,7 +810,8 @@ void line_hits_finish(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
   b.line_fast = fast;
   b.line_meta = meta;
   b.line_inline = inl;
-  b.nline_hits = (int64_t)nhits;
+  b.nline_hits = cap;
+  b.nhits_dev = n_dev;
 }
 
 void build_line_hits(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,

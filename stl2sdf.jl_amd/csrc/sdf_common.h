@@ -397,4 +397,12 @@ __device__ inline float tri_pre_dist2(float px, float py, float pz, Get T) {
   return fminf(fminf(face, d0), fminf(d1, d2));
 }
 
+// Number of work items when the host holds only a capacity and the device
+// holds the actual count (null: the capacity is the count).
+__device__ inline int64_t chunk_count(int64_t cap, const uint64_t *dev) {
+  if (!dev) return cap;
+  const int64_t n = (int64_t)*dev;
+  return n < cap ? n : cap;
+}
+
 }  // namespace sdf

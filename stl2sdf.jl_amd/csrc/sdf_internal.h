@@ -90,7 +90,19 @@ struct Timer {
   hipEvent_t a = nullptr, b = nullptr;
 };
 
+// Sources of a status block: word i = *p[i] (32- or 64-bit), 0 when null.
+struct StatusSrc {
+  const void *p[16] = {};
+  int is64[16] = {};
+  int n = 0;
+  void set(int i, const void *ptr, bool wide) {
+    p[i] = ptr;
+    is64[i] = wide ? 1 : 0;
+  }
+};
+
 // ---------------- kernel launchers (implemented in .hip files) -----------
+void launch_status(hipStream_t s, const StatusSrc &src, uint64_t *out);
 // build side
 void launch_fill_u32(hipStream_t s, uint32_t *p, int64_t n, uint32_t v);
 void launch_fill_u64(hipStream_t s, uint64_t *p, int64_t n, uint64_t v);
@@ -118,6 +130,10 @@ struct Bins {
   uint64_t *chunk_off = nullptr;  // ntiles + 1 (first work item of a tile)
   uint32_t *chunk_tile = nullptr;  // work item -> tile
   int64_t nchunks = 0;
+  // Capacities above; the actual counts live on the device (written by the
+  // scans).  A count above its capacity means the lists were truncated and
+  // the call is repeated with larger capacities.
+  const uint64_t *npairs_dev = nullptr, *nhits_dev = nullptr, *nchunks_dev = nullptr;
 };
 
 void build_tile_bins(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
@@ -136,15 +152,16 @@ struct LineBuild {
 };
 void tile_bins_begin(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
                      const GridP &g, double radius, TileBuild &tb);
+// cap < 0: size the lists exactly (reads the count back: host wait).
 void tile_bins_finish(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
-                      const GridP &g, TileBuild &tb, Bins &b);
+                      const GridP &g, TileBuild &tb, Bins &b, int64_t cap = -1);
 void line_hits_begin(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
                      const GridP &g, LineBuild &lb, Bins &b);
 void line_hits_finish(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
-                      const GridP &g, LineBuild &lb, Bins &b);
+                      const GridP &g, LineBuild &lb, Bins &b, int64_t cap = -1);
 void build_line_hits(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
                      const GridP &g, Bins &b);
-void build_dist_chunks(hipStream_t s, Arena &ar, const GridP &g, Bins &b);
+void build_dist_chunks(hipStream_t s, Arena &ar, const GridP &g, Bins &b, int64_t cap = -1);
 
 // query side
 struct FusedOut {
@@ -194,12 +211,17 @@ struct Bvh;
 // Adaptive vote for pending items: rays [r0, r1) via the LBVH, then the
 // vote replay (final_phase = 0: items needing more rays go to `next`), then
 // the winding fallback.  pts == nullptr: lattice points of g.
+// Adaptive vote for *n_dev pending items (capacity cap) over rays r0..r1-1,
+// with no host waits: grids are fixed, counts are read on the device.
+// final_phase 0 carries undecided items to `next` (count d_counts[1]);
+// otherwise every item is finished.  Items that still need the winding
+// fallback (count d_counts[2]) are finished if it is <= wind_cap (else the
+// caller repeats the call with a larger wind_cap).  d_counts: 4 words.
 void resolve_pending(hipStream_t s, Arena &ar, const GridP &g, const double *pts,
-                     const Tri64 *tri, int64_t nel, const Bvh &bvh,
-                     const RayParams &rp, const Pending *items, uint32_t nitems,
-                     int r0, int r1, int final_phase, FusedOut out,
-                     Pending *next, uint32_t *d_counts, uint32_t *err,
-                     uint32_t *n_next_host, uint32_t *n_wind_host);
+                     const Tri64 *tri, int64_t nel, const Bvh &bvh, const RayParams &rp,
+                     const Pending *items, const uint32_t *n_dev, uint32_t cap, int r0,
+                     int r1, int final_phase, FusedOut out, Pending *next,
+                     uint32_t *d_counts, uint32_t wind_cap, uint32_t *err);
 
 void launch_dist_points_bvh(hipStream_t s, const double *pts, int64_t npts,
                             const Tri64 *tri, const Bvh &bvh, double h,

@@ -22,9 +22,11 @@ struct Bvh {
 };
 
 Bvh build_lbvh(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t n);
-// Stable LSD sort of (key, value) pairs by the low key_bits bits.
+// Stable LSD sort of (key, value) pairs by the low key_bits bits; n is a
+// capacity when n_dev (the actual count, on the device) is given.
 void radix_sort_u32_pairs(hipStream_t s, Arena &ar, uint32_t *keys,
-                          uint32_t *vals, int64_t n, int key_bits);
+                          uint32_t *vals, int64_t n, int key_bits,
+                          const uint64_t *n_dev = nullptr);
 
 constexpr int kStackDepth = 64;
 

@@ -109,9 +109,10 @@ __global__ void morton_kernel(const float *__restrict__ box, int64_t n,
 // ---- radix sort (LSD, stable) ----
 
 __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(
-    const uint32_t *__restrict__ keys, int64_t n, int shift, int64_t nblocks,
-    uint32_t *__restrict__ hist) {
+    const uint32_t *__restrict__ keys, int64_t n, const uint64_t *__restrict__ n_dev,
+    int shift, int64_t nblocks, uint32_t *__restrict__ hist) {
   __shared__ uint32_t h[kRadix];
+  if (n_dev && (int64_t)*n_dev < n) n = (int64_t)*n_dev;
   for (int i = threadIdx.x; i < kRadix; i += blockDim.x) h[i] = 0;
   __syncthreads();
   const int64_t base = (int64_t)blockIdx.x * kSortTile;
@@ -130,7 +131,9 @@ __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(
 __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
     uint32_t *__restrict__ kout, uint32_t *__restrict__ vout, int64_t n,
-    int shift, int64_t nblocks, const uint64_t *__restrict__ off) {
+    const uint64_t *__restrict__ n_dev, int shift, int64_t nblocks,
+    const uint64_t *__restrict__ off) {
+  if (n_dev && (int64_t)*n_dev < n) n = (int64_t)*n_dev;
   constexpr int kWaves = kSortThreads / 64;
   __shared__ uint32_t running[kRadix];
   __shared__ uint32_t wave_cnt[kWaves][kRadix];
@@ -299,7 +302,7 @@ __global__ void node_boxes_kernel(const float *__restrict__ seg, SegLevels L,
 }
 
 void radix_sort_pairs(hipStream_t s, Arena &ar, uint32_t *keys, uint32_t *vals,
-                      int64_t n, int key_bits) {
+                      int64_t n, int key_bits, const uint64_t *n_dev) {
   if (n <= 1) return;
   int64_t nblocks = (n + kSortTile - 1) / kSortTile;
   uint32_t *k2 = ar.get<uint32_t>(ar.tmp("sort_k2"), n);
@@ -310,11 +313,11 @@ void radix_sort_pairs(hipStream_t s, Arena &ar, uint32_t *keys, uint32_t *vals,
   int passes = 0;
   for (int shift = 0; shift < key_bits; shift += kRadixBits, passes++) {
     hipLaunchKernelGGL(radix_hist_kernel, dim3((unsigned)nblocks),
-                       dim3(kSortThreads), 0, s, ka, n, shift, nblocks, hist);
+                       dim3(kSortThreads), 0, s, ka, n, n_dev, shift, nblocks, hist);
     SDF_HIP(hipGetLastError());
     scan_counts(s, ar, hist, off, kRadix * nblocks);
     hipLaunchKernelGGL(radix_scatter_kernel, dim3((unsigned)nblocks),
-                       dim3(kSortThreads), 0, s, ka, va, kb, vb, n, shift,
+                       dim3(kSortThreads), 0, s, ka, va, kb, vb, n, n_dev, shift,
                        nblocks, off);
     SDF_HIP(hipGetLastError());
     uint32_t *t = ka;
@@ -335,8 +338,8 @@ void radix_sort_pairs(hipStream_t s, Arena &ar, uint32_t *keys, uint32_t *vals,
 }  // namespace
 
 void radix_sort_u32_pairs(hipStream_t s, Arena &ar, uint32_t *keys,
-                          uint32_t *vals, int64_t n, int key_bits) {
-  radix_sort_pairs(s, ar, keys, vals, n, key_bits);
+                          uint32_t *vals, int64_t n, int key_bits, const uint64_t *n_dev) {
+  radix_sort_pairs(s, ar, keys, vals, n, key_bits, n_dev);
 }
 
 Bvh build_lbvh(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t n) {
@@ -355,7 +358,7 @@ Bvh build_lbvh(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t n) {
   hipLaunchKernelGGL(morton_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s,
                      b.leaf_box, n, cb, codes, leaf_tri);
   SDF_HIP(hipGetLastError());
-  radix_sort_pairs(s, ar, codes, leaf_tri, n, 30);
+  radix_sort_pairs(s, ar, codes, leaf_tri, n, 30, nullptr);
   b.leaf_tri = leaf_tri;
   if (n == 1) {
     b.nodes = nullptr;

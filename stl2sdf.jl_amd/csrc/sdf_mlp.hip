@@ -215,7 +215,8 @@ __global__ __launch_bounds__(WAVES * 64) void mlp_tile_radius_kernel(
     GridP g, TileDims td, MlpNorm nm, const uint16_t *__restrict__ afwd,
     const float *__restrict__ params, const float *__restrict__ w4b,
     const uint64_t *__restrict__ chunk_off, const uint32_t *__restrict__ chunk_tile,
-    int64_t nchunks, float scale, float delta, float h, float *__restrict__ r2out) {
+    int64_t nchunks, const uint64_t *__restrict__ nchunks_dev, float scale, float delta,
+    float h, float *__restrict__ r2out) {
   __shared__ u16x8 sA[3 * kMlpPacked / 8];
   __shared__ float sb[3 * 128 + 128 + 1];
   for (int i = threadIdx.x; i < 3 * kMlpPacked / 8; i += blockDim.x)
@@ -231,6 +232,7 @@ __global__ __launch_bounds__(WAVES * 64) void mlp_tile_radius_kernel(
   const int lane = threadIdx.x & 63;
   const int q = lane >> 4;
   const int64_t nw = (int64_t)gridDim.x * WAVES;
+  nchunks = chunk_count(nchunks, nchunks_dev);
   for (int64_t item = (int64_t)blockIdx.x * WAVES + (threadIdx.x >> 6); item < nchunks;
        item += nw) {
     const uint32_t tile = chunk_tile[item];
@@ -648,7 +650,8 @@ void mlp_tile_radius(hipStream_t s, const MlpDev &m, const MlpNorm &nm, const Gr
   hipLaunchKernelGGL((mlp_tile_radius_kernel<kNT, kWaves>), dim3(blocks), dim3(kWaves * 64), 0,
                      s, g, td, nm, (const uint16_t *)m.afwd, (const float *)m.params,
                      (const float *)m.w4b, (const uint64_t *)b.chunk_off,
-                     (const uint32_t *)b.chunk_tile, b.nchunks, scale, delta, (float)g.h, r2);
+                     (const uint32_t *)b.chunk_tile, b.nchunks, b.nchunks_dev, scale, delta,
+                     (float)g.h, r2);
   SDF_HIP(hipGetLastError());
 }
 

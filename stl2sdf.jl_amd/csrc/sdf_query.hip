@@ -97,10 +97,11 @@ __global__ __launch_bounds__(kTilePts) void dist_chunks_kernel(
     GridP g, TileDims td, const Tri64 *__restrict__ tri,
     const uint64_t *__restrict__ tile_off, const uint32_t *__restrict__ tile_tris,
     const uint64_t *__restrict__ chunk_off, const uint32_t *__restrict__ chunk_tile,
-    int64_t nchunks, const uint32_t *__restrict__ only, float *__restrict__ d2buf) {
+    int64_t nchunks, const uint64_t *__restrict__ nchunks_dev,
+    const uint32_t *__restrict__ only, float *__restrict__ d2buf) {
   __shared__ float s_pre[kTriPreFloats][kTilePts];
   const int64_t item = xcd_remap(blockIdx.x, gridDim.x);
-  if (item >= nchunks) return;
+  if (item >= chunk_count(nchunks, nchunks_dev)) return;
   const int lane = threadIdx.x;
   const int64_t tile = chunk_tile[item];
   const uint64_t first = chunk_off[tile];
@@ -160,12 +161,13 @@ __global__ __launch_bounds__(kTilePts) void dist_chunks_learned_kernel(
     GridP g, TileDims td, const Tri64 *__restrict__ tri,
     const uint64_t *__restrict__ tile_off, const uint32_t *__restrict__ tile_tris,
     const uint64_t *__restrict__ chunk_off, const uint32_t *__restrict__ chunk_tile,
-    int64_t nchunks, const float *__restrict__ r2buf, float *__restrict__ d2buf,
+    int64_t nchunks, const uint64_t *__restrict__ nchunks_dev,
+    const float *__restrict__ r2buf, float *__restrict__ d2buf,
     unsigned long long *__restrict__ stats) {
   __shared__ float s_pre[kTriPreFloats][kTilePts];
   __shared__ float s_box[6][kTilePts];
   const int64_t item = xcd_remap(blockIdx.x, gridDim.x);
-  if (item >= nchunks) return;
+  if (item >= chunk_count(nchunks, nchunks_dev)) return;
   const int lane = threadIdx.x;
   const int64_t tile = chunk_tile[item];
   const uint64_t first = chunk_off[tile];
@@ -241,11 +243,12 @@ __global__ __launch_bounds__(kTilePts) void dist_chunks_learned_kernel(
 // the fallback list (packed first_item * 64 + lane); counts them in stats[0].
 __global__ __launch_bounds__(256) void learned_check_kernel(
     const uint64_t *__restrict__ chunk_off, const uint32_t *__restrict__ chunk_tile,
-    int64_t nchunks, const float *__restrict__ r2buf, const float *__restrict__ d2buf,
+    int64_t nchunks, const uint64_t *__restrict__ nchunks_dev,
+    const float *__restrict__ r2buf, const float *__restrict__ d2buf,
     float h2, uint64_t *__restrict__ list, unsigned long long *__restrict__ stats) {
   const int64_t item = blockIdx.x * 4LL + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  if (item >= nchunks) return;
+  if (item >= chunk_count(nchunks, nchunks_dev)) return;
   const uint64_t first = chunk_off[chunk_tile[item]];
   if ((uint64_t)item != first) return;
   const float R2 = r2buf[first * kTilePts + lane];
@@ -335,7 +338,8 @@ __global__ __launch_bounds__(64) void fused_lattice_kernel(
     const uint32_t *__restrict__ meta, const LineInline *__restrict__ inl,
     int64_t base1, int64_t base2, int do_sign, FusedOut out,
     Pending *__restrict__ pend, uint32_t *npend, uint32_t cap,
-    Pending *__restrict__ pend0, uint32_t *npend0, uint32_t cap0, uint32_t *err) {
+    Pending *__restrict__ pend0, uint32_t *npend0, uint32_t cap0, uint32_t *err,
+    int64_t chunk_cap) {
   TileDims bd;
   bd.tx = sd.sx;
   bd.ty = sd.sy;
@@ -379,7 +383,7 @@ __global__ __launch_bounds__(64) void fused_lattice_kernel(
     if (d2buf) {
       const int64_t tile = ((int64_t)(k / kTileZ) * td.ty + j / kTileY) * td.tx + i / kTileX;
       const uint64_t c0 = chunk_off[tile];
-      if (chunk_off[tile + 1] > c0) {
+      if (chunk_off[tile + 1] > c0 && (int64_t)c0 < chunk_cap) {
         const int tl = (i & (kTileX - 1)) + kTileX * ((j & (kTileY - 1)) + kTileY * (k & (kTileZ - 1)));
         const float best = d2buf[c0 * kTilePts + tl];
         if (best < __builtin_huge_valf()) {
@@ -542,9 +546,12 @@ __device__ inline void item_point(const GridP &g, const double *pts, int64_t idx
 constexpr int kWaveStack = 2048;
 constexpr int kWaveHits = 1024;
 
+constexpr int kRayWaves = 8192;  // persistent grid of the ray stage (waves)
+
 __global__ __launch_bounds__(64) void ray_wave_kernel(
     GridP g, const double *__restrict__ pts, const Tri64 *__restrict__ tri, Bvh bvh,
-    RayParams rp, const Pending *__restrict__ items, uint32_t nitems, int r0, int r1,
+    RayParams rp, const Pending *__restrict__ items, uint32_t nitems,
+    const uint32_t *__restrict__ nitems_dev, int r0, int r1,
     unsigned long long *__restrict__ bits, uint32_t *__restrict__ fb, uint32_t *nfb,
     uint32_t fb_cap, uint32_t *err) {
   __shared__ int32_t stack[kWaveStack];
@@ -553,6 +560,7 @@ __global__ __launch_bounds__(64) void ray_wave_kernel(
   const int lane = threadIdx.x;
   const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   const int nr = r1 - r0;
+  if (nitems_dev && *nitems_dev < nitems) nitems = *nitems_dev;
   const int64_t total = (int64_t)nitems * nr;
   for (int64_t w = blockIdx.x; w < total; w += gridDim.x) {
     const uint32_t it = (uint32_t)(w / nr);
@@ -662,10 +670,11 @@ __global__ __launch_bounds__(64) void ray_wave_kernel(
 __global__ __launch_bounds__(256) void ray_bits_brute_kernel(
     GridP g, const double *__restrict__ pts, const Tri64 *__restrict__ tri,
     int64_t nel, RayParams rp, const Pending *__restrict__ items,
-    const uint32_t *__restrict__ fb, uint32_t nfb,
+    const uint32_t *__restrict__ fb, uint32_t nfb, const uint32_t *__restrict__ nfb_dev,
     unsigned long long *__restrict__ bits, uint32_t *err) {
   __shared__ double s_t[kHitCap];
   __shared__ int s_n, s_cnt;
+  if (nfb_dev && *nfb_dev < nfb) nfb = *nfb_dev;
   for (uint32_t q = blockIdx.x; q < nfb; q += gridDim.x) {
     const uint32_t it = fb[q] / 64u;
     const int r = (int)(fb[q] % 64u);
@@ -681,12 +690,13 @@ __global__ __launch_bounds__(256) void ray_bits_brute_kernel(
 // parities.  final_phase == 0: only rays below min_rays are available;
 // items that need the batch loop are appended to `next` with their state.
 __global__ void vote_kernel(RayParams rp, const Pending *__restrict__ items,
-                            uint32_t nitems,
+                            uint32_t nitems, const uint32_t *__restrict__ nitems_dev,
                             const unsigned long long *__restrict__ bits,
                             int final_phase, FusedOut out,
                             Pending *__restrict__ next, uint32_t *nnext,
                             uint32_t *__restrict__ wind, uint32_t *nwind,
                             double *__restrict__ wconf) {
+  if (nitems_dev && *nitems_dev < nitems) nitems = *nitems_dev;
   for (uint32_t it = blockIdx.x * blockDim.x + threadIdx.x; it < nitems;
        it += gridDim.x * blockDim.x) {
     Pending P = items[it];
@@ -738,10 +748,11 @@ __global__ void vote_kernel(RayParams rp, const Pending *__restrict__ items,
 __global__ __launch_bounds__(256) void winding_partial_kernel(
     GridP g, const double *__restrict__ pts, const Tri64 *__restrict__ tri,
     int64_t nel, const Pending *__restrict__ items,
-    const uint32_t *__restrict__ wind, uint32_t nwind, int chunks,
-    int64_t chunk, double *__restrict__ partial) {
+    const uint32_t *__restrict__ wind, uint32_t nwind, const uint32_t *__restrict__ nwind_dev,
+    int chunks, int64_t chunk, double *__restrict__ partial) {
 #pragma clang fp contract(off)
   __shared__ double s_red[256];
+  if (nwind_dev && *nwind_dev < nwind) nwind = *nwind_dev;
   const int64_t nblk = (int64_t)nwind * chunks;
   for (int64_t q = blockIdx.x; q < nblk; q += gridDim.x) {
     const uint32_t w = (uint32_t)(q / chunks);
@@ -782,9 +793,10 @@ __global__ __launch_bounds__(256) void winding_partial_kernel(
 
 __global__ void winding_final_kernel(const Pending *__restrict__ items,
                                      const uint32_t *__restrict__ wind,
-                                     uint32_t nwind, int chunks,
-                                     const double *__restrict__ partial,
+                                     uint32_t nwind, const uint32_t *__restrict__ nwind_dev,
+                                     int chunks, const double *__restrict__ partial,
                                      FusedOut out) {
+  if (nwind_dev && *nwind_dev < nwind) nwind = *nwind_dev;
   for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < nwind;
        w += gridDim.x * blockDim.x) {
     double tot = 0.0;
@@ -812,6 +824,13 @@ __global__ void dist_points_bvh_kernel(const double *__restrict__ pts,
   }
 }
 
+__global__ void fill_bits_kernel(unsigned long long *p, uint32_t cap,
+                                 const uint32_t *__restrict__ n_dev) {
+  uint32_t n = *n_dev < cap ? *n_dev : cap;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    p[i] = 0ull;
+}
+
 __global__ void init_pending_all_kernel(Pending *pend, int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
@@ -832,7 +851,7 @@ void launch_dist_chunks(hipStream_t s, const GridP &g, const Tri64 *tri,
   TileDims td = tile_dims(g);
   hipLaunchKernelGGL(dist_chunks_kernel, dim3((unsigned)b.nchunks), dim3(kTilePts), 0,
                      s, g, td, tri, b.tile_off, b.tile_tris, b.chunk_off, b.chunk_tile,
-                     b.nchunks, only, d2buf);
+                     b.nchunks, b.nchunks_dev, only, d2buf);
   SDF_HIP(hipGetLastError());
 }
 
@@ -843,10 +862,10 @@ void launch_dist_chunks_learned(hipStream_t s, const GridP &g, const Tri64 *tri,
   TileDims td = tile_dims(g);
   hipLaunchKernelGGL(dist_chunks_learned_kernel, dim3((unsigned)b.nchunks), dim3(kTilePts), 0,
                      s, g, td, tri, b.tile_off, b.tile_tris, b.chunk_off, b.chunk_tile,
-                     b.nchunks, r2buf, d2buf, stats);
+                     b.nchunks, b.nchunks_dev, r2buf, d2buf, stats);
   const float hf = (float)g.h;
   hipLaunchKernelGGL(learned_check_kernel, dim3((unsigned)((b.nchunks + 3) / 4)), dim3(256),
-                     0, s, b.chunk_off, b.chunk_tile, b.nchunks, r2buf,
+                     0, s, b.chunk_off, b.chunk_tile, b.nchunks, b.nchunks_dev, r2buf,
                      (const float *)d2buf, hf * hf, list, stats);
   SDF_HIP(hipGetLastError());
 }
@@ -881,7 +900,7 @@ void launch_fused_lattice(hipStream_t s, const GridP &g, const Tri64 *tri,
                      (const LineFast *)b.line_fast, (const uint32_t *)b.line_meta,
                      (const LineInline *)b.line_inline,
                      b.line_base[1], b.line_base[2], do_sign ? 1 : 0, out, pend, npend,
-                     pend_cap, pend0, npend0, pend0_cap, err);
+                     pend_cap, pend0, npend0, pend0_cap, err, b.nchunks);
   SDF_HIP(hipGetLastError());
 }
 
@@ -895,49 +914,32 @@ static int blocks_for(int64_t n, int t) {
 // Decide pending items with the adaptive vote: rays [r0, r1) by LBVH
 // traversal, overflowing rays by brute force, then the vote replay.
 void resolve_pending(hipStream_t s, Arena &ar, const GridP &g, const double *pts,
-                     const Tri64 *tri, int64_t nel, const Bvh &bvh,
-                     const RayParams &rp, const Pending *items, uint32_t nitems,
-                     int r0, int r1, int final_phase, FusedOut out,
-                     Pending *next, uint32_t *d_counts, uint32_t *err,
-                     uint32_t *n_next_host, uint32_t *n_wind_host) {
-  // d_counts: [0] fallback count, [1] next count, [2] winding count
-  if (nitems == 0) {
-    *n_next_host = 0;
-    *n_wind_host = 0;
-    return;
-  }
+                     const Tri64 *tri, int64_t nel, const Bvh &bvh, const RayParams &rp,
+                     const Pending *items, const uint32_t *n_dev, uint32_t cap, int r0,
+                     int r1, int final_phase, FusedOut out, Pending *next,
+                     uint32_t *d_counts, uint32_t wind_cap, uint32_t *err) {
+  // d_counts: [0] fallback rays, [1] next items, [2] winding items
+  if (cap == 0) return;
   if (r1 > kNumDirs) r1 = kNumDirs;
-  unsigned long long *bits = ar.get<unsigned long long>("res_bits", nitems);
-  launch_fill_u64(s, (uint64_t *)bits, nitems, 0ull);
   launch_fill_u32(s, d_counts, 4, 0u);
-  uint32_t fb_cap = nitems * 4u + 1024u;
-  uint32_t *fb = ar.get<uint32_t>("res_fb", fb_cap);
+  unsigned long long *bits = ar.get<unsigned long long>(ar.tmp("res_bits"), cap);
+  hipLaunchKernelGGL(fill_bits_kernel, dim3(256), dim3(256), 0, s, bits, cap, n_dev);
+  const uint32_t fb_cap = cap * 4u + 1024u;
+  uint32_t *fb = ar.get<uint32_t>(ar.tmp("res_fb"), fb_cap);
   if (r1 > r0) {
-    const int64_t work = (int64_t)nitems * (r1 - r0);
-    const unsigned nblk = (unsigned)(work < (1 << 20) ? work : (1 << 20));
-    hipLaunchKernelGGL(ray_wave_kernel, dim3(nblk), dim3(64), 0, s, g, pts, tri, bvh,
-                       rp, items, nitems, r0, r1, bits, fb, d_counts, fb_cap, err);
+    hipLaunchKernelGGL(ray_wave_kernel, dim3(kRayWaves), dim3(64), 0, s, g, pts, tri, bvh, rp,
+                       items, cap, n_dev, r0, r1, bits, fb, d_counts, fb_cap, err);
+    hipLaunchKernelGGL(ray_bits_brute_kernel, dim3(1024), dim3(256), 0, s, g, pts, tri, nel, rp,
+                       items, (const uint32_t *)fb, fb_cap, (const uint32_t *)d_counts, bits,
+                       err);
     SDF_HIP(hipGetLastError());
-    uint32_t nfb = 0;
-    SDF_HIP(hipMemcpyAsync(&nfb, d_counts, 4, hipMemcpyDeviceToHost, s));
-    SDF_HIP(hipStreamSynchronize(s));
-    if (nfb > fb_cap) nfb = fb_cap;
-    if (nfb > 0) {
-      hipLaunchKernelGGL(ray_bits_brute_kernel, dim3(nfb < 65536u ? nfb : 65536u),
-                         dim3(256), 0, s, g, pts, tri, nel, rp, items, fb, nfb,
-                         bits, err);
-      SDF_HIP(hipGetLastError());
-    }
   }
-  uint32_t *wind = ar.get<uint32_t>("res_wind", nitems);
-  hipLaunchKernelGGL(vote_kernel, dim3(blocks_for(nitems, 256)), dim3(256), 0, s,
-                     rp, items, nitems, bits, final_phase, out, next,
-                     d_counts + 1, wind, d_counts + 2, (double *)nullptr);
+  uint32_t *wind = ar.get<uint32_t>(ar.tmp("res_wind"), cap);
+  hipLaunchKernelGGL(vote_kernel, dim3(1024), dim3(256), 0, s, rp, items, cap, n_dev, bits,
+                     final_phase, out, next, d_counts + 1, wind, d_counts + 2,
+                     (double *)nullptr);
   SDF_HIP(hipGetLastError());
-  uint32_t cnt[4] = {0, 0, 0, 0};
-  SDF_HIP(hipMemcpyAsync(cnt, d_counts, 16, hipMemcpyDeviceToHost, s));
-  SDF_HIP(hipStreamSynchronize(s));
-  if (cnt[2] > 0) {
+  if (rp.use_winding && wind_cap > 0) {
     int64_t chunk = 4096;
     int chunks = (int)((nel + chunk - 1) / chunk);
     if (chunks < 1) chunks = 1;
@@ -945,19 +947,15 @@ void resolve_pending(hipStream_t s, Arena &ar, const GridP &g, const double *pts
       chunks = 1024;
       chunk = (nel + chunks - 1) / chunks;
     }
-    double *partial = ar.get<double>("res_wpartial", (size_t)cnt[2] * chunks);
-    int64_t nblk = (int64_t)cnt[2] * chunks;
-    hipLaunchKernelGGL(winding_partial_kernel,
-                       dim3((unsigned)(nblk < 1048576 ? nblk : 1048576)), dim3(256),
-                       0, s, g, pts, tri, nel, items, wind, cnt[2], chunks, chunk,
-                       partial);
-    SDF_HIP(hipGetLastError());
-    hipLaunchKernelGGL(winding_final_kernel, dim3(blocks_for(cnt[2], 256)),
-                       dim3(256), 0, s, items, wind, cnt[2], chunks, partial, out);
+    double *partial = ar.get<double>(ar.tmp("res_wpartial"), (size_t)wind_cap * chunks);
+    hipLaunchKernelGGL(winding_partial_kernel, dim3(4096), dim3(256), 0, s, g, pts, tri, nel,
+                       items, (const uint32_t *)wind, wind_cap,
+                       (const uint32_t *)(d_counts + 2), chunks, chunk, partial);
+    hipLaunchKernelGGL(winding_final_kernel, dim3(64), dim3(256), 0, s, items,
+                       (const uint32_t *)wind, wind_cap, (const uint32_t *)(d_counts + 2),
+                       chunks, (const double *)partial, out);
     SDF_HIP(hipGetLastError());
   }
-  *n_next_host = cnt[1];
-  *n_wind_host = cnt[2];
 }
 
 void launch_dist_points_bvh(hipStream_t s, const double *pts, int64_t npts,
