@@ -464,11 +464,11 @@ __global__ void line_bin_big_kernel(const Tri64 *__restrict__ tri, GridP g,
 }
 
 __global__ void tile_chunks_kernel(const uint64_t *__restrict__ tile_off,
-                                   int64_t ntiles, uint32_t *__restrict__ nchunk) {
+                                   int64_t ntiles, int clen, uint32_t *__restrict__ nchunk) {
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < ntiles;
        t += (int64_t)gridDim.x * blockDim.x) {
     const uint64_t c = tile_off[t + 1] - tile_off[t];
-    nchunk[t] = (uint32_t)((c + kTilePts - 1) / kTilePts);
+    nchunk[t] = (uint32_t)((c + clen - 1) / clen);
   }
 }
 
@@ -732,12 +732,14 @@ void build_tile_bins(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
   tile_bins_finish(s, ar, tri, nel, g, tb, b);
 }
 
-void build_dist_chunks(hipStream_t s, Arena &ar, const GridP &g, Bins &b, int64_t cap) {
+void build_dist_chunks(hipStream_t s, Arena &ar, const GridP &g, Bins &b, int64_t cap,
+                       int chunk_len) {
+  if (chunk_len < 1 || chunk_len > kTilePts) chunk_len = kTilePts;
   TileDims td = tile_dims(g);
   uint32_t *nch = ar.get<uint32_t>("chunk_cnt", td.ntiles + 1);
   uint64_t *coff = ar.get<uint64_t>("chunk_off", td.ntiles + 2);
   hipLaunchKernelGGL(tile_chunks_kernel, dim3(grid_for(td.ntiles, 256)), dim3(256),
-                     0, s, (const uint64_t *)b.tile_off, td.ntiles, nch);
+                     0, s, (const uint64_t *)b.tile_off, td.ntiles, chunk_len, nch);
   SDF_HIP(hipGetLastError());
   scan_counts(s, ar, nch, coff, td.ntiles);
   if (cap < 0) cap = (int64_t)read_u64(s, coff + td.ntiles);
@@ -751,6 +753,7 @@ void build_dist_chunks(hipStream_t s, Arena &ar, const GridP &g, Bins &b, int64_
   b.chunk_tile = items;
   b.nchunks = cap;
   b.nchunks_dev = coff + td.ntiles;
+  b.chunk_len = chunk_len;
 }
 
 void line_hits_begin(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,

@@ -405,4 +405,19 @@ __device__ inline int64_t chunk_count(int64_t cap, const uint64_t *dev) {
   return n < cap ? n : cap;
 }
 
+// Sign bricks: 8 x 8 lattice columns x kSignZ points per 64-lane wave.
+constexpr int kSignZ = 4;
+
+struct SignDims {
+  int32_t sx, sy, sz;
+};
+
+__host__ __device__ inline SignDims sign_dims(const GridP &g) {
+  SignDims d;
+  d.sx = (g.nx + 7) / 8;
+  d.sy = (g.ny + 7) / 8;
+  d.sz = (g.nz + kSignZ - 1) / kSignZ;
+  return d;
+}
+
 }  // namespace sdf

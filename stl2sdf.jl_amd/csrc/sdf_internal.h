@@ -134,7 +134,71 @@ struct Bins {
   // scans).  A count above its capacity means the lists were truncated and
   // the call is repeated with larger capacities.
   const uint64_t *npairs_dev = nullptr, *nhits_dev = nullptr, *nchunks_dev = nullptr;
+  int32_t chunk_len = kTilePts;  // triangles per distance work item (<= 64)
 };
+
+struct FusedOut {
+  double *sdf = nullptr;    // dist * sign (nullable)
+  double *dist = nullptr;   // unsigned distance (nullable)
+  double *signs = nullptr;  // +-1 (nullable)
+  double *conf = nullptr;   // confidence (nullable)
+};
+
+// Most jobs (meshes) one queue launch serves.
+constexpr int kMaxJobs = 256;
+
+// Distance work of one lattice: items (tile, chunk of <= clen triangles).
+struct DistJob {
+  GridP g;
+  TileDims td;
+  const Tri64 *tri;
+  const uint64_t *tile_off;
+  const uint32_t *tile_tris;
+  const uint64_t *chunk_off;
+  const uint32_t *chunk_tile;
+  int64_t nchunks;               // capacity
+  const uint64_t *nchunks_dev;   // actual count (null: the capacity)
+  const uint32_t *only;          // per first item: 0 skips the tile (null: all)
+  float *d2buf;
+  int32_t clen;
+  int32_t pad_;
+};
+
+// Sign / combine work of one lattice: 8 x 8 x kSignZ bricks.
+struct FusedJob {
+  GridP g;
+  TileDims td;
+  TileDims bd;  // brick counts
+  const uint64_t *chunk_off;
+  const float *d2buf;
+  const uint64_t *line_off;
+  const LineFast *fast;
+  const uint32_t *meta;
+  const LineInline *inl;
+  int64_t base1, base2;
+  FusedOut out;
+  Pending *pend;
+  uint32_t *npend;
+  Pending *pend0;
+  uint32_t *npend0;
+  uint32_t *err;
+  int64_t chunk_cap;
+  int64_t nbricks;  // launch size of the brick order
+  uint32_t cap, cap0;
+  int32_t do_sign;
+  int32_t pad_;
+};
+DistJob make_dist_job(const GridP &g, const Tri64 *tri, const Bins &b, float *d2buf,
+                      const uint32_t *only);
+FusedJob make_fused_job(const GridP &g, const Bins &b, const float *d2buf, bool do_sign,
+                        FusedOut out, Pending *pend, uint32_t *npend, uint32_t pend_cap,
+                        Pending *pend0, uint32_t *npend0, uint32_t pend0_cap, uint32_t *err);
+// One persistent launch over the items of njobs jobs (device tables).
+void launch_dist_queue(hipStream_t s, const DistJob *d_jobs, int njobs, uint64_t *d_prefix,
+                       unsigned long long *d_counter, int blocks, int grab);
+void launch_fused_queue(hipStream_t s, const FusedJob *d_jobs, int njobs,
+                        const uint64_t *d_prefix, unsigned long long *d_counter, int blocks,
+                        int grab);
 
 void build_tile_bins(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
                      const GridP &g, double radius, Bins &b);
@@ -161,16 +225,10 @@ void line_hits_finish(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
                       const GridP &g, LineBuild &lb, Bins &b, int64_t cap = -1);
 void build_line_hits(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
                      const GridP &g, Bins &b);
-void build_dist_chunks(hipStream_t s, Arena &ar, const GridP &g, Bins &b, int64_t cap = -1);
+void build_dist_chunks(hipStream_t s, Arena &ar, const GridP &g, Bins &b, int64_t cap = -1,
+                       int chunk_len = kTilePts);
 
 // query side
-struct FusedOut {
-  double *sdf = nullptr;    // dist * sign (nullable)
-  double *dist = nullptr;   // unsigned distance (nullable)
-  double *signs = nullptr;  // +-1 (nullable)
-  double *conf = nullptr;   // confidence (nullable)
-};
-
 // Distance work items -> squared distances per point of non-empty tiles
 // (d2buf: 64 floats per work item slot, indexed by the tile's first item).
 void launch_dist_chunks(hipStream_t s, const GridP &g, const Tri64 *tri,
@@ -221,7 +279,8 @@ void resolve_pending(hipStream_t s, Arena &ar, const GridP &g, const double *pts
                      const Tri64 *tri, int64_t nel, const Bvh &bvh, const RayParams &rp,
                      const Pending *items, const uint32_t *n_dev, uint32_t cap, int r0,
                      int r1, int final_phase, FusedOut out, Pending *next,
-                     uint32_t *d_counts, uint32_t wind_cap, uint32_t *err);
+                     uint32_t *d_counts, uint32_t wind_cap, uint32_t *err,
+                     int stack = 2048);
 
 void launch_dist_points_bvh(hipStream_t s, const double *pts, int64_t npts,
                             const Tri64 *tri, const Bvh &bvh, double h,

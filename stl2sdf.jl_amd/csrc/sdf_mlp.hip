@@ -663,7 +663,7 @@ int64_t mlp_fit_samples(hipStream_t s, Arena &ar, const GridP &gc, const MlpNorm
                         const double *dc, double hc, float4 **out, int64_t *nband) {
   const int64_t n = (int64_t)gc.nx * gc.ny * gc.nz;
   unsigned long long *cnt = ar.get<unsigned long long>("fit_cnt", 2);
-  SDF_HIP(hipMemsetAsync(cnt, 0, 2 * sizeof(unsigned long long), s));
+  launch_fill_u64(s, (uint64_t *)cnt, 2, 0ull);
   int blocks = blocks_for(n, 256);
   if (blocks > 4096) blocks = 4096;
   hipLaunchKernelGGL(fit_count_kernel, dim3(blocks), dim3(256), 0, s, dc, n, hc, cnt);

@@ -93,15 +93,22 @@ __device__ inline bool inline_parity(const LineInline &r, const double *xs,
   return true;
 }
 
-__global__ __launch_bounds__(kTilePts) void dist_chunks_kernel(
-    GridP g, TileDims td, const Tri64 *__restrict__ tri,
-    const uint64_t *__restrict__ tile_off, const uint32_t *__restrict__ tile_tris,
-    const uint64_t *__restrict__ chunk_off, const uint32_t *__restrict__ chunk_tile,
-    int64_t nchunks, const uint64_t *__restrict__ nchunks_dev,
-    const uint32_t *__restrict__ only, float *__restrict__ d2buf) {
-  __shared__ float s_pre[kTriPreFloats][kTilePts];
-  const int64_t item = xcd_remap(blockIdx.x, gridDim.x);
-  if (item >= chunk_count(nchunks, nchunks_dev)) return;
+// One distance work item (tile, chunk of <= clen triangles) for the 64-lane
+// block: the chunk's triangles are staged in LDS as precomputed records and
+// every lane evaluates its tile point against all of them.  Uniform per
+// block (contains a barrier).
+__device__ inline void dist_item(const DistJob &J, int64_t item,
+                                 float (&s_pre)[kTriPreFloats][kTilePts]) {
+  const GridP &g = J.g;
+  const TileDims &td = J.td;
+  const Tri64 *__restrict__ tri = J.tri;
+  const uint64_t *__restrict__ tile_off = J.tile_off;
+  const uint32_t *__restrict__ tile_tris = J.tile_tris;
+  const uint64_t *__restrict__ chunk_off = J.chunk_off;
+  float *__restrict__ d2buf = J.d2buf;
+  if (item >= chunk_count(J.nchunks, J.nchunks_dev)) return;
+  const uint32_t *__restrict__ chunk_tile = J.chunk_tile;
+  const uint32_t *__restrict__ only = J.only;
   const int lane = threadIdx.x;
   const int64_t tile = chunk_tile[item];
   const uint64_t first = chunk_off[tile];
@@ -112,8 +119,8 @@ __global__ __launch_bounds__(kTilePts) void dist_chunks_kernel(
   const int32_t tz = (int32_t)(tile / ((int64_t)td.tx * td.ty));
   const int32_t i0 = tx * kTileX, j0 = ty * kTileY, k0 = tz * kTileZ;
   const Vec3d T0 = lattice_point(g, i0, j0, k0);
-  const uint64_t b = tile_off[tile] + (uint64_t)q * kTilePts;
-  const uint64_t e = min(b + (uint64_t)kTilePts, tile_off[tile + 1]);
+  const uint64_t b = tile_off[tile] + (uint64_t)q * J.clen;
+  const uint64_t e = min(b + (uint64_t)J.clen, tile_off[tile + 1]);
   const int n = (int)(e - b);
   if (lane < n) {
     const Tri64 t = tri[tile_tris[b + lane]];
@@ -146,6 +153,53 @@ __global__ __launch_bounds__(kTilePts) void dist_chunks_kernel(
     *slot = best;
   else
     atomicMin((unsigned int *)slot, __float_as_uint(best));
+  __syncthreads();  // s_pre is reused by the block's next item
+}
+
+__global__ __launch_bounds__(kTilePts) void dist_chunks_kernel(DistJob J) {
+  __shared__ float s_pre[kTriPreFloats][kTilePts];
+  dist_item(J, xcd_remap(blockIdx.x, gridDim.x), s_pre);
+}
+
+// Persistent work queue over the distance items of many jobs: each block
+// takes `grab` consecutive units of the concatenated item space with one
+// atomic, and maps a unit to (job, item) through the job prefix sums.
+__global__ __launch_bounds__(kTilePts) void dist_queue_kernel(
+    const DistJob *__restrict__ jobs, int njobs, const uint64_t *__restrict__ prefix,
+    unsigned long long *__restrict__ counter, int grab) {
+  __shared__ float s_pre[kTriPreFloats][kTilePts];
+  __shared__ uint64_t s_pref[kMaxJobs + 1];
+  __shared__ unsigned long long s_base;
+  for (int i = threadIdx.x; i <= njobs; i += blockDim.x) s_pref[i] = prefix[i];
+  __syncthreads();
+  const uint64_t total = s_pref[njobs];
+  while (true) {
+    if (threadIdx.x == 0) s_base = atomicAdd(counter, (unsigned long long)grab);
+    __syncthreads();
+    const uint64_t base = s_base;
+    __syncthreads();
+    if (base >= total) break;
+    const uint64_t end = base + grab < total ? base + grab : total;
+    int j = 0;
+    for (uint64_t u = base; u < end; u++) {
+      while (j + 1 < njobs && s_pref[j + 1] <= u) j++;
+      dist_item(jobs[j], (int64_t)(u - s_pref[j]), s_pre);
+    }
+  }
+}
+
+// Prefix sums of the jobs' actual item counts (one thread; <= kMaxJobs jobs).
+__global__ void dist_prefix_kernel(const DistJob *__restrict__ jobs, int njobs,
+                                   uint64_t *__restrict__ prefix,
+                                   unsigned long long *__restrict__ counter) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  uint64_t acc = 0;
+  prefix[0] = 0;
+  for (int j = 0; j < njobs; j++) {
+    acc += (uint64_t)chunk_count(jobs[j].nchunks, jobs[j].nchunks_dev);
+    prefix[j + 1] = acc;
+  }
+  *counter = 0ull;
 }
 
 // Learned-radius variant of dist_chunks_kernel.  Lane p carries a squared
@@ -161,7 +215,7 @@ __global__ __launch_bounds__(kTilePts) void dist_chunks_learned_kernel(
     GridP g, TileDims td, const Tri64 *__restrict__ tri,
     const uint64_t *__restrict__ tile_off, const uint32_t *__restrict__ tile_tris,
     const uint64_t *__restrict__ chunk_off, const uint32_t *__restrict__ chunk_tile,
-    int64_t nchunks, const uint64_t *__restrict__ nchunks_dev,
+    int64_t nchunks, const uint64_t *__restrict__ nchunks_dev, int clen,
     const float *__restrict__ r2buf, float *__restrict__ d2buf,
     unsigned long long *__restrict__ stats) {
   __shared__ float s_pre[kTriPreFloats][kTilePts];
@@ -177,8 +231,8 @@ __global__ __launch_bounds__(kTilePts) void dist_chunks_learned_kernel(
   const int32_t tz = (int32_t)(tile / ((int64_t)td.tx * td.ty));
   const int32_t i0 = tx * kTileX, j0 = ty * kTileY, k0 = tz * kTileZ;
   const Vec3d T0 = lattice_point(g, i0, j0, k0);
-  const uint64_t b = tile_off[tile] + (uint64_t)q * kTilePts;
-  const uint64_t e = min(b + (uint64_t)kTilePts, tile_off[tile + 1]);
+  const uint64_t b = tile_off[tile] + (uint64_t)q * clen;
+  const uint64_t e = min(b + (uint64_t)clen, tile_off[tile + 1]);
   const int n = (int)(e - b);
   if (lane < n) {
     const Tri64 t = tri[tile_tris[b + lane]];
@@ -317,36 +371,31 @@ __global__ __launch_bounds__(256) void learned_fallback_kernel(
 // results (h when its distance tile lists no triangle), the six axis-ray
 // votes (certified fast path, else the reference's arithmetic), combine,
 // outputs; points whose votes are not unanimous go to the pending list.
-constexpr int kSignZ = 4;
 
-struct SignDims {
-  int32_t sx, sy, sz;
-};
-
-__host__ __device__ inline SignDims sign_dims(const GridP &g) {
-  SignDims d;
-  d.sx = (g.nx + 7) / 8;
-  d.sy = (g.ny + 7) / 8;
-  d.sz = (g.nz + kSignZ - 1) / kSignZ;
-  return d;
-}
-
-__global__ __launch_bounds__(64) void fused_lattice_kernel(
-    GridP g, TileDims td, SignDims sd, const uint64_t *__restrict__ chunk_off,
-    const float *__restrict__ d2buf, const uint64_t *__restrict__ line_off,
-    const LineHit *__restrict__ hits, const LineFast *__restrict__ fast,
-    const uint32_t *__restrict__ meta, const LineInline *__restrict__ inl,
-    int64_t base1, int64_t base2, int do_sign, FusedOut out,
-    Pending *__restrict__ pend, uint32_t *npend, uint32_t cap,
-    Pending *__restrict__ pend0, uint32_t *npend0, uint32_t cap0, uint32_t *err,
-    int64_t chunk_cap) {
-  TileDims bd;
-  bd.tx = sd.sx;
-  bd.ty = sd.sy;
-  bd.tz = sd.sz;
-  bd.ntiles = (int64_t)sd.sx * sd.sy * sd.sz;
+// One sign brick (8 x 8 x 4 points) of job J for the 64-lane block; m is
+// the brick's position in the brick launch order.
+__device__ inline void fused_brick(const FusedJob &J, int64_t m) {
+  const GridP &g = J.g;
+  const TileDims &td = J.td;
+  const uint64_t *__restrict__ chunk_off = J.chunk_off;
+  const float *__restrict__ d2buf = J.d2buf;
+  const uint64_t *__restrict__ line_off = J.line_off;
+  const LineFast *__restrict__ fast = J.fast;
+  const uint32_t *__restrict__ meta = J.meta;
+  const LineInline *__restrict__ inl = J.inl;
+  const int64_t base1 = J.base1, base2 = J.base2;
+  const int do_sign = J.do_sign;
+  const FusedOut out = J.out;
+  Pending *__restrict__ pend = J.pend;
+  uint32_t *npend = J.npend;
+  const uint32_t cap = J.cap;
+  Pending *__restrict__ pend0 = J.pend0;
+  uint32_t *npend0 = J.npend0;
+  const uint32_t cap0 = J.cap0;
+  uint32_t *err = J.err;
+  const int64_t chunk_cap = J.chunk_cap;
   int32_t bx, by, bz;
-  if (!brick_tile(xcd_remap(blockIdx.x, gridDim.x), bd, &bx, &by, &bz)) return;
+  if (!brick_tile(m, J.bd, &bx, &by, &bz)) return;
   const int lane = threadIdx.x;
   const int32_t i = bx * 8 + (lane & 7);
   const int32_t j = by * 8 + (lane >> 3);
@@ -455,6 +504,35 @@ __global__ __launch_bounds__(64) void fused_lattice_kernel(
   }
 }
 
+__global__ __launch_bounds__(64) void fused_lattice_kernel(FusedJob J) {
+  fused_brick(J, xcd_remap(blockIdx.x, gridDim.x));
+}
+
+// Persistent work queue over the sign bricks of many jobs (static brick
+// counts: prefix computed on the host).
+__global__ __launch_bounds__(64) void fused_queue_kernel(
+    const FusedJob *__restrict__ jobs, int njobs, const uint64_t *__restrict__ prefix,
+    unsigned long long *__restrict__ counter, int grab) {
+  __shared__ uint64_t s_pref[kMaxJobs + 1];
+  __shared__ unsigned long long s_base;
+  for (int i = threadIdx.x; i <= njobs; i += blockDim.x) s_pref[i] = prefix[i];
+  __syncthreads();
+  const uint64_t total = s_pref[njobs];
+  while (true) {
+    if (threadIdx.x == 0) s_base = atomicAdd(counter, (unsigned long long)grab);
+    __syncthreads();
+    const uint64_t base = s_base;
+    __syncthreads();
+    if (base >= total) break;
+    const uint64_t end = base + grab < total ? base + grab : total;
+    int j = 0;
+    for (uint64_t u = base; u < end; u++) {
+      while (j + 1 < njobs && s_pref[j + 1] <= u) j++;
+      fused_brick(jobs[j], (int64_t)(u - s_pref[j]));
+    }
+  }
+}
+
 // Parity of one ray over all triangles, one workgroup (reference:
 // check_ray_inside_bvh with a brute-force contact list).
 __device__ int block_ray_parity(const double *o, const double *d,
@@ -543,11 +621,11 @@ __device__ inline void item_point(const GridP &g, const double *pts, int64_t idx
 // when it is the smallest or its strict predecessor lies more than eps
 // below, equal values counting once.  Stack or hit-list overflow sends the
 // ray to the brute-force fallback.
-constexpr int kWaveStack = 2048;
 constexpr int kWaveHits = 1024;
 
 constexpr int kRayWaves = 8192;  // persistent grid of the ray stage (waves)
 
+template <int kWaveStack>
 __global__ __launch_bounds__(64) void ray_wave_kernel(
     GridP g, const double *__restrict__ pts, const Tri64 *__restrict__ tri, Bvh bvh,
     RayParams rp, const Pending *__restrict__ items, uint32_t nitems,
@@ -845,13 +923,41 @@ __global__ void init_pending_all_kernel(Pending *pend, int64_t n) {
 
 }  // namespace
 
+DistJob make_dist_job(const GridP &g, const Tri64 *tri, const Bins &b, float *d2buf,
+                      const uint32_t *only) {
+  DistJob J;
+  J.g = g;
+  J.td = tile_dims(g);
+  J.tri = tri;
+  J.tile_off = b.tile_off;
+  J.tile_tris = b.tile_tris;
+  J.chunk_off = b.chunk_off;
+  J.chunk_tile = b.chunk_tile;
+  J.nchunks = b.nchunks;
+  J.nchunks_dev = b.nchunks_dev;
+  J.only = only;
+  J.d2buf = d2buf;
+  J.clen = b.chunk_len;
+  J.pad_ = 0;
+  return J;
+}
+
 void launch_dist_chunks(hipStream_t s, const GridP &g, const Tri64 *tri,
                         const Bins &b, float *d2buf, const uint32_t *only) {
   if (b.nchunks <= 0) return;
-  TileDims td = tile_dims(g);
-  hipLaunchKernelGGL(dist_chunks_kernel, dim3((unsigned)b.nchunks), dim3(kTilePts), 0,
-                     s, g, td, tri, b.tile_off, b.tile_tris, b.chunk_off, b.chunk_tile,
-                     b.nchunks, b.nchunks_dev, only, d2buf);
+  hipLaunchKernelGGL(dist_chunks_kernel, dim3((unsigned)b.nchunks), dim3(kTilePts), 0, s,
+                     make_dist_job(g, tri, b, d2buf, only));
+  SDF_HIP(hipGetLastError());
+}
+
+void launch_dist_queue(hipStream_t s, const DistJob *d_jobs, int njobs, uint64_t *d_prefix,
+                       unsigned long long *d_counter, int blocks, int grab) {
+  if (njobs <= 0) return;
+  if (njobs > kMaxJobs) throw Error{-1, "too many jobs for one queue launch"};
+  hipLaunchKernelGGL(dist_prefix_kernel, dim3(1), dim3(64), 0, s, d_jobs, njobs, d_prefix,
+                     d_counter);
+  hipLaunchKernelGGL(dist_queue_kernel, dim3(blocks), dim3(kTilePts), 0, s, d_jobs, njobs,
+                     (const uint64_t *)d_prefix, d_counter, grab);
   SDF_HIP(hipGetLastError());
 }
 
@@ -862,7 +968,7 @@ void launch_dist_chunks_learned(hipStream_t s, const GridP &g, const Tri64 *tri,
   TileDims td = tile_dims(g);
   hipLaunchKernelGGL(dist_chunks_learned_kernel, dim3((unsigned)b.nchunks), dim3(kTilePts), 0,
                      s, g, td, tri, b.tile_off, b.tile_tris, b.chunk_off, b.chunk_tile,
-                     b.nchunks, b.nchunks_dev, r2buf, d2buf, stats);
+                     b.nchunks, b.nchunks_dev, (int)b.chunk_len, r2buf, d2buf, stats);
   const float hf = (float)g.h;
   hipLaunchKernelGGL(learned_check_kernel, dim3((unsigned)((b.nchunks + 3) / 4)), dim3(256),
                      0, s, b.chunk_off, b.chunk_tile, b.nchunks, b.nchunks_dev, r2buf,
@@ -880,27 +986,60 @@ void launch_learned_fallback(hipStream_t s, const GridP &g, const Tri64 *tri, co
   SDF_HIP(hipGetLastError());
 }
 
+FusedJob make_fused_job(const GridP &g, const Bins &b, const float *d2buf, bool do_sign,
+                        FusedOut out, Pending *pend, uint32_t *npend, uint32_t pend_cap,
+                        Pending *pend0, uint32_t *npend0, uint32_t pend0_cap, uint32_t *err) {
+  FusedJob J;
+  J.g = g;
+  J.td = tile_dims(g);
+  const SignDims sd = sign_dims(g);
+  J.bd.tx = sd.sx;
+  J.bd.ty = sd.sy;
+  J.bd.tz = sd.sz;
+  J.bd.ntiles = (int64_t)sd.sx * sd.sy * sd.sz;
+  J.chunk_off = b.chunk_off;
+  J.d2buf = d2buf;
+  J.line_off = b.line_off;
+  J.fast = b.line_fast;
+  J.meta = b.line_meta;
+  J.inl = b.line_inline;
+  J.base1 = b.line_base[1];
+  J.base2 = b.line_base[2];
+  J.out = out;
+  J.pend = pend;
+  J.npend = npend;
+  J.pend0 = pend0;
+  J.npend0 = npend0;
+  J.err = err;
+  J.chunk_cap = b.nchunks;
+  J.nbricks = J.td.ntiles > 0 ? brick_launch_size(J.bd) : 0;
+  J.cap = pend_cap;
+  J.cap0 = pend0_cap;
+  J.do_sign = do_sign ? 1 : 0;
+  J.pad_ = 0;
+  return J;
+}
+
 void launch_fused_lattice(hipStream_t s, const GridP &g, const Tri64 *tri,
                           const Bins &b, const float *d2buf, bool do_sign,
                           FusedOut out, Pending *pend, uint32_t *npend,
                           uint32_t pend_cap, Pending *pend0, uint32_t *npend0,
                           uint32_t pend0_cap, uint32_t *err) {
   (void)tri;
-  TileDims td = tile_dims(g);
-  if (td.ntiles <= 0) return;
-  const SignDims sd = sign_dims(g);
-  TileDims bd;
-  bd.tx = sd.sx;
-  bd.ty = sd.sy;
-  bd.tz = sd.sz;
-  bd.ntiles = (int64_t)sd.sx * sd.sy * sd.sz;
-  hipLaunchKernelGGL(fused_lattice_kernel, dim3((unsigned)brick_launch_size(bd)),
-                     dim3(64), 0, s, g, td, sd, (const uint64_t *)b.chunk_off,
-                     d2buf, (const uint64_t *)b.line_off, (const LineHit *)b.line_hits,
-                     (const LineFast *)b.line_fast, (const uint32_t *)b.line_meta,
-                     (const LineInline *)b.line_inline,
-                     b.line_base[1], b.line_base[2], do_sign ? 1 : 0, out, pend, npend,
-                     pend_cap, pend0, npend0, pend0_cap, err, b.nchunks);
+  const FusedJob J = make_fused_job(g, b, d2buf, do_sign, out, pend, npend, pend_cap, pend0,
+                                    npend0, pend0_cap, err);
+  if (J.nbricks <= 0) return;
+  hipLaunchKernelGGL(fused_lattice_kernel, dim3((unsigned)J.nbricks), dim3(64), 0, s, J);
+  SDF_HIP(hipGetLastError());
+}
+
+void launch_fused_queue(hipStream_t s, const FusedJob *d_jobs, int njobs,
+                        const uint64_t *d_prefix, unsigned long long *d_counter, int blocks,
+                        int grab) {
+  if (njobs <= 0) return;
+  if (njobs > kMaxJobs) throw Error{-1, "too many jobs for one queue launch"};
+  hipLaunchKernelGGL(fused_queue_kernel, dim3(blocks), dim3(64), 0, s, d_jobs, njobs, d_prefix,
+                     d_counter, grab);
   SDF_HIP(hipGetLastError());
 }
 
@@ -917,7 +1056,7 @@ void resolve_pending(hipStream_t s, Arena &ar, const GridP &g, const double *pts
                      const Tri64 *tri, int64_t nel, const Bvh &bvh, const RayParams &rp,
                      const Pending *items, const uint32_t *n_dev, uint32_t cap, int r0,
                      int r1, int final_phase, FusedOut out, Pending *next,
-                     uint32_t *d_counts, uint32_t wind_cap, uint32_t *err) {
+                     uint32_t *d_counts, uint32_t wind_cap, uint32_t *err, int stack) {
   // d_counts: [0] fallback rays, [1] next items, [2] winding items
   if (cap == 0) return;
   if (r1 > kNumDirs) r1 = kNumDirs;
@@ -927,8 +1066,14 @@ void resolve_pending(hipStream_t s, Arena &ar, const GridP &g, const double *pts
   const uint32_t fb_cap = cap * 4u + 1024u;
   uint32_t *fb = ar.get<uint32_t>(ar.tmp("res_fb"), fb_cap);
   if (r1 > r0) {
-    hipLaunchKernelGGL(ray_wave_kernel, dim3(kRayWaves), dim3(64), 0, s, g, pts, tri, bvh, rp,
-                       items, cap, n_dev, r0, r1, bits, fb, d_counts, fb_cap, err);
+    // LDS traversal stack depth (entries); a ray that overflows it is redone
+    // by the brute-force kernel, so the depth only trades occupancy
+    if (stack <= 1024)
+      hipLaunchKernelGGL(ray_wave_kernel<1024>, dim3(kRayWaves), dim3(64), 0, s, g, pts, tri,
+                         bvh, rp, items, cap, n_dev, r0, r1, bits, fb, d_counts, fb_cap, err);
+    else
+      hipLaunchKernelGGL(ray_wave_kernel<2048>, dim3(kRayWaves), dim3(64), 0, s, g, pts, tri,
+                         bvh, rp, items, cap, n_dev, r0, r1, bits, fb, d_counts, fb_cap, err);
     hipLaunchKernelGGL(ray_bits_brute_kernel, dim3(1024), dim3(256), 0, s, g, pts, tri, nel, rp,
                        items, (const uint32_t *)fb, fb_cap, (const uint32_t *)d_counts, bits,
                        err);

@@ -1,0 +1,76 @@
+"""Batched signed distance on the MI355X library (no counterpart in the
+reference, which computes one mesh per call).
+
+``SignedDistanceBatch(meshes, grids)`` uploads the meshes once and builds a
+batch in the library: per-mesh binning and LBVH chains run concurrently,
+the distance and sign stages of all meshes are one persistent work-queue
+launch each, and the whole pass is captured into a hipGraph that ``run()``
+replays.  Each result equals ``SignedDistances.signed_distance(mesh, grid)``.
+Device memory comes from torch (plumbing only).
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+
+INFO = ("meshes", "grid_points", "last_ms", "chunk_len", "ray_stack", "dist_blocks",
+        "dist_grab", "fused_blocks", "fused_grab", "candidates")
+
+
+class SignedDistanceBatch:
+    def __init__(self, meshes, grids, device=0, autotune=True):
+        import torch
+        if len(meshes) != len(grids) or not meshes:
+            raise ValueError("need one grid per mesh")
+        self.device = device
+        dev = torch.device(f"cuda:{device}")
+        self._X, self._I, self.out = [], [], []
+        for mesh, grid in zip(meshes, grids):
+            X = np.ascontiguousarray(np.asarray(mesh.X, np.float64).T)   # nnp x 3 = 3 x nnp col-major
+            I = np.ascontiguousarray(np.asarray(mesh.IEN, np.int64).T)
+            self._X.append(torch.from_numpy(X).to(dev))
+            self._I.append(torch.from_numpy(I).to(dev))
+            self.out.append(torch.empty(int(grid.ngp), dtype=torch.float64, device=dev))
+        n = len(meshes)
+        ptr = lambda ts: (C.c_void_p * n)(*[t.data_ptr() for t in ts])
+        nnp = np.array([m.X.shape[1] for m in meshes], np.int64)
+        nel = np.array([m.IEN.shape[1] for m in meshes], np.int64)
+        amin = np.concatenate([np.asarray(g.AABB_min, np.float64) for g in grids])
+        h = np.array([g.cell_size for g in grids], np.float64)
+        N = np.concatenate([np.asarray(g.N, np.int64) for g in grids])
+        L = _lib.lib()
+        c = _lib.ctx(device)
+        torch.cuda.synchronize(dev)
+        self._h = C.c_void_p()
+        _lib.check(L.sdf_batch_create(c, n, ptr(self._X), _lib.i64ptr(nnp), ptr(self._I),
+                                      _lib.i64ptr(nel), _lib.dptr(amin), _lib.dptr(h),
+                                      _lib.i64ptr(N), ptr(self.out), 1 if autotune else 0,
+                                      C.byref(self._h)))
+
+    def run(self):
+        _lib.check(_lib.lib().sdf_batch_run(self._h))
+
+    def info(self):
+        out = np.zeros(256)
+        n = C.c_int32()
+        _lib.check(_lib.lib().sdf_batch_info(self._h, _lib.dptr(out), 256, C.byref(n)))
+        d = dict(zip(INFO, out[:len(INFO)].tolist()))
+        k = int(d["candidates"])
+        d["tuning"] = [(int(out[len(INFO) + 2 * i]), float(out[len(INFO) + 2 * i + 1]))
+                       for i in range(k)]
+        return d
+
+    def results(self):
+        return [t.cpu().numpy() for t in self.out]
+
+    def close(self):
+        if self._h:
+            _lib.check(_lib.lib().sdf_batch_destroy(self._h))
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

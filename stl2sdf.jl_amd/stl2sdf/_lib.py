@@ -45,6 +45,12 @@ SIGNATURES = {
     "sdf_signed_distance_device": (C.c_int32, [_vp, _dp, C.c_double, _i64p, _vp]),
     "sdf_ray_directions": (C.c_int32, [_dp]),
     "sdf_mesh_set": (C.c_int32, [_vp, _dp, C.c_int64, _i64p, C.c_int64]),
+    "sdf_batch_create": (C.c_int32, [_vp, C.c_int32, C.POINTER(_vp), _i64p, C.POINTER(_vp),
+                                     _i64p, _dp, _dp, _i64p, C.POINTER(_vp), C.c_int32,
+                                     C.POINTER(_vp)]),
+    "sdf_batch_run": (C.c_int32, [_vp]),
+    "sdf_batch_info": (C.c_int32, [_vp, _dp, C.c_int32, _i32p]),
+    "sdf_batch_destroy": (C.c_int32, [_vp]),
     "sdf_mlp_set_params": (C.c_int32, [_vp, _fp]),
     "sdf_mlp_get_params": (C.c_int32, [_vp, _fp]),
     "sdf_mlp_forward": (C.c_int32, [_vp, _fp, C.c_int64, _fp]),

@@ -162,3 +162,20 @@ def write_stl_binary(path, X, IEN):
         f.write(b"binary stl written by stl2sdf synthetic".ljust(80, b" "))
         f.write(np.uint32(len(v)).tobytes())
         f.write(rec.tobytes())
+
+
+def batch_family(count=64, seed=2, tri_range=(2000, 500000), grid_points=(64, 96, 128, 192)):
+    """Seeded batch of star surfaces: triangle counts log-uniform in
+    tri_range, each with its own harmonic seed and a grid with a uniformly
+    chosen number of points on its longest axis.  Returns a list of
+    (X, IEN, points_on_longest_axis, info) with float32-rounded vertices."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for m in range(count):
+        tris = float(np.exp(rng.uniform(np.log(tri_range[0]), np.log(tri_range[1]))))
+        n = max(4, int(round((1.0 + np.sqrt(1.0 + 2.0 * tris)) / 2.0)))
+        g = int(rng.choice(grid_points))
+        X, IEN, info = star_surface(n, n, seed=int(rng.integers(0, 2 ** 31)))
+        info["target_tris"] = tris
+        out.append((to_float32(X), IEN, g, info))
+    return out
