@@ -29,6 +29,19 @@ sys.path.insert(0, os.path.join(REPO, "stl2sdf.jl_amd"))
 sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_PROFILE = os.path.join(REPO, "profiles", "fused_lattice_star512_hbm_traffic.md")
+
+
+def measured_traffic(workload):
+    """HBM bytes per launch of the fused kernel from the committed counter
+    summary (FETCH_SIZE / WRITE_SIZE passes, copy-calibrated); None when the
+    summary does not cover this workload."""
+    if workload != "star512" or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    for line in open(TRAFFIC_PROFILE):
+        if line.startswith("fused_lattice_kernel traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
 
 
 def log(msg):
@@ -249,8 +262,9 @@ def main():
         "counters": {"pending_points": cnt[0], "tile_pairs": cnt[1], "line_hits": cnt[2],
                      "device_bytes": cnt[3]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "fused_lattice_kernel", "alg_bytes_per_launch": alg_bytes},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload),
+                     "kernel": "fused_lattice_kernel", "alg_bytes_per_launch": alg_bytes,
+                     "fused_ms": fused_ms},
     }
     res["paths"] = {"value": "plain GPU path (the drop-in default)",
                     "learned_radius": "same workload with the learned search radius"}

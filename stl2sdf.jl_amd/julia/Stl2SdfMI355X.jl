@@ -112,4 +112,35 @@ function RBFs_smoothing(dist::Vector, my_grid::Grid, Is_interpolation::Bool, smo
     return fine, new_grid
 end
 
+# Learned search radius (no counterpart in the reference): fit once per mesh
+# and grid, then compute the same field as signed_distance.
+function learned_fit(mesh::TriangularMesh, grid::Grid; n_coarse::Int = 128,
+                     steps::Int = 1500, batch::Int = 4096, lr::Float32 = 2f-3,
+                     seed::UInt64 = UInt64(1), quantile::Float64 = 0.99)
+    stats = zeros(Float64, 9)
+    GC.@preserve mesh grid stats begin
+        _check(ccall((:sdf_mesh_set, LIB), Int32,
+            (Ptr{Cvoid}, Ptr{Float64}, Int64, Ptr{Int64}, Int64),
+            ctx(), mesh.X, mesh.nnp, mesh.IEN, mesh.nel))
+        _check(ccall((:sdf_learned_fit, LIB), Int32,
+            (Ptr{Cvoid}, Ptr{Float64}, Float64, Ptr{Int64}, Int32, Int32, Int32, Float32,
+             UInt64, Float64, Ptr{Float64}, Int32),
+            ctx(), grid.AABB_min, grid.cell_size, grid.N, n_coarse, steps, batch, lr,
+            seed, quantile, stats, Int32(9)))
+    end
+    return stats
+end
+
+function learned_signed_distance(mesh::TriangularMesh, grid::Grid)
+    out = Vector{Float64}(undef, grid.ngp)
+    GC.@preserve mesh grid out begin
+        _check(ccall((:sdf_learned_signed_distance, LIB), Int32,
+            (Ptr{Cvoid}, Ptr{Float64}, Int64, Ptr{Int64}, Int64, Ptr{Float64}, Float64,
+             Ptr{Int64}, Ptr{Float64}),
+            ctx(), mesh.X, mesh.nnp, mesh.IEN, mesh.nel, grid.AABB_min, grid.cell_size,
+            grid.N, out))
+    end
+    return out
+end
+
 end # module
