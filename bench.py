@@ -217,9 +217,9 @@ def main():
         torch.cuda.synchronize()
         ldt = time.perf_counter() - t0
         _lib.check(L.sdf_ctx_enable_timing(ctx, 0))
-        lst = np.zeros(7)
+        lst = np.zeros(8)
         nst = C.c_int32()
-        _lib.check(L.sdf_learned_stats(ctx, _lib.dptr(lst), 7, C.byref(nst)))
+        _lib.check(L.sdf_learned_stats(ctx, _lib.dptr(lst), 8, C.byref(nst)))
         step()
         torch.cuda.synchronize()
         identical = bool(_t.equal(out, out_l))
@@ -230,9 +230,9 @@ def main():
             "identical_to_plain": identical,
             "stages_ms": {k: float(np.mean(v)) for k, v in lstages.items()},
             "fit": fit,
-            "run": dict(zip(("fallback_nodes", "pairs_evaluated", "pairs_plain",
-                             "node_pairs_radius", "node_pairs_h", "delta", "hc"),
-                            lst.tolist())),
+            "run": dict(zip(("fallback_nodes", "exact_evals", "tile_pairs",
+                             "node_pairs_radius", "node_pairs_h", "delta", "hc",
+                             "exact_evals_plain"), lst.tolist())),
         }
         log(f"[bench] learned: {lval:.4g} pts/s, identical={identical}")
 

@@ -223,10 +223,12 @@ sdf_status sdf_learned_signed_distance(sdf_ctx *ctx, const double *X, int64_t nn
 sdf_status sdf_learned_signed_distance_device(sdf_ctx *ctx, const double aabb_min[3],
                                               double h, const int64_t N[3],
                                               double *d_sdf_out);
-/* Counters of the last learned call (synchronises), up to 7: nodes whose
- * radius held no triangle (settled by exact LBVH queries), evaluated (tile,
- * triangle) pairs, pairs the plain loop evaluates, (node, triangle) pairs
- * within the learned radius and within h (box/plane lower bound), delta, hc. */
+/* Counters of the last learned call (synchronises), up to 8: nodes whose
+ * radius held no triangle (settled by exact LBVH queries), exact
+ * evaluations (per wave: its longest candidate list, summed), (tile,
+ * triangle) pairs, (node, triangle) pairs within the learned radius and
+ * within h (box/plane lower bound), delta, hc, and the exact evaluations the
+ * plain loop makes on the same tiles. */
 sdf_status sdf_learned_stats(sdf_ctx *ctx, double *out, int32_t cap, int32_t *n);
 
 /* ------------------------------------------------------------------

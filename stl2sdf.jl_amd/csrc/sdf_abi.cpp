@@ -32,7 +32,7 @@ namespace {
 thread_local std::string g_last_error;
 
 // device counters of a learned-radius run (layout: launch_dist_chunks_learned)
-constexpr int kLearnedStats = 776;
+constexpr int kLearnedStats = 1032;
 
 template <typename F>
 sdf_status guarded(F &&f) {
@@ -121,7 +121,7 @@ struct sdf_ctx {
   bool capturing = false;        // inside stream capture: no host waits allowed
   // tunables: triangles per distance work item, LDS ray-stack depth
   int chunk_len = 64;
-  int ray_stack = 2048;
+  int ray_stack = 1024;
   // timing
   bool timing = false;
   std::vector<hipEvent_t> ev;
@@ -665,7 +665,7 @@ double now_ms() {
 // capacity, the whole pass is captured into one hipGraph and replayed.
 struct BatchTune {
   int chunk_len = 64;       // triangles per distance work item
-  int ray_stack = 2048;     // LDS ray-stack depth of the vote stage
+  int ray_stack = 1024;     // LDS ray-stack depth of the vote stage
   int dist_blocks = 2048, dist_grab = 4;
   int fused_blocks = 2048, fused_grab = 16;
 };
@@ -910,7 +910,7 @@ void batch_autotune(sdf_batch *b) {
     b->tune.fused_grab = pick;
   }
   {  // LDS ray-stack depth
-    const int depth[2] = {2048, 1024};
+    const int depth[2] = {1024, 2048};
     double best = 1e30;
     int pick = b->tune.ray_stack;
     for (int d : depth) {
@@ -1604,19 +1604,21 @@ sdf_status sdf_learned_stats(sdf_ctx *c, double *out, int32_t cap, int32_t *n) {
       SDF_HIP(hipMemcpyAsync(h.data(), c->lstats, sizeof(unsigned long long) * kLearnedStats,
                              hipMemcpyDeviceToHost, c->stream));
       SDF_HIP(hipStreamSynchronize(c->stream));
-      unsigned long long v = 0, lr = 0, lh = 0;
+      unsigned long long v = 0, lr = 0, lh = 0, vp = 0;
       for (int i = 0; i < 256; i++) {
         v += h[8 + i];
         lr += h[264 + i];
         lh += h[520 + i];
+        vp += h[776 + i];
       }
       st[0] = (double)h[0];
       st[1] = (double)v;
       st[3] = (double)lr;
       st[4] = (double)lh;
+      st[7] = (double)vp;
     }
     int w = 0;
-    for (; w < 7 && w < cap && out; w++) out[w] = st[w];
+    for (; w < 8 && w < cap && out; w++) out[w] = st[w];
     if (n) *n = w;
   });
 }

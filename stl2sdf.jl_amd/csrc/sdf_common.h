@@ -284,10 +284,22 @@ struct Tri64 {
 // L2).  Map launch index b of B so that XCD x receives the contiguous range
 // [x*B/8, (x+1)*B/8) of work items: neighbouring tiles then share one L2.
 constexpr int kNumXcd = 8;
+// Tile index -> tile coordinates with 32-bit divisions (tile counts stay
+// below 2^31).
+__device__ inline void tile_coords(uint32_t tile, const TileDims &td, int32_t *tx,
+                                   int32_t *ty, int32_t *tz) {
+  const uint32_t q = tile / (uint32_t)td.tx;
+  *tx = (int32_t)(tile - q * (uint32_t)td.tx);
+  *ty = (int32_t)(q % (uint32_t)td.ty);
+  *tz = (int32_t)(q / (uint32_t)td.ty);
+}
+
 __device__ inline int64_t xcd_remap(int64_t b, int64_t B) {
-  const int64_t per = B / kNumXcd, rem = B % kNumXcd;
-  const int64_t x = b % kNumXcd, local = b / kNumXcd;
-  return x * per + (x < rem ? x : rem) + local;
+  // 32-bit arithmetic: launch sizes stay below 2^31 blocks
+  const uint32_t bb = (uint32_t)b, BB = (uint32_t)B;
+  const uint32_t per = BB / kNumXcd, rem = BB % kNumXcd;
+  const uint32_t x = bb % kNumXcd, local = bb / kNumXcd;
+  return (int64_t)(x * per + (x < rem ? x : rem) + local);
 }
 
 // Cache-blocked tile order: bricks of 16x16x16 tiles (bricks x-fastest),
@@ -303,12 +315,14 @@ __host__ __device__ inline int64_t brick_launch_size(const TileDims &td) {
 }
 __device__ inline bool brick_tile(int64_t m, const TileDims &td, int32_t *tx,
                                   int32_t *ty, int32_t *tz) {
+  // 32-bit decode (brick counts and positions stay below 2^31)
   const int32_t e = 1 << kBrickBits;
-  const int64_t brick = m >> (3 * kBrickBits);
+  const uint32_t brick = (uint32_t)(m >> (3 * kBrickBits));
   const uint32_t w = (uint32_t)(m & ((1 << (3 * kBrickBits)) - 1));
-  const int32_t nbx = (td.tx + e - 1) / e, nby = (td.ty + e - 1) / e;
-  const int32_t bx = (int32_t)(brick % nbx), by = (int32_t)((brick / nbx) % nby),
-                bz = (int32_t)(brick / ((int64_t)nbx * nby));
+  const uint32_t nbx = (uint32_t)((td.tx + e - 1) / e), nby = (uint32_t)((td.ty + e - 1) / e);
+  const uint32_t q = brick / nbx;
+  const int32_t bx = (int32_t)(brick - q * nbx), by = (int32_t)(q % nby),
+                bz = (int32_t)(q / nby);
   int32_t lx = 0, ly = 0, lz = 0;
 #pragma unroll
   for (int b = 0; b < kBrickBits; b++) {

@@ -235,9 +235,10 @@ void launch_dist_chunks(hipStream_t s, const GridP &g, const Tri64 *tri,
                         const Bins &b, float *d2buf, const uint32_t *only = nullptr);
 // Distance work items with a learned per-point squared radius r2buf (slots
 // as d2buf); nodes whose radius held no triangle are appended to `list`.
-// stats: [0] fallback nodes, [8..263] evaluated (tile, triangle) pairs,
-// [264..519] (node, triangle) pairs within the learned radius (lower bound),
-// [520..775] the same within h.
+// stats: [0] fallback nodes, [8..263] exact evaluations per wave (its
+// longest candidate list), [264..519] (node, triangle) pairs within the
+// learned radius (lower bound), [520..775] the same within h, [776..1031]
+// the exact evaluations the plain loop makes (longest list within h).
 void launch_dist_chunks_learned(hipStream_t s, const GridP &g, const Tri64 *tri,
                                 const Bins &b, const float *r2buf, float *d2buf,
                                 uint64_t *list, unsigned long long *stats);
@@ -280,7 +281,7 @@ void resolve_pending(hipStream_t s, Arena &ar, const GridP &g, const double *pts
                      const Pending *items, const uint32_t *n_dev, uint32_t cap, int r0,
                      int r1, int final_phase, FusedOut out, Pending *next,
                      uint32_t *d_counts, uint32_t wind_cap, uint32_t *err,
-                     int stack = 2048);
+                     int stack = 1024);
 
 void launch_dist_points_bvh(hipStream_t s, const double *pts, int64_t npts,
                             const Tri64 *tri, const Bvh &bvh, double h,

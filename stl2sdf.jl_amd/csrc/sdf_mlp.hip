@@ -238,9 +238,8 @@ __global__ __launch_bounds__(WAVES * 64) void mlp_tile_radius_kernel(
     const uint32_t tile = chunk_tile[item];
     const uint64_t first = chunk_off[tile];
     if ((uint64_t)item != first) continue;  // one pass per tile
-    const int32_t tx = (int32_t)(tile % td.tx);
-    const int32_t ty = (int32_t)((tile / td.tx) % td.ty);
-    const int32_t tz = (int32_t)(tile / ((int64_t)td.tx * td.ty));
+    int32_t tx, ty, tz;
+    tile_coords((uint32_t)tile, td, &tx, &ty, &tz);
 #pragma unroll 1
     for (int pass = 0; pass < 4 / NT; pass++) {
       const u16x8 *W = sA + opaque_zero();

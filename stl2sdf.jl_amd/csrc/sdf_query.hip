@@ -98,7 +98,7 @@ __device__ inline bool inline_parity(const LineInline &r, const double *xs,
 // every lane evaluates its tile point against all of them.  Uniform per
 // block (contains a barrier).
 __device__ inline void dist_item(const DistJob &J, int64_t item,
-                                 float (&s_pre)[kTriPreFloats][kTilePts]) {
+                                 float (&s_pre)[kTriPreFloats + 6][kTilePts]) {
   const GridP &g = J.g;
   const TileDims &td = J.td;
   const Tri64 *__restrict__ tri = J.tri;
@@ -114,9 +114,8 @@ __device__ inline void dist_item(const DistJob &J, int64_t item,
   const uint64_t first = chunk_off[tile];
   if (only && only[first] == 0) return;  // re-run restricted to flagged tiles
   const int64_t q = item - (int64_t)first;
-  const int32_t tx = (int32_t)(tile % td.tx);
-  const int32_t ty = (int32_t)((tile / td.tx) % td.ty);
-  const int32_t tz = (int32_t)(tile / ((int64_t)td.tx * td.ty));
+  int32_t tx, ty, tz;
+  tile_coords((uint32_t)tile, td, &tx, &ty, &tz);
   const int32_t i0 = tx * kTileX, j0 = ty * kTileY, k0 = tz * kTileZ;
   const Vec3d T0 = lattice_point(g, i0, j0, k0);
   const uint64_t b = tile_off[tile] + (uint64_t)q * J.clen;
@@ -135,6 +134,11 @@ __device__ inline void dist_item(const DistJob &J, int64_t item,
     tri_pre_make(v, v + 3, v + 6, rec);
 #pragma unroll
     for (int f = 0; f < kTriPreFloats; f++) s_pre[f][lane] = rec[f];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {  // the triangle's box
+      s_pre[kTriPreFloats + a][lane] = fminf(fminf(v[a], v[3 + a]), v[6 + a]);
+      s_pre[kTriPreFloats + 3 + a][lane] = fmaxf(fmaxf(v[a], v[3 + a]), v[6 + a]);
+    }
   }
   __syncthreads();
   // this lane's point, relative to the tile origin (exact small multiples of h)
@@ -143,8 +147,28 @@ __device__ inline void dist_item(const DistJob &J, int64_t item,
   const int32_t k = k0 + lane / (kTileX * kTileY);
   const Vec3d p = lattice_point(g, i, j, k);
   const float px = (float)(p.x - T0.x), py = (float)(p.y - T0.y), pz = (float)(p.z - T0.z);
-  float best = __builtin_huge_valf();
+  // Phase 1: the triangles whose lower bound (the larger of the squared
+  // distances to the triangle's box and plane) is within h, inflated by
+  // 2^-10 to cover the bound's rounding against the exact evaluation.  Only
+  // these can give a distance below h; the nearest one is always among them.
+  const float hf = (float)g.h;
+  const float H2 = hf * hf * (1.0f + 0x1p-10f);
+  unsigned long long cand = 0ull;
   for (int c = 0; c < n; c++) {
+    const float ex = fmaxf(fmaxf(s_pre[kTriPreFloats][c] - px, px - s_pre[kTriPreFloats + 3][c]), 0.f);
+    const float ey = fmaxf(fmaxf(s_pre[kTriPreFloats + 1][c] - py, py - s_pre[kTriPreFloats + 4][c]), 0.f);
+    const float ez = fmaxf(fmaxf(s_pre[kTriPreFloats + 2][c] - pz, pz - s_pre[kTriPreFloats + 5][c]), 0.f);
+    const float box2 = ex * ex + ey * ey + ez * ez;
+    const float pn = (px - s_pre[0][c]) * s_pre[21][c] + (py - s_pre[1][c]) * s_pre[22][c] +
+                     (pz - s_pre[2][c]) * s_pre[23][c];
+    const float lb2 = fmaxf(box2, pn * pn * s_pre[24][c]);
+    if (lb2 <= H2) cand |= 1ull << c;
+  }
+  // Phase 2: exact distances for this lane's candidates only.
+  float best = __builtin_huge_valf();
+  while (cand) {
+    const int c = __builtin_ctzll(cand);
+    cand &= cand - 1ull;
     const float d2 = tri_pre_dist2(px, py, pz, [&](int f) { return s_pre[f][c]; });
     best = fminf(best, d2);
   }
@@ -157,7 +181,7 @@ __device__ inline void dist_item(const DistJob &J, int64_t item,
 }
 
 __global__ __launch_bounds__(kTilePts) void dist_chunks_kernel(DistJob J) {
-  __shared__ float s_pre[kTriPreFloats][kTilePts];
+  __shared__ float s_pre[kTriPreFloats + 6][kTilePts];
   dist_item(J, xcd_remap(blockIdx.x, gridDim.x), s_pre);
 }
 
@@ -167,7 +191,7 @@ __global__ __launch_bounds__(kTilePts) void dist_chunks_kernel(DistJob J) {
 __global__ __launch_bounds__(kTilePts) void dist_queue_kernel(
     const DistJob *__restrict__ jobs, int njobs, const uint64_t *__restrict__ prefix,
     unsigned long long *__restrict__ counter, int grab) {
-  __shared__ float s_pre[kTriPreFloats][kTilePts];
+  __shared__ float s_pre[kTriPreFloats + 6][kTilePts];
   __shared__ uint64_t s_pref[kMaxJobs + 1];
   __shared__ unsigned long long s_base;
   for (int i = threadIdx.x; i <= njobs; i += blockDim.x) s_pref[i] = prefix[i];
@@ -202,15 +226,17 @@ __global__ void dist_prefix_kernel(const DistJob *__restrict__ jobs, int njobs,
   *counter = 0ull;
 }
 
-// Learned-radius variant of dist_chunks_kernel.  Lane p carries a squared
-// search radius R2(p) <= h^2 proposed by the network; a triangle is
-// evaluated only when some lane's lower bound (the larger of the squared
-// distances to the triangle's box and to its plane) is within R2 inflated by
-// 2^-10 (which covers the rounding of the bound against the exact
-// evaluation).  A lane that ends with best >= R2 < h^2 has no triangle within
-// its radius and is settled afterwards by an exact re-run of its tile; every
-// other lane's best is the same value the full loop gives.  stats[b & 255]
-// collects evaluated (tile, triangle) pairs per block slot.
+// Learned-radius variant of the distance item.  Lane p carries a squared
+// search radius R2(p) <= h^2 proposed by the network and evaluates only the
+// triangles whose lower bound (the larger of the squared distances to the
+// triangle's box and to its plane) is within R2 inflated by 2^-10 (which
+// covers the rounding of the bound against the exact evaluation).  A lane
+// that ends with best >= R2 < h^2 has no triangle within its radius and is
+// settled afterwards by an exact query; every other lane's best is the value
+// the full loop gives.  Counters per block slot (b & 255): [8..] the wave's
+// exact evaluations (its longest candidate list), [264..] candidates within
+// the learned radius, [520..] candidates within h, [776..] the wave's
+// longest list within h (the exact evaluations of the plain loop).
 __global__ __launch_bounds__(kTilePts) void dist_chunks_learned_kernel(
     GridP g, TileDims td, const Tri64 *__restrict__ tri,
     const uint64_t *__restrict__ tile_off, const uint32_t *__restrict__ tile_tris,
@@ -226,9 +252,8 @@ __global__ __launch_bounds__(kTilePts) void dist_chunks_learned_kernel(
   const int64_t tile = chunk_tile[item];
   const uint64_t first = chunk_off[tile];
   const int64_t q = item - (int64_t)first;
-  const int32_t tx = (int32_t)(tile % td.tx);
-  const int32_t ty = (int32_t)((tile / td.tx) % td.ty);
-  const int32_t tz = (int32_t)(tile / ((int64_t)td.tx * td.ty));
+  int32_t tx, ty, tz;
+  tile_coords((uint32_t)tile, td, &tx, &ty, &tz);
   const int32_t i0 = tx * kTileX, j0 = ty * kTileY, k0 = tz * kTileZ;
   const Vec3d T0 = lattice_point(g, i0, j0, k0);
   const uint64_t b = tile_off[tile] + (uint64_t)q * clen;
@@ -262,8 +287,10 @@ __global__ __launch_bounds__(kTilePts) void dist_chunks_learned_kernel(
   const float R2 = r2buf[first * kTilePts + lane] * (1.0f + 0x1p-10f);
   const float hf = (float)g.h;
   const float H2 = hf * hf * (1.0f + 0x1p-10f);
-  float best = __builtin_huge_valf();
-  int visited = 0, lanes_r = 0, lanes_h = 0;
+  // phase 1: candidates within this lane's radius (and, for the counters,
+  // within h); phase 2: exact distances of the candidates only
+  unsigned long long cand = 0ull;
+  int lanes_h = 0;
   for (int c = 0; c < n; c++) {
     const float ex = fmaxf(fmaxf(s_box[0][c] - px, px - s_box[3][c]), 0.f);
     const float ey = fmaxf(fmaxf(s_box[1][c] - py, py - s_box[4][c]), 0.f);
@@ -272,14 +299,26 @@ __global__ __launch_bounds__(kTilePts) void dist_chunks_learned_kernel(
     const float pn = (px - s_pre[0][c]) * s_pre[21][c] + (py - s_pre[1][c]) * s_pre[22][c] +
                      (pz - s_pre[2][c]) * s_pre[23][c];
     const float lb2 = fmaxf(box2, pn * pn * s_pre[24][c]);
-    const unsigned long long need = __ballot(lb2 <= R2);
-    lanes_r += __popcll(need);
-    lanes_h += __popcll(__ballot(lb2 <= H2));
-    if (need) {
-      const float d2 = tri_pre_dist2(px, py, pz, [&](int f) { return s_pre[f][c]; });
-      best = fminf(best, d2);
-      visited++;
-    }
+    if (lb2 <= R2) cand |= 1ull << c;
+    lanes_h += lb2 <= H2 ? 1 : 0;
+  }
+  const int lanes_r = __popcll(cand);
+  float best = __builtin_huge_valf();
+  int visited = 0;
+  while (cand) {
+    const int c = __builtin_ctzll(cand);
+    cand &= cand - 1ull;
+    const float d2 = tri_pre_dist2(px, py, pz, [&](int f) { return s_pre[f][c]; });
+    best = fminf(best, d2);
+    visited++;
+  }
+  // wave totals for the counters
+  int tot_r = lanes_r, tot_h = lanes_h, vmax = visited, hmax = lanes_h;
+  for (int off = 32; off > 0; off >>= 1) {
+    tot_r += __shfl_xor(tot_r, off);
+    tot_h += __shfl_xor(tot_h, off);
+    vmax = max(vmax, __shfl_xor(vmax, off));
+    hmax = max(hmax, __shfl_xor(hmax, off));
   }
   float *slot = d2buf + first * kTilePts + lane;
   if (chunk_off[tile + 1] - first == 1)
@@ -287,9 +326,10 @@ __global__ __launch_bounds__(kTilePts) void dist_chunks_learned_kernel(
   else
     atomicMin((unsigned int *)slot, __float_as_uint(best));
   if (lane == 0 && stats) {
-    atomicAdd(stats + 8 + (blockIdx.x & 255), (unsigned long long)visited);
-    atomicAdd(stats + 264 + (blockIdx.x & 255), (unsigned long long)lanes_r);
-    atomicAdd(stats + 520 + (blockIdx.x & 255), (unsigned long long)lanes_h);
+    atomicAdd(stats + 8 + (blockIdx.x & 255), (unsigned long long)vmax);
+    atomicAdd(stats + 264 + (blockIdx.x & 255), (unsigned long long)tot_r);
+    atomicAdd(stats + 520 + (blockIdx.x & 255), (unsigned long long)tot_h);
+    atomicAdd(stats + 776 + (blockIdx.x & 255), (unsigned long long)hmax);
   }
 }
 
@@ -335,9 +375,8 @@ __global__ __launch_bounds__(256) void learned_fallback_kernel(
     const uint64_t first = code / kTilePts;
     const int lane = (int)(code % kTilePts);
     const int64_t tile = chunk_tile[first];
-    const int32_t tx = (int32_t)(tile % td.tx);
-    const int32_t ty = (int32_t)((tile / td.tx) % td.ty);
-    const int32_t tz = (int32_t)(tile / ((int64_t)td.tx * td.ty));
+    int32_t tx, ty, tz;
+    tile_coords((uint32_t)tile, td, &tx, &ty, &tz);
     const int32_t i0 = tx * kTileX, j0 = ty * kTileY, k0 = tz * kTileZ;
     const Vec3d T0 = lattice_point(g, i0, j0, k0);
     const Vec3d p = lattice_point(g, i0 + (lane & (kTileX - 1)), j0 + ((lane / kTileX) & (kTileY - 1)),
@@ -372,9 +411,20 @@ __global__ __launch_bounds__(256) void learned_fallback_kernel(
 // votes (certified fast path, else the reference's arithmetic), combine,
 // outputs; points whose votes are not unanimous go to the pending list.
 
+// LDS staging of one sign brick: its 32 x-line and 32 y-line summaries and
+// the distance slots of its 4 distance tiles, fetched by one round of
+// independent loads so each lane's per-point reads come from LDS.
+struct FusedShared {
+  LineInline lx[kSignZ][8];  // x-line (k0 + q, 8 by + w)
+  LineInline ly[kSignZ][8];  // y-line (k0 + q, 8 bx + w)
+  float d2[4][kTilePts];     // tiles (tz local, ty local) = (t >> 1, t & 1)
+  uint32_t slot[4];          // first work item + 1 of each tile, 0: no distance work
+};
+
 // One sign brick (8 x 8 x 4 points) of job J for the 64-lane block; m is
-// the brick's position in the brick launch order.
-__device__ inline void fused_brick(const FusedJob &J, int64_t m) {
+// the brick's position in the brick launch order.  Uniform per block
+// (contains barriers).
+__device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S) {
   const GridP &g = J.g;
   const TileDims &td = J.td;
   const uint64_t *__restrict__ chunk_off = J.chunk_off;
@@ -395,117 +445,152 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m) {
   uint32_t *err = J.err;
   const int64_t chunk_cap = J.chunk_cap;
   int32_t bx, by, bz;
-  if (!brick_tile(m, J.bd, &bx, &by, &bz)) return;
+  if (!brick_tile(m, J.bd, &bx, &by, &bz)) return;  // uniform
   const int lane = threadIdx.x;
   const int32_t i = bx * 8 + (lane & 7);
   const int32_t j = by * 8 + (lane >> 3);
   const int32_t k0 = bz * kSignZ;
-  if (i >= g.nx || j >= g.ny) return;
+  const bool in_grid = i < g.nx && j < g.ny;
   const int nk = min(kSignZ, g.nz - k0);
-  const double x = lattice_coord(g.ox, g.h, i), y = lattice_coord(g.oy, g.h, j);
-  double zq[kSignZ];
-#pragma unroll
-  for (int q = 0; q < kSignZ; q++) zq[q] = lattice_coord(g.oz, g.h, k0 + q);
 
-  int zpos[kSignZ], zneg[kSignZ];
-  bool zok[kSignZ];
+  // ---- stage: the lane's own z-line summary (registers), the brick's x- and
+  // y-line summaries and its tiles' distance slots (LDS)
+  LineInline rz;
   if (do_sign) {
-    // z-line (i, j): one inline record classifies all 4 points
-    const int64_t Lz = base2 + (int64_t)j * g.nx + i;
-    const LineInline rz = inl[Lz];
-    if (!inline_parity<kSignZ>(rz, zq, zpos, zneg, zok)) {
-      const uint64_t hb = line_off[Lz], he = line_off[Lz + 1];
-      const uint32_t mz = meta[Lz];
-#pragma unroll
-      for (int q = 0; q < kSignZ; q++)
-        zok[q] = q < nk ? axis_pair_parity_fast(fast, hb, he, mz, zq[q], &zpos[q], &zneg[q])
-                        : true;
+    if (in_grid) rz = inl[base2 + (int64_t)j * g.nx + i];
+    const int q = (lane >> 3) & 3, w = lane & 7, kk = k0 + q;
+    if (lane < 32) {
+      const int32_t jj = by * 8 + w;
+      if (kk < g.nz && jj < g.ny) S.lx[q][w] = inl[(int64_t)kk * g.ny + jj];
+    } else {
+      const int32_t ii = bx * 8 + w;
+      if (kk < g.nz && ii < g.nx) S.ly[q][w] = inl[base1 + (int64_t)kk * g.nx + ii];
     }
   }
+  if (d2buf) {
+    if (lane < 4) {
+      const int32_t tyy = 2 * by + (lane & 1), tzz = 2 * bz + (lane >> 1);
+      uint32_t sl = 0;
+      if (tyy < td.ty && tzz < td.tz) {
+        const int64_t tile = ((int64_t)tzz * td.ty + tyy) * td.tx + bx;
+        const uint64_t c0 = chunk_off[tile], c1 = chunk_off[tile + 1];
+        if (c1 > c0 && (int64_t)c0 < chunk_cap) sl = (uint32_t)c0 + 1u;
+      }
+      S.slot[lane] = sl;
+    }
+    __syncthreads();
+    const int t = lane >> 4;
+    const uint32_t sl = S.slot[t];
+    float4 v = make_float4(__builtin_huge_valf(), __builtin_huge_valf(), __builtin_huge_valf(),
+                           __builtin_huge_valf());
+    if (sl) v = *reinterpret_cast<const float4 *>(d2buf + (uint64_t)(sl - 1u) * kTilePts +
+                                                   (lane & 15) * 4);
+    *reinterpret_cast<float4 *>(&S.d2[t][(lane & 15) * 4]) = v;
+  }
+  __syncthreads();
 
+  if (in_grid) {
+    const double x = lattice_coord(g.ox, g.h, i), y = lattice_coord(g.oy, g.h, j);
+    double zq[kSignZ];
 #pragma unroll
-  for (int q = 0; q < kSignZ; q++) {
-    if (q >= nk) break;
-    const int32_t k = k0 + q;
-    const int64_t idx = ((int64_t)k * g.ny + j) * g.nx + i;
-    double dist = g.h;
-    if (d2buf) {
-      const int64_t tile = ((int64_t)(k / kTileZ) * td.ty + j / kTileY) * td.tx + i / kTileX;
-      const uint64_t c0 = chunk_off[tile];
-      if (chunk_off[tile + 1] > c0 && (int64_t)c0 < chunk_cap) {
+    for (int q = 0; q < kSignZ; q++) zq[q] = lattice_coord(g.oz, g.h, k0 + q);
+    int zpos[kSignZ], zneg[kSignZ];
+    bool zok[kSignZ];
+    if (do_sign) {
+      // z-line (i, j): one summary classifies all 4 points
+      if (!inline_parity<kSignZ>(rz, zq, zpos, zneg, zok)) {
+        const int64_t Lz = base2 + (int64_t)j * g.nx + i;
+        const uint64_t hb = line_off[Lz], he = line_off[Lz + 1];
+        const uint32_t mz = meta[Lz];
+#pragma unroll
+        for (int q = 0; q < kSignZ; q++)
+          zok[q] = q < nk ? axis_pair_parity_fast(fast, hb, he, mz, zq[q], &zpos[q], &zneg[q])
+                          : true;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kSignZ; q++) {
+      if (q >= nk) break;  // uniform per wave
+      const int32_t k = k0 + q;
+      const int64_t idx = ((int64_t)k * g.ny + j) * g.nx + i;
+      double dist = g.h;
+      if (d2buf) {
+        const int t = (q >> 1) * 2 + ((lane >> 3) >> 2);
         const int tl = (i & (kTileX - 1)) + kTileX * ((j & (kTileY - 1)) + kTileY * (k & (kTileZ - 1)));
-        const float best = d2buf[c0 * kTilePts + tl];
+        const float best = S.d2[t][tl];
         if (best < __builtin_huge_valf()) {
           const double d = sqrt((double)best);
           dist = d < g.h ? d : g.h;
         }
       }
-    }
-    double sgn = 1.0;
-    bool pending = false;
-    if (do_sign) {
-      const double o[3] = {x, y, zq[q]};
-      int inside = zpos[q] + zneg[q];
-      int a, bpar;
-      bool okq, amb = !zok[q];
-      const int64_t Lx = (int64_t)k * g.ny + j;
-      const int64_t Ly = base1 + (int64_t)k * g.nx + i;
-      const LineInline rx = inl[Lx];
-      const LineInline ry = inl[Ly];
-      if (!inline_parity<1>(rx, &x, &a, &bpar, &okq) || !okq)
-        amb |= !axis_pair_parity_fast(fast, line_off[Lx], line_off[Lx + 1], meta[Lx], x, &a,
-                                      &bpar);
-      inside += a + bpar;
-      if (!inline_parity<1>(ry, &y, &a, &bpar, &okq) || !okq)
-        amb |= !axis_pair_parity_fast(fast, line_off[Ly], line_off[Ly + 1], meta[Ly], y, &a,
-                                      &bpar);
-      inside += a + bpar;
-      (void)o;
-      if (amb) {
-        // a decision sat inside its rounding margin: the point restarts the
-        // vote from ray 1 with the reference's arithmetic (LBVH path)
-        pending = true;
-        const uint32_t slot = atomicAdd(npend0, 1u);
-        if (slot < cap0) {
-          Pending pr;
-          pr.idx = idx;
-          pr.dist = dist;
-          pr.inside = 0;
-          pr.used = 0;
-          pend0[slot] = pr;
-        } else {
-          atomicOr(err, kErrPendingOverflow);
+      double sgn = 1.0;
+      bool pending = false;
+      if (do_sign) {
+        int inside = zpos[q] + zneg[q];
+        int a, bpar;
+        bool okq, amb = !zok[q];
+        const LineInline &rx = S.lx[q][lane >> 3];
+        const LineInline &ry = S.ly[q][lane & 7];
+        if (!inline_parity<1>(rx, &x, &a, &bpar, &okq) || !okq) {
+          const int64_t Lx = (int64_t)k * g.ny + j;
+          amb |= !axis_pair_parity_fast(fast, line_off[Lx], line_off[Lx + 1], meta[Lx], x, &a,
+                                        &bpar);
         }
-      } else if (inside == 6) {
-        sgn = 1.0;
-      } else if (inside == 0) {
-        sgn = -1.0;
-      } else {
-        pending = true;
-        const uint32_t slot = atomicAdd(npend, 1u);
-        if (slot < cap) {
-          Pending pr;
-          pr.idx = idx;
-          pr.dist = dist;
-          pr.inside = inside;
-          pr.used = 6;
-          pend[slot] = pr;
+        inside += a + bpar;
+        if (!inline_parity<1>(ry, &y, &a, &bpar, &okq) || !okq) {
+          const int64_t Ly = base1 + (int64_t)k * g.nx + i;
+          amb |= !axis_pair_parity_fast(fast, line_off[Ly], line_off[Ly + 1], meta[Ly], y, &a,
+                                        &bpar);
+        }
+        inside += a + bpar;
+        if (amb) {
+          // a decision sat inside its rounding margin: the point restarts the
+          // vote from ray 1 with the reference's arithmetic (LBVH path)
+          pending = true;
+          const uint32_t slot = atomicAdd(npend0, 1u);
+          if (slot < cap0) {
+            Pending pr;
+            pr.idx = idx;
+            pr.dist = dist;
+            pr.inside = 0;
+            pr.used = 0;
+            pend0[slot] = pr;
+          } else {
+            atomicOr(err, kErrPendingOverflow);
+          }
+        } else if (inside == 6) {
+          sgn = 1.0;
+        } else if (inside == 0) {
+          sgn = -1.0;
         } else {
-          atomicOr(err, kErrPendingOverflow);
+          pending = true;
+          const uint32_t slot = atomicAdd(npend, 1u);
+          if (slot < cap) {
+            Pending pr;
+            pr.idx = idx;
+            pr.dist = dist;
+            pr.inside = inside;
+            pr.used = 6;
+            pend[slot] = pr;
+          } else {
+            atomicOr(err, kErrPendingOverflow);
+          }
         }
       }
-    }
-    if (out.dist) out.dist[idx] = dist;
-    if (!pending) {
-      if (out.sdf) out.sdf[idx] = dist * sgn;
-      if (out.signs) out.signs[idx] = sgn;
-      if (out.conf) out.conf[idx] = 1.0;
+      if (out.dist) out.dist[idx] = dist;
+      if (!pending) {
+        if (out.sdf) out.sdf[idx] = dist * sgn;
+        if (out.signs) out.signs[idx] = sgn;
+        if (out.conf) out.conf[idx] = 1.0;
+      }
     }
   }
+  __syncthreads();  // S is reused by the block's next brick
 }
 
 __global__ __launch_bounds__(64) void fused_lattice_kernel(FusedJob J) {
-  fused_brick(J, xcd_remap(blockIdx.x, gridDim.x));
+  __shared__ FusedShared S;
+  fused_brick(J, xcd_remap(blockIdx.x, gridDim.x), S);
 }
 
 // Persistent work queue over the sign bricks of many jobs (static brick
@@ -513,6 +598,7 @@ __global__ __launch_bounds__(64) void fused_lattice_kernel(FusedJob J) {
 __global__ __launch_bounds__(64) void fused_queue_kernel(
     const FusedJob *__restrict__ jobs, int njobs, const uint64_t *__restrict__ prefix,
     unsigned long long *__restrict__ counter, int grab) {
+  __shared__ FusedShared S;
   __shared__ uint64_t s_pref[kMaxJobs + 1];
   __shared__ unsigned long long s_base;
   for (int i = threadIdx.x; i <= njobs; i += blockDim.x) s_pref[i] = prefix[i];
@@ -528,7 +614,7 @@ __global__ __launch_bounds__(64) void fused_queue_kernel(
     int j = 0;
     for (uint64_t u = base; u < end; u++) {
       while (j + 1 < njobs && s_pref[j + 1] <= u) j++;
-      fused_brick(jobs[j], (int64_t)(u - s_pref[j]));
+      fused_brick(jobs[j], (int64_t)(u - s_pref[j]), S);
     }
   }
 }
@@ -621,7 +707,9 @@ __device__ inline void item_point(const GridP &g, const double *pts, int64_t idx
 // when it is the smallest or its strict predecessor lies more than eps
 // below, equal values counting once.  Stack or hit-list overflow sends the
 // ray to the brute-force fallback.
-constexpr int kWaveHits = 1024;
+// Hits one wave-traversed ray can hold (LDS); a ray with more is redone by
+// the brute-force kernel.  Small, so that many rays are in flight per CU.
+constexpr int kWaveHits = 256;
 
 constexpr int kRayWaves = 8192;  // persistent grid of the ray stage (waves)
 
@@ -1062,7 +1150,7 @@ void resolve_pending(hipStream_t s, Arena &ar, const GridP &g, const double *pts
   if (r1 > kNumDirs) r1 = kNumDirs;
   launch_fill_u32(s, d_counts, 4, 0u);
   unsigned long long *bits = ar.get<unsigned long long>(ar.tmp("res_bits"), cap);
-  hipLaunchKernelGGL(fill_bits_kernel, dim3(256), dim3(256), 0, s, bits, cap, n_dev);
+  hipLaunchKernelGGL(fill_bits_kernel, dim3(128), dim3(256), 0, s, bits, cap, n_dev);
   const uint32_t fb_cap = cap * 4u + 1024u;
   uint32_t *fb = ar.get<uint32_t>(ar.tmp("res_fb"), fb_cap);
   if (r1 > r0) {
@@ -1074,13 +1162,13 @@ void resolve_pending(hipStream_t s, Arena &ar, const GridP &g, const double *pts
     else
       hipLaunchKernelGGL(ray_wave_kernel<2048>, dim3(kRayWaves), dim3(64), 0, s, g, pts, tri,
                          bvh, rp, items, cap, n_dev, r0, r1, bits, fb, d_counts, fb_cap, err);
-    hipLaunchKernelGGL(ray_bits_brute_kernel, dim3(1024), dim3(256), 0, s, g, pts, tri, nel, rp,
+    hipLaunchKernelGGL(ray_bits_brute_kernel, dim3(256), dim3(256), 0, s, g, pts, tri, nel, rp,
                        items, (const uint32_t *)fb, fb_cap, (const uint32_t *)d_counts, bits,
                        err);
     SDF_HIP(hipGetLastError());
   }
   uint32_t *wind = ar.get<uint32_t>(ar.tmp("res_wind"), cap);
-  hipLaunchKernelGGL(vote_kernel, dim3(1024), dim3(256), 0, s, rp, items, cap, n_dev, bits,
+  hipLaunchKernelGGL(vote_kernel, dim3(256), dim3(256), 0, s, rp, items, cap, n_dev, bits,
                      final_phase, out, next, d_counts + 1, wind, d_counts + 2,
                      (double *)nullptr);
   SDF_HIP(hipGetLastError());
@@ -1093,10 +1181,10 @@ void resolve_pending(hipStream_t s, Arena &ar, const GridP &g, const double *pts
       chunk = (nel + chunks - 1) / chunks;
     }
     double *partial = ar.get<double>(ar.tmp("res_wpartial"), (size_t)wind_cap * chunks);
-    hipLaunchKernelGGL(winding_partial_kernel, dim3(4096), dim3(256), 0, s, g, pts, tri, nel,
+    hipLaunchKernelGGL(winding_partial_kernel, dim3(1024), dim3(256), 0, s, g, pts, tri, nel,
                        items, (const uint32_t *)wind, wind_cap,
                        (const uint32_t *)(d_counts + 2), chunks, chunk, partial);
-    hipLaunchKernelGGL(winding_final_kernel, dim3(64), dim3(256), 0, s, items,
+    hipLaunchKernelGGL(winding_final_kernel, dim3(16), dim3(256), 0, s, items,
                        (const uint32_t *)wind, wind_cap, (const uint32_t *)(d_counts + 2),
                        chunks, (const double *)partial, out);
     SDF_HIP(hipGetLastError());
