@@ -227,9 +227,11 @@ const sdf::Bvh &need_bvh(sdf_ctx *c) {
   return c->bvh;
 }
 
-// Build the LBVH on helper stream q (joined later through join[2]).
+// Build the LBVH on helper stream q, after the work recorded in join[0] (the
+// start of the current lattice pass); joined later through join[2].
 void bvh_async(sdf_ctx *c, int q) {
   if (c->bvh_ready) return;
+  SDF_HIP(hipStreamWaitEvent(c->aux[q], c->join[0], 0));
   c->ar.scope = "bvh/";
   c->bvh = sdf::build_lbvh(c->aux[q], c->ar, c->tri, c->nel);
   c->ar.scope.clear();
@@ -315,10 +317,16 @@ enum : int {
   kStWindB1, kStWindB2, kStCount
 };
 
-// Stage 1: fork the helper chains; tile bins and distance work items on
-// the main stream, axis-line hits and the LBVH on the helpers; the distance
-// buffer and pending lists are allocated.
-void stage_bins(sdf_ctx *c, LatticeRun &L) {
+// Stage 1: the build chains.  Tile bins and distance work items run on the
+// main stream; the axis-line hits on helper stream 0 (needed by the sign
+// stage) and the LBVH on helper stream 1 (needed only by the vote stage),
+// both forked from the main stream at the start of the pass.  Issue order
+// matters because launching the pass's ~100 kernels takes the host longer
+// than the GPU needs for the small ones, and a running distance kernel
+// leaves no room for other streams: tile counting, then the whole line
+// chain, then the rest of the tile chain (stage_bins_tiles), the distance
+// kernel (stage_dist), and only then the LBVH (stage_bins_bvh).
+void stage_bins_tiles(sdf_ctx *c, LatticeRun &L) {
   const sdf::GridP &g = L.g;
   const int64_t npts = grid_points(g);
   sdf::Bins &bins = L.bins;
@@ -327,23 +335,20 @@ void stage_bins(sdf_ctx *c, LatticeRun &L) {
                        (L.do_sign && c->caps.hits < 0)))
     fail(SDF_ERR_INTERNAL, "capture needs known capacities");
   SDF_HIP(hipEventRecord(c->join[0], c->stream));
-  for (int q = 0; q < 2; q++) SDF_HIP(hipStreamWaitEvent(c->aux[q], c->join[0], 0));
   L.bvh_was_ready = c->bvh_ready;
-  sdf::LineBuild lb;
-  sdf::TileBuild tb;
   c->mark("bin_tiles");
+  L.d2buf = nullptr;
+  sdf::TileBuild tb;
   if (L.do_dist) sdf::tile_bins_begin(c->stream, c->ar, c->tri, c->nel, g, g.h, tb);
   if (L.do_sign) {
+    SDF_HIP(hipStreamWaitEvent(c->aux[0], c->join[0], 0));
+    sdf::LineBuild lb;
     c->ar.scope = "line/";
     sdf::line_hits_begin(c->aux[0], c->ar, c->tri, c->nel, g, lb, bins);
-    c->ar.scope.clear();
-    bvh_async(c, 1);
-    c->ar.scope = "line/";
     sdf::line_hits_finish(c->aux[0], c->ar, c->tri, c->nel, g, lb, bins, c->caps.hits);
     c->ar.scope.clear();
     SDF_HIP(hipEventRecord(c->join[1], c->aux[0]));
   }
-  L.d2buf = nullptr;
   if (L.do_dist) {
     sdf::tile_bins_finish(c->stream, c->ar, c->tri, c->nel, g, tb, bins, c->caps.pairs);
     sdf::build_dist_chunks(c->stream, c->ar, g, bins, c->caps.chunks, c->chunk_len);
@@ -357,6 +362,15 @@ void stage_bins(sdf_ctx *c, LatticeRun &L) {
                          : (uint32_t)(npts < (1 << 20) ? (npts > 0 ? npts : 1) : (1 << 20));
   L.pend = c->ar.get<sdf::Pending>("pend", L.cap);
   L.pend0 = c->ar.get<sdf::Pending>("pend0", L.cap0);
+}
+
+void stage_bins_bvh(sdf_ctx *c, LatticeRun &L) {
+  if (L.do_sign) bvh_async(c, 1);
+}
+
+void stage_bins(sdf_ctx *c, LatticeRun &L) {
+  stage_bins_tiles(c, L);
+  stage_bins_bvh(c, L);
 }
 
 // Stage 2: exact distances of the tile points (plain or learned radius).
@@ -434,8 +448,9 @@ void stage_resolve(sdf_ctx *c, LatticeRun &L) {
 }
 
 void lattice_pass(sdf_ctx *c, LatticeRun &L) {
-  stage_bins(c, L);
+  stage_bins_tiles(c, L);
   stage_dist(c, L);
+  stage_bins_bvh(c, L);
   stage_sign(c, L);
   stage_resolve(c, L);
 }

@@ -116,9 +116,21 @@ __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(
   for (int i = threadIdx.x; i < kRadix; i += blockDim.x) h[i] = 0;
   __syncthreads();
   const int64_t base = (int64_t)blockIdx.x * kSortTile;
+  const int lane = threadIdx.x & 63;
+  const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   for (int r = 0; r < kSortItems; r++) {
-    int64_t i = base + (int64_t)r * kSortThreads + threadIdx.x;
-    if (i < n) atomicAdd(&h[(keys[i] >> shift) & (kRadix - 1)], 1u);
+    const int64_t i = base + (int64_t)r * kSortThreads + threadIdx.x;
+    const bool valid = i < n;
+    const uint32_t dig = valid ? (keys[i] >> shift) & (kRadix - 1) : 0u;
+    // lanes of the wave with the same digit (keys are often coherent): the
+    // first of them adds the group's count, one LDS atomic per distinct digit
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < kRadixBits; b++) {
+      const uint64_t bal = __ballot((dig >> b) & 1u);
+      peers &= ((dig >> b) & 1u) ? bal : ~bal;
+    }
+    if (valid && (peers & below) == 0ull) atomicAdd(&h[dig], (uint32_t)__popcll(peers));
   }
   __syncthreads();
   for (int d = threadIdx.x; d < kRadix; d += blockDim.x)
@@ -383,13 +395,15 @@ Bvh build_lbvh(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t n) {
                      b.leaf_box, leaf_tri, n, seg);
   SDF_HIP(hipGetLastError());
   m = n;
-  for (int lev = 1; lev < L.nlev; lev++) {
-    int64_t mm = (m + 1) / 2;
+  int lev = 1;
+  for (; lev < L.nlev; lev++) {
+    const int64_t mm = (m + 1) / 2;
     hipLaunchKernelGGL(seg_level_kernel, dim3(blocks_for(mm, 256)), dim3(256), 0,
                        s, seg + L.off[lev - 1], m, seg + L.off[lev], mm);
     SDF_HIP(hipGetLastError());
     m = mm;
   }
+
   hipLaunchKernelGGL(node_boxes_kernel, dim3(blocks_for(n - 1, 256)), dim3(256),
                      0, s, seg, L, range, n, b.nodes);
   SDF_HIP(hipGetLastError());
