@@ -83,12 +83,26 @@ sdf_status sdf_grid_setup(const double xmin[3], const double xmax[3],
  * -- src/SignedDistances/dfOnTriangularMesh.jl:19-209.
  * dist_out[npts]: exact point-triangle distance clamped at h (min(d, h)),
  * FP32 arithmetic on lattice-relative coordinates, FP64 output.  The
- * reference's closest-point matrix xp is not produced. */
+ * closest-point matrix xp comes from sdf_eval_distances_xp. */
 sdf_status sdf_eval_distances(sdf_ctx *ctx, const double *X, int64_t nnp,
                               const int64_t *IEN, int64_t nel,
                               const double aabb_min[3], double h,
                               const int64_t N[3], const double *points,
                               int64_t npts, double *dist_out);
+
+/* evalDistancesOnTriMesh(mesh, grid, points) -> (dist, xp)
+ * -- src/SignedDistances/dfOnTriangularMesh.jl:19-209 (xp: :108-186, merged
+ * at :194-199).  dist_out as sdf_eval_distances; xp_out[3 * npts]
+ * (column-major 3 x npts): the closest point on the mesh to each point,
+ * exact FP64 over all triangles (Voronoi-region closest point, smallest
+ * (squared distance, triangle index) on ties) at every distance, also where
+ * the reference's candidate search is heuristic (beyond ~2.5h); all zeros
+ * for a mesh with no triangle, as the reference's initial xp. */
+sdf_status sdf_eval_distances_xp(sdf_ctx *ctx, const double *X, int64_t nnp,
+                                 const int64_t *IEN, int64_t nel,
+                                 const double aabb_min[3], double h,
+                                 const int64_t N[3], const double *points,
+                                 int64_t npts, double *dist_out, double *xp_out);
 
 /* raycast_sign_detection(mesh, grid, points; min_rays, max_rays,
  * confidence_threshold, use_winding_fallback) -> (signs, confidences)

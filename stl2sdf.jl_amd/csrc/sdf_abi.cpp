@@ -1262,11 +1262,13 @@ sdf_status sdf_ray_directions(double *dirs_out) {
   });
 }
 
-sdf_status sdf_eval_distances(sdf_ctx *c, const double *X, int64_t nnp,
-                              const int64_t *IEN, int64_t nel,
-                              const double aabb_min[3], double h,
-                              const int64_t N[3], const double *points,
-                              int64_t npts, double *dist_out) {
+namespace {
+
+// evalDistancesOnTriMesh: dist always, xp (3 doubles per point) when asked.
+sdf_status eval_distances(sdf_ctx *c, const double *X, int64_t nnp, const int64_t *IEN,
+                          int64_t nel, const double aabb_min[3], double h, const int64_t N[3],
+                          const double *points, int64_t npts, double *dist_out,
+                          double *xp_out, bool want_xp) {
   return guarded([&] {
     if (!c) fail(SDF_ERR_ARG, "ctx is NULL");
     std::lock_guard<std::mutex> lk(c->mu);
@@ -1274,16 +1276,19 @@ sdf_status sdf_eval_distances(sdf_ctx *c, const double *X, int64_t nnp,
     if (!points) npts = grid_points(g);
     if (npts < 0) fail(SDF_ERR_ARG, "npts < 0");
     if (npts > 0 && !dist_out) fail(SDF_ERR_ARG, "dist_out is NULL");
+    if (want_xp && npts > 0 && !xp_out) fail(SDF_ERR_ARG, "xp_out is NULL");
     begin_call(c);
     set_mesh_host(c, X, nnp, IEN, nel);
     double *d_dist = c->ar.get<double>("out_dist", npts + 1);
-    if (is_lattice(g, points, npts)) {
+    const bool lattice = is_lattice(g, points, npts);
+    double *d_pts = nullptr;
+    if (lattice) {
       sdf::FusedOut out;
       out.dist = d_dist;
       sdf::RayParams rp = ray_params(6, 30, 0.6, 1);
       run_lattice(c, g, true, false, rp, out);
     } else {
-      double *d_pts = c->ar.get<double>("in_pts", 3 * npts + 3);
+      d_pts = c->ar.get<double>("in_pts", 3 * npts + 3);
       SDF_HIP(hipMemcpyAsync(d_pts, points, (size_t)(3 * npts) * sizeof(double),
                              hipMemcpyHostToDevice, c->stream));
       const sdf::Bvh &bvh = need_bvh(c);
@@ -1295,11 +1300,49 @@ sdf_status sdf_eval_distances(sdf_ctx *c, const double *X, int64_t nnp,
       c->mark("end");
       check_err_flags(c);
     }
+    double *d_xp = nullptr;
+    if (want_xp && npts > 0) {
+      d_xp = c->ar.get<double>("out_xp", 3 * npts + 3);
+      if (c->nel == 0) {
+        // no triangle: the reference's zero-initialised xp
+        SDF_HIP(hipMemsetAsync(d_xp, 0, (size_t)(3 * npts) * sizeof(double), c->stream));
+      } else {
+        const sdf::Bvh &bvh = need_bvh(c);
+        c->mark("closest");
+        sdf::launch_closest_points(c->stream, g, lattice ? nullptr : d_pts, npts, c->tri, bvh,
+                                   d_xp, c->d_words);
+        c->mark("end");
+        check_err_flags(c);
+      }
+    }
     if (npts > 0)
       SDF_HIP(hipMemcpyAsync(dist_out, d_dist, (size_t)npts * sizeof(double),
                              hipMemcpyDeviceToHost, c->stream));
+    if (d_xp)
+      SDF_HIP(hipMemcpyAsync(xp_out, d_xp, (size_t)(3 * npts) * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
     SDF_HIP(hipStreamSynchronize(c->stream));
   });
+}
+
+}  // namespace
+
+sdf_status sdf_eval_distances(sdf_ctx *c, const double *X, int64_t nnp,
+                              const int64_t *IEN, int64_t nel,
+                              const double aabb_min[3], double h,
+                              const int64_t N[3], const double *points,
+                              int64_t npts, double *dist_out) {
+  return eval_distances(c, X, nnp, IEN, nel, aabb_min, h, N, points, npts, dist_out, nullptr,
+                        false);
+}
+
+sdf_status sdf_eval_distances_xp(sdf_ctx *c, const double *X, int64_t nnp,
+                                 const int64_t *IEN, int64_t nel,
+                                 const double aabb_min[3], double h,
+                                 const int64_t N[3], const double *points,
+                                 int64_t npts, double *dist_out, double *xp_out) {
+  return eval_distances(c, X, nnp, IEN, nel, aabb_min, h, N, points, npts, dist_out, xp_out,
+                        true);
 }
 
 sdf_status sdf_raycast_signs(sdf_ctx *c, const double *X, int64_t nnp,

@@ -283,6 +283,10 @@ void resolve_pending(hipStream_t s, Arena &ar, const GridP &g, const double *pts
                      uint32_t *d_counts, uint32_t wind_cap, uint32_t *err,
                      int stack = 1024);
 
+// xp of evalDistancesOnTriMesh: exact FP64 closest point on the mesh for
+// every lattice point (pts == nullptr) or caller point, 3 doubles per point.
+void launch_closest_points(hipStream_t s, const GridP &g, const double *pts, int64_t npts,
+                           const Tri64 *tri, const Bvh &bvh, double *xp, uint32_t *err);
 void launch_dist_points_bvh(hipStream_t s, const double *pts, int64_t npts,
                             const Tri64 *tri, const Bvh &bvh, double h,
                             float slack, double *dist, uint32_t *err);

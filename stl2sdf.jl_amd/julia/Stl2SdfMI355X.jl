@@ -34,15 +34,16 @@ function ctx()
 end
 
 function evalDistancesOnTriMesh(mesh::TriangularMesh, grid::Grid, points::Matrix{Float64})
-    dist = Vector{Float64}(undef, size(points, 2))
-    GC.@preserve mesh grid points dist begin
-        _check(ccall((:sdf_eval_distances, LIB), Int32,
+    n = size(points, 2)
+    dist = Vector{Float64}(undef, n)
+    xp = Matrix{Float64}(undef, 3, n)
+    GC.@preserve mesh grid points dist xp begin
+        _check(ccall((:sdf_eval_distances_xp, LIB), Int32,
             (Ptr{Cvoid}, Ptr{Float64}, Int64, Ptr{Int64}, Int64, Ptr{Float64}, Float64,
-             Ptr{Int64}, Ptr{Float64}, Int64, Ptr{Float64}),
+             Ptr{Int64}, Ptr{Float64}, Int64, Ptr{Float64}, Ptr{Float64}),
             ctx(), mesh.X, mesh.nnp, mesh.IEN, mesh.nel, grid.AABB_min, grid.cell_size,
-            grid.N, points, size(points, 2), dist))
+            grid.N, points, n, dist, xp))
     end
-    xp = zeros(Float64, 3, 0)   # closest points are not produced
     return dist, xp
 end
 

@@ -1,5 +1,6 @@
 """VTK ImageData (.vti) output, mirroring exportSdfToVTI
-(src/DataExport/ExportToVTI.jl:22-67).
+(src/DataExport/ExportToVTI.jl:22-67), and the projected-node filter
+SelectProjectedNodes (src/DataExport/DataPostProcess.jl:21-58).
 
 The file carries the same content as the reference's WriteVTK output: a
 point field named ``value_label`` on an ImageData grid with origin
@@ -12,6 +13,28 @@ import zlib
 import numpy as np
 
 _TYPES = {np.dtype(np.float64): "Float64", np.dtype(np.float32): "Float32"}
+
+
+def SelectProjectedNodes(mesh, grid, xp, points):
+    """SelectProjectedNodes(mesh, grid, xp, points) -- DataPostProcess.jl:21-58.
+
+    Keeps the points whose projection is non-zero (sum |xp[:, i]| > 1e-10)
+    and returns (X, Xp, mean_PD, max_PD): X and Xp as 3 x count arrays (the
+    reference's vectors of 3-vectors, same order), and the mean and maximum
+    of |X - Xp|; 0.0 and 0.0 when nothing is kept.  Host-side filtering of
+    arrays the caller already holds, like the reference's loop."""
+    ngp = int(grid.ngp)
+    xp = np.asarray(xp, dtype=np.float64).reshape(3, -1)
+    points = np.asarray(points, dtype=np.float64).reshape(3, -1)
+    if xp.shape[1] != ngp or points.shape[1] != ngp:
+        raise ValueError(f"xp and points must be 3 x {ngp}")
+    keep = np.abs(xp).sum(axis=0) > 1.0e-10
+    X = np.ascontiguousarray(points[:, keep])
+    Xp = np.ascontiguousarray(xp[:, keep])
+    if X.shape[1] == 0:
+        return X, Xp, 0.0, 0.0
+    pd = np.sqrt(((X - Xp) ** 2).sum(axis=0))
+    return X, Xp, float(pd.mean()), float(pd.max())
 
 
 def exportSdfToVTI(filename, grid, values, value_label, smooth=None, block=32768 * 8):

@@ -23,13 +23,16 @@ def _grid_args(grid):
     return _lib.f64_3(grid.AABB_min), float(grid.cell_size), _lib.i64_3(grid.N)
 
 
-def evalDistancesOnTriMesh(mesh, grid, points=None, device=0):
+def evalDistancesOnTriMesh(mesh, grid, points=None, device=0, want_xp=True):
     """evalDistancesOnTriMesh(mesh, grid, points) -- dfOnTriangularMesh.jl:19-209.
 
-    Returns (dist, xp).  dist[i] = min(exact distance, grid.cell_size); xp is
-    not produced by the GPU path and is returned as None.  points=None means
-    the grid's lattice (the usual generateGridPoints(grid) input, which is
-    also recognised when passed explicitly)."""
+    Returns (dist, xp).  dist[i] = min(exact distance, grid.cell_size);
+    xp[:, i] is the exact closest point on the mesh (3 x npts, FP64) at any
+    distance (the reference's is the closest among its candidate triangles,
+    which equals it within ~2.5 cells).  want_xp=False skips xp and returns
+    None for it.  points=None means the grid's lattice (the usual
+    generateGridPoints(grid) input, which is also recognised when passed
+    explicitly)."""
     X, IEN = _mesh_args(mesh)
     amin, h, N = _grid_args(grid)
     if points is None:
@@ -41,10 +44,16 @@ def evalDistancesOnTriMesh(mesh, grid, points=None, device=0):
         pp = _lib.dptr(P)
     dist = np.empty(npts, dtype=np.float64)
     L = _lib.lib()
-    _lib.check(L.sdf_eval_distances(
+    if not want_xp:
+        _lib.check(L.sdf_eval_distances(
+            _lib.ctx(device), _lib.dptr(X), X.shape[1], _lib.i64ptr(IEN), IEN.shape[1],
+            _lib.dptr(amin), h, _lib.i64ptr(N), pp, npts, _lib.dptr(dist)))
+        return dist, None
+    xp = np.empty((3, npts), dtype=np.float64, order="F")
+    _lib.check(L.sdf_eval_distances_xp(
         _lib.ctx(device), _lib.dptr(X), X.shape[1], _lib.i64ptr(IEN), IEN.shape[1],
-        _lib.dptr(amin), h, _lib.i64ptr(N), pp, npts, _lib.dptr(dist)))
-    return dist, None
+        _lib.dptr(amin), h, _lib.i64ptr(N), pp, npts, _lib.dptr(dist), _lib.dptr(xp)))
+    return dist, xp
 
 
 def raycast_sign_detection(mesh, grid, points=None, min_rays=6, max_rays=30,

@@ -36,6 +36,8 @@ SIGNATURES = {
     "sdf_grid_setup": (C.c_int32, [_dp, _dp, C.c_int64, C.c_int64, _dp, _dp, _i64p, _dp, _i64p]),
     "sdf_eval_distances": (C.c_int32, [_vp, _dp, C.c_int64, _i64p, C.c_int64, _dp, C.c_double,
                                        _i64p, _dp, C.c_int64, _dp]),
+    "sdf_eval_distances_xp": (C.c_int32, [_vp, _dp, C.c_int64, _i64p, C.c_int64, _dp,
+                                          C.c_double, _i64p, _dp, C.c_int64, _dp, _dp]),
     "sdf_raycast_signs": (C.c_int32, [_vp, _dp, C.c_int64, _i64p, C.c_int64, _dp, C.c_double,
                                       _i64p, _dp, C.c_int64, C.c_int32, C.c_int32, C.c_double,
                                       C.c_int32, _dp, _dp]),

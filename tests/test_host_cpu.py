@@ -54,3 +54,26 @@ def test_fine_grid_of_rbf_smoothing():
     fg = MeshGrid.Grid(g.AABB_min, g.AABB_max, int(max(g.N)) * 2, 0)
     assert abs(fg.cell_size - g.cell_size / 2) < 1e-15
     assert all(abs(int(a) - 2 * int(b)) <= 1 for a, b in zip(fg.N, g.N))
+
+
+def test_select_projected_nodes():
+    """SelectProjectedNodes (DataPostProcess.jl:21-58): keeps the points with
+    sum |xp| > 1e-10 in order, mean and max of |X - Xp|, 0.0 / 0.0 when none."""
+    from stl2sdf import MeshGrid
+    from stl2sdf.DataExport import SelectProjectedNodes
+    X = np.asfortranarray(np.array([[0.0, 0, 0], [1.0, 0, 0], [0.0, 1, 0]]).T)
+    mesh = MeshGrid.TriangularMesh(X, np.array([[1], [2], [3]]))
+    grid = MeshGrid.Grid(np.zeros(3), np.ones(3), 1, 0)     # 2 x 2 x 2 points
+    pts = MeshGrid.generateGridPoints(grid)
+    xp = np.zeros((3, 8))
+    xp[:, 1] = [3.0, 0.0, 0.0]          # |X - Xp| = 2
+    xp[:, 5] = [1.0, 0.0, 5.0]          # |X - Xp| = 4
+    xp[:, 6] = [1e-11, 0.0, 0.0]        # below the threshold: dropped
+    Xs, Xp, mean_pd, max_pd = SelectProjectedNodes(mesh, grid, xp, pts)
+    assert Xs.shape == (3, 2) and np.array_equal(Xs, pts[:, [1, 5]])
+    assert np.array_equal(Xp, xp[:, [1, 5]])
+    assert mean_pd == 3.0 and max_pd == 4.0
+    Xs, Xp, mean_pd, max_pd = SelectProjectedNodes(mesh, grid, np.zeros((3, 8)), pts)
+    assert Xs.shape == (3, 0) and mean_pd == 0.0 and max_pd == 0.0
+    with pytest.raises(ValueError):
+        SelectProjectedNodes(mesh, grid, np.zeros((3, 7)), pts)

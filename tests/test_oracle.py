@@ -213,6 +213,51 @@ def test_exact_distance_closed_forms_and_clamp():
     assert np.all(d <= np.abs(r - 1.0) + 0.02)
 
 
+def test_exact_closest_point_closed_forms_and_tie_rule():
+    """xp of the exact distance: the projection in the face region, the
+    vertex and edge points elsewhere, and on an exact tie the triangle with
+    the lower index."""
+    X = np.asfortranarray(np.array(T0).T)
+    IEN = np.array([[1], [2], [3]])
+    P = np.asfortranarray(np.array([[0.2, 0.2, 0.7], [-0.3, -0.4, 0.0], [0.5, -2.0, 0.0],
+                                    [2.0, 2.0, 0.0], [0.25, 0.25, 100.0]]).T)
+    _, xp = oracle.dist_exact(X, IEN, P, 10.0, want_xp=True)
+    assert np.allclose(xp[:, 0], [0.2, 0.2, 0.0], rtol=0, atol=1e-15)  # face: foot of the normal
+    assert np.array_equal(xp[:, 1], [0.0, 0.0, 0.0])     # vertex a
+    assert np.array_equal(xp[:, 2], [0.5, 0.0, 0.0])     # edge y = 0
+    assert np.allclose(xp[:, 3], [0.5, 0.5, 0.0], atol=1e-15)  # hypotenuse midpoint
+    assert np.allclose(xp[:, 4], [0.25, 0.25, 0.0], rtol=0, atol=1e-15)  # beyond h: not clamped
+    # two mirror triangles at z = +-1; the origin is 1 from both
+    up = [[0.0, 0, 1], [1.0, 0, 1], [0.0, 1, 1]]
+    dn = [[0.0, 0, -1], [1.0, 0, -1], [0.0, 1, -1]]
+    X2 = np.asfortranarray(np.array(up + dn).T)
+    P0 = np.asfortranarray(np.array([[0.1, 0.1, 0.0]]).T)
+    for first, z in ((np.array([[1, 4], [2, 5], [3, 6]]), 1.0),
+                     (np.array([[4, 1], [5, 2], [6, 3]]), -1.0)):
+        _, xp = oracle.dist_exact(X2, first, P0, 10.0, want_xp=True)
+        assert np.allclose(xp[:, 0], [0.1, 0.1, z], rtol=0, atol=1e-15)
+
+
+def test_exact_closest_point_lies_on_mesh_and_realises_distance():
+    """Independent of the closest-point formula: |p - xp| equals the exact
+    distance (pinned above against an area-coordinate formula) and xp is on
+    the mesh (its own distance to the mesh is zero)."""
+    mesh, grid = _cases.star_case(n=24, n_max=16, seed=3)
+    rng = np.random.default_rng(11)
+    lo, hi = mesh.X.min(1), mesh.X.max(1)
+    P = np.asfortranarray(rng.uniform(lo - 1.0, hi + 1.0, size=(400, 3)).T)
+    d, xp = oracle.dist_exact(mesh.X, mesh.IEN, P, 1e9, want_xp=True)
+    r = np.linalg.norm(P - xp, axis=0)
+    assert np.all(np.abs(r - d) <= 1e-12 * (1 + d))
+    d_on = oracle.dist_exact(mesh.X, mesh.IEN, np.asfortranarray(xp), 1e9)
+    assert d_on.max() <= 1e-12
+    for k in range(0, 400, 40):   # brute force with the independent formula
+        V = mesh.X[:, mesh.IEN - 1]
+        best = min(_dist_np(P[:, k], V[:, 0, e], V[:, 1, e], V[:, 2, e])
+                   for e in range(mesh.nel))
+        assert abs(best - d[k]) <= 1e-12 * (1 + best)
+
+
 def test_reference_faithful_distance_bounds():
     mesh, grid = _cases.icosphere_case(subdiv=3, n_max=16)
     P = MeshGrid.generateGridPoints(grid)
