@@ -97,7 +97,8 @@ struct sdf_ctx {
   // helper streams for independent build chains (line hits, LBVH); default
   // priority, joined to the main stream with events
   hipStream_t aux[2] = {nullptr, nullptr};
-  hipEvent_t join[3] = {nullptr, nullptr, nullptr};
+  // [0] pass start, [1] line hits done, [2] LBVH done, [3] tile chain done
+  hipEvent_t join[4] = {nullptr, nullptr, nullptr, nullptr};
   std::mutex mu;
   sdf::Arena ar;
   // device mesh
@@ -227,11 +228,11 @@ const sdf::Bvh &need_bvh(sdf_ctx *c) {
   return c->bvh;
 }
 
-// Build the LBVH on helper stream q, after the work recorded in join[0] (the
-// start of the current lattice pass); joined later through join[2].
+// Build the LBVH on helper stream q after the work recorded in join[3] (the
+// end of the current pass's tile chain); joined later through join[2].
 void bvh_async(sdf_ctx *c, int q) {
   if (c->bvh_ready) return;
-  SDF_HIP(hipStreamWaitEvent(c->aux[q], c->join[0], 0));
+  SDF_HIP(hipStreamWaitEvent(c->aux[q], c->join[3], 0));
   c->ar.scope = "bvh/";
   c->bvh = sdf::build_lbvh(c->aux[q], c->ar, c->tri, c->nel);
   c->ar.scope.clear();
@@ -319,13 +320,12 @@ enum : int {
 
 // Stage 1: the build chains.  Tile bins and distance work items run on the
 // main stream; the axis-line hits on helper stream 0 (needed by the sign
-// stage) and the LBVH on helper stream 1 (needed only by the vote stage),
-// both forked from the main stream at the start of the pass.  Issue order
-// matters because launching the pass's ~100 kernels takes the host longer
-// than the GPU needs for the small ones, and a running distance kernel
-// leaves no room for other streams: tile counting, then the whole line
-// chain, then the rest of the tile chain (stage_bins_tiles), the distance
-// kernel (stage_dist), and only then the LBVH (stage_bins_bvh).
+// stage) beside them, and the LBVH on helper stream 1 (needed only by the
+// vote stage) from the end of the tile chain, beside the distance and sign
+// kernels.  Issue order: tile counting, the line chain, the rest of the
+// tile chain (stage_bins_tiles), the distance and sign kernels (stage_dist,
+// stage_sign), then the LBVH (stage_bins_bvh), so the host's ~100 launches
+// never hold back a kernel on the critical path.
 void stage_bins_tiles(sdf_ctx *c, LatticeRun &L) {
   const sdf::GridP &g = L.g;
   const int64_t npts = grid_points(g);
@@ -356,6 +356,10 @@ void stage_bins_tiles(sdf_ctx *c, LatticeRun &L) {
     sdf::launch_fill_u32(c->stream, (uint32_t *)L.d2buf,
                          (int64_t)bins.nchunks * sdf::kTilePts, 0x7F800000u);
   }
+  // the LBVH starts when the tile chain is done: beside the compute-bound
+  // distance and sign kernels it costs little, beside the latency-bound
+  // tile chain it stretches every small kernel
+  SDF_HIP(hipEventRecord(c->join[3], c->stream));
   L.cap = c->caps.pend ? c->caps.pend
                        : (uint32_t)(npts < (1 << 22) ? (npts > 0 ? npts : 1) : (1 << 22));
   L.cap0 = c->caps.pend0 ? c->caps.pend0
@@ -449,9 +453,9 @@ void stage_resolve(sdf_ctx *c, LatticeRun &L) {
 
 void lattice_pass(sdf_ctx *c, LatticeRun &L) {
   stage_bins_tiles(c, L);
-  stage_dist(c, L);
-  stage_bins_bvh(c, L);
+  stage_dist(c, L);  // the learned variant builds the LBVH itself, first
   stage_sign(c, L);
+  stage_bins_bvh(c, L);
   stage_resolve(c, L);
 }
 
@@ -721,7 +725,7 @@ sdf_ctx *batch_child(sdf_batch *b, int i) {
   k->stream = b->s[i % 4];
   k->aux[0] = b->aux[i % 4][0];
   k->aux[1] = b->aux[i % 4][1];
-  for (int q = 0; q < 3; q++) SDF_HIP(hipEventCreateWithFlags(&k->join[q], hipEventDisableTiming));
+  for (int q = 0; q < 4; q++) SDF_HIP(hipEventCreateWithFlags(&k->join[q], hipEventDisableTiming));
   SDF_HIP(hipMalloc(&k->d_words, 256));
   k->d_status = b->d_status + 16 * i;
   k->own_status = false;
@@ -738,7 +742,7 @@ void batch_free(sdf_batch *b) {
   for (sdf_ctx *k : b->kids) {
     if (!k) continue;
     k->ar.release();
-    for (int q = 0; q < 3; q++)
+    for (int q = 0; q < 4; q++)
       if (k->join[q]) (void)hipEventDestroy(k->join[q]);
     if (k->d_words) (void)hipFree(k->d_words);
     delete k;
@@ -967,7 +971,7 @@ sdf_status sdf_ctx_create(int32_t device, sdf_ctx **out) {
       c->stream = c->own;
       for (int q = 0; q < 2; q++)
         SDF_HIP(hipStreamCreateWithFlags(&c->aux[q], hipStreamNonBlocking));
-      for (int q = 0; q < 3; q++)
+      for (int q = 0; q < 4; q++)
         SDF_HIP(hipEventCreateWithFlags(&c->join[q], hipEventDisableTiming));
       SDF_HIP(hipMalloc(&c->d_words, 256));
       SDF_HIP(hipMalloc(&c->d_status, 16 * sizeof(uint64_t)));
@@ -991,7 +995,7 @@ sdf_status sdf_ctx_destroy(sdf_ctx *c) {
         (void)hipStreamSynchronize(c->aux[q]);
         (void)hipStreamDestroy(c->aux[q]);
       }
-    for (int q = 0; q < 3; q++)
+    for (int q = 0; q < 4; q++)
       if (c->join[q]) (void)hipEventDestroy(c->join[q]);
     if (c->d_words) (void)hipFree(c->d_words);
     if (c->d_status && c->own_status) (void)hipFree(c->d_status);

@@ -235,36 +235,70 @@ __global__ void karras_kernel(const uint32_t *__restrict__ codes, int64_t n,
 // of that range, taken from a segment tree over the sorted leaves (level L
 // entry k covers leaves [k 2^L, (k+1) 2^L)), O(log n) boxes per node.
 // Unions are exact, so the result equals a bottom-up refit.
-__global__ void seg_leaf_kernel(const float *__restrict__ leaf_box,
-                                const uint32_t *__restrict__ leaf_tri, int64_t n,
-                                float *__restrict__ seg0) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t t = leaf_tri[i];
-    for (int q = 0; q < 6; q++) seg0[6 * i + q] = leaf_box[6 * t + q];
-  }
-}
-
-__global__ void seg_level_kernel(const float *__restrict__ below, int64_t nb,
-                                 float *__restrict__ level, int64_t m) {
-  for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < m;
-       k += (int64_t)gridDim.x * blockDim.x) {
-    float b[6];
-    for (int q = 0; q < 6; q++) b[q] = below[12 * k + q];
-    if (2 * k + 1 < nb) {
-      for (int q = 0; q < 3; q++) {
-        b[q] = fminf(b[q], below[12 * k + 6 + q]);
-        b[3 + q] = fmaxf(b[3 + q], below[12 * k + 9 + q]);
-      }
-    }
-    for (int q = 0; q < 6; q++) level[6 * k + q] = b[q];
-  }
-}
-
 struct SegLevels {
   int64_t off[64];  // float offset of each level inside the segment array
   int nlev;
 };
+
+// Levels lev_in .. lev_in + nlev - 1 of the segment tree in one launch: a
+// workgroup reduces 512 consecutive entries of level lev_in - 1 through up
+// to 9 levels in LDS and writes every level.  Entries past a level's end
+// are empty boxes, and a union with an empty box returns the other operand
+// exactly, so each entry equals the pairwise union of its two children.
+// leaf_tri != nullptr: the input is the leaf boxes in sorted order, also
+// written as level 0.
+constexpr int kSegSpan = 512;
+constexpr int kSegLevelsPerLaunch = 9;  // log2(kSegSpan)
+
+__global__ __launch_bounds__(256) void seg_levels_kernel(float *__restrict__ seg, SegLevels L,
+                                                         int lev_in, int nlev, int64_t m_in,
+                                                         const float *__restrict__ leaf_box,
+                                                         const uint32_t *__restrict__ leaf_tri) {
+  __shared__ float A[kSegSpan][6];
+  __shared__ float Bf[kSegSpan / 2][6];
+  const int tid = threadIdx.x;
+  const int64_t base = (int64_t)blockIdx.x * kSegSpan;
+  const float *below = seg + L.off[lev_in - 1];
+  for (int i = tid; i < kSegSpan; i += blockDim.x) {
+    const int64_t gi = base + i;
+    float b[6];
+    if (gi < m_in) {
+      if (leaf_tri) {
+        const int64_t t = leaf_tri[gi];
+        for (int q = 0; q < 6; q++) b[q] = leaf_box[6 * t + q];
+        for (int q = 0; q < 6; q++) seg[6 * gi + q] = b[q];
+      } else {
+        for (int q = 0; q < 6; q++) b[q] = below[6 * gi + q];
+      }
+    } else {
+      b[0] = b[1] = b[2] = __builtin_huge_valf();
+      b[3] = b[4] = b[5] = -__builtin_huge_valf();
+    }
+    for (int q = 0; q < 6; q++) A[i][q] = b[q];
+  }
+  __syncthreads();
+  int64_t size = m_in;
+  for (int r = 1; r <= nlev; r++) {
+    size = (size + 1) / 2;
+    const int cnt = kSegSpan >> r;
+    float(*src)[6] = (r & 1) ? A : Bf;
+    float(*dst)[6] = (r & 1) ? Bf : A;
+    if (tid < cnt) {
+      float b[6];
+      for (int q = 0; q < 3; q++) {
+        b[q] = fminf(src[2 * tid][q], src[2 * tid + 1][q]);
+        b[3 + q] = fmaxf(src[2 * tid][3 + q], src[2 * tid + 1][3 + q]);
+      }
+      for (int q = 0; q < 6; q++) dst[tid][q] = b[q];
+      const int64_t k = (int64_t)blockIdx.x * cnt + tid;
+      if (k < size) {
+        float *out = seg + L.off[lev_in - 1 + r];
+        for (int q = 0; q < 6; q++) out[6 * k + q] = b[q];
+      }
+    }
+    __syncthreads();
+  }
+}
 
 __device__ inline void seg_query(const float *__restrict__ seg, const SegLevels &L,
                                  int64_t lo, int64_t hi, float *b) {
@@ -362,8 +396,11 @@ Bvh build_lbvh(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t n) {
   uint32_t *cb = ar.get<uint32_t>("bvh_cbounds", 8);
   launch_fill_u32(s, cb, 3, 0xFFFFFFFFu);
   launch_fill_u32(s, cb + 3, 3, 0u);
-  hipLaunchKernelGGL(leaf_boxes_kernel, dim3(blocks_for(n, 256)), dim3(256), 0,
-                     s, tri, n, b.leaf_box, cb);
+  // few workgroups: each adds its share of the centroid bounds with six
+  // atomics on the same six words, which serialise
+  const int lb_blocks = blocks_for(n, 256) < 512 ? blocks_for(n, 256) : 512;
+  hipLaunchKernelGGL(leaf_boxes_kernel, dim3(lb_blocks), dim3(256), 0, s, tri, n, b.leaf_box,
+                     cb);
   SDF_HIP(hipGetLastError());
   uint32_t *codes = ar.get<uint32_t>("bvh_codes", n);
   uint32_t *leaf_tri = ar.get<uint32_t>("bvh_leaf_tri", n);
@@ -391,17 +428,16 @@ Bvh build_lbvh(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t n) {
     m = (m + 1) / 2;
   }
   float *seg = ar.get<float>("bvh_seg", 6 * total);
-  hipLaunchKernelGGL(seg_leaf_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s,
-                     b.leaf_box, leaf_tri, n, seg);
-  SDF_HIP(hipGetLastError());
+  // level 0 (sorted leaf boxes) and the levels above, 9 per launch
   m = n;
-  int lev = 1;
-  for (; lev < L.nlev; lev++) {
-    const int64_t mm = (m + 1) / 2;
-    hipLaunchKernelGGL(seg_level_kernel, dim3(blocks_for(mm, 256)), dim3(256), 0,
-                       s, seg + L.off[lev - 1], m, seg + L.off[lev], mm);
+  for (int lev = 1; lev < L.nlev;) {
+    const int k = L.nlev - lev < kSegLevelsPerLaunch ? L.nlev - lev : kSegLevelsPerLaunch;
+    hipLaunchKernelGGL(seg_levels_kernel, dim3((unsigned)((m + kSegSpan - 1) / kSegSpan)),
+                       dim3(256), 0, s, seg, L, lev, k, m, (const float *)b.leaf_box,
+                       lev == 1 ? (const uint32_t *)leaf_tri : (const uint32_t *)nullptr);
     SDF_HIP(hipGetLastError());
-    m = mm;
+    for (int q = 0; q < k; q++) m = (m + 1) / 2;
+    lev += k;
   }
 
   hipLaunchKernelGGL(node_boxes_kernel, dim3(blocks_for(n - 1, 256)), dim3(256),

@@ -93,6 +93,40 @@ __device__ inline bool inline_parity(const LineInline &r, const double *xs,
   return true;
 }
 
+// Both rays of a line for every point whose coordinate lies in [lo, hi],
+// when no crossing is within its rounding margin of that range: each
+// crossing is then right of every point (fl(c - hi) > eps + D) or left of
+// every point (fl(c - lo) < -eps - D).  Rounded subtraction is monotone, so
+// every point's comparisons in inline_parity come out the same as at the
+// range's ends, none of them ambiguous; the parities are shared.  Returns
+// false when some crossing is near the range (or the summary is not
+// usable); the caller then classifies each point.
+__device__ inline bool inline_uniform(const LineInline &r, double lo, double hi, int *pos,
+                                      int *neg) {
+  const uint32_t n = r.info & 0xFFu;
+  if ((r.info & kLineExact) || n > (uint32_t)kInlineHits) return false;
+  const int ncl = (int)((r.info >> 8) & 0xFFu);
+  int first_pos = -1, last_neg = -1;
+  bool uni = true;
+#pragma unroll
+  for (int h2 = 0; h2 < kInlineHits; h2++) {
+    if (h2 < (int)n) {
+      const double D = (double)r.delta[h2];
+      const int cid = r.cid[h2];
+      if (r.c[h2] - hi > kRayEps + D) {
+        if (first_pos < 0) first_pos = cid;
+      } else if (r.c[h2] - lo < -kRayEps - D) {
+        last_neg = cid;
+      } else {
+        uni = false;
+      }
+    }
+  }
+  *pos = first_pos >= 0 ? ((ncl - first_pos) & 1) : 0;
+  *neg = last_neg >= 0 ? ((last_neg + 1) & 1) : 0;
+  return uni;
+}
+
 // One distance work item (tile, chunk of <= clen triangles) for the 64-lane
 // block: the chunk's triangles are staged in LDS as precomputed records and
 // every lane evaluates its tile point against all of them.  Uniform per
@@ -417,6 +451,9 @@ __global__ __launch_bounds__(256) void learned_fallback_kernel(
 struct FusedShared {
   LineInline lx[kSignZ][8];  // x-line (k0 + q, 8 by + w)
   LineInline ly[kSignZ][8];  // y-line (k0 + q, 8 bx + w)
+  // per x- and y-line: 1 | pos << 1 | neg << 2 when both rays are the same
+  // for all 8 points of the line in the brick (inline_uniform), else 0
+  uint8_t ux[kSignZ][8], uy[kSignZ][8];
   float d2[4][kTilePts];     // tiles (tz local, ty local) = (t >> 1, t & 1)
   uint32_t slot[4];          // first work item + 1 of each tile, 0: no distance work
 };
@@ -456,16 +493,37 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S)
   // ---- stage: the lane's own z-line summary (registers), the brick's x- and
   // y-line summaries and its tiles' distance slots (LDS)
   LineInline rz;
+  bool zuni = false;  // the lane's z-line is the same for its nk points
+  int zp = 0, zn = 0;
   if (do_sign) {
     if (in_grid) rz = inl[base2 + (int64_t)j * g.nx + i];
     const int q = (lane >> 3) & 3, w = lane & 7, kk = k0 + q;
+    int p, n;
+    uint8_t u = 0;
     if (lane < 32) {
       const int32_t jj = by * 8 + w;
-      if (kk < g.nz && jj < g.ny) S.lx[q][w] = inl[(int64_t)kk * g.ny + jj];
+      if (kk < g.nz && jj < g.ny) {
+        const LineInline r = inl[(int64_t)kk * g.ny + jj];
+        S.lx[q][w] = r;
+        if (inline_uniform(r, lattice_coord(g.ox, g.h, bx * 8),
+                           lattice_coord(g.ox, g.h, min(bx * 8 + 7, g.nx - 1)), &p, &n))
+          u = (uint8_t)(1 | (p << 1) | (n << 2));
+      }
+      S.ux[q][w] = u;
     } else {
       const int32_t ii = bx * 8 + w;
-      if (kk < g.nz && ii < g.nx) S.ly[q][w] = inl[base1 + (int64_t)kk * g.nx + ii];
+      if (kk < g.nz && ii < g.nx) {
+        const LineInline r = inl[base1 + (int64_t)kk * g.nx + ii];
+        S.ly[q][w] = r;
+        if (inline_uniform(r, lattice_coord(g.oy, g.h, by * 8),
+                           lattice_coord(g.oy, g.h, min(by * 8 + 7, g.ny - 1)), &p, &n))
+          u = (uint8_t)(1 | (p << 1) | (n << 2));
+      }
+      S.uy[q][w] = u;
     }
+    if (in_grid)
+      zuni = inline_uniform(rz, lattice_coord(g.oz, g.h, k0),
+                            lattice_coord(g.oz, g.h, k0 + nk - 1), &zp, &zn);
   }
   if (d2buf) {
     if (lane < 4) {
@@ -497,8 +555,16 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S)
     int zpos[kSignZ], zneg[kSignZ];
     bool zok[kSignZ];
     if (do_sign) {
-      // z-line (i, j): one summary classifies all 4 points
-      if (!inline_parity<kSignZ>(rz, zq, zpos, zneg, zok)) {
+      // z-line (i, j): shared parities, else one summary classifies all 4
+      // points, else the line's crossing list
+      if (zuni) {
+#pragma unroll
+        for (int q = 0; q < kSignZ; q++) {
+          zpos[q] = zp;
+          zneg[q] = zn;
+          zok[q] = true;
+        }
+      } else if (!inline_parity<kSignZ>(rz, zq, zpos, zneg, zok)) {
         const int64_t Lz = base2 + (int64_t)j * g.nx + i;
         const uint64_t hb = line_off[Lz], he = line_off[Lz + 1];
         const uint32_t mz = meta[Lz];
@@ -529,18 +595,29 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S)
         int inside = zpos[q] + zneg[q];
         int a, bpar;
         bool okq, amb = !zok[q];
-        const LineInline &rx = S.lx[q][lane >> 3];
-        const LineInline &ry = S.ly[q][lane & 7];
-        if (!inline_parity<1>(rx, &x, &a, &bpar, &okq) || !okq) {
-          const int64_t Lx = (int64_t)k * g.ny + j;
-          amb |= !axis_pair_parity_fast(fast, line_off[Lx], line_off[Lx + 1], meta[Lx], x, &a,
-                                        &bpar);
+        const uint32_t ux = S.ux[q][lane >> 3], uy = S.uy[q][lane & 7];
+        if (ux & 1u) {
+          a = (int)((ux >> 1) & 1u);
+          bpar = (int)((ux >> 2) & 1u);
+        } else {
+          const LineInline &rx = S.lx[q][lane >> 3];
+          if (!inline_parity<1>(rx, &x, &a, &bpar, &okq) || !okq) {
+            const int64_t Lx = (int64_t)k * g.ny + j;
+            amb |= !axis_pair_parity_fast(fast, line_off[Lx], line_off[Lx + 1], meta[Lx], x,
+                                          &a, &bpar);
+          }
         }
         inside += a + bpar;
-        if (!inline_parity<1>(ry, &y, &a, &bpar, &okq) || !okq) {
-          const int64_t Ly = base1 + (int64_t)k * g.nx + i;
-          amb |= !axis_pair_parity_fast(fast, line_off[Ly], line_off[Ly + 1], meta[Ly], y, &a,
-                                        &bpar);
+        if (uy & 1u) {
+          a = (int)((uy >> 1) & 1u);
+          bpar = (int)((uy >> 2) & 1u);
+        } else {
+          const LineInline &ry = S.ly[q][lane & 7];
+          if (!inline_parity<1>(ry, &y, &a, &bpar, &okq) || !okq) {
+            const int64_t Ly = base1 + (int64_t)k * g.nx + i;
+            amb |= !axis_pair_parity_fast(fast, line_off[Ly], line_off[Ly + 1], meta[Ly], y,
+                                          &a, &bpar);
+          }
         }
         inside += a + bpar;
         if (amb) {

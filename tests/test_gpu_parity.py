@@ -156,3 +156,33 @@ def test_large_triangles_cube_fine_grid():
     mesh = MeshGrid.TriangularMesh(np.asfortranarray(v.T), np.asfortranarray((f + 1).T))
     grid = MeshGrid.Grid(np.zeros(3), np.ones(3), 98, 3)
     _check(mesh, grid)
+
+
+def test_list_capacity_growth_on_one_context():
+    """A context sized on a small mesh, then given a mesh with many more
+    (tile, triangle) pairs and axis-line hits: the first pass overflows its
+    lists, reads only what was stored, and the call repeats with larger
+    capacities; the results equal the oracle's."""
+    import ctypes as C
+    L = _lib.lib()
+    h = C.c_void_p()
+    _lib.check(L.sdf_ctx_create(0, C.byref(h)))
+    try:
+        for n, n_max in ((12, 10), (120, 30)):
+            mesh, grid = _cases.star_case(n=n, n_max=n_max)
+            X = np.asfortranarray(mesh.X, np.float64)
+            IEN = np.asfortranarray(mesh.IEN, np.int64)
+            amin, N = _lib.f64_3(grid.AABB_min), _lib.i64_3(grid.N)
+            sdf = np.empty(int(grid.ngp))
+            print(f"case: nel={mesh.nel} ngp={grid.ngp}", flush=True)
+            _lib.check(L.sdf_signed_distance(h, _lib.dptr(X), X.shape[1], _lib.i64ptr(IEN),
+                                             IEN.shape[1], _lib.dptr(amin),
+                                             float(grid.cell_size), _lib.i64ptr(N),
+                                             _lib.dptr(sdf)))
+            P = MeshGrid.generateGridPoints(grid)
+            d_ref = oracle.dist_exact(mesh.X, mesh.IEN, P, grid.cell_size)
+            s_ref, _ = oracle.signs(mesh.X, mesh.IEN, P)
+            assert np.array_equal(np.signbit(sdf), s_ref < 0)   # d * s keeps the sign at 0
+            assert np.max(np.abs(np.abs(sdf) - d_ref)) <= DIST_TOL * _cases.diag(mesh)
+    finally:
+        _lib.check(L.sdf_ctx_destroy(h))
