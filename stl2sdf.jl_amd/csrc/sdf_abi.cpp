@@ -240,30 +240,42 @@ void bvh_async(sdf_ctx *c, int q) {
   SDF_HIP(hipEventRecord(c->join[2], c->aux[q]));
 }
 
-// Adaptive vote for the *n_dev items of a list (capacity cap) starting at
-// ray r0 (all rays below r0 already counted in the items' state).  Items
-// that need more than the first min_rays rays are carried to a second pass.
-// cnt: 8 device words (two phases x 4); winding counts land in cnt[2], cnt[6].
+// Adaptive vote for the *n_dev items of a list (capacity cap) whose rays
+// below r0 are already counted in their state.  Phases: the first
+// min_rays rays (when r0 < min_rays), the first batch of 4, then the rest;
+// each phase hands the items it cannot finish to the next, so later (and
+// longer) phases trace rays only for the survivors.  cnt: 4 device words
+// per phase (at most 12); winding counts land in word 2 of each phase.
 void resolve_items(sdf_ctx *c, const sdf::GridP &g, const double *d_pts,
                    const sdf::RayParams &rp, sdf::Pending *items, const uint32_t *n_dev,
-                   uint32_t cap, int r0, sdf::FusedOut out, uint32_t *cnt,
-                   const char *next_name) {
+                   uint32_t cap, int r0, sdf::FusedOut out, uint32_t *cnt) {
   if (cap == 0) return;
   const sdf::Bvh &bvh = c->bvh;
   const int rmax = rp.max_rays < sdf::kNumDirs ? rp.max_rays : sdf::kNumDirs;
-  if (r0 < rp.min_rays) {
-    sdf::Pending *next = c->ar.get<sdf::Pending>(next_name, cap);
-    sdf::resolve_pending(c->stream, c->ar, g, d_pts, c->tri, c->nel, bvh, rp, items, n_dev,
-                         cap, r0, rp.min_rays, 0, out, next, cnt, c->caps.wind, c->d_words,
-                         c->ray_stack);
-    sdf::resolve_pending(c->stream, c->ar, g, d_pts, c->tri, c->nel, bvh, rp, next,
-                         cnt + 1, cap, rp.min_rays, rmax, 1, out, nullptr, cnt + 4,
+  int ph[3][2], np = 0, a = r0;
+  if (a < rp.min_rays) {
+    ph[np][0] = a;
+    ph[np++][1] = rp.min_rays;
+    a = rp.min_rays;
+  }
+  if (a + 4 < rmax) {
+    ph[np][0] = a;
+    ph[np++][1] = a + 4;
+    a += 4;
+  }
+  ph[np][0] = a;
+  ph[np++][1] = rmax;
+  static const char *names[2] = {"res_next0", "res_next1"};
+  const sdf::Pending *cur = items;
+  const uint32_t *cur_n = n_dev;
+  for (int p = 0; p < np; p++) {
+    const bool last = p + 1 == np;
+    sdf::Pending *next = last ? nullptr : c->ar.get<sdf::Pending>(names[p], cap);
+    sdf::resolve_pending(c->stream, c->ar, g, d_pts, c->tri, c->nel, bvh, rp, cur, cur_n, cap,
+                         ph[p][0], ph[p][1], last ? 1 : 0, out, next, nullptr, cap, cnt + 4 * p,
                          c->caps.wind, c->d_words, c->ray_stack);
-  } else {
-    sdf::launch_fill_u32(c->stream, cnt + 4, 4, 0u);
-    sdf::resolve_pending(c->stream, c->ar, g, d_pts, c->tri, c->nel, bvh, rp, items, n_dev,
-                         cap, r0, rmax, 1, out, nullptr, cnt, c->caps.wind, c->d_words,
-                         c->ray_stack);
+    cur = next;
+    cur_n = cnt + 4 * p + 1;
   }
 }
 
@@ -428,10 +440,16 @@ void stage_resolve(sdf_ctx *c, LatticeRun &L) {
       SDF_HIP(hipStreamWaitEvent(c->stream, c->join[2], 0));
     }
     c->mark("resolve");
-    resolve_items(c, L.g, nullptr, L.rp, L.pend, c->d_words + 1, L.cap, 6, L.out, cnt,
-                  "res_nextA");
-    resolve_items(c, L.g, nullptr, L.rp, L.pend0, c->d_words + 2, L.cap0, 0, L.out, cnt + 8,
-                  "res_nextB");
+    // points whose axis-ray decisions fell inside a rounding margin vote
+    // from ray 1; those still open after the first min_rays rays join the
+    // list of points the six axis rays left undecided
+    if (L.cap0 > 0)
+      sdf::resolve_pending(c->stream, c->ar, L.g, nullptr, c->tri, c->nel, c->bvh, L.rp,
+                           L.pend0, c->d_words + 2, L.cap0, 0, L.rp.min_rays, 0, L.out, L.pend,
+                           c->d_words + 1, L.cap, cnt + 8, c->caps.wind, c->d_words,
+                           c->ray_stack);
+    resolve_items(c, L.g, nullptr, L.rp, L.pend, c->d_words + 1, L.cap, L.rp.min_rays, L.out,
+                  cnt);
   }
   c->mark("end");
   sdf::StatusSrc src;
@@ -1393,12 +1411,12 @@ sdf_status sdf_raycast_signs(sdf_ctx *c, const double *X, int64_t nnp,
         sdf::launch_fill_u32(c->stream, nd, 1, (uint32_t)npts);
         uint32_t *cnt = c->d_words + 4;
         c->mark("resolve");
-        resolve_items(c, g, d_pts, rp, pend, nd, (uint32_t)npts, 0, out, cnt, "res_nextA");
+        resolve_items(c, g, d_pts, rp, pend, nd, (uint32_t)npts, 0, out, cnt);
         c->mark("end");
-        uint32_t hc[8];
+        uint32_t hc[12];
         SDF_HIP(hipMemcpyAsync(hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, c->stream));
         SDF_HIP(hipStreamSynchronize(c->stream));
-        const uint32_t w = std::max(hc[2], hc[6]);
+        const uint32_t w = std::max(std::max(hc[2], hc[6]), hc[10]);
         if (w <= c->caps.wind || attempt >= 2) break;
         c->caps.wind = w + w / 8 + 64;
       }

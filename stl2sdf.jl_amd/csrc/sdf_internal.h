@@ -272,15 +272,16 @@ struct Bvh;
 // the winding fallback.  pts == nullptr: lattice points of g.
 // Adaptive vote for *n_dev pending items (capacity cap) over rays r0..r1-1,
 // with no host waits: grids are fixed, counts are read on the device.
-// final_phase 0 carries undecided items to `next` (count d_counts[1]);
+// final_phase 0 carries items whose vote needs a ray >= r1 to `next`
+// (count *nnext, or d_counts[1] when nnext is null; capacity next_cap);
 // otherwise every item is finished.  Items that still need the winding
 // fallback (count d_counts[2]) are finished if it is <= wind_cap (else the
 // caller repeats the call with a larger wind_cap).  d_counts: 4 words.
 void resolve_pending(hipStream_t s, Arena &ar, const GridP &g, const double *pts,
                      const Tri64 *tri, int64_t nel, const Bvh &bvh, const RayParams &rp,
                      const Pending *items, const uint32_t *n_dev, uint32_t cap, int r0,
-                     int r1, int final_phase, FusedOut out, Pending *next,
-                     uint32_t *d_counts, uint32_t wind_cap, uint32_t *err,
+                     int r1, int final_phase, FusedOut out, Pending *next, uint32_t *nnext,
+                     uint32_t next_cap, uint32_t *d_counts, uint32_t wind_cap, uint32_t *err,
                      int stack = 1024);
 
 // xp of evalDistancesOnTriMesh: exact FP64 closest point on the mesh for

@@ -240,63 +240,55 @@ struct SegLevels {
   int nlev;
 };
 
-// Levels lev_in .. lev_in + nlev - 1 of the segment tree in one launch: a
-// workgroup reduces 512 consecutive entries of level lev_in - 1 through up
-// to 9 levels in LDS and writes every level.  Entries past a level's end
-// are empty boxes, and a union with an empty box returns the other operand
-// exactly, so each entry equals the pairwise union of its two children.
+// Levels lev_in .. lev_in + nlev - 1 (nlev <= 6) of the segment tree in one
+// launch, without LDS (so it runs beside kernels that hold the LDS): each
+// wavefront takes 64 consecutive entries of level lev_in - 1, and after r
+// cross-lane butterfly steps every lane holds the union of its aligned
+// block of 2^r entries, i.e. the level lev_in - 1 + r entry, which the
+// block's first lane writes.  Entries past a level's end are empty boxes,
+// and a union with an empty box returns the other operand exactly, so
+// each entry equals the pairwise union of its two children.
 // leaf_tri != nullptr: the input is the leaf boxes in sorted order, also
 // written as level 0.
-constexpr int kSegSpan = 512;
-constexpr int kSegLevelsPerLaunch = 9;  // log2(kSegSpan)
+constexpr int kSegSpan = 64;
+constexpr int kSegLevelsPerLaunch = 6;  // log2(kSegSpan)
 
-__global__ __launch_bounds__(256) void seg_levels_kernel(float *__restrict__ seg, SegLevels L,
-                                                         int lev_in, int nlev, int64_t m_in,
-                                                         const float *__restrict__ leaf_box,
-                                                         const uint32_t *__restrict__ leaf_tri) {
-  __shared__ float A[kSegSpan][6];
-  __shared__ float Bf[kSegSpan / 2][6];
-  const int tid = threadIdx.x;
-  const int64_t base = (int64_t)blockIdx.x * kSegSpan;
-  const float *below = seg + L.off[lev_in - 1];
-  for (int i = tid; i < kSegSpan; i += blockDim.x) {
-    const int64_t gi = base + i;
-    float b[6];
-    if (gi < m_in) {
-      if (leaf_tri) {
-        const int64_t t = leaf_tri[gi];
-        for (int q = 0; q < 6; q++) b[q] = leaf_box[6 * t + q];
-        for (int q = 0; q < 6; q++) seg[6 * gi + q] = b[q];
-      } else {
-        for (int q = 0; q < 6; q++) b[q] = below[6 * gi + q];
-      }
+__global__ __launch_bounds__(64) void seg_levels_kernel(float *__restrict__ seg, SegLevels L,
+                                                        int lev_in, int nlev, int64_t m_in,
+                                                        const float *__restrict__ leaf_box,
+                                                        const uint32_t *__restrict__ leaf_tri) {
+  const int lane = threadIdx.x;
+  const int64_t gi = (int64_t)blockIdx.x * kSegSpan + lane;
+  float b[6];
+  if (gi < m_in) {
+    if (leaf_tri) {
+      const int64_t t = leaf_tri[gi];
+      for (int q = 0; q < 6; q++) b[q] = leaf_box[6 * t + q];
+      for (int q = 0; q < 6; q++) seg[6 * gi + q] = b[q];
     } else {
-      b[0] = b[1] = b[2] = __builtin_huge_valf();
-      b[3] = b[4] = b[5] = -__builtin_huge_valf();
+      const float *below = seg + L.off[lev_in - 1];
+      for (int q = 0; q < 6; q++) b[q] = below[6 * gi + q];
     }
-    for (int q = 0; q < 6; q++) A[i][q] = b[q];
+  } else {
+    b[0] = b[1] = b[2] = __builtin_huge_valf();
+    b[3] = b[4] = b[5] = -__builtin_huge_valf();
   }
-  __syncthreads();
   int64_t size = m_in;
   for (int r = 1; r <= nlev; r++) {
     size = (size + 1) / 2;
-    const int cnt = kSegSpan >> r;
-    float(*src)[6] = (r & 1) ? A : Bf;
-    float(*dst)[6] = (r & 1) ? Bf : A;
-    if (tid < cnt) {
-      float b[6];
-      for (int q = 0; q < 3; q++) {
-        b[q] = fminf(src[2 * tid][q], src[2 * tid + 1][q]);
-        b[3 + q] = fmaxf(src[2 * tid][3 + q], src[2 * tid + 1][3 + q]);
-      }
-      for (int q = 0; q < 6; q++) dst[tid][q] = b[q];
-      const int64_t k = (int64_t)blockIdx.x * cnt + tid;
+    const int w = 1 << (r - 1);
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      b[q] = fminf(b[q], __shfl_xor(b[q], w));
+      b[3 + q] = fmaxf(b[3 + q], __shfl_xor(b[3 + q], w));
+    }
+    if ((lane & ((w << 1) - 1)) == 0) {
+      const int64_t k = gi >> r;
       if (k < size) {
         float *out = seg + L.off[lev_in - 1 + r];
         for (int q = 0; q < 6; q++) out[6 * k + q] = b[q];
       }
     }
-    __syncthreads();
   }
 }
 
@@ -428,12 +420,12 @@ Bvh build_lbvh(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t n) {
     m = (m + 1) / 2;
   }
   float *seg = ar.get<float>("bvh_seg", 6 * total);
-  // level 0 (sorted leaf boxes) and the levels above, 9 per launch
+  // level 0 (sorted leaf boxes) and the levels above, 6 per launch
   m = n;
   for (int lev = 1; lev < L.nlev;) {
     const int k = L.nlev - lev < kSegLevelsPerLaunch ? L.nlev - lev : kSegLevelsPerLaunch;
     hipLaunchKernelGGL(seg_levels_kernel, dim3((unsigned)((m + kSegSpan - 1) / kSegSpan)),
-                       dim3(256), 0, s, seg, L, lev, k, m, (const float *)b.leaf_box,
+                       dim3(kSegSpan), 0, s, seg, L, lev, k, m, (const float *)b.leaf_box,
                        lev == 1 ? (const uint32_t *)leaf_tri : (const uint32_t *)nullptr);
     SDF_HIP(hipGetLastError());
     for (int q = 0; q < k; q++) m = (m + 1) / 2;

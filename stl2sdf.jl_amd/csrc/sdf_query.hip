@@ -155,6 +155,17 @@ __device__ inline void dist_item(const DistJob &J, int64_t item,
   const uint64_t b = tile_off[tile] + (uint64_t)q * J.clen;
   const uint64_t e = min(b + (uint64_t)J.clen, tile_off[tile + 1]);
   const int n = (int)(e - b);
+  const float hf = (float)g.h;
+  const float H2 = hf * hf * (1.0f + 0x1p-10f);
+  // Phase 1, one triangle per lane: which of the tile's 8 rows of 8 points
+  // (fixed j, k; row r = (j - j0) + 4 (k - k0)) can hold a point within h
+  // of the triangle.  The lower bound is the larger of the squared distance
+  // from the row's box to the triangle's box and the squared distance of
+  // the row segment to the triangle's plane (0 when its ends straddle the
+  // plane); it is below every row point's own bound, and inflated by 2^-10
+  // to cover its rounding against the exact evaluation, so the nearest
+  // triangle of every point within h is kept.
+  uint32_t rowbits = 0;
   if (lane < n) {
     const Tri64 t = tri[tile_tris[b + lane]];
     float v[9];
@@ -168,11 +179,43 @@ __device__ inline void dist_item(const DistJob &J, int64_t item,
     tri_pre_make(v, v + 3, v + 6, rec);
 #pragma unroll
     for (int f = 0; f < kTriPreFloats; f++) s_pre[f][lane] = rec[f];
+    float bmin[3], bmax[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {  // the triangle's box
-      s_pre[kTriPreFloats + a][lane] = fminf(fminf(v[a], v[3 + a]), v[6 + a]);
-      s_pre[kTriPreFloats + 3 + a][lane] = fmaxf(fmaxf(v[a], v[3 + a]), v[6 + a]);
+      bmin[a] = fminf(fminf(v[a], v[3 + a]), v[6 + a]);
+      bmax[a] = fmaxf(fmaxf(v[a], v[3 + a]), v[6 + a]);
+      s_pre[kTriPreFloats + a][lane] = bmin[a];
+      s_pre[kTriPreFloats + 3 + a][lane] = bmax[a];
     }
+    // row coordinates relative to the tile origin, as the lanes form them
+    const float x1 = (float)(lattice_coord(g.ox, g.h, i0 + kTileX - 1) - T0.x);
+    const float ex = fmaxf(fmaxf(bmin[0] - x1, -bmax[0]), 0.f);  // rows span x in [0, x1]
+    const float nx = rec[21], ny = rec[22], nz = rec[23], inn = rec[24];
+    const float s0x = (0.f - rec[0]) * nx, s1x = (x1 - rec[0]) * nx;
+#pragma unroll
+    for (int zz = 0; zz < kTileZ; zz++) {
+      const float pz = (float)(lattice_coord(g.oz, g.h, k0 + zz) - T0.z);
+      const float ez = fmaxf(fmaxf(bmin[2] - pz, pz - bmax[2]), 0.f);
+      const float cz = (pz - rec[2]) * nz;
+#pragma unroll
+      for (int yy = 0; yy < kTileY; yy++) {
+        const float py = (float)(lattice_coord(g.oy, g.h, j0 + yy) - T0.y);
+        const float ey = fmaxf(fmaxf(bmin[1] - py, py - bmax[1]), 0.f);
+        const float box2 = ex * ex + ey * ey + ez * ez;
+        const float c = (py - rec[1]) * ny + cz;
+        const float s0 = s0x + c, s1 = s1x + c;
+        const float sm = ((s0 > 0.f) == (s1 > 0.f)) ? fminf(fabsf(s0), fabsf(s1)) : 0.f;
+        if (fmaxf(box2, sm * sm * inn) <= H2) rowbits |= 1u << (yy + kTileY * zz);
+      }
+    }
+  }
+  // each lane takes the candidate mask of its row
+  const int myrow = lane / kTileX;
+  unsigned long long cand = 0ull;
+#pragma unroll
+  for (int r = 0; r < kTileY * kTileZ; r++) {
+    const unsigned long long m = __ballot((rowbits >> r) & 1u);
+    if (myrow == r) cand = m;
   }
   __syncthreads();
   // this lane's point, relative to the tile origin (exact small multiples of h)
@@ -181,23 +224,6 @@ __device__ inline void dist_item(const DistJob &J, int64_t item,
   const int32_t k = k0 + lane / (kTileX * kTileY);
   const Vec3d p = lattice_point(g, i, j, k);
   const float px = (float)(p.x - T0.x), py = (float)(p.y - T0.y), pz = (float)(p.z - T0.z);
-  // Phase 1: the triangles whose lower bound (the larger of the squared
-  // distances to the triangle's box and plane) is within h, inflated by
-  // 2^-10 to cover the bound's rounding against the exact evaluation.  Only
-  // these can give a distance below h; the nearest one is always among them.
-  const float hf = (float)g.h;
-  const float H2 = hf * hf * (1.0f + 0x1p-10f);
-  unsigned long long cand = 0ull;
-  for (int c = 0; c < n; c++) {
-    const float ex = fmaxf(fmaxf(s_pre[kTriPreFloats][c] - px, px - s_pre[kTriPreFloats + 3][c]), 0.f);
-    const float ey = fmaxf(fmaxf(s_pre[kTriPreFloats + 1][c] - py, py - s_pre[kTriPreFloats + 4][c]), 0.f);
-    const float ez = fmaxf(fmaxf(s_pre[kTriPreFloats + 2][c] - pz, pz - s_pre[kTriPreFloats + 5][c]), 0.f);
-    const float box2 = ex * ex + ey * ey + ez * ez;
-    const float pn = (px - s_pre[0][c]) * s_pre[21][c] + (py - s_pre[1][c]) * s_pre[22][c] +
-                     (pz - s_pre[2][c]) * s_pre[23][c];
-    const float lb2 = fmaxf(box2, pn * pn * s_pre[24][c]);
-    if (lb2 <= H2) cand |= 1ull << c;
-  }
   // Phase 2: exact distances for this lane's candidates only.
   float best = __builtin_huge_valf();
   while (cand) {
@@ -930,37 +956,41 @@ __global__ __launch_bounds__(256) void ray_bits_brute_kernel(
 }
 
 // Replay of adaptive_point_in_mesh_raycast (:234-288) on precomputed ray
-// parities.  final_phase == 0: only rays below min_rays are available;
-// items that need the batch loop are appended to `next` with their state.
+// parities.  Only rays below `avail` are available: an item whose vote
+// needs a later ray (the first min_rays, or the next batch of 4) is
+// appended to `next` (count *nnext, capacity next_cap) with its state.
 __global__ void vote_kernel(RayParams rp, const Pending *__restrict__ items,
                             uint32_t nitems, const uint32_t *__restrict__ nitems_dev,
-                            const unsigned long long *__restrict__ bits,
-                            int final_phase, FusedOut out,
-                            Pending *__restrict__ next, uint32_t *nnext,
-                            uint32_t *__restrict__ wind, uint32_t *nwind,
-                            double *__restrict__ wconf) {
+                            const unsigned long long *__restrict__ bits, int avail,
+                            FusedOut out, Pending *__restrict__ next, uint32_t *nnext,
+                            uint32_t next_cap, uint32_t *__restrict__ wind, uint32_t *nwind,
+                            uint32_t *err) {
   if (nitems_dev && *nitems_dev < nitems) nitems = *nitems_dev;
   for (uint32_t it = blockIdx.x * blockDim.x + threadIdx.x; it < nitems;
        it += gridDim.x * blockDim.x) {
     Pending P = items[it];
     const unsigned long long m = bits[it];
     int inside = P.inside, total = P.used, used = P.used;
-    for (int r = used; r < rp.min_rays; r++) {
-      if ((m >> r) & 1ull) inside++;
-      total++;
+    bool carry = false;
+    if (used < rp.min_rays) {
+      if (rp.min_rays > avail) {
+        carry = true;
+      } else {
+        for (int r = used; r < rp.min_rays; r++) {
+          if ((m >> r) & 1ull) inside++;
+          total++;
+        }
+        used = rp.min_rays;
+      }
     }
-    if (used < rp.min_rays) used = rp.min_rays;
     double ratio = total > 0 ? (double)inside / (double)total : 0.0;
     double conf = ratio > 1.0 - ratio ? ratio : 1.0 - ratio;
-    if (!final_phase && conf < 0.85 && used < rp.max_rays) {
-      P.inside = inside;
-      P.used = used;
-      uint32_t slot = atomicAdd(nnext, 1u);
-      next[slot] = P;
-      continue;
-    }
-    while (conf < 0.85 && used < rp.max_rays) {
+    while (!carry && conf < 0.85 && used < rp.max_rays) {
       int batch = rp.max_rays - used < 4 ? rp.max_rays - used : 4;
+      if (used + batch > avail) {
+        carry = true;
+        break;
+      }
       for (int r = used; r < used + batch; r++) {
         if (r < kNumDirs) {
           if ((m >> r) & 1ull) inside++;
@@ -971,6 +1001,16 @@ __global__ void vote_kernel(RayParams rp, const Pending *__restrict__ items,
       ratio = (double)inside / (double)total;
       conf = ratio > 1.0 - ratio ? ratio : 1.0 - ratio;
       if (conf > 0.95) break;
+    }
+    if (carry) {
+      P.inside = inside;
+      P.used = used;
+      const uint32_t slot = atomicAdd(nnext, 1u);
+      if (slot < next_cap)
+        next[slot] = P;
+      else
+        atomicOr(err, kErrPendingOverflow);
+      continue;
     }
     double sgn = ratio > 0.5 ? 1.0 : -1.0;
     if (rp.use_winding && conf < rp.conf_thr) {
@@ -1220,9 +1260,10 @@ static int blocks_for(int64_t n, int t) {
 void resolve_pending(hipStream_t s, Arena &ar, const GridP &g, const double *pts,
                      const Tri64 *tri, int64_t nel, const Bvh &bvh, const RayParams &rp,
                      const Pending *items, const uint32_t *n_dev, uint32_t cap, int r0,
-                     int r1, int final_phase, FusedOut out, Pending *next,
-                     uint32_t *d_counts, uint32_t wind_cap, uint32_t *err, int stack) {
-  // d_counts: [0] fallback rays, [1] next items, [2] winding items
+                     int r1, int final_phase, FusedOut out, Pending *next, uint32_t *nnext,
+                     uint32_t next_cap, uint32_t *d_counts, uint32_t wind_cap, uint32_t *err,
+                     int stack) {
+  // d_counts: [0] fallback rays, [1] next items (unless nnext), [2] winding items
   if (cap == 0) return;
   if (r1 > kNumDirs) r1 = kNumDirs;
   launch_fill_u32(s, d_counts, 4, 0u);
@@ -1246,10 +1287,12 @@ void resolve_pending(hipStream_t s, Arena &ar, const GridP &g, const double *pts
   }
   uint32_t *wind = ar.get<uint32_t>(ar.tmp("res_wind"), cap);
   hipLaunchKernelGGL(vote_kernel, dim3(256), dim3(256), 0, s, rp, items, cap, n_dev, bits,
-                     final_phase, out, next, d_counts + 1, wind, d_counts + 2,
-                     (double *)nullptr);
+                     final_phase ? kNumDirs : r1, out, next, nnext ? nnext : d_counts + 1,
+                     next_cap, wind, d_counts + 2, err);
   SDF_HIP(hipGetLastError());
-  if (rp.use_winding && wind_cap > 0) {
+  // a non-final phase decides only items with confidence >= 0.85, which
+  // never need the winding fallback
+  if (final_phase && rp.use_winding && wind_cap > 0) {
     int64_t chunk = 4096;
     int chunks = (int)((nel + chunk - 1) / chunk);
     if (chunks < 1) chunks = 1;
