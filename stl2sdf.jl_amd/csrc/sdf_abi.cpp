@@ -120,9 +120,16 @@ struct sdf_ctx {
   uint64_t *d_status = nullptr;  // sizes of the last lattice call (16 words)
   bool own_status = true;        // d_status allocated by this context
   bool capturing = false;        // inside stream capture: no host waits allowed
-  // tunables: triangles per distance work item, LDS ray-stack depth
+  // tunables: triangles per distance work item, LDS ray-stack depth,
+  // workgroups per CU of the distance and sign kernels (0: one per item).
+  // The distance kernel's LDS allows 16 workgroups per CU; at 10 the LBVH
+  // build beside it gets room and slows the next kernels less (measured:
+  // about 3 % per step), while the scalar-bound sign kernel needs all its
+  // waves.
   int chunk_len = 64;
   int ray_stack = 1024;
+  int dist_per_cu = 10, fused_per_cu = 0;
+  int ncu = 256;
   // timing
   bool timing = false;
   std::vector<hipEvent_t> ev;
@@ -417,7 +424,8 @@ void stage_dist(sdf_ctx *c, LatticeRun &L) {
     sdf::launch_learned_fallback(c->stream, g, c->tri, c->bvh, bins, list, c->lstats, slack,
                                  L.d2buf, c->d_words);
   } else {
-    sdf::launch_dist_chunks(c->stream, g, c->tri, bins, L.d2buf);
+    sdf::launch_dist_chunks(c->stream, g, c->tri, bins, L.d2buf, nullptr,
+                            (int64_t)c->dist_per_cu * c->ncu);
   }
 }
 
@@ -427,7 +435,8 @@ void stage_sign(sdf_ctx *c, LatticeRun &L) {
   if (L.do_sign) SDF_HIP(hipStreamWaitEvent(c->stream, c->join[1], 0));
   c->mark("fused");
   sdf::launch_fused_lattice(c->stream, L.g, c->tri, L.bins, L.d2buf, L.do_sign, L.out, L.pend,
-                            c->d_words + 1, L.cap, L.pend0, c->d_words + 2, L.cap0, c->d_words);
+                            c->d_words + 1, L.cap, L.pend0, c->d_words + 2, L.cap0, c->d_words,
+                            (int64_t)c->fused_per_cu * c->ncu);
 }
 
 // Stage 4: adaptive vote for the pending points (waits for the LBVH), then
@@ -993,6 +1002,7 @@ sdf_status sdf_ctx_create(int32_t device, sdf_ctx **out) {
         SDF_HIP(hipEventCreateWithFlags(&c->join[q], hipEventDisableTiming));
       SDF_HIP(hipMalloc(&c->d_words, 256));
       SDF_HIP(hipMalloc(&c->d_status, 16 * sizeof(uint64_t)));
+      SDF_HIP(hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device));
     } catch (...) {
       delete c;
       throw;

@@ -712,7 +712,11 @@ void tile_bins_finish(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
     SDF_HIP(hipGetLastError());
     int bits = 1;
     while (bits < 32 && ((int64_t)1 << bits) <= td.ntiles) bits++;
-    radix_sort_u32_pairs(s, ar, keys, tile_tris, cap, bits, n_dev);
+    // own scratch names: the sorted lists may stay in the scratch pair
+    const std::string scope = ar.scope;
+    ar.scope = scope + "tile/";
+    radix_sort_u32_pairs(s, ar, keys, tile_tris, cap, bits, n_dev, &keys, &tile_tris);
+    ar.scope = scope;
     hipLaunchKernelGGL(tile_offsets_kernel, dim3(grid_for(td.ntiles + 1, 256)), dim3(256),
                        0, s, (const uint32_t *)keys, cap, n_dev, td.ntiles, tile_off);
     SDF_HIP(hipGetLastError());

@@ -231,8 +231,10 @@ void build_dist_chunks(hipStream_t s, Arena &ar, const GridP &g, Bins &b, int64_
 // query side
 // Distance work items -> squared distances per point of non-empty tiles
 // (d2buf: 64 floats per work item slot, indexed by the tile's first item).
+// max_blocks > 0 caps the grid (grid-stride over the items).
 void launch_dist_chunks(hipStream_t s, const GridP &g, const Tri64 *tri,
-                        const Bins &b, float *d2buf, const uint32_t *only = nullptr);
+                        const Bins &b, float *d2buf, const uint32_t *only = nullptr,
+                        int64_t max_blocks = 0);
 // Distance work items with a learned per-point squared radius r2buf (slots
 // as d2buf); nodes whose radius held no triangle are appended to `list`.
 // stats: [0] fallback nodes, [8..263] exact evaluations per wave (its
@@ -256,7 +258,7 @@ void launch_fused_lattice(hipStream_t s, const GridP &g, const Tri64 *tri,
                           const Bins &b, const float *d2buf, bool do_sign,
                           FusedOut out, Pending *pend, uint32_t *npend,
                           uint32_t pend_cap, Pending *pend0, uint32_t *npend0,
-                          uint32_t pend0_cap, uint32_t *err);
+                          uint32_t pend0_cap, uint32_t *err, int64_t max_blocks = 0);
 
 struct RayParams {
   double dirs[kNumDirs * 3];

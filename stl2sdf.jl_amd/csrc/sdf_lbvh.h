@@ -23,10 +23,13 @@ struct Bvh {
 
 Bvh build_lbvh(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t n);
 // Stable LSD sort of (key, value) pairs by the low key_bits bits; n is a
-// capacity when n_dev (the actual count, on the device) is given.
+// capacity when n_dev (the actual count, on the device) is given.  With
+// keys_out, the sorted pairs are left in whichever buffers the last pass
+// wrote (the inputs or the arena's scratch pair), returned there.
 void radix_sort_u32_pairs(hipStream_t s, Arena &ar, uint32_t *keys,
                           uint32_t *vals, int64_t n, int key_bits,
-                          const uint64_t *n_dev = nullptr);
+                          const uint64_t *n_dev = nullptr, uint32_t **keys_out = nullptr,
+                          uint32_t **vals_out = nullptr);
 
 constexpr int kStackDepth = 64;
 

@@ -339,8 +339,15 @@ __global__ void node_boxes_kernel(const float *__restrict__ seg, SegLevels L,
   }
 }
 
+// Sorted pairs end in (keys, vals), or, when keys_out is given, in the pair
+// of buffers the last pass wrote (*keys_out, *vals_out; no copy back).
 void radix_sort_pairs(hipStream_t s, Arena &ar, uint32_t *keys, uint32_t *vals,
-                      int64_t n, int key_bits, const uint64_t *n_dev) {
+                      int64_t n, int key_bits, const uint64_t *n_dev,
+                      uint32_t **keys_out = nullptr, uint32_t **vals_out = nullptr) {
+  if (keys_out) {
+    *keys_out = keys;
+    *vals_out = vals;
+  }
   if (n <= 1) return;
   int64_t nblocks = (n + kSortTile - 1) / kSortTile;
   uint32_t *k2 = ar.get<uint32_t>(ar.tmp("sort_k2"), n);
@@ -365,7 +372,10 @@ void radix_sort_pairs(hipStream_t s, Arena &ar, uint32_t *keys, uint32_t *vals,
     va = vb;
     vb = t;
   }
-  if (passes & 1) {  // odd pass count: the result sits in the scratch pair
+  if (keys_out) {
+    *keys_out = ka;
+    *vals_out = va;
+  } else if (passes & 1) {  // odd pass count: the result sits in the scratch pair
     SDF_HIP(hipMemcpyAsync(keys, ka, (size_t)n * sizeof(uint32_t),
                            hipMemcpyDeviceToDevice, s));
     SDF_HIP(hipMemcpyAsync(vals, va, (size_t)n * sizeof(uint32_t),
@@ -376,8 +386,9 @@ void radix_sort_pairs(hipStream_t s, Arena &ar, uint32_t *keys, uint32_t *vals,
 }  // namespace
 
 void radix_sort_u32_pairs(hipStream_t s, Arena &ar, uint32_t *keys,
-                          uint32_t *vals, int64_t n, int key_bits, const uint64_t *n_dev) {
-  radix_sort_pairs(s, ar, keys, vals, n, key_bits, n_dev);
+                          uint32_t *vals, int64_t n, int key_bits, const uint64_t *n_dev,
+                          uint32_t **keys_out, uint32_t **vals_out) {
+  radix_sort_pairs(s, ar, keys, vals, n, key_bits, n_dev, keys_out, vals_out);
 }
 
 Bvh build_lbvh(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t n) {

@@ -240,9 +240,13 @@ __device__ inline void dist_item(const DistJob &J, int64_t item,
   __syncthreads();  // s_pre is reused by the block's next item
 }
 
+// One work item per workgroup and round; a grid smaller than the item
+// count leaves wave slots and LDS free for kernels on other streams.
 __global__ __launch_bounds__(kTilePts) void dist_chunks_kernel(DistJob J) {
   __shared__ float s_pre[kTriPreFloats + 6][kTilePts];
-  dist_item(J, xcd_remap(blockIdx.x, gridDim.x), s_pre);
+  const int64_t n = chunk_count(J.nchunks, J.nchunks_dev);
+  for (int64_t it = xcd_remap(blockIdx.x, gridDim.x); it < n; it += gridDim.x)
+    dist_item(J, it, s_pre);
 }
 
 // Persistent work queue over the distance items of many jobs: each block
@@ -693,7 +697,8 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S)
 
 __global__ __launch_bounds__(64) void fused_lattice_kernel(FusedJob J) {
   __shared__ FusedShared S;
-  fused_brick(J, xcd_remap(blockIdx.x, gridDim.x), S);
+  for (int64_t m = xcd_remap(blockIdx.x, gridDim.x); m < J.nbricks; m += gridDim.x)
+    fused_brick(J, m, S);
 }
 
 // Persistent work queue over the sign bricks of many jobs (static brick
@@ -1148,9 +1153,10 @@ DistJob make_dist_job(const GridP &g, const Tri64 *tri, const Bins &b, float *d2
 }
 
 void launch_dist_chunks(hipStream_t s, const GridP &g, const Tri64 *tri,
-                        const Bins &b, float *d2buf, const uint32_t *only) {
+                        const Bins &b, float *d2buf, const uint32_t *only, int64_t max_blocks) {
   if (b.nchunks <= 0) return;
-  hipLaunchKernelGGL(dist_chunks_kernel, dim3((unsigned)b.nchunks), dim3(kTilePts), 0, s,
+  const int64_t grid = max_blocks > 0 && max_blocks < b.nchunks ? max_blocks : b.nchunks;
+  hipLaunchKernelGGL(dist_chunks_kernel, dim3((unsigned)grid), dim3(kTilePts), 0, s,
                      make_dist_job(g, tri, b, d2buf, only));
   SDF_HIP(hipGetLastError());
 }
@@ -1229,12 +1235,13 @@ void launch_fused_lattice(hipStream_t s, const GridP &g, const Tri64 *tri,
                           const Bins &b, const float *d2buf, bool do_sign,
                           FusedOut out, Pending *pend, uint32_t *npend,
                           uint32_t pend_cap, Pending *pend0, uint32_t *npend0,
-                          uint32_t pend0_cap, uint32_t *err) {
+                          uint32_t pend0_cap, uint32_t *err, int64_t max_blocks) {
   (void)tri;
   const FusedJob J = make_fused_job(g, b, d2buf, do_sign, out, pend, npend, pend_cap, pend0,
                                     npend0, pend0_cap, err);
   if (J.nbricks <= 0) return;
-  hipLaunchKernelGGL(fused_lattice_kernel, dim3((unsigned)J.nbricks), dim3(64), 0, s, J);
+  const int64_t grid = max_blocks > 0 && max_blocks < J.nbricks ? max_blocks : J.nbricks;
+  hipLaunchKernelGGL(fused_lattice_kernel, dim3((unsigned)grid), dim3(64), 0, s, J);
   SDF_HIP(hipGetLastError());
 }
 
