@@ -121,6 +121,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--fit-steps", type=int, default=1500)
+    ap.add_argument("--tune", action="append", default=[], metavar="NAME=VALUE",
+                    help="launch tunable of the library (sdf_ctx_tune); results do not change")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
                     help="only copy an MB-sized device buffer 3 times (counter calibration)")
     args = ap.parse_args()
@@ -152,6 +154,9 @@ def main():
     L = _lib.lib()
     ctx = _lib.ctx(0)
     _lib.check(L.sdf_ctx_set_stream(ctx, C.c_void_p(stream.cuda_stream)))
+    for t in args.tune:
+        name, value = t.split("=")
+        _lib.check(L.sdf_ctx_tune(ctx, name.encode(), int(value)))
     dX = torch.from_numpy(np.ascontiguousarray(mesh.X.T)).to(dev)      # 3 x nnp column-major
     dI = torch.from_numpy(np.ascontiguousarray(mesh.IEN.T)).to(dev)
     out = torch.empty(grid.ngp, dtype=torch.float64, device=dev)
@@ -257,7 +262,7 @@ def main():
         "data": "synthetic (seeded star surface, float32 vertices)",
         "config": {"workload": desc, "nel": mesh.nel, "nnp": mesh.nnp,
                    "grid_N": [int(v) for v in grid.N], "ngp": int(grid.ngp),
-                   "cell_size": h},
+                   "cell_size": h, "tune": args.tune},
         "stages_ms": stage_ms,
         "counters": {"pending_points": cnt[0], "tile_pairs": cnt[1], "line_hits": cnt[2],
                      "device_bytes": cnt[3]},

@@ -62,6 +62,12 @@ sdf_status sdf_ctx_destroy(sdf_ctx *ctx);
 /* Run on a caller-owned hipStream_t (NULL: back to the context's own). */
 sdf_status sdf_ctx_set_stream(sdf_ctx *ctx, void *hip_stream);
 sdf_status sdf_ctx_synchronize(sdf_ctx *ctx);
+/* Launch tunables of the lattice path (no counterpart in the reference;
+ * results do not depend on them): "chunk_len" triangles per distance work
+ * item (1..64, default 64), "ray_stack" LDS ray-stack depth of the vote
+ * stage (1024 or 2048), "dist_per_cu" / "fused_per_cu" workgroups per CU
+ * of the distance / sign kernels (0: one per work item; defaults 8 / 0). */
+sdf_status sdf_ctx_tune(sdf_ctx *ctx, const char *name, int64_t value);
 /* Stage timers (hipEvents) of the last call, when enabled: names are
  * written as one ';'-separated string, milliseconds into ms[0..n). */
 sdf_status sdf_ctx_enable_timing(sdf_ctx *ctx, int32_t on);

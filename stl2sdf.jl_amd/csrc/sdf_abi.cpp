@@ -122,13 +122,13 @@ struct sdf_ctx {
   bool capturing = false;        // inside stream capture: no host waits allowed
   // tunables: triangles per distance work item, LDS ray-stack depth,
   // workgroups per CU of the distance and sign kernels (0: one per item).
-  // The distance kernel's LDS allows 16 workgroups per CU; at 10 the LBVH
-  // build beside it gets room and slows the next kernels less (measured:
-  // about 3 % per step), while the scalar-bound sign kernel needs all its
-  // waves.
+  // The distance kernel's LDS allows 16 workgroups per CU; at 8 the line
+  // hits and the LBVH build beside it get room and finish before the sign
+  // kernel needs them (measured best of 6..14 on the 1M-triangle star),
+  // while the scalar-bound sign kernel needs all its waves.
   int chunk_len = 64;
   int ray_stack = 1024;
-  int dist_per_cu = 10, fused_per_cu = 0;
+  int dist_per_cu = 8, fused_per_cu = 0;
   int ncu = 256;
   // timing
   bool timing = false;
@@ -338,13 +338,12 @@ enum : int {
 };
 
 // Stage 1: the build chains.  Tile bins and distance work items run on the
-// main stream; the axis-line hits on helper stream 0 (needed by the sign
-// stage) beside them, and the LBVH on helper stream 1 (needed only by the
-// vote stage) from the end of the tile chain, beside the distance and sign
-// kernels.  Issue order: tile counting, the line chain, the rest of the
-// tile chain (stage_bins_tiles), the distance and sign kernels (stage_dist,
-// stage_sign), then the LBVH (stage_bins_bvh), so the host's ~100 launches
-// never hold back a kernel on the critical path.
+// main stream; the axis-line hits (needed by the sign stage) on helper
+// stream 0 and the LBVH (needed only by the vote stage) on helper stream 1,
+// both from the end of the tile chain, beside the distance and sign
+// kernels.  Issue order: the tile chain, the line chain (stage_bins_tiles),
+// the distance and sign kernels (stage_dist, stage_sign), then the LBVH
+// (stage_bins_bvh); the host stays ahead of the GPU throughout.
 void stage_bins_tiles(sdf_ctx *c, LatticeRun &L) {
   const sdf::GridP &g = L.g;
   const int64_t npts = grid_points(g);
@@ -357,10 +356,22 @@ void stage_bins_tiles(sdf_ctx *c, LatticeRun &L) {
   L.bvh_was_ready = c->bvh_ready;
   c->mark("bin_tiles");
   L.d2buf = nullptr;
-  sdf::TileBuild tb;
-  if (L.do_dist) sdf::tile_bins_begin(c->stream, c->ar, c->tri, c->nel, g, g.h, tb);
+  if (L.do_dist) {
+    sdf::TileBuild tb;
+    sdf::tile_bins_begin(c->stream, c->ar, c->tri, c->nel, g, g.h, tb);
+    sdf::tile_bins_finish(c->stream, c->ar, c->tri, c->nel, g, tb, bins, c->caps.pairs);
+    sdf::build_dist_chunks(c->stream, c->ar, g, bins, c->caps.chunks, c->chunk_len);
+    L.d2buf = c->ar.get<float>("d2buf", (size_t)bins.nchunks * sdf::kTilePts + 1);
+    sdf::launch_fill_u32(c->stream, (uint32_t *)L.d2buf,
+                         (int64_t)bins.nchunks * sdf::kTilePts, 0x7F800000u);
+  }
+  // the helper chains start when the tile chain is done: beside the
+  // distance kernel (whose grid leaves room on every CU) they cost little,
+  // beside the latency-bound tile chain they stretch every small kernel
+  // (measured: about 35 us per step)
+  SDF_HIP(hipEventRecord(c->join[3], c->stream));
   if (L.do_sign) {
-    SDF_HIP(hipStreamWaitEvent(c->aux[0], c->join[0], 0));
+    SDF_HIP(hipStreamWaitEvent(c->aux[0], c->join[3], 0));
     sdf::LineBuild lb;
     c->ar.scope = "line/";
     sdf::line_hits_begin(c->aux[0], c->ar, c->tri, c->nel, g, lb, bins);
@@ -368,17 +379,6 @@ void stage_bins_tiles(sdf_ctx *c, LatticeRun &L) {
     c->ar.scope.clear();
     SDF_HIP(hipEventRecord(c->join[1], c->aux[0]));
   }
-  if (L.do_dist) {
-    sdf::tile_bins_finish(c->stream, c->ar, c->tri, c->nel, g, tb, bins, c->caps.pairs);
-    sdf::build_dist_chunks(c->stream, c->ar, g, bins, c->caps.chunks, c->chunk_len);
-    L.d2buf = c->ar.get<float>("d2buf", (size_t)bins.nchunks * sdf::kTilePts + 1);
-    sdf::launch_fill_u32(c->stream, (uint32_t *)L.d2buf,
-                         (int64_t)bins.nchunks * sdf::kTilePts, 0x7F800000u);
-  }
-  // the LBVH starts when the tile chain is done: beside the compute-bound
-  // distance and sign kernels it costs little, beside the latency-bound
-  // tile chain it stretches every small kernel
-  SDF_HIP(hipEventRecord(c->join[3], c->stream));
   L.cap = c->caps.pend ? c->caps.pend
                        : (uint32_t)(npts < (1 << 22) ? (npts > 0 ? npts : 1) : (1 << 22));
   L.cap0 = c->caps.pend0 ? c->caps.pend0
@@ -1045,6 +1045,26 @@ sdf_status sdf_ctx_synchronize(sdf_ctx *c) {
     if (!c) fail(SDF_ERR_ARG, "ctx is NULL");
     use_device(c);
     SDF_HIP(hipStreamSynchronize(c->stream));
+  });
+}
+
+sdf_status sdf_ctx_tune(sdf_ctx *c, const char *name, int64_t value) {
+  return guarded([&] {
+    if (!c || !name) fail(SDF_ERR_ARG, "NULL argument");
+    std::lock_guard<std::mutex> lk(c->mu);
+    const std::string n = name;
+    if (n == "chunk_len") {
+      if (value < 1 || value > sdf::kTilePts) fail(SDF_ERR_ARG, "chunk_len must be in 1..64");
+      c->chunk_len = (int)value;
+    } else if (n == "ray_stack") {
+      if (value != 1024 && value != 2048) fail(SDF_ERR_ARG, "ray_stack must be 1024 or 2048");
+      c->ray_stack = (int)value;
+    } else if (n == "dist_per_cu" || n == "fused_per_cu") {
+      if (value < 0 || value > 64) fail(SDF_ERR_ARG, n + " must be in 0..64");
+      (n == "dist_per_cu" ? c->dist_per_cu : c->fused_per_cu) = (int)value;
+    } else {
+      fail(SDF_ERR_ARG, "unknown tunable " + n);
+    }
   });
 }
 

@@ -29,6 +29,7 @@ SIGNATURES = {
     "sdf_ctx_create": (C.c_int32, [C.c_int32, C.POINTER(_vp)]),
     "sdf_ctx_destroy": (C.c_int32, [_vp]),
     "sdf_ctx_set_stream": (C.c_int32, [_vp, _vp]),
+    "sdf_ctx_tune": (C.c_int32, [_vp, C.c_char_p, C.c_int64]),
     "sdf_ctx_synchronize": (C.c_int32, [_vp]),
     "sdf_ctx_enable_timing": (C.c_int32, [_vp, C.c_int32]),
     "sdf_ctx_stage_times": (C.c_int32, [_vp, C.c_char_p, C.c_int32, _dp, C.c_int32, _i32p]),
