@@ -8,8 +8,9 @@ the whole device-resident signed-distance path: triangle gather from X/IEN,
 binning, fused distance + axis-ray kernel, adaptive vote for undecided
 points, written as a float64 SDF array in HBM.
 
-The JSON line carries the stage split, the roofline of the dominant (fused)
-kernel from HIP events on its stream, the learned-radius path on the same
+Timed steps run without instrumentation; the JSON line also carries the
+stage split and the roofline of the dominant (fused) kernel, both from HIP
+events on the kernels' stream over 5 further steps with the stage timers on, the learned-radius path on the same
 workload (network fitted once on a 128-point coarse lattice, not timed; each
 timed step as above with the radius-limited distance loop; its output is
 checked bit-for-bit against the plain path's), and the CPU oracle timed on a
@@ -42,6 +43,21 @@ def measured_traffic(workload):
         if line.startswith("fused_lattice_kernel traffic_bytes_per_launch:"):
             return float(line.split(":")[1])
     return None
+
+
+def staged(L, ctx, step, torch, n=5):
+    """Per-stage times (HIP events on the stream the kernels run on) over n
+    extra steps with the library's stage timers on; lists of ms per stage."""
+    from stl2sdf import _lib
+    _lib.check(L.sdf_ctx_enable_timing(ctx, 1))
+    stages = {}
+    for _ in range(n):
+        step()
+        for k, v in _lib.stage_times(ctx).items():
+            stages.setdefault(k, []).append(v)
+    torch.cuda.synchronize()
+    _lib.check(L.sdf_ctx_enable_timing(ctx, 0))
+    return stages
 
 
 def log(msg):
@@ -174,20 +190,17 @@ def main():
         step()
         torch.cuda.synchronize()
         log(f"[bench] warmup {w + 1}/{args.warmup} done")
-    _lib.check(L.sdf_ctx_enable_timing(ctx, 1))
-    stages = {}
+    # timed steps without instrumentation, then a few steps with the stage
+    # events on (their time is not counted)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for s in range(args.steps):
         step()
-        st = _lib.stage_times(ctx)
-        for k, v in st.items():
-            stages.setdefault(k, []).append(v)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    _lib.check(L.sdf_ctx_enable_timing(ctx, 0))
-    cnt = _lib.counters(ctx)
     log(f"[bench] timed {args.steps} steps in {dt:.3f} s")
+    stages = staged(L, ctx, step, torch)
+    cnt = _lib.counters(ctx)
 
     learned = None
     if not args.no_learned:
@@ -211,17 +224,13 @@ def main():
         for w in range(args.warmup):
             lstep()
             torch.cuda.synchronize()
-        _lib.check(L.sdf_ctx_enable_timing(ctx, 1))
-        lstages = {}
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(args.steps):
             lstep()
-            for k, v in _lib.stage_times(ctx).items():
-                lstages.setdefault(k, []).append(v)
         torch.cuda.synchronize()
         ldt = time.perf_counter() - t0
-        _lib.check(L.sdf_ctx_enable_timing(ctx, 0))
+        lstages = staged(L, ctx, lstep, torch)
         lst = np.zeros(8)
         nst = C.c_int32()
         _lib.check(L.sdf_learned_stats(ctx, _lib.dptr(lst), 8, C.byref(nst)))

@@ -136,8 +136,10 @@ sdf_status sdf_signed_distance(sdf_ctx *ctx, const double *X, int64_t nnp,
 
 /* Device-resident form of the same steps: the mesh is given as device
  * pointers (same layouts) and is kept by the context until the next
- * sdf_mesh_set_device call; sdf_signed_distance_device writes ngp doubles
- * to a device buffer, asynchronously on the context stream. */
+ * sdf_mesh_set_device call, which does not wait for the device (an IEN
+ * index outside 1..nnp is reported by every later call on that mesh);
+ * sdf_signed_distance_device writes ngp doubles to a device buffer on the
+ * context stream and returns after one status read. */
 sdf_status sdf_mesh_set_device(sdf_ctx *ctx, const double *d_X, int64_t nnp,
                                const int64_t *d_IEN, int64_t nel);
 sdf_status sdf_signed_distance_device(sdf_ctx *ctx, const double aabb_min[3],

@@ -188,18 +188,18 @@ int64_t grid_points(const sdf::GridP &g) {
   return (int64_t)g.nx * g.ny * g.nz;
 }
 
-uint32_t read_word(sdf_ctx *c, int i) {
-  uint32_t v = 0;
-  SDF_HIP(hipMemcpyAsync(&v, c->d_words + i, sizeof(v), hipMemcpyDeviceToHost,
-                         c->stream));
-  SDF_HIP(hipStreamSynchronize(c->stream));
-  return v;
-}
+// d_words[kMeshErrWord] holds the mesh upload's error flags (an IEN index
+// outside 1..nnp); only a new mesh clears it, so every call on that mesh
+// reports it, and uploading a device mesh needs no host wait.
+constexpr int kMeshErrWord = 31;
 
-void reset_words(sdf_ctx *c) { sdf::launch_fill_u32(c->stream, c->d_words, 32, 0u); }
+void reset_words(sdf_ctx *c) { sdf::launch_fill_u32(c->stream, c->d_words, kMeshErrWord, 0u); }
 
 void check_err_flags(sdf_ctx *c) {
-  uint32_t e = read_word(c, 0);
+  uint32_t w[kMeshErrWord + 1];
+  SDF_HIP(hipMemcpyAsync(w, c->d_words, sizeof(w), hipMemcpyDeviceToHost, c->stream));
+  SDF_HIP(hipStreamSynchronize(c->stream));
+  const uint32_t e = w[0] | w[kMeshErrWord];
   if (e & sdf::kErrBadIndex)
     fail(SDF_ERR_ARG, "IEN holds a vertex index outside 1..nnp");
   if (e & sdf::kErrHitOverflow)
@@ -221,7 +221,8 @@ void set_mesh_device(sdf_ctx *c, const double *dX, int64_t nnp,
   c->nel = nel;
   c->nnp = nnp;
   c->mark("gather");
-  sdf::launch_gather_tris(c->stream, dX, nnp, dIEN, nel, c->tri, c->d_words);
+  sdf::launch_fill_u32(c->stream, c->d_words + kMeshErrWord, 1, 0u);
+  sdf::launch_gather_tris(c->stream, dX, nnp, dIEN, nel, c->tri, c->d_words + kMeshErrWord);
   c->mesh_ready = true;
   c->bvh_ready = false;
 }
@@ -334,7 +335,7 @@ struct LatticeRun {
 
 enum : int {
   kStErr = 0, kStPend, kStPend0, kStPairs, kStHits, kStChunks, kStWindA1, kStWindA2,
-  kStWindB1, kStWindB2, kStCount
+  kStWindB1, kStWindB2, kStMeshErr, kStCount
 };
 
 // Stage 1: the build chains.  Tile bins and distance work items run on the
@@ -464,6 +465,7 @@ void stage_resolve(sdf_ctx *c, LatticeRun &L) {
   sdf::StatusSrc src;
   src.n = kStCount;
   src.set(kStErr, c->d_words, false);
+  src.set(kStMeshErr, c->d_words + kMeshErrWord, false);
   src.set(kStPend, c->d_words + 1, false);
   src.set(kStPend0, c->d_words + 2, false);
   src.set(kStPairs, L.bins.npairs_dev, true);
@@ -513,7 +515,8 @@ bool lattice_settle(sdf_ctx *c, LatticeRun &L, const uint64_t *st) {
     c->caps.wind = (uint32_t)std::min<uint64_t>(w + w / 8 + 64, 0xFFFFFFF0u);
     again = true;
   }
-  const uint32_t e = (uint32_t)st[kStErr] & ~(uint32_t)sdf::kErrPendingOverflow;
+  const uint32_t e = ((uint32_t)st[kStErr] | (uint32_t)st[kStMeshErr]) &
+                     ~(uint32_t)sdf::kErrPendingOverflow;
   if (e & sdf::kErrBadIndex) fail(SDF_ERR_ARG, "IEN holds a vertex index outside 1..nnp");
   if (again) return true;
   if (e & sdf::kErrHitOverflow)
@@ -1492,7 +1495,6 @@ sdf_status sdf_mesh_set_device(sdf_ctx *c, const double *d_X, int64_t nnp,
     std::lock_guard<std::mutex> lk(c->mu);
     begin_call(c);
     set_mesh_device(c, d_X, nnp, d_IEN, nel);
-    check_err_flags(c);
   });
 }
 

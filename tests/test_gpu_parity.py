@@ -186,3 +186,50 @@ def test_list_capacity_growth_on_one_context():
             assert np.max(np.abs(np.abs(sdf) - d_ref)) <= DIST_TOL * _cases.diag(mesh)
     finally:
         _lib.check(L.sdf_ctx_destroy(h))
+
+
+def test_bad_index_on_device_mesh_reported_by_next_call():
+    """sdf_mesh_set_device does not wait for the device; an IEN index outside
+    1..nnp is reported by every call on that mesh, and a valid mesh clears it.
+    Device buffers come from the HIP runtime the library itself uses."""
+    import ctypes as C
+    mesh, grid = _cases.icosphere_case(subdiv=1, n_max=8)
+    L = _lib.lib()
+    c = _lib.ctx(0)
+    hip = C.CDLL("libamdhip64.so.7")   # the runtime already loaded for the library
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipFree.argtypes = [C.c_void_p]
+    bufs = []
+
+    def dev(a):
+        a = np.ascontiguousarray(a)
+        p = C.c_void_p()
+        assert hip.hipMalloc(C.byref(p), a.nbytes) == 0
+        bufs.append(p)
+        assert hip.hipMemcpy(p, a.ctypes.data, a.nbytes, 1) == 0   # host to device
+        return p
+
+    try:
+        X = dev(np.asfortranarray(mesh.X, np.float64).T.copy())    # 3 x nnp column-major
+        bad = mesh.IEN.astype(np.int64).copy()
+        bad[1, 3] = mesh.nnp + 1
+        Ib = dev(np.asfortranarray(bad).T.copy())
+        Ig = dev(np.asfortranarray(mesh.IEN, np.int64).T.copy())
+        out = dev(np.zeros(int(grid.ngp)))
+        amin, N = _lib.f64_3(grid.AABB_min), _lib.i64_3(grid.N)
+        call = lambda: L.sdf_signed_distance_device(c, _lib.dptr(amin), float(grid.cell_size),
+                                                    _lib.i64ptr(N), out)
+        _lib.check(L.sdf_mesh_set_device(c, X, mesh.nnp, Ib, mesh.nel))
+        for _ in range(2):
+            with pytest.raises(_lib.SdfError, match="IEN"):
+                _lib.check(call())
+        _lib.check(L.sdf_mesh_set_device(c, X, mesh.nnp, Ig, mesh.nel))
+        _lib.check(call())
+        got = np.empty(int(grid.ngp))
+        assert hip.hipMemcpy(got.ctypes.data, out, got.nbytes, 2) == 0   # device to host
+        ref = SignedDistances.signed_distance(mesh, grid)
+        assert np.array_equal(got, ref)
+    finally:
+        for p in bufs:
+            hip.hipFree(p)
