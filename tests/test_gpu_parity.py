@@ -233,3 +233,21 @@ def test_bad_index_on_device_mesh_reported_by_next_call():
     finally:
         for p in bufs:
             hip.hipFree(p)
+
+
+def test_grid_covering_part_of_the_mesh():
+    """A grid over part of the mesh: triangles outside the grid still count
+    for the rays (the LBVH holds the whole mesh), so signs and distances
+    equal the oracle's, as for the full grid."""
+    mesh, full = _cases.star_case(n=40, n_max=24, seed=2)
+    xmin, xmax = MeshGrid.getMesh_AABB(mesh.X)
+    mid = 0.5 * (np.asarray(xmin) + np.asarray(xmax))
+    part = MeshGrid.Grid(np.asarray(xmin), mid, 20, 2)      # lower octant and a margin
+    for grid in (full, part):
+        P = MeshGrid.generateGridPoints(grid)
+        print(f"case: nel={mesh.nel} ngp={grid.ngp} N={list(grid.N)}", flush=True)
+        sdf = SignedDistances.signed_distance(mesh, grid)
+        s_ref, _ = oracle.signs(mesh.X, mesh.IEN, P)
+        d_ref = oracle.dist_exact(mesh.X, mesh.IEN, P, grid.cell_size)
+        assert np.array_equal(np.signbit(sdf), s_ref < 0)
+        assert np.max(np.abs(np.abs(sdf) - d_ref)) <= DIST_TOL * _cases.diag(mesh)
