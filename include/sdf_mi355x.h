@@ -74,8 +74,7 @@ sdf_status sdf_ctx_enable_timing(sdf_ctx *ctx, int32_t on);
 sdf_status sdf_ctx_stage_times(sdf_ctx *ctx, char *names, int32_t names_cap,
                                double *ms, int32_t cap, int32_t *n);
 /* Counters of the last call: [0] points handed to the adaptive vote (left
- * undecided by the six axis rays, or decided inside a rounding margin; a
- * point of the second kind still open after six rays counts twice),
+ * undecided by the six axis rays, or decided inside a rounding margin),
  * [1] triangles listed in distance tiles, [2] axis-line hits,
  * [3] device bytes held by the context. */
 sdf_status sdf_ctx_counters(sdf_ctx *ctx, int64_t *out, int32_t cap,

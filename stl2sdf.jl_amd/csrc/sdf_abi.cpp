@@ -248,42 +248,44 @@ void bvh_async(sdf_ctx *c, int q) {
   SDF_HIP(hipEventRecord(c->join[2], c->aux[q]));
 }
 
-// Adaptive vote for the *n_dev items of a list (capacity cap) whose rays
-// below r0 are already counted in their state.  Phases: the first
-// min_rays rays (when r0 < min_rays), the first batch of 4, then the rest;
-// each phase hands the items it cannot finish to the next, so later (and
-// longer) phases trace rays only for the survivors.  cnt: 4 device words
-// per phase (at most 12); winding counts land in word 2 of each phase.
+// Adaptive vote for the items of one or two lists (PendLists: list k's
+// rays below r0[k] are already counted in its items' state).  Phases, from
+// a = the larger r0: the first min_rays rays (when a < min_rays), the next
+// batch of 4, then the rest; every list takes part in the first phase,
+// tracing its own rays from its r0, and each phase hands the items it
+// cannot finish to the next, so later (and longer) phases trace rays only
+// for the survivors.  cnt: 4 device words per phase (at most 12); winding
+// counts land in word 2 of each phase.
 void resolve_items(sdf_ctx *c, const sdf::GridP &g, const double *d_pts,
-                   const sdf::RayParams &rp, sdf::Pending *items, const uint32_t *n_dev,
-                   uint32_t cap, int r0, sdf::FusedOut out, uint32_t *cnt) {
-  if (cap == 0) return;
+                   const sdf::RayParams &rp, const sdf::PendLists &first, sdf::FusedOut out,
+                   uint32_t *cnt) {
+  const uint64_t slots = first.slots();
+  if (slots == 0) return;
+  const uint32_t cap = slots < 0xFFFFFFF0ull ? (uint32_t)slots : 0xFFFFFFF0u;
   const sdf::Bvh &bvh = c->bvh;
   const int rmax = rp.max_rays < sdf::kNumDirs ? rp.max_rays : sdf::kNumDirs;
-  int ph[3][2], np = 0, a = r0;
+  int a = first.r0[0];
+  for (int k = 1; k < first.count; k++) a = first.r0[k] > a ? first.r0[k] : a;
+  int ph[3], np = 0;  // phase ends
   if (a < rp.min_rays) {
-    ph[np][0] = a;
-    ph[np++][1] = rp.min_rays;
+    ph[np++] = rp.min_rays;
     a = rp.min_rays;
   }
   if (a + 4 < rmax) {
-    ph[np][0] = a;
-    ph[np++][1] = a + 4;
+    ph[np++] = a + 4;
     a += 4;
   }
-  ph[np][0] = a;
-  ph[np++][1] = rmax;
+  ph[np++] = rmax;
   static const char *names[2] = {"res_next0", "res_next1"};
-  const sdf::Pending *cur = items;
-  const uint32_t *cur_n = n_dev;
+  sdf::PendLists cur = first;
   for (int p = 0; p < np; p++) {
     const bool last = p + 1 == np;
     sdf::Pending *next = last ? nullptr : c->ar.get<sdf::Pending>(names[p], cap);
-    sdf::resolve_pending(c->stream, c->ar, g, d_pts, c->tri, c->nel, bvh, rp, cur, cur_n, cap,
-                         ph[p][0], ph[p][1], last ? 1 : 0, out, next, nullptr, cap, cnt + 4 * p,
-                         c->caps.wind, c->d_words, c->ray_stack);
-    cur = next;
-    cur_n = cnt + 4 * p + 1;
+    sdf::resolve_pending(c->stream, c->ar, g, d_pts, c->tri, c->nel, bvh, rp, cur, ph[p],
+                         last ? 1 : 0, out, next, nullptr, cap, cnt + 4 * p, c->caps.wind,
+                         c->d_words, c->ray_stack);
+    cur = sdf::PendLists();
+    cur.add(next, cnt + 4 * p + 1, cap, ph[p]);
   }
 }
 
@@ -451,15 +453,12 @@ void stage_resolve(sdf_ctx *c, LatticeRun &L) {
     }
     c->mark("resolve");
     // points whose axis-ray decisions fell inside a rounding margin vote
-    // from ray 1; those still open after the first min_rays rays join the
-    // list of points the six axis rays left undecided
-    if (L.cap0 > 0)
-      sdf::resolve_pending(c->stream, c->ar, L.g, nullptr, c->tri, c->nel, c->bvh, L.rp,
-                           L.pend0, c->d_words + 2, L.cap0, 0, L.rp.min_rays, 0, L.out, L.pend,
-                           c->d_words + 1, L.cap, cnt + 8, c->caps.wind, c->d_words,
-                           c->ray_stack);
-    resolve_items(c, L.g, nullptr, L.rp, L.pend, c->d_words + 1, L.cap, L.rp.min_rays, L.out,
-                  cnt);
+    // from ray 1, in the same launches as the points the six axis rays left
+    // undecided (which vote from ray min_rays + 1)
+    sdf::PendLists lists;
+    if (L.cap0 > 0) lists.add(L.pend0, c->d_words + 2, L.cap0, 0);
+    lists.add(L.pend, c->d_words + 1, L.cap, L.rp.min_rays);
+    resolve_items(c, L.g, nullptr, L.rp, lists, L.out, cnt);
   }
   c->mark("end");
   sdf::StatusSrc src;
@@ -1444,7 +1443,9 @@ sdf_status sdf_raycast_signs(sdf_ctx *c, const double *X, int64_t nnp,
         sdf::launch_fill_u32(c->stream, nd, 1, (uint32_t)npts);
         uint32_t *cnt = c->d_words + 4;
         c->mark("resolve");
-        resolve_items(c, g, d_pts, rp, pend, nd, (uint32_t)npts, 0, out, cnt);
+        sdf::PendLists lists;
+        lists.add(pend, nd, (uint32_t)npts, 0);
+        resolve_items(c, g, d_pts, rp, lists, out, cnt);
         c->mark("end");
         uint32_t hc[12];
         SDF_HIP(hipMemcpyAsync(hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, c->stream));

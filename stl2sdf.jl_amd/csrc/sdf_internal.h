@@ -269,22 +269,40 @@ struct RayParams {
 
 struct Bvh;
 
-// Adaptive vote for pending items: rays [r0, r1) via the LBVH, then the
-// vote replay (final_phase = 0: items needing more rays go to `next`), then
-// the winding fallback.  pts == nullptr: lattice points of g.
-// Adaptive vote for *n_dev pending items (capacity cap) over rays r0..r1-1,
-// with no host waits: grids are fixed, counts are read on the device.
+// One or two pending lists voted in one phase.  List k holds *n_dev[k]
+// items (capacity cap[k]) whose rays below r0[k] are already counted in
+// their state; an item's position over both lists (its slot) is its index
+// in list 0, or cap[0] + its index in list 1.
+struct PendLists {
+  const Pending *items[2] = {nullptr, nullptr};
+  const uint32_t *n_dev[2] = {nullptr, nullptr};
+  uint32_t cap[2] = {0, 0};
+  int32_t r0[2] = {0, 0};
+  int32_t count = 0;
+  void add(const Pending *p, const uint32_t *n, uint32_t c, int r) {
+    items[count] = p;
+    n_dev[count] = n;
+    cap[count] = c;
+    r0[count] = r;
+    count++;
+  }
+  uint64_t slots() const { return (uint64_t)cap[0] + (count > 1 ? cap[1] : 0u); }
+};
+
+// Adaptive vote for pending items, with no host waits (grids are fixed,
+// counts are read on the device): each list traces its rays [r0[k], r1)
+// through the LBVH (overflowing rays by brute force), then the vote replay.
 // final_phase 0 carries items whose vote needs a ray >= r1 to `next`
 // (count *nnext, or d_counts[1] when nnext is null; capacity next_cap);
 // otherwise every item is finished.  Items that still need the winding
 // fallback (count d_counts[2]) are finished if it is <= wind_cap (else the
 // caller repeats the call with a larger wind_cap).  d_counts: 4 words.
+// pts == nullptr: lattice points of g.
 void resolve_pending(hipStream_t s, Arena &ar, const GridP &g, const double *pts,
                      const Tri64 *tri, int64_t nel, const Bvh &bvh, const RayParams &rp,
-                     const Pending *items, const uint32_t *n_dev, uint32_t cap, int r0,
-                     int r1, int final_phase, FusedOut out, Pending *next, uint32_t *nnext,
-                     uint32_t next_cap, uint32_t *d_counts, uint32_t wind_cap, uint32_t *err,
-                     int stack = 1024);
+                     const PendLists &lists, int r1, int final_phase, FusedOut out,
+                     Pending *next, uint32_t *nnext, uint32_t next_cap, uint32_t *d_counts,
+                     uint32_t wind_cap, uint32_t *err, int stack = 1024);
 
 // xp of evalDistancesOnTriMesh: exact FP64 closest point on the mesh for
 // every lattice point (pts == nullptr) or caller point, 3 doubles per point.

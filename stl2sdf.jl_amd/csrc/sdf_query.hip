@@ -807,6 +807,18 @@ __device__ inline void item_point(const GridP &g, const double *pts, int64_t idx
   }
 }
 
+// Live count of list k.
+__device__ inline uint32_t list_count(const PendLists &L, int k) {
+  uint32_t n = L.cap[k];
+  if (L.n_dev[k] && *L.n_dev[k] < n) n = *L.n_dev[k];
+  return n;
+}
+
+// The item in a slot over both lists.
+__device__ inline const Pending &slot_item(const PendLists &L, uint64_t slot) {
+  return slot < L.cap[0] ? L.items[0][slot] : L.items[1][slot - L.cap[0]];
+}
+
 // Wavefront-cooperative ray parity: one wavefront per (item, ray).  The 64
 // lanes pop up to 64 nodes at a time from an LDS stack, test both children
 // (FP64 slab test on boxes rounded outward), run the reference's FP64
@@ -825,23 +837,32 @@ constexpr int kRayWaves = 8192;  // persistent grid of the ray stage (waves)
 template <int kWaveStack>
 __global__ __launch_bounds__(64) void ray_wave_kernel(
     GridP g, const double *__restrict__ pts, const Tri64 *__restrict__ tri, Bvh bvh,
-    RayParams rp, const Pending *__restrict__ items, uint32_t nitems,
-    const uint32_t *__restrict__ nitems_dev, int r0, int r1,
-    unsigned long long *__restrict__ bits, uint32_t *__restrict__ fb, uint32_t *nfb,
-    uint32_t fb_cap, uint32_t *err) {
+    RayParams rp, PendLists lists, int r1, unsigned long long *__restrict__ bits,
+    uint64_t *__restrict__ fb, uint32_t *nfb, uint32_t fb_cap, uint32_t *err) {
   __shared__ int32_t stack[kWaveStack];
   __shared__ double hit_t[kWaveHits];
   __shared__ int s_top, s_nh, s_ovf, s_cnt;
   const int lane = threadIdx.x;
   const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-  const int nr = r1 - r0;
-  if (nitems_dev && *nitems_dev < nitems) nitems = *nitems_dev;
-  const int64_t total = (int64_t)nitems * nr;
+  // list 0 contributes n0 * nr0 (item, ray) pairs, then list 1
+  const uint32_t n0 = list_count(lists, 0);
+  const uint32_t n1 = lists.count > 1 ? list_count(lists, 1) : 0u;
+  const int nr0 = r1 > lists.r0[0] ? r1 - lists.r0[0] : 0;
+  const int nr1 = lists.count > 1 && r1 > lists.r0[1] ? r1 - lists.r0[1] : 0;
+  const int64_t w0 = (int64_t)n0 * nr0;
+  const int64_t total = w0 + (int64_t)n1 * nr1;
   for (int64_t w = blockIdx.x; w < total; w += gridDim.x) {
-    const uint32_t it = (uint32_t)(w / nr);
-    const int r = r0 + (int)(w % nr);
+    uint64_t slot;
+    int r;
+    if (w < w0) {
+      slot = (uint64_t)(w / nr0);
+      r = lists.r0[0] + (int)(w % nr0);
+    } else {
+      slot = (uint64_t)lists.cap[0] + (uint64_t)((w - w0) / nr1);
+      r = lists.r0[1] + (int)((w - w0) % nr1);
+    }
     double o[3];
-    item_point(g, pts, items[it].idx, o);
+    item_point(g, pts, slot_item(lists, slot).idx, o);
     const double *d = rp.dirs + 3 * r;
     double inv[3];
     for (int a = 0; a < 3; a++) inv[a] = d[a] != 0.0 ? 1.0 / d[a] : 0.0;
@@ -929,13 +950,13 @@ __global__ __launch_bounds__(64) void ray_wave_kernel(
     __syncthreads();
     if (lane == 0) {
       if (ovf) {
-        const uint32_t slot = atomicAdd(nfb, 1u);
-        if (slot < fb_cap)
-          fb[slot] = it * 64u + (uint32_t)r;
+        const uint32_t q = atomicAdd(nfb, 1u);
+        if (q < fb_cap)
+          fb[q] = slot * 64u + (uint64_t)r;
         else
           atomicOr(err, kErrHitOverflow);
       } else if (s_cnt & 1) {
-        atomicOr(&bits[it], 1ull << r);
+        atomicOr(&bits[slot], 1ull << r);
       }
     }
     __syncthreads();
@@ -944,19 +965,19 @@ __global__ __launch_bounds__(64) void ray_wave_kernel(
 
 __global__ __launch_bounds__(256) void ray_bits_brute_kernel(
     GridP g, const double *__restrict__ pts, const Tri64 *__restrict__ tri,
-    int64_t nel, RayParams rp, const Pending *__restrict__ items,
-    const uint32_t *__restrict__ fb, uint32_t nfb, const uint32_t *__restrict__ nfb_dev,
-    unsigned long long *__restrict__ bits, uint32_t *err) {
+    int64_t nel, RayParams rp, PendLists lists, const uint64_t *__restrict__ fb,
+    uint32_t nfb, const uint32_t *__restrict__ nfb_dev, unsigned long long *__restrict__ bits,
+    uint32_t *err) {
   __shared__ double s_t[kHitCap];
   __shared__ int s_n, s_cnt;
   if (nfb_dev && *nfb_dev < nfb) nfb = *nfb_dev;
   for (uint32_t q = blockIdx.x; q < nfb; q += gridDim.x) {
-    const uint32_t it = fb[q] / 64u;
+    const uint64_t slot = fb[q] / 64u;
     const int r = (int)(fb[q] % 64u);
     double o[3];
-    item_point(g, pts, items[it].idx, o);
+    item_point(g, pts, slot_item(lists, slot).idx, o);
     int par = block_ray_parity(o, rp.dirs + 3 * r, tri, nel, s_t, &s_n, &s_cnt, err);
-    if (threadIdx.x == 0 && par) atomicOr(&bits[it], 1ull << r);
+    if (threadIdx.x == 0 && par) atomicOr(&bits[slot], 1ull << r);
     __syncthreads();
   }
 }
@@ -965,17 +986,18 @@ __global__ __launch_bounds__(256) void ray_bits_brute_kernel(
 // parities.  Only rays below `avail` are available: an item whose vote
 // needs a later ray (the first min_rays, or the next batch of 4) is
 // appended to `next` (count *nnext, capacity next_cap) with its state.
-__global__ void vote_kernel(RayParams rp, const Pending *__restrict__ items,
-                            uint32_t nitems, const uint32_t *__restrict__ nitems_dev,
+__global__ void vote_kernel(RayParams rp, PendLists lists,
                             const unsigned long long *__restrict__ bits, int avail,
                             FusedOut out, Pending *__restrict__ next, uint32_t *nnext,
-                            uint32_t next_cap, uint32_t *__restrict__ wind, uint32_t *nwind,
+                            uint32_t next_cap, uint64_t *__restrict__ wind, uint32_t *nwind,
                             uint32_t *err) {
-  if (nitems_dev && *nitems_dev < nitems) nitems = *nitems_dev;
-  for (uint32_t it = blockIdx.x * blockDim.x + threadIdx.x; it < nitems;
-       it += gridDim.x * blockDim.x) {
-    Pending P = items[it];
-    const unsigned long long m = bits[it];
+  const uint64_t n0 = list_count(lists, 0);
+  const uint64_t n = n0 + (lists.count > 1 ? list_count(lists, 1) : 0u);
+  for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < n;
+       t += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t slot = t < n0 ? t : (uint64_t)lists.cap[0] + (t - n0);
+    Pending P = slot_item(lists, slot);
+    const unsigned long long m = bits[slot];
     int inside = P.inside, total = P.used, used = P.used;
     bool carry = false;
     if (used < rp.min_rays) {
@@ -1020,8 +1042,8 @@ __global__ void vote_kernel(RayParams rp, const Pending *__restrict__ items,
     }
     double sgn = ratio > 0.5 ? 1.0 : -1.0;
     if (rp.use_winding && conf < rp.conf_thr) {
-      uint32_t slot = atomicAdd(nwind, 1u);
-      wind[slot] = it;
+      const uint32_t q = atomicAdd(nwind, 1u);
+      wind[q] = slot;
       continue;
     }
     if (out.sdf) out.sdf[P.idx] = P.dist * sgn;
@@ -1036,8 +1058,8 @@ __global__ void vote_kernel(RayParams rp, const Pending *__restrict__ items,
 // result is deterministic.
 __global__ __launch_bounds__(256) void winding_partial_kernel(
     GridP g, const double *__restrict__ pts, const Tri64 *__restrict__ tri,
-    int64_t nel, const Pending *__restrict__ items,
-    const uint32_t *__restrict__ wind, uint32_t nwind, const uint32_t *__restrict__ nwind_dev,
+    int64_t nel, PendLists lists,
+    const uint64_t *__restrict__ wind, uint32_t nwind, const uint32_t *__restrict__ nwind_dev,
     int chunks, int64_t chunk, double *__restrict__ partial) {
 #pragma clang fp contract(off)
   __shared__ double s_red[256];
@@ -1046,7 +1068,7 @@ __global__ __launch_bounds__(256) void winding_partial_kernel(
   for (int64_t q = blockIdx.x; q < nblk; q += gridDim.x) {
     const uint32_t w = (uint32_t)(q / chunks);
     const int64_t c = q % chunks;
-    const Pending P = items[wind[w]];
+    const Pending P = slot_item(lists, wind[w]);
     double o[3];
     item_point(g, pts, P.idx, o);
     const int64_t e0 = c * chunk, e1 = min(nel, e0 + chunk);
@@ -1080,8 +1102,7 @@ __global__ __launch_bounds__(256) void winding_partial_kernel(
   }
 }
 
-__global__ void winding_final_kernel(const Pending *__restrict__ items,
-                                     const uint32_t *__restrict__ wind,
+__global__ void winding_final_kernel(PendLists lists, const uint64_t *__restrict__ wind,
                                      uint32_t nwind, const uint32_t *__restrict__ nwind_dev,
                                      int chunks, const double *__restrict__ partial,
                                      FusedOut out) {
@@ -1092,7 +1113,7 @@ __global__ void winding_final_kernel(const Pending *__restrict__ items,
     for (int c = 0; c < chunks; c++) tot += partial[(int64_t)w * chunks + c];
     tot = tot / 12.566370614359172;  // Float64(4pi)
     const double sgn = fabs(tot) > 0.5 ? 1.0 : -1.0;
-    const Pending P = items[wind[w]];
+    const Pending P = slot_item(lists, wind[w]);
     if (out.sdf) out.sdf[P.idx] = P.dist * sgn;
     if (out.signs) out.signs[P.idx] = sgn;
     if (out.conf) out.conf[P.idx] = 0.8;
@@ -1113,11 +1134,15 @@ __global__ void dist_points_bvh_kernel(const double *__restrict__ pts,
   }
 }
 
-__global__ void fill_bits_kernel(unsigned long long *p, uint32_t cap,
-                                 const uint32_t *__restrict__ n_dev) {
-  uint32_t n = *n_dev < cap ? *n_dev : cap;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    p[i] = 0ull;
+// Clears the ray bits of the live slots of both lists.
+__global__ void fill_bits_kernel(unsigned long long *p, PendLists lists) {
+  for (int k = 0; k < lists.count; k++) {
+    const uint32_t n = list_count(lists, k);
+    unsigned long long *q = p + (k ? lists.cap[0] : 0u);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += gridDim.x * blockDim.x)
+      q[i] = 0ull;
+  }
 }
 
 __global__ void init_pending_all_kernel(Pending *pend, int64_t n) {
@@ -1263,38 +1288,41 @@ static int blocks_for(int64_t n, int t) {
   return (int)b;
 }
 
-// Decide pending items with the adaptive vote: rays [r0, r1) by LBVH
-// traversal, overflowing rays by brute force, then the vote replay.
+// Decide pending items with the adaptive vote: each list's rays [r0, r1)
+// by LBVH traversal, overflowing rays by brute force, then the vote replay.
 void resolve_pending(hipStream_t s, Arena &ar, const GridP &g, const double *pts,
                      const Tri64 *tri, int64_t nel, const Bvh &bvh, const RayParams &rp,
-                     const Pending *items, const uint32_t *n_dev, uint32_t cap, int r0,
-                     int r1, int final_phase, FusedOut out, Pending *next, uint32_t *nnext,
-                     uint32_t next_cap, uint32_t *d_counts, uint32_t wind_cap, uint32_t *err,
-                     int stack) {
+                     const PendLists &lists, int r1, int final_phase, FusedOut out,
+                     Pending *next, uint32_t *nnext, uint32_t next_cap, uint32_t *d_counts,
+                     uint32_t wind_cap, uint32_t *err, int stack) {
   // d_counts: [0] fallback rays, [1] next items (unless nnext), [2] winding items
-  if (cap == 0) return;
+  const uint64_t slots = lists.slots();
+  if (slots == 0) return;
   if (r1 > kNumDirs) r1 = kNumDirs;
   launch_fill_u32(s, d_counts, 4, 0u);
-  unsigned long long *bits = ar.get<unsigned long long>(ar.tmp("res_bits"), cap);
-  hipLaunchKernelGGL(fill_bits_kernel, dim3(128), dim3(256), 0, s, bits, cap, n_dev);
-  const uint32_t fb_cap = cap * 4u + 1024u;
-  uint32_t *fb = ar.get<uint32_t>(ar.tmp("res_fb"), fb_cap);
-  if (r1 > r0) {
+  unsigned long long *bits = ar.get<unsigned long long>(ar.tmp("res_bits"), slots);
+  hipLaunchKernelGGL(fill_bits_kernel, dim3(128), dim3(256), 0, s, bits, lists);
+  const uint64_t fb_want = slots * 4u + 1024u;
+  const uint32_t fb_cap = fb_want < 0xFFFFFFF0ull ? (uint32_t)fb_want : 0xFFFFFFF0u;
+  uint64_t *fb = ar.get<uint64_t>(ar.tmp("res_fb"), fb_cap);
+  bool rays = false;
+  for (int k = 0; k < lists.count; k++) rays = rays || r1 > lists.r0[k];
+  if (rays) {
     // LDS traversal stack depth (entries); a ray that overflows it is redone
     // by the brute-force kernel, so the depth only trades occupancy
     if (stack <= 1024)
       hipLaunchKernelGGL(ray_wave_kernel<1024>, dim3(kRayWaves), dim3(64), 0, s, g, pts, tri,
-                         bvh, rp, items, cap, n_dev, r0, r1, bits, fb, d_counts, fb_cap, err);
+                         bvh, rp, lists, r1, bits, fb, d_counts, fb_cap, err);
     else
       hipLaunchKernelGGL(ray_wave_kernel<2048>, dim3(kRayWaves), dim3(64), 0, s, g, pts, tri,
-                         bvh, rp, items, cap, n_dev, r0, r1, bits, fb, d_counts, fb_cap, err);
+                         bvh, rp, lists, r1, bits, fb, d_counts, fb_cap, err);
     hipLaunchKernelGGL(ray_bits_brute_kernel, dim3(256), dim3(256), 0, s, g, pts, tri, nel, rp,
-                       items, (const uint32_t *)fb, fb_cap, (const uint32_t *)d_counts, bits,
+                       lists, (const uint64_t *)fb, fb_cap, (const uint32_t *)d_counts, bits,
                        err);
     SDF_HIP(hipGetLastError());
   }
-  uint32_t *wind = ar.get<uint32_t>(ar.tmp("res_wind"), cap);
-  hipLaunchKernelGGL(vote_kernel, dim3(256), dim3(256), 0, s, rp, items, cap, n_dev, bits,
+  uint64_t *wind = ar.get<uint64_t>(ar.tmp("res_wind"), slots);
+  hipLaunchKernelGGL(vote_kernel, dim3(256), dim3(256), 0, s, rp, lists, bits,
                      final_phase ? kNumDirs : r1, out, next, nnext ? nnext : d_counts + 1,
                      next_cap, wind, d_counts + 2, err);
   SDF_HIP(hipGetLastError());
@@ -1310,10 +1338,10 @@ void resolve_pending(hipStream_t s, Arena &ar, const GridP &g, const double *pts
     }
     double *partial = ar.get<double>(ar.tmp("res_wpartial"), (size_t)wind_cap * chunks);
     hipLaunchKernelGGL(winding_partial_kernel, dim3(1024), dim3(256), 0, s, g, pts, tri, nel,
-                       items, (const uint32_t *)wind, wind_cap,
+                       lists, (const uint64_t *)wind, wind_cap,
                        (const uint32_t *)(d_counts + 2), chunks, chunk, partial);
-    hipLaunchKernelGGL(winding_final_kernel, dim3(16), dim3(256), 0, s, items,
-                       (const uint32_t *)wind, wind_cap, (const uint32_t *)(d_counts + 2),
+    hipLaunchKernelGGL(winding_final_kernel, dim3(16), dim3(256), 0, s, lists,
+                       (const uint64_t *)wind, wind_cap, (const uint32_t *)(d_counts + 2),
                        chunks, (const double *)partial, out);
     SDF_HIP(hipGetLastError());
   }
