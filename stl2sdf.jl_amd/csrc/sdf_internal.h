@@ -165,28 +165,35 @@ struct DistJob {
 };
 
 // Sign / combine work of one lattice: 8 x 8 x kSignZ bricks.
+// Fields of a sign job that only its rare paths read (a line's crossing
+// list, the pending lists, error flags); the kernel keeps them in LDS, not
+// in scalar registers.
+struct FusedCold {
+  const uint64_t *line_off;
+  const LineFast *fast;
+  const uint32_t *meta;
+  Pending *pend;
+  uint32_t *npend;
+  Pending *pend0;
+  uint32_t *npend0;
+  uint32_t *err;
+  uint32_t cap, cap0;
+};
+
 struct FusedJob {
   GridP g;
   TileDims td;
   TileDims bd;  // brick counts
   const uint64_t *chunk_off;
   const float *d2buf;
-  const uint64_t *line_off;
-  const LineFast *fast;
-  const uint32_t *meta;
   const LineInline *inl;
   int64_t base1, base2;
   FusedOut out;
-  Pending *pend;
-  uint32_t *npend;
-  Pending *pend0;
-  uint32_t *npend0;
-  uint32_t *err;
   int64_t chunk_cap;
   int64_t nbricks;  // launch size of the brick order
-  uint32_t cap, cap0;
   int32_t do_sign;
   int32_t pad_;
+  FusedCold cold;
 };
 DistJob make_dist_job(const GridP &g, const Tri64 *tri, const Bins &b, float *d2buf,
                       const uint32_t *only);

@@ -35,11 +35,12 @@ __device__ inline float order_float(uint32_t u) {
   return __uint_as_float((u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u);
 }
 
-// Triangle boxes (outward-rounded FP32) and the centroid bounds (reduced per
-// block in LDS, then one atomic per block and bound).
+// Triangle boxes (outward-rounded FP32) and the centroid bounds, reduced
+// per block in LDS into part[6 * block + k] (no atomics: the blocks' bounds
+// would all hit the same six words).
 __global__ __launch_bounds__(256) void leaf_boxes_kernel(
     const Tri64 *__restrict__ tri, int64_t n, float *__restrict__ box,
-    uint32_t *__restrict__ cbounds) {
+    uint32_t *__restrict__ part) {
   __shared__ uint32_t red[6][256];
   uint32_t cmin[3] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu};
   uint32_t cmax[3] = {0u, 0u, 0u};
@@ -70,8 +71,29 @@ __global__ __launch_bounds__(256) void leaf_boxes_kernel(
       }
     __syncthreads();
   }
-  if (tid < 3) atomicMin(&cbounds[tid], red[tid][0]);
-  else if (tid < 6) atomicMax(&cbounds[tid], red[tid][0]);
+  if (tid < 6) part[6 * (int64_t)blockIdx.x + tid] = red[tid][0];
+}
+
+// The centroid bounds from the per-block parts (one workgroup).
+__global__ __launch_bounds__(256) void bounds_reduce_kernel(const uint32_t *__restrict__ part,
+                                                            int64_t nparts,
+                                                            uint32_t *__restrict__ cbounds) {
+  __shared__ uint32_t red[6][256];
+  const int tid = threadIdx.x;
+  uint32_t v[6] = {0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
+  for (int64_t b = tid; b < nparts; b += blockDim.x)
+    for (int k = 0; k < 6; k++)
+      v[k] = k < 3 ? min(v[k], part[6 * b + k]) : max(v[k], part[6 * b + k]);
+  for (int k = 0; k < 6; k++) red[k][tid] = v[k];
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (tid < off)
+      for (int k = 0; k < 6; k++)
+        red[k][tid] = k < 3 ? min(red[k][tid], red[k][tid + off])
+                            : max(red[k][tid], red[k][tid + off]);
+    __syncthreads();
+  }
+  if (tid < 6) cbounds[tid] = red[tid][0];
 }
 
 __device__ inline uint32_t expand10(uint32_t v) {
@@ -107,6 +129,65 @@ __global__ void morton_kernel(const float *__restrict__ box, int64_t n,
 }
 
 // ---- radix sort (LSD, stable) ----
+//
+// A pass: per-block digit counts (digit-major, hist[d * nblocks + b]), the
+// exclusive scan of each digit's row (one workgroup per digit, which also
+// writes the row total), then the stable scatter, whose blocks add the
+// digit bases (exclusive scan of the 256 totals) themselves.
+static_assert(kSortThreads == kRadix, "one thread per digit in the scatter prologue");
+
+// Exclusive prefix sum over the 256 threads of a block; *total gets the
+// block's sum.  Contains barriers: every thread of the block calls it.
+__device__ inline uint64_t block_scan_256(uint64_t v, uint64_t *total) {
+  __shared__ uint64_t s_wsum[kSortThreads / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint64_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint64_t y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) s_wsum[w] = x;
+  __syncthreads();
+  uint64_t pre = 0, tot = 0;
+#pragma unroll
+  for (int k = 0; k < kSortThreads / 64; k++) {
+    const uint64_t q = s_wsum[k];
+    if (k < w) pre += q;
+    tot += q;
+  }
+  __syncthreads();
+  *total = tot;
+  return pre + x - v;
+}
+
+__global__ __launch_bounds__(kSortThreads) void radix_rows_kernel(
+    const uint32_t *__restrict__ hist, int64_t nblocks, uint64_t *__restrict__ off,
+    uint64_t *__restrict__ total) {
+  constexpr int kItems = 8;
+  const uint32_t *row = hist + (int64_t)blockIdx.x * nblocks;
+  uint64_t *orow = off + (int64_t)blockIdx.x * nblocks;
+  uint64_t carry = 0;
+  for (int64_t base = 0; base < nblocks; base += (int64_t)kSortThreads * kItems) {
+    const int64_t b0 = base + (int64_t)threadIdx.x * kItems;
+    uint32_t v[kItems];
+    uint64_t sum = 0;
+#pragma unroll
+    for (int q = 0; q < kItems; q++) {
+      v[q] = b0 + q < nblocks ? row[b0 + q] : 0u;
+      sum += v[q];
+    }
+    uint64_t tot;
+    uint64_t pre = block_scan_256(sum, &tot) + carry;
+#pragma unroll
+    for (int q = 0; q < kItems; q++) {
+      if (b0 + q < nblocks) orow[b0 + q] = pre;
+      pre += v[q];
+    }
+    carry += tot;
+  }
+  if (threadIdx.x == 0) total[blockIdx.x] = carry;
+}
 
 __global__ __launch_bounds__(kSortThreads) void radix_hist_kernel(
     const uint32_t *__restrict__ keys, int64_t n, const uint64_t *__restrict__ n_dev,
@@ -144,16 +225,18 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
     const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
     uint32_t *__restrict__ kout, uint32_t *__restrict__ vout, int64_t n,
     const uint64_t *__restrict__ n_dev, int shift, int64_t nblocks,
-    const uint64_t *__restrict__ off) {
+    const uint64_t *__restrict__ off, const uint64_t *__restrict__ total) {
   if (n_dev && (int64_t)*n_dev < n) n = (int64_t)*n_dev;
   constexpr int kWaves = kSortThreads / 64;
   __shared__ uint32_t running[kRadix];
   __shared__ uint32_t wave_cnt[kWaves][kRadix];
   __shared__ uint64_t base_off[kRadix];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  for (int d = tid; d < kRadix; d += blockDim.x) {
-    running[d] = 0;
-    base_off[d] = off[(int64_t)d * nblocks + blockIdx.x];
+  {
+    uint64_t all;
+    const uint64_t digit_base = block_scan_256(total[tid], &all);
+    running[tid] = 0;
+    base_off[tid] = digit_base + off[(int64_t)tid * nblocks + blockIdx.x];
   }
   const int64_t base = (int64_t)blockIdx.x * kSortTile;
   const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -354,16 +437,18 @@ void radix_sort_pairs(hipStream_t s, Arena &ar, uint32_t *keys, uint32_t *vals,
   uint32_t *v2 = ar.get<uint32_t>(ar.tmp("sort_v2"), n);
   uint32_t *hist = ar.get<uint32_t>(ar.tmp("sort_hist"), kRadix * nblocks + 1);
   uint64_t *off = ar.get<uint64_t>(ar.tmp("sort_off"), kRadix * nblocks + 2);
+  uint64_t *total = ar.get<uint64_t>(ar.tmp("sort_total"), kRadix);
   uint32_t *ka = keys, *va = vals, *kb = k2, *vb = v2;
   int passes = 0;
   for (int shift = 0; shift < key_bits; shift += kRadixBits, passes++) {
     hipLaunchKernelGGL(radix_hist_kernel, dim3((unsigned)nblocks),
                        dim3(kSortThreads), 0, s, ka, n, n_dev, shift, nblocks, hist);
     SDF_HIP(hipGetLastError());
-    scan_counts(s, ar, hist, off, kRadix * nblocks);
+    hipLaunchKernelGGL(radix_rows_kernel, dim3(kRadix), dim3(kSortThreads), 0, s,
+                       (const uint32_t *)hist, nblocks, off, total);
     hipLaunchKernelGGL(radix_scatter_kernel, dim3((unsigned)nblocks),
                        dim3(kSortThreads), 0, s, ka, va, kb, vb, n, n_dev, shift,
-                       nblocks, off);
+                       nblocks, (const uint64_t *)off, (const uint64_t *)total);
     SDF_HIP(hipGetLastError());
     uint32_t *t = ka;
     ka = kb;
@@ -397,13 +482,12 @@ Bvh build_lbvh(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t n) {
   if (n <= 0) return b;
   b.leaf_box = ar.get<float>("bvh_leaf_box", 6 * n);
   uint32_t *cb = ar.get<uint32_t>("bvh_cbounds", 8);
-  launch_fill_u32(s, cb, 3, 0xFFFFFFFFu);
-  launch_fill_u32(s, cb + 3, 3, 0u);
-  // few workgroups: each adds its share of the centroid bounds with six
-  // atomics on the same six words, which serialise
-  const int lb_blocks = blocks_for(n, 256) < 512 ? blocks_for(n, 256) : 512;
+  const int lb_blocks = blocks_for(n, 256);
+  uint32_t *part = ar.get<uint32_t>("bvh_cbounds_part", 6 * (size_t)lb_blocks);
   hipLaunchKernelGGL(leaf_boxes_kernel, dim3(lb_blocks), dim3(256), 0, s, tri, n, b.leaf_box,
-                     cb);
+                     part);
+  hipLaunchKernelGGL(bounds_reduce_kernel, dim3(1), dim3(256), 0, s, (const uint32_t *)part,
+                     (int64_t)lb_blocks, cb);
   SDF_HIP(hipGetLastError());
   uint32_t *codes = ar.get<uint32_t>("bvh_codes", n);
   uint32_t *leaf_tri = ar.get<uint32_t>("bvh_leaf_tri", n);

@@ -127,12 +127,21 @@ __device__ inline bool inline_uniform(const LineInline &r, double lo, double hi,
   return uni;
 }
 
+// LDS of one distance workgroup: the chunk's triangle records, the
+// (row, triangle) jobs of phase 2 and the per-point minima.
+constexpr int kTileRows = kTileY * kTileZ;
+struct DistShared {
+  float pre[kTriPreFloats + 6][kTilePts];
+  uint16_t job[kTileRows * kTilePts];  // row << 8 | triangle slot
+  uint32_t best[kTilePts];             // squared distances as ordered bits
+};
+
 // One distance work item (tile, chunk of <= clen triangles) for the 64-lane
-// block: the chunk's triangles are staged in LDS as precomputed records and
-// every lane evaluates its tile point against all of them.  Uniform per
-// block (contains a barrier).
-__device__ inline void dist_item(const DistJob &J, int64_t item,
-                                 float (&s_pre)[kTriPreFloats + 6][kTilePts]) {
+// block: the chunk's triangles are staged in LDS as precomputed records, a
+// row test keeps each row's candidates, and the exact distances of every
+// (row, candidate) job are spread evenly over the wave, 8 lanes (the row's
+// points) per job.  Uniform per block (contains barriers).
+__device__ inline void dist_item(const DistJob &J, int64_t item, DistShared &S) {
   const GridP &g = J.g;
   const TileDims &td = J.td;
   const Tri64 *__restrict__ tri = J.tri;
@@ -178,14 +187,14 @@ __device__ inline void dist_item(const DistJob &J, int64_t item,
     float rec[kTriPreFloats];
     tri_pre_make(v, v + 3, v + 6, rec);
 #pragma unroll
-    for (int f = 0; f < kTriPreFloats; f++) s_pre[f][lane] = rec[f];
+    for (int f = 0; f < kTriPreFloats; f++) S.pre[f][lane] = rec[f];
     float bmin[3], bmax[3];
 #pragma unroll
     for (int a = 0; a < 3; a++) {  // the triangle's box
       bmin[a] = fminf(fminf(v[a], v[3 + a]), v[6 + a]);
       bmax[a] = fmaxf(fmaxf(v[a], v[3 + a]), v[6 + a]);
-      s_pre[kTriPreFloats + a][lane] = bmin[a];
-      s_pre[kTriPreFloats + 3 + a][lane] = bmax[a];
+      S.pre[kTriPreFloats + a][lane] = bmin[a];
+      S.pre[kTriPreFloats + 3 + a][lane] = bmax[a];
     }
     // row coordinates relative to the tile origin, as the lanes form them
     const float x1 = (float)(lattice_coord(g.ox, g.h, i0 + kTileX - 1) - T0.x);
@@ -209,44 +218,54 @@ __device__ inline void dist_item(const DistJob &J, int64_t item,
       }
     }
   }
-  // each lane takes the candidate mask of its row
-  const int myrow = lane / kTileX;
-  unsigned long long cand = 0ull;
+  // job list: row 0's candidates in lane order, then row 1's, ...
+  const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  int njobs = 0;
 #pragma unroll
-  for (int r = 0; r < kTileY * kTileZ; r++) {
+  for (int r = 0; r < kTileRows; r++) {
     const unsigned long long m = __ballot((rowbits >> r) & 1u);
-    if (myrow == r) cand = m;
+    if ((rowbits >> r) & 1u) S.job[njobs + __popcll(m & below)] = (uint16_t)((r << 8) | lane);
+    njobs += __popcll(m);
+  }
+  S.best[lane] = 0x7F800000u;  // +inf
+  __syncthreads();
+  // this lane's x and the rows' y and z, relative to the tile origin (exact
+  // small multiples of h), as the row test formed them
+  const float px = (float)(lattice_coord(g.ox, g.h, i0 + (lane & (kTileX - 1))) - T0.x);
+  const float py0 = (float)(lattice_coord(g.oy, g.h, j0) - T0.y);
+  const float py1 = (float)(lattice_coord(g.oy, g.h, j0 + 1) - T0.y);
+  const float py2 = (float)(lattice_coord(g.oy, g.h, j0 + 2) - T0.y);
+  const float py3 = (float)(lattice_coord(g.oy, g.h, j0 + 3) - T0.y);
+  const float pz0 = (float)(lattice_coord(g.oz, g.h, k0) - T0.z);
+  const float pz1 = (float)(lattice_coord(g.oz, g.h, k0 + 1) - T0.z);
+  // Phase 2: exact distances, job q = lane / 8 + 8 t for t = 0, 1, ...
+  for (int q = lane / kTileX; q < njobs; q += kTilePts / kTileX) {
+    const uint32_t e = S.job[q];
+    const int r = (int)(e >> 8), c = (int)(e & 63u);
+    const int ry = r & (kTileY - 1);
+    const float py = ry < 2 ? (ry == 0 ? py0 : py1) : (ry == 2 ? py2 : py3);
+    const float pz = r < kTileY ? pz0 : pz1;
+    const float d2 = tri_pre_dist2(px, py, pz, [&](int f) { return S.pre[f][c]; });
+    // squared distances are >= +0, so their bits order like the values
+    atomicMin(&S.best[r * kTileX + (lane & (kTileX - 1))], __float_as_uint(d2));
   }
   __syncthreads();
-  // this lane's point, relative to the tile origin (exact small multiples of h)
-  const int32_t i = i0 + (lane & (kTileX - 1));
-  const int32_t j = j0 + ((lane / kTileX) & (kTileY - 1));
-  const int32_t k = k0 + lane / (kTileX * kTileY);
-  const Vec3d p = lattice_point(g, i, j, k);
-  const float px = (float)(p.x - T0.x), py = (float)(p.y - T0.y), pz = (float)(p.z - T0.z);
-  // Phase 2: exact distances for this lane's candidates only.
-  float best = __builtin_huge_valf();
-  while (cand) {
-    const int c = __builtin_ctzll(cand);
-    cand &= cand - 1ull;
-    const float d2 = tri_pre_dist2(px, py, pz, [&](int f) { return s_pre[f][c]; });
-    best = fminf(best, d2);
-  }
+  const float best = __uint_as_float(S.best[lane]);
   float *slot = d2buf + first * kTilePts + lane;
   if (chunk_off[tile + 1] - first == 1)
     *slot = best;
   else
     atomicMin((unsigned int *)slot, __float_as_uint(best));
-  __syncthreads();  // s_pre is reused by the block's next item
+  __syncthreads();  // S is reused by the block's next item
 }
 
 // One work item per workgroup and round; a grid smaller than the item
 // count leaves wave slots and LDS free for kernels on other streams.
 __global__ __launch_bounds__(kTilePts) void dist_chunks_kernel(DistJob J) {
-  __shared__ float s_pre[kTriPreFloats + 6][kTilePts];
+  __shared__ DistShared S;
   const int64_t n = chunk_count(J.nchunks, J.nchunks_dev);
   for (int64_t it = xcd_remap(blockIdx.x, gridDim.x); it < n; it += gridDim.x)
-    dist_item(J, it, s_pre);
+    dist_item(J, it, S);
 }
 
 // Persistent work queue over the distance items of many jobs: each block
@@ -255,7 +274,7 @@ __global__ __launch_bounds__(kTilePts) void dist_chunks_kernel(DistJob J) {
 __global__ __launch_bounds__(kTilePts) void dist_queue_kernel(
     const DistJob *__restrict__ jobs, int njobs, const uint64_t *__restrict__ prefix,
     unsigned long long *__restrict__ counter, int grab) {
-  __shared__ float s_pre[kTriPreFloats + 6][kTilePts];
+  __shared__ DistShared S;
   __shared__ uint64_t s_pref[kMaxJobs + 1];
   __shared__ unsigned long long s_base;
   for (int i = threadIdx.x; i <= njobs; i += blockDim.x) s_pref[i] = prefix[i];
@@ -271,7 +290,7 @@ __global__ __launch_bounds__(kTilePts) void dist_queue_kernel(
     int j = 0;
     for (uint64_t u = base; u < end; u++) {
       while (j + 1 < njobs && s_pref[j + 1] <= u) j++;
-      dist_item(jobs[j], (int64_t)(u - s_pref[j]), s_pre);
+      dist_item(jobs[j], (int64_t)(u - s_pref[j]), S);
     }
   }
 }
@@ -488,28 +507,25 @@ struct FusedShared {
   uint32_t slot[4];          // first work item + 1 of each tile, 0: no distance work
 };
 
+// Output sets of the sign kernel (bits); an instantiation keeps only the
+// pointers of its set in registers.
+enum : int { kOutSdf = 1, kOutDist = 2, kOutSigns = 4, kOutConf = 8, kOutAll = 15 };
+
 // One sign brick (8 x 8 x 4 points) of job J for the 64-lane block; m is
-// the brick's position in the brick launch order.  Uniform per block
-// (contains barriers).
-__device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S) {
+// the brick's position in the brick launch order; C is the LDS copy of the
+// job's cold fields; outputs outside kOut are not written.  Uniform per
+// block (contains barriers).
+template <int kOut>
+__device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S,
+                                   const FusedCold &C) {
   const GridP &g = J.g;
   const TileDims &td = J.td;
   const uint64_t *__restrict__ chunk_off = J.chunk_off;
   const float *__restrict__ d2buf = J.d2buf;
-  const uint64_t *__restrict__ line_off = J.line_off;
-  const LineFast *__restrict__ fast = J.fast;
-  const uint32_t *__restrict__ meta = J.meta;
   const LineInline *__restrict__ inl = J.inl;
   const int64_t base1 = J.base1, base2 = J.base2;
   const int do_sign = J.do_sign;
   const FusedOut out = J.out;
-  Pending *__restrict__ pend = J.pend;
-  uint32_t *npend = J.npend;
-  const uint32_t cap = J.cap;
-  Pending *__restrict__ pend0 = J.pend0;
-  uint32_t *npend0 = J.npend0;
-  const uint32_t cap0 = J.cap0;
-  uint32_t *err = J.err;
   const int64_t chunk_cap = J.chunk_cap;
   int32_t bx, by, bz;
   if (!brick_tile(m, J.bd, &bx, &by, &bz)) return;  // uniform
@@ -605,11 +621,11 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S)
       // crossing list
       if (!inline_parity<kSignZ>(rz, zq, zpos, zneg, zok)) {
         const int64_t Lz = base2 + (int64_t)j * g.nx + i;
-        const uint64_t hb = line_off[Lz], he = line_off[Lz + 1];
-        const uint32_t mz = meta[Lz];
+        const uint64_t hb = C.line_off[Lz], he = C.line_off[Lz + 1];
+        const uint32_t mz = C.meta[Lz];
 #pragma unroll
         for (int q = 0; q < kSignZ; q++)
-          zok[q] = q < nk ? axis_pair_parity_fast(fast, hb, he, mz, zq[q], &zpos[q], &zneg[q])
+          zok[q] = q < nk ? axis_pair_parity_fast(C.fast, hb, he, mz, zq[q], &zpos[q], &zneg[q])
                           : true;
       }
     }
@@ -646,8 +662,8 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S)
             const LineInline &rx = S.lx[q][lane >> 3];
             if (!inline_parity<1>(rx, &x, &a, &bpar, &okq) || !okq) {
               const int64_t Lx = (int64_t)k * g.ny + j;
-              amb |= !axis_pair_parity_fast(fast, line_off[Lx], line_off[Lx + 1], meta[Lx], x,
-                                            &a, &bpar);
+              amb |= !axis_pair_parity_fast(C.fast, C.line_off[Lx], C.line_off[Lx + 1],
+                                            C.meta[Lx], x, &a, &bpar);
             }
           }
           inside += a + bpar;
@@ -658,8 +674,8 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S)
             const LineInline &ry = S.ly[q][lane & 7];
             if (!inline_parity<1>(ry, &y, &a, &bpar, &okq) || !okq) {
               const int64_t Ly = base1 + (int64_t)k * g.nx + i;
-              amb |= !axis_pair_parity_fast(fast, line_off[Ly], line_off[Ly + 1], meta[Ly], y,
-                                            &a, &bpar);
+              amb |= !axis_pair_parity_fast(C.fast, C.line_off[Ly], C.line_off[Ly + 1],
+                                            C.meta[Ly], y, &a, &bpar);
             }
           }
           inside += a + bpar;
@@ -674,20 +690,20 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S)
           pr.dist = dist;
           pr.inside = amb ? 0 : inside;
           pr.used = amb ? 0 : 6;
-          uint32_t *cnt = amb ? npend0 : npend;
-          Pending *list = amb ? pend0 : pend;
+          uint32_t *cnt = amb ? C.npend0 : C.npend;
+          Pending *list = amb ? C.pend0 : C.pend;
           const uint32_t slot = atomicAdd(cnt, 1u);
-          if (slot < (amb ? cap0 : cap))
+          if (slot < (amb ? C.cap0 : C.cap))
             list[slot] = pr;
           else
-            atomicOr(err, kErrPendingOverflow);
+            atomicOr(C.err, kErrPendingOverflow);
         }
       }
       const double sgn = inside == 6 ? 1.0 : -1.0;
-      if (out.dist) out.dist[idx] = dist;
-      if (out.sdf) out.sdf[idx] = dist * sgn;
-      if (out.signs) out.signs[idx] = sgn;
-      if (out.conf) out.conf[idx] = 1.0;
+      if ((kOut & kOutDist) && out.dist) out.dist[idx] = dist;
+      if ((kOut & kOutSdf) && out.sdf) out.sdf[idx] = dist * sgn;
+      if ((kOut & kOutSigns) && out.signs) out.signs[idx] = sgn;
+      if ((kOut & kOutConf) && out.conf) out.conf[idx] = 1.0;
     }
   }
   __syncthreads();  // S is reused by the block's next brick
@@ -695,11 +711,15 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S)
 
 // Six waves per SIMD: more would spill, fewer hide less of the kernel's
 // latency (measured 0.48 ms against 0.52 unconstrained and 0.61 at eight).
+template <int kOut>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6)))
 void fused_lattice_kernel(FusedJob J) {
   __shared__ FusedShared S;
+  __shared__ FusedCold C;
+  if (threadIdx.x == 0) C = J.cold;
+  __syncthreads();
   for (int64_t m = xcd_remap(blockIdx.x, gridDim.x); m < J.nbricks; m += gridDim.x)
-    fused_brick(J, m, S);
+    fused_brick<kOut>(J, m, S, C);
 }
 
 // Persistent work queue over the sign bricks of many jobs (static brick
@@ -708,6 +728,7 @@ __global__ __launch_bounds__(64) void fused_queue_kernel(
     const FusedJob *__restrict__ jobs, int njobs, const uint64_t *__restrict__ prefix,
     unsigned long long *__restrict__ counter, int grab) {
   __shared__ FusedShared S;
+  __shared__ FusedCold C;
   __shared__ uint64_t s_pref[kMaxJobs + 1];
   __shared__ unsigned long long s_base;
   for (int i = threadIdx.x; i <= njobs; i += blockDim.x) s_pref[i] = prefix[i];
@@ -723,7 +744,9 @@ __global__ __launch_bounds__(64) void fused_queue_kernel(
     int j = 0;
     for (uint64_t u = base; u < end; u++) {
       while (j + 1 < njobs && s_pref[j + 1] <= u) j++;
-      fused_brick(jobs[j], (int64_t)(u - s_pref[j]), S);
+      if (threadIdx.x == 0) C = jobs[j].cold;
+      __syncthreads();
+      fused_brick<kOutAll>(jobs[j], (int64_t)(u - s_pref[j]), S, C);
     }
   }
 }
@@ -1236,24 +1259,24 @@ FusedJob make_fused_job(const GridP &g, const Bins &b, const float *d2buf, bool 
   J.bd.ntiles = (int64_t)sd.sx * sd.sy * sd.sz;
   J.chunk_off = b.chunk_off;
   J.d2buf = d2buf;
-  J.line_off = b.line_off;
-  J.fast = b.line_fast;
-  J.meta = b.line_meta;
   J.inl = b.line_inline;
   J.base1 = b.line_base[1];
   J.base2 = b.line_base[2];
   J.out = out;
-  J.pend = pend;
-  J.npend = npend;
-  J.pend0 = pend0;
-  J.npend0 = npend0;
-  J.err = err;
   J.chunk_cap = b.nchunks;
   J.nbricks = J.td.ntiles > 0 ? brick_launch_size(J.bd) : 0;
-  J.cap = pend_cap;
-  J.cap0 = pend0_cap;
   J.do_sign = do_sign ? 1 : 0;
   J.pad_ = 0;
+  J.cold.line_off = b.line_off;
+  J.cold.fast = b.line_fast;
+  J.cold.meta = b.line_meta;
+  J.cold.pend = pend;
+  J.cold.npend = npend;
+  J.cold.pend0 = pend0;
+  J.cold.npend0 = npend0;
+  J.cold.err = err;
+  J.cold.cap = pend_cap;
+  J.cold.cap0 = pend0_cap;
   return J;
 }
 
@@ -1267,7 +1290,17 @@ void launch_fused_lattice(hipStream_t s, const GridP &g, const Tri64 *tri,
                                     npend0, pend0_cap, err);
   if (J.nbricks <= 0) return;
   const int64_t grid = max_blocks > 0 && max_blocks < J.nbricks ? max_blocks : J.nbricks;
-  hipLaunchKernelGGL(fused_lattice_kernel, dim3((unsigned)grid), dim3(64), 0, s, J);
+  const int set = (out.sdf ? kOutSdf : 0) | (out.dist ? kOutDist : 0) |
+                  (out.signs ? kOutSigns : 0) | (out.conf ? kOutConf : 0);
+  if (set == kOutSdf)
+    hipLaunchKernelGGL(fused_lattice_kernel<kOutSdf>, dim3((unsigned)grid), dim3(64), 0, s, J);
+  else if (set == kOutDist)
+    hipLaunchKernelGGL(fused_lattice_kernel<kOutDist>, dim3((unsigned)grid), dim3(64), 0, s, J);
+  else if (set == (kOutSigns | kOutConf))
+    hipLaunchKernelGGL(fused_lattice_kernel<kOutSigns | kOutConf>, dim3((unsigned)grid),
+                       dim3(64), 0, s, J);
+  else
+    hipLaunchKernelGGL(fused_lattice_kernel<kOutAll>, dim3((unsigned)grid), dim3(64), 0, s, J);
   SDF_HIP(hipGetLastError());
 }
 
