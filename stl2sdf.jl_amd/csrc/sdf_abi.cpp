@@ -129,6 +129,8 @@ struct sdf_ctx {
   int chunk_len = 64;
   int ray_stack = 1024;
   int dist_per_cu = 8, fused_per_cu = 0;
+  int lines_early = 0;  // 1: the line chain starts with the tile chain
+  int line_per_cu = 0;  // workgroups per CU of the line kernels (0: no cap)
   int ncu = 256;
   // timing
   bool timing = false;
@@ -357,6 +359,17 @@ void stage_bins_tiles(sdf_ctx *c, LatticeRun &L) {
     fail(SDF_ERR_INTERNAL, "capture needs known capacities");
   SDF_HIP(hipEventRecord(c->join[0], c->stream));
   L.bvh_was_ready = c->bvh_ready;
+  auto line_chain = [&](hipEvent_t after) {
+    SDF_HIP(hipStreamWaitEvent(c->aux[0], after, 0));
+    sdf::LineBuild lb;
+    lb.max_blocks = (int64_t)c->line_per_cu * c->ncu;
+    c->ar.scope = "line/";
+    sdf::line_hits_begin(c->aux[0], c->ar, c->tri, c->nel, g, lb, bins);
+    sdf::line_hits_finish(c->aux[0], c->ar, c->tri, c->nel, g, lb, bins, c->caps.hits);
+    c->ar.scope.clear();
+    SDF_HIP(hipEventRecord(c->join[1], c->aux[0]));
+  };
+  if (L.do_sign && c->lines_early) line_chain(c->join[0]);
   c->mark("bin_tiles");
   L.d2buf = nullptr;
   if (L.do_dist) {
@@ -373,15 +386,7 @@ void stage_bins_tiles(sdf_ctx *c, LatticeRun &L) {
   // beside the latency-bound tile chain they stretch every small kernel
   // (measured: about 35 us per step)
   SDF_HIP(hipEventRecord(c->join[3], c->stream));
-  if (L.do_sign) {
-    SDF_HIP(hipStreamWaitEvent(c->aux[0], c->join[3], 0));
-    sdf::LineBuild lb;
-    c->ar.scope = "line/";
-    sdf::line_hits_begin(c->aux[0], c->ar, c->tri, c->nel, g, lb, bins);
-    sdf::line_hits_finish(c->aux[0], c->ar, c->tri, c->nel, g, lb, bins, c->caps.hits);
-    c->ar.scope.clear();
-    SDF_HIP(hipEventRecord(c->join[1], c->aux[0]));
-  }
+  if (L.do_sign && !c->lines_early) line_chain(c->join[3]);
   L.cap = c->caps.pend ? c->caps.pend
                        : (uint32_t)(npts < (1 << 22) ? (npts > 0 ? npts : 1) : (1 << 22));
   L.cap0 = c->caps.pend0 ? c->caps.pend0
@@ -1061,9 +1066,14 @@ sdf_status sdf_ctx_tune(sdf_ctx *c, const char *name, int64_t value) {
     } else if (n == "ray_stack") {
       if (value != 1024 && value != 2048) fail(SDF_ERR_ARG, "ray_stack must be 1024 or 2048");
       c->ray_stack = (int)value;
-    } else if (n == "dist_per_cu" || n == "fused_per_cu") {
+    } else if (n == "lines_early") {
+      if (value != 0 && value != 1) fail(SDF_ERR_ARG, "lines_early must be 0 or 1");
+      c->lines_early = (int)value;
+    } else if (n == "dist_per_cu" || n == "fused_per_cu" || n == "line_per_cu") {
       if (value < 0 || value > 64) fail(SDF_ERR_ARG, n + " must be in 0..64");
-      (n == "dist_per_cu" ? c->dist_per_cu : c->fused_per_cu) = (int)value;
+      (n == "dist_per_cu" ? c->dist_per_cu
+                          : (n == "fused_per_cu" ? c->fused_per_cu : c->line_per_cu)) =
+          (int)value;
     } else {
       fail(SDF_ERR_ARG, "unknown tunable " + n);
     }

@@ -50,6 +50,13 @@ inline int grid_for(int64_t n, int threads) {
   return (int)b;
 }
 
+// grid_for, at most max_blocks workgroups when max_blocks > 0 (grid-stride
+// kernels only).
+inline int capped_grid(int64_t n, int threads, int64_t max_blocks) {
+  const int b = grid_for(n, threads);
+  return max_blocks > 0 && max_blocks < b ? (int)max_blocks : b;
+}
+
 // One thread per triangle: 1-based IEN -> 9 doubles.  Out-of-range indices
 // raise kErrBadIndex and read vertex 1 instead (never out of bounds).
 __global__ void gather_tris_kernel(const double *__restrict__ X, int64_t nnp,
@@ -564,8 +571,8 @@ __global__ void line_prep_kernel(GridP g, const uint64_t *__restrict__ line_off,
     }
     const double len = on[ax] - o0[ax];
     const int j = (ax + 1) % 3, k = (ax + 2) % 3;
-    double dmax = 0.0;
-    for (int64_t q = 0; q < n; q++) {
+    // crossing coordinate and certified bound of hit q
+    auto summarize = [&](int64_t q, double *c_out, float *delta_out) {
       const LineHit r = hits[hb + q];
       const double t0 = line_hit_t(r, o0);
       const double c = o0[ax] + t0;
@@ -584,44 +591,101 @@ __global__ void line_prep_kernel(GridP g, const uint64_t *__restrict__ line_off,
       double delta = 2.0 * (beta * len + 2.0 * E + u * (fabs(c) + fabs(t0) + 2.0 * len)) +
                      1e-300;
       delta *= 1.0 + 1e-6;
-      LineFast lf;
-      lf.c = c;
-      lf.delta = __double2float_ru(delta);
-      lf.cid = 0;
-      lf.pad_ = 0;
-      // insertion into the sorted prefix (global memory, small n)
-      int64_t p = q;
-      while (p > 0 && fast[hb + p - 1].c > c) {
-        fast[hb + p] = fast[hb + p - 1];
-        p--;
-      }
-      fast[hb + p] = lf;
-      dmax = fmax(dmax, (double)lf.delta);
-    }
+      *c_out = c;
+      *delta_out = __double2float_ru(delta);
+    };
+    // clusters of the sorted hits: a gap beyond eps by more than the bounds
+    // starts a new cluster; a gap within the bounds of eps has no fast path
+    auto gap_class = [&](double gap, double dmax) -> int {
+      const double m = 2.0 * dmax + 4.0 * u * fabs(gap) + 1e-300;
+      if (gap - kRayEps > m) return 1;
+      return (gap - kRayEps < -m) ? 0 : -1;
+    };
     uint32_t cl = 0;
     bool ok = true;
-    double prev = fast[hb].c;
-    for (int64_t q = 1; q < n && ok; q++) {
-      const double c = fast[hb + q].c;
-      const double gap = c - prev;
-      const double m = 2.0 * dmax + 4.0 * u * fabs(gap) + 1e-300;
-      if (gap - kRayEps > m)
-        cl++;
-      else if (!(gap - kRayEps < -m))
-        ok = false;
-      fast[hb + q].cid = (uint16_t)cl;
-      prev = c;
+    if (n <= kInlineHits) {
+      // short line: sorted in registers (stable bubble passes), no global
+      // round trips
+      double cs[kInlineHits];
+      float ds[kInlineHits];
+#pragma unroll
+      for (int q = 0; q < kInlineHits; q++) {
+        if (q < n) {
+          summarize(q, &cs[q], &ds[q]);
+        } else {
+          cs[q] = __builtin_huge_val();
+          ds[q] = 0.f;
+        }
+      }
+#pragma unroll
+      for (int pass = kInlineHits - 1; pass > 0; pass--)
+#pragma unroll
+        for (int q = 0; q < pass; q++)
+          if (cs[q] > cs[q + 1]) {
+            const double tc = cs[q];
+            cs[q] = cs[q + 1];
+            cs[q + 1] = tc;
+            const float td = ds[q];
+            ds[q] = ds[q + 1];
+            ds[q + 1] = td;
+          }
+      double dmax = 0.0;
+#pragma unroll
+      for (int q = 0; q < kInlineHits; q++) dmax = fmax(dmax, (double)ds[q]);
+      uint8_t cid[kInlineHits] = {0, 0, 0, 0};
+#pragma unroll
+      for (int q = 1; q < kInlineHits; q++) {
+        if (q < n && ok) {
+          const int gc = gap_class(cs[q] - cs[q - 1], dmax);
+          if (gc < 0) ok = false;
+          cl += gc > 0 ? 1u : 0u;
+          cid[q] = (uint8_t)cl;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < kInlineHits; q++) {
+        if (q < n) {
+          LineFast lf;
+          lf.c = cs[q];
+          lf.delta = ds[q];
+          lf.cid = cid[q];
+          lf.pad_ = 0;
+          fast[hb + q] = lf;
+          if (ok) {
+            li.c[q] = cs[q];
+            li.delta[q] = ds[q];
+            li.cid[q] = cid[q];
+          }
+        }
+      }
+    } else {
+      double dmax = 0.0;
+      for (int64_t q = 0; q < n; q++) {
+        LineFast lf;
+        summarize(q, &lf.c, &lf.delta);
+        lf.cid = 0;
+        lf.pad_ = 0;
+        // insertion into the sorted prefix (global memory)
+        int64_t p = q;
+        while (p > 0 && fast[hb + p - 1].c > lf.c) {
+          fast[hb + p] = fast[hb + p - 1];
+          p--;
+        }
+        fast[hb + p] = lf;
+        dmax = fmax(dmax, (double)lf.delta);
+      }
+      double prev = fast[hb].c;
+      for (int64_t q = 1; q < n && ok; q++) {
+        const double c = fast[hb + q].c;
+        const int gc = gap_class(c - prev, dmax);
+        if (gc < 0) ok = false;
+        cl += gc > 0 ? 1u : 0u;
+        fast[hb + q].cid = (uint16_t)cl;
+        prev = c;
+      }
     }
     meta[L] = ok ? (cl + 1u) : kLineExact;
     li.info = (uint32_t)(n < 255 ? n : 255) | ((cl + 1u) & 0xFFu) << 8 | (ok ? 0u : kLineExact);
-    if (n <= kInlineHits && ok) {
-      for (int64_t q = 0; q < n; q++) {
-        const LineFast f = fast[hb + q];
-        li.c[q] = f.c;
-        li.delta[q] = f.delta;
-        li.cid[q] = (uint8_t)f.cid;
-      }
-    }
     inl[L] = li;
   }
 }
@@ -777,7 +841,8 @@ void line_hits_begin(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
   launch_fill_u32(s, lb.lcnt, lb.nlines + 1, 0u);
   launch_fill_u32(s, lb.nbig, 1, 0u);
   if (nel > 0) {
-    hipLaunchKernelGGL(line_bin_small_kernel, dim3(grid_for(nel, 256)), dim3(256), 0, s,
+    hipLaunchKernelGGL(line_bin_small_kernel, dim3(capped_grid(nel, 256, lb.max_blocks)),
+                       dim3(256), 0, s,
                        tri, nel, g, lb.nlx, lb.nlx + lb.nly, 0, lb.lcnt,
                        (const uint64_t *)nullptr, (LineHit *)nullptr, (uint64_t)0, lb.big,
                        lb.nbig);
@@ -797,7 +862,8 @@ void line_hits_finish(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
   LineHit *hits = ar.get<LineHit>("line_hits", cap + 1);
   if (cap > 0 && nel > 0) {
     launch_fill_u32(s, lb.lcnt, lb.nlines + 1, 0u);  // reused as the cursor
-    hipLaunchKernelGGL(line_bin_small_kernel, dim3(grid_for(nel, 256)), dim3(256), 0, s,
+    hipLaunchKernelGGL(line_bin_small_kernel, dim3(capped_grid(nel, 256, lb.max_blocks)),
+                       dim3(256), 0, s,
                        tri, nel, g, lb.nlx, lb.nlx + lb.nly, 1, lb.lcnt,
                        (const uint64_t *)lb.loff, hits, (uint64_t)cap, lb.big, lb.nbig);
     hipLaunchKernelGGL(line_bin_big_kernel, dim3(1024), dim3(256), 0, s, tri, g, lb.nlx,
@@ -808,7 +874,8 @@ void line_hits_finish(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
   LineFast *fast = ar.get<LineFast>("line_fast", cap + 1);
   uint32_t *meta = ar.get<uint32_t>("line_meta", lb.nlines + 1);
   LineInline *inl = ar.get<LineInline>("line_inline", lb.nlines + 1);
-  hipLaunchKernelGGL(line_prep_kernel, dim3(grid_for(lb.nlines, 256)), dim3(256), 0, s, g,
+  hipLaunchKernelGGL(line_prep_kernel, dim3(capped_grid(lb.nlines, 256, lb.max_blocks)),
+                     dim3(256), 0, s, g,
                      (const uint64_t *)lb.loff, (const LineHit *)hits, (uint64_t)cap, lb.nlx,
                      lb.nlx + lb.nly, lb.nlines, fast, meta, inl);
   SDF_HIP(hipGetLastError());

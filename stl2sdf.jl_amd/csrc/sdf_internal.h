@@ -220,6 +220,7 @@ struct LineBuild {
   int64_t nlx = 0, nly = 0, nlines = 0;
   uint32_t *lcnt = nullptr, *big = nullptr, *nbig = nullptr;
   uint64_t *loff = nullptr;
+  int64_t max_blocks = 0;  // grid cap of the per-triangle and per-line kernels (0: none)
 };
 void tile_bins_begin(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
                      const GridP &g, double radius, TileBuild &tb);

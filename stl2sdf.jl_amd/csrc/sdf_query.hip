@@ -1125,21 +1125,30 @@ __global__ __launch_bounds__(256) void winding_partial_kernel(
   }
 }
 
-__global__ void winding_final_kernel(PendLists lists, const uint64_t *__restrict__ wind,
-                                     uint32_t nwind, const uint32_t *__restrict__ nwind_dev,
-                                     int chunks, const double *__restrict__ partial,
-                                     FusedOut out) {
+// One wave per point: lanes add the chunk sums at stride 64 in chunk order,
+// then a fixed butterfly; every lane holds the same total.
+__global__ __launch_bounds__(64) void winding_final_kernel(PendLists lists,
+                                                           const uint64_t *__restrict__ wind,
+                                                           uint32_t nwind,
+                                                           const uint32_t *__restrict__ nwind_dev,
+                                                           int chunks,
+                                                           const double *__restrict__ partial,
+                                                           FusedOut out) {
+#pragma clang fp contract(off)
   if (nwind_dev && *nwind_dev < nwind) nwind = *nwind_dev;
-  for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < nwind;
-       w += gridDim.x * blockDim.x) {
+  for (uint32_t w = blockIdx.x; w < nwind; w += gridDim.x) {
     double tot = 0.0;
-    for (int c = 0; c < chunks; c++) tot += partial[(int64_t)w * chunks + c];
-    tot = tot / 12.566370614359172;  // Float64(4pi)
-    const double sgn = fabs(tot) > 0.5 ? 1.0 : -1.0;
-    const Pending P = slot_item(lists, wind[w]);
-    if (out.sdf) out.sdf[P.idx] = P.dist * sgn;
-    if (out.signs) out.signs[P.idx] = sgn;
-    if (out.conf) out.conf[P.idx] = 0.8;
+    for (int c = threadIdx.x; c < chunks; c += 64) tot += partial[(int64_t)w * chunks + c];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
+    if (threadIdx.x == 0) {
+      tot = tot / 12.566370614359172;  // Float64(4pi)
+      const double sgn = fabs(tot) > 0.5 ? 1.0 : -1.0;
+      const Pending P = slot_item(lists, wind[w]);
+      if (out.sdf) out.sdf[P.idx] = P.dist * sgn;
+      if (out.signs) out.signs[P.idx] = sgn;
+      if (out.conf) out.conf[P.idx] = 0.8;
+    }
   }
 }
 
@@ -1157,8 +1166,10 @@ __global__ void dist_points_bvh_kernel(const double *__restrict__ pts,
   }
 }
 
-// Clears the ray bits of the live slots of both lists.
-__global__ void fill_bits_kernel(unsigned long long *p, PendLists lists) {
+// Clears the ray bits of the live slots of both lists and the phase's four
+// counters.
+__global__ void fill_bits_kernel(unsigned long long *p, PendLists lists, uint32_t *counts) {
+  if (blockIdx.x == 0 && threadIdx.x < 4) counts[threadIdx.x] = 0u;
   for (int k = 0; k < lists.count; k++) {
     const uint32_t n = list_count(lists, k);
     unsigned long long *q = p + (k ? lists.cap[0] : 0u);
@@ -1332,9 +1343,8 @@ void resolve_pending(hipStream_t s, Arena &ar, const GridP &g, const double *pts
   const uint64_t slots = lists.slots();
   if (slots == 0) return;
   if (r1 > kNumDirs) r1 = kNumDirs;
-  launch_fill_u32(s, d_counts, 4, 0u);
   unsigned long long *bits = ar.get<unsigned long long>(ar.tmp("res_bits"), slots);
-  hipLaunchKernelGGL(fill_bits_kernel, dim3(128), dim3(256), 0, s, bits, lists);
+  hipLaunchKernelGGL(fill_bits_kernel, dim3(128), dim3(256), 0, s, bits, lists, d_counts);
   const uint64_t fb_want = slots * 4u + 1024u;
   const uint32_t fb_cap = fb_want < 0xFFFFFFF0ull ? (uint32_t)fb_want : 0xFFFFFFF0u;
   uint64_t *fb = ar.get<uint64_t>(ar.tmp("res_fb"), fb_cap);
@@ -1373,7 +1383,8 @@ void resolve_pending(hipStream_t s, Arena &ar, const GridP &g, const double *pts
     hipLaunchKernelGGL(winding_partial_kernel, dim3(1024), dim3(256), 0, s, g, pts, tri, nel,
                        lists, (const uint64_t *)wind, wind_cap,
                        (const uint32_t *)(d_counts + 2), chunks, chunk, partial);
-    hipLaunchKernelGGL(winding_final_kernel, dim3(16), dim3(256), 0, s, lists,
+    hipLaunchKernelGGL(winding_final_kernel, dim3(wind_cap < 1024u ? wind_cap : 1024u), dim3(64),
+                       0, s, lists,
                        (const uint64_t *)wind, wind_cap, (const uint32_t *)(d_counts + 2),
                        chunks, (const double *)partial, out);
     SDF_HIP(hipGetLastError());
