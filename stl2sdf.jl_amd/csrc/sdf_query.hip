@@ -103,28 +103,27 @@ __device__ inline bool inline_parity(const LineInline &r, const double *xs,
 // usable); the caller then classifies each point.
 __device__ inline bool inline_uniform(const LineInline &r, double lo, double hi, int *pos,
                                       int *neg) {
+  // selects, not branches: the per-hit cases would otherwise cost scalar
+  // exec-mask work on every call
   const uint32_t n = r.info & 0xFFu;
-  if ((r.info & kLineExact) || n > (uint32_t)kInlineHits) return false;
+  const bool usable = !(r.info & kLineExact) && n <= (uint32_t)kInlineHits;
   const int ncl = (int)((r.info >> 8) & 0xFFu);
   int first_pos = -1, last_neg = -1;
   bool uni = true;
 #pragma unroll
   for (int h2 = 0; h2 < kInlineHits; h2++) {
-    if (h2 < (int)n) {
-      const double D = (double)r.delta[h2];
-      const int cid = r.cid[h2];
-      if (r.c[h2] - hi > kRayEps + D) {
-        if (first_pos < 0) first_pos = cid;
-      } else if (r.c[h2] - lo < -kRayEps - D) {
-        last_neg = cid;
-      } else {
-        uni = false;
-      }
-    }
+    const bool valid = h2 < (int)n;
+    const double D = (double)r.delta[h2];
+    const int cid = r.cid[h2];
+    const bool above = r.c[h2] - hi > kRayEps + D;
+    const bool below = !above & (r.c[h2] - lo < -kRayEps - D);
+    first_pos = (valid & above & (first_pos < 0)) ? cid : first_pos;
+    last_neg = (valid & below) ? cid : last_neg;
+    uni = uni & (!valid | above | below);
   }
   *pos = first_pos >= 0 ? ((ncl - first_pos) & 1) : 0;
   *neg = last_neg >= 0 ? ((last_neg + 1) & 1) : 0;
-  return uni;
+  return usable & uni;
 }
 
 // LDS of one distance workgroup: the chunk's triangle records, the
@@ -543,30 +542,24 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S,
   int zp = 0, zn = 0;
   if (do_sign) {
     if (in_grid) rz = inl[base2 + (int64_t)j * g.nx + i];
+    // lanes 0-31 stage the x-lines (k0 + q, 8 by + w), lanes 32-63 the
+    // y-lines (k0 + q, 8 bx + w): one code path, operands by select
     const int q = (lane >> 3) & 3, w = lane & 7, kk = k0 + q;
+    const bool xl = lane < 32;
+    const int32_t a = (xl ? by : bx) * 8 + w;  // the line's y (x-lines) or x (y-lines)
+    const int32_t b0 = (xl ? bx : by) * 8;     // first brick coordinate along the line
+    const int32_t along = xl ? g.nx : g.ny, across = xl ? g.ny : g.nx;
+    const bool valid = kk < g.nz && a < across;
+    const int64_t L = (xl ? 0 : base1) + (int64_t)kk * across + a;
+    const LineInline r = inl[valid ? L : 0];
+    LineInline *dst = xl ? &S.lx[q][w] : &S.ly[q][w];
+    *dst = r;
+    const double o = xl ? g.ox : g.oy;
     int p, n;
-    uint8_t u = 0;
-    if (lane < 32) {
-      const int32_t jj = by * 8 + w;
-      if (kk < g.nz && jj < g.ny) {
-        const LineInline r = inl[(int64_t)kk * g.ny + jj];
-        S.lx[q][w] = r;
-        if (inline_uniform(r, lattice_coord(g.ox, g.h, bx * 8),
-                           lattice_coord(g.ox, g.h, min(bx * 8 + 7, g.nx - 1)), &p, &n))
-          u = (uint8_t)(1 | (p << 1) | (n << 2));
-      }
-      S.ux[q][w] = u;
-    } else {
-      const int32_t ii = bx * 8 + w;
-      if (kk < g.nz && ii < g.nx) {
-        const LineInline r = inl[base1 + (int64_t)kk * g.nx + ii];
-        S.ly[q][w] = r;
-        if (inline_uniform(r, lattice_coord(g.oy, g.h, by * 8),
-                           lattice_coord(g.oy, g.h, min(by * 8 + 7, g.ny - 1)), &p, &n))
-          u = (uint8_t)(1 | (p << 1) | (n << 2));
-      }
-      S.uy[q][w] = u;
-    }
+    const bool uni = inline_uniform(r, lattice_coord(o, g.h, b0),
+                                    lattice_coord(o, g.h, min(b0 + 7, along - 1)), &p, &n);
+    const uint8_t u = valid && uni ? (uint8_t)(1 | (p << 1) | (n << 2)) : (uint8_t)0;
+    *(xl ? &S.ux[q][w] : &S.uy[q][w]) = u;
     if (in_grid)
       zuni = inline_uniform(rz, lattice_coord(g.oz, g.h, k0),
                             lattice_coord(g.oz, g.h, k0 + nk - 1), &zp, &zn);
