@@ -540,6 +540,8 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S,
   LineInline rz;
   bool zuni = false;  // the lane's z-line is the same for its nk points
   int zp = 0, zn = 0;
+  // every line this lane staged or owns is shared by the brick's points
+  bool lines_shared = true;
   if (do_sign) {
     if (in_grid) rz = inl[base2 + (int64_t)j * g.nx + i];
     // lanes 0-31 stage the x-lines (k0 + q, 8 by + w), lanes 32-63 the
@@ -563,7 +565,12 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S,
     if (in_grid)
       zuni = inline_uniform(rz, lattice_coord(g.oz, g.h, k0),
                             lattice_coord(g.oz, g.h, k0 + nk - 1), &zp, &zn);
+    // a staged line outside the grid (or a layer beyond nz) serves no point
+    lines_shared = (!valid | uni) & (!in_grid | zuni);
   }
+  // wave-uniform shortcut: every line of the brick is shared by its points
+  // (no crossing near the brick), so each point only adds six stored parities
+  const bool brick_uni = __all(lines_shared);
   if (d2buf) {
     if (lane < 4) {
       const int32_t tyy = 2 * by + (lane & 1), tzz = 2 * bz + (lane >> 1);
@@ -586,16 +593,6 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S,
   }
   __syncthreads();
 
-  // wave-uniform shortcut: every line of the brick is shared by its points
-  // (no crossing near the brick), so each point only adds six stored parities
-  bool lane_uni = true;
-  if (do_sign && in_grid) {
-    lane_uni = zuni;
-#pragma unroll
-    for (int q = 0; q < kSignZ; q++)
-      if (q < nk) lane_uni = lane_uni && ((S.ux[q][lane >> 3] & S.uy[q][lane & 7] & 1u) != 0u);
-  }
-  const bool brick_uni = __all(lane_uni);
   if (in_grid) {
     const double x = lattice_coord(g.ox, g.h, i), y = lattice_coord(g.oy, g.h, j);
     double zq[kSignZ];
@@ -673,7 +670,7 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S,
           }
           inside += a + bpar;
         }
-        if (amb || (inside != 0 && inside != 6)) {
+        if (amb | ((inside != 0) & (inside != 6))) {
           // amb: a decision sat inside its rounding margin, so the point
           // restarts the vote from ray 1 with the reference's arithmetic;
           // otherwise the six votes disagree.  The vote stage writes the
@@ -702,10 +699,10 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S,
   __syncthreads();  // S is reused by the block's next brick
 }
 
-// Six waves per SIMD: more would spill, fewer hide less of the kernel's
-// latency (measured 0.48 ms against 0.52 unconstrained and 0.61 at eight).
+// Five waves per SIMD (92 registers, no spills to scratch): at six the
+// allocator spills to scratch and the kernel runs 4 % slower.
 template <int kOut>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6)))
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5)))
 void fused_lattice_kernel(FusedJob J) {
   __shared__ FusedShared S;
   __shared__ FusedCold C;
