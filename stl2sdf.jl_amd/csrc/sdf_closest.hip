@@ -203,13 +203,13 @@ __global__ __launch_bounds__(64) void closest_points_kernel(
         int32_t next = -1;
         if (__ballot(f0 <= lim2)) {
           if (c0 < 0)
-            leaf(bvh.leaf_tri[-(c0 + 1)]);
+            leaf((uint32_t)(-(c0 + 1)));
           else
             next = c0;
         }
         if (__ballot(f1 <= lim2)) {  // lim2 may have shrunk in the first leaf
           if (c1 < 0) {
-            leaf(bvh.leaf_tri[-(c1 + 1)]);
+            leaf((uint32_t)(-(c1 + 1)));
           } else if (next < 0) {
             next = c1;
           } else {

@@ -6,8 +6,8 @@
 namespace sdf {
 
 // Internal node: both children's boxes (lo xyz, hi xyz) so one 64-byte
-// fetch tests two children.  child >= 0: internal node; child < 0: leaf
-// -(child+1) in sorted order.
+// fetch tests two children.  child >= 0: internal node; child < 0: the
+// leaf of triangle -(child+1).
 struct BvhNode {
   float box[2][6];
   int32_t child[2];
@@ -119,13 +119,13 @@ __device__ inline double nearest_bvh(const Bvh &bvh, const Tri64 *__restrict__ t
       int32_t next = -0x7FFFFFFF;
       if (d0 <= lim2) {
         if (c0 < 0)
-          leaf(bvh.leaf_tri[-(c0 + 1)]);
+          leaf((uint32_t)(-(c0 + 1)));
         else
           next = c0;
       }
       if (d1 <= lim2) {
         if (c1 < 0) {
-          leaf(bvh.leaf_tri[-(c1 + 1)]);
+          leaf((uint32_t)(-(c1 + 1)));
         } else if (next == -0x7FFFFFFF) {
           next = c1;
         } else {
@@ -184,13 +184,13 @@ __device__ inline float nearest_bvh_with(const Bvh &bvh, const float *pf, float 
     int32_t next = -0x7FFFFFFF;
     if (d0 <= lim2) {
       if (c0 < 0)
-        best2 = fminf(best2, leaf(bvh.leaf_tri[-(c0 + 1)]));
+        best2 = fminf(best2, leaf((uint32_t)(-(c0 + 1))));
       else
         next = c0;
     }
     if (d1 <= lim2) {
       if (c1 < 0) {
-        best2 = fminf(best2, leaf(bvh.leaf_tri[-(c1 + 1)]));
+        best2 = fminf(best2, leaf((uint32_t)(-(c1 + 1))));
       } else if (next == -0x7FFFFFFF) {
         next = c1;
       } else {

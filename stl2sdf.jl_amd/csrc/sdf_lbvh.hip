@@ -403,8 +403,11 @@ __device__ inline void seg_query(const float *__restrict__ seg, const SegLevels 
   }
 }
 
+// Child boxes of every internal node; a leaf child's reference changes from
+// its sorted position to its triangle id (traversals skip the leaf_tri load).
 __global__ void node_boxes_kernel(const float *__restrict__ seg, SegLevels L,
                                   const int32_t *__restrict__ range, int64_t n,
+                                  const uint32_t *__restrict__ leaf_tri,
                                   BvhNode *__restrict__ nodes) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n - 1;
        i += (int64_t)gridDim.x * blockDim.x) {
@@ -414,6 +417,7 @@ __global__ void node_boxes_kernel(const float *__restrict__ seg, SegLevels L,
       if (ch < 0) {
         const int64_t leaf = -(int64_t)(ch + 1);
         for (int q = 0; q < 6; q++) b[q] = seg[6 * leaf + q];
+        nodes[i].child[c] = -(int32_t)leaf_tri[leaf] - 1;
       } else {
         seg_query(seg, L, range[2 * ch], range[2 * ch + 1], b);
       }
@@ -528,7 +532,7 @@ Bvh build_lbvh(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t n) {
   }
 
   hipLaunchKernelGGL(node_boxes_kernel, dim3(blocks_for(n - 1, 256)), dim3(256),
-                     0, s, seg, L, range, n, b.nodes);
+                     0, s, seg, L, range, n, (const uint32_t *)leaf_tri, b.nodes);
   SDF_HIP(hipGetLastError());
   return b;
 }

@@ -912,7 +912,7 @@ __global__ __launch_bounds__(64) void ray_wave_kernel(
           if (!ray_box(o, d, inv, nd.box[c])) continue;
           const int32_t ch = nd.child[c];
           if (ch < 0) {
-            const Tri64 q = tri[bvh.leaf_tri[-(ch + 1)]];
+            const Tri64 q = tri[-(ch + 1)];
             double tv;
             if (mt_ref(o, d, q.v, q.v + 3, q.v + 6, &tv)) {
               const int slot = atomicAdd(&s_nh, 1);
