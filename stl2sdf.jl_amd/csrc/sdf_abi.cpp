@@ -346,9 +346,13 @@ enum : int {
 // main stream; the axis-line hits (needed by the sign stage) on helper
 // stream 0 and the LBVH (needed only by the vote stage) on helper stream 1,
 // both from the end of the tile chain, beside the distance and sign
-// kernels.  Issue order: the tile chain, the line chain (stage_bins_tiles),
-// the distance and sign kernels (stage_dist, stage_sign), then the LBVH
-// (stage_bins_bvh); the host stays ahead of the GPU throughout.
+// kernels.  Issue order: the tile chain (stage_bins_tiles), the distance
+// kernel (stage_dist), the line chain (stage_bins_lines), the LBVH
+// (stage_bins_bvh), then the sign kernel: the distance kernel is queued
+// before the host spends its time issuing the helper chains, so the main
+// stream does not idle after the tile chain.
+void stage_bins_lines(sdf_ctx *c, LatticeRun &L);
+
 void stage_bins_tiles(sdf_ctx *c, LatticeRun &L) {
   const sdf::GridP &g = L.g;
   const int64_t npts = grid_points(g);
@@ -359,17 +363,7 @@ void stage_bins_tiles(sdf_ctx *c, LatticeRun &L) {
     fail(SDF_ERR_INTERNAL, "capture needs known capacities");
   SDF_HIP(hipEventRecord(c->join[0], c->stream));
   L.bvh_was_ready = c->bvh_ready;
-  auto line_chain = [&](hipEvent_t after) {
-    SDF_HIP(hipStreamWaitEvent(c->aux[0], after, 0));
-    sdf::LineBuild lb;
-    lb.max_blocks = (int64_t)c->line_per_cu * c->ncu;
-    c->ar.scope = "line/";
-    sdf::line_hits_begin(c->aux[0], c->ar, c->tri, c->nel, g, lb, bins);
-    sdf::line_hits_finish(c->aux[0], c->ar, c->tri, c->nel, g, lb, bins, c->caps.hits);
-    c->ar.scope.clear();
-    SDF_HIP(hipEventRecord(c->join[1], c->aux[0]));
-  };
-  if (L.do_sign && c->lines_early) line_chain(c->join[0]);
+  if (c->lines_early) stage_bins_lines(c, L);
   c->mark("bin_tiles");
   L.d2buf = nullptr;
   if (L.do_dist) {
@@ -386,7 +380,6 @@ void stage_bins_tiles(sdf_ctx *c, LatticeRun &L) {
   // beside the latency-bound tile chain they stretch every small kernel
   // (measured: about 35 us per step)
   SDF_HIP(hipEventRecord(c->join[3], c->stream));
-  if (L.do_sign && !c->lines_early) line_chain(c->join[3]);
   L.cap = c->caps.pend ? c->caps.pend
                        : (uint32_t)(npts < (1 << 22) ? (npts > 0 ? npts : 1) : (1 << 22));
   L.cap0 = c->caps.pend0 ? c->caps.pend0
@@ -395,12 +388,27 @@ void stage_bins_tiles(sdf_ctx *c, LatticeRun &L) {
   L.pend0 = c->ar.get<sdf::Pending>("pend0", L.cap0);
 }
 
+// The axis-line hits on helper stream 0, from the start of the tile chain
+// (lines_early) or from its end (join[3]); joined through join[1].
+void stage_bins_lines(sdf_ctx *c, LatticeRun &L) {
+  if (!L.do_sign) return;
+  SDF_HIP(hipStreamWaitEvent(c->aux[0], c->join[c->lines_early ? 0 : 3], 0));
+  sdf::LineBuild lb;
+  lb.max_blocks = (int64_t)c->line_per_cu * c->ncu;
+  c->ar.scope = "line/";
+  sdf::line_hits_begin(c->aux[0], c->ar, c->tri, c->nel, L.g, lb, L.bins);
+  sdf::line_hits_finish(c->aux[0], c->ar, c->tri, c->nel, L.g, lb, L.bins, c->caps.hits);
+  c->ar.scope.clear();
+  SDF_HIP(hipEventRecord(c->join[1], c->aux[0]));
+}
+
 void stage_bins_bvh(sdf_ctx *c, LatticeRun &L) {
   if (L.do_sign) bvh_async(c, 1);
 }
 
 void stage_bins(sdf_ctx *c, LatticeRun &L) {
   stage_bins_tiles(c, L);
+  if (!c->lines_early) stage_bins_lines(c, L);
   stage_bins_bvh(c, L);
 }
 
@@ -487,8 +495,9 @@ void stage_resolve(sdf_ctx *c, LatticeRun &L) {
 void lattice_pass(sdf_ctx *c, LatticeRun &L) {
   stage_bins_tiles(c, L);
   stage_dist(c, L);  // the learned variant builds the LBVH itself, first
-  stage_sign(c, L);
+  if (!c->lines_early) stage_bins_lines(c, L);
   stage_bins_bvh(c, L);
+  stage_sign(c, L);
   stage_resolve(c, L);
 }
 

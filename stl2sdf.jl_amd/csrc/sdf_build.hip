@@ -383,31 +383,36 @@ __device__ inline LineRange tri_line_range(const Tri64 &t, const GridP &g) {
   return r;
 }
 
+// Line (ax, a, b): its id and its origin, the lattice point with index 0
+// along the axis.
+__device__ inline int64_t line_origin(const GridP &g, int ax, int32_t a, int32_t b,
+                                      int64_t base1, int64_t base2, double *o) {
+  if (ax == 0) {  // lines over (j, k)
+    o[0] = lattice_coord(g.ox, g.h, 0);
+    o[1] = lattice_coord(g.oy, g.h, a);
+    o[2] = lattice_coord(g.oz, g.h, b);
+    return (int64_t)b * g.ny + a;
+  } else if (ax == 1) {  // lines over (i, k)
+    o[0] = lattice_coord(g.ox, g.h, a);
+    o[1] = lattice_coord(g.oy, g.h, 0);
+    o[2] = lattice_coord(g.oz, g.h, b);
+    return base1 + (int64_t)b * g.nx + a;
+  }
+  // lines over (i, j)
+  o[0] = lattice_coord(g.ox, g.h, a);
+  o[1] = lattice_coord(g.oy, g.h, b);
+  o[2] = lattice_coord(g.oz, g.h, 0);
+  return base2 + (int64_t)b * g.nx + a;
+}
+
 __device__ inline void line_visit(const Tri64 &tr, const GridP &g, int ax,
                                   int32_t a, int32_t b, int64_t base1,
                                   int64_t base2, int mode,
                                   uint32_t *__restrict__ cnt,
                                   const uint64_t *__restrict__ off,
                                   LineHit *__restrict__ hits, uint64_t cap) {
-  // origin: the lattice point with index 0 along the axis
   double o[3];
-  int64_t line;
-  if (ax == 0) {  // lines over (j, k)
-    o[0] = lattice_coord(g.ox, g.h, 0);
-    o[1] = lattice_coord(g.oy, g.h, a);
-    o[2] = lattice_coord(g.oz, g.h, b);
-    line = (int64_t)b * g.ny + a;
-  } else if (ax == 1) {  // lines over (i, k)
-    o[0] = lattice_coord(g.ox, g.h, a);
-    o[1] = lattice_coord(g.oy, g.h, 0);
-    o[2] = lattice_coord(g.oz, g.h, b);
-    line = base1 + (int64_t)b * g.nx + a;
-  } else {  // lines over (i, j)
-    o[0] = lattice_coord(g.ox, g.h, a);
-    o[1] = lattice_coord(g.oy, g.h, b);
-    o[2] = lattice_coord(g.oz, g.h, 0);
-    line = base2 + (int64_t)b * g.nx + a;
-  }
+  const int64_t line = line_origin(g, ax, a, b, base1, base2, o);
   LineHit rec;
   if (!line_hit_test(o, ax, tr.v, tr.v + 3, tr.v + 6, mode == 0 ? nullptr : &rec))
     return;
@@ -440,6 +445,61 @@ __global__ void line_bin_small_kernel(const Tri64 *__restrict__ tri, int64_t nel
       for (int32_t b = r.lo[ax][1]; b <= r.hi[ax][1]; b++)
         for (int32_t a = r.lo[ax][0]; a <= r.hi[ax][0]; a++)
           line_visit(tr, g, ax, a, b, base1, base2, mode, cnt, off, hits, cap);
+  }
+}
+
+// Emit pass for triangles with at most kSmallMax candidate lines: the
+// hits are found first (up to kEmitBatch line ids in registers), then their
+// cursor atomics are issued together and waited for once, then the records
+// are written.  A triangle with more hits takes the one-at-a-time path.
+constexpr int kEmitBatch = 4;
+
+__global__ void line_emit_small_kernel(const Tri64 *__restrict__ tri, int64_t nel, GridP g,
+                                       int64_t base1, int64_t base2,
+                                       uint32_t *__restrict__ cursor,
+                                       const uint64_t *__restrict__ off,
+                                       LineHit *__restrict__ hits, uint64_t cap) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nel;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const Tri64 tr = tri[t];
+    const LineRange r = tri_line_range(tr, g);
+    const int64_t total = r.cnt[0] + r.cnt[1] + r.cnt[2];
+    if (total == 0 || total > kSmallMax) continue;
+    int nh = 0;
+    int64_t ln[kEmitBatch] = {0, 0, 0, 0};
+    int lax[kEmitBatch] = {0, 0, 0, 0};
+    for (int ax = 0; ax < 3; ax++)
+      for (int32_t b = r.lo[ax][1]; b <= r.hi[ax][1]; b++)
+        for (int32_t a = r.lo[ax][0]; a <= r.hi[ax][0]; a++) {
+          double o[3];
+          const int64_t line = line_origin(g, ax, a, b, base1, base2, o);
+          if (!line_hit_test(o, ax, tr.v, tr.v + 3, tr.v + 6, nullptr)) continue;
+#pragma unroll
+          for (int k = 0; k < kEmitBatch; k++) {
+            ln[k] = nh == k ? line : ln[k];
+            lax[k] = nh == k ? ax : lax[k];
+          }
+          nh++;
+        }
+    if (nh > kEmitBatch) {
+      for (int ax = 0; ax < 3; ax++)
+        for (int32_t b = r.lo[ax][1]; b <= r.hi[ax][1]; b++)
+          for (int32_t a = r.lo[ax][0]; a <= r.hi[ax][0]; a++)
+            line_visit(tr, g, ax, a, b, base1, base2, 1, cursor, off, hits, cap);
+      continue;
+    }
+    uint32_t slot[kEmitBatch];
+#pragma unroll
+    for (int k = 0; k < kEmitBatch; k++) slot[k] = k < nh ? atomicAdd(&cursor[ln[k]], 1u) : 0u;
+#pragma unroll
+    for (int k = 0; k < kEmitBatch; k++) {
+      if (k < nh) {
+        LineHit rec;
+        line_hit_record(lax[k], tr.v, tr.v + 3, tr.v + 6, &rec);
+        const uint64_t pos = off[ln[k]] + slot[k];
+        if (pos < cap) hits[pos] = rec;  // beyond: overflow, reported by the sizes
+      }
+    }
   }
 }
 
@@ -862,10 +922,9 @@ void line_hits_finish(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
   LineHit *hits = ar.get<LineHit>("line_hits", cap + 1);
   if (cap > 0 && nel > 0) {
     launch_fill_u32(s, lb.lcnt, lb.nlines + 1, 0u);  // reused as the cursor
-    hipLaunchKernelGGL(line_bin_small_kernel, dim3(capped_grid(nel, 256, lb.max_blocks)),
-                       dim3(256), 0, s,
-                       tri, nel, g, lb.nlx, lb.nlx + lb.nly, 1, lb.lcnt,
-                       (const uint64_t *)lb.loff, hits, (uint64_t)cap, lb.big, lb.nbig);
+    hipLaunchKernelGGL(line_emit_small_kernel, dim3(capped_grid(nel, 256, lb.max_blocks)),
+                       dim3(256), 0, s, tri, nel, g, lb.nlx, lb.nlx + lb.nly, lb.lcnt,
+                       (const uint64_t *)lb.loff, hits, (uint64_t)cap);
     hipLaunchKernelGGL(line_bin_big_kernel, dim3(1024), dim3(256), 0, s, tri, g, lb.nlx,
                        lb.nlx + lb.nly, 1, lb.lcnt, (const uint64_t *)lb.loff, hits,
                        (uint64_t)cap, (const uint32_t *)lb.big, (const uint32_t *)lb.nbig);

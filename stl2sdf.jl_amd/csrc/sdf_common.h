@@ -121,6 +121,26 @@ struct LineHit {
   double f;
 };
 
+// The record line_hit_test writes for a hit of the triangle on an axis line
+// (it does not depend on the line), by the same arithmetic.
+__host__ __device__ inline void line_hit_record(int axis, const double *v1, const double *v2,
+                                                const double *v3, LineHit *rec) {
+#pragma clang fp contract(off)
+  double d[3] = {0.0, 0.0, 0.0};
+  d[axis] = 1.0;
+  double e1[3] = {v2[0] - v1[0], v2[1] - v1[1], v2[2] - v1[2]};
+  double e2[3] = {v3[0] - v1[0], v3[1] - v1[1], v3[2] - v1[2]};
+  double hv[3];
+  cross3_ref(d, e2, hv);
+  const double a = dot3_ref(e1, hv);
+  for (int c = 0; c < 3; c++) {
+    rec->v1[c] = v1[c];
+    rec->e1[c] = e1[c];
+    rec->e2[c] = e2[c];
+  }
+  rec->f = 1.0 / a;
+}
+
 // u/v/|a| part of MT for the +axis direction on an axis-aligned grid line.
 // For d = +e_axis those quantities do not depend on the origin's coordinate
 // along the axis (the along-axis component only enters through products with
