@@ -593,7 +593,63 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S,
   }
   __syncthreads();
 
-  if (in_grid) {
+  // the point's distance: min(sqrt(best), h) of its distance slot
+  auto point_dist = [&](int q) -> double {
+    double dist = g.h;
+    if (d2buf) {
+      const int k = k0 + q;
+      const int t = (q >> 1) * 2 + ((lane >> 3) >> 2);
+      const int tl = (i & (kTileX - 1)) + kTileX * ((j & (kTileY - 1)) + kTileY * (k & (kTileZ - 1)));
+      const float best = S.d2[t][tl];
+      if (best < __builtin_huge_valf()) {
+        const double d = sqrt((double)best);
+        dist = d < g.h ? d : g.h;
+      }
+    }
+    return dist;
+  };
+  // pending entry when a decision sat inside its rounding margin (amb: the
+  // point restarts the vote from ray 1 with the reference's arithmetic) or
+  // the six votes disagree; the vote stage writes such a point's outputs
+  // later, after the provisional ones written here
+  auto finish = [&](int64_t idx, double dist, int inside, bool amb) {
+    if (do_sign && (amb | ((inside != 0) & (inside != 6)))) {
+      Pending pr;
+      pr.idx = idx;
+      pr.dist = dist;
+      pr.inside = amb ? 0 : inside;
+      pr.used = amb ? 0 : 6;
+      uint32_t *cnt = amb ? C.npend0 : C.npend;
+      Pending *list = amb ? C.pend0 : C.pend;
+      const uint32_t slot = atomicAdd(cnt, 1u);
+      if (slot < (amb ? C.cap0 : C.cap))
+        list[slot] = pr;
+      else
+        atomicOr(C.err, kErrPendingOverflow);
+    }
+    const double sgn = inside == 6 ? 1.0 : -1.0;
+    if ((kOut & kOutDist) && out.dist) out.dist[idx] = dist;
+    if ((kOut & kOutSdf) && out.sdf) out.sdf[idx] = dist * sgn;
+    if ((kOut & kOutSigns) && out.signs) out.signs[idx] = sgn;
+    if ((kOut & kOutConf) && out.conf) out.conf[idx] = 1.0;
+  };
+  if (in_grid && brick_uni) {
+    // every line of the brick is shared by its points: the six parities are
+    // the stored ones
+    const int64_t plane = (int64_t)g.nx * g.ny;
+    const int64_t idx0 = ((int64_t)k0 * g.ny + j) * g.nx + i;
+#pragma unroll
+    for (int q = 0; q < kSignZ; q++) {
+      if (q >= nk) break;  // uniform per wave
+      int inside = 0;
+      if (do_sign) {
+        const uint32_t ux = S.ux[q][lane >> 3], uy = S.uy[q][lane & 7];
+        inside = zp + zn + (int)(((ux >> 1) & 1u) + ((ux >> 2) & 1u) + ((uy >> 1) & 1u) +
+                                 ((uy >> 2) & 1u));
+      }
+      finish(idx0 + q * plane, point_dist(q), inside, false);
+    }
+  } else if (in_grid) {
     const double x = lattice_coord(g.ox, g.h, i), y = lattice_coord(g.oy, g.h, j);
     double zq[kSignZ];
 #pragma unroll
@@ -606,7 +662,7 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S,
       zneg[q] = zn;
       zok[q] = true;
     }
-    if (do_sign && !brick_uni && !zuni) {
+    if (do_sign && !zuni) {
       // z-line (i, j): one summary classifies all 4 points, else the line's
       // crossing list
       if (!inline_parity<kSignZ>(rz, zq, zpos, zneg, zok)) {
@@ -624,76 +680,40 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S,
       if (q >= nk) break;  // uniform per wave
       const int32_t k = k0 + q;
       const int64_t idx = ((int64_t)k * g.ny + j) * g.nx + i;
-      double dist = g.h;
-      if (d2buf) {
-        const int t = (q >> 1) * 2 + ((lane >> 3) >> 2);
-        const int tl = (i & (kTileX - 1)) + kTileX * ((j & (kTileY - 1)) + kTileY * (k & (kTileZ - 1)));
-        const float best = S.d2[t][tl];
-        if (best < __builtin_huge_valf()) {
-          const double d = sqrt((double)best);
-          dist = d < g.h ? d : g.h;
-        }
-      }
       int inside = 0;
       bool amb = false;
       if (do_sign) {
         const uint32_t ux = S.ux[q][lane >> 3], uy = S.uy[q][lane & 7];
         inside = zpos[q] + zneg[q];
-        if (brick_uni) {
-          inside += (int)(((ux >> 1) & 1u) + ((ux >> 2) & 1u) + ((uy >> 1) & 1u) + ((uy >> 2) & 1u));
+        int a, bpar;
+        bool okq;
+        amb = !zok[q];
+        if (ux & 1u) {
+          a = (int)((ux >> 1) & 1u);
+          bpar = (int)((ux >> 2) & 1u);
         } else {
-          int a, bpar;
-          bool okq;
-          amb = !zok[q];
-          if (ux & 1u) {
-            a = (int)((ux >> 1) & 1u);
-            bpar = (int)((ux >> 2) & 1u);
-          } else {
-            const LineInline &rx = S.lx[q][lane >> 3];
-            if (!inline_parity<1>(rx, &x, &a, &bpar, &okq) || !okq) {
-              const int64_t Lx = (int64_t)k * g.ny + j;
-              amb |= !axis_pair_parity_fast(C.fast, C.line_off[Lx], C.line_off[Lx + 1],
-                                            C.meta[Lx], x, &a, &bpar);
-            }
+          const LineInline &rx = S.lx[q][lane >> 3];
+          if (!inline_parity<1>(rx, &x, &a, &bpar, &okq) || !okq) {
+            const int64_t Lx = (int64_t)k * g.ny + j;
+            amb |= !axis_pair_parity_fast(C.fast, C.line_off[Lx], C.line_off[Lx + 1],
+                                          C.meta[Lx], x, &a, &bpar);
           }
-          inside += a + bpar;
-          if (uy & 1u) {
-            a = (int)((uy >> 1) & 1u);
-            bpar = (int)((uy >> 2) & 1u);
-          } else {
-            const LineInline &ry = S.ly[q][lane & 7];
-            if (!inline_parity<1>(ry, &y, &a, &bpar, &okq) || !okq) {
-              const int64_t Ly = base1 + (int64_t)k * g.nx + i;
-              amb |= !axis_pair_parity_fast(C.fast, C.line_off[Ly], C.line_off[Ly + 1],
-                                            C.meta[Ly], y, &a, &bpar);
-            }
+        }
+        inside += a + bpar;
+        if (uy & 1u) {
+          a = (int)((uy >> 1) & 1u);
+          bpar = (int)((uy >> 2) & 1u);
+        } else {
+          const LineInline &ry = S.ly[q][lane & 7];
+          if (!inline_parity<1>(ry, &y, &a, &bpar, &okq) || !okq) {
+            const int64_t Ly = base1 + (int64_t)k * g.nx + i;
+            amb |= !axis_pair_parity_fast(C.fast, C.line_off[Ly], C.line_off[Ly + 1],
+                                          C.meta[Ly], y, &a, &bpar);
           }
-          inside += a + bpar;
         }
-        if (amb | ((inside != 0) & (inside != 6))) {
-          // amb: a decision sat inside its rounding margin, so the point
-          // restarts the vote from ray 1 with the reference's arithmetic;
-          // otherwise the six votes disagree.  The vote stage writes the
-          // point's outputs later (after the provisional ones below).
-          Pending pr;
-          pr.idx = idx;
-          pr.dist = dist;
-          pr.inside = amb ? 0 : inside;
-          pr.used = amb ? 0 : 6;
-          uint32_t *cnt = amb ? C.npend0 : C.npend;
-          Pending *list = amb ? C.pend0 : C.pend;
-          const uint32_t slot = atomicAdd(cnt, 1u);
-          if (slot < (amb ? C.cap0 : C.cap))
-            list[slot] = pr;
-          else
-            atomicOr(C.err, kErrPendingOverflow);
-        }
+        inside += a + bpar;
       }
-      const double sgn = inside == 6 ? 1.0 : -1.0;
-      if ((kOut & kOutDist) && out.dist) out.dist[idx] = dist;
-      if ((kOut & kOutSdf) && out.sdf) out.sdf[idx] = dist * sgn;
-      if ((kOut & kOutSigns) && out.signs) out.signs[idx] = sgn;
-      if ((kOut & kOutConf) && out.conf) out.conf[idx] = 1.0;
+      finish(idx, point_dist(q), inside, amb);
     }
   }
   __syncthreads();  // S is reused by the block's next brick
