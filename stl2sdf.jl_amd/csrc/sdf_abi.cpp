@@ -240,11 +240,21 @@ const sdf::Bvh &need_bvh(sdf_ctx *c) {
 
 // Build the LBVH on helper stream q after the work recorded in join[3] (the
 // end of the current pass's tile chain); joined later through join[2].
-void bvh_async(sdf_ctx *c, int q) {
+// Morton codes are quantised over the lattice's box (it covers the mesh the
+// grid was built around; any box gives a valid hierarchy).
+void bvh_async(sdf_ctx *c, int q, const sdf::GridP &g) {
   if (c->bvh_ready) return;
   SDF_HIP(hipStreamWaitEvent(c->aux[q], c->join[3], 0));
   c->ar.scope = "bvh/";
-  c->bvh = sdf::build_lbvh(c->aux[q], c->ar, c->tri, c->nel);
+  sdf::QBox qb;
+  const double o[3] = {g.ox, g.oy, g.oz};
+  const int32_t nn[3] = {g.nx, g.ny, g.nz};
+  for (int d = 0; d < 3; d++) {
+    qb.lo[d] = (float)o[d];
+    const double e = (double)(nn[d] > 1 ? nn[d] - 1 : 1) * g.h;
+    qb.ext[d] = e > 0.0 ? (float)e : 1.0f;
+  }
+  c->bvh = sdf::build_lbvh(c->aux[q], c->ar, c->tri, c->nel, &qb);
   c->ar.scope.clear();
   c->bvh_ready = true;
   SDF_HIP(hipEventRecord(c->join[2], c->aux[q]));
@@ -403,7 +413,7 @@ void stage_bins_lines(sdf_ctx *c, LatticeRun &L) {
 }
 
 void stage_bins_bvh(sdf_ctx *c, LatticeRun &L) {
-  if (L.do_sign) bvh_async(c, 1);
+  if (L.do_sign) bvh_async(c, 1, L.g);
 }
 
 void stage_bins(sdf_ctx *c, LatticeRun &L) {
@@ -431,7 +441,7 @@ void stage_dist(sdf_ctx *c, LatticeRun &L) {
     c->mark("dist_learned");
     sdf::launch_dist_chunks_learned(c->stream, g, c->tri, bins, r2, L.d2buf, list, c->lstats);
     if (!L.bvh_was_ready) {
-      bvh_async(c, 1);
+      bvh_async(c, 1, g);
       SDF_HIP(hipStreamWaitEvent(c->stream, c->join[2], 0));
     }
     c->mark("fallback");

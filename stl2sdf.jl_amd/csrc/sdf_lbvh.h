@@ -21,7 +21,14 @@ struct Bvh {
   uint32_t *leaf_tri = nullptr;  // sorted leaf -> triangle id
 };
 
-Bvh build_lbvh(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t n);
+// Morton quantisation box: lo corner and extent per axis (extent > 0).
+struct QBox {
+  float lo[3], ext[3];
+};
+
+// qbox null: the box of the triangles' centroids (one reduction pass).
+Bvh build_lbvh(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t n,
+               const QBox *qbox = nullptr);
 // Stable LSD sort of (key, value) pairs by the low key_bits bits; n is a
 // capacity when n_dev (the actual count, on the device) is given.  With
 // keys_out, the sorted pairs are left in whichever buffers the last pass

@@ -128,6 +128,31 @@ __global__ void morton_kernel(const float *__restrict__ box, int64_t n,
   }
 }
 
+// Leaf boxes and Morton codes in one pass, quantised over a box given by
+// the caller (a lattice's box, which covers the mesh it was built around;
+// codes of centres outside it are clamped, which only coarsens the tree).
+__global__ void leaf_morton_kernel(const Tri64 *__restrict__ tri, int64_t n, QBox qb,
+                                   float *__restrict__ box, uint32_t *__restrict__ keys,
+                                   uint32_t *__restrict__ vals) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const Tri64 r = tri[t];
+    uint32_t q[3];
+    for (int d = 0; d < 3; d++) {
+      const double a = r.v[d], b = r.v[3 + d], c = r.v[6 + d];
+      const float lo = __double2float_rd(fmin(a, fmin(b, c)));
+      const float hi = __double2float_ru(fmax(a, fmax(b, c)));
+      box[6 * t + d] = lo;
+      box[6 * t + 3 + d] = hi;
+      float x = (0.5f * (lo + hi) - qb.lo[d]) / qb.ext[d] * 1024.0f;
+      x = fminf(fmaxf(x, 0.0f), 1023.0f);
+      q[d] = (uint32_t)x;
+    }
+    keys[t] = (expand10(q[0]) << 2) | (expand10(q[1]) << 1) | expand10(q[2]);
+    vals[t] = (uint32_t)t;
+  }
+}
+
 // ---- radix sort (LSD, stable) ----
 //
 // A pass: per-block digit counts (digit-major, hist[d * nblocks + b]), the
@@ -480,24 +505,29 @@ void radix_sort_u32_pairs(hipStream_t s, Arena &ar, uint32_t *keys,
   radix_sort_pairs(s, ar, keys, vals, n, key_bits, n_dev, keys_out, vals_out);
 }
 
-Bvh build_lbvh(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t n) {
+Bvh build_lbvh(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t n, const QBox *qbox) {
   Bvh b;
   b.n = n;
   if (n <= 0) return b;
   b.leaf_box = ar.get<float>("bvh_leaf_box", 6 * n);
-  uint32_t *cb = ar.get<uint32_t>("bvh_cbounds", 8);
-  const int lb_blocks = blocks_for(n, 256);
-  uint32_t *part = ar.get<uint32_t>("bvh_cbounds_part", 6 * (size_t)lb_blocks);
-  hipLaunchKernelGGL(leaf_boxes_kernel, dim3(lb_blocks), dim3(256), 0, s, tri, n, b.leaf_box,
-                     part);
-  hipLaunchKernelGGL(bounds_reduce_kernel, dim3(1), dim3(256), 0, s, (const uint32_t *)part,
-                     (int64_t)lb_blocks, cb);
-  SDF_HIP(hipGetLastError());
   uint32_t *codes = ar.get<uint32_t>("bvh_codes", n);
   uint32_t *leaf_tri = ar.get<uint32_t>("bvh_leaf_tri", n);
-  hipLaunchKernelGGL(morton_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s,
-                     b.leaf_box, n, cb, codes, leaf_tri);
-  SDF_HIP(hipGetLastError());
+  if (qbox) {
+    hipLaunchKernelGGL(leaf_morton_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, tri, n,
+                       *qbox, b.leaf_box, codes, leaf_tri);
+    SDF_HIP(hipGetLastError());
+  } else {
+    uint32_t *cb = ar.get<uint32_t>("bvh_cbounds", 8);
+    const int lb_blocks = blocks_for(n, 256);
+    uint32_t *part = ar.get<uint32_t>("bvh_cbounds_part", 6 * (size_t)lb_blocks);
+    hipLaunchKernelGGL(leaf_boxes_kernel, dim3(lb_blocks), dim3(256), 0, s, tri, n, b.leaf_box,
+                       part);
+    hipLaunchKernelGGL(bounds_reduce_kernel, dim3(1), dim3(256), 0, s, (const uint32_t *)part,
+                       (int64_t)lb_blocks, cb);
+    hipLaunchKernelGGL(morton_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s,
+                       b.leaf_box, n, cb, codes, leaf_tri);
+    SDF_HIP(hipGetLastError());
+  }
   radix_sort_pairs(s, ar, codes, leaf_tri, n, 30, nullptr);
   b.leaf_tri = leaf_tri;
   if (n == 1) {
