@@ -106,7 +106,9 @@ struct sdf_ctx {
   int64_t nel = 0, nnp = 0;
   bool mesh_ready = false;
   sdf::Bvh bvh;
-  bool bvh_ready = false;
+  sdf::LineBuild lb;  // the line chain between its counting and emit passes
+  bool bvh_ready = false;  // the ray hierarchy of the current mesh is built
+  bool bvh_full = false;   // and the per-triangle one too
   // small device words: [0] error flags, [1] pending count, [2] pending
   // count of ambiguous axis decisions, [4..19] resolve-phase counters
   uint32_t *d_words = nullptr;
@@ -128,8 +130,8 @@ struct sdf_ctx {
   // while the scalar-bound sign kernel needs all its waves.
   int chunk_len = 64;
   int ray_stack = 1024;
-  int dist_per_cu = 8, fused_per_cu = 0;
-  int lines_early = 0;  // 1: the line chain starts with the tile chain
+  int dist_per_cu = 10, fused_per_cu = 0;
+  int lines_early = 0;  // 1: the line chain starts with the tile chain; 2: its count pass
   int line_per_cu = 0;  // workgroups per CU of the line kernels (0: no cap)
   int ncu = 256;
   // timing
@@ -227,13 +229,16 @@ void set_mesh_device(sdf_ctx *c, const double *dX, int64_t nnp,
   sdf::launch_gather_tris(c->stream, dX, nnp, dIEN, nel, c->tri, c->d_words + kMeshErrWord);
   c->mesh_ready = true;
   c->bvh_ready = false;
+  c->bvh_full = false;
 }
 
+// Both hierarchies (nearest-point queries need the per-triangle one).
 const sdf::Bvh &need_bvh(sdf_ctx *c) {
-  if (!c->bvh_ready) {
+  if (!c->bvh_ready || !c->bvh_full) {
     c->mark("lbvh");
     c->bvh = sdf::build_lbvh(c->stream, c->ar, c->tri, c->nel);
     c->bvh_ready = true;
+    c->bvh_full = true;
   }
   return c->bvh;
 }
@@ -242,8 +247,8 @@ const sdf::Bvh &need_bvh(sdf_ctx *c) {
 // end of the current pass's tile chain); joined later through join[2].
 // Morton codes are quantised over the lattice's box (it covers the mesh the
 // grid was built around; any box gives a valid hierarchy).
-void bvh_async(sdf_ctx *c, int q, const sdf::GridP &g) {
-  if (c->bvh_ready) return;
+void bvh_async(sdf_ctx *c, int q, const sdf::GridP &g, bool full) {
+  if (c->bvh_ready && (c->bvh_full || !full)) return;
   SDF_HIP(hipStreamWaitEvent(c->aux[q], c->join[3], 0));
   c->ar.scope = "bvh/";
   sdf::QBox qb;
@@ -254,9 +259,10 @@ void bvh_async(sdf_ctx *c, int q, const sdf::GridP &g) {
     const double e = (double)(nn[d] > 1 ? nn[d] - 1 : 1) * g.h;
     qb.ext[d] = e > 0.0 ? (float)e : 1.0f;
   }
-  c->bvh = sdf::build_lbvh(c->aux[q], c->ar, c->tri, c->nel, &qb);
+  c->bvh = sdf::build_lbvh(c->aux[q], c->ar, c->tri, c->nel, &qb, full);
   c->ar.scope.clear();
   c->bvh_ready = true;
+  c->bvh_full = full;
   SDF_HIP(hipEventRecord(c->join[2], c->aux[q]));
 }
 
@@ -361,7 +367,7 @@ enum : int {
 // (stage_bins_bvh), then the sign kernel: the distance kernel is queued
 // before the host spends its time issuing the helper chains, so the main
 // stream does not idle after the tile chain.
-void stage_bins_lines(sdf_ctx *c, LatticeRun &L);
+void stage_bins_lines(sdf_ctx *c, LatticeRun &L, int part);
 
 void stage_bins_tiles(sdf_ctx *c, LatticeRun &L) {
   const sdf::GridP &g = L.g;
@@ -373,7 +379,8 @@ void stage_bins_tiles(sdf_ctx *c, LatticeRun &L) {
     fail(SDF_ERR_INTERNAL, "capture needs known capacities");
   SDF_HIP(hipEventRecord(c->join[0], c->stream));
   L.bvh_was_ready = c->bvh_ready;
-  if (c->lines_early) stage_bins_lines(c, L);
+  if (c->lines_early == 1) stage_bins_lines(c, L, 3);
+  if (c->lines_early == 2) stage_bins_lines(c, L, 1);
   c->mark("bin_tiles");
   L.d2buf = nullptr;
   if (L.do_dist) {
@@ -398,27 +405,34 @@ void stage_bins_tiles(sdf_ctx *c, LatticeRun &L) {
   L.pend0 = c->ar.get<sdf::Pending>("pend0", L.cap0);
 }
 
-// The axis-line hits on helper stream 0, from the start of the tile chain
-// (lines_early) or from its end (join[3]); joined through join[1].
-void stage_bins_lines(sdf_ctx *c, LatticeRun &L) {
+// The axis-line hits on helper stream 0, joined through join[1].
+// lines_early 0: all of it from the end of the tile chain (join[3]);
+// 1: all of it from its start (join[0]); 2: the counting pass from its
+// start, the emit and summary passes from its end.
+void stage_bins_lines(sdf_ctx *c, LatticeRun &L, int part) {
   if (!L.do_sign) return;
-  SDF_HIP(hipStreamWaitEvent(c->aux[0], c->join[c->lines_early ? 0 : 3], 0));
-  sdf::LineBuild lb;
-  lb.max_blocks = (int64_t)c->line_per_cu * c->ncu;
   c->ar.scope = "line/";
-  sdf::line_hits_begin(c->aux[0], c->ar, c->tri, c->nel, L.g, lb, L.bins);
-  sdf::line_hits_finish(c->aux[0], c->ar, c->tri, c->nel, L.g, lb, L.bins, c->caps.hits);
+  if (part & 1) {
+    SDF_HIP(hipStreamWaitEvent(c->aux[0], c->join[c->lines_early ? 0 : 3], 0));
+    c->lb = sdf::LineBuild();
+    c->lb.max_blocks = (int64_t)c->line_per_cu * c->ncu;
+    sdf::line_hits_begin(c->aux[0], c->ar, c->tri, c->nel, L.g, c->lb, L.bins);
+  }
+  if (part & 2) {
+    if (c->lines_early == 2) SDF_HIP(hipStreamWaitEvent(c->aux[0], c->join[3], 0));
+    sdf::line_hits_finish(c->aux[0], c->ar, c->tri, c->nel, L.g, c->lb, L.bins, c->caps.hits);
+    SDF_HIP(hipEventRecord(c->join[1], c->aux[0]));
+  }
   c->ar.scope.clear();
-  SDF_HIP(hipEventRecord(c->join[1], c->aux[0]));
 }
 
 void stage_bins_bvh(sdf_ctx *c, LatticeRun &L) {
-  if (L.do_sign) bvh_async(c, 1, L.g);
+  if (L.do_sign) bvh_async(c, 1, L.g, false);  // the vote stage needs only the rays' one
 }
 
 void stage_bins(sdf_ctx *c, LatticeRun &L) {
   stage_bins_tiles(c, L);
-  if (!c->lines_early) stage_bins_lines(c, L);
+  if (c->lines_early != 1) stage_bins_lines(c, L, c->lines_early == 2 ? 2 : 3);
   stage_bins_bvh(c, L);
 }
 
@@ -440,8 +454,8 @@ void stage_dist(sdf_ctx *c, LatticeRun &L) {
                          (float)c->mlp_delta, r2);
     c->mark("dist_learned");
     sdf::launch_dist_chunks_learned(c->stream, g, c->tri, bins, r2, L.d2buf, list, c->lstats);
-    if (!L.bvh_was_ready) {
-      bvh_async(c, 1, g);
+    if (!c->bvh_ready || !c->bvh_full) {  // the fallback queries the per-triangle one
+      bvh_async(c, 1, g, true);
       SDF_HIP(hipStreamWaitEvent(c->stream, c->join[2], 0));
     }
     c->mark("fallback");
@@ -505,7 +519,7 @@ void stage_resolve(sdf_ctx *c, LatticeRun &L) {
 void lattice_pass(sdf_ctx *c, LatticeRun &L) {
   stage_bins_tiles(c, L);
   stage_dist(c, L);  // the learned variant builds the LBVH itself, first
-  if (!c->lines_early) stage_bins_lines(c, L);
+  if (c->lines_early != 1) stage_bins_lines(c, L, c->lines_early == 2 ? 2 : 3);
   stage_bins_bvh(c, L);
   stage_sign(c, L);
   stage_resolve(c, L);
@@ -1086,7 +1100,7 @@ sdf_status sdf_ctx_tune(sdf_ctx *c, const char *name, int64_t value) {
       if (value != 1024 && value != 2048) fail(SDF_ERR_ARG, "ray_stack must be 1024 or 2048");
       c->ray_stack = (int)value;
     } else if (n == "lines_early") {
-      if (value != 0 && value != 1) fail(SDF_ERR_ARG, "lines_early must be 0 or 1");
+      if (value < 0 || value > 2) fail(SDF_ERR_ARG, "lines_early must be 0, 1 or 2");
       c->lines_early = (int)value;
     } else if (n == "dist_per_cu" || n == "fused_per_cu" || n == "line_per_cu") {
       if (value < 0 || value > 64) fail(SDF_ERR_ARG, n + " must be in 0..64");

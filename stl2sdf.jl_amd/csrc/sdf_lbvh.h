@@ -14,11 +14,19 @@ struct BvhNode {
   int32_t pad_[2];
 };
 
+// Triangles per leaf of the ray hierarchy.
+constexpr int kRayLeaf = 8;
+
 struct Bvh {
   int64_t n = 0;               // leaves (triangles)
-  BvhNode *nodes = nullptr;    // n-1 internal nodes, root = 0
+  BvhNode *nodes = nullptr;    // n-1 internal nodes, root = 0 (null unless built full)
   float *leaf_box = nullptr;   // 6 floats per triangle (by triangle id)
   uint32_t *leaf_tri = nullptr;  // sorted leaf -> triangle id
+  // The ray hierarchy: leaves are runs of kRayLeaf consecutive sorted
+  // triangles (run c = leaf_tri[c kRayLeaf ..]); nc - 1 internal nodes
+  // (root 0) whose leaf children are -(c+1).  nc == 1: no nodes.
+  int64_t nc = 0;
+  BvhNode *cnodes = nullptr;
 };
 
 // Morton quantisation box: lo corner and extent per axis (extent > 0).
@@ -27,8 +35,10 @@ struct QBox {
 };
 
 // qbox null: the box of the triangles' centroids (one reduction pass).
+// The ray hierarchy is always built; the per-triangle hierarchy (nodes,
+// for nearest-point queries) only when full.
 Bvh build_lbvh(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t n,
-               const QBox *qbox = nullptr);
+               const QBox *qbox = nullptr, bool full = true);
 // Stable LSD sort of (key, value) pairs by the low key_bits bits; n is a
 // capacity when n_dev (the actual count, on the device) is given.  With
 // keys_out, the sorted pairs are left in whichever buffers the last pass

@@ -302,32 +302,35 @@ __global__ __launch_bounds__(kSortThreads) void radix_scatter_kernel(
 
 // ---- Karras hierarchy ----
 
-__device__ inline int delta(const uint32_t *__restrict__ codes, int64_t n,
+// Leaf i's code is codes[i * stride] (stride 1: triangles; kRayLeaf: the
+// first triangle of each run of the ray hierarchy).
+__device__ inline int delta(const uint32_t *__restrict__ codes, int64_t stride, int64_t n,
                             int64_t i, int64_t j) {
   if (j < 0 || j >= n) return -1;
-  uint32_t a = codes[i], b = codes[j];
+  uint32_t a = codes[i * stride], b = codes[j * stride];
   if (a == b) return 32 + __clz((uint32_t)(i ^ j));
   return __clz(a ^ b);
 }
 
-__global__ void karras_kernel(const uint32_t *__restrict__ codes, int64_t n,
+__global__ void karras_kernel(const uint32_t *__restrict__ codes, int64_t stride, int64_t n,
                               BvhNode *__restrict__ nodes,
                               int32_t *__restrict__ range) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n - 1;
        i += (int64_t)gridDim.x * blockDim.x) {
-    const int d = (delta(codes, n, i, i + 1) - delta(codes, n, i, i - 1)) >= 0 ? 1 : -1;
-    const int dmin = delta(codes, n, i, i - d);
+    auto dl = [&](int64_t a, int64_t b) { return delta(codes, stride, n, a, b); };
+    const int d = (dl(i, i + 1) - dl(i, i - 1)) >= 0 ? 1 : -1;
+    const int dmin = dl(i, i - d);
     int64_t lmax = 2;
-    while (delta(codes, n, i, i + lmax * d) > dmin) lmax *= 2;
+    while (dl(i, i + lmax * d) > dmin) lmax *= 2;
     int64_t l = 0;
     for (int64_t t = lmax / 2; t >= 1; t /= 2)
-      if (delta(codes, n, i, i + (l + t) * d) > dmin) l += t;
+      if (dl(i, i + (l + t) * d) > dmin) l += t;
     const int64_t j = i + l * d;
-    const int dnode = delta(codes, n, i, j);
+    const int dnode = dl(i, j);
     int64_t s = 0, step = l;
     do {
       step = (step + 1) >> 1;
-      if (delta(codes, n, i, i + (s + step) * d) > dnode) s += step;
+      if (dl(i, i + (s + step) * d) > dnode) s += step;
     } while (step > 1);
     const int64_t gamma = i + s * d + (d < 0 ? -1 : 0);
     const int64_t lo = i < j ? i : j, hi = i < j ? j : i;
@@ -451,6 +454,31 @@ __global__ void node_boxes_kernel(const float *__restrict__ seg, SegLevels L,
   }
 }
 
+// Child boxes of the ray hierarchy's nodes: a leaf (run c of kRayLeaf
+// sorted triangles) is segment-tree level 3 entry c; an internal child
+// covers the runs [lo, hi], i.e. the sorted leaves [8 lo, min(8 hi + 7, n-1)].
+static_assert(kRayLeaf == 8, "runs are segment-tree level 3");
+__global__ void run_node_boxes_kernel(const float *__restrict__ seg, SegLevels L,
+                                      const int32_t *__restrict__ range, int64_t nc, int64_t n,
+                                      BvhNode *__restrict__ nodes) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nc - 1;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    for (int c = 0; c < 2; c++) {
+      const int32_t ch = nodes[i].child[c];
+      float b[6];
+      if (ch < 0) {
+        const float *lv = seg + L.off[3] + 6 * (int64_t)(-(ch + 1));
+        for (int q = 0; q < 6; q++) b[q] = lv[q];
+      } else {
+        const int64_t lo = (int64_t)range[2 * ch] * kRayLeaf;
+        const int64_t hi0 = (int64_t)range[2 * ch + 1] * kRayLeaf + kRayLeaf - 1;
+        seg_query(seg, L, lo, hi0 < n - 1 ? hi0 : n - 1, b);
+      }
+      for (int q = 0; q < 6; q++) nodes[i].box[c][q] = b[q];
+    }
+  }
+}
+
 // Sorted pairs end in (keys, vals), or, when keys_out is given, in the pair
 // of buffers the last pass wrote (*keys_out, *vals_out; no copy back).
 void radix_sort_pairs(hipStream_t s, Arena &ar, uint32_t *keys, uint32_t *vals,
@@ -505,7 +533,8 @@ void radix_sort_u32_pairs(hipStream_t s, Arena &ar, uint32_t *keys,
   radix_sort_pairs(s, ar, keys, vals, n, key_bits, n_dev, keys_out, vals_out);
 }
 
-Bvh build_lbvh(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t n, const QBox *qbox) {
+Bvh build_lbvh(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t n, const QBox *qbox,
+               bool full) {
   Bvh b;
   b.n = n;
   if (n <= 0) return b;
@@ -530,15 +559,8 @@ Bvh build_lbvh(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t n, const QBox
   }
   radix_sort_pairs(s, ar, codes, leaf_tri, n, 30, nullptr);
   b.leaf_tri = leaf_tri;
-  if (n == 1) {
-    b.nodes = nullptr;
-    return b;
-  }
-  b.nodes = ar.get<BvhNode>("bvh_nodes", n - 1);
-  int32_t *range = ar.get<int32_t>("bvh_range", 2 * (n - 1));
-  hipLaunchKernelGGL(karras_kernel, dim3(blocks_for(n - 1, 256)), dim3(256), 0,
-                     s, codes, n, b.nodes, range);
-  SDF_HIP(hipGetLastError());
+  b.nc = (n + kRayLeaf - 1) / kRayLeaf;
+  if (n == 1) return b;
   SegLevels L;
   int64_t total = 0, m = n;
   L.nlev = 0;
@@ -560,10 +582,25 @@ Bvh build_lbvh(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t n, const QBox
     for (int q = 0; q < k; q++) m = (m + 1) / 2;
     lev += k;
   }
-
-  hipLaunchKernelGGL(node_boxes_kernel, dim3(blocks_for(n - 1, 256)), dim3(256),
-                     0, s, seg, L, range, n, (const uint32_t *)leaf_tri, b.nodes);
-  SDF_HIP(hipGetLastError());
+  if (b.nc > 1) {
+    // the ray hierarchy over runs of kRayLeaf sorted triangles
+    b.cnodes = ar.get<BvhNode>("bvh_cnodes", b.nc - 1);
+    int32_t *crange = ar.get<int32_t>("bvh_crange", 2 * (b.nc - 1));
+    hipLaunchKernelGGL(karras_kernel, dim3(blocks_for(b.nc - 1, 256)), dim3(256), 0, s,
+                       (const uint32_t *)codes, (int64_t)kRayLeaf, b.nc, b.cnodes, crange);
+    hipLaunchKernelGGL(run_node_boxes_kernel, dim3(blocks_for(b.nc - 1, 256)), dim3(256), 0, s,
+                       (const float *)seg, L, (const int32_t *)crange, b.nc, n, b.cnodes);
+    SDF_HIP(hipGetLastError());
+  }
+  if (full) {
+    b.nodes = ar.get<BvhNode>("bvh_nodes", n - 1);
+    int32_t *range = ar.get<int32_t>("bvh_range", 2 * (n - 1));
+    hipLaunchKernelGGL(karras_kernel, dim3(blocks_for(n - 1, 256)), dim3(256), 0, s,
+                       (const uint32_t *)codes, (int64_t)1, n, b.nodes, range);
+    hipLaunchKernelGGL(node_boxes_kernel, dim3(blocks_for(n - 1, 256)), dim3(256),
+                       0, s, seg, L, range, n, (const uint32_t *)leaf_tri, b.nodes);
+    SDF_HIP(hipGetLastError());
+  }
   return b;
 }
 

@@ -852,11 +852,13 @@ __device__ inline const Pending &slot_item(const PendLists &L, uint64_t slot) {
   return slot < L.cap[0] ? L.items[0][slot] : L.items[1][slot - L.cap[0]];
 }
 
-// Wavefront-cooperative ray parity: one wavefront per (item, ray).  The 64
-// lanes pop up to 64 nodes at a time from an LDS stack, test both children
-// (FP64 slab test on boxes rounded outward), run the reference's FP64
-// Moller-Trumbore on leaf triangles and push internal children with a
-// ballot prefix sum.  Hits go to an LDS list; the unique count follows the
+// Wavefront-cooperative ray parity: one wavefront per (item, ray) over the
+// ray hierarchy (leaves are runs of kRayLeaf sorted triangles).  The 64
+// lanes pop up to 64 nodes at a time from an LDS stack and test both
+// children (FP64 slab test on boxes rounded outward); internal children
+// are pushed and reached runs queued with ballot prefix sums; in the same
+// iteration up to 8 queued runs are tested, 8 lanes per run, with the
+// reference's FP64 Moller-Trumbore.  Hits go to an LDS list; the unique count follows the
 // reference rule (sorted, consecutive gap > eps) computed as: a value counts
 // when it is the smallest or its strict predecessor lies more than eps
 // below, equal values counting once.  Stack or hit-list overflow sends the
@@ -866,6 +868,7 @@ __device__ inline const Pending &slot_item(const PendLists &L, uint64_t slot) {
 constexpr int kWaveHits = 256;
 
 constexpr int kRayWaves = 8192;  // persistent grid of the ray stage (waves)
+constexpr int kRunQueue = 512;   // queued runs of one ray (LDS); more: brute force
 
 template <int kWaveStack>
 __global__ __launch_bounds__(64) void ray_wave_kernel(
@@ -873,8 +876,9 @@ __global__ __launch_bounds__(64) void ray_wave_kernel(
     RayParams rp, PendLists lists, int r1, unsigned long long *__restrict__ bits,
     uint64_t *__restrict__ fb, uint32_t *nfb, uint32_t fb_cap, uint32_t *err) {
   __shared__ int32_t stack[kWaveStack];
+  __shared__ int32_t runq[kRunQueue];  // runs reached, waiting for their triangle tests
   __shared__ double hit_t[kWaveHits];
-  __shared__ int s_top, s_nh, s_ovf, s_cnt;
+  __shared__ int s_top, s_nq, s_nh, s_ovf, s_cnt;
   const int lane = threadIdx.x;
   const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   // list 0 contributes n0 * nr0 (item, ray) pairs, then list 1
@@ -901,46 +905,43 @@ __global__ __launch_bounds__(64) void ray_wave_kernel(
     for (int a = 0; a < 3; a++) inv[a] = d[a] != 0.0 ? 1.0 / d[a] : 0.0;
     if (lane == 0) {
       s_top = 0;
+      s_nq = 0;
       s_nh = 0;
       s_ovf = 0;
       s_cnt = 0;
-      if (bvh.n == 1) {
-        const uint32_t t = bvh.leaf_tri[0];
-        const Tri64 q = tri[t];
-        double tv;
-        if (mt_ref(o, d, q.v, q.v + 3, q.v + 6, &tv)) {
-          hit_t[0] = tv;
-          s_nh = 1;
-        }
-      } else if (bvh.n > 1) {
+      if (bvh.nc == 1) {
+        runq[0] = 0;
+        s_nq = 1;
+      } else if (bvh.nc > 1) {
         stack[0] = 0;
         s_top = 1;
       }
     }
     __syncthreads();
+    // Each iteration: up to 64 nodes of the stack (a lane per node, both
+    // children's boxes), then up to 8 queued runs (8 lanes per run, a
+    // triangle per lane).
     while (true) {
-      const int top = s_top;
-      if (top == 0 || s_ovf) break;
+      const int top = s_top, nq = s_nq;
+      if ((top == 0 && nq == 0) || s_ovf) break;
       const int take = top < 64 ? top : 64;
       const int base = top - take;
       const int32_t node = lane < take ? stack[base + lane] : -1;
+      const int qtake = nq < 8 ? nq : 8;
+      const int qbase = nq - qtake;
+      const int32_t run = (lane >> 3) < qtake ? runq[qbase + (lane >> 3)] : -1;
       __syncthreads();
-      int32_t push0 = -1, push1 = -1;
+      int32_t push0 = -1, push1 = -1, leaf0 = -1, leaf1 = -1;
       if (node >= 0) {
-        const BvhNode nd = bvh.nodes[node];
+        const BvhNode nd = bvh.cnodes[node];
         for (int c = 0; c < 2; c++) {
           if (!ray_box(o, d, inv, nd.box[c])) continue;
           const int32_t ch = nd.child[c];
           if (ch < 0) {
-            const Tri64 q = tri[-(ch + 1)];
-            double tv;
-            if (mt_ref(o, d, q.v, q.v + 3, q.v + 6, &tv)) {
-              const int slot = atomicAdd(&s_nh, 1);
-              if (slot < kWaveHits)
-                hit_t[slot] = tv;
-              else
-                s_ovf = 1;
-            }
+            if (leaf0 < 0)
+              leaf0 = -(ch + 1);
+            else
+              leaf1 = -(ch + 1);
           } else if (push0 < 0) {
             push0 = ch;
           } else {
@@ -948,18 +949,42 @@ __global__ __launch_bounds__(64) void ray_wave_kernel(
           }
         }
       }
+      if (run >= 0) {
+        const int64_t li = (int64_t)run * kRayLeaf + (lane & 7);
+        if (li < bvh.n) {
+          const Tri64 q = tri[bvh.leaf_tri[li]];
+          double tv;
+          if (mt_ref(o, d, q.v, q.v + 3, q.v + 6, &tv)) {
+            const int slot = atomicAdd(&s_nh, 1);
+            if (slot < kWaveHits)
+              hit_t[slot] = tv;
+            else
+              s_ovf = 1;
+          }
+        }
+      }
       const uint64_t b1 = __ballot(push0 >= 0), b2 = __ballot(push1 >= 0);
       const int pos = base + __popcll(b1 & below) + __popcll(b2 & below);
       const int ntop = base + __popcll(b1) + __popcll(b2);
-      if (ntop > kWaveStack) {
+      const uint64_t c1 = __ballot(leaf0 >= 0), c2 = __ballot(leaf1 >= 0);
+      const int qpos = qbase + __popcll(c1 & below) + __popcll(c2 & below);
+      const int nnq = qbase + __popcll(c1) + __popcll(c2);
+      if (ntop > kWaveStack || nnq > kRunQueue) {
         if (lane == 0) s_ovf = 1;
       } else {
-        int q = pos;
-        if (push0 >= 0) stack[q++] = push0;
-        if (push1 >= 0) stack[q] = push1;
+        int p = pos;
+        if (push0 >= 0) stack[p++] = push0;
+        if (push1 >= 0) stack[p] = push1;
+        p = qpos;
+        if (leaf0 >= 0) runq[p++] = leaf0;
+        if (leaf1 >= 0) runq[p] = leaf1;
       }
       __syncthreads();
-      if (lane == 0) s_top = ntop > kWaveStack ? 0 : ntop;
+      if (lane == 0) {
+        const bool ok = ntop <= kWaveStack && nnq <= kRunQueue;
+        s_top = ok ? ntop : 0;
+        s_nq = ok ? nnq : 0;
+      }
       __syncthreads();
     }
     const int n = s_nh;
