@@ -455,9 +455,11 @@ __global__ void node_boxes_kernel(const float *__restrict__ seg, SegLevels L,
 }
 
 // Child boxes of the ray hierarchy's nodes: a leaf (run c of kRayLeaf
-// sorted triangles) is segment-tree level 3 entry c; an internal child
-// covers the runs [lo, hi], i.e. the sorted leaves [8 lo, min(8 hi + 7, n-1)].
-static_assert(kRayLeaf == 8, "runs are segment-tree level 3");
+// sorted triangles) is segment-tree level log2(kRayLeaf) entry c; an
+// internal child covers the runs [lo, hi], i.e. the sorted leaves
+// [kRayLeaf lo, min(kRayLeaf (hi + 1) - 1, n - 1)].
+constexpr int kRayLeafLevel = kRayLeaf == 4 ? 2 : (kRayLeaf == 8 ? 3 : (kRayLeaf == 16 ? 4 : -1));
+static_assert(kRayLeafLevel > 0, "runs are a segment-tree level");
 __global__ void run_node_boxes_kernel(const float *__restrict__ seg, SegLevels L,
                                       const int32_t *__restrict__ range, int64_t nc, int64_t n,
                                       BvhNode *__restrict__ nodes) {
@@ -467,7 +469,7 @@ __global__ void run_node_boxes_kernel(const float *__restrict__ seg, SegLevels L
       const int32_t ch = nodes[i].child[c];
       float b[6];
       if (ch < 0) {
-        const float *lv = seg + L.off[3] + 6 * (int64_t)(-(ch + 1));
+        const float *lv = seg + L.off[kRayLeafLevel] + 6 * (int64_t)(-(ch + 1));
         for (int q = 0; q < 6; q++) b[q] = lv[q];
       } else {
         const int64_t lo = (int64_t)range[2 * ch] * kRayLeaf;

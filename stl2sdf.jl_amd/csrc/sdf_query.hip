@@ -857,8 +857,8 @@ __device__ inline const Pending &slot_item(const PendLists &L, uint64_t slot) {
 // lanes pop up to 64 nodes at a time from an LDS stack and test both
 // children (FP64 slab test on boxes rounded outward); internal children
 // are pushed and reached runs queued with ballot prefix sums; in the same
-// iteration up to 8 queued runs are tested, 8 lanes per run, with the
-// reference's FP64 Moller-Trumbore.  Hits go to an LDS list; the unique count follows the
+// iteration up to 64 / kRayLeaf queued runs are tested, kRayLeaf lanes per
+// run, with the reference's FP64 Moller-Trumbore.  Hits go to an LDS list; the unique count follows the
 // reference rule (sorted, consecutive gap > eps) computed as: a value counts
 // when it is the smallest or its strict predecessor lies more than eps
 // below, equal values counting once.  Stack or hit-list overflow sends the
@@ -919,17 +919,18 @@ __global__ __launch_bounds__(64) void ray_wave_kernel(
     }
     __syncthreads();
     // Each iteration: up to 64 nodes of the stack (a lane per node, both
-    // children's boxes), then up to 8 queued runs (8 lanes per run, a
-    // triangle per lane).
+    // children's boxes), then up to 64 / kRayLeaf queued runs (kRayLeaf
+    // lanes per run, a triangle per lane).
     while (true) {
       const int top = s_top, nq = s_nq;
       if ((top == 0 && nq == 0) || s_ovf) break;
       const int take = top < 64 ? top : 64;
       const int base = top - take;
       const int32_t node = lane < take ? stack[base + lane] : -1;
-      const int qtake = nq < 8 ? nq : 8;
+      constexpr int kRunsPerStep = 64 / kRayLeaf;
+      const int qtake = nq < kRunsPerStep ? nq : kRunsPerStep;
       const int qbase = nq - qtake;
-      const int32_t run = (lane >> 3) < qtake ? runq[qbase + (lane >> 3)] : -1;
+      const int32_t run = lane / kRayLeaf < qtake ? runq[qbase + lane / kRayLeaf] : -1;
       __syncthreads();
       int32_t push0 = -1, push1 = -1, leaf0 = -1, leaf1 = -1;
       if (node >= 0) {
@@ -950,7 +951,7 @@ __global__ __launch_bounds__(64) void ray_wave_kernel(
         }
       }
       if (run >= 0) {
-        const int64_t li = (int64_t)run * kRayLeaf + (lane & 7);
+        const int64_t li = (int64_t)run * kRayLeaf + (lane & (kRayLeaf - 1));
         if (li < bvh.n) {
           const Tri64 q = tri[bvh.leaf_tri[li]];
           double tv;
