@@ -425,77 +425,82 @@ __device__ inline void line_visit(const Tri64 &tr, const GridP &g, int ax,
   }
 }
 
-__global__ void line_bin_small_kernel(const Tri64 *__restrict__ tri, int64_t nel,
-                                      GridP g, int64_t base1, int64_t base2,
-                                      int mode, uint32_t *__restrict__ cnt,
-                                      const uint64_t *__restrict__ off,
-                                      LineHit *__restrict__ hits, uint64_t cap,
-                                      uint32_t *__restrict__ big, uint32_t *nbig) {
+// Triangles with at most kSmallMax candidate lines, one thread each.  The
+// counting pass runs the exact hit test and, for a triangle with at most
+// kEmitBatch hits, keeps their line ids (hitn[t] hits at hitl[kEmitBatch t
+// ..]; hitn kHitsMany: more, tested again by the emit pass).  The emit pass
+// then issues a triangle's cursor atomics together, waits once, and writes
+// the records.
+constexpr int kEmitBatch = 4;
+constexpr uint8_t kHitsMany = 255;
+
+__global__ void line_count_small_kernel(const Tri64 *__restrict__ tri, int64_t nel, GridP g,
+                                        int64_t base1, int64_t base2,
+                                        uint32_t *__restrict__ cnt,
+                                        uint32_t *__restrict__ big, uint32_t *nbig,
+                                        uint8_t *__restrict__ hitn,
+                                        uint32_t *__restrict__ hitl) {
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nel;
        t += (int64_t)gridDim.x * blockDim.x) {
     const Tri64 tr = tri[t];
     const LineRange r = tri_line_range(tr, g);
     const int64_t total = r.cnt[0] + r.cnt[1] + r.cnt[2];
-    if (total == 0) continue;
+    int nh = 0;
+    uint32_t ln[kEmitBatch] = {0, 0, 0, 0};
     if (total > kSmallMax) {
-      if (mode == 0) big[atomicAdd(nbig, 1u)] = (uint32_t)t;
-      continue;
+      big[atomicAdd(nbig, 1u)] = (uint32_t)t;
+    } else if (total > 0) {
+      for (int ax = 0; ax < 3; ax++)
+        for (int32_t b = r.lo[ax][1]; b <= r.hi[ax][1]; b++)
+          for (int32_t a = r.lo[ax][0]; a <= r.hi[ax][0]; a++) {
+            double o[3];
+            const int64_t line = line_origin(g, ax, a, b, base1, base2, o);
+            if (!line_hit_test(o, ax, tr.v, tr.v + 3, tr.v + 6, nullptr)) continue;
+            atomicAdd(&cnt[line], 1u);
+#pragma unroll
+            for (int k = 0; k < kEmitBatch; k++) ln[k] = nh == k ? (uint32_t)line : ln[k];
+            nh++;
+          }
     }
-    for (int ax = 0; ax < 3; ax++)
-      for (int32_t b = r.lo[ax][1]; b <= r.hi[ax][1]; b++)
-        for (int32_t a = r.lo[ax][0]; a <= r.hi[ax][0]; a++)
-          line_visit(tr, g, ax, a, b, base1, base2, mode, cnt, off, hits, cap);
+    hitn[t] = nh > kEmitBatch ? kHitsMany : (uint8_t)nh;
+    if (nh > 0 && nh <= kEmitBatch) {
+#pragma unroll
+      for (int k = 0; k < kEmitBatch; k++) hitl[kEmitBatch * t + k] = ln[k];
+    }
   }
 }
 
-// Emit pass for triangles with at most kSmallMax candidate lines: the
-// hits are found first (up to kEmitBatch line ids in registers), then their
-// cursor atomics are issued together and waited for once, then the records
-// are written.  A triangle with more hits takes the one-at-a-time path.
-constexpr int kEmitBatch = 4;
-
 __global__ void line_emit_small_kernel(const Tri64 *__restrict__ tri, int64_t nel, GridP g,
                                        int64_t base1, int64_t base2,
+                                       const uint8_t *__restrict__ hitn,
+                                       const uint32_t *__restrict__ hitl,
                                        uint32_t *__restrict__ cursor,
                                        const uint64_t *__restrict__ off,
                                        LineHit *__restrict__ hits, uint64_t cap) {
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nel;
        t += (int64_t)gridDim.x * blockDim.x) {
+    const int nh = hitn[t];
+    if (nh == 0) continue;
     const Tri64 tr = tri[t];
-    const LineRange r = tri_line_range(tr, g);
-    const int64_t total = r.cnt[0] + r.cnt[1] + r.cnt[2];
-    if (total == 0 || total > kSmallMax) continue;
-    int nh = 0;
-    int64_t ln[kEmitBatch] = {0, 0, 0, 0};
-    int lax[kEmitBatch] = {0, 0, 0, 0};
-    for (int ax = 0; ax < 3; ax++)
-      for (int32_t b = r.lo[ax][1]; b <= r.hi[ax][1]; b++)
-        for (int32_t a = r.lo[ax][0]; a <= r.hi[ax][0]; a++) {
-          double o[3];
-          const int64_t line = line_origin(g, ax, a, b, base1, base2, o);
-          if (!line_hit_test(o, ax, tr.v, tr.v + 3, tr.v + 6, nullptr)) continue;
-#pragma unroll
-          for (int k = 0; k < kEmitBatch; k++) {
-            ln[k] = nh == k ? line : ln[k];
-            lax[k] = nh == k ? ax : lax[k];
-          }
-          nh++;
-        }
-    if (nh > kEmitBatch) {
+    if (nh == kHitsMany) {
+      const LineRange r = tri_line_range(tr, g);
       for (int ax = 0; ax < 3; ax++)
         for (int32_t b = r.lo[ax][1]; b <= r.hi[ax][1]; b++)
           for (int32_t a = r.lo[ax][0]; a <= r.hi[ax][0]; a++)
             line_visit(tr, g, ax, a, b, base1, base2, 1, cursor, off, hits, cap);
       continue;
     }
-    uint32_t slot[kEmitBatch];
+    uint32_t ln[kEmitBatch], slot[kEmitBatch];
+#pragma unroll
+    for (int k = 0; k < kEmitBatch; k++) ln[k] = hitl[kEmitBatch * t + k];
 #pragma unroll
     for (int k = 0; k < kEmitBatch; k++) slot[k] = k < nh ? atomicAdd(&cursor[ln[k]], 1u) : 0u;
 #pragma unroll
     for (int k = 0; k < kEmitBatch; k++) {
       if (k < nh) {
+        const int ax = ln[k] < base1 ? 0 : (ln[k] < base2 ? 1 : 2);
         LineHit rec;
-        line_hit_record(lax[k], tr.v, tr.v + 3, tr.v + 6, &rec);
+        line_hit_record(ax, tr.v, tr.v + 3, tr.v + 6, &rec);
         const uint64_t pos = off[ln[k]] + slot[k];
         if (pos < cap) hits[pos] = rec;  // beyond: overflow, reported by the sizes
       }
@@ -890,6 +895,7 @@ void line_hits_begin(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
   lb.nly = (int64_t)g.nx * g.nz;
   const int64_t nlz = (int64_t)g.nx * g.ny;
   lb.nlines = lb.nlx + lb.nly + nlz;
+  if (lb.nlines >= (int64_t)0xFFFFFFFFll) throw Error{-1, "too many grid lines"};
   b.line_base[0] = 0;
   b.line_base[1] = lb.nlx;
   b.line_base[2] = lb.nlx + lb.nly;
@@ -901,11 +907,11 @@ void line_hits_begin(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
   launch_fill_u32(s, lb.lcnt, lb.nlines + 1, 0u);
   launch_fill_u32(s, lb.nbig, 1, 0u);
   if (nel > 0) {
-    hipLaunchKernelGGL(line_bin_small_kernel, dim3(capped_grid(nel, 256, lb.max_blocks)),
-                       dim3(256), 0, s,
-                       tri, nel, g, lb.nlx, lb.nlx + lb.nly, 0, lb.lcnt,
-                       (const uint64_t *)nullptr, (LineHit *)nullptr, (uint64_t)0, lb.big,
-                       lb.nbig);
+    lb.hitn = ar.get<uint8_t>("line_hitn", nel + 1);
+    lb.hitl = ar.get<uint32_t>("line_hitl", (size_t)kEmitBatch * nel + 1);
+    hipLaunchKernelGGL(line_count_small_kernel, dim3(capped_grid(nel, 256, lb.max_blocks)),
+                       dim3(256), 0, s, tri, nel, g, lb.nlx, lb.nlx + lb.nly, lb.lcnt, lb.big,
+                       lb.nbig, lb.hitn, lb.hitl);
     hipLaunchKernelGGL(line_bin_big_kernel, dim3(1024), dim3(256), 0, s, tri, g, lb.nlx,
                        lb.nlx + lb.nly, 0, lb.lcnt, (const uint64_t *)nullptr,
                        (LineHit *)nullptr, (uint64_t)0, (const uint32_t *)lb.big,
@@ -923,7 +929,8 @@ void line_hits_finish(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
   if (cap > 0 && nel > 0) {
     launch_fill_u32(s, lb.lcnt, lb.nlines + 1, 0u);  // reused as the cursor
     hipLaunchKernelGGL(line_emit_small_kernel, dim3(capped_grid(nel, 256, lb.max_blocks)),
-                       dim3(256), 0, s, tri, nel, g, lb.nlx, lb.nlx + lb.nly, lb.lcnt,
+                       dim3(256), 0, s, tri, nel, g, lb.nlx, lb.nlx + lb.nly,
+                       (const uint8_t *)lb.hitn, (const uint32_t *)lb.hitl, lb.lcnt,
                        (const uint64_t *)lb.loff, hits, (uint64_t)cap);
     hipLaunchKernelGGL(line_bin_big_kernel, dim3(1024), dim3(256), 0, s, tri, g, lb.nlx,
                        lb.nlx + lb.nly, 1, lb.lcnt, (const uint64_t *)lb.loff, hits,
