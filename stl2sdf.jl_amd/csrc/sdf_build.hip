@@ -587,11 +587,17 @@ __global__ void line_prep_kernel(GridP g, const uint64_t *__restrict__ line_off,
       li.c[q] = 0.0;
       li.delta[q] = 0.f;
       li.cid[q] = 0;
+      li.seg[q] = 0;
     }
-    li.pad_[0] = li.pad_[1] = 0u;
+    // sign-brick segments along the line (kSegLen points each)
+    const int lax = L < base1 ? 0 : (L < base2 ? 1 : 2);
+    const int32_t lnax = lax == 0 ? g.nx : (lax == 1 ? g.ny : g.nz);
+    const int seglen = kSegLen[lax];
+    const int64_t nseg = ((int64_t)lnax + seglen - 1) / seglen;
+    const uint32_t segflag = nseg <= 255 ? kLineSeg : 0u;
     if (n == 0) {
       meta[L] = 0u;
-      li.info = 0u;
+      li.info = segflag;
       inl[L] = li;
       continue;
     }
@@ -707,6 +713,15 @@ __global__ void line_prep_kernel(GridP g, const uint64_t *__restrict__ line_off,
           cid[q] = (uint8_t)cl;
         }
       }
+      // the segments each crossing leaves uniform (below a: crossing above
+      // them; from b on: crossing below them), by the comparisons the sign
+      // kernel would make, found by bisection (they are monotone in s)
+      const double oa = lax == 0 ? g.ox : (lax == 1 ? g.oy : g.oz);
+      auto seg_lo = [&](int64_t q) { return lattice_coord(oa, g.h, q * seglen); };
+      auto seg_hi = [&](int64_t q) {
+        const int64_t e = q * seglen + seglen - 1;
+        return lattice_coord(oa, g.h, e < lnax - 1 ? e : lnax - 1);
+      };
 #pragma unroll
       for (int q = 0; q < kInlineHits; q++) {
         if (q < n) {
@@ -720,6 +735,28 @@ __global__ void line_prep_kernel(GridP g, const uint64_t *__restrict__ line_off,
             li.c[q] = cs[q];
             li.delta[q] = ds[q];
             li.cid[q] = cid[q];
+            if (segflag) {
+              const double D = (double)ds[q];
+              int64_t lo = 0, hi = nseg;
+              while (lo < hi) {
+                const int64_t mid = (lo + hi) / 2;
+                if (cs[q] - seg_hi(mid) > kRayEps + D)
+                  lo = mid + 1;
+                else
+                  hi = mid;
+              }
+              const int64_t sa = lo;
+              lo = 0;
+              hi = nseg;
+              while (lo < hi) {
+                const int64_t mid = (lo + hi) / 2;
+                if (cs[q] - seg_lo(mid) < -kRayEps - D)
+                  hi = mid;
+                else
+                  lo = mid + 1;
+              }
+              li.seg[q] = (uint16_t)(sa | (lo << 8));
+            }
           }
         }
       }
@@ -750,7 +787,8 @@ __global__ void line_prep_kernel(GridP g, const uint64_t *__restrict__ line_off,
       }
     }
     meta[L] = ok ? (cl + 1u) : kLineExact;
-    li.info = (uint32_t)(n < 255 ? n : 255) | ((cl + 1u) & 0xFFu) << 8 | (ok ? 0u : kLineExact);
+    li.info = (uint32_t)(n < 255 ? n : 255) | ((cl + 1u) & 0xFFu) << 8 | (ok ? 0u : kLineExact) |
+              segflag;
     inl[L] = li;
   }
 }

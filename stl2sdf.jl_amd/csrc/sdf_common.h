@@ -195,14 +195,24 @@ constexpr int kLineFastMax = 64;             // lines with more hits: exact path
 
 // Inline copy of a short line's summary (up to 4 hits) indexed directly by
 // line id: one 64-byte load instead of offsets + records.  info: bits 0-7
-// hit count (255 = more than 254), 8-15 cluster count, bit 31 kLineExact.
+// hit count (255 = more than 254), 8-15 cluster count, bit 30 kLineSeg,
+// bit 31 kLineExact.
+//
+// Sign bricks cut a line into segments of kSegLen[axis] points (segment s
+// = points [s L, min(s L + L, n) - 1]).  With kLineSeg, seg[h] = a | b << 8
+// says where crossing h stands for every segment: for s < a it lies above
+// the segment (c - hi_s > eps + delta), for s >= b below it (c - lo_s <
+// -eps - delta), for a <= s < b neither (the segment is not uniform).  The
+// rounded differences are monotone in s, so these are the same decisions
+// the per-segment comparisons make.
 constexpr int kInlineHits = 4;
+constexpr uint32_t kLineSeg = 0x40000000u;
 struct __attribute__((aligned(16))) LineInline {
   double c[kInlineHits];
   float delta[kInlineHits];
   uint8_t cid[kInlineHits];
   uint32_t info;
-  uint32_t pad_[2];
+  uint16_t seg[kInlineHits];
 };
 
 // t for origin o on the +axis ray of a recorded line hit (reference order).
@@ -441,6 +451,8 @@ __device__ inline int64_t chunk_count(int64_t cap, const uint64_t *dev) {
 
 // Sign bricks: 8 x 8 lattice columns x kSignZ points per 64-lane wave.
 constexpr int kSignZ = 4;
+// Points per sign-brick segment of an x-, y- and z-line.
+constexpr int kSegLen[3] = {8, 8, kSignZ};
 
 struct SignDims {
   int32_t sx, sy, sz;

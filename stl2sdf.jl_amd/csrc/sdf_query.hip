@@ -126,6 +126,30 @@ __device__ inline bool inline_uniform(const LineInline &r, double lo, double hi,
   return usable & uni;
 }
 
+// inline_uniform for segment s of the line from the crossings' segment
+// ranges (kLineSeg): the same decisions by integer compares.
+__device__ inline bool seg_uniform(const LineInline &r, int s, int *pos, int *neg) {
+  const uint32_t n = r.info & 0xFFu;
+  const bool usable = !(r.info & kLineExact) && n <= (uint32_t)kInlineHits;
+  const int ncl = (int)((r.info >> 8) & 0xFFu);
+  int first_pos = -1, last_neg = -1;
+  bool uni = true;
+#pragma unroll
+  for (int h2 = 0; h2 < kInlineHits; h2++) {
+    const bool valid = h2 < (int)n;
+    const int cid = r.cid[h2];
+    const int a = r.seg[h2] & 0xFF, b = r.seg[h2] >> 8;
+    const bool above = s < a;
+    const bool below = !above & (s >= b);
+    first_pos = (valid & above & (first_pos < 0)) ? cid : first_pos;
+    last_neg = (valid & below) ? cid : last_neg;
+    uni = uni & (!valid | above | below);
+  }
+  *pos = first_pos >= 0 ? ((ncl - first_pos) & 1) : 0;
+  *neg = last_neg >= 0 ? ((last_neg + 1) & 1) : 0;
+  return usable & uni;
+}
+
 // LDS of one distance workgroup: the chunk's triangle records, the
 // (row, triangle) jobs of phase 2 and the per-point minima.
 constexpr int kTileRows = kTileY * kTileZ;
@@ -556,15 +580,20 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S,
     const LineInline r = inl[valid ? L : 0];
     LineInline *dst = xl ? &S.lx[q][w] : &S.ly[q][w];
     *dst = r;
-    const double o = xl ? g.ox : g.oy;
     int p, n;
-    const bool uni = inline_uniform(r, lattice_coord(o, g.h, b0),
-                                    lattice_coord(o, g.h, min(b0 + 7, along - 1)), &p, &n);
+    const bool uni =
+        (r.info & kLineSeg)
+            ? seg_uniform(r, xl ? bx : by, &p, &n)
+            : inline_uniform(r, lattice_coord(xl ? g.ox : g.oy, g.h, b0),
+                             lattice_coord(xl ? g.ox : g.oy, g.h, min(b0 + 7, along - 1)), &p,
+                             &n);
     const uint8_t u = valid && uni ? (uint8_t)(1 | (p << 1) | (n << 2)) : (uint8_t)0;
     *(xl ? &S.ux[q][w] : &S.uy[q][w]) = u;
     if (in_grid)
-      zuni = inline_uniform(rz, lattice_coord(g.oz, g.h, k0),
-                            lattice_coord(g.oz, g.h, k0 + nk - 1), &zp, &zn);
+      zuni = (rz.info & kLineSeg) ? seg_uniform(rz, bz, &zp, &zn)
+                                  : inline_uniform(rz, lattice_coord(g.oz, g.h, k0),
+                                                   lattice_coord(g.oz, g.h, k0 + nk - 1), &zp,
+                                                   &zn);
     // a staged line outside the grid (or a layer beyond nz) serves no point
     lines_shared = (!valid | uni) & (!in_grid | zuni);
   }
