@@ -128,12 +128,14 @@ __global__ void morton_kernel(const float *__restrict__ box, int64_t n,
   }
 }
 
-// Leaf boxes and Morton codes in one pass, quantised over a box given by
-// the caller (a lattice's box, which covers the mesh it was built around;
-// codes of centres outside it are clamped, which only coarsens the tree).
-__global__ void leaf_morton_kernel(const Tri64 *__restrict__ tri, int64_t n, QBox qb,
+// Leaf boxes and Morton codes (bits per axis: 8 or 10) in one pass,
+// quantised over a box given by the caller (a lattice's box, which covers
+// the mesh it was built around; codes of centres outside it are clamped,
+// which only coarsens the tree).
+__global__ void leaf_morton_kernel(const Tri64 *__restrict__ tri, int64_t n, QBox qb, int bits,
                                    float *__restrict__ box, uint32_t *__restrict__ keys,
                                    uint32_t *__restrict__ vals) {
+  const float cells = (float)(1u << bits);
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n;
        t += (int64_t)gridDim.x * blockDim.x) {
     const Tri64 r = tri[t];
@@ -144,8 +146,8 @@ __global__ void leaf_morton_kernel(const Tri64 *__restrict__ tri, int64_t n, QBo
       const float hi = __double2float_ru(fmax(a, fmax(b, c)));
       box[6 * t + d] = lo;
       box[6 * t + 3 + d] = hi;
-      float x = (0.5f * (lo + hi) - qb.lo[d]) / qb.ext[d] * 1024.0f;
-      x = fminf(fmaxf(x, 0.0f), 1023.0f);
+      float x = (0.5f * (lo + hi) - qb.lo[d]) / qb.ext[d] * cells;
+      x = fminf(fmaxf(x, 0.0f), cells - 1.0f);
       q[d] = (uint32_t)x;
     }
     keys[t] = (expand10(q[0]) << 2) | (expand10(q[1]) << 1) | expand10(q[2]);
@@ -543,9 +545,13 @@ Bvh build_lbvh(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t n, const QBox
   b.leaf_box = ar.get<float>("bvh_leaf_box", 6 * n);
   uint32_t *codes = ar.get<uint32_t>("bvh_codes", n);
   uint32_t *leaf_tri = ar.get<uint32_t>("bvh_leaf_tri", n);
+  // the ray hierarchy alone groups runs of kRayLeaf triangles, for which
+  // 8 bits per axis (3 sort passes) order space finely enough; nearest-point
+  // queries use the per-triangle hierarchy, with 10
+  const int bits = qbox && !full ? 8 : 10;
   if (qbox) {
     hipLaunchKernelGGL(leaf_morton_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s, tri, n,
-                       *qbox, b.leaf_box, codes, leaf_tri);
+                       *qbox, bits, b.leaf_box, codes, leaf_tri);
     SDF_HIP(hipGetLastError());
   } else {
     uint32_t *cb = ar.get<uint32_t>("bvh_cbounds", 8);
@@ -559,7 +565,7 @@ Bvh build_lbvh(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t n, const QBox
                        b.leaf_box, n, cb, codes, leaf_tri);
     SDF_HIP(hipGetLastError());
   }
-  radix_sort_pairs(s, ar, codes, leaf_tri, n, 30, nullptr);
+  radix_sort_pairs(s, ar, codes, leaf_tri, n, 3 * bits, nullptr);
   b.leaf_tri = leaf_tri;
   b.nc = (n + kRayLeaf - 1) / kRayLeaf;
   if (n == 1) return b;
