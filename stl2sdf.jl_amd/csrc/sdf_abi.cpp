@@ -133,6 +133,7 @@ struct sdf_ctx {
   int dist_per_cu = 10, fused_per_cu = 0;
   int lines_early = 0;  // 1: the line chain starts with the tile chain; 2: its count pass
   int line_per_cu = 0;  // workgroups per CU of the line kernels (0: no cap)
+  int lbvh_early = 0;   // 1: the LBVH build starts with the tile chain
   int ncu = 256;
   // timing
   bool timing = false;
@@ -247,9 +248,9 @@ const sdf::Bvh &need_bvh(sdf_ctx *c) {
 // end of the current pass's tile chain); joined later through join[2].
 // Morton codes are quantised over the lattice's box (it covers the mesh the
 // grid was built around; any box gives a valid hierarchy).
-void bvh_async(sdf_ctx *c, int q, const sdf::GridP &g, bool full) {
+void bvh_async(sdf_ctx *c, int q, const sdf::GridP &g, bool full, int after = 3) {
   if (c->bvh_ready && (c->bvh_full || !full)) return;
-  SDF_HIP(hipStreamWaitEvent(c->aux[q], c->join[3], 0));
+  SDF_HIP(hipStreamWaitEvent(c->aux[q], c->join[after], 0));
   c->ar.scope = "bvh/";
   sdf::QBox qb;
   const double o[3] = {g.ox, g.oy, g.oz};
@@ -397,6 +398,9 @@ void stage_bins_tiles(sdf_ctx *c, LatticeRun &L) {
   // beside the latency-bound tile chain they stretch every small kernel
   // (measured: about 35 us per step)
   SDF_HIP(hipEventRecord(c->join[3], c->stream));
+  // issued after the tile chain (so that chain's launches are not delayed)
+  // but free to start beside it
+  if (L.do_sign && c->lbvh_early) bvh_async(c, 1, L.g, L.learned, 0);
   L.cap = c->caps.pend ? c->caps.pend
                        : (uint32_t)(npts < (1 << 22) ? (npts > 0 ? npts : 1) : (1 << 22));
   L.cap0 = c->caps.pend0 ? c->caps.pend0
@@ -454,8 +458,8 @@ void stage_dist(sdf_ctx *c, LatticeRun &L) {
                          (float)c->mlp_delta, r2);
     c->mark("dist_learned");
     sdf::launch_dist_chunks_learned(c->stream, g, c->tri, bins, r2, L.d2buf, list, c->lstats);
-    if (!c->bvh_ready || !c->bvh_full) {  // the fallback queries the per-triangle one
-      bvh_async(c, 1, g, true);
+    if (!L.bvh_was_ready || !c->bvh_full) {  // the fallback queries the per-triangle one
+      bvh_async(c, 1, g, true);  // (issued already when it started with the tile chain)
       SDF_HIP(hipStreamWaitEvent(c->stream, c->join[2], 0));
     }
     c->mark("fallback");
@@ -1099,6 +1103,9 @@ sdf_status sdf_ctx_tune(sdf_ctx *c, const char *name, int64_t value) {
     } else if (n == "ray_stack") {
       if (value != 1024 && value != 2048) fail(SDF_ERR_ARG, "ray_stack must be 1024 or 2048");
       c->ray_stack = (int)value;
+    } else if (n == "lbvh_early") {
+      if (value != 0 && value != 1) fail(SDF_ERR_ARG, "lbvh_early must be 0 or 1");
+      c->lbvh_early = (int)value;
     } else if (n == "lines_early") {
       if (value < 0 || value > 2) fail(SDF_ERR_ARG, "lines_early must be 0, 1 or 2");
       c->lines_early = (int)value;
