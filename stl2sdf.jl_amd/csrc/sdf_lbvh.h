@@ -14,6 +14,19 @@ struct BvhNode {
   int32_t pad_[2];
 };
 
+// Four-wide view of a binary hierarchy: wide node i holds the children of
+// binary node i's internal children and its leaf children as they are, so
+// one 128-byte fetch tests up to four boxes and a traversal from the root
+// visits every other level.  child: >= 0 node index, kNoChild for an empty
+// slot, else a leaf reference as in the binary nodes.
+constexpr int32_t kNoChild = INT32_MIN;
+
+struct alignas(128) BvhWide {
+  float box[4][6];
+  int32_t child[4];
+  int32_t pad_[4];
+};
+
 // Triangles per leaf of the ray hierarchy.
 constexpr int kRayLeaf = 8;
 
@@ -27,6 +40,7 @@ struct Bvh {
   // (root 0) whose leaf children are -(c+1).  nc == 1: no nodes.
   int64_t nc = 0;
   BvhNode *cnodes = nullptr;
+  BvhWide *cwide = nullptr;  // four-wide view of cnodes (nc - 1 nodes)
 };
 
 // Morton quantisation box: lo corner and extent per axis (extent > 0).

@@ -483,6 +483,38 @@ __global__ void run_node_boxes_kernel(const float *__restrict__ seg, SegLevels L
   }
 }
 
+// Wide node i from binary node i: the two children of each internal child
+// and each leaf child as it is; unused slots get an empty box.  Four lanes
+// per node, one slot each, so a wave writes 16 whole nodes.
+__global__ void wide_nodes_kernel(const BvhNode *__restrict__ nodes, int64_t nn,
+                                  BvhWide *__restrict__ wide) {
+  const int64_t total = 4 * nn;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = t >> 2;
+    const int slot = (int)(t & 3);
+    const int32_t c0 = nodes[i].child[0], c1 = nodes[i].child[1];
+    const int n0 = c0 >= 0 ? 2 : 1, n1 = c1 >= 0 ? 2 : 1;
+    const float *src = nullptr;
+    int32_t ch = kNoChild;
+    if (slot < n0) {
+      src = c0 >= 0 ? nodes[c0].box[slot] : nodes[i].box[0];
+      ch = c0 >= 0 ? nodes[c0].child[slot] : c0;
+    } else if (slot < n0 + n1) {
+      const int k = slot - n0;
+      src = c1 >= 0 ? nodes[c1].box[k] : nodes[i].box[1];
+      ch = c1 >= 0 ? nodes[c1].child[k] : c1;
+    }
+    float *dst = wide[i].box[slot];
+    for (int q = 0; q < 3; q++) {
+      dst[q] = src ? src[q] : __builtin_huge_valf();
+      dst[3 + q] = src ? src[3 + q] : -__builtin_huge_valf();
+    }
+    wide[i].child[slot] = ch;
+    wide[i].pad_[slot] = 0;
+  }
+}
+
 // Sorted pairs end in (keys, vals), or, when keys_out is given, in the pair
 // of buffers the last pass wrote (*keys_out, *vals_out; no copy back).
 void radix_sort_pairs(hipStream_t s, Arena &ar, uint32_t *keys, uint32_t *vals,
@@ -598,6 +630,9 @@ Bvh build_lbvh(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t n, const QBox
                        (const uint32_t *)codes, (int64_t)kRayLeaf, b.nc, b.cnodes, crange);
     hipLaunchKernelGGL(run_node_boxes_kernel, dim3(blocks_for(b.nc - 1, 256)), dim3(256), 0, s,
                        (const float *)seg, L, (const int32_t *)crange, b.nc, n, b.cnodes);
+    b.cwide = ar.get<BvhWide>("bvh_cwide", b.nc - 1);
+    hipLaunchKernelGGL(wide_nodes_kernel, dim3(blocks_for(4 * (b.nc - 1), 256)), dim3(256), 0,
+                       s, (const BvhNode *)b.cnodes, b.nc - 1, b.cwide);
     SDF_HIP(hipGetLastError());
   }
   if (full) {

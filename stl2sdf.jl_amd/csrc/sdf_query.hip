@@ -882,10 +882,11 @@ __device__ inline const Pending &slot_item(const PendLists &L, uint64_t slot) {
 }
 
 // Wavefront-cooperative ray parity: one wavefront per (item, ray) over the
-// ray hierarchy (leaves are runs of kRayLeaf sorted triangles).  The 64
-// lanes pop up to 64 nodes at a time from an LDS stack and test both
-// children (FP64 slab test on boxes rounded outward); internal children
-// are pushed and reached runs queued with ballot prefix sums; in the same
+// four-wide view of the ray hierarchy (leaves are runs of kRayLeaf sorted
+// triangles).  The 64 lanes pop up to 16 wide nodes at a time from an LDS
+// stack, four lanes per node, and test one child box each (FP64 slab test
+// on boxes rounded outward); internal children are pushed and reached runs
+// queued with ballot prefix sums; in the same
 // iteration up to 64 / kRayLeaf queued runs are tested, kRayLeaf lanes per
 // run, with the reference's FP64 Moller-Trumbore.  Hits go to an LDS list; the unique count follows the
 // reference rule (sorted, consecutive gap > eps) computed as: a value counts
@@ -947,36 +948,29 @@ __global__ __launch_bounds__(64) void ray_wave_kernel(
       }
     }
     __syncthreads();
-    // Each iteration: up to 64 nodes of the stack (a lane per node, both
-    // children's boxes), then up to 64 / kRayLeaf queued runs (kRayLeaf
-    // lanes per run, a triangle per lane).
+    // Each iteration: up to 16 wide nodes of the stack (a lane per child
+    // box), then up to 64 / kRayLeaf queued runs (kRayLeaf lanes per run, a
+    // triangle per lane).
     while (true) {
       const int top = s_top, nq = s_nq;
       if ((top == 0 && nq == 0) || s_ovf) break;
-      const int take = top < 64 ? top : 64;
+      const int take = top < 16 ? top : 16;
       const int base = top - take;
-      const int32_t node = lane < take ? stack[base + lane] : -1;
+      const int32_t node = (lane >> 2) < take ? stack[base + (lane >> 2)] : -1;
       constexpr int kRunsPerStep = 64 / kRayLeaf;
       const int qtake = nq < kRunsPerStep ? nq : kRunsPerStep;
       const int qbase = nq - qtake;
       const int32_t run = lane / kRayLeaf < qtake ? runq[qbase + lane / kRayLeaf] : -1;
       __syncthreads();
-      int32_t push0 = -1, push1 = -1, leaf0 = -1, leaf1 = -1;
+      int32_t push = -1, leaf = -1;
       if (node >= 0) {
-        const BvhNode nd = bvh.cnodes[node];
-        for (int c = 0; c < 2; c++) {
-          if (!ray_box(o, d, inv, nd.box[c])) continue;
-          const int32_t ch = nd.child[c];
-          if (ch < 0) {
-            if (leaf0 < 0)
-              leaf0 = -(ch + 1);
-            else
-              leaf1 = -(ch + 1);
-          } else if (push0 < 0) {
-            push0 = ch;
-          } else {
-            push1 = ch;
-          }
+        const BvhWide &nd = bvh.cwide[node];
+        const int32_t ch = nd.child[lane & 3];
+        if (ch != kNoChild && ray_box(o, d, inv, nd.box[lane & 3])) {
+          if (ch < 0)
+            leaf = -(ch + 1);
+          else
+            push = ch;
         }
       }
       if (run >= 0) {
@@ -993,21 +987,14 @@ __global__ __launch_bounds__(64) void ray_wave_kernel(
           }
         }
       }
-      const uint64_t b1 = __ballot(push0 >= 0), b2 = __ballot(push1 >= 0);
-      const int pos = base + __popcll(b1 & below) + __popcll(b2 & below);
-      const int ntop = base + __popcll(b1) + __popcll(b2);
-      const uint64_t c1 = __ballot(leaf0 >= 0), c2 = __ballot(leaf1 >= 0);
-      const int qpos = qbase + __popcll(c1 & below) + __popcll(c2 & below);
-      const int nnq = qbase + __popcll(c1) + __popcll(c2);
+      const uint64_t b1 = __ballot(push >= 0), c1 = __ballot(leaf >= 0);
+      const int ntop = base + __popcll(b1);
+      const int nnq = qbase + __popcll(c1);
       if (ntop > kWaveStack || nnq > kRunQueue) {
         if (lane == 0) s_ovf = 1;
       } else {
-        int p = pos;
-        if (push0 >= 0) stack[p++] = push0;
-        if (push1 >= 0) stack[p] = push1;
-        p = qpos;
-        if (leaf0 >= 0) runq[p++] = leaf0;
-        if (leaf1 >= 0) runq[p] = leaf1;
+        if (push >= 0) stack[base + __popcll(b1 & below)] = push;
+        if (leaf >= 0) runq[qbase + __popcll(c1 & below)] = leaf;
       }
       __syncthreads();
       if (lane == 0) {
