@@ -215,6 +215,22 @@ struct __attribute__((aligned(16))) LineInline {
   uint16_t seg[kInlineHits];
 };
 
+// The integer tail of a LineInline (cid, info, seg: its last 16 bytes), all
+// a kLineSeg uniformity test reads, in one 16-byte load.
+struct LineTail {
+  uint32_t cid4;          // cid[h] in byte h
+  uint32_t info;
+  uint32_t seg01, seg23;  // seg[0] | seg[1] << 16, seg[2] | seg[3] << 16
+};
+static_assert(sizeof(LineInline) == 64 && __builtin_offsetof(LineInline, cid) == 48 &&
+                  __builtin_offsetof(LineInline, info) == 52 &&
+                  __builtin_offsetof(LineInline, seg) == 56,
+              "LineTail mirrors the last 16 bytes of LineInline");
+__device__ inline LineTail load_line_tail(const LineInline *p) {
+  const uint4 v = *reinterpret_cast<const uint4 *>(reinterpret_cast<const char *>(p) + 48);
+  return LineTail{v.x, v.y, v.z, v.w};
+}
+
 // t for origin o on the +axis ray of a recorded line hit (reference order).
 __host__ __device__ inline double line_hit_t(const LineHit &r, const double *o) {
 #pragma clang fp contract(off)

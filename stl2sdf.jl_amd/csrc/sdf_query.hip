@@ -128,7 +128,7 @@ __device__ inline bool inline_uniform(const LineInline &r, double lo, double hi,
 
 // inline_uniform for segment s of the line from the crossings' segment
 // ranges (kLineSeg): the same decisions by integer compares.
-__device__ inline bool seg_uniform(const LineInline &r, int s, int *pos, int *neg) {
+__device__ inline bool seg_uniform(const LineTail &r, int s, int *pos, int *neg) {
   const uint32_t n = r.info & 0xFFu;
   const bool usable = !(r.info & kLineExact) && n <= (uint32_t)kInlineHits;
   const int ncl = (int)((r.info >> 8) & 0xFFu);
@@ -137,8 +137,9 @@ __device__ inline bool seg_uniform(const LineInline &r, int s, int *pos, int *ne
 #pragma unroll
   for (int h2 = 0; h2 < kInlineHits; h2++) {
     const bool valid = h2 < (int)n;
-    const int cid = r.cid[h2];
-    const int a = r.seg[h2] & 0xFF, b = r.seg[h2] >> 8;
+    const int cid = (int)((r.cid4 >> (8 * h2)) & 0xFFu);
+    const uint32_t sg = ((h2 < 2 ? r.seg01 : r.seg23) >> (16 * (h2 & 1))) & 0xFFFFu;
+    const int a = (int)(sg & 0xFFu), b = (int)(sg >> 8);
     const bool above = s < a;
     const bool below = !above & (s >= b);
     first_pos = (valid & above & (first_pos < 0)) ? cid : first_pos;
@@ -559,39 +560,41 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S,
   const bool in_grid = i < g.nx && j < g.ny;
   const int nk = min(kSignZ, g.nz - k0);
 
-  // ---- stage: the lane's own z-line summary (registers), the brick's x- and
-  // y-line summaries and its tiles' distance slots (LDS)
-  LineInline rz;
+  // ---- stage: the uniformity of the lane's own z-line and of one of the
+  // brick's x- or y-lines from their summaries' integer tails, then (only for
+  // a brick with a line that is not uniform) the full x- and y-line summaries
+  // and, always, its tiles' distance slots (LDS)
   bool zuni = false;  // the lane's z-line is the same for its nk points
   int zp = 0, zn = 0;
   // every line this lane staged or owns is shared by the brick's points
   bool lines_shared = true;
+  // lanes 0-31 stage the x-lines (k0 + sq, 8 by + sw), lanes 32-63 the
+  // y-lines (k0 + sq, 8 bx + sw): one code path, operands by select
+  const int sq = (lane >> 3) & 3, sw = lane & 7;
+  const bool xl = lane < 32;
+  int64_t Lxy = 0;  // the staged line (0 when it serves no point)
+  const int64_t Lz = in_grid ? base2 + (int64_t)j * g.nx + i : 0;
   if (do_sign) {
-    if (in_grid) rz = inl[base2 + (int64_t)j * g.nx + i];
-    // lanes 0-31 stage the x-lines (k0 + q, 8 by + w), lanes 32-63 the
-    // y-lines (k0 + q, 8 bx + w): one code path, operands by select
-    const int q = (lane >> 3) & 3, w = lane & 7, kk = k0 + q;
-    const bool xl = lane < 32;
-    const int32_t a = (xl ? by : bx) * 8 + w;  // the line's y (x-lines) or x (y-lines)
-    const int32_t b0 = (xl ? bx : by) * 8;     // first brick coordinate along the line
+    const int kk = k0 + sq;
+    const int32_t a = (xl ? by : bx) * 8 + sw;  // the line's y (x-lines) or x (y-lines)
+    const int32_t b0 = (xl ? bx : by) * 8;      // first brick coordinate along the line
     const int32_t along = xl ? g.nx : g.ny, across = xl ? g.ny : g.nx;
     const bool valid = kk < g.nz && a < across;
-    const int64_t L = (xl ? 0 : base1) + (int64_t)kk * across + a;
-    const LineInline r = inl[valid ? L : 0];
-    LineInline *dst = xl ? &S.lx[q][w] : &S.ly[q][w];
-    *dst = r;
+    Lxy = valid ? (xl ? 0 : base1) + (int64_t)kk * across + a : 0;
+    const LineTail r = load_line_tail(inl + Lxy);
+    const LineTail rz = load_line_tail(inl + Lz);
     int p, n;
     const bool uni =
         (r.info & kLineSeg)
             ? seg_uniform(r, xl ? bx : by, &p, &n)
-            : inline_uniform(r, lattice_coord(xl ? g.ox : g.oy, g.h, b0),
+            : inline_uniform(inl[Lxy], lattice_coord(xl ? g.ox : g.oy, g.h, b0),
                              lattice_coord(xl ? g.ox : g.oy, g.h, min(b0 + 7, along - 1)), &p,
                              &n);
     const uint8_t u = valid && uni ? (uint8_t)(1 | (p << 1) | (n << 2)) : (uint8_t)0;
-    *(xl ? &S.ux[q][w] : &S.uy[q][w]) = u;
+    *(xl ? &S.ux[sq][sw] : &S.uy[sq][sw]) = u;
     if (in_grid)
       zuni = (rz.info & kLineSeg) ? seg_uniform(rz, bz, &zp, &zn)
-                                  : inline_uniform(rz, lattice_coord(g.oz, g.h, k0),
+                                  : inline_uniform(inl[Lz], lattice_coord(g.oz, g.h, k0),
                                                    lattice_coord(g.oz, g.h, k0 + nk - 1), &zp,
                                                    &zn);
     // a staged line outside the grid (or a layer beyond nz) serves no point
@@ -600,6 +603,7 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S,
   // wave-uniform shortcut: every line of the brick is shared by its points
   // (no crossing near the brick), so each point only adds six stored parities
   const bool brick_uni = __all(lines_shared);
+  if (do_sign && !brick_uni) *(xl ? &S.lx[sq][sw] : &S.ly[sq][sw]) = inl[Lxy];
   if (d2buf) {
     if (lane < 4) {
       const int32_t tyy = 2 * by + (lane & 1), tzz = 2 * bz + (lane >> 1);
@@ -694,8 +698,7 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S,
     if (do_sign && !zuni) {
       // z-line (i, j): one summary classifies all 4 points, else the line's
       // crossing list
-      if (!inline_parity<kSignZ>(rz, zq, zpos, zneg, zok)) {
-        const int64_t Lz = base2 + (int64_t)j * g.nx + i;
+      if (!inline_parity<kSignZ>(inl[Lz], zq, zpos, zneg, zok)) {
         const uint64_t hb = C.line_off[Lz], he = C.line_off[Lz + 1];
         const uint32_t mz = C.meta[Lz];
 #pragma unroll
@@ -748,10 +751,11 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S,
   __syncthreads();  // S is reused by the block's next brick
 }
 
-// Five waves per SIMD (92 registers, no spills to scratch): at six the
-// allocator spills to scratch and the kernel runs 4 % slower.
+// Six waves per SIMD (80 registers, no spills to scratch).  The kernel is
+// bound by the latency of its per-brick loads, so occupancy pays: 0.358 ms
+// at six against 0.370 ms at five on the star surface (82.6 M points).
 template <int kOut>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5)))
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6)))
 void fused_lattice_kernel(FusedJob J) {
   __shared__ FusedShared S;
   __shared__ FusedCold C;
