@@ -527,8 +527,6 @@ struct FusedShared {
   // per x- and y-line: 1 | pos << 1 | neg << 2 when both rays are the same
   // for all 8 points of the line in the brick (inline_uniform), else 0
   uint8_t ux[kSignZ][8], uy[kSignZ][8];
-  float d2[4][kTilePts];     // tiles (tz local, ty local) = (t >> 1, t & 1)
-  uint32_t slot[4];          // first work item + 1 of each tile, 0: no distance work
 };
 
 // Output sets of the sign kernel (bits); an instantiation keeps only the
@@ -548,7 +546,15 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S,
   const float *__restrict__ d2buf = J.d2buf;
   const LineInline *__restrict__ inl = J.inl;
   const int64_t base1 = J.base1, base2 = J.base2;
-  const int do_sign = J.do_sign;
+  // stages fixed at compile time for the specialised output sets (the
+  // launcher picks them only when the job matches): bit 0 distances (d2buf
+  // set), bit 1 signs; 0 = both read from the job
+  constexpr int kStages = kOut == kOutSdf                  ? 3
+                          : kOut == kOutDist               ? 1
+                          : kOut == (kOutSigns | kOutConf) ? 2
+                                                           : 0;
+  const bool has_d2 = kStages ? (kStages & 1) != 0 : d2buf != nullptr;
+  const int do_sign = kStages ? (kStages >> 1) : J.do_sign;
   const FusedOut out = J.out;
   const int64_t chunk_cap = J.chunk_cap;
   int32_t bx, by, bz;
@@ -560,29 +566,73 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S,
   const bool in_grid = i < g.nx && j < g.ny;
   const int nk = min(kSignZ, g.nz - k0);
 
-  // ---- stage: the uniformity of the lane's own z-line and of one of the
-  // brick's x- or y-lines from their summaries' integer tails, then (only for
-  // a brick with a line that is not uniform) the full x- and y-line summaries
-  // and, always, its tiles' distance slots (LDS)
+  // ---- issue the brick's loads: the integer tails of the lane's own z-line
+  // summary and of one x- or y-line summary together with the four distance
+  // tiles' offsets, then the lane's four squared distances.  Vector loads
+  // complete in issue order, so the line tails go first and the uniformity
+  // test below runs while the distances are in flight.
+  // Lanes 0-31 stage the x-lines (k0 + sq, 8 by + sw), lanes 32-63 the
+  // y-lines (k0 + sq, 8 bx + sw): one code path, operands by select.
+  const int sq = (lane >> 3) & 3, sw = lane & 7;
+  const bool xl = lane < 32;
+  const int32_t la = (xl ? by : bx) * 8 + sw;  // the line's y (x-lines) or x (y-lines)
+  const int32_t across = xl ? g.ny : g.nx;
+  const bool valid = k0 + sq < g.nz && la < across;
+  // the staged line (0 when it serves no point) and the lane's own z-line
+  const int64_t Lxy = valid ? (xl ? 0 : base1) + (int64_t)(k0 + sq) * across + la : 0;
+  const int64_t Lz = in_grid ? base2 + (int64_t)j * g.nx + i : 0;
+  LineTail r = {}, rz = {};
+  if (do_sign) {
+    r = load_line_tail(inl + Lxy);
+    rz = load_line_tail(inl + Lz);
+  }
+  // The lane's squared distances of points (i, j, k0 + q), straight from the
+  // brick's four distance tiles (8 x 4 x 2 points; tile t = 2 tz local + ty
+  // local).  A tile's slot (its first work item) is wave-uniform; for each q
+  // the wave reads two runs of 32 consecutive floats.
+  static_assert(kTileX == 8 && 2 * kTileY == 8 && 2 * kTileZ == kSignZ,
+                "a sign brick covers 1 x 2 x 2 distance tiles");
+  float best[kSignZ];
+#pragma unroll
+  for (int q = 0; q < kSignZ; q++) best[q] = __builtin_huge_valf();
+  if (has_d2) {
+    uint64_t c0[4], c1[4];
+    bool tok[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {  // no branches: the eight loads go out together
+      const int32_t tyy = 2 * by + (t & 1), tzz = 2 * bz + (t >> 1);
+      tok[t] = tyy < td.ty && tzz < td.tz;
+      const int64_t tile = tok[t] ? ((int64_t)tzz * td.ty + tyy) * td.tx + bx : 0;
+      c0[t] = chunk_off[tile];
+      c1[t] = chunk_off[tile + 1];
+    }
+    // a tile without distance work reads the spare tile of +inf after the
+    // last work item: the loads are unconditional and need no select
+    uint32_t slot[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+      slot[t] = tok[t] && c1[t] > c0[t] && (int64_t)c0[t] < chunk_cap ? (uint32_t)c0[t]
+                                                                       : (uint32_t)chunk_cap;
+    const int tyl = lane >> 5;  // the lane's y tile in the brick
+#pragma unroll
+    for (int q = 0; q < kSignZ; q++) {
+      const uint32_t sl = tyl ? slot[(q >> 1) * 2 + 1] : slot[(q >> 1) * 2];
+      const int tl = (lane & 7) + kTileX * (((lane >> 3) & 3) + kTileY * (q & 1));
+      best[q] = d2buf[(uint64_t)sl * kTilePts + tl];
+    }
+  }
+
+  // ---- uniformity of the lane's z-line over its nk points and of its staged
+  // line over the brick's 8 points on it, from the tails; the full x- and
+  // y-line summaries go to LDS only for a brick with a line that is not
+  // uniform
   bool zuni = false;  // the lane's z-line is the same for its nk points
   int zp = 0, zn = 0;
   // every line this lane staged or owns is shared by the brick's points
   bool lines_shared = true;
-  // lanes 0-31 stage the x-lines (k0 + sq, 8 by + sw), lanes 32-63 the
-  // y-lines (k0 + sq, 8 bx + sw): one code path, operands by select
-  const int sq = (lane >> 3) & 3, sw = lane & 7;
-  const bool xl = lane < 32;
-  int64_t Lxy = 0;  // the staged line (0 when it serves no point)
-  const int64_t Lz = in_grid ? base2 + (int64_t)j * g.nx + i : 0;
   if (do_sign) {
-    const int kk = k0 + sq;
-    const int32_t a = (xl ? by : bx) * 8 + sw;  // the line's y (x-lines) or x (y-lines)
-    const int32_t b0 = (xl ? bx : by) * 8;      // first brick coordinate along the line
-    const int32_t along = xl ? g.nx : g.ny, across = xl ? g.ny : g.nx;
-    const bool valid = kk < g.nz && a < across;
-    Lxy = valid ? (xl ? 0 : base1) + (int64_t)kk * across + a : 0;
-    const LineTail r = load_line_tail(inl + Lxy);
-    const LineTail rz = load_line_tail(inl + Lz);
+    const int32_t b0 = (xl ? bx : by) * 8;  // first brick coordinate along the line
+    const int32_t along = xl ? g.nx : g.ny;
     int p, n;
     const bool uni =
         (r.info & kLineSeg)
@@ -604,40 +654,14 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S,
   // (no crossing near the brick), so each point only adds six stored parities
   const bool brick_uni = __all(lines_shared);
   if (do_sign && !brick_uni) *(xl ? &S.lx[sq][sw] : &S.ly[sq][sw]) = inl[Lxy];
-  if (d2buf) {
-    if (lane < 4) {
-      const int32_t tyy = 2 * by + (lane & 1), tzz = 2 * bz + (lane >> 1);
-      uint32_t sl = 0;
-      if (tyy < td.ty && tzz < td.tz) {
-        const int64_t tile = ((int64_t)tzz * td.ty + tyy) * td.tx + bx;
-        const uint64_t c0 = chunk_off[tile], c1 = chunk_off[tile + 1];
-        if (c1 > c0 && (int64_t)c0 < chunk_cap) sl = (uint32_t)c0 + 1u;
-      }
-      S.slot[lane] = sl;
-    }
-    __syncthreads();
-    const int t = lane >> 4;
-    const uint32_t sl = S.slot[t];
-    float4 v = make_float4(__builtin_huge_valf(), __builtin_huge_valf(), __builtin_huge_valf(),
-                           __builtin_huge_valf());
-    if (sl) v = *reinterpret_cast<const float4 *>(d2buf + (uint64_t)(sl - 1u) * kTilePts +
-                                                   (lane & 15) * 4);
-    *reinterpret_cast<float4 *>(&S.d2[t][(lane & 15) * 4]) = v;
-  }
   __syncthreads();
 
-  // the point's distance: min(sqrt(best), h) of its distance slot
+  // the point's distance: min(sqrt(best), h), h when no triangle was near
   auto point_dist = [&](int q) -> double {
     double dist = g.h;
-    if (d2buf) {
-      const int k = k0 + q;
-      const int t = (q >> 1) * 2 + ((lane >> 3) >> 2);
-      const int tl = (i & (kTileX - 1)) + kTileX * ((j & (kTileY - 1)) + kTileY * (k & (kTileZ - 1)));
-      const float best = S.d2[t][tl];
-      if (best < __builtin_huge_valf()) {
-        const double d = sqrt((double)best);
-        dist = d < g.h ? d : g.h;
-      }
+    if (best[q] < __builtin_huge_valf()) {
+      const double d = sqrt((double)best[q]);
+      dist = d < g.h ? d : g.h;
     }
     return dist;
   };
@@ -1359,11 +1383,14 @@ void launch_fused_lattice(hipStream_t s, const GridP &g, const Tri64 *tri,
   const int64_t grid = max_blocks > 0 && max_blocks < J.nbricks ? max_blocks : J.nbricks;
   const int set = (out.sdf ? kOutSdf : 0) | (out.dist ? kOutDist : 0) |
                   (out.signs ? kOutSigns : 0) | (out.conf ? kOutConf : 0);
-  if (set == kOutSdf)
+  // the specialised sets fix their stages (fused_brick): sdf = distances +
+  // signs, dist = distances, signs + conf = signs; anything else runs the
+  // generic instantiation
+  if (set == kOutSdf && do_sign && d2buf)
     hipLaunchKernelGGL(fused_lattice_kernel<kOutSdf>, dim3((unsigned)grid), dim3(64), 0, s, J);
-  else if (set == kOutDist)
+  else if (set == kOutDist && !do_sign && d2buf)
     hipLaunchKernelGGL(fused_lattice_kernel<kOutDist>, dim3((unsigned)grid), dim3(64), 0, s, J);
-  else if (set == (kOutSigns | kOutConf))
+  else if (set == (kOutSigns | kOutConf) && do_sign && !d2buf)
     hipLaunchKernelGGL(fused_lattice_kernel<kOutSigns | kOutConf>, dim3((unsigned)grid),
                        dim3(64), 0, s, J);
   else
