@@ -521,6 +521,17 @@ __global__ __launch_bounds__(256) void learned_fallback_kernel(
 // LDS staging of one sign brick: its 32 x-line and 32 y-line summaries and
 // the distance slots of its 4 distance tiles, fetched by one round of
 // independent loads so each lane's per-point reads come from LDS.
+// LDS hand-off between the lanes of a one-wave workgroup (64 lanes): a
+// wave's LDS operations complete in order, so a wavefront-scope fence, which
+// keeps the compiler from moving memory operations across it, is all that is
+// needed.  A workgroup barrier's release fence would also wait for every
+// global store still in flight.
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 struct FusedShared {
   LineInline lx[kSignZ][8];  // x-line (k0 + q, 8 by + w)
   LineInline ly[kSignZ][8];  // y-line (k0 + q, 8 bx + w)
@@ -533,16 +544,20 @@ struct FusedShared {
 // pointers of its set in registers.
 enum : int { kOutSdf = 1, kOutDist = 2, kOutSigns = 4, kOutConf = 8, kOutAll = 15 };
 
-// One sign brick (8 x 8 x 4 points) of job J for the 64-lane block; m is
-// the brick's position in the brick launch order; C is the LDS copy of the
-// job's cold fields; outputs outside kOut are not written.  Uniform per
-// block (contains barriers).
+// One sign brick (8 x 8 x 4 points) of job J for the 64-lane block (one
+// wave: its LDS hand-offs are wave_lds_sync); m is the brick's position in
+// the brick launch order; C is the LDS copy of the job's cold fields;
+// outputs outside kOut are not written.  Uniform per block.
 template <int kOut>
 __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S,
                                    const FusedCold &C) {
   const GridP &g = J.g;
   const TileDims &td = J.td;
-  const uint64_t *__restrict__ chunk_off = J.chunk_off;
+  // the tile offsets are read-only here and a brick's are wave-uniform:
+  // through the constant address space they become scalar loads, whose wait
+  // does not also wait for the previous brick's vector stores
+  const __attribute__((address_space(4))) uint64_t *chunk_off_k =
+      (const __attribute__((address_space(4))) uint64_t *)J.chunk_off;
   const float *__restrict__ d2buf = J.d2buf;
   const LineInline *__restrict__ inl = J.inl;
   const int64_t base1 = J.base1, base2 = J.base2;
@@ -603,8 +618,8 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S,
       const int32_t tyy = 2 * by + (t & 1), tzz = 2 * bz + (t >> 1);
       tok[t] = tyy < td.ty && tzz < td.tz;
       const int64_t tile = tok[t] ? ((int64_t)tzz * td.ty + tyy) * td.tx + bx : 0;
-      c0[t] = chunk_off[tile];
-      c1[t] = chunk_off[tile + 1];
+      c0[t] = chunk_off_k[tile];
+      c1[t] = chunk_off_k[tile + 1];
     }
     // a tile without distance work reads the spare tile of +inf after the
     // last work item: the loads are unconditional and need no select
@@ -642,19 +657,25 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S,
                              &n);
     const uint8_t u = valid && uni ? (uint8_t)(1 | (p << 1) | (n << 2)) : (uint8_t)0;
     *(xl ? &S.ux[sq][sw] : &S.uy[sq][sw]) = u;
-    if (in_grid)
-      zuni = (rz.info & kLineSeg) ? seg_uniform(rz, bz, &zp, &zn)
-                                  : inline_uniform(inl[Lz], lattice_coord(g.oz, g.h, k0),
-                                                   lattice_coord(g.oz, g.h, k0 + nk - 1), &zp,
-                                                   &zn);
+    // every lane (one outside the grid tests line 0 and ignores the result),
+    // so no path leaves the tail load unread
+    zuni = (rz.info & kLineSeg) ? seg_uniform(rz, bz, &zp, &zn)
+                                : inline_uniform(inl[Lz], lattice_coord(g.oz, g.h, k0),
+                                                 lattice_coord(g.oz, g.h, k0 + nk - 1), &zp,
+                                                 &zn);
     // a staged line outside the grid (or a layer beyond nz) serves no point
     lines_shared = (!valid | uni) & (!in_grid | zuni);
   }
+  // the distances are read here, on every path: a load no path reads would
+  // still be in flight at the next brick, whose first write to its register
+  // then waits for every load and store before it
+#pragma unroll
+  for (int q = 0; q < kSignZ; q++) asm volatile("" : "+v"(best[q]));
   // wave-uniform shortcut: every line of the brick is shared by its points
   // (no crossing near the brick), so each point only adds six stored parities
   const bool brick_uni = __all(lines_shared);
   if (do_sign && !brick_uni) *(xl ? &S.lx[sq][sw] : &S.ly[sq][sw]) = inl[Lxy];
-  __syncthreads();
+  wave_lds_sync();
 
   // the point's distance: min(sqrt(best), h), h when no triangle was near
   auto point_dist = [&](int q) -> double {
@@ -772,7 +793,7 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S,
       finish(idx, point_dist(q), inside, amb);
     }
   }
-  __syncthreads();  // S is reused by the block's next brick
+  wave_lds_sync();  // S is reused by the block's next brick
 }
 
 // Six waves per SIMD (80 registers, no spills to scratch).  The kernel is
@@ -784,9 +805,13 @@ void fused_lattice_kernel(FusedJob J) {
   __shared__ FusedShared S;
   __shared__ FusedCold C;
   if (threadIdx.x == 0) C = J.cold;
-  __syncthreads();
-  for (int64_t m = xcd_remap(blockIdx.x, gridDim.x); m < J.nbricks; m += gridDim.x)
-    fused_brick<kOut>(J, m, S, C);
+  wave_lds_sync();
+  // 32-bit brick loop (the launcher keeps nbricks below 2^31): the loop test
+  // stays on the scalar unit and takes no vector register that a load in
+  // flight may still target
+  const uint32_t nb = (uint32_t)J.nbricks;
+  for (uint32_t m = (uint32_t)xcd_remap(blockIdx.x, gridDim.x); m < nb; m += gridDim.x)
+    fused_brick<kOut>(J, (int64_t)m, S, C);
 }
 
 // Persistent work queue over the sign bricks of many jobs (static brick
@@ -1380,6 +1405,7 @@ void launch_fused_lattice(hipStream_t s, const GridP &g, const Tri64 *tri,
   const FusedJob J = make_fused_job(g, b, d2buf, do_sign, out, pend, npend, pend_cap, pend0,
                                     npend0, pend0_cap, err);
   if (J.nbricks <= 0) return;
+  if (J.nbricks >= ((int64_t)1 << 31)) throw Error{-1, "too many sign bricks for one launch"};
   const int64_t grid = max_blocks > 0 && max_blocks < J.nbricks ? max_blocks : J.nbricks;
   const int set = (out.sdf ? kOutSdf : 0) | (out.dist ? kOutDist : 0) |
                   (out.signs ? kOutSigns : 0) | (out.conf ? kOutConf : 0);
