@@ -459,9 +459,20 @@ __device__ inline float tri_pre_dist2(float px, float py, float pz, Get T) {
 
 // Number of work items when the host holds only a capacity and the device
 // holds the actual count (null: the capacity is the count).
+// Read-only view of device memory that nothing writes while the reading
+// kernel runs: wave-uniform loads through it are scalar loads (scalar cache,
+// their own wait counter, so waiting for them never waits for the wave's
+// vector loads and stores).
+template <typename T>
+using KConst = const __attribute__((address_space(4))) T *;
+template <typename T>
+__device__ inline KConst<T> kconst(const T *p) {
+  return (KConst<T>)p;
+}
+
 __device__ inline int64_t chunk_count(int64_t cap, const uint64_t *dev) {
   if (!dev) return cap;
-  const int64_t n = (int64_t)*dev;
+  const int64_t n = (int64_t)*kconst(dev);
   return n < cap ? n : cap;
 }
 

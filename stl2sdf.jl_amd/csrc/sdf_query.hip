@@ -169,17 +169,17 @@ __device__ inline void dist_item(const DistJob &J, int64_t item, DistShared &S) 
   const GridP &g = J.g;
   const TileDims &td = J.td;
   const Tri64 *__restrict__ tri = J.tri;
-  const uint64_t *__restrict__ tile_off = J.tile_off;
   const uint32_t *__restrict__ tile_tris = J.tile_tris;
-  const uint64_t *__restrict__ chunk_off = J.chunk_off;
   float *__restrict__ d2buf = J.d2buf;
   if (item >= chunk_count(J.nchunks, J.nchunks_dev)) return;
-  const uint32_t *__restrict__ chunk_tile = J.chunk_tile;
-  const uint32_t *__restrict__ only = J.only;
+  // the item's header (tile, its work items, its triangle range) is
+  // wave-uniform and read-only here: scalar loads
+  const auto tile_off = kconst(J.tile_off);
+  const auto chunk_off = kconst(J.chunk_off);
   const int lane = threadIdx.x;
-  const int64_t tile = chunk_tile[item];
-  const uint64_t first = chunk_off[tile];
-  if (only && only[first] == 0) return;  // re-run restricted to flagged tiles
+  const int64_t tile = kconst(J.chunk_tile)[item];
+  const uint64_t first = chunk_off[tile], next = chunk_off[tile + 1];
+  if (J.only && kconst(J.only)[first] == 0) return;  // re-run restricted to flagged tiles
   const int64_t q = item - (int64_t)first;
   int32_t tx, ty, tz;
   tile_coords((uint32_t)tile, td, &tx, &ty, &tz);
@@ -276,7 +276,7 @@ __device__ inline void dist_item(const DistJob &J, int64_t item, DistShared &S) 
   __syncthreads();
   const float best = __uint_as_float(S.best[lane]);
   float *slot = d2buf + first * kTilePts + lane;
-  if (chunk_off[tile + 1] - first == 1)
+  if (next - first == 1)
     *slot = best;
   else
     atomicMin((unsigned int *)slot, __float_as_uint(best));
