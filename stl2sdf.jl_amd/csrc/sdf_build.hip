@@ -545,11 +545,24 @@ __global__ void tile_chunks_kernel(const uint64_t *__restrict__ tile_off,
 }
 
 __global__ void chunk_items_kernel(const uint64_t *__restrict__ chunk_off,
-                                   int64_t ntiles, uint32_t *__restrict__ items,
-                                   uint64_t cap) {
+                                   const uint64_t *__restrict__ tile_off, int64_t ntiles,
+                                   int clen, uint32_t *__restrict__ items,
+                                   DistItem *__restrict__ recs, uint64_t cap) {
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < ntiles;
        t += (int64_t)gridDim.x * blockDim.x) {
-    for (uint64_t q = chunk_off[t]; q < chunk_off[t + 1] && q < cap; q++) items[q] = (uint32_t)t;
+    const uint64_t c0 = chunk_off[t], c1 = chunk_off[t + 1];
+    const uint64_t tb = tile_off[t], te = tile_off[t + 1];
+    const uint64_t single = c1 - c0 == 1 ? 1ull << 55 : 0ull;
+    for (uint64_t q = c0; q < c1 && q < cap; q++) {
+      items[q] = (uint32_t)t;
+      const uint64_t b = tb + (q - c0) * (uint64_t)clen;
+      const uint64_t n = min((uint64_t)clen, te - b);
+      DistItem r;
+      r.tile = (uint32_t)t;
+      r.first = (uint32_t)c0;
+      r.range = b | n << 48 | single;
+      recs[q] = r;
+    }
   }
 }
 
@@ -915,13 +928,18 @@ void build_dist_chunks(hipStream_t s, Arena &ar, const GridP &g, Bins &b, int64_
   scan_counts(s, ar, nch, coff, td.ntiles);
   if (cap < 0) cap = (int64_t)read_u64(s, coff + td.ntiles);
   uint32_t *items = ar.get<uint32_t>("chunk_tile", cap + 1);
+  DistItem *recs = ar.get<DistItem>("chunk_item", cap + 1);
   if (cap > 0) {
+    // the packed header keeps triangle-list indices below 2^48
+    if (b.ntile_pairs >= ((int64_t)1 << 48)) throw Error{-1, "too many tile-triangle pairs"};
     hipLaunchKernelGGL(chunk_items_kernel, dim3(grid_for(td.ntiles, 256)), dim3(256),
-                       0, s, (const uint64_t *)coff, td.ntiles, items, (uint64_t)cap);
+                       0, s, (const uint64_t *)coff, (const uint64_t *)b.tile_off, td.ntiles,
+                       chunk_len, items, recs, (uint64_t)cap);
     SDF_HIP(hipGetLastError());
   }
   b.chunk_off = coff;
   b.chunk_tile = items;
+  b.chunk_item = recs;
   b.nchunks = cap;
   b.nchunks_dev = coff + td.ntiles;
   b.chunk_len = chunk_len;

@@ -114,6 +114,18 @@ void launch_gather_tris(hipStream_t s, const double *X, int64_t nnp,
 void scan_counts(hipStream_t s, Arena &ar, const uint32_t *in, uint64_t *out,
                  int64_t n);
 
+// Header of one distance work item (a chunk of <= clen of a tile's
+// triangles), one 16-byte scalar load: the tile, the tile's first work item
+// and, packed, the chunk's first index into the tile's triangle list
+// (bits 0-47), its triangle count (bits 48-54) and whether it is the tile's
+// only chunk (bit 55).
+struct DistItem {
+  uint32_t tile;
+  uint32_t first;
+  uint64_t range;
+};
+constexpr uint64_t kItemBeginMask = (1ull << 48) - 1;
+
 struct Bins {
   // tile binning (distance)
   uint64_t *tile_off = nullptr;  // ntiles + 1
@@ -129,6 +141,7 @@ struct Bins {
   // distance work items: each non-empty tile's list in chunks of 64
   uint64_t *chunk_off = nullptr;  // ntiles + 1 (first work item of a tile)
   uint32_t *chunk_tile = nullptr;  // work item -> tile
+  DistItem *chunk_item = nullptr;  // work item -> its header (distance kernel)
   int64_t nchunks = 0;
   // Capacities above; the actual counts live on the device (written by the
   // scans).  A count above its capacity means the lists were truncated and
@@ -156,6 +169,7 @@ struct DistJob {
   const uint32_t *tile_tris;
   const uint64_t *chunk_off;
   const uint32_t *chunk_tile;
+  const DistItem *items;         // per work item: its header
   int64_t nchunks;               // capacity
   const uint64_t *nchunks_dev;   // actual count (null: the capacity)
   const uint32_t *only;          // per first item: 0 skips the tile (null: all)

@@ -165,29 +165,31 @@ struct DistShared {
 // row test keeps each row's candidates, and the exact distances of every
 // (row, candidate) job are spread evenly over the wave, 8 lanes (the row's
 // points) per job.  Uniform per block (contains barriers).
-__device__ inline void dist_item(const DistJob &J, int64_t item, DistShared &S) {
+// bounded: the caller has checked item against the item count.
+__device__ inline void dist_item(const DistJob &J, int64_t item, DistShared &S,
+                                 bool bounded = false) {
   const GridP &g = J.g;
   const TileDims &td = J.td;
   const Tri64 *__restrict__ tri = J.tri;
   const uint32_t *__restrict__ tile_tris = J.tile_tris;
   float *__restrict__ d2buf = J.d2buf;
-  if (item >= chunk_count(J.nchunks, J.nchunks_dev)) return;
-  // the item's header (tile, its work items, its triangle range) is
-  // wave-uniform and read-only here: scalar loads
-  const auto tile_off = kconst(J.tile_off);
-  const auto chunk_off = kconst(J.chunk_off);
+  if (!bounded && item >= chunk_count(J.nchunks, J.nchunks_dev)) return;
+  // the item's header, wave-uniform and read-only here: one 16-byte scalar
+  // load (as two words: tile | first << 32, range)
+  static_assert(sizeof(DistItem) == 16, "DistItem is two 64-bit words");
+  const auto hp = kconst(reinterpret_cast<const uint64_t *>(J.items)) + 2 * item;
+  const uint64_t h0 = hp[0], range = hp[1];
   const int lane = threadIdx.x;
-  const int64_t tile = kconst(J.chunk_tile)[item];
-  const uint64_t first = chunk_off[tile], next = chunk_off[tile + 1];
+  const uint32_t tile = (uint32_t)h0;
+  const uint64_t first = h0 >> 32;
+  const bool single = (range >> 55) & 1ull;  // the tile's only work item
   if (J.only && kconst(J.only)[first] == 0) return;  // re-run restricted to flagged tiles
-  const int64_t q = item - (int64_t)first;
   int32_t tx, ty, tz;
-  tile_coords((uint32_t)tile, td, &tx, &ty, &tz);
+  tile_coords(tile, td, &tx, &ty, &tz);
   const int32_t i0 = tx * kTileX, j0 = ty * kTileY, k0 = tz * kTileZ;
   const Vec3d T0 = lattice_point(g, i0, j0, k0);
-  const uint64_t b = tile_off[tile] + (uint64_t)q * J.clen;
-  const uint64_t e = min(b + (uint64_t)J.clen, tile_off[tile + 1]);
-  const int n = (int)(e - b);
+  const uint64_t b = range & kItemBeginMask;
+  const int n = (int)((range >> 48) & 127ull);
   const float hf = (float)g.h;
   const float H2 = hf * hf * (1.0f + 0x1p-10f);
   // Phase 1, one triangle per lane: which of the tile's 8 rows of 8 points
@@ -276,7 +278,7 @@ __device__ inline void dist_item(const DistJob &J, int64_t item, DistShared &S) 
   __syncthreads();
   const float best = __uint_as_float(S.best[lane]);
   float *slot = d2buf + first * kTilePts + lane;
-  if (next - first == 1)
+  if (single)
     *slot = best;
   else
     atomicMin((unsigned int *)slot, __float_as_uint(best));
@@ -289,7 +291,7 @@ __global__ __launch_bounds__(kTilePts) void dist_chunks_kernel(DistJob J) {
   __shared__ DistShared S;
   const int64_t n = chunk_count(J.nchunks, J.nchunks_dev);
   for (int64_t it = xcd_remap(blockIdx.x, gridDim.x); it < n; it += gridDim.x)
-    dist_item(J, it, S);
+    dist_item(J, it, S, true);
 }
 
 // Persistent work queue over the distance items of many jobs: each block
@@ -1308,6 +1310,7 @@ DistJob make_dist_job(const GridP &g, const Tri64 *tri, const Bins &b, float *d2
   J.tile_tris = b.tile_tris;
   J.chunk_off = b.chunk_off;
   J.chunk_tile = b.chunk_tile;
+  J.items = b.chunk_item;
   J.nchunks = b.nchunks;
   J.nchunks_dev = b.nchunks_dev;
   J.only = only;
