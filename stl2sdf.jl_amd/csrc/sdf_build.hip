@@ -234,10 +234,16 @@ constexpr int64_t kSmallMax = 256;  // candidates looped by one thread
 // then sort the pairs by tile (stable LDS radix sort).  Tile lists come out
 // in triangle order, independent of scheduling.  Triangles with more than
 // kSmallMax candidate tiles are counted and emitted by one workgroup each.
+// Also zeroes the nzero words of the tiles' run bounds (a side job that
+// saves the chain a launch).
 __global__ void tile_count_small_kernel(const Tri64 *__restrict__ tri, int64_t nel,
                                         GridP g, TileDims td, double R,
                                         uint32_t *__restrict__ cnt,
-                                        uint32_t *__restrict__ big, uint32_t *nbig) {
+                                        uint32_t *__restrict__ big, uint32_t *nbig,
+                                        uint64_t *__restrict__ zero, int64_t nzero) {
+  for (int64_t z = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; z < nzero;
+       z += (int64_t)gridDim.x * blockDim.x)
+    zero[z] = 0ull;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nel;
        t += (int64_t)gridDim.x * blockDim.x) {
     const Tri64 tr = tri[t];
@@ -331,23 +337,18 @@ __global__ __launch_bounds__(256) void tile_big_kernel(
   }
 }
 
-// CSR offsets from the sorted tile keys: off[t] = first index whose key is
-// >= t (binary search per tile; no serial runs over empty stretches).
-__global__ void tile_offsets_kernel(const uint32_t *__restrict__ keys, int64_t n,
-                                    const uint64_t *__restrict__ n_dev, int64_t ntiles,
-                                    uint64_t *__restrict__ off) {
+// Run bounds of every non-empty tile from the sorted tile keys: the first
+// pair of a run writes its begin, the last its end (one coalesced pass; the
+// bounds of empty tiles stay 0).
+__global__ void tile_bounds_kernel(const uint32_t *__restrict__ keys, int64_t n,
+                                   const uint64_t *__restrict__ n_dev,
+                                   uint64_t *__restrict__ begin, uint64_t *__restrict__ end) {
   if (n_dev && (int64_t)*n_dev < n) n = (int64_t)*n_dev;
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t <= ntiles;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    int64_t lo = 0, hi = n;
-    while (lo < hi) {
-      const int64_t mid = (lo + hi) >> 1;
-      if ((int64_t)keys[mid] < t)
-        lo = mid + 1;
-      else
-        hi = mid;
-    }
-    off[t] = (uint64_t)lo;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t k = keys[i];
+    if (i == 0 || keys[i - 1] != k) begin[k] = (uint64_t)i;
+    if (i == n - 1 || keys[i + 1] != k) end[k] = (uint64_t)(i + 1);
   }
 }
 
@@ -536,22 +537,25 @@ __global__ void line_bin_big_kernel(const Tri64 *__restrict__ tri, GridP g,
 }
 
 __global__ void tile_chunks_kernel(const uint64_t *__restrict__ tile_off,
-                                   int64_t ntiles, int clen, uint32_t *__restrict__ nchunk) {
+                                   const uint64_t *__restrict__ tile_end, int64_t ntiles,
+                                   int clen, uint32_t *__restrict__ nchunk) {
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < ntiles;
        t += (int64_t)gridDim.x * blockDim.x) {
-    const uint64_t c = tile_off[t + 1] - tile_off[t];
+    const uint64_t c = tile_end[t] - tile_off[t];
     nchunk[t] = (uint32_t)((c + clen - 1) / clen);
   }
 }
 
 __global__ void chunk_items_kernel(const uint64_t *__restrict__ chunk_off,
-                                   const uint64_t *__restrict__ tile_off, int64_t ntiles,
+                                   const uint64_t *__restrict__ tile_off,
+                                   const uint64_t *__restrict__ tile_end, int64_t ntiles,
                                    int clen, uint32_t *__restrict__ items,
                                    DistItem *__restrict__ recs, uint64_t cap) {
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < ntiles;
        t += (int64_t)gridDim.x * blockDim.x) {
     const uint64_t c0 = chunk_off[t], c1 = chunk_off[t + 1];
-    const uint64_t tb = tile_off[t], te = tile_off[t + 1];
+    if (c1 == c0) continue;  // empty tile
+    const uint64_t tb = tile_off[t], te = tile_end[t];
     const uint64_t single = c1 - c0 == 1 ? 1ull << 55 : 0ull;
     for (uint64_t q = c0; q < c1 && q < cap; q++) {
       items[q] = (uint32_t)t;
@@ -860,11 +864,13 @@ void tile_bins_begin(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
   tb.tcnt = ar.get<uint32_t>("bin_tri_cnt", nel + 1);
   tb.toff = ar.get<uint64_t>("bin_tri_off", nel + 2);
   tb.big = ar.get<uint32_t>("bin_big", nel + 1);
+  tb.bounds = ar.get<uint64_t>("bin_tile_bounds", 2 * (td.ntiles + 1));
   tb.nbig = ar.get<uint32_t>("bin_nbig", 1);
   launch_fill_u32(s, tb.nbig, 1, 0u);
   if (nel > 0) {
     hipLaunchKernelGGL(tile_count_small_kernel, dim3(grid_for(nel, 256)), dim3(256), 0, s,
-                       tri, nel, g, td, tb.R, tb.tcnt, tb.big, tb.nbig);
+                       tri, nel, g, td, tb.R, tb.tcnt, tb.big, tb.nbig, tb.bounds,
+                       2 * (td.ntiles + 1));
     hipLaunchKernelGGL(tile_big_kernel, dim3(1024), dim3(256), 0, s, tri, g, td, tb.R, 0,
                        (const uint32_t *)tb.big, (const uint32_t *)tb.nbig, tb.tcnt,
                        (const uint64_t *)nullptr, (uint32_t *)nullptr, (uint32_t *)nullptr,
@@ -877,7 +883,7 @@ void tile_bins_begin(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
 void tile_bins_finish(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
                       const GridP &g, TileBuild &tb, Bins &b, int64_t cap) {
   TileDims td = tile_dims(g);
-  uint64_t *tile_off = ar.get<uint64_t>("bin_tile_off", td.ntiles + 2);
+  uint64_t *tile_off = tb.bounds, *tile_end = tb.bounds + td.ntiles + 1;
   const uint64_t *n_dev = nel > 0 ? tb.toff + nel : nullptr;
   if (cap < 0) cap = nel > 0 ? (int64_t)read_u64(s, tb.toff + nel) : 0;
   uint32_t *keys = ar.get<uint32_t>("bin_tile_keys", cap + 1);
@@ -897,13 +903,14 @@ void tile_bins_finish(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
     ar.scope = scope + "tile/";
     radix_sort_u32_pairs(s, ar, keys, tile_tris, cap, bits, n_dev, &keys, &tile_tris);
     ar.scope = scope;
-    hipLaunchKernelGGL(tile_offsets_kernel, dim3(grid_for(td.ntiles + 1, 256)), dim3(256),
-                       0, s, (const uint32_t *)keys, cap, n_dev, td.ntiles, tile_off);
+    hipLaunchKernelGGL(tile_bounds_kernel, dim3(grid_for(cap, 256)), dim3(256), 0, s,
+                       (const uint32_t *)keys, cap, n_dev, tile_off, tile_end);
     SDF_HIP(hipGetLastError());
-  } else {
-    launch_fill_u64(s, tile_off, td.ntiles + 1, 0ull);
+  } else if (nel <= 0) {
+    launch_fill_u64(s, tb.bounds, 2 * (td.ntiles + 1), 0ull);  // no counting pass ran
   }
   b.tile_off = tile_off;
+  b.tile_end = tile_end;
   b.tile_tris = tile_tris;
   b.ntile_pairs = cap;
   b.npairs_dev = n_dev;
@@ -923,7 +930,8 @@ void build_dist_chunks(hipStream_t s, Arena &ar, const GridP &g, Bins &b, int64_
   uint32_t *nch = ar.get<uint32_t>("chunk_cnt", td.ntiles + 1);
   uint64_t *coff = ar.get<uint64_t>("chunk_off", td.ntiles + 2);
   hipLaunchKernelGGL(tile_chunks_kernel, dim3(grid_for(td.ntiles, 256)), dim3(256),
-                     0, s, (const uint64_t *)b.tile_off, td.ntiles, chunk_len, nch);
+                     0, s, (const uint64_t *)b.tile_off, (const uint64_t *)b.tile_end,
+                     td.ntiles, chunk_len, nch);
   SDF_HIP(hipGetLastError());
   scan_counts(s, ar, nch, coff, td.ntiles);
   if (cap < 0) cap = (int64_t)read_u64(s, coff + td.ntiles);
@@ -933,8 +941,9 @@ void build_dist_chunks(hipStream_t s, Arena &ar, const GridP &g, Bins &b, int64_
     // the packed header keeps triangle-list indices below 2^48
     if (b.ntile_pairs >= ((int64_t)1 << 48)) throw Error{-1, "too many tile-triangle pairs"};
     hipLaunchKernelGGL(chunk_items_kernel, dim3(grid_for(td.ntiles, 256)), dim3(256),
-                       0, s, (const uint64_t *)coff, (const uint64_t *)b.tile_off, td.ntiles,
-                       chunk_len, items, recs, (uint64_t)cap);
+                       0, s, (const uint64_t *)coff, (const uint64_t *)b.tile_off,
+                       (const uint64_t *)b.tile_end, td.ntiles, chunk_len, items, recs,
+                       (uint64_t)cap);
     SDF_HIP(hipGetLastError());
   }
   b.chunk_off = coff;

@@ -128,7 +128,8 @@ constexpr uint64_t kItemBeginMask = (1ull << 48) - 1;
 
 struct Bins {
   // tile binning (distance)
-  uint64_t *tile_off = nullptr;  // ntiles + 1
+  uint64_t *tile_off = nullptr;  // per tile: first pair of its run (0 when empty)
+  uint64_t *tile_end = nullptr;  // per tile: end of its run (0 when empty)
   uint32_t *tile_tris = nullptr;
   // axis-line hits (signs)
   uint64_t *line_off = nullptr;  // nlines + 1
@@ -229,6 +230,9 @@ struct TileBuild {
   double R = 0.0;
   uint32_t *tcnt = nullptr, *big = nullptr, *nbig = nullptr;
   uint64_t *toff = nullptr;
+  // per tile: begin and end of its run in the sorted pair list (both 0 for
+  // an empty tile; zeroed by the counting pass), ntiles + 1 each, adjacent
+  uint64_t *bounds = nullptr;
 };
 struct LineBuild {
   int64_t nlx = 0, nly = 0, nlines = 0;

@@ -348,7 +348,8 @@ __global__ void dist_prefix_kernel(const DistJob *__restrict__ jobs, int njobs,
 // longest list within h (the exact evaluations of the plain loop).
 __global__ __launch_bounds__(kTilePts) void dist_chunks_learned_kernel(
     GridP g, TileDims td, const Tri64 *__restrict__ tri,
-    const uint64_t *__restrict__ tile_off, const uint32_t *__restrict__ tile_tris,
+    const uint64_t *__restrict__ tile_off, const uint64_t *__restrict__ tile_end,
+    const uint32_t *__restrict__ tile_tris,
     const uint64_t *__restrict__ chunk_off, const uint32_t *__restrict__ chunk_tile,
     int64_t nchunks, const uint64_t *__restrict__ nchunks_dev, int clen,
     const float *__restrict__ r2buf, float *__restrict__ d2buf,
@@ -366,7 +367,7 @@ __global__ __launch_bounds__(kTilePts) void dist_chunks_learned_kernel(
   const int32_t i0 = tx * kTileX, j0 = ty * kTileY, k0 = tz * kTileZ;
   const Vec3d T0 = lattice_point(g, i0, j0, k0);
   const uint64_t b = tile_off[tile] + (uint64_t)q * clen;
-  const uint64_t e = min(b + (uint64_t)clen, tile_off[tile + 1]);
+  const uint64_t e = min(b + (uint64_t)clen, tile_end[tile]);
   const int n = (int)(e - b);
   if (lane < n) {
     const Tri64 t = tri[tile_tris[b + lane]];
@@ -1346,7 +1347,8 @@ void launch_dist_chunks_learned(hipStream_t s, const GridP &g, const Tri64 *tri,
   if (b.nchunks <= 0) return;
   TileDims td = tile_dims(g);
   hipLaunchKernelGGL(dist_chunks_learned_kernel, dim3((unsigned)b.nchunks), dim3(kTilePts), 0,
-                     s, g, td, tri, b.tile_off, b.tile_tris, b.chunk_off, b.chunk_tile,
+                     s, g, td, tri, b.tile_off, b.tile_end, b.tile_tris, b.chunk_off,
+                     b.chunk_tile,
                      b.nchunks, b.nchunks_dev, (int)b.chunk_len, r2buf, d2buf, stats);
   const float hf = (float)g.h;
   hipLaunchKernelGGL(learned_check_kernel, dim3((unsigned)((b.nchunks + 3) / 4)), dim3(256),
