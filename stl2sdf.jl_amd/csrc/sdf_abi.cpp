@@ -388,12 +388,11 @@ void stage_bins_tiles(sdf_ctx *c, LatticeRun &L) {
     sdf::TileBuild tb;
     sdf::tile_bins_begin(c->stream, c->ar, c->tri, c->nel, g, g.h, tb);
     sdf::tile_bins_finish(c->stream, c->ar, c->tri, c->nel, g, tb, bins, c->caps.pairs);
-    sdf::build_dist_chunks(c->stream, c->ar, g, bins, c->caps.chunks, c->chunk_len);
-    // one spare tile of +inf after the last work item: the sign kernel reads
-    // it for tiles without distance work, so its loads are unconditional
-    L.d2buf = c->ar.get<float>("d2buf", ((size_t)bins.nchunks + 1) * sdf::kTilePts);
-    sdf::launch_fill_u32(c->stream, (uint32_t *)L.d2buf,
-                         ((int64_t)bins.nchunks + 1) * sdf::kTilePts, 0x7F800000u);
+    // the distance buffer, with a spare slot of +inf after the last work item
+    // (the sign kernel reads it for tiles without distance work, so its
+    // loads are unconditional)
+    sdf::build_dist_chunks(c->stream, c->ar, g, bins, c->caps.chunks, c->chunk_len,
+                           &L.d2buf);
   }
   // the helper chains start when the tile chain is done: beside the
   // distance kernel (whose grid leaves room on every CU) they cost little,

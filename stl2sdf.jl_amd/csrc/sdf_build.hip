@@ -124,6 +124,31 @@ __global__ void scan_add_kernel(uint64_t *__restrict__ out, int64_t n,
     out[i] += boff[i / kScanBlock];
 }
 
+// Second pass for up to kScanFlatBlocks scan blocks: workgroup b sums the
+// totals of blocks 0..b-1 itself and adds that to block b's outputs (one
+// launch instead of a scan of the totals and an add pass).
+constexpr int64_t kScanFlatBlocks = 8192;
+__global__ __launch_bounds__(kScanThreads) void scan_add_flat_kernel(
+    uint64_t *__restrict__ out, int64_t n_out, const uint64_t *__restrict__ bsum) {
+  __shared__ uint64_t red[kScanThreads / 64];
+  const int64_t b = blockIdx.x;
+  if (b == 0) return;  // uniform: block 0 adds nothing
+  uint64_t p = 0;
+  for (int64_t j = threadIdx.x; j < b; j += kScanThreads) p += bsum[j];
+  for (int off = 32; off > 0; off >>= 1) p += __shfl_down(p, off);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = p;
+  __syncthreads();
+  uint64_t pre = 0;
+#pragma unroll
+  for (int w = 0; w < kScanThreads / 64; w++) pre += red[w];
+  const int64_t base = b * kScanBlock;
+#pragma unroll
+  for (int q = 0; q < kScanItems; q++) {
+    const int64_t i = base + q * kScanThreads + threadIdx.x;
+    if (i < n_out) out[i] += pre;
+  }
+}
+
 template <typename TIn>
 void scan_rec(hipStream_t s, Arena &ar, const TIn *in, int64_t n_in,
               uint64_t *out, int64_t n_out, int level) {
@@ -133,7 +158,11 @@ void scan_rec(hipStream_t s, Arena &ar, const TIn *in, int64_t n_in,
   hipLaunchKernelGGL(scan_block_kernel<TIn>, dim3((unsigned)nb),
                      dim3(kScanThreads), 0, s, in, n_in, out, n_out, bsum);
   SDF_HIP(hipGetLastError());
-  if (nb > 1) {
+  if (nb > 1 && nb <= kScanFlatBlocks) {
+    hipLaunchKernelGGL(scan_add_flat_kernel, dim3((unsigned)nb), dim3(kScanThreads), 0, s,
+                       out, n_out, (const uint64_t *)bsum);
+    SDF_HIP(hipGetLastError());
+  } else if (nb > 1) {
     uint64_t *boff = ar.get<uint64_t>(ar.tmp("scan_boff" + std::to_string(level)), nb + 1);
     scan_rec<uint64_t>(s, ar, bsum, nb, boff, nb, level + 1);
     hipLaunchKernelGGL(scan_add_kernel, dim3(grid_for(n_out, 256)), dim3(256),
@@ -550,11 +579,21 @@ __global__ void chunk_items_kernel(const uint64_t *__restrict__ chunk_off,
                                    const uint64_t *__restrict__ tile_off,
                                    const uint64_t *__restrict__ tile_end, int64_t ntiles,
                                    int clen, uint32_t *__restrict__ items,
-                                   DistItem *__restrict__ recs, uint64_t cap) {
+                                   DistItem *__restrict__ recs, uint64_t cap,
+                                   float *__restrict__ d2buf) {
+  const float4 inf4 = make_float4(__builtin_huge_valf(), __builtin_huge_valf(),
+                                  __builtin_huge_valf(), __builtin_huge_valf());
+  if (d2buf && blockIdx.x == 0 && threadIdx.x < kTilePts / 4)  // the spare slot
+    reinterpret_cast<float4 *>(d2buf + cap * kTilePts)[threadIdx.x] = inf4;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < ntiles;
        t += (int64_t)gridDim.x * blockDim.x) {
     const uint64_t c0 = chunk_off[t], c1 = chunk_off[t + 1];
     if (c1 == c0) continue;  // empty tile
+    if (d2buf && c1 - c0 > 1 && c0 < cap) {  // the slot several work items atomicMin into
+      float4 *slot = reinterpret_cast<float4 *>(d2buf + c0 * kTilePts);
+#pragma unroll
+      for (int k = 0; k < kTilePts / 4; k++) slot[k] = inf4;
+    }
     const uint64_t tb = tile_off[t], te = tile_end[t];
     const uint64_t single = c1 - c0 == 1 ? 1ull << 55 : 0ull;
     for (uint64_t q = c0; q < c1 && q < cap; q++) {
@@ -924,7 +963,7 @@ void build_tile_bins(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
 }
 
 void build_dist_chunks(hipStream_t s, Arena &ar, const GridP &g, Bins &b, int64_t cap,
-                       int chunk_len) {
+                       int chunk_len, float **d2buf) {
   if (chunk_len < 1 || chunk_len > kTilePts) chunk_len = kTilePts;
   TileDims td = tile_dims(g);
   uint32_t *nch = ar.get<uint32_t>("chunk_cnt", td.ntiles + 1);
@@ -937,13 +976,19 @@ void build_dist_chunks(hipStream_t s, Arena &ar, const GridP &g, Bins &b, int64_
   if (cap < 0) cap = (int64_t)read_u64(s, coff + td.ntiles);
   uint32_t *items = ar.get<uint32_t>("chunk_tile", cap + 1);
   DistItem *recs = ar.get<DistItem>("chunk_item", cap + 1);
+  float *d2 = nullptr;
+  if (d2buf) {
+    d2 = ar.get<float>("d2buf", ((size_t)cap + 1) * kTilePts);
+    *d2buf = d2;
+    if (cap <= 0) launch_fill_u32(s, (uint32_t *)d2, kTilePts, 0x7F800000u);  // spare slot
+  }
   if (cap > 0) {
     // the packed header keeps triangle-list indices below 2^48
     if (b.ntile_pairs >= ((int64_t)1 << 48)) throw Error{-1, "too many tile-triangle pairs"};
     hipLaunchKernelGGL(chunk_items_kernel, dim3(grid_for(td.ntiles, 256)), dim3(256),
                        0, s, (const uint64_t *)coff, (const uint64_t *)b.tile_off,
                        (const uint64_t *)b.tile_end, td.ntiles, chunk_len, items, recs,
-                       (uint64_t)cap);
+                       (uint64_t)cap, d2);
     SDF_HIP(hipGetLastError());
   }
   b.chunk_off = coff;

@@ -253,8 +253,12 @@ void line_hits_finish(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
                       const GridP &g, LineBuild &lb, Bins &b, int64_t cap = -1);
 void build_line_hits(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
                      const GridP &g, Bins &b);
+// d2buf (nullable): also allocates the distance buffer (one slot of
+// kTilePts floats per work item plus a spare slot of +inf after the last)
+// and sets to +inf the slots that take atomicMin (first slots of tiles with
+// several work items), the only slots read before they are written.
 void build_dist_chunks(hipStream_t s, Arena &ar, const GridP &g, Bins &b, int64_t cap = -1,
-                       int chunk_len = kTilePts);
+                       int chunk_len = kTilePts, float **d2buf = nullptr);
 
 // query side
 // Distance work items -> squared distances per point of non-empty tiles
