@@ -83,8 +83,8 @@ __global__ void gather_tris_kernel(const double *__restrict__ X, int64_t nnp,
 // out gets n_out entries; bsum[b] = block total.
 template <typename TIn>
 __global__ __launch_bounds__(kScanThreads) void scan_block_kernel(
-    const TIn *__restrict__ in, int64_t n_in, uint64_t *__restrict__ out,
-    int64_t n_out, uint64_t *__restrict__ bsum) {
+    const TIn *__restrict__ in, const TIn *__restrict__ in2, int64_t n_in,
+    uint64_t *__restrict__ out, int64_t n_out, uint64_t *__restrict__ bsum) {
   __shared__ uint64_t buf[2][kScanThreads];
   const int tid = threadIdx.x;
   const int64_t base = (int64_t)blockIdx.x * kScanBlock + (int64_t)tid * kScanItems;
@@ -93,7 +93,7 @@ __global__ __launch_bounds__(kScanThreads) void scan_block_kernel(
 #pragma unroll
   for (int q = 0; q < kScanItems; q++) {
     int64_t i = base + q;
-    uint64_t x = (i < n_in) ? (uint64_t)in[i] : 0ull;
+    uint64_t x = (i < n_in) ? (uint64_t)in[i] + (in2 ? (uint64_t)in2[i] : 0ull) : 0ull;
     v[q] = tot;
     tot += x;
   }
@@ -149,14 +149,15 @@ __global__ __launch_bounds__(kScanThreads) void scan_add_flat_kernel(
   }
 }
 
+// Exclusive scan of in (+ in2 elementwise when not null) into out.
 template <typename TIn>
 void scan_rec(hipStream_t s, Arena &ar, const TIn *in, int64_t n_in,
-              uint64_t *out, int64_t n_out, int level) {
+              uint64_t *out, int64_t n_out, int level, const TIn *in2 = nullptr) {
   int64_t nb = (n_out + kScanBlock - 1) / kScanBlock;
   if (nb < 1) nb = 1;
   uint64_t *bsum = ar.get<uint64_t>(ar.tmp("scan_bsum" + std::to_string(level)), nb + 1);
   hipLaunchKernelGGL(scan_block_kernel<TIn>, dim3((unsigned)nb),
-                     dim3(kScanThreads), 0, s, in, n_in, out, n_out, bsum);
+                     dim3(kScanThreads), 0, s, in, in2, n_in, out, n_out, bsum);
   SDF_HIP(hipGetLastError());
   if (nb > 1 && nb <= kScanFlatBlocks) {
     hipLaunchKernelGGL(scan_add_flat_kernel, dim3((unsigned)nb), dim3(kScanThreads), 0, s,
@@ -456,20 +457,24 @@ __device__ inline void line_visit(const Tri64 &tr, const GridP &g, int ax,
 }
 
 // Triangles with at most kSmallMax candidate lines, one thread each.  The
-// counting pass runs the exact hit test and, for a triangle with at most
-// kEmitBatch hits, keeps their line ids (hitn[t] hits at hitl[kEmitBatch t
-// ..]; hitn kHitsMany: more, tested again by the emit pass).  The emit pass
-// then issues a triangle's cursor atomics together, waits once, and writes
-// the records.
+// counting pass runs the exact hit test; a triangle's first kEmitBatch hits
+// (in candidate order) keep their line ids and take their slots in `cnt` by
+// one batch of returning atomics at the end (hitn[t] = hits, kHitsMany for
+// more; lines and slots at hitl / hslot[kEmitBatch t ..]), further hits are
+// counted in `cnt2`.  The emit pass then writes the first hits' records at
+// their slots without atomics; for a triangle with more hits it tests again,
+// skips the first kEmitBatch hits and takes cursor slots in `cnt`, which
+// continue after every slot the counting pass gave.
 constexpr int kEmitBatch = 4;
 constexpr uint8_t kHitsMany = 255;
 
 __global__ void line_count_small_kernel(const Tri64 *__restrict__ tri, int64_t nel, GridP g,
                                         int64_t base1, int64_t base2,
-                                        uint32_t *__restrict__ cnt,
+                                        uint32_t *__restrict__ cnt, uint32_t *__restrict__ cnt2,
                                         uint32_t *__restrict__ big, uint32_t *nbig,
                                         uint8_t *__restrict__ hitn,
-                                        uint32_t *__restrict__ hitl) {
+                                        uint32_t *__restrict__ hitl,
+                                        uint32_t *__restrict__ hslot) {
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nel;
        t += (int64_t)gridDim.x * blockDim.x) {
     const Tri64 tr = tri[t];
@@ -486,16 +491,22 @@ __global__ void line_count_small_kernel(const Tri64 *__restrict__ tri, int64_t n
             double o[3];
             const int64_t line = line_origin(g, ax, a, b, base1, base2, o);
             if (!line_hit_test(o, ax, tr.v, tr.v + 3, tr.v + 6, nullptr)) continue;
-            atomicAdd(&cnt[line], 1u);
+            if (nh >= kEmitBatch) atomicAdd(&cnt2[line], 1u);
 #pragma unroll
             for (int k = 0; k < kEmitBatch; k++) ln[k] = nh == k ? (uint32_t)line : ln[k];
             nh++;
           }
     }
     hitn[t] = nh > kEmitBatch ? kHitsMany : (uint8_t)nh;
-    if (nh > 0 && nh <= kEmitBatch) {
+    if (nh > 0) {
+      uint32_t sl[kEmitBatch];
 #pragma unroll
-      for (int k = 0; k < kEmitBatch; k++) hitl[kEmitBatch * t + k] = ln[k];
+      for (int k = 0; k < kEmitBatch; k++) sl[k] = k < nh ? atomicAdd(&cnt[ln[k]], 1u) : 0u;
+#pragma unroll
+      for (int k = 0; k < kEmitBatch; k++) {
+        hitl[kEmitBatch * t + k] = ln[k];
+        hslot[kEmitBatch * t + k] = sl[k];
+      }
     }
   }
 }
@@ -504,6 +515,7 @@ __global__ void line_emit_small_kernel(const Tri64 *__restrict__ tri, int64_t ne
                                        int64_t base1, int64_t base2,
                                        const uint8_t *__restrict__ hitn,
                                        const uint32_t *__restrict__ hitl,
+                                       const uint32_t *__restrict__ hslot,
                                        uint32_t *__restrict__ cursor,
                                        const uint64_t *__restrict__ off,
                                        LineHit *__restrict__ hits, uint64_t cap) {
@@ -512,28 +524,32 @@ __global__ void line_emit_small_kernel(const Tri64 *__restrict__ tri, int64_t ne
     const int nh = hitn[t];
     if (nh == 0) continue;
     const Tri64 tr = tri[t];
-    if (nh == kHitsMany) {
-      const LineRange r = tri_line_range(tr, g);
-      for (int ax = 0; ax < 3; ax++)
-        for (int32_t b = r.lo[ax][1]; b <= r.hi[ax][1]; b++)
-          for (int32_t a = r.lo[ax][0]; a <= r.hi[ax][0]; a++)
-            line_visit(tr, g, ax, a, b, base1, base2, 1, cursor, off, hits, cap);
-      continue;
-    }
-    uint32_t ln[kEmitBatch], slot[kEmitBatch];
-#pragma unroll
-    for (int k = 0; k < kEmitBatch; k++) ln[k] = hitl[kEmitBatch * t + k];
-#pragma unroll
-    for (int k = 0; k < kEmitBatch; k++) slot[k] = k < nh ? atomicAdd(&cursor[ln[k]], 1u) : 0u;
+    const int nf = nh < kEmitBatch ? nh : kEmitBatch;  // hits with a slot
 #pragma unroll
     for (int k = 0; k < kEmitBatch; k++) {
-      if (k < nh) {
-        const int ax = ln[k] < base1 ? 0 : (ln[k] < base2 ? 1 : 2);
+      if (k < nf) {
+        const uint32_t ln = hitl[kEmitBatch * t + k];
+        const int ax = ln < base1 ? 0 : (ln < base2 ? 1 : 2);
         LineHit rec;
         line_hit_record(ax, tr.v, tr.v + 3, tr.v + 6, &rec);
-        const uint64_t pos = off[ln[k]] + slot[k];
+        const uint64_t pos = off[ln] + hslot[kEmitBatch * t + k];
         if (pos < cap) hits[pos] = rec;  // beyond: overflow, reported by the sizes
       }
+    }
+    if (nh == kHitsMany) {  // the hits after the first kEmitBatch, in the same order
+      const LineRange r = tri_line_range(tr, g);
+      int h = 0;
+      for (int ax = 0; ax < 3; ax++)
+        for (int32_t b = r.lo[ax][1]; b <= r.hi[ax][1]; b++)
+          for (int32_t a = r.lo[ax][0]; a <= r.hi[ax][0]; a++) {
+            double o[3];
+            const int64_t line = line_origin(g, ax, a, b, base1, base2, o);
+            LineHit rec;
+            if (!line_hit_test(o, ax, tr.v, tr.v + 3, tr.v + 6, &rec)) continue;
+            if (h++ < kEmitBatch) continue;
+            const uint64_t pos = off[line] + atomicAdd(&cursor[line], 1u);
+            if (pos < cap) hits[pos] = rec;
+          }
     }
   }
 }
@@ -1010,25 +1026,30 @@ void line_hits_begin(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
   b.line_base[1] = lb.nlx;
   b.line_base[2] = lb.nlx + lb.nly;
   b.line_base[3] = lb.nlines;
-  lb.lcnt = ar.get<uint32_t>("line_cnt", lb.nlines + 1);
+  // lcnt: the first hits of each triangle (and, from the emit pass on, the
+  // cursor of the others); lcnt2: the other hits; one fill clears both
+  lb.lcnt = ar.get<uint32_t>("line_cnt", 2 * (lb.nlines + 1));
+  lb.lcnt2 = lb.lcnt + lb.nlines + 1;
   lb.loff = ar.get<uint64_t>("line_off", lb.nlines + 2);
   lb.big = ar.get<uint32_t>("line_big", nel + 1);
   lb.nbig = ar.get<uint32_t>("line_nbig", 1);
-  launch_fill_u32(s, lb.lcnt, lb.nlines + 1, 0u);
+  launch_fill_u32(s, lb.lcnt, 2 * (lb.nlines + 1), 0u);
   launch_fill_u32(s, lb.nbig, 1, 0u);
   if (nel > 0) {
     lb.hitn = ar.get<uint8_t>("line_hitn", nel + 1);
     lb.hitl = ar.get<uint32_t>("line_hitl", (size_t)kEmitBatch * nel + 1);
+    lb.hslot = ar.get<uint32_t>("line_hslot", (size_t)kEmitBatch * nel + 1);
     hipLaunchKernelGGL(line_count_small_kernel, dim3(capped_grid(nel, 256, lb.max_blocks)),
-                       dim3(256), 0, s, tri, nel, g, lb.nlx, lb.nlx + lb.nly, lb.lcnt, lb.big,
-                       lb.nbig, lb.hitn, lb.hitl);
+                       dim3(256), 0, s, tri, nel, g, lb.nlx, lb.nlx + lb.nly, lb.lcnt, lb.lcnt2,
+                       lb.big, lb.nbig, lb.hitn, lb.hitl, lb.hslot);
     hipLaunchKernelGGL(line_bin_big_kernel, dim3(1024), dim3(256), 0, s, tri, g, lb.nlx,
-                       lb.nlx + lb.nly, 0, lb.lcnt, (const uint64_t *)nullptr,
+                       lb.nlx + lb.nly, 0, lb.lcnt2, (const uint64_t *)nullptr,
                        (LineHit *)nullptr, (uint64_t)0, (const uint32_t *)lb.big,
                        (const uint32_t *)lb.nbig);
     SDF_HIP(hipGetLastError());
   }
-  scan_counts(s, ar, lb.lcnt, lb.loff, lb.nlines);
+  scan_rec<uint32_t>(s, ar, lb.lcnt, lb.nlines, lb.loff, lb.nlines + 1, 0,
+                     (const uint32_t *)lb.lcnt2);
 }
 
 void line_hits_finish(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
@@ -1037,11 +1058,12 @@ void line_hits_finish(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
   if (cap < 0) cap = (int64_t)read_u64(s, lb.loff + lb.nlines);
   LineHit *hits = ar.get<LineHit>("line_hits", cap + 1);
   if (cap > 0 && nel > 0) {
-    launch_fill_u32(s, lb.lcnt, lb.nlines + 1, 0u);  // reused as the cursor
+    // lcnt continues as the cursor of the hits without a slot
     hipLaunchKernelGGL(line_emit_small_kernel, dim3(capped_grid(nel, 256, lb.max_blocks)),
                        dim3(256), 0, s, tri, nel, g, lb.nlx, lb.nlx + lb.nly,
-                       (const uint8_t *)lb.hitn, (const uint32_t *)lb.hitl, lb.lcnt,
-                       (const uint64_t *)lb.loff, hits, (uint64_t)cap);
+                       (const uint8_t *)lb.hitn, (const uint32_t *)lb.hitl,
+                       (const uint32_t *)lb.hslot, lb.lcnt, (const uint64_t *)lb.loff, hits,
+                       (uint64_t)cap);
     hipLaunchKernelGGL(line_bin_big_kernel, dim3(1024), dim3(256), 0, s, tri, g, lb.nlx,
                        lb.nlx + lb.nly, 1, lb.lcnt, (const uint64_t *)lb.loff, hits,
                        (uint64_t)cap, (const uint32_t *)lb.big, (const uint32_t *)lb.nbig);

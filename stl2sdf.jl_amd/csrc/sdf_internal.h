@@ -236,8 +236,9 @@ struct TileBuild {
 };
 struct LineBuild {
   int64_t nlx = 0, nly = 0, nlines = 0;
-  uint32_t *lcnt = nullptr, *big = nullptr, *nbig = nullptr;
+  uint32_t *lcnt = nullptr, *lcnt2 = nullptr, *big = nullptr, *nbig = nullptr;
   uint64_t *loff = nullptr;
+  uint32_t *hslot = nullptr;  // per triangle: slots of its first hits in lcnt
   int64_t max_blocks = 0;  // grid cap of the per-triangle and per-line kernels (0: none)
   uint8_t *hitn = nullptr;   // per triangle: hits found by the counting pass (255: many)
   uint32_t *hitl = nullptr;  // their line ids, 4 per triangle
