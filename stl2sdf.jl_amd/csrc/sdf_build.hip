@@ -58,24 +58,39 @@ inline int capped_grid(int64_t n, int threads, int64_t max_blocks) {
 }
 
 // One thread per triangle: 1-based IEN -> 9 doubles.  Out-of-range indices
-// raise kErrBadIndex and read vertex 1 instead (never out of bounds).
-__global__ void gather_tris_kernel(const double *__restrict__ X, int64_t nnp,
-                                   const int64_t *__restrict__ IEN, int64_t nel,
-                                   Tri64 *__restrict__ tri, uint32_t *err) {
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nel;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    Tri64 r;
-    for (int c = 0; c < 3; c++) {
-      int64_t id = IEN[3 * t + c];
-      if (id < 1 || id > nnp) {
-        atomicOr(err, kErrBadIndex);
-        id = 1;
+// raise kErrBadIndex and read vertex 1 instead (never out of bounds).  A
+// workgroup gathers its 256 triangles into LDS and writes their records as
+// one contiguous run (a thread's own 72-byte record would spread each store
+// instruction over 4.6 KB).
+constexpr int kGatherThreads = 256;
+__global__ __launch_bounds__(kGatherThreads) void gather_tris_kernel(
+    const double *__restrict__ X, int64_t nnp, const int64_t *__restrict__ IEN, int64_t nel,
+    Tri64 *__restrict__ tri, uint32_t *err) {
+  __shared__ double s[kGatherThreads * 9];
+  double *out = reinterpret_cast<double *>(tri);
+  for (int64_t base = (int64_t)blockIdx.x * kGatherThreads; base < nel;
+       base += (int64_t)gridDim.x * kGatherThreads) {
+    const int64_t t = base + threadIdx.x;
+    if (t < nel) {
+      for (int c = 0; c < 3; c++) {
+        int64_t id = IEN[3 * t + c];
+        if (id < 1 || id > nnp) {
+          atomicOr(err, kErrBadIndex);
+          id = 1;
+        }
+        s[9 * threadIdx.x + 3 * c + 0] = X[3 * (id - 1) + 0];
+        s[9 * threadIdx.x + 3 * c + 1] = X[3 * (id - 1) + 1];
+        s[9 * threadIdx.x + 3 * c + 2] = X[3 * (id - 1) + 2];
       }
-      r.v[3 * c + 0] = X[3 * (id - 1) + 0];
-      r.v[3 * c + 1] = X[3 * (id - 1) + 1];
-      r.v[3 * c + 2] = X[3 * (id - 1) + 2];
     }
-    tri[t] = r;
+    __syncthreads();
+    const int64_t nv = 9 * (min((int64_t)kGatherThreads, nel - base));
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+      const int e = k * kGatherThreads + threadIdx.x;
+      if (e < nv) out[9 * base + e] = s[e];
+    }
+    __syncthreads();  // s is reused by the next round
   }
 }
 
@@ -902,8 +917,8 @@ void launch_gather_tris(hipStream_t s, const double *X, int64_t nnp,
                         const int64_t *IEN, int64_t nel, Tri64 *tri,
                         uint32_t *err) {
   if (nel <= 0) return;
-  hipLaunchKernelGGL(gather_tris_kernel, dim3(grid_for(nel, 256)), dim3(256),
-                     0, s, X, nnp, IEN, nel, tri, err);
+  hipLaunchKernelGGL(gather_tris_kernel, dim3(grid_for(nel, kGatherThreads)),
+                     dim3(kGatherThreads), 0, s, X, nnp, IEN, nel, tri, err);
   SDF_HIP(hipGetLastError());
 }
 
