@@ -680,11 +680,14 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S,
   if (do_sign && !brick_uni) *(xl ? &S.lx[sq][sw] : &S.ly[sq][sw]) = inl[Lxy];
   wave_lds_sync();
 
-  // the point's distance: min(sqrt(best), h), h when no triangle was near
+  // the point's distance: min(sqrt(best), h), h when no triangle was near.
+  // best is an FP32 squared distance: its correctly rounded FP32 root adds at
+  // most half an FP32 ulp (below 3e-10 for distances up to h = 0.005) and
+  // costs a fraction of the FP64 root's correction sequence
   auto point_dist = [&](int q) -> double {
     double dist = g.h;
     if (best[q] < __builtin_huge_valf()) {
-      const double d = sqrt((double)best[q]);
+      const double d = (double)__fsqrt_rn(best[q]);
       dist = d < g.h ? d : g.h;
     }
     return dist;
@@ -799,11 +802,13 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S,
   wave_lds_sync();  // S is reused by the block's next brick
 }
 
-// Six waves per SIMD (80 registers, no spills to scratch).  The kernel is
-// bound by the latency of its per-brick loads, so occupancy pays: 0.358 ms
-// at six against 0.370 ms at five on the star surface (82.6 M points).
+// Seven waves per SIMD (72 registers, no spills to scratch).  Per brick the
+// kernel issues about 380 scalar and 440 vector instructions and waits on
+// two load round trips, so occupancy pays: on the star surface (82.6 M
+// points) 0.358 ms at six waves against 0.370 ms at five, and with the FP32
+// root 0.327 ms at seven against 0.329 ms at six.
 template <int kOut>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6)))
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(7)))
 void fused_lattice_kernel(FusedJob J) {
   __shared__ FusedShared S;
   __shared__ FusedCold C;
