@@ -33,16 +33,43 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_PROFILE = os.path.join(REPO, "profiles", "fused_lattice_star512_hbm_traffic.md")
 
 
+# the sources the fused kernel is compiled from; the traffic summary records
+# their hash, and a summary measured on other sources is not reported
+KERNEL_SOURCES = ["stl2sdf.jl_amd/csrc/sdf_query.hip", "stl2sdf.jl_amd/csrc/sdf_common.h",
+                  "stl2sdf.jl_amd/csrc/sdf_internal.h"]
+
+
+def kernel_source_hash():
+    import hashlib
+    h = hashlib.sha256()
+    for rel in KERNEL_SOURCES:
+        with open(os.path.join(REPO, rel), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
 def measured_traffic(workload):
     """HBM bytes per launch of the fused kernel from the committed counter
-    summary (FETCH_SIZE / WRITE_SIZE passes, copy-calibrated); None when the
-    summary does not cover this workload."""
+    summary (FETCH_SIZE / WRITE_SIZE passes, copy-calibrated), with the
+    summary's source hash and commit.  traffic is None when the summary does
+    not cover this workload or was measured on other kernel sources."""
+    info = {"traffic": None, "traffic_profile": os.path.relpath(TRAFFIC_PROFILE, REPO),
+            "traffic_source_hash": None, "traffic_commit": None,
+            "kernel_source_hash": kernel_source_hash(), "traffic_stale": None}
     if workload != "star512" or not os.path.exists(TRAFFIC_PROFILE):
-        return None
+        return info
+    value = None
     for line in open(TRAFFIC_PROFILE):
         if line.startswith("fused_lattice_kernel traffic_bytes_per_launch:"):
-            return float(line.split(":")[1])
-    return None
+            value = float(line.split(":")[1])
+        elif line.startswith("kernel_source_hash:"):
+            info["traffic_source_hash"] = line.split(":")[1].strip()
+        elif line.startswith("measured_at_commit:"):
+            info["traffic_commit"] = line.split(":")[1].strip()
+    info["traffic_stale"] = info["traffic_source_hash"] != info["kernel_source_hash"]
+    if not info["traffic_stale"]:
+        info["traffic"] = value
+    return info
 
 
 def staged(L, ctx, step, torch, n=5):
@@ -85,7 +112,11 @@ def make_workload(name):
 
 
 def cpu_baseline(mesh, grid, target_s, threads):
-    """Oracle (reference-faithful distance + BVH ray vote) on a random sample."""
+    """Oracle (reference-faithful distance + BVH ray vote) on random samples
+    of the grid's lattice points.  Each oracle call rebuilds its trees (a
+    fixed cost that a full grid amortises), so two sample sizes are timed
+    (three runs each, median) and the per-point cost is their slope; the
+    larger size is chosen so that its runs take about target_s in all."""
     from oracle import oracle
     rng = np.random.default_rng(0)
     N = np.asarray(grid.N, np.int64)
@@ -108,23 +139,22 @@ def cpu_baseline(mesh, grid, target_s, threads):
         oracle.signs(mesh.X, mesh.IEN, P, threads=threads)
         return time.perf_counter() - t0
 
-    # Each oracle call rebuilds its trees (a fixed cost that a full grid
-    # amortises), so time two sample sizes and take the per-point slope.
-    n1 = 4096
-    t1 = run(n1)
-    while t1 < 1.0 and n1 < (1 << 20):
-        n1 *= 2
-        t1 = run(n1)
-    build_s = None
-    n2 = n1
-    t2 = t1
-    while t2 - t1 < target_s and n2 < (1 << 24):
-        n2 = int(n2 * 4)
-        t2 = run(n2)
+    def median3(n):
+        ts = sorted(run(n) for _ in range(3))
+        return ts[1], ts
+
+    n1 = 1 << 14
+    t1, ts1 = median3(n1)
+    # per-point cost from a probe twice the size, then the size for target_s
+    tp = run(2 * n1)
+    per_pt = max((tp - t1) / n1, 1e-9)
+    n2 = int(min(max(target_s / 3.0 - t1, 1.0) / per_pt + n1, 1 << 24))
+    n2 = max(n2, 4 * n1)
+    t2, ts2 = median3(n2)
     slope = (t2 - t1) / max(n2 - n1, 1)
     build_s = max(t1 - slope * n1, 0.0)
     full_s = build_s + slope * grid.ngp
-    return grid.ngp / full_s, (n1, t1, n2, t2, build_s, full_s)
+    return grid.ngp / full_s, (n1, ts1, n2, ts2, build_s, full_s)
 
 
 def main():
@@ -276,23 +306,33 @@ def main():
         "counters": {"pending_points": cnt[0], "tile_pairs": cnt[1], "line_hits": cnt[2],
                      "device_bytes": cnt[3]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(args.workload),
+                     "frac": achieved / HBM_PEAK_GBS,
                      "kernel": "fused_lattice_kernel", "alg_bytes_per_launch": alg_bytes,
-                     "fused_ms": fused_ms},
+                     "fused_ms": fused_ms, **measured_traffic(args.workload)},
     }
     res["paths"] = {"value": "plain GPU path (the drop-in default)",
                     "learned_radius": "same workload with the learned search radius"}
     if learned is not None:
         res["learned_radius"] = learned
     if not args.no_cpu_baseline:
-        threads = min(16, len(os.sched_getaffinity(0)))
-        log(f"[bench] cpu baseline with {threads} threads")
-        v, (n1, t1, n2, t2, b_s, f_s) = cpu_baseline(mesh, grid, args.cpu_seconds, threads)
+        # every host core this process may use: the affinity mask, bounded by
+        # the OpenMP thread count the environment sets (the share of the
+        # host's cores given to one GPU where that is set)
+        affinity = len(os.sched_getaffinity(0))
+        omp = os.environ.get("OMP_NUM_THREADS", "")
+        threads = min(affinity, int(omp)) if omp.isdigit() and int(omp) > 0 else affinity
+        log(f"[bench] cpu baseline with {threads} threads (nproc {os.cpu_count()}, "
+            f"affinity {affinity}, OMP_NUM_THREADS {omp or 'unset'})")
+        v, (n1, ts1, n2, ts2, b_s, f_s) = cpu_baseline(mesh, grid, args.cpu_seconds, threads)
+        fmt = lambda ts: "/".join(f"{t:.2f}" for t in ts)
         res["cpu_baseline"] = {
             "value": v, "unit": "signed grid points/s", "cores": threads, "kind": "port",
+            "nproc": os.cpu_count(), "affinity_cores": affinity,
+            "omp_num_threads": omp or None,
             "sample": (f"oracle (reference-faithful distance + bounding-sphere BVH adaptive ray "
-                       f"vote, FP64) on {n1} and {n2} uniformly random lattice points of the same "
-                       f"grid ({t1:.1f} s, {t2:.1f} s); tree builds {b_s:.1f} s; full grid "
+                       f"vote, FP64, {threads} threads) on {n1} and {n2} uniformly random "
+                       f"lattice points of the same grid, 3 runs each ({fmt(ts1)} s; {fmt(ts2)} "
+                       f"s), medians' slope = per-point cost; tree builds {b_s:.1f} s; full grid "
                        f"extrapolated to {f_s:.0f} s")}
         res["gpu_over_cpu"] = value / v
     print(json.dumps(res))

@@ -85,6 +85,13 @@ sdf_status sdf_ctx_stage_times(sdf_ctx *ctx, char *names, int32_t names_cap,
 sdf_status sdf_ctx_counters(sdf_ctx *ctx, int64_t *out, int32_t cap,
                             int32_t *n);
 
+/* Diagnostics, no counterpart in the reference: linear indices of the
+ * lattice points the last lattice call with signs handed to the adaptive
+ * vote (left undecided by the six axis rays, or decided inside a rounding
+ * margin), ascending, up to cap of them; *n receives the total. */
+sdf_status sdf_ctx_vote_points(sdf_ctx *ctx, int64_t *idx_out, int64_t cap,
+                               int64_t *n);
+
 /* Grid(X_min, X_max, N_max, margin) -- src/MeshGrid/Grid.jl:9-33. */
 sdf_status sdf_grid_setup(const double xmin[3], const double xmax[3],
                           int64_t n_max, int64_t margin, double aabb_min[3],

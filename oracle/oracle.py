@@ -40,6 +40,8 @@ def lib():
         L.orc_ray_triangle.restype = C.c_int
         L.orc_dist_exact.argtypes = [_dp, C.c_int64, _i64p, C.c_int64, _dp, C.c_int64,
                                      C.c_double, _dp, _dp, C.c_int]
+        L.orc_dist_exact_bvh.argtypes = [_dp, C.c_int64, _i64p, C.c_int64, _dp, C.c_int64,
+                                         C.c_double, _dp, C.c_int]
         L.orc_dist_reference.argtypes = [_dp, C.c_int64, _i64p, C.c_int64, _dp, C.c_int64,
                                          C.c_double, _dp, C.c_int]
         L.orc_signs.argtypes = [_dp, C.c_int64, _i64p, C.c_int64, _dp, C.c_int64, _dp,
@@ -120,6 +122,16 @@ def dist_exact(X, IEN, pts, h, threads=0, want_xp=False):
     lib().orc_dist_exact(_d(X), X.shape[1], _i(IEN), IEN.shape[1], _d(P), P.shape[1], float(h),
                          _d(out), _d(xp) if want_xp else None, int(threads))
     return (out, xp) if want_xp else out
+
+
+def dist_exact_bvh(X, IEN, pts, h, threads=0):
+    """dist_exact's min(d, h) through a bounding-volume hierarchy (same bits)."""
+    X, IEN = _mesh(X, IEN)
+    P = np.asfortranarray(pts, np.float64)
+    out = np.empty(P.shape[1])
+    lib().orc_dist_exact_bvh(_d(X), X.shape[1], _i(IEN), IEN.shape[1], _d(P), P.shape[1],
+                             float(h), _d(out), int(threads))
+    return out
 
 
 def dist_reference(X, IEN, pts, h, threads=0):

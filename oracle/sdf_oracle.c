@@ -17,6 +17,7 @@
  *   adaptive vote in orc_signs    RaycastingSignDetection.jl:234-288, 337-404
  *   orc_winding_sign              RaycastingSignDetection.jl:295-328
  *   orc_dist_exact                exact min(d, h); clamp of dfOnTriangularMesh.jl:202-206
+ *   orc_dist_exact_bvh            the same values through a bounding-volume hierarchy
  *   orc_dist_reference            dfOnTriangularMesh.jl:19-209, 226-251, 329-392
  *   orc_components                SdfArtifactRemoval.jl:148-194 (sequential semantics)
  *   orc_remove_artifacts          SdfArtifactRemoval.jl:134-248
@@ -613,6 +614,79 @@ void orc_dist_exact(const double *X, int64_t nnp, const int64_t *IEN,
     dist[i] = d > h ? h : d;
     if (xp) memcpy(xp + 3 * i, bcp, sizeof bcp);
   }
+}
+
+/* Squared distance from p to a box (0 inside). */
+static double box_dist2(const double p[3], const double lo[3], const double hi[3]) {
+  double s = 0.0;
+  for (int d = 0; d < 3; d++) {
+    double e = 0.0;
+    if (p[d] < lo[d]) e = lo[d] - p[d];
+    else if (p[d] > hi[d]) e = p[d] - hi[d];
+    s += e * e;
+  }
+  return s;
+}
+
+/* min(d, h) of orc_dist_exact, with the bounding-sphere hierarchy of the
+ * sign oracle (each node box holds the spheres, each sphere its triangle).
+ * The search bound starts at (h (1 + 1e-6))^2 and shrinks to the smallest
+ * squared distance found; a node is skipped only when its box distance
+ * exceeds the bound by a relative 1e-9, far above the rounding of either
+ * distance, so every triangle that could hold the minimum is evaluated with
+ * the same tri_dist2.  The result equals orc_dist_exact's bit for bit: the
+ * same minimum when it is below the bound, and h otherwise. */
+void orc_dist_exact_bvh(const double *X, int64_t nnp, const int64_t *IEN,
+                        int64_t nel, const double *pts, int64_t npts, double h,
+                        double *dist, int nthreads) {
+  (void)nnp;
+  sbvh b;
+  memset(&b, 0, sizeof(b));
+  if (nel > 0) sbvh_build(&b, X, IEN, nel);
+  const double hb = h * (1.0 + 1e-6);
+  const double bound0 = hb * hb;
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel
+#endif
+  {
+    int64_t cap = 256;
+    int64_t *stack = (int64_t *)malloc((size_t)cap * sizeof(int64_t));
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 64)
+#endif
+    for (int64_t i = 0; i < npts; i++) {
+      const double *p = pts + 3 * i;
+      double best = INFINITY, bound = bound0;
+      int64_t sp = 0;
+      if (nel > 0) stack[sp++] = 0;
+      while (sp > 0) {
+        const bnode *nd = &b.nodes[stack[--sp]];
+        if (box_dist2(p, nd->lo, nd->hi) > bound * (1.0 + 1e-9)) continue;
+        if (nd->left < 0) {
+          for (int64_t q = nd->first; q < nd->first + nd->count; q++) {
+            const int64_t el = b.perm[q];
+            double cp[3];
+            double d2 = tri_dist2(p, vtx(X, IEN, el, 0), vtx(X, IEN, el, 1),
+                                  vtx(X, IEN, el, 2), cp);
+            if (d2 < best) best = d2;
+            if (d2 < bound) bound = d2;
+          }
+        } else {
+          if (sp + 2 > cap) {
+            cap *= 2;
+            stack = (int64_t *)realloc(stack, (size_t)cap * sizeof(int64_t));
+          }
+          stack[sp++] = nd->right;
+          stack[sp++] = nd->left;
+        }
+      }
+      const double d = sqrt(best);
+      dist[i] = d > h ? h : d;
+    }
+    free(stack);
+  }
+  if (nel > 0) sbvh_free(&b);
 }
 
 /* ------------------------------------------------------------------ */

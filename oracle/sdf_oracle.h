@@ -47,6 +47,12 @@ void orc_dist_exact(const double *X, int64_t nnp, const int64_t *IEN,
                     int64_t nel, const double *pts, int64_t npts, double h,
                     double *dist, double *xp, int nthreads);
 
+/* orc_dist_exact's min(d, h) (no xp) through a bounding-volume hierarchy:
+ * bit-identical results, for meshes too large for the brute-force loop. */
+void orc_dist_exact_bvh(const double *X, int64_t nnp, const int64_t *IEN,
+                        int64_t nel, const double *pts, int64_t npts, double h,
+                        double *dist, int nthreads);
+
 /* Reference-faithful distance, dfOnTriangularMesh.jl:19-209: triangle-centre
  * candidates within 2.5h (sorted by triangle index), 3-nearest-vertex
  * fallback, barycentric face test, edge projections, vertex test only when no

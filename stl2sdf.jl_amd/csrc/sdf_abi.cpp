@@ -141,6 +141,9 @@ struct sdf_ctx {
   std::vector<std::string> ev_name;
   int nev = 0;
   int64_t counters[4] = {0, 0, 0, 0};
+  // the pending lists of the last lattice call (points handed to the vote)
+  const sdf::Pending *last_pend[2] = {nullptr, nullptr};
+  uint64_t last_npend[2] = {0, 0};
   // learned search radius
   sdf::MlpDev mlp;
   bool mlp_alloc = false;
@@ -331,7 +334,10 @@ sdf::RayParams ray_params(int32_t min_rays, int32_t max_rays, double thr,
   sdf::RayParams rp;
   ray_directions(rp.dirs);
   rp.min_rays = min_rays;
-  rp.max_rays = max_rays;
+  // rays past the 36th are skipped by the reference's vote loop
+  // (RaycastingSignDetection.jl:265-271): they change neither the count nor
+  // the confidence, so the vote stops at 36 rays with the same result
+  rp.max_rays = max_rays < sdf::kNumDirs ? max_rays : sdf::kNumDirs;
   rp.conf_thr = thr;
   rp.use_winding = use_winding ? 1 : 0;
   return rp;
@@ -594,6 +600,10 @@ void run_lattice(sdf_ctx *c, const sdf::GridP &g, bool do_dist, bool do_sign,
     read_status(c, st);
     if (!lattice_settle(c, L, st)) {
       c->counters[0] = (int64_t)(st[kStPend] + st[kStPend0]);
+      c->last_pend[0] = L.do_sign ? L.pend0 : nullptr;
+      c->last_pend[1] = L.do_sign ? L.pend : nullptr;
+      c->last_npend[0] = L.do_sign ? std::min<uint64_t>(st[kStPend0], L.cap0) : 0;
+      c->last_npend[1] = L.do_sign ? std::min<uint64_t>(st[kStPend], L.cap) : 0;
       c->counters[1] = (int64_t)st[kStPairs];
       c->counters[2] = (int64_t)st[kStHits];
       c->counters[3] = (int64_t)c->ar.bytes();
@@ -1159,6 +1169,27 @@ sdf_status sdf_ctx_counters(sdf_ctx *c, int64_t *out, int32_t cap, int32_t *n) {
     int w = 0;
     for (int i = 0; i < 4 && i < cap; i++, w++) out[i] = c->counters[i];
     if (n) *n = w;
+  });
+}
+
+sdf_status sdf_ctx_vote_points(sdf_ctx *c, int64_t *idx_out, int64_t cap, int64_t *n) {
+  return guarded([&] {
+    if (!c) fail(SDF_ERR_ARG, "ctx is NULL");
+    if (cap < 0 || (cap > 0 && !idx_out)) fail(SDF_ERR_ARG, "idx_out is NULL");
+    std::lock_guard<std::mutex> lk(c->mu);
+    use_device(c);
+    std::vector<int64_t> all;
+    for (int k = 0; k < 2; k++) {
+      if (!c->last_pend[k] || c->last_npend[k] == 0) continue;
+      std::vector<sdf::Pending> h((size_t)c->last_npend[k]);
+      SDF_HIP(hipMemcpyAsync(h.data(), c->last_pend[k], h.size() * sizeof(sdf::Pending),
+                             hipMemcpyDeviceToHost, c->stream));
+      SDF_HIP(hipStreamSynchronize(c->stream));
+      for (const sdf::Pending &p : h) all.push_back(p.idx);
+    }
+    std::sort(all.begin(), all.end());
+    for (int64_t i = 0; i < cap && i < (int64_t)all.size(); i++) idx_out[i] = all[(size_t)i];
+    if (n) *n = (int64_t)all.size();
   });
 }
 

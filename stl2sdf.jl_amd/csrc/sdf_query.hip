@@ -1169,6 +1169,10 @@ __global__ void vote_kernel(RayParams rp, PendLists lists,
     if (carry) {
       P.inside = inside;
       P.used = used;
+      if (!next) {  // the last phase has no list to hand items to
+        atomicOr(err, kErrPendingOverflow);
+        continue;
+      }
       const uint32_t slot = atomicAdd(nnext, 1u);
       if (slot < next_cap)
         next[slot] = P;

@@ -34,6 +34,7 @@ SIGNATURES = {
     "sdf_ctx_enable_timing": (C.c_int32, [_vp, C.c_int32]),
     "sdf_ctx_stage_times": (C.c_int32, [_vp, C.c_char_p, C.c_int32, _dp, C.c_int32, _i32p]),
     "sdf_ctx_counters": (C.c_int32, [_vp, _i64p, C.c_int32, _i32p]),
+    "sdf_ctx_vote_points": (C.c_int32, [_vp, _i64p, C.c_int64, _i64p]),
     "sdf_grid_setup": (C.c_int32, [_dp, _dp, C.c_int64, C.c_int64, _dp, _dp, _i64p, _dp, _i64p]),
     "sdf_eval_distances": (C.c_int32, [_vp, _dp, C.c_int64, _i64p, C.c_int64, _dp, C.c_double,
                                        _i64p, _dp, C.c_int64, _dp]),
@@ -153,6 +154,15 @@ def stage_times(c):
     check(lib().sdf_ctx_stage_times(c, names, 1024, dptr(ms), 64, C.byref(n)))
     keys = names.value.decode().split(";") if n.value else []
     return dict(zip(keys, ms[: n.value].tolist()))
+
+
+def vote_points(c):
+    """Linear indices of the points the last lattice call sent to the vote."""
+    n = C.c_int64(0)
+    check(lib().sdf_ctx_vote_points(c, None, 0, C.byref(n)))
+    out = np.zeros(max(n.value, 1), dtype=np.int64)
+    check(lib().sdf_ctx_vote_points(c, i64ptr(out), n.value, C.byref(n)))
+    return out[: n.value]
 
 
 def counters(c):

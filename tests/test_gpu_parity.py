@@ -251,3 +251,23 @@ def test_grid_covering_part_of_the_mesh():
         d_ref = oracle.dist_exact(mesh.X, mesh.IEN, P, grid.cell_size)
         assert np.array_equal(np.signbit(sdf), s_ref < 0)
         assert np.max(np.abs(np.abs(sdf) - d_ref)) <= DIST_TOL * _cases.diag(mesh)
+
+
+@pytest.mark.parametrize("min_rays,max_rays", [(6, 40), (6, 36), (5, 37)])
+def test_max_rays_beyond_the_direction_table(min_rays, max_rays):
+    """max_rays above the 36 directions: the reference skips rays past the
+    36th (RaycastingSignDetection.jl:265-271), so the vote ends at 36 rays
+    with the same counts; the lattice path and the general-point path both
+    equal the oracle, which runs the reference's loop unclamped."""
+    mesh, grid = _cases.open_case()
+    P = MeshGrid.generateGridPoints(grid)
+    print(f"case: open mesh, min_rays={min_rays} max_rays={max_rays}", flush=True)
+    s_ref, c_ref = oracle.signs(mesh.X, mesh.IEN, P, min_rays=min_rays, max_rays=max_rays)
+    s, c = SignedDistances.raycast_sign_detection(mesh, grid, None, min_rays=min_rays,
+                                                  max_rays=max_rays)
+    assert np.array_equal(s, s_ref) and np.array_equal(c, c_ref)
+    Q = np.asfortranarray(P[:, ::7] + 1e-3 * grid.cell_size)
+    s2, c2 = SignedDistances.raycast_sign_detection(mesh, grid, Q, min_rays=min_rays,
+                                                    max_rays=max_rays)
+    s2_ref, c2_ref = oracle.signs(mesh.X, mesh.IEN, Q, min_rays=min_rays, max_rays=max_rays)
+    assert np.array_equal(s2, s2_ref) and np.array_equal(c2, c2_ref)

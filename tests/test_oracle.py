@@ -116,9 +116,12 @@ def test_ray_parity_dedup_on_shared_edges():
         assert oracle.ray_inside(X, IEN, c, d)
     # outside point, ray through two faces -> even
     assert not oracle.ray_inside(X, IEN, [-0.5, 0.5, 0.5], [1.0, 0, 0])
-    # outside point whose ray passes exactly along the diagonal of the bottom
-    # face (coplanar: |a| < eps for those triangles) and through far faces
-    assert not oracle.ray_inside(X, IEN, [0.3, 0.3, -1.0], [0, 0, 1.0]) is None
+    # outside point whose ray crosses the bottom and the top face exactly on
+    # their diagonals (two triangles each at the same t, counted once) and
+    # runs parallel to the four side faces (|a| < eps): two crossings
+    assert oracle.ray_inside(X, IEN, [0.3, 0.3, -1.0], [0, 0, 1.0]) is False
+    # the same ray from the cube's inside crosses only the top face
+    assert oracle.ray_inside(X, IEN, [0.3, 0.3, 0.5], [0, 0, 1.0]) is True
 
 
 def test_signs_closed_form_and_bvh_equals_brute():
