@@ -175,6 +175,23 @@ sdf_status sdf_remove_artifacts_device(sdf_ctx *ctx, double *d_sdf,
                                        double min_component_ratio,
                                        int64_t *nflipped);
 
+/* Allocation-free variant of sdf_remove_artifacts_device.  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory
+ * (serving loops, harnesses that forbid hipMalloc, set-up next to HIP graph
+ * capture).  Same arguments and result, bit for bit; it calls no hipMalloc,
+ * hipFree or allocating library routine.  Its scratch (component labels
+ * and sizes, 8 bytes per node) is a static per-device array of
+ * SDF_NOALLOC_MAX_NODES + 1 nodes (64 MiB) that each GPU the library runs on
+ * holds until the process ends; grids with more nodes are rejected with
+ * SDF_ERR_LIMIT.  Every call on one GPU shares that array, from any context
+ * or thread: calls that overlap in time on one GPU race.  Synchronous
+ * (*nflipped is set on return). */
+#define SDF_NOALLOC_MAX_NODES 8388608
+sdf_status sdf_remove_artifacts_device_noalloc(sdf_ctx *ctx, double *d_sdf,
+                                               const int64_t N[3], double threshold,
+                                               double min_component_ratio,
+                                               int64_t *nflipped);
+
 /* analyze_sdf_components(sdf, grid; threshold) -> Dict(root => size)
  * -- SdfArtifactRemoval.jl:259-318.  Roots are canonical: the smallest
  * 0-based linear index of each component.  roots_out/sizes_out receive up to

@@ -88,6 +88,38 @@ void ray_directions(double *d) {
   }
 }
 
+// ---------------------------------------------------------------- static scratch
+// Device memory of sdf_remove_artifacts_device_noalloc: component labels
+// (int32) and sizes (uint32) for up to SDF_NOALLOC_MAX_NODES + 1 nodes, then
+// four 64-bit words, each part 256-byte aligned.  One copy per GPU, held
+// until the process ends; every call on a GPU shares it.
+constexpr size_t kScrNodes = (size_t)SDF_NOALLOC_MAX_NODES + 1;
+constexpr size_t kScrPart = (kScrNodes * 4 + 255) / 256 * 256;
+constexpr size_t kScrLab = 0, kScrSize = kScrPart, kScrWords = 2 * kScrPart;
+constexpr size_t kScratchBytes = 2 * kScrPart + 256;
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// The array's device address, per device (the host-side name is only a
+// stand-in): looked up once per device by the first context created on it,
+// with that device current.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+void *scratch_address(int device) {
+  if (device < 0 || device >= kMaxDevices) fail(SDF_ERR_ARG, "device index beyond 63");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      fail(SDF_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = ptr;
+  }
+  return g_scratch_ptr[device];
+}
+
 }  // namespace
 
 struct sdf_ctx {
@@ -101,6 +133,7 @@ struct sdf_ctx {
   hipEvent_t join[4] = {nullptr, nullptr, nullptr, nullptr};
   std::mutex mu;
   sdf::Arena ar;
+  void *scratch = nullptr;  // this device's static scratch array
   // device mesh
   sdf::Tri64 *tri = nullptr;
   int64_t nel = 0, nnp = 0;
@@ -1058,6 +1091,7 @@ sdf_status sdf_ctx_create(int32_t device, sdf_ctx **out) {
       SDF_HIP(hipMalloc(&c->d_words, 256));
       SDF_HIP(hipMalloc(&c->d_status, 16 * sizeof(uint64_t)));
       SDF_HIP(hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device));
+      c->scratch = scratch_address(device);  // with `device` current
     } catch (...) {
       delete c;
       throw;
@@ -1232,12 +1266,23 @@ sdf::GridP lattice_only(const int64_t N[3]) {
   return make_grid(o, 1.0, N);
 }
 
+// scratch: NULL takes the buffers from the context's arena; otherwise the
+// static scratch array (no allocation, no stage events).
 int64_t remove_artifacts_dev(sdf_ctx *c, double *d_sdf, const sdf::GridP &g,
-                             double thr, double ratio) {
+                             double thr, double ratio, void *scratch = nullptr) {
   if (grid_points(g) > 0x7FFFFFFFll)
     fail(SDF_ERR_LIMIT, "artifact removal supports < 2^31 grid points");
-  c->mark("ccl");
-  sdf::CclResult r = sdf::run_ccl(c->stream, c->ar, g, d_sdf, thr);
+  sdf::CclResult r;
+  if (scratch) {
+    unsigned char *base = (unsigned char *)scratch;
+    r.lab = (int32_t *)(base + kScrLab);
+    r.size = (uint32_t *)(base + kScrSize);
+    r.words = (unsigned long long *)(base + kScrWords);
+    sdf::run_ccl_into(c->stream, g, d_sdf, thr, r);
+  } else {
+    c->mark("ccl");
+    r = sdf::run_ccl(c->stream, c->ar, g, d_sdf, thr);
+  }
   unsigned long long w[2] = {0, 0};
   SDF_HIP(hipMemcpyAsync(w, r.words, sizeof(w), hipMemcpyDeviceToHost, c->stream));
   SDF_HIP(hipStreamSynchronize(c->stream));
@@ -1247,12 +1292,12 @@ int64_t remove_artifacts_dev(sdf_ctx *c, double *d_sdf, const sdf::GridP &g,
   // max(1, round(Int, ratio * largest)) with round-half-even (:209)
   double x = nearbyint(ratio * (double)largest_size);
   uint32_t min_size = x < 1.0 ? 1u : (x > 4.0e9 ? 0xFFFFFFFFu : (uint32_t)x);
-  c->mark("flip");
+  if (!scratch) c->mark("flip");
   sdf::ccl_flip(c->stream, r, largest, min_size, d_sdf);
   unsigned long long f = 0;
   SDF_HIP(hipMemcpyAsync(&f, r.words + 2, sizeof(f), hipMemcpyDeviceToHost, c->stream));
   SDF_HIP(hipStreamSynchronize(c->stream));
-  c->mark("end");
+  if (!scratch) c->mark("end");
   return (int64_t)f;
 }
 
@@ -1289,6 +1334,26 @@ sdf_status sdf_remove_artifacts_device(sdf_ctx *c, double *d_sdf, const int64_t 
     if (grid_points(g) > 0 && !d_sdf) fail(SDF_ERR_ARG, "sdf is NULL");
     begin_call(c);
     int64_t f = remove_artifacts_dev(c, d_sdf, g, thr, ratio);
+    if (nflipped) *nflipped = f;
+  });
+}
+
+sdf_status sdf_remove_artifacts_device_noalloc(sdf_ctx *c, double *d_sdf, const int64_t N[3],
+                                               double thr, double ratio,
+                                               int64_t *nflipped) {
+  return guarded([&] {
+    if (!c) fail(SDF_ERR_ARG, "ctx is NULL");
+    std::lock_guard<std::mutex> lk(c->mu);
+    sdf::GridP g = lattice_only(N);
+    const int64_t n = grid_points(g);
+    if (n > (int64_t)SDF_NOALLOC_MAX_NODES)
+      fail(SDF_ERR_LIMIT, "grid has " + std::to_string(n) + " nodes; the allocation-free "
+                          "entry point takes at most " +
+                              std::to_string((int64_t)SDF_NOALLOC_MAX_NODES));
+    if (n > 0 && !d_sdf) fail(SDF_ERR_ARG, "sdf is NULL");
+    if (!c->scratch) fail(SDF_ERR_INTERNAL, "no scratch address for this device");
+    use_device(c);
+    const int64_t f = remove_artifacts_dev(c, d_sdf, g, thr, ratio, c->scratch);
     if (nflipped) *nflipped = f;
   });
 }

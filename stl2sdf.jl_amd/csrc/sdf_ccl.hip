@@ -197,13 +197,22 @@ CclResult run_ccl(hipStream_t s, Arena &ar, const GridP &g, const double *sdf,
                   double thr) {
   const int64_t n = (int64_t)g.nx * g.ny * g.nz;
   CclResult r;
-  r.n = n;
   r.lab = ar.get<int32_t>("ccl_lab", n + 1);
   r.size = ar.get<uint32_t>("ccl_size", n + 1);
   r.words = ar.get<unsigned long long>("ccl_words", 4);
+  run_ccl_into(s, g, sdf, thr, r);
+  return r;
+}
+
+void run_ccl_into(hipStream_t s, const GridP &g, const double *sdf, double thr,
+                  CclResult &r) {
+  const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+  r.n = n;
+  // every word the kernels accumulate into starts at zero (fill kernels, so
+  // a buffer that kept an earlier call's contents is cleared the same way)
   launch_fill_u64(s, (uint64_t *)r.words, 4, 0ull);
   launch_fill_u32(s, r.size, n + 1, 0u);
-  if (n == 0) return r;
+  if (n == 0) return;
   hipLaunchKernelGGL(ccl_init_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, s,
                      sdf, n, thr, r.lab);
   SDF_HIP(hipGetLastError());
@@ -216,7 +225,6 @@ CclResult run_ccl(hipStream_t s, Arena &ar, const GridP &g, const double *sdf,
   hipLaunchKernelGGL(ccl_largest_kernel, dim3(blocks_for(n, 256) < 2048 ? blocks_for(n, 256) : 2048),
                      dim3(256), 0, s, n, r.lab, r.size, r.words, r.words + 1);
   SDF_HIP(hipGetLastError());
-  return r;
 }
 
 void ccl_flip(hipStream_t s, const CclResult &r, int32_t largest,

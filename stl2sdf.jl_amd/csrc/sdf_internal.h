@@ -354,6 +354,10 @@ struct CclResult {
   uint32_t *size = nullptr; // component size, indexed by root
   unsigned long long *words = nullptr;  // [0] largest packed, [1] ncomp, [2] flipped
 };
+// Labels and sizes into caller-given buffers (n + 1 entries each, 4 words):
+// allocates nothing.
+void run_ccl_into(hipStream_t s, const GridP &g, const double *sdf, double thr,
+                  CclResult &r);
 CclResult run_ccl(hipStream_t s, Arena &ar, const GridP &g, const double *sdf,
                   double thr);
 void ccl_flip(hipStream_t s, const CclResult &r, int32_t largest,

@@ -71,6 +71,8 @@ SIGNATURES = {
     "sdf_remove_artifacts": (C.c_int32, [_vp, _dp, _i64p, C.c_double, C.c_double, _i64p]),
     "sdf_remove_artifacts_device": (C.c_int32, [_vp, _vp, _i64p, C.c_double, C.c_double,
                                                 _i64p]),
+    "sdf_remove_artifacts_device_noalloc": (C.c_int32, [_vp, _vp, _i64p, C.c_double,
+                                                        C.c_double, _i64p]),
     "sdf_analyze_components": (C.c_int32, [_vp, _dp, _i64p, C.c_double, _i64p, _i64p, _i64p,
                                            C.c_int64, _i64p]),
     "sdf_rbf_smoothing": (C.c_int32, [_vp, _dp, _i64p, C.c_double, C.c_int32, C.c_int32,
