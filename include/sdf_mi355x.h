@@ -218,6 +218,18 @@ sdf_status sdf_rbf_smoothing(sdf_ctx *ctx, const double *dist,
                              double threshold, double reltol, float *fine_out,
                              int32_t *iters_out, double *rel_residual_out);
 
+/* import_stl(filename) -> (X, IEN) -- src/DataImport/ImportStlData.jl:10-27
+ * with the package's readers (:102-196: a file whose first five bytes are
+ * "solid" is read as ASCII with Float64 coordinates, any other as binary
+ * with float32 ones) and process_stl_mesh's merge (:198-309: each triangle
+ * corner joins the first earlier unique vertex within 1e-10 in its 1e-10
+ * hash cell, then in the 26 neighbouring cells in the reference's order).
+ * Host code, no device work.  Two calls: with X and IEN NULL, *nnp and *nel
+ * receive the sizes; then, with *nnp / *nel the buffer capacities, X
+ * (3 x nnp doubles) and IEN (3 x nel int64, 1-based) are filled. */
+sdf_status sdf_import_stl(const char *path, double *X, int64_t *nnp, int64_t *IEN,
+                          int64_t *nel);
+
 /* The 36 prioritised ray directions (generate_ray_directions_prioritized,
  * RaycastingSignDetection.jl:162-227), 3 doubles each. */
 sdf_status sdf_ray_directions(double *dirs_out /* 108 */);

@@ -61,6 +61,8 @@ def lib():
         L.orc_rbf_fine.argtypes = [_fp, _dp, _dp, _i64p, C.c_double, C.c_int, C.c_double, _fp,
                                    C.c_int]
         L.orc_read_stl_binary.argtypes = [C.c_char_p, _dp, _i64p, _i64p, _i64p]
+        L.orc_process_stl_mesh.argtypes = [_dp, C.c_int64, _dp, _i64p]
+        L.orc_process_stl_mesh.restype = C.c_int64
         L.orc_read_stl_binary.restype = C.c_int
         L.orc_bf16.argtypes = [C.c_float]
         L.orc_bf16.restype = C.c_float
@@ -219,6 +221,17 @@ def read_stl_binary(path):
     if rc != 0:
         raise IOError(f"orc_read_stl_binary failed: {rc}")
     return X, IEN
+
+
+def process_stl_mesh(corners):
+    """process_stl_mesh on triangle corners (n*3 x 3): (X 3 x nnp, IEN 3 x nel)."""
+    V = np.ascontiguousarray(np.asarray(corners, np.float64).reshape(-1, 3))
+    nv = V.shape[0]
+    X = np.empty(3 * max(nv, 1))
+    IEN = np.empty(max(nv, 1), np.int64)
+    nu = lib().orc_process_stl_mesh(_d(V), nv, _d(X), _i(IEN))
+    return (np.asfortranarray(X[:3 * nu].reshape(nu, 3).T),
+            np.asfortranarray(IEN[:nv].reshape(-1, 3).T))
 
 
 # ---------------------------------------------------------------- MLP

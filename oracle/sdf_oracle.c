@@ -1220,42 +1220,19 @@ static uint64_t hkey(int64_t a, int64_t b, int64_t c) {
   return h;
 }
 
-int orc_read_stl_binary(const char *path, double *X, int64_t *nnp,
-                        int64_t *IEN, int64_t *nel) {
-  FILE *f = fopen(path, "rb");
-  if (!f) return -1;
-  unsigned char hdr[80];
-  uint32_t nt = 0;
-  if (fread(hdr, 1, 80, f) != 80 || fread(&nt, 4, 1, f) != 1) {
-    fclose(f);
-    return -2;
-  }
-  float *raw = (float *)malloc((size_t)nt * 9 * sizeof(float) + 1);
-  for (uint32_t t = 0; t < nt; t++) {
-    float rec[12];
-    uint16_t attr;
-    if (fread(rec, 4, 12, f) != 12 || fread(&attr, 2, 1, f) != 1) {
-      free(raw);
-      fclose(f);
-      return -3;
-    }
-    memcpy(raw + 9 * (size_t)t, rec + 3, 9 * sizeof(float));
-  }
-  fclose(f);
-  /* process_stl_mesh (ImportStlData.jl:198-309) */
+/* process_stl_mesh (ImportStlData.jl:198-309) over nv triangle corners V
+ * (3 doubles each): the unique vertices into U (3 per vertex, capacity nv)
+ * and each corner's 1-based vertex index into tri; returns the count. */
+static int64_t stl_merge(const double *V, int64_t nv, double *U, int64_t *tri) {
   const double tol = 1e-10;
   const double scale = floor(1.0 / tol);
-  int64_t nv = 3 * (int64_t)nt;
   int64_t cap = 1;
   while (cap < 2 * nv + 16) cap <<= 1;
   hslot *tab = (hslot *)calloc((size_t)cap, sizeof(hslot));
   int64_t *next = (int64_t *)malloc((size_t)(nv + 1) * sizeof(int64_t));
-  double *U = (double *)malloc((size_t)(3 * nv + 3) * sizeof(double));
-  int64_t *tri = (int64_t *)malloc((size_t)(nv + 1) * sizeof(int64_t));
   int64_t nu = 0;
   for (int64_t q = 0; q < nv; q++) {
-    double v[3] = {(double)raw[3 * q], (double)raw[3 * q + 1],
-                   (double)raw[3 * q + 2]};
+    double v[3] = {V[3 * q], V[3 * q + 1], V[3 * q + 2]};
     int64_t g[3];
     double pic[3];
     for (int d = 0; d < 3; d++) {
@@ -1326,6 +1303,51 @@ int orc_read_stl_binary(const char *path, double *X, int64_t *nnp,
     }
     tri[q] = found + 1;
   }
+  free(tab);
+  free(next);
+  return nu;
+}
+
+int64_t orc_process_stl_mesh(const double *V, int64_t nv, double *X, int64_t *IEN) {
+  double *U = (double *)malloc((size_t)(3 * nv + 3) * sizeof(double));
+  int64_t *tri = (int64_t *)malloc((size_t)(nv + 1) * sizeof(int64_t));
+  const int64_t nu = stl_merge(V, nv, U, tri);
+  if (X) memcpy(X, U, (size_t)(3 * nu) * sizeof(double));
+  if (IEN) memcpy(IEN, tri, (size_t)nv * sizeof(int64_t));
+  free(U);
+  free(tri);
+  return nu;
+}
+
+int orc_read_stl_binary(const char *path, double *X, int64_t *nnp,
+                        int64_t *IEN, int64_t *nel) {
+  FILE *f = fopen(path, "rb");
+  if (!f) return -1;
+  unsigned char hdr[80];
+  uint32_t nt = 0;
+  if (fread(hdr, 1, 80, f) != 80 || fread(&nt, 4, 1, f) != 1) {
+    fclose(f);
+    return -2;
+  }
+  float *raw = (float *)malloc((size_t)nt * 9 * sizeof(float) + 1);
+  for (uint32_t t = 0; t < nt; t++) {
+    float rec[12];
+    uint16_t attr;
+    if (fread(rec, 4, 12, f) != 12 || fread(&attr, 2, 1, f) != 1) {
+      free(raw);
+      fclose(f);
+      return -3;
+    }
+    memcpy(raw + 9 * (size_t)t, rec + 3, 9 * sizeof(float));
+  }
+  fclose(f);
+  int64_t nv = 3 * (int64_t)nt;
+  double *V = (double *)malloc((size_t)(3 * nv + 3) * sizeof(double));
+  for (int64_t q = 0; q < 3 * nv; q++) V[q] = (double)raw[q];
+  double *U = (double *)malloc((size_t)(3 * nv + 3) * sizeof(double));
+  int64_t *tri = (int64_t *)malloc((size_t)(nv + 1) * sizeof(int64_t));
+  int64_t nu = stl_merge(V, nv, U, tri);
+  free(V);
   int rc = 0;
   if (X && IEN) {
     if (*nnp < nu || *nel < (int64_t)nt)
@@ -1338,8 +1360,6 @@ int orc_read_stl_binary(const char *path, double *X, int64_t *nnp,
   *nnp = nu;
   *nel = nt;
   free(raw);
-  free(tab);
-  free(next);
   free(U);
   free(tri);
   return rc;

@@ -111,6 +111,11 @@ void orc_rbf_fine(const float *weights, const double aabb_min[3],
 int orc_read_stl_binary(const char *path, double *X, int64_t *nnp,
                         int64_t *IEN, int64_t *nel);
 
+/* process_stl_mesh (ImportStlData.jl:198-309) on nv triangle corners V
+ * (3 doubles each, 3 corners per triangle): X (3 x nnp, capacity 3 nv) and
+ * IEN (nv 1-based indices) when not NULL; returns nnp. */
+int64_t orc_process_stl_mesh(const double *V, int64_t nv, double *X, int64_t *IEN);
+
 #ifdef __cplusplus
 }
 #endif

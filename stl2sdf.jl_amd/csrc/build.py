@@ -16,7 +16,7 @@ LIB_DIR = os.path.join(PKG, "lib")
 LIB = os.path.join(LIB_DIR, "libsdf_mi355x.so")
 OBJ_DIR = os.path.join(PKG, "_build")
 
-SOURCES = ["sdf_build.hip", "sdf_query.hip", "sdf_lbvh.hip", "sdf_ccl.hip", "sdf_rbf.hip", "sdf_mlp.hip", "sdf_closest.hip", "sdf_abi.cpp"]
+SOURCES = ["sdf_build.hip", "sdf_query.hip", "sdf_lbvh.hip", "sdf_ccl.hip", "sdf_rbf.hip", "sdf_mlp.hip", "sdf_closest.hip", "sdf_stl.cpp", "sdf_abi.cpp"]
 HEADERS = ["sdf_common.h", "sdf_internal.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -27,6 +27,11 @@
 
 using sdf::Error;
 
+namespace sdf {
+bool import_stl_file(const char *path, std::vector<double> &X, std::vector<int64_t> &IEN,
+                     std::string &err);  // sdf_stl.cpp
+}
+
 namespace {
 
 thread_local std::string g_last_error;
@@ -1453,6 +1458,28 @@ sdf_status sdf_rbf_smoothing(sdf_ctx *c, const double *dist, const int64_t N[3],
     SDF_HIP(hipStreamSynchronize(c->stream));
     if (iters_out) *iters_out = iters;
     if (rel_out) *rel_out = rel;
+  });
+}
+
+sdf_status sdf_import_stl(const char *path, double *X, int64_t *nnp, int64_t *IEN,
+                          int64_t *nel) {
+  return guarded([&] {
+    if (!path || !nnp || !nel) fail(SDF_ERR_ARG, "NULL argument");
+    if ((X == nullptr) != (IEN == nullptr)) fail(SDF_ERR_ARG, "X and IEN must both be given");
+    std::vector<double> hx;
+    std::vector<int64_t> hi;
+    std::string err;
+    if (!sdf::import_stl_file(path, hx, hi, err)) fail(SDF_ERR_ARG, err);
+    const int64_t nv = (int64_t)(hx.size() / 3), nt = (int64_t)(hi.size() / 3);
+    if (X) {
+      if (*nnp < nv || *nel < nt)
+        fail(SDF_ERR_ARG, "X / IEN smaller than the mesh (" + std::to_string(nv) +
+                              " vertices, " + std::to_string(nt) + " triangles)");
+      memcpy(X, hx.data(), hx.size() * sizeof(double));
+      memcpy(IEN, hi.data(), hi.size() * sizeof(int64_t));
+    }
+    *nnp = nv;
+    *nel = nt;
   });
 }
 

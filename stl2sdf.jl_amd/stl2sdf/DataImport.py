@@ -1,13 +1,16 @@
-"""STL import (host side), mirroring src/DataImport/ImportStlData.jl.
+"""STL import, mirroring src/DataImport/ImportStlData.jl.
 
-import_stl(path) -> (X, IEN): X 3 x nnp float64, IEN 3 x nel int64 1-based.
-Binary and ASCII files are read like the reference's readers (:127-196).
-Vertex de-duplication follows process_stl_mesh (:198-309): a vertex joins
-the first earlier unique vertex within 1e-10; STL coordinates come from
-float32 (binary) or decimal text (ASCII), so vertices within 1e-10 of each
-other are bitwise equal in practice and the first-occurrence order of
-``np.unique`` reproduces the reference's numbering.
+import_stl(path) -> (X, IEN): X 3 x nnp float64, IEN 3 x nel int64 1-based,
+read by the library's native reader (sdf_import_stl): the reference's own
+readers (:102-196; a file starting with "solid" is ASCII with Float64
+coordinates, any other binary with float32 ones) and process_stl_mesh's
+vertex merge (:198-309: a corner joins the first earlier unique vertex
+within 1e-10, searched in its 1e-10 hash cell and then its 26 neighbours in
+the reference's order).  The numpy readers below return the raw corners
+(3 per triangle) for tests and tools.
 """
+import ctypes as C
+
 import numpy as np
 
 
@@ -26,32 +29,40 @@ def read_binary_stl(path):
 
 
 def read_ascii_stl(path):
-    verts = []
+    """Corners of read_ascii_stl (ImportStlData.jl:163-196): a short "vertex"
+    line counts but adds no vertex, and a triangle is the last three vertices
+    added whenever a vertex makes the count a multiple of three."""
+    verts, corners, count = [], [], 0
     with open(path, "r", errors="replace") as f:
         for line in f:
             s = line.strip()
             if s.startswith("vertex"):
+                count += 1
                 parts = s.split()
                 if len(parts) >= 4:
                     verts.append([float(parts[1]), float(parts[2]), float(parts[3])])
-    V = np.array(verts, dtype=np.float64).reshape(-1, 3)
-    return V[: 3 * (len(V) // 3)]
-
-
-def process_stl_mesh(V):
-    """Dedup per-face vertices (n*3 x 3) -> (X 3 x nnp, IEN 3 x nel)."""
-    if len(V) == 0:
-        return np.zeros((3, 0), order="F"), np.zeros((3, 0), np.int64, order="F")
-    _, first, inv = np.unique(V, axis=0, return_index=True, return_inverse=True)
-    order = np.argsort(first, kind="stable")          # first-occurrence order
-    rank = np.empty_like(order)
-    rank[order] = np.arange(len(order))
-    X = V[first[order]]
-    IEN = (rank[inv.reshape(-1)] + 1).reshape(-1, 3)
-    return np.asfortranarray(X.T), np.asfortranarray(IEN.T.astype(np.int64))
+                    if count % 3 == 0:
+                        if len(verts) < 3:
+                            raise ValueError("triangle before the first vertex")
+                        corners += verts[-3:]
+    return np.array(corners, dtype=np.float64).reshape(-1, 3)
 
 
 def import_stl(path):
-    """import_stl -- ImportStlData.jl:10-27."""
-    V = read_ascii_stl(path) if _detect_ascii(path) else read_binary_stl(path)
-    return process_stl_mesh(V)
+    """import_stl -- ImportStlData.jl:10-27 (native reader and merge)."""
+    from . import _lib
+    L = _lib.lib()
+    nnp, nel = C.c_int64(0), C.c_int64(0)
+    p = str(path).encode()
+    _lib.check(L.sdf_import_stl(p, None, C.byref(nnp), None, C.byref(nel)))
+    X = np.empty((3, nnp.value), order="F")
+    IEN = np.empty((3, nel.value), np.int64, order="F")
+    _lib.check(L.sdf_import_stl(p, _lib.dptr(X), C.byref(nnp), _lib.i64ptr(IEN),
+                                C.byref(nel)))
+    return X, IEN
+
+
+def read_corners(path):
+    """The raw triangle corners (n*3 x 3 float64) as the reference's readers
+    produce them, before the merge."""
+    return read_ascii_stl(path) if _detect_ascii(path) else read_binary_stl(path)

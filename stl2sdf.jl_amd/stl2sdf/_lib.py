@@ -48,6 +48,7 @@ SIGNATURES = {
     "sdf_mesh_set_device": (C.c_int32, [_vp, _vp, C.c_int64, _vp, C.c_int64]),
     "sdf_signed_distance_device": (C.c_int32, [_vp, _dp, C.c_double, _i64p, _vp]),
     "sdf_ray_directions": (C.c_int32, [_dp]),
+    "sdf_import_stl": (C.c_int32, [C.c_char_p, _dp, _i64p, _i64p, _i64p]),
     "sdf_mesh_set": (C.c_int32, [_vp, _dp, C.c_int64, _i64p, C.c_int64]),
     "sdf_batch_create": (C.c_int32, [_vp, C.c_int32, C.POINTER(_vp), _i64p, C.POINTER(_vp),
                                      _i64p, _dp, _dp, _i64p, C.POINTER(_vp), C.c_int32,
