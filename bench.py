@@ -145,11 +145,13 @@ def cpu_baseline(mesh, grid, target_s, threads):
 
     n1 = 1 << 14
     t1, ts1 = median3(n1)
-    # per-point cost from a probe twice the size, then the size for target_s
-    tp = run(2 * n1)
-    per_pt = max((tp - t1) / n1, 1e-9)
-    n2 = int(min(max(target_s / 3.0 - t1, 1.0) / per_pt + n1, 1 << 24))
-    n2 = max(n2, 4 * n1)
+    # per-point cost from one probe of 2^18 points, then the size whose runs
+    # take about target_s / 3 each
+    np_ = 1 << 18
+    tp = run(np_)
+    per_pt = max((tp - t1) / (np_ - n1), 1e-9)
+    n2 = int((max(target_s / 3.0 - t1, 0.5)) / per_pt) + n1
+    n2 = min(max(n2, np_), 1 << 24)
     t2, ts2 = median3(n2)
     slope = (t2 - t1) / max(n2 - n1, 1)
     build_s = max(t1 - slope * n1, 0.0)

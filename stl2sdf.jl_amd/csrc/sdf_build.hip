@@ -880,6 +880,59 @@ __global__ void line_prep_kernel(GridP g, const uint64_t *__restrict__ line_off,
   }
 }
 
+// Segment bytes (SegLayout) of every axis line, one thread per line: per
+// segment, the uniformity test of the sign kernel on the line's summary
+// (seg_uniform on the crossings' segment ranges, or inline_uniform on the
+// segment's coordinate range when the line has none), so the sign kernel
+// reads one byte per line and brick.  With segment ranges and at most 128
+// segments the test runs on all segments at once (seg_words_masks).
+// Consecutive threads take
+// consecutive lines of an axis, whose words of one segment group are
+// adjacent.
+__global__ __launch_bounds__(256) void seg_bytes_kernel(GridP g,
+                                                        const LineInline *__restrict__ inl,
+                                                        int64_t base1, int64_t base2,
+                                                        int64_t nlines, SegLayout sl,
+                                                        uint32_t *__restrict__ segw) {
+  for (int64_t L = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; L < nlines;
+       L += (int64_t)gridDim.x * blockDim.x) {
+    const int ax = L < base1 ? 0 : (L < base2 ? 1 : 2);
+    const int64_t l = L - (ax == 0 ? 0 : (ax == 1 ? base1 : base2));
+    const LineTail r = load_line_tail(inl + L);
+    uint32_t *out = segw + sl.off[ax] + l;
+    const int64_t stride = sl.nl[ax];
+    const int ns = sl.ns[ax];
+    const int nw = (ns + 3) / 4;
+    if ((r.info & kLineSeg) && ns <= 128) {
+      seg_words_masks(r, ns, [&](int w, uint32_t word) { out[(int64_t)w * stride] = word; });
+    } else {
+      const LineInline li = inl[L];
+      const double oa = ax == 0 ? g.ox : (ax == 1 ? g.oy : g.oz);
+      const int32_t na = ax == 0 ? g.nx : (ax == 1 ? g.ny : g.nz);
+      const int len = kSegLen[ax];
+      const bool seg = (r.info & kLineSeg) != 0u;
+      uint32_t word = 0u;
+      for (int sgi = 0; sgi < ns; sgi++) {
+        int p, q;
+        bool u;
+        if (seg) {
+          u = seg_uniform(r, sgi, &p, &q);
+        } else {
+          const int64_t e = (int64_t)sgi * len + len - 1;
+          const double lo = lattice_coord(oa, g.h, (int64_t)sgi * len);
+          const double hi = lattice_coord(oa, g.h, e < na - 1 ? e : na - 1);
+          u = inline_uniform(li, lo, hi, &p, &q);
+        }
+        word |= (u ? (uint32_t)(1 | (p << 1) | (q << 2)) : 0u) << (8 * (sgi & 3));
+        if ((sgi & 3) == 3 || sgi == ns - 1) {
+          out[(int64_t)(sgi >> 2) * stride] = word;
+          word = 0u;
+        }
+      }
+    }
+  }
+}
+
 uint64_t read_u64(hipStream_t s, const uint64_t *d) {
   uint64_t v = 0;
   SDF_HIP(hipMemcpyAsync(&v, d, sizeof(v), hipMemcpyDeviceToHost, s));
@@ -1091,8 +1144,14 @@ void line_hits_finish(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
                      dim3(256), 0, s, g,
                      (const uint64_t *)lb.loff, (const LineHit *)hits, (uint64_t)cap, lb.nlx,
                      lb.nlx + lb.nly, lb.nlines, fast, meta, inl);
+  const SegLayout sl = seg_layout(g);
+  uint32_t *segw = ar.get<uint32_t>("line_segw", (size_t)sl.total + 1);
+  hipLaunchKernelGGL(seg_bytes_kernel, dim3(capped_grid(lb.nlines, 256, lb.max_blocks)),
+                     dim3(256), 0, s, g, (const LineInline *)inl, lb.nlx, lb.nlx + lb.nly,
+                     lb.nlines, sl, segw);
   SDF_HIP(hipGetLastError());
   b.line_off = lb.loff;
+  b.seg_words = segw;
   b.line_hits = hits;
   b.line_fast = fast;
   b.line_meta = meta;

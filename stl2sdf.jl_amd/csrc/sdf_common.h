@@ -193,6 +193,11 @@ struct LineFast {
 constexpr uint32_t kLineExact = 0x80000000u;  // line meta flag: no fast path
 constexpr int kLineFastMax = 64;             // lines with more hits: exact path
 
+// Sign bricks: 8 x 8 lattice columns x kSignZ points per 64-lane wave.
+constexpr int kSignZ = 4;
+// Points per sign-brick segment of an x-, y- and z-line.
+constexpr int kSegLen[3] = {8, 8, kSignZ};
+
 // Inline copy of a short line's summary (up to 4 hits) indexed directly by
 // line id: one 64-byte load instead of offsets + records.  info: bits 0-7
 // hit count (255 = more than 254), 8-15 cluster count, bit 30 kLineSeg,
@@ -229,6 +234,171 @@ static_assert(sizeof(LineInline) == 64 && __builtin_offsetof(LineInline, cid) ==
 __device__ inline LineTail load_line_tail(const LineInline *p) {
   const uint4 v = *reinterpret_cast<const uint4 *>(reinterpret_cast<const char *>(p) + 48);
   return LineTail{v.x, v.y, v.z, v.w};
+}
+
+// Both rays of a line for every point whose coordinate lies in [lo, hi],
+// when no crossing is within its rounding margin of that range: each
+// crossing is then right of every point (fl(c - hi) > eps + D) or left of
+// every point (fl(c - lo) < -eps - D).  Rounded subtraction is monotone, so
+// every point's comparisons in inline_parity come out the same as at the
+// range's ends, none of them ambiguous; the parities are shared.  Returns
+// false when some crossing is near the range (or the summary is not
+// usable); the caller then classifies each point.
+__host__ __device__ inline bool inline_uniform(const LineInline &r, double lo, double hi, int *pos,
+                                      int *neg) {
+  // selects, not branches: the per-hit cases would otherwise cost scalar
+  // exec-mask work on every call
+  const uint32_t n = r.info & 0xFFu;
+  const bool usable = !(r.info & kLineExact) && n <= (uint32_t)kInlineHits;
+  const int ncl = (int)((r.info >> 8) & 0xFFu);
+  int first_pos = -1, last_neg = -1;
+  bool uni = true;
+#pragma unroll
+  for (int h2 = 0; h2 < kInlineHits; h2++) {
+    const bool valid = h2 < (int)n;
+    const double D = (double)r.delta[h2];
+    const int cid = r.cid[h2];
+    const bool above = r.c[h2] - hi > kRayEps + D;
+    const bool below = !above & (r.c[h2] - lo < -kRayEps - D);
+    first_pos = (valid & above & (first_pos < 0)) ? cid : first_pos;
+    last_neg = (valid & below) ? cid : last_neg;
+    uni = uni & (!valid | above | below);
+  }
+  *pos = first_pos >= 0 ? ((ncl - first_pos) & 1) : 0;
+  *neg = last_neg >= 0 ? ((last_neg + 1) & 1) : 0;
+  return usable & uni;
+}
+
+// inline_uniform for segment s of the line from the crossings' segment
+// ranges (kLineSeg): the same decisions by integer compares.
+__host__ __device__ inline bool seg_uniform(const LineTail &r, int s, int *pos, int *neg) {
+  const uint32_t n = r.info & 0xFFu;
+  const bool usable = !(r.info & kLineExact) && n <= (uint32_t)kInlineHits;
+  const int ncl = (int)((r.info >> 8) & 0xFFu);
+  int first_pos = -1, last_neg = -1;
+  bool uni = true;
+#pragma unroll
+  for (int h2 = 0; h2 < kInlineHits; h2++) {
+    const bool valid = h2 < (int)n;
+    const int cid = (int)((r.cid4 >> (8 * h2)) & 0xFFu);
+    const uint32_t sg = ((h2 < 2 ? r.seg01 : r.seg23) >> (16 * (h2 & 1))) & 0xFFFFu;
+    const int a = (int)(sg & 0xFFu), b = (int)(sg >> 8);
+    const bool above = s < a;
+    const bool below = !above & (s >= b);
+    first_pos = (valid & above & (first_pos < 0)) ? cid : first_pos;
+    last_neg = (valid & below) ? cid : last_neg;
+    uni = uni & (!valid | above | below);
+  }
+  *pos = first_pos >= 0 ? ((ncl - first_pos) & 1) : 0;
+  *neg = last_neg >= 0 ? ((last_neg + 1) & 1) : 0;
+  return usable & uni;
+}
+
+// Sign-brick segment bytes: for every axis line and every segment of it
+// (kSegLen points), bit 0 is set when the segment's points compare every
+// crossing of the line the same way, so they share the line's two ray
+// parities, which bits 1 (+axis ray) and 2 (-axis ray) hold; a segment
+// that is not uniform has byte 0.  Four consecutive segments of a line share
+// a 32-bit word (segment s in byte s % 4 of word s / 4).  Layout, in words:
+// x-lines [wx][ny nz], then y-lines [wy][nx nz], then z-lines [wz][nx ny],
+// each line index as in the line lists (x: k ny + j, y: k nx + i,
+// z: j nx + i), w = ceil(segments / 4).
+struct SegLayout {
+  int64_t off[3];  // first word of each axis' part
+  int64_t nl[3];   // lines per axis (the stride between words of a line)
+  int32_t ns[3];   // segments per line
+  int32_t pad_;
+  int64_t total;   // words
+};
+__host__ __device__ inline SegLayout seg_layout(const GridP &g) {
+  SegLayout sl;
+  sl.nl[0] = (int64_t)g.ny * g.nz;
+  sl.nl[1] = (int64_t)g.nx * g.nz;
+  sl.nl[2] = (int64_t)g.nx * g.ny;
+  const int32_t na[3] = {g.nx, g.ny, g.nz};
+  int64_t off = 0;
+  for (int a = 0; a < 3; a++) {
+    sl.ns[a] = (na[a] + kSegLen[a] - 1) / kSegLen[a];
+    sl.off[a] = off;
+    off += (int64_t)((sl.ns[a] + 3) / 4) * sl.nl[a];
+  }
+  sl.pad_ = 0;
+  sl.total = off;
+  return sl;
+}
+
+// Bits [0, a) of a 128-bit mask (lo, hi), a in 0..128.
+__host__ __device__ inline void low_bits128(int a, uint64_t &lo, uint64_t &hi) {
+  lo = a >= 64 ? ~0ull : (a <= 0 ? 0ull : ((1ull << a) - 1ull));
+  hi = a >= 128 ? ~0ull : (a <= 64 ? 0ull : ((1ull << (a - 64)) - 1ull));
+}
+
+// Bits 0-3 of a nibble to bits 0, 8, 16, 24.
+__host__ __device__ inline uint32_t spread_nibble(uint32_t x) {
+  return (x & 1u) | ((x & 2u) << 7) | ((x & 4u) << 14) | ((x & 8u) << 21);
+}
+
+// The segment words of a line with segment ranges (kLineSeg) and ns <= 128
+// segments: seg_uniform for every segment at once, as 128-bit masks.
+// Crossing h is above the segments of A_h = [0, a_h) and below those of
+// B_h = [b_h, ns) minus A_h; a segment is uniform when every crossing is
+// above or below it, its +ray parity comes from the first crossing above it
+// (A_h minus the earlier A's), its -ray parity from the last one below it
+// (B_h minus the later B's).  put(w, word) receives words 0 .. (ns+3)/4-1.
+template <typename Put>
+__host__ __device__ inline void seg_words_masks(const LineTail &r, int ns, Put put) {
+  const uint32_t n = r.info & 0xFFu;
+  const bool usable = !(r.info & kLineExact) && n <= (uint32_t)kInlineHits;
+  const int ncl = (int)((r.info >> 8) & 0xFFu);
+  uint64_t Nlo, Nhi;
+  low_bits128(ns, Nlo, Nhi);
+  uint64_t ulo = usable ? Nlo : 0ull, uhi = usable ? Nhi : 0ull;
+  uint64_t plo = 0ull, phi = 0ull, qlo = 0ull, qhi = 0ull;
+  uint64_t salo = 0ull, sahi = 0ull;  // union of the earlier A's
+  uint64_t blo[kInlineHits], bhi[kInlineHits];
+  int cid[kInlineHits];
+#pragma unroll
+  for (int h2 = 0; h2 < kInlineHits; h2++) {
+    const bool valid = usable && h2 < (int)n;
+    cid[h2] = (int)((r.cid4 >> (8 * h2)) & 0xFFu);
+    const uint32_t sg = ((h2 < 2 ? r.seg01 : r.seg23) >> (16 * (h2 & 1))) & 0xFFFFu;
+    uint64_t alo, ahi, lbl, lbh;
+    low_bits128((int)(sg & 0xFFu), alo, ahi);
+    low_bits128((int)(sg >> 8), lbl, lbh);
+    alo &= Nlo;
+    ahi &= Nhi;
+    const uint64_t b_lo = Nlo & ~lbl & ~alo, b_hi = Nhi & ~lbh & ~ahi;
+    blo[h2] = valid ? b_lo : 0ull;
+    bhi[h2] = valid ? b_hi : 0ull;
+    if (valid) {
+      ulo &= alo | b_lo;
+      uhi &= ahi | b_hi;
+      if ((ncl - cid[h2]) & 1) {
+        plo |= alo & ~salo;
+        phi |= ahi & ~sahi;
+      }
+      salo |= alo;
+      sahi |= ahi;
+    }
+  }
+  uint64_t sblo = 0ull, sbhi = 0ull;  // union of the later B's
+#pragma unroll
+  for (int h2 = kInlineHits - 1; h2 >= 0; h2--) {
+    if ((cid[h2] + 1) & 1) {
+      qlo |= blo[h2] & ~sblo;
+      qhi |= bhi[h2] & ~sbhi;
+    }
+    sblo |= blo[h2];
+    sbhi |= bhi[h2];
+  }
+  const int nw = (ns + 3) / 4;
+  for (int w = 0; w < nw; w++) {
+    const int s0 = 4 * w;
+    const uint32_t u = (uint32_t)((s0 < 64 ? ulo >> s0 : uhi >> (s0 - 64)) & 0xFull);
+    const uint32_t p = (uint32_t)((s0 < 64 ? plo >> s0 : phi >> (s0 - 64)) & 0xFull);
+    const uint32_t q = (uint32_t)((s0 < 64 ? qlo >> s0 : qhi >> (s0 - 64)) & 0xFull);
+    put(w, spread_nibble(u) | (spread_nibble(u & p) << 1) | (spread_nibble(u & q) << 2));
+  }
 }
 
 // t for origin o on the +axis ray of a recorded line hit (reference order).
@@ -359,7 +529,7 @@ __host__ __device__ inline int64_t brick_launch_size(const TileDims &td) {
   return ((td.tx + e - 1) / e) * ((td.ty + e - 1) / e) * ((td.tz + e - 1) / e) *
          (e * e * e);
 }
-__device__ inline bool brick_tile(int64_t m, const TileDims &td, int32_t *tx,
+__host__ __device__ inline bool brick_tile(int64_t m, const TileDims &td, int32_t *tx,
                                   int32_t *ty, int32_t *tz) {
   // 32-bit decode (brick counts and positions stay below 2^31)
   const int32_t e = 1 << kBrickBits;
@@ -380,6 +550,22 @@ __device__ inline bool brick_tile(int64_t m, const TileDims &td, int32_t *tx,
   *ty = by * e + ly;
   *tz = bz * e + lz;
   return *tx < td.tx && *ty < td.ty && *tz < td.tz;
+}
+
+// Launch-order position (brick_tile's m) of the brick with linear index b
+// (x fastest).
+__host__ __device__ inline int64_t brick_order_of(uint32_t b, const TileDims &td) {
+  const int32_t e = 1 << kBrickBits;
+  const uint32_t tx = (uint32_t)td.tx, ty = (uint32_t)td.ty;
+  const uint32_t x = b % tx, y = (b / tx) % ty, z = b / (tx * ty);
+  const uint32_t nbx = (tx + e - 1) / e, nby = (ty + e - 1) / e;
+  const uint32_t blk = ((z >> kBrickBits) * nby + (y >> kBrickBits)) * nbx + (x >> kBrickBits);
+  uint32_t w = 0;
+#pragma unroll
+  for (int k = 0; k < kBrickBits; k++)
+    w |= (((x >> k) & 1u) << (3 * k)) | (((y >> k) & 1u) << (3 * k + 1)) |
+         (((z >> k) & 1u) << (3 * k + 2));
+  return ((int64_t)blk << (3 * kBrickBits)) | (int64_t)w;
 }
 
 // Per-triangle data for the branch-free point-triangle distance, relative
@@ -476,10 +662,6 @@ __device__ inline int64_t chunk_count(int64_t cap, const uint64_t *dev) {
   return n < cap ? n : cap;
 }
 
-// Sign bricks: 8 x 8 lattice columns x kSignZ points per 64-lane wave.
-constexpr int kSignZ = 4;
-// Points per sign-brick segment of an x-, y- and z-line.
-constexpr int kSegLen[3] = {8, 8, kSignZ};
 
 struct SignDims {
   int32_t sx, sy, sz;

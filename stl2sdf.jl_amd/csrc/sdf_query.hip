@@ -93,64 +93,6 @@ __device__ inline bool inline_parity(const LineInline &r, const double *xs,
   return true;
 }
 
-// Both rays of a line for every point whose coordinate lies in [lo, hi],
-// when no crossing is within its rounding margin of that range: each
-// crossing is then right of every point (fl(c - hi) > eps + D) or left of
-// every point (fl(c - lo) < -eps - D).  Rounded subtraction is monotone, so
-// every point's comparisons in inline_parity come out the same as at the
-// range's ends, none of them ambiguous; the parities are shared.  Returns
-// false when some crossing is near the range (or the summary is not
-// usable); the caller then classifies each point.
-__device__ inline bool inline_uniform(const LineInline &r, double lo, double hi, int *pos,
-                                      int *neg) {
-  // selects, not branches: the per-hit cases would otherwise cost scalar
-  // exec-mask work on every call
-  const uint32_t n = r.info & 0xFFu;
-  const bool usable = !(r.info & kLineExact) && n <= (uint32_t)kInlineHits;
-  const int ncl = (int)((r.info >> 8) & 0xFFu);
-  int first_pos = -1, last_neg = -1;
-  bool uni = true;
-#pragma unroll
-  for (int h2 = 0; h2 < kInlineHits; h2++) {
-    const bool valid = h2 < (int)n;
-    const double D = (double)r.delta[h2];
-    const int cid = r.cid[h2];
-    const bool above = r.c[h2] - hi > kRayEps + D;
-    const bool below = !above & (r.c[h2] - lo < -kRayEps - D);
-    first_pos = (valid & above & (first_pos < 0)) ? cid : first_pos;
-    last_neg = (valid & below) ? cid : last_neg;
-    uni = uni & (!valid | above | below);
-  }
-  *pos = first_pos >= 0 ? ((ncl - first_pos) & 1) : 0;
-  *neg = last_neg >= 0 ? ((last_neg + 1) & 1) : 0;
-  return usable & uni;
-}
-
-// inline_uniform for segment s of the line from the crossings' segment
-// ranges (kLineSeg): the same decisions by integer compares.
-__device__ inline bool seg_uniform(const LineTail &r, int s, int *pos, int *neg) {
-  const uint32_t n = r.info & 0xFFu;
-  const bool usable = !(r.info & kLineExact) && n <= (uint32_t)kInlineHits;
-  const int ncl = (int)((r.info >> 8) & 0xFFu);
-  int first_pos = -1, last_neg = -1;
-  bool uni = true;
-#pragma unroll
-  for (int h2 = 0; h2 < kInlineHits; h2++) {
-    const bool valid = h2 < (int)n;
-    const int cid = (int)((r.cid4 >> (8 * h2)) & 0xFFu);
-    const uint32_t sg = ((h2 < 2 ? r.seg01 : r.seg23) >> (16 * (h2 & 1))) & 0xFFFFu;
-    const int a = (int)(sg & 0xFFu), b = (int)(sg >> 8);
-    const bool above = s < a;
-    const bool below = !above & (s >= b);
-    first_pos = (valid & above & (first_pos < 0)) ? cid : first_pos;
-    last_neg = (valid & below) ? cid : last_neg;
-    uni = uni & (!valid | above | below);
-  }
-  *pos = first_pos >= 0 ? ((ncl - first_pos) & 1) : 0;
-  *neg = last_neg >= 0 ? ((last_neg + 1) & 1) : 0;
-  return usable & uni;
-}
-
 // LDS of one distance workgroup: the chunk's triangle records, the
 // (row, triangle) jobs of phase 2 and the per-point minima.
 constexpr int kTileRows = kTileY * kTileZ;
@@ -584,11 +526,11 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S,
   const bool in_grid = i < g.nx && j < g.ny;
   const int nk = min(kSignZ, g.nz - k0);
 
-  // ---- issue the brick's loads: the integer tails of the lane's own z-line
-  // summary and of one x- or y-line summary together with the four distance
-  // tiles' offsets, then the lane's four squared distances.  Vector loads
-  // complete in issue order, so the line tails go first and the uniformity
-  // test below runs while the distances are in flight.
+  // ---- issue the brick's loads: the segment bytes of the lane's own z-line
+  // and of one x- or y-line together with the four distance tiles' offsets,
+  // then the lane's four squared distances.  Vector loads complete in issue
+  // order, so the bytes go first and the uniformity test below runs while
+  // the distances are in flight.
   // Lanes 0-31 stage the x-lines (k0 + sq, 8 by + sw), lanes 32-63 the
   // y-lines (k0 + sq, 8 bx + sw): one code path, operands by select.
   const int sq = (lane >> 3) & 3, sw = lane & 7;
@@ -599,10 +541,21 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S,
   // the staged line (0 when it serves no point) and the lane's own z-line
   const int64_t Lxy = valid ? (xl ? 0 : base1) + (int64_t)(k0 + sq) * across + la : 0;
   const int64_t Lz = in_grid ? base2 + (int64_t)j * g.nx + i : 0;
-  LineTail r = {}, rz = {};
+  // segment bytes (SegLayout): segment bx of the staged x-line or by of the
+  // staged y-line, segment bz of the lane's z-line; a line that serves no
+  // point reads word 0 of its part and takes 1 (uniform, both parities even)
+  uint32_t bxy = 1u, bzl = 1u;
   if (do_sign) {
-    r = load_line_tail(inl + Lxy);
-    rz = load_line_tail(inl + Lz);
+    const int32_t sxy = xl ? bx : by;
+    const int64_t nl = (int64_t)across * g.nz;
+    const int64_t oxy = (xl ? 0 : J.segoff1) +
+                        (valid ? (int64_t)(sxy >> 2) * nl + (int64_t)(k0 + sq) * across + la : 0);
+    const int64_t oz =
+        J.segoff2 +
+        (in_grid ? (int64_t)(bz >> 2) * ((int64_t)g.nx * g.ny) + (int64_t)j * g.nx + i : 0);
+    const uint32_t wxy = J.segw[oxy], wz = J.segw[oz];
+    bxy = valid ? (wxy >> (8 * (sxy & 3))) & 0xFFu : 1u;
+    bzl = in_grid ? (wz >> (8 * (bz & 3))) & 0xFFu : 1u;
   }
   // The lane's squared distances of points (i, j, k0 + q), straight from the
   // brick's four distance tiles (8 x 4 x 2 points; tile t = 2 tz local + ty
@@ -613,6 +566,7 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S,
   float best[kSignZ];
 #pragma unroll
   for (int q = 0; q < kSignZ; q++) best[q] = __builtin_huge_valf();
+  bool work = false;  // wave-uniform: some tile of the brick has distance work
   if (has_d2) {
     uint64_t c0[4], c1[4];
     bool tok[4];
@@ -628,15 +582,20 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S,
     // last work item: the loads are unconditional and need no select
     uint32_t slot[4];
 #pragma unroll
-    for (int t = 0; t < 4; t++)
+    for (int t = 0; t < 4; t++) {
       slot[t] = tok[t] && c1[t] > c0[t] && (int64_t)c0[t] < chunk_cap ? (uint32_t)c0[t]
                                                                        : (uint32_t)chunk_cap;
-    const int tyl = lane >> 5;  // the lane's y tile in the brick
+      work |= slot[t] != (uint32_t)chunk_cap;
+    }
+    // a brick no triangle is near (most of the grid) keeps +inf: distance h
+    if (work) {
+      const int tyl = lane >> 5;  // the lane's y tile in the brick
 #pragma unroll
-    for (int q = 0; q < kSignZ; q++) {
-      const uint32_t sl = tyl ? slot[(q >> 1) * 2 + 1] : slot[(q >> 1) * 2];
-      const int tl = (lane & 7) + kTileX * (((lane >> 3) & 3) + kTileY * (q & 1));
-      best[q] = d2buf[(uint64_t)sl * kTilePts + tl];
+      for (int q = 0; q < kSignZ; q++) {
+        const uint32_t sl = tyl ? slot[(q >> 1) * 2 + 1] : slot[(q >> 1) * 2];
+        const int tl = (lane & 7) + kTileX * (((lane >> 3) & 3) + kTileY * (q & 1));
+        best[q] = d2buf[(uint64_t)sl * kTilePts + tl];
+      }
     }
   }
 
@@ -649,23 +608,12 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S,
   // every line this lane staged or owns is shared by the brick's points
   bool lines_shared = true;
   if (do_sign) {
-    const int32_t b0 = (xl ? bx : by) * 8;  // first brick coordinate along the line
-    const int32_t along = xl ? g.nx : g.ny;
-    int p, n;
-    const bool uni =
-        (r.info & kLineSeg)
-            ? seg_uniform(r, xl ? bx : by, &p, &n)
-            : inline_uniform(inl[Lxy], lattice_coord(xl ? g.ox : g.oy, g.h, b0),
-                             lattice_coord(xl ? g.ox : g.oy, g.h, min(b0 + 7, along - 1)), &p,
-                             &n);
-    const uint8_t u = valid && uni ? (uint8_t)(1 | (p << 1) | (n << 2)) : (uint8_t)0;
-    *(xl ? &S.ux[sq][sw] : &S.uy[sq][sw]) = u;
-    // every lane (one outside the grid tests line 0 and ignores the result),
-    // so no path leaves the tail load unread
-    zuni = (rz.info & kLineSeg) ? seg_uniform(rz, bz, &zp, &zn)
-                                : inline_uniform(inl[Lz], lattice_coord(g.oz, g.h, k0),
-                                                 lattice_coord(g.oz, g.h, k0 + nk - 1), &zp,
-                                                 &zn);
+    // the byte is 1 | pos << 1 | neg << 2 for a uniform segment, else 0
+    const bool uni = (bxy & 1u) != 0u;
+    *(xl ? &S.ux[sq][sw] : &S.uy[sq][sw]) = valid ? (uint8_t)bxy : (uint8_t)0;
+    zuni = (bzl & 1u) != 0u;
+    zp = (int)((bzl >> 1) & 1u);
+    zn = (int)((bzl >> 2) & 1u);
     // a staged line outside the grid (or a layer beyond nz) serves no point
     lines_shared = (!valid | uni) & (!in_grid | zuni);
   }
@@ -677,6 +625,13 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S,
   // wave-uniform shortcut: every line of the brick is shared by its points
   // (no crossing near the brick), so each point only adds six stored parities
   const bool brick_uni = __all(lines_shared);
+  // every line of the brick holds the same uniform byte, with equal parities
+  // for its two rays (1: all even, 7: all odd): every point's six votes agree
+  // (outside, or inside), with no staged parities to add up.  Lane 0 is
+  // always inside the grid.
+  const uint32_t b0 = __builtin_amdgcn_readfirstlane(bzl);
+  const bool same = do_sign && brick_uni && (b0 == 1u || b0 == 7u) &&
+                    __all((!in_grid | (bzl == b0)) & (!valid | (bxy == b0)));
   if (do_sign && !brick_uni) *(xl ? &S.lx[sq][sw] : &S.ly[sq][sw]) = inl[Lxy];
   wave_lds_sync();
 
@@ -717,7 +672,16 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S,
     if ((kOut & kOutSigns) && out.signs) out.signs[idx] = sgn;
     if ((kOut & kOutConf) && out.conf) out.conf[idx] = 1.0;
   };
-  if (in_grid && brick_uni) {
+  if (in_grid && same) {
+    const int64_t plane = (int64_t)g.nx * g.ny;
+    const int64_t idx0 = ((int64_t)k0 * g.ny + j) * g.nx + i;
+    const int inside = b0 == 7u ? 6 : 0;
+#pragma unroll
+    for (int q = 0; q < kSignZ; q++) {
+      if (q >= nk) break;  // uniform per wave
+      finish(idx0 + q * plane, work ? point_dist(q) : g.h, inside, false);
+    }
+  } else if (in_grid && brick_uni) {
     // every line of the brick is shared by its points: the six parities are
     // the stored ones
     const int64_t plane = (int64_t)g.nx * g.ny;
@@ -1390,6 +1354,12 @@ FusedJob make_fused_job(const GridP &g, const Bins &b, const float *d2buf, bool 
   J.chunk_off = b.chunk_off;
   J.d2buf = d2buf;
   J.inl = b.line_inline;
+  J.segw = b.seg_words;
+  {
+    const SegLayout sl = seg_layout(g);
+    J.segoff1 = sl.off[1];
+    J.segoff2 = sl.off[2];
+  }
   J.base1 = b.line_base[1];
   J.base2 = b.line_base[2];
   J.out = out;
