@@ -160,6 +160,19 @@ sdf_status sdf_signed_distance_device(sdf_ctx *ctx, const double aabb_min[3],
 sdf_status sdf_mesh_set(sdf_ctx *ctx, const double *X, int64_t nnp, const int64_t *IEN,
                         int64_t nel);
 
+/* The device part of stl_to_sdf (src/stl_to_sdf.jl:101-126) on the mesh of
+ * the last sdf_mesh_set[_device] call: evalDistancesOnTriMesh,
+ * raycast_sign_detection (default ray options), sdf_dists = dists .* signs
+ * and, when remove_artifacts is nonzero, remove_sdf_artifacts!(sdf_dists,
+ * grid; threshold = 0, min_component_ratio), all on the context stream into
+ * the device buffer d_sdf_out (ngp doubles), which never crosses PCIe.
+ * *nflipped (nullable) receives the artifact count (0 without removal).
+ * Synchronous; two status reads (lattice sizes, flip count). */
+sdf_status sdf_signed_fused(sdf_ctx *ctx, const double aabb_min[3], double h,
+                            const int64_t N[3], int32_t remove_artifacts,
+                            double min_component_ratio, double *d_sdf_out,
+                            int64_t *nflipped);
+
 /* remove_sdf_artifacts!(sdf, grid; threshold, min_component_ratio)
  * -- src/SignedDistances/SdfArtifactRemoval.jl:134-248.
  * sdf_inout[(N+1) product] is modified in place: interior nodes
@@ -217,6 +230,15 @@ sdf_status sdf_rbf_smoothing(sdf_ctx *ctx, const double *dist,
                              int32_t is_interpolation, int32_t smooth,
                              double threshold, double reltol, float *fine_out,
                              int32_t *iters_out, double *rel_residual_out);
+/* Device-buffer form of sdf_rbf_smoothing: d_dist (ngp doubles) and
+ * d_fine_out (prod(N*smooth + 1) floats) are device pointers; the same
+ * arithmetic and results.  Synchronous. */
+sdf_status sdf_rbf_smoothing_device(sdf_ctx *ctx, const double *d_dist,
+                                    const int64_t N[3], double h,
+                                    int32_t is_interpolation, int32_t smooth,
+                                    double threshold, double reltol,
+                                    float *d_fine_out, int32_t *iters_out,
+                                    double *rel_residual_out);
 
 /* import_stl(filename) -> (X, IEN) -- src/DataImport/ImportStlData.jl:10-27
  * with the package's readers (:102-196: a file whose first five bytes are

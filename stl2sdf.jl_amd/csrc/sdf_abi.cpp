@@ -1415,6 +1415,46 @@ sdf_status sdf_analyze_components(sdf_ctx *c, const double *sdf, const int64_t N
   });
 }
 
+namespace {
+
+// RBFs_smoothing on device data: dd (ngp doubles) -> fine (prod(N*smooth+1)
+// floats), both device buffers, on the context stream.  Returns after the
+// last kernel is queued; the caller synchronises.
+void rbf_dev(sdf_ctx *c, const double *dd, const sdf::GridP &g, int32_t is_interp,
+             int32_t smooth, double threshold, double reltol, float *fine, int32_t *iters_out,
+             double *rel_out) {
+  const int64_t n = grid_points(g);
+  if (reltol <= 0.0) reltol = 3.4526698300124393e-4;  // sqrt(eps(Float32))
+  float *b = c->ar.get<float>("rbf_b", n + 1);
+  float *w = c->ar.get<float>("rbf_w", n + 1);
+  c->mark("rbf_prepare");
+  sdf::rbf_prepare(c->stream, c->ar, dd, n, b);
+  int iters = 0;
+  double rel = 0.0;
+  c->mark("rbf_weights");
+  if (is_interp) {
+    iters = sdf::rbf_weights(c->stream, c->ar, g, b, threshold, reltol, 50, 2000, w, &rel);
+  } else {
+    SDF_HIP(hipMemcpyAsync(w, b, (size_t)n * sizeof(float), hipMemcpyDeviceToDevice,
+                           c->stream));
+  }
+  c->mark("rbf_fine");
+  sdf::rbf_fine(c->stream, c->ar, g, smooth, threshold, w, fine);
+  c->mark("end");
+  if (iters_out) *iters_out = iters;
+  if (rel_out) *rel_out = rel;
+}
+
+// Argument checks shared by both RBF entry points; returns the fine count.
+int64_t rbf_check(const sdf::GridP &g, int32_t smooth, double threshold) {
+  if (smooth < 1 || smooth > 8) fail(SDF_ERR_ARG, "smooth must be in 1..8");
+  if (!(threshold > 0.0 && threshold < 1.0)) fail(SDF_ERR_ARG, "threshold must be in (0, 1)");
+  return ((int64_t)(g.nx - 1) * smooth + 1) * ((int64_t)(g.ny - 1) * smooth + 1) *
+         ((int64_t)(g.nz - 1) * smooth + 1);
+}
+
+}  // namespace
+
 sdf_status sdf_rbf_smoothing(sdf_ctx *c, const double *dist, const int64_t N[3],
                              double h, int32_t is_interp, int32_t smooth,
                              double threshold, double reltol, float *fine_out,
@@ -1424,40 +1464,36 @@ sdf_status sdf_rbf_smoothing(sdf_ctx *c, const double *dist, const int64_t N[3],
     std::lock_guard<std::mutex> lk(c->mu);
     sdf::GridP g = lattice_only(N);
     (void)h;  // the kernel is expressed in cells: sigma = h
-    if (smooth < 1 || smooth > 8) fail(SDF_ERR_ARG, "smooth must be in 1..8");
-    if (!(threshold > 0.0 && threshold < 1.0)) fail(SDF_ERR_ARG, "threshold must be in (0, 1)");
+    const int64_t nf = rbf_check(g, smooth, threshold);
     const int64_t n = grid_points(g);
-    const int64_t nf = ((int64_t)(g.nx - 1) * smooth + 1) * ((int64_t)(g.ny - 1) * smooth + 1) *
-                       ((int64_t)(g.nz - 1) * smooth + 1);
     if (n > 0 && (!dist || !fine_out)) fail(SDF_ERR_ARG, "NULL array");
-    if (reltol <= 0.0) reltol = 3.4526698300124393e-4;  // sqrt(eps(Float32))
     begin_call(c);
     double *dd = c->ar.get<double>("rbf_in", n + 1);
-    float *b = c->ar.get<float>("rbf_b", n + 1);
-    float *w = c->ar.get<float>("rbf_w", n + 1);
     float *fine = c->ar.get<float>("rbf_fine", nf + 1);
     if (n > 0)
       SDF_HIP(hipMemcpyAsync(dd, dist, (size_t)n * sizeof(double), hipMemcpyHostToDevice,
                              c->stream));
-    c->mark("rbf_prepare");
-    sdf::rbf_prepare(c->stream, c->ar, dd, n, b);
-    int iters = 0;
-    double rel = 0.0;
-    c->mark("rbf_weights");
-    if (is_interp) {
-      iters = sdf::rbf_weights(c->stream, c->ar, g, b, threshold, reltol, 50, 2000, w, &rel);
-    } else {
-      SDF_HIP(hipMemcpyAsync(w, b, (size_t)n * sizeof(float), hipMemcpyDeviceToDevice,
-                             c->stream));
-    }
-    c->mark("rbf_fine");
-    sdf::rbf_fine(c->stream, c->ar, g, smooth, threshold, w, fine);
-    c->mark("end");
+    rbf_dev(c, dd, g, is_interp, smooth, threshold, reltol, fine, iters_out, rel_out);
     SDF_HIP(hipMemcpyAsync(fine_out, fine, (size_t)nf * sizeof(float), hipMemcpyDeviceToHost,
                            c->stream));
     SDF_HIP(hipStreamSynchronize(c->stream));
-    if (iters_out) *iters_out = iters;
-    if (rel_out) *rel_out = rel;
+  });
+}
+
+sdf_status sdf_rbf_smoothing_device(sdf_ctx *c, const double *d_dist, const int64_t N[3],
+                                    double h, int32_t is_interp, int32_t smooth,
+                                    double threshold, double reltol, float *d_fine_out,
+                                    int32_t *iters_out, double *rel_out) {
+  return guarded([&] {
+    if (!c) fail(SDF_ERR_ARG, "ctx is NULL");
+    std::lock_guard<std::mutex> lk(c->mu);
+    sdf::GridP g = lattice_only(N);
+    (void)h;
+    rbf_check(g, smooth, threshold);
+    if (grid_points(g) > 0 && (!d_dist || !d_fine_out)) fail(SDF_ERR_ARG, "NULL array");
+    begin_call(c);
+    rbf_dev(c, d_dist, g, is_interp, smooth, threshold, reltol, d_fine_out, iters_out, rel_out);
+    SDF_HIP(hipStreamSynchronize(c->stream));
   });
 }
 
@@ -1685,6 +1721,27 @@ sdf_status sdf_signed_distance_device(sdf_ctx *c, const double aabb_min[3],
     sdf::FusedOut out;
     out.sdf = d_sdf_out;
     run_lattice(c, g, true, true, ray_params(6, 30, 0.6, 1), out);
+  });
+}
+
+sdf_status sdf_signed_fused(sdf_ctx *c, const double aabb_min[3], double h,
+                            const int64_t N[3], int32_t remove_artifacts,
+                            double min_component_ratio, double *d_sdf_out,
+                            int64_t *nflipped) {
+  return guarded([&] {
+    if (!c) fail(SDF_ERR_ARG, "ctx is NULL");
+    std::lock_guard<std::mutex> lk(c->mu);
+    sdf::GridP g = make_grid(aabb_min, h, N);
+    if (grid_points(g) > 0 && !d_sdf_out) fail(SDF_ERR_ARG, "d_sdf_out is NULL");
+    if (remove_artifacts && !(min_component_ratio > 0.0 && min_component_ratio < 1.0))
+      fail(SDF_ERR_ARG, "artifact_ratio must be in range (0, 1)");
+    begin_call(c);
+    sdf::FusedOut out;
+    out.sdf = d_sdf_out;
+    run_lattice(c, g, true, true, ray_params(6, 30, 0.6, 1), out);
+    int64_t f = 0;
+    if (remove_artifacts) f = remove_artifacts_dev(c, d_sdf_out, g, 0.0, min_component_ratio);
+    if (nflipped) *nflipped = f;
   });
 }
 

@@ -7,6 +7,9 @@
 #   remove_sdf_artifacts!    src/SignedDistances/SdfArtifactRemoval.jl:134
 #   analyze_sdf_components   src/SignedDistances/SdfArtifactRemoval.jl:259
 #   RBFs_smoothing           src/SdfSmoothing/RBFs4Smoothing.jl:275
+#   import_stl               src/DataImport/ImportStlData.jl:10
+# plus device-memory forms of stl_to_sdf's steps 4-7 (src/stl_to_sdf.jl:101-126)
+# and of RBFs_smoothing, on device pointers (e.g. pointer(::ROCArray)),
 # taking the reference's own TriangularMesh / Grid objects.  Julia arrays are
 # passed by pointer (column-major 3 x n Float64, 1-based Int64 IEN), so no
 # conversion happens on the Julia side.
@@ -111,6 +114,55 @@ function RBFs_smoothing(dist::Vector, my_grid::Grid, Is_interpolation::Bool, smo
     end
     new_grid = Grid(my_grid.AABB_min, my_grid.AABB_max, maximum(my_grid.N) * smooth, 0)
     return fine, new_grid
+end
+
+# import_stl(filename) -> (X, IEN): the package's STL readers and its 1e-10
+# vertex merge (ImportStlData.jl:102-309), in the library's host code.
+function import_stl(filename::String)
+    nnp = Ref{Int64}(0)
+    nel = Ref{Int64}(0)
+    _check(ccall((:sdf_import_stl, LIB), Int32,
+        (Cstring, Ptr{Float64}, Ref{Int64}, Ptr{Int64}, Ref{Int64}),
+        filename, C_NULL, nnp, C_NULL, nel))
+    X = Matrix{Float64}(undef, 3, nnp[])
+    IEN = Matrix{Int64}(undef, 3, nel[])
+    _check(ccall((:sdf_import_stl, LIB), Int32,
+        (Cstring, Ptr{Float64}, Ref{Int64}, Ptr{Int64}, Ref{Int64}),
+        filename, X, nnp, IEN, nel))
+    return X, IEN
+end
+
+# stl_to_sdf.jl:101-126 into device memory: distances, signs, their product
+# and (remove_artifacts) remove_sdf_artifacts! on the device; d_sdf points to
+# grid.ngp Float64 in device memory.  Returns the number of flipped nodes.
+function signed_fused!(d_sdf::Ptr{Float64}, mesh::TriangularMesh, grid::Grid;
+                       remove_artifacts::Bool = false, artifact_ratio::Float64 = 0.01)
+    nflipped = Ref{Int64}(0)
+    GC.@preserve mesh grid begin
+        _check(ccall((:sdf_mesh_set, LIB), Int32,
+            (Ptr{Cvoid}, Ptr{Float64}, Int64, Ptr{Int64}, Int64),
+            ctx(), mesh.X, mesh.nnp, mesh.IEN, mesh.nel))
+        _check(ccall((:sdf_signed_fused, LIB), Int32,
+            (Ptr{Cvoid}, Ptr{Float64}, Float64, Ptr{Int64}, Int32, Float64, Ptr{Float64},
+             Ref{Int64}),
+            ctx(), grid.AABB_min, grid.cell_size, grid.N, Int32(remove_artifacts),
+            artifact_ratio, d_sdf, nflipped))
+    end
+    return nflipped[]
+end
+
+# RBFs_smoothing on device memory: d_dist holds my_grid.ngp Float64, d_fine
+# receives prod(N .* smooth .+ 1) Float32.  Returns the fine grid.
+function RBFs_smoothing!(d_fine::Ptr{Float32}, d_dist::Ptr{Float64}, my_grid::Grid,
+                         Is_interpolation::Bool, smooth::Int, threshold::Float64 = 1e-3)
+    GC.@preserve my_grid begin
+        _check(ccall((:sdf_rbf_smoothing_device, LIB), Int32,
+            (Ptr{Cvoid}, Ptr{Float64}, Ptr{Int64}, Float64, Int32, Int32, Float64, Float64,
+             Ptr{Float32}, Ptr{Int32}, Ptr{Float64}),
+            ctx(), d_dist, my_grid.N, my_grid.cell_size, Int32(Is_interpolation), smooth,
+            threshold, 0.0, d_fine, C_NULL, C_NULL))
+    end
+    return Grid(my_grid.AABB_min, my_grid.AABB_max, maximum(my_grid.N) * smooth, 0)
 end
 
 # Learned search radius (no counterpart in the reference): fit once per mesh

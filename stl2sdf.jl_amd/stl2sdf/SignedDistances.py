@@ -95,6 +95,26 @@ def signed_distance(mesh, grid, device=0):
     return out
 
 
+def signed_fused_device(mesh, grid, remove_artifacts=False, min_component_ratio=0.01,
+                        device=0):
+    """The device part of stl_to_sdf (stl_to_sdf.jl:101-126): distances,
+    signs (default ray options), their product and, with remove_artifacts,
+    remove_sdf_artifacts!(sdf, grid; min_component_ratio) in one library
+    call.  Returns (buffer, nflipped): the field stays in the device buffer
+    (``_lib.DeviceBuffer``, grid.ngp doubles); only the mesh crosses PCIe."""
+    X, IEN = _mesh_args(mesh)
+    amin, h, N = _grid_args(grid)
+    L = _lib.lib()
+    c = _lib.ctx(device)
+    _lib.check(L.sdf_mesh_set(c, _lib.dptr(X), X.shape[1], _lib.i64ptr(IEN), IEN.shape[1]))
+    buf = _lib.DeviceBuffer(8 * int(grid.ngp), device)
+    n = C.c_int64(0)
+    _lib.check(L.sdf_signed_fused(c, _lib.dptr(amin), h, _lib.i64ptr(N),
+                                  1 if remove_artifacts else 0, float(min_component_ratio),
+                                  buf.ptr, C.byref(n)))
+    return buf, int(n.value)
+
+
 def _check_len(sdf_values, grid):
     v = np.asarray(sdf_values)
     if v.ndim != 1 or v.shape[0] != grid.ngp:

@@ -78,6 +78,11 @@ SIGNATURES = {
                                            C.c_int64, _i64p]),
     "sdf_rbf_smoothing": (C.c_int32, [_vp, _dp, _i64p, C.c_double, C.c_int32, C.c_int32,
                                       C.c_double, C.c_double, _fp, _i32p, _dp]),
+    "sdf_rbf_smoothing_device": (C.c_int32, [_vp, _vp, _i64p, C.c_double, C.c_int32,
+                                             C.c_int32, C.c_double, C.c_double, _vp, _i32p,
+                                             _dp]),
+    "sdf_signed_fused": (C.c_int32, [_vp, _dp, C.c_double, _i64p, C.c_int32, C.c_double, _vp,
+                                     _i64p]),
 }
 
 
@@ -124,6 +129,70 @@ def ctx(device=0):
             _ctx[device] = h
         c = h
     return c
+
+
+_hip = None
+
+
+def hip():
+    """The HIP runtime the library runs on (loaded with it), for device
+    buffers handed to the ``_device`` entry points."""
+    global _hip
+    lib()
+    with _lock:
+        if _hip is None:
+            H = C.CDLL("libamdhip64.so.7")
+            H.hipSetDevice.argtypes = [C.c_int]
+            H.hipMalloc.argtypes = [C.POINTER(_vp), C.c_size_t]
+            H.hipFree.argtypes = [_vp]
+            H.hipMemcpy.argtypes = [_vp, _vp, C.c_size_t, C.c_int]
+            for f in ("hipSetDevice", "hipMalloc", "hipFree", "hipMemcpy"):
+                getattr(H, f).restype = C.c_int
+            _hip = H
+    return _hip
+
+
+class DeviceBuffer:
+    """A device allocation of ``nbytes`` on ``device`` (hipMalloc), freed
+    with the object.  ``ptr`` is the device address for the ``_device``
+    entry points; to_host / from_host copy synchronously."""
+
+    def __init__(self, nbytes, device=0):
+        H = hip()
+        self.device = device
+        self.nbytes = int(nbytes)
+        self.ptr = _vp()
+        if H.hipSetDevice(device) != 0:
+            raise SdfError(-1, f"hipSetDevice({device}) failed")
+        if H.hipMalloc(C.byref(self.ptr), max(self.nbytes, 1)) != 0:
+            raise SdfError(-1, f"hipMalloc of {self.nbytes} bytes failed")
+
+    def from_host(self, a):
+        a = np.ascontiguousarray(a)
+        if a.nbytes > self.nbytes:
+            raise ValueError("array larger than the buffer")
+        if a.nbytes and hip().hipMemcpy(self.ptr, a.ctypes.data, a.nbytes, 1) != 0:
+            raise SdfError(-1, "hipMemcpy host to device failed")
+        return self
+
+    def to_host(self, dtype, count):
+        out = np.empty(int(count), dtype=dtype)
+        if out.nbytes > self.nbytes:
+            raise ValueError("read beyond the buffer")
+        if out.nbytes and hip().hipMemcpy(out.ctypes.data, self.ptr, out.nbytes, 2) != 0:
+            raise SdfError(-1, "hipMemcpy device to host failed")
+        return out
+
+    def free(self):
+        if self.ptr is not None and self.ptr.value:
+            hip().hipFree(self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 def dptr(a):

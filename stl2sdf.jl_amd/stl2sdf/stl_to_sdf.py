@@ -1,15 +1,18 @@
 """stl_to_sdf and Options, mirroring src/stl_to_sdf.jl:21-147.
 
 The orchestration stays on the host as in the reference; distances, signs,
-the combine step, artifact removal and RBF smoothing run in
-libsdf_mi355x.so."""
+the combine step and artifact removal run in one libsdf_mi355x.so call and
+the RBF smoothing reads that call's device buffer, so the coarse field
+crosses PCIe once (for its export) and the fine one once (for its own)."""
 import os
+
+import numpy as np
 
 from . import SignedDistances
 from .DataExport import exportSdfToVTI
 from .DataImport import import_stl
 from .MeshGrid import TriangularMesh, noninteractive_sdf_grid_setup
-from .SdfSmoothing import RBFs_smoothing
+from .SdfSmoothing import RBFs_smoothing_device, check_fine_grid
 
 
 class Options:
@@ -47,14 +50,18 @@ def stl_to_sdf(stl_filename, options=None, out_dir=".", device=0):
     X, IEN = import_stl(stl_filename)
     mesh = TriangularMesh(X, IEN)
     grid = noninteractive_sdf_grid_setup(mesh, o.cell_size)
-    sdf = SignedDistances.signed_distance(mesh, grid, device=device)
-    if o.remove_artifacts:
-        SignedDistances.remove_sdf_artifacts_(sdf, grid, min_component_ratio=o.artifact_ratio,
-                                              device=device)
+    # distances -> signs -> product -> artifact removal in one device call;
+    # the field crosses PCIe once, for the export
+    d_sdf, _ = SignedDistances.signed_fused_device(
+        mesh, grid, o.remove_artifacts, o.artifact_ratio, device=device)
+    sdf = d_sdf.to_host(np.float64, grid.ngp)
     exportSdfToVTI(os.path.join(out_dir, f"{base}_sdf.vti"), grid, sdf, "distance")
     if o.smoothing_method is not None:
-        fine, fine_grid = RBFs_smoothing(sdf, grid, o.smoothing_method == "interpolation",
-                                         o.grid_refinement, device=device)
-        exportSdfToVTI(os.path.join(out_dir, f"{base}_fine_sdf.vti"), fine_grid, fine,
-                       "distance")
+        # the reference fails in the fine export when the rebuilt fine grid
+        # does not hold prod(N * s + 1) points: fail before the solve
+        check_fine_grid(grid, o.grid_refinement)
+        d_fine, fgrid = RBFs_smoothing_device(d_sdf, grid, o.smoothing_method == "interpolation",
+                                              o.grid_refinement, device=device)
+        fine = d_fine.to_host(np.float32, int(np.prod(np.asarray(grid.N) * o.grid_refinement + 1)))
+        exportSdfToVTI(os.path.join(out_dir, f"{base}_fine_sdf.vti"), fgrid, fine, "distance")
     return ()

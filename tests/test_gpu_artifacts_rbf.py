@@ -155,3 +155,54 @@ def test_stl_to_sdf_reference_test_options(tmp_path):
     fv, fdims, _, _, _ = read_vti(str(tmp_path / "artifacts_fine_sdf.vti"))
     assert fdims == dims and fv.dtype == np.float32
     assert np.max(np.abs(fv - v)) <= 2e-3 * np.abs(v).max()  # interpolation at s = 1
+
+
+def _shell_mesh(n=60, n_max=64):
+    Xb, Ib, _ = synthetic.star_surface(n, n, 0)
+    Xs, Is = synthetic.shells(40, 0.03, 0.08, Xb.min(1), Xb.max(1), seed=1)
+    X, IEN = synthetic.merge((synthetic.to_float32(Xb), Ib), (synthetic.to_float32(Xs), Is))
+    mesh = MeshGrid.TriangularMesh(X, IEN)
+    xmin, xmax = MeshGrid.getMesh_AABB(mesh.X)
+    return mesh, MeshGrid.Grid(xmin, xmax, n_max, 3)
+
+
+@pytest.mark.parametrize("remove", [False, True])
+def test_signed_fused_equals_the_separate_steps(remove):
+    """sdf_signed_fused (distance, sign, product and artifact removal on the
+    device) gives the separate entry points' field bit for bit."""
+    mesh, grid = _shell_mesh()
+    buf, n = SignedDistances.signed_fused_device(mesh, grid, remove, 0.01)
+    fused = buf.to_host(np.float64, grid.ngp)
+    ref = SignedDistances.signed_distance(mesh, grid)
+    nref = SignedDistances.remove_sdf_artifacts_(ref, grid, min_component_ratio=0.01) \
+        if remove else 0
+    print(f"flipped {n} (separate steps {nref})", flush=True)
+    assert n == nref and (n > 0) == remove
+    assert np.array_equal(fused.view(np.uint64), ref.view(np.uint64))
+
+
+@pytest.mark.parametrize("interp,s", [(True, 1), (True, 2), (False, 2)])
+def test_rbf_device_equals_host_entry_point(interp, s):
+    N = np.array([24, 20, 18])
+    grid = _grid_for(N)
+    vals = _random_field(tuple((N + 1)[::-1]), 7)
+    info_h, info_d = {}, {}
+    fine, fg = SdfSmoothing.RBFs_smoothing(vals, grid, interp, s, info=info_h)
+    from stl2sdf import _lib
+    d_in = _lib.DeviceBuffer(vals.nbytes).from_host(vals)
+    d_fine, fg2 = SdfSmoothing.RBFs_smoothing_device(d_in, grid, interp, s, info=info_d)
+    fine_d = d_fine.to_host(np.float32, fine.size)
+    assert np.array_equal(fine.view(np.uint32), fine_d.view(np.uint32))
+    assert info_h == info_d and np.array_equal(fg.N, fg2.N)
+
+
+def test_stl_to_sdf_fine_grid_mismatch_fails_before_the_solve(tmp_path):
+    # cell size 0.25: N = [245, 85, 21], the rebuilt fine grid at s = 2 is
+    # [490, 171, 42]; the coarse export is written, then the fine one's
+    # length error is raised (ExportToVTI.jl:43-47) without the solve
+    stl = os.path.join(_cases.GOLDEN, "artifacts.stl")
+    opts = Options(smoothing_method="interpolation", grid_refinement=2, cell_size=0.25)
+    with pytest.raises(ValueError, match="doesn't match grid dimensions"):
+        stl_to_sdf(stl, options=opts, out_dir=str(tmp_path))
+    assert os.path.exists(tmp_path / "artifacts_sdf.vti")
+    assert not os.path.exists(tmp_path / "artifacts_fine_sdf.vti")

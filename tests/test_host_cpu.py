@@ -56,6 +56,31 @@ def test_fine_grid_of_rbf_smoothing():
     assert all(abs(int(a) - 2 * int(b)) <= 1 for a, b in zip(fg.N, g.N))
 
 
+def test_fine_grid_guard_matches_the_export_error(tmp_path):
+    """RBFs_smoothing returns Grid(AABB_min, AABB_max, maximum(N) * s, 0)
+    (RBFs4Smoothing.jl:316), rebuilt with ceil rounding, beside
+    prod(N * s + 1) values; where the two disagree the reference's fine
+    export fails (ExportToVTI.jl:43-47).  The guard raises that same error
+    before the solve, and only there."""
+    from stl2sdf.SdfSmoothing import check_fine_grid, fine_grid
+    mesh, g = _cases.artifacts_case(0.25)
+    assert g.N.tolist() == [245, 85, 21]
+    for s, fine_n in ((1, [245, 86, 21]), (2, [490, 171, 42])):
+        fg = fine_grid(g, s)
+        assert fg.N.tolist() == fine_n
+        values = np.zeros(int(np.prod(g.N * s + 1)), np.float32)
+        with pytest.raises(ValueError) as export_err:
+            exportSdfToVTI(str(tmp_path / "f"), fg, values, "distance")
+        with pytest.raises(ValueError) as guard_err:
+            check_fine_grid(g, s)
+        assert str(guard_err.value) == str(export_err.value)
+    # the reference's own test grid (cell size 0.5) exports at s = 1 and 2
+    _, g5 = _cases.artifacts_case(0.5)
+    for s in (1, 2):
+        check_fine_grid(g5, s)
+        assert np.array_equal(fine_grid(g5, s).N, g5.N * s)
+
+
 def test_select_projected_nodes():
     """SelectProjectedNodes (DataPostProcess.jl:21-58): keeps the points with
     sum |xp| > 1e-10 in order, mean and max of |X - Xp|, 0.0 / 0.0 when none."""
