@@ -1149,6 +1149,8 @@ void line_hits_finish(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
   hipLaunchKernelGGL(seg_bytes_kernel, dim3(capped_grid(lb.nlines, 256, lb.max_blocks)),
                      dim3(256), 0, s, g, (const LineInline *)inl, lb.nlx, lb.nlx + lb.nly,
                      lb.nlines, sl, segw);
+  // the sentinel read by lanes whose line serves no point (byte 1 everywhere)
+  launch_fill_u32(s, segw + sl.total, 1, 0x01010101u);
   SDF_HIP(hipGetLastError());
   b.line_off = lb.loff;
   b.seg_words = segw;

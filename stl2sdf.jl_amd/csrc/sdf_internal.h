@@ -138,7 +138,8 @@ struct Bins {
   LineFast *line_fast = nullptr;   // per hit, sorted by c within each line
   uint32_t *line_meta = nullptr;   // per line: cluster count | kLineExact
   LineInline *line_inline = nullptr;  // per line: inline summary (<= 4 hits)
-  uint32_t *seg_words = nullptr;      // sign-brick segment bytes, 4 per word (SegLayout)
+  uint32_t *seg_words = nullptr;      // sign-brick segment bytes, 4 per word (SegLayout),
+                                      // then the sentinel word 0x01010101
   int64_t ntile_pairs = 0, nline_hits = 0;
   // distance work items: each non-empty tile's list in chunks of 64
   uint64_t *chunk_off = nullptr;  // ntiles + 1 (first work item of a tile)
@@ -203,8 +204,9 @@ struct FusedJob {
   const uint64_t *chunk_off;
   const float *d2buf;
   const LineInline *inl;
-  const uint32_t *segw;      // segment bytes, 4 per word (SegLayout)
+  const uint32_t *segw;      // segment bytes, 4 per word (SegLayout), then a sentinel
   int64_t segoff1, segoff2;  // first words of the y- and z-line parts
+  int64_t segtotal;          // the sentinel word (0x01010101)
   int64_t base1, base2;
   FusedOut out;
   int64_t chunk_cap;

@@ -541,35 +541,14 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S,
   // the staged line (0 when it serves no point) and the lane's own z-line
   const int64_t Lxy = valid ? (xl ? 0 : base1) + (int64_t)(k0 + sq) * across + la : 0;
   const int64_t Lz = in_grid ? base2 + (int64_t)j * g.nx + i : 0;
-  // segment bytes (SegLayout): segment bx of the staged x-line or by of the
-  // staged y-line, segment bz of the lane's z-line; a line that serves no
-  // point reads word 0 of its part and takes 1 (uniform, both parities even)
-  uint32_t bxy = 1u, bzl = 1u;
-  if (do_sign) {
-    const int32_t sxy = xl ? bx : by;
-    const int64_t nl = (int64_t)across * g.nz;
-    const int64_t oxy = (xl ? 0 : J.segoff1) +
-                        (valid ? (int64_t)(sxy >> 2) * nl + (int64_t)(k0 + sq) * across + la : 0);
-    const int64_t oz =
-        J.segoff2 +
-        (in_grid ? (int64_t)(bz >> 2) * ((int64_t)g.nx * g.ny) + (int64_t)j * g.nx + i : 0);
-    const uint32_t wxy = J.segw[oxy], wz = J.segw[oz];
-    bxy = valid ? (wxy >> (8 * (sxy & 3))) & 0xFFu : 1u;
-    bzl = in_grid ? (wz >> (8 * (bz & 3))) & 0xFFu : 1u;
-  }
-  // The lane's squared distances of points (i, j, k0 + q), straight from the
-  // brick's four distance tiles (8 x 4 x 2 points; tile t = 2 tz local + ty
-  // local).  A tile's slot (its first work item) is wave-uniform; for each q
-  // the wave reads two runs of 32 consecutive floats.
+  // The brick's four distance tiles (8 x 4 x 2 points; tile t = 2 tz local
+  // + ty local) and their offsets: wave-uniform scalar loads, issued first so
+  // that they are in flight with the segment bytes' vector loads.
   static_assert(kTileX == 8 && 2 * kTileY == 8 && 2 * kTileZ == kSignZ,
                 "a sign brick covers 1 x 2 x 2 distance tiles");
-  float best[kSignZ];
-#pragma unroll
-  for (int q = 0; q < kSignZ; q++) best[q] = __builtin_huge_valf();
-  bool work = false;  // wave-uniform: some tile of the brick has distance work
+  uint64_t c0[4] = {0, 0, 0, 0}, c1[4] = {0, 0, 0, 0};
+  bool tok[4] = {false, false, false, false};
   if (has_d2) {
-    uint64_t c0[4], c1[4];
-    bool tok[4];
 #pragma unroll
     for (int t = 0; t < 4; t++) {  // no branches: the eight loads go out together
       const int32_t tyy = 2 * by + (t & 1), tzz = 2 * bz + (t >> 1);
@@ -578,6 +557,35 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S,
       c0[t] = chunk_off_k[tile];
       c1[t] = chunk_off_k[tile + 1];
     }
+  }
+  // segment bytes (SegLayout): segment bx of the staged x-line or by of the
+  // staged y-line, segment bz of the lane's z-line.  A line that serves no
+  // point reads the sentinel word after the last one (byte 1 at every
+  // position: uniform, both parities even), so the loads and the byte
+  // extraction are unconditional and nothing waits on them before the
+  // distance loads are issued.
+  uint32_t bxy = 1u, bzl = 1u;
+  if (do_sign) {
+    const int32_t sxy = xl ? bx : by;
+    const int64_t nl = (int64_t)across * g.nz;
+    const int64_t oxy = valid ? (xl ? 0 : J.segoff1) + (int64_t)(sxy >> 2) * nl +
+                                    (int64_t)(k0 + sq) * across + la
+                              : J.segtotal;
+    const int64_t oz = in_grid ? J.segoff2 + (int64_t)(bz >> 2) * ((int64_t)g.nx * g.ny) +
+                                     (int64_t)j * g.nx + i
+                               : J.segtotal;
+    const uint32_t wxy = J.segw[oxy], wz = J.segw[oz];
+    bxy = (wxy >> (8 * (sxy & 3))) & 0xFFu;
+    bzl = (wz >> (8 * (bz & 3))) & 0xFFu;
+  }
+  // The lane's squared distances of points (i, j, k0 + q), straight from the
+  // four tiles: a tile's slot (its first work item) is wave-uniform; for each
+  // q the wave reads two runs of 32 consecutive floats.
+  float best[kSignZ];
+#pragma unroll
+  for (int q = 0; q < kSignZ; q++) best[q] = __builtin_huge_valf();
+  bool work = false;  // wave-uniform: some tile of the brick has distance work
+  if (has_d2) {
     // a tile without distance work reads the spare tile of +inf after the
     // last work item: the loads are unconditional and need no select
     uint32_t slot[4];
@@ -610,7 +618,8 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S,
   if (do_sign) {
     // the byte is 1 | pos << 1 | neg << 2 for a uniform segment, else 0
     const bool uni = (bxy & 1u) != 0u;
-    *(xl ? &S.ux[sq][sw] : &S.uy[sq][sw]) = valid ? (uint8_t)bxy : (uint8_t)0;
+    // (a line that serves no point holds the sentinel's 1, read by no point)
+    *(xl ? &S.ux[sq][sw] : &S.uy[sq][sw]) = (uint8_t)bxy;
     zuni = (bzl & 1u) != 0u;
     zp = (int)((bzl >> 1) & 1u);
     zn = (int)((bzl >> 2) & 1u);
@@ -1359,6 +1368,7 @@ FusedJob make_fused_job(const GridP &g, const Bins &b, const float *d2buf, bool 
     const SegLayout sl = seg_layout(g);
     J.segoff1 = sl.off[1];
     J.segoff2 = sl.off[2];
+    J.segtotal = sl.total;
   }
   J.base1 = b.line_base[1];
   J.base2 = b.line_base[2];
