@@ -6,7 +6,10 @@ batch in the library: per-mesh binning and LBVH chains run concurrently,
 the distance and sign stages of all meshes are one persistent work-queue
 launch each, and the whole pass is captured into a hipGraph that ``run()``
 replays.  Each result equals ``SignedDistances.signed_distance(mesh, grid)``.
-Device memory comes from torch (plumbing only).
+Device buffers come from the library's own HIP runtime
+(``_lib.DeviceBuffer``), not from torch: torch ships its own HIP and HSA
+runtimes, and a process that loads this library before importing torch ends
+up with both copies, whose allocations the other copy does not know.
 """
 import ctypes as C
 
@@ -20,20 +23,19 @@ INFO = ("meshes", "grid_points", "last_ms", "chunk_len", "ray_stack", "dist_bloc
 
 class SignedDistanceBatch:
     def __init__(self, meshes, grids, device=0, autotune=True):
-        import torch
         if len(meshes) != len(grids) or not meshes:
             raise ValueError("need one grid per mesh")
         self.device = device
-        dev = torch.device(f"cuda:{device}")
-        self._X, self._I, self.out = [], [], []
+        self._X, self._I, self.out, self._ngp = [], [], [], []
         for mesh, grid in zip(meshes, grids):
             X = np.ascontiguousarray(np.asarray(mesh.X, np.float64).T)   # nnp x 3 = 3 x nnp col-major
             I = np.ascontiguousarray(np.asarray(mesh.IEN, np.int64).T)
-            self._X.append(torch.from_numpy(X).to(dev))
-            self._I.append(torch.from_numpy(I).to(dev))
-            self.out.append(torch.empty(int(grid.ngp), dtype=torch.float64, device=dev))
+            self._X.append(_lib.DeviceBuffer(X.nbytes, device).from_host(X))
+            self._I.append(_lib.DeviceBuffer(I.nbytes, device).from_host(I))
+            self.out.append(_lib.DeviceBuffer(8 * int(grid.ngp), device))
+            self._ngp.append(int(grid.ngp))
         n = len(meshes)
-        ptr = lambda ts: (C.c_void_p * n)(*[t.data_ptr() for t in ts])
+        ptr = lambda bs: (C.c_void_p * n)(*[b.ptr.value for b in bs])
         nnp = np.array([m.X.shape[1] for m in meshes], np.int64)
         nel = np.array([m.IEN.shape[1] for m in meshes], np.int64)
         amin = np.concatenate([np.asarray(g.AABB_min, np.float64) for g in grids])
@@ -41,12 +43,16 @@ class SignedDistanceBatch:
         N = np.concatenate([np.asarray(g.N, np.int64) for g in grids])
         L = _lib.lib()
         c = _lib.ctx(device)
-        torch.cuda.synchronize(dev)
         self._h = C.c_void_p()
         _lib.check(L.sdf_batch_create(c, n, ptr(self._X), _lib.i64ptr(nnp), ptr(self._I),
                                       _lib.i64ptr(nel), _lib.dptr(amin), _lib.dptr(h),
                                       _lib.i64ptr(N), ptr(self.out), 1 if autotune else 0,
                                       C.byref(self._h)))
+
+    def set_vertices(self, k, X):
+        """Overwrite mesh k's vertices in place (same count), for replays."""
+        X = np.ascontiguousarray(np.asarray(X, np.float64).T)
+        self._X[k].from_host(X)
 
     def run(self):
         _lib.check(_lib.lib().sdf_batch_run(self._h))
@@ -62,7 +68,7 @@ class SignedDistanceBatch:
         return d
 
     def results(self):
-        return [t.cpu().numpy() for t in self.out]
+        return [b.to_host(np.float64, n) for b, n in zip(self.out, self._ngp)]
 
     def close(self):
         if self._h:
