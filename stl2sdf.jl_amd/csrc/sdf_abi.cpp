@@ -136,6 +136,7 @@ struct sdf_ctx {
   hipStream_t aux[2] = {nullptr, nullptr};
   // [0] pass start, [1] line hits done, [2] LBVH done, [3] tile chain done
   hipEvent_t join[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev_plan = nullptr;  // the sign kernel's brick plan is written
   std::mutex mu;
   sdf::Arena ar;
   void *scratch = nullptr;  // this device's static scratch array
@@ -396,6 +397,7 @@ struct LatticeRun {
   bool bvh_was_ready = false;
   uint32_t cap = 0, cap0 = 0;
   sdf::Pending *pend = nullptr, *pend0 = nullptr;
+  sdf::BrickPlan *plan = nullptr;  // the sign kernel's per-brick plan
 };
 
 enum : int {
@@ -443,6 +445,17 @@ void stage_bins_tiles(sdf_ctx *c, LatticeRun &L) {
   // beside the latency-bound tile chain they stretch every small kernel
   // (measured: about 35 us per step)
   SDF_HIP(hipEventRecord(c->join[3], c->stream));
+  // the sign kernel's brick plan (coordinates and distance slots, a few
+  // microseconds): with signs on helper stream 1 ahead of the LBVH, so the
+  // distance kernel starts at once; the sign stage waits for ev_plan
+  L.plan = c->ar.get<sdf::BrickPlan>("sign_plan", sdf::brick_plan_size(g) + 1);
+  if (L.do_sign) {
+    SDF_HIP(hipStreamWaitEvent(c->aux[1], c->join[3], 0));
+    sdf::launch_brick_plan(c->aux[1], g, bins, L.d2buf, L.plan);
+    SDF_HIP(hipEventRecord(c->ev_plan, c->aux[1]));
+  } else {
+    sdf::launch_brick_plan(c->stream, g, bins, L.d2buf, L.plan);
+  }
   // issued after the tile chain (so that chain's launches are not delayed)
   // but free to start beside it
   if (L.do_sign && c->lbvh_early) bvh_async(c, 1, L.g, L.learned, 0);
@@ -521,11 +534,14 @@ void stage_dist(sdf_ctx *c, LatticeRun &L) {
 // Stage 3: signs and combine (waits for the axis-line hits).
 void stage_sign(sdf_ctx *c, LatticeRun &L) {
   c->mark("bin_lines_wait");
-  if (L.do_sign) SDF_HIP(hipStreamWaitEvent(c->stream, c->join[1], 0));
+  if (L.do_sign) {
+    SDF_HIP(hipStreamWaitEvent(c->stream, c->join[1], 0));
+    SDF_HIP(hipStreamWaitEvent(c->stream, c->ev_plan, 0));
+  }
   c->mark("fused");
   sdf::launch_fused_lattice(c->stream, L.g, c->tri, L.bins, L.d2buf, L.do_sign, L.out, L.pend,
                             c->d_words + 1, L.cap, L.pend0, c->d_words + 2, L.cap0, c->d_words,
-                            (int64_t)c->fused_per_cu * c->ncu);
+                            L.plan, (int64_t)c->fused_per_cu * c->ncu);
 }
 
 // Stage 4: adaptive vote for the pending points (waits for the LBVH), then
@@ -846,6 +862,7 @@ sdf_ctx *batch_child(sdf_batch *b, int i) {
   k->aux[0] = b->aux[i % 4][0];
   k->aux[1] = b->aux[i % 4][1];
   for (int q = 0; q < 4; q++) SDF_HIP(hipEventCreateWithFlags(&k->join[q], hipEventDisableTiming));
+  SDF_HIP(hipEventCreateWithFlags(&k->ev_plan, hipEventDisableTiming));
   SDF_HIP(hipMalloc(&k->d_words, 256));
   k->d_status = b->d_status + 16 * i;
   k->own_status = false;
@@ -864,6 +881,7 @@ void batch_free(sdf_batch *b) {
     k->ar.release();
     for (int q = 0; q < 4; q++)
       if (k->join[q]) (void)hipEventDestroy(k->join[q]);
+    if (k->ev_plan) (void)hipEventDestroy(k->ev_plan);
     if (k->d_words) (void)hipFree(k->d_words);
     delete k;
   }
@@ -1093,6 +1111,7 @@ sdf_status sdf_ctx_create(int32_t device, sdf_ctx **out) {
         SDF_HIP(hipStreamCreateWithFlags(&c->aux[q], hipStreamNonBlocking));
       for (int q = 0; q < 4; q++)
         SDF_HIP(hipEventCreateWithFlags(&c->join[q], hipEventDisableTiming));
+      SDF_HIP(hipEventCreateWithFlags(&c->ev_plan, hipEventDisableTiming));
       SDF_HIP(hipMalloc(&c->d_words, 256));
       SDF_HIP(hipMalloc(&c->d_status, 16 * sizeof(uint64_t)));
       SDF_HIP(hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device));
@@ -1119,6 +1138,7 @@ sdf_status sdf_ctx_destroy(sdf_ctx *c) {
       }
     for (int q = 0; q < 4; q++)
       if (c->join[q]) (void)hipEventDestroy(c->join[q]);
+    if (c->ev_plan) (void)hipEventDestroy(c->ev_plan);
     if (c->d_words) (void)hipFree(c->d_words);
     if (c->d_status && c->own_status) (void)hipFree(c->d_status);
     if (c->own) (void)hipStreamDestroy(c->own);

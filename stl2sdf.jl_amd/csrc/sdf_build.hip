@@ -640,6 +640,47 @@ __global__ void chunk_items_kernel(const uint64_t *__restrict__ chunk_off,
   }
 }
 
+// Segment bytes (SegLayout) of line l of axis ax from its finished summary
+// li: per segment, the uniformity test of the sign kernel on the summary
+// (seg_uniform on the crossings' segment ranges, or inline_uniform on the
+// segment's coordinate range when the line has none), so the sign kernel
+// reads one byte per line and brick.  With segment ranges and at most 128
+// segments the test runs on all segments at once (seg_words_masks).
+__device__ inline void write_seg_words(const GridP &g, const LineInline &li, int ax, int64_t l,
+                                       const SegLayout &sl, uint32_t *__restrict__ segw) {
+  LineTail r;
+  memcpy(&r, reinterpret_cast<const char *>(&li) + 48, sizeof(r));
+  uint32_t *out = segw + sl.off[ax] + l;
+  const int64_t stride = sl.nl[ax];
+  const int ns = sl.ns[ax];
+  if ((r.info & kLineSeg) && ns <= 128) {
+    seg_words_masks(r, ns, [&](int w, uint32_t word) { out[(int64_t)w * stride] = word; });
+    return;
+  }
+  const double oa = ax == 0 ? g.ox : (ax == 1 ? g.oy : g.oz);
+  const int32_t na = ax == 0 ? g.nx : (ax == 1 ? g.ny : g.nz);
+  const int len = kSegLen[ax];
+  const bool seg = (r.info & kLineSeg) != 0u;
+  uint32_t word = 0u;
+  for (int sgi = 0; sgi < ns; sgi++) {
+    int p, q;
+    bool u;
+    if (seg) {
+      u = seg_uniform(r, sgi, &p, &q);
+    } else {
+      const int64_t e = (int64_t)sgi * len + len - 1;
+      const double lo = lattice_coord(oa, g.h, (int64_t)sgi * len);
+      const double hi = lattice_coord(oa, g.h, e < na - 1 ? e : na - 1);
+      u = inline_uniform(li, lo, hi, &p, &q);
+    }
+    word |= (u ? (uint32_t)(1 | (p << 1) | (q << 2)) : 0u) << (8 * (sgi & 3));
+    if ((sgi & 3) == 3 || sgi == ns - 1) {
+      out[(int64_t)(sgi >> 2) * stride] = word;
+      word = 0u;
+    }
+  }
+}
+
 // Fast-path summaries of each axis line (one thread per line).
 //
 // Error model (FP64 without contraction, unit roundoff u = 2^-53): for a hit
@@ -662,7 +703,8 @@ __global__ void line_prep_kernel(GridP g, const uint64_t *__restrict__ line_off,
                                  int64_t base1, int64_t base2, int64_t nlines,
                                  LineFast *__restrict__ fast,
                                  uint32_t *__restrict__ meta,
-                                 LineInline *__restrict__ inl) {
+                                 LineInline *__restrict__ inl, SegLayout sl,
+                                 uint32_t *__restrict__ segw) {
 #pragma clang fp contract(off)
   const double u = 1.1102230246251565e-16;
   for (int64_t L = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; L < nlines;
@@ -682,16 +724,19 @@ __global__ void line_prep_kernel(GridP g, const uint64_t *__restrict__ line_off,
     const int seglen = kSegLen[lax];
     const int64_t nseg = ((int64_t)lnax + seglen - 1) / seglen;
     const uint32_t segflag = nseg <= 255 ? kLineSeg : 0u;
+    const int64_t lidx = L - (lax == 0 ? 0 : (lax == 1 ? base1 : base2));
     if (n == 0) {
       meta[L] = 0u;
       li.info = segflag;
       inl[L] = li;
+      write_seg_words(g, li, lax, lidx, sl, segw);
       continue;
     }
     if (n > kLineFastMax || he > cap) {  // he > cap: overflow, the call is re-run
       meta[L] = kLineExact;
       li.info = kLineExact | 255u;
       inl[L] = li;
+      write_seg_words(g, li, lax, lidx, sl, segw);
       continue;
     }
     int ax;
@@ -877,59 +922,7 @@ __global__ void line_prep_kernel(GridP g, const uint64_t *__restrict__ line_off,
     li.info = (uint32_t)(n < 255 ? n : 255) | ((cl + 1u) & 0xFFu) << 8 | (ok ? 0u : kLineExact) |
               segflag;
     inl[L] = li;
-  }
-}
-
-// Segment bytes (SegLayout) of every axis line, one thread per line: per
-// segment, the uniformity test of the sign kernel on the line's summary
-// (seg_uniform on the crossings' segment ranges, or inline_uniform on the
-// segment's coordinate range when the line has none), so the sign kernel
-// reads one byte per line and brick.  With segment ranges and at most 128
-// segments the test runs on all segments at once (seg_words_masks).
-// Consecutive threads take
-// consecutive lines of an axis, whose words of one segment group are
-// adjacent.
-__global__ __launch_bounds__(256) void seg_bytes_kernel(GridP g,
-                                                        const LineInline *__restrict__ inl,
-                                                        int64_t base1, int64_t base2,
-                                                        int64_t nlines, SegLayout sl,
-                                                        uint32_t *__restrict__ segw) {
-  for (int64_t L = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; L < nlines;
-       L += (int64_t)gridDim.x * blockDim.x) {
-    const int ax = L < base1 ? 0 : (L < base2 ? 1 : 2);
-    const int64_t l = L - (ax == 0 ? 0 : (ax == 1 ? base1 : base2));
-    const LineTail r = load_line_tail(inl + L);
-    uint32_t *out = segw + sl.off[ax] + l;
-    const int64_t stride = sl.nl[ax];
-    const int ns = sl.ns[ax];
-    const int nw = (ns + 3) / 4;
-    if ((r.info & kLineSeg) && ns <= 128) {
-      seg_words_masks(r, ns, [&](int w, uint32_t word) { out[(int64_t)w * stride] = word; });
-    } else {
-      const LineInline li = inl[L];
-      const double oa = ax == 0 ? g.ox : (ax == 1 ? g.oy : g.oz);
-      const int32_t na = ax == 0 ? g.nx : (ax == 1 ? g.ny : g.nz);
-      const int len = kSegLen[ax];
-      const bool seg = (r.info & kLineSeg) != 0u;
-      uint32_t word = 0u;
-      for (int sgi = 0; sgi < ns; sgi++) {
-        int p, q;
-        bool u;
-        if (seg) {
-          u = seg_uniform(r, sgi, &p, &q);
-        } else {
-          const int64_t e = (int64_t)sgi * len + len - 1;
-          const double lo = lattice_coord(oa, g.h, (int64_t)sgi * len);
-          const double hi = lattice_coord(oa, g.h, e < na - 1 ? e : na - 1);
-          u = inline_uniform(li, lo, hi, &p, &q);
-        }
-        word |= (u ? (uint32_t)(1 | (p << 1) | (q << 2)) : 0u) << (8 * (sgi & 3));
-        if ((sgi & 3) == 3 || sgi == ns - 1) {
-          out[(int64_t)(sgi >> 2) * stride] = word;
-          word = 0u;
-        }
-      }
-    }
+    write_seg_words(g, li, lax, lidx, sl, segw);
   }
 }
 
@@ -1140,15 +1133,13 @@ void line_hits_finish(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
   LineFast *fast = ar.get<LineFast>("line_fast", cap + 1);
   uint32_t *meta = ar.get<uint32_t>("line_meta", lb.nlines + 1);
   LineInline *inl = ar.get<LineInline>("line_inline", lb.nlines + 1);
+  // the summaries, then each line's segment bytes (SegLayout) from them
+  const SegLayout sl = seg_layout(g);
+  uint32_t *segw = ar.get<uint32_t>("line_segw", (size_t)sl.total + 1);
   hipLaunchKernelGGL(line_prep_kernel, dim3(capped_grid(lb.nlines, 256, lb.max_blocks)),
                      dim3(256), 0, s, g,
                      (const uint64_t *)lb.loff, (const LineHit *)hits, (uint64_t)cap, lb.nlx,
-                     lb.nlx + lb.nly, lb.nlines, fast, meta, inl);
-  const SegLayout sl = seg_layout(g);
-  uint32_t *segw = ar.get<uint32_t>("line_segw", (size_t)sl.total + 1);
-  hipLaunchKernelGGL(seg_bytes_kernel, dim3(capped_grid(lb.nlines, 256, lb.max_blocks)),
-                     dim3(256), 0, s, g, (const LineInline *)inl, lb.nlx, lb.nlx + lb.nly,
-                     lb.nlines, sl, segw);
+                     lb.nlx + lb.nly, lb.nlines, fast, meta, inl, sl, segw);
   // the sentinel read by lanes whose line serves no point (byte 1 everywhere)
   launch_fill_u32(s, segw + sl.total, 1, 0x01010101u);
   SDF_HIP(hipGetLastError());

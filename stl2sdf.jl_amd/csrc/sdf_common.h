@@ -667,6 +667,19 @@ struct SignDims {
   int32_t sx, sy, sz;
 };
 
+// What the sign kernel needs of one position m of the brick launch order,
+// worked out before it runs (brick_plan_kernel): the brick's coordinates
+// (xy = bx | by << 16, z = bz), its four distance slots (the first work
+// item of each of its 1 x 2 x 2 distance tiles, or the spare slot
+// chunk_cap for a tile without work) and flags (bit 0: some tile has
+// distance work; bit 1: m is a brick of the grid, not padding).  32 bytes,
+// one scalar load.
+struct BrickPlan {
+  uint32_t slot[4];
+  uint32_t xy, z, flags, pad_;
+};
+static_assert(sizeof(BrickPlan) == 32, "one 8-dword scalar load per brick");
+
 __host__ __device__ inline SignDims sign_dims(const GridP &g) {
   SignDims d;
   d.sx = (g.nx + 7) / 8;

@@ -204,6 +204,7 @@ struct FusedJob {
   const uint64_t *chunk_off;
   const float *d2buf;
   const LineInline *inl;
+  const BrickPlan *plan;     // per launch position (null: decoded in the kernel)
   const uint32_t *segw;      // segment bytes, 4 per word (SegLayout), then a sentinel
   int64_t segoff1, segoff2;  // first words of the y- and z-line parts
   int64_t segtotal;          // the sentinel word (0x01010101)
@@ -292,11 +293,18 @@ void launch_learned_fallback(hipStream_t s, const GridP &g, const Tri64 *tri, co
 // Per tile: distance from d2buf (or h), six axis rays, combine, outputs.
 // Points with a vote inside its rounding margin go to pend0 (restart from
 // ray 1 on the LBVH); non-unanimous votes go to pend (continue at ray 7).
+// Launch positions of the sign kernel (BrickPlan entries it reads).
+int64_t brick_plan_size(const GridP &g);
+// The sign kernel's per-brick plan (coordinates, distance slots) from the
+// distance items' offsets (d2buf null: coordinates only).
+void launch_brick_plan(hipStream_t s, const GridP &g, const Bins &b, const float *d2buf,
+                       BrickPlan *plan);
 void launch_fused_lattice(hipStream_t s, const GridP &g, const Tri64 *tri,
                           const Bins &b, const float *d2buf, bool do_sign,
                           FusedOut out, Pending *pend, uint32_t *npend,
                           uint32_t pend_cap, Pending *pend0, uint32_t *npend0,
-                          uint32_t pend0_cap, uint32_t *err, int64_t max_blocks = 0);
+                          uint32_t pend0_cap, uint32_t *err, const BrickPlan *plan,
+                          int64_t max_blocks = 0);
 
 struct RayParams {
   double dirs[kNumDirs * 3];

@@ -492,8 +492,10 @@ enum : int { kOutSdf = 1, kOutDist = 2, kOutSigns = 4, kOutConf = 8, kOutAll = 1
 // One sign brick (8 x 8 x 4 points) of job J for the 64-lane block (one
 // wave: its LDS hand-offs are wave_lds_sync); m is the brick's position in
 // the brick launch order; C is the LDS copy of the job's cold fields;
-// outputs outside kOut are not written.  Uniform per block.
-template <int kOut>
+// outputs outside kOut are not written.  kPlan: the brick's coordinates
+// and distance slots come from J.plan[m] (BrickPlan, one scalar load)
+// instead of being decoded and looked up here.  Uniform per block.
+template <int kOut, bool kPlan>
 __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S,
                                    const FusedCold &C) {
   const GridP &g = J.g;
@@ -518,7 +520,22 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S,
   const FusedOut out = J.out;
   const int64_t chunk_cap = J.chunk_cap;
   int32_t bx, by, bz;
-  if (!brick_tile(m, J.bd, &bx, &by, &bz)) return;  // uniform
+  BrickPlan P;
+  if (kPlan) {
+    // eight adjacent words through the constant view: one scalar load
+    const KConst<uint32_t> pw = kconst(reinterpret_cast<const uint32_t *>(J.plan + m));
+#pragma unroll
+    for (int t = 0; t < 4; t++) P.slot[t] = pw[t];
+    P.xy = pw[4];
+    P.z = pw[5];
+    P.flags = pw[6];
+    if (!(P.flags & 2u)) return;  // padding position (uniform)
+    bx = (int32_t)(P.xy & 0xFFFFu);
+    by = (int32_t)(P.xy >> 16);
+    bz = (int32_t)P.z;
+  } else if (!brick_tile(m, J.bd, &bx, &by, &bz)) {
+    return;  // uniform
+  }
   const int lane = threadIdx.x;
   const int32_t i = bx * 8 + (lane & 7);
   const int32_t j = by * 8 + (lane >> 3);
@@ -548,7 +565,7 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S,
                 "a sign brick covers 1 x 2 x 2 distance tiles");
   uint64_t c0[4] = {0, 0, 0, 0}, c1[4] = {0, 0, 0, 0};
   bool tok[4] = {false, false, false, false};
-  if (has_d2) {
+  if (has_d2 && !kPlan) {
 #pragma unroll
     for (int t = 0; t < 4; t++) {  // no branches: the eight loads go out together
       const int32_t tyy = 2 * by + (t & 1), tzz = 2 * bz + (t >> 1);
@@ -589,11 +606,17 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S,
     // a tile without distance work reads the spare tile of +inf after the
     // last work item: the loads are unconditional and need no select
     uint32_t slot[4];
+    if (kPlan) {
 #pragma unroll
-    for (int t = 0; t < 4; t++) {
-      slot[t] = tok[t] && c1[t] > c0[t] && (int64_t)c0[t] < chunk_cap ? (uint32_t)c0[t]
-                                                                       : (uint32_t)chunk_cap;
-      work |= slot[t] != (uint32_t)chunk_cap;
+      for (int t = 0; t < 4; t++) slot[t] = P.slot[t];
+      work = (P.flags & 1u) != 0u;
+    } else {
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        slot[t] = tok[t] && c1[t] > c0[t] && (int64_t)c0[t] < chunk_cap ? (uint32_t)c0[t]
+                                                                         : (uint32_t)chunk_cap;
+        work |= slot[t] != (uint32_t)chunk_cap;
+      }
     }
     // a brick no triangle is near (most of the grid) keeps +inf: distance h
     if (work) {
@@ -792,7 +815,44 @@ void fused_lattice_kernel(FusedJob J) {
   // flight may still target
   const uint32_t nb = (uint32_t)J.nbricks;
   for (uint32_t m = (uint32_t)xcd_remap(blockIdx.x, gridDim.x); m < nb; m += gridDim.x)
-    fused_brick<kOut>(J, (int64_t)m, S, C);
+    fused_brick<kOut, true>(J, (int64_t)m, S, C);
+}
+
+// The sign kernel's plan, one thread per launch position: the brick decode
+// and the distance-slot lookup the kernel would otherwise repeat per brick
+// on its scalar unit, with the same rules (a tile outside the grid or
+// without work items, or whose first item lies beyond the capacity, takes
+// the spare slot chunk_cap).
+__global__ __launch_bounds__(256) void brick_plan_kernel(FusedJob J, BrickPlan *__restrict__ plan) {
+  const TileDims &td = J.td;
+  const int64_t chunk_cap = J.chunk_cap;
+  for (int64_t m = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; m < J.nbricks;
+       m += (int64_t)gridDim.x * blockDim.x) {
+    BrickPlan P;
+    P.slot[0] = P.slot[1] = P.slot[2] = P.slot[3] = (uint32_t)chunk_cap;
+    P.xy = P.z = P.flags = P.pad_ = 0u;
+    int32_t bx, by, bz;
+    if (brick_tile(m, J.bd, &bx, &by, &bz)) {
+      P.xy = (uint32_t)bx | (uint32_t)by << 16;
+      P.z = (uint32_t)bz;
+      P.flags = 2u;
+      if (J.d2buf && J.chunk_off) {
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+          const int32_t tyy = 2 * by + (t & 1), tzz = 2 * bz + (t >> 1);
+          if (tyy < td.ty && tzz < td.tz) {
+            const int64_t tile = ((int64_t)tzz * td.ty + tyy) * td.tx + bx;
+            const uint64_t c0 = J.chunk_off[tile], c1 = J.chunk_off[tile + 1];
+            if (c1 > c0 && (int64_t)c0 < chunk_cap) {
+              P.slot[t] = (uint32_t)c0;
+              P.flags |= 1u;
+            }
+          }
+        }
+      }
+    }
+    plan[m] = P;
+  }
 }
 
 // Persistent work queue over the sign bricks of many jobs (static brick
@@ -819,7 +879,7 @@ __global__ __launch_bounds__(64) void fused_queue_kernel(
       while (j + 1 < njobs && s_pref[j + 1] <= u) j++;
       if (threadIdx.x == 0) C = jobs[j].cold;
       __syncthreads();
-      fused_brick<kOutAll>(jobs[j], (int64_t)(u - s_pref[j]), S, C);
+      fused_brick<kOutAll, false>(jobs[j], (int64_t)(u - s_pref[j]), S, C);
     }
   }
 }
@@ -1364,6 +1424,7 @@ FusedJob make_fused_job(const GridP &g, const Bins &b, const float *d2buf, bool 
   J.d2buf = d2buf;
   J.inl = b.line_inline;
   J.segw = b.seg_words;
+  J.plan = nullptr;
   {
     const SegLayout sl = seg_layout(g);
     J.segoff1 = sl.off[1];
@@ -1394,11 +1455,14 @@ void launch_fused_lattice(hipStream_t s, const GridP &g, const Tri64 *tri,
                           const Bins &b, const float *d2buf, bool do_sign,
                           FusedOut out, Pending *pend, uint32_t *npend,
                           uint32_t pend_cap, Pending *pend0, uint32_t *npend0,
-                          uint32_t pend0_cap, uint32_t *err, int64_t max_blocks) {
+                          uint32_t pend0_cap, uint32_t *err, const BrickPlan *plan,
+                          int64_t max_blocks) {
   (void)tri;
-  const FusedJob J = make_fused_job(g, b, d2buf, do_sign, out, pend, npend, pend_cap, pend0,
-                                    npend0, pend0_cap, err);
+  FusedJob J = make_fused_job(g, b, d2buf, do_sign, out, pend, npend, pend_cap, pend0,
+                              npend0, pend0_cap, err);
   if (J.nbricks <= 0) return;
+  if (!plan) throw Error{-1, "the sign kernel needs its brick plan"};
+  J.plan = plan;
   if (J.nbricks >= ((int64_t)1 << 31)) throw Error{-1, "too many sign bricks for one launch"};
   const int64_t grid = max_blocks > 0 && max_blocks < J.nbricks ? max_blocks : J.nbricks;
   const int set = (out.sdf ? kOutSdf : 0) | (out.dist ? kOutDist : 0) |
@@ -1415,6 +1479,30 @@ void launch_fused_lattice(hipStream_t s, const GridP &g, const Tri64 *tri,
                        dim3(64), 0, s, J);
   else
     hipLaunchKernelGGL(fused_lattice_kernel<kOutAll>, dim3((unsigned)grid), dim3(64), 0, s, J);
+  SDF_HIP(hipGetLastError());
+}
+
+int64_t brick_plan_size(const GridP &g) {
+  const SignDims sd = sign_dims(g);
+  TileDims bd;
+  bd.tx = sd.sx;
+  bd.ty = sd.sy;
+  bd.tz = sd.sz;
+  bd.ntiles = (int64_t)sd.sx * sd.sy * sd.sz;
+  return tile_dims(g).ntiles > 0 ? brick_launch_size(bd) : 0;
+}
+
+void launch_brick_plan(hipStream_t s, const GridP &g, const Bins &b, const float *d2buf,
+                       BrickPlan *plan) {
+  const FusedJob J = make_fused_job(g, b, d2buf, false, FusedOut(), nullptr, nullptr, 0,
+                                    nullptr, nullptr, 0, nullptr);
+  if (J.nbricks <= 0) return;
+  if (J.nbricks >= ((int64_t)1 << 31)) throw Error{-1, "too many sign bricks for one launch"};
+  if (J.bd.tx >= 65536 || J.bd.ty >= 65536)
+    throw Error{-3, "grid too wide for the sign kernel's 16-bit brick coordinates"};
+  const int64_t blocks = (J.nbricks + 255) / 256;
+  hipLaunchKernelGGL(brick_plan_kernel, dim3((unsigned)(blocks < 65536 ? blocks : 65536)),
+                     dim3(256), 0, s, J, plan);
   SDF_HIP(hipGetLastError());
 }
 
