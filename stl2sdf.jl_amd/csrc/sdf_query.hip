@@ -798,13 +798,13 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S,
   wave_lds_sync();  // S is reused by the block's next brick
 }
 
-// Seven waves per SIMD (72 registers, no spills to scratch).  Per brick the
-// kernel issues about 380 scalar and 440 vector instructions and waits on
-// two load round trips, so occupancy pays: on the star surface (82.6 M
-// points) 0.358 ms at six waves against 0.370 ms at five, and with the FP32
-// root 0.327 ms at seven against 0.329 ms at six.
+// Eight waves per SIMD (64 registers, no spills to scratch).  A brick waits
+// on two load round trips (its plan and segment bytes, then its distances),
+// so occupancy pays: on the star surface (82.6 M points) 0.294-0.297 ms at
+// eight waves against 0.300-0.303 ms at seven (earlier builds: 0.358 ms at
+// six against 0.370 ms at five).
 template <int kOut>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(7)))
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8)))
 void fused_lattice_kernel(FusedJob J) {
   __shared__ FusedShared S;
   __shared__ FusedCold C;
