@@ -159,6 +159,65 @@ def cpu_baseline(mesh, grid, target_s, threads):
     return grid.ngp / full_s, (n1, ts1, n2, ts2, build_s, full_s)
 
 
+def batch_bench(L, ctx, torch, runs=5):
+    """The batched configuration: 64 seeded star surfaces, triangle counts
+    log-uniform in [2k, 500k], grids of 64/96/128/192 points on the longest
+    axis, as one autotuned batch; timed over `runs` runs (median), beside
+    the same meshes one after another through the single-mesh device entry
+    point on one context, and checked against it bit for bit."""
+    from stl2sdf import Batch, MeshGrid, synthetic
+    meshes, grids = [], []
+    for X, IEN, g, _ in synthetic.batch_family(64, 2):
+        mesh = MeshGrid.TriangularMesh(X, IEN)
+        xmin, xmax = MeshGrid.getMesh_AABB(mesh.X)
+        meshes.append(mesh)
+        grids.append(MeshGrid.Grid(xmin, xmax, g - 7, 3))
+    pts = sum(int(g.ngp) for g in grids)
+    log(f"[bench] batch: {len(meshes)} meshes, {sum(m.nel for m in meshes)} triangles, "
+        f"{pts} points")
+    b = Batch.SignedDistanceBatch(meshes, grids, autotune=True)
+    b.run()
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        b.run()
+        ts.append(time.perf_counter() - t0)
+    t_batch = float(np.median(ts))
+    info = b.info()
+    # the same meshes one by one through sdf_signed_distance_device
+    from stl2sdf import _lib
+    outs = [_lib.DeviceBuffer(8 * int(g.ngp)) for g in grids]
+    args_ = [(_lib.f64_3(g.AABB_min), float(g.cell_size), _lib.i64_3(g.N)) for g in grids]
+
+    def seq():
+        for i, m in enumerate(meshes):
+            _lib.check(L.sdf_mesh_set_device(ctx, b._X[i].ptr, m.nnp, b._I[i].ptr, m.nel))
+            amin, h, N = args_[i]
+            _lib.check(L.sdf_signed_distance_device(ctx, _lib.dptr(amin), h, _lib.i64ptr(N),
+                                                    outs[i].ptr))
+        torch.cuda.synchronize()
+
+    seq()
+    tq = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        seq()
+        tq.append(time.perf_counter() - t0)
+    t_seq = float(np.median(tq))
+    got = b.results()
+    identical = all(np.array_equal(got[i], outs[i].to_host(np.float64, int(g.ngp)))
+                    for i, g in enumerate(grids))
+    b.close()
+    log(f"[bench] batch {t_batch * 1e3:.2f} ms, one by one {t_seq * 1e3:.2f} ms, "
+        f"identical={identical}")
+    return {"value": pts / t_batch, "unit": "signed grid points/s", "ms_per_run": t_batch * 1e3,
+            "meshes": len(meshes), "triangles": int(sum(m.nel for m in meshes)),
+            "points": pts, "sequential_ms": t_seq * 1e3, "vs_sequential": t_seq / t_batch,
+            "identical_to_single_mesh": identical,
+            "tune": {k: info[k] for k in ("lanes", "chunk_len", "ray_stack")},
+            "candidates": info["tuning"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -169,6 +228,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--fit-steps", type=int, default=1500)
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the batched 64-mesh family (the batched configuration)")
     ap.add_argument("--tune", action="append", default=[], metavar="NAME=VALUE",
                     help="launch tunable of the library (sdf_ctx_tune); results do not change")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
@@ -282,6 +343,8 @@ def main():
         }
         log(f"[bench] learned: {lval:.4g} pts/s, identical={identical}")
 
+    batched = None if args.no_batch else batch_bench(L, ctx, torch)
+
     ms_step = dt / args.steps * 1e3
     value = grid.ngp / (dt / args.steps)
     stage_ms = {k: float(np.mean(v)) for k, v in stages.items()}
@@ -316,6 +379,8 @@ def main():
                     "learned_radius": "same workload with the learned search radius"}
     if learned is not None:
         res["learned_radius"] = learned
+    if batched is not None:
+        res["batched"] = batched
     if not args.no_cpu_baseline:
         # every host core this process may use: the affinity mask, bounded by
         # the OpenMP thread count the environment sets (the share of the

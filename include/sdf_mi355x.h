@@ -320,28 +320,32 @@ sdf_status sdf_learned_stats(sdf_ctx *ctx, double *out, int32_t cap, int32_t *n)
 /* ------------------------------------------------------------------
  * Batches.  No counterpart in the reference (it handles one mesh per
  * call): many device-resident meshes, each with its own grid, computed as
- * by sdf_signed_distance_device in one submission.  Per-mesh binning and
- * LBVH chains run concurrently; the distance and sign stages of all meshes
- * are one persistent work-queue launch each; after planning passes that fix
- * every list capacity the whole pass is captured into a hipGraph, which
- * sdf_batch_run replays.  autotune != 0 times candidate work-item sizes,
- * queue shapes and ray-stack depths on the batch itself first.
+ * by sdf_signed_distance_device (bit for bit) in one submission.  The
+ * meshes are dealt round-robin to worker contexts ("lanes", each with its
+ * own streams and buffers); a run issues every mesh's pass with no host
+ * wait, so the lanes' passes overlap on the GPU, and reads all status
+ * blocks once at the end.  Creation plans every mesh (sizes its lists and
+ * the lanes' buffers) and, with autotune != 0, first times candidate lane
+ * counts, distance work-item lengths and ray-stack depths on the batch
+ * itself and keeps the fastest.
  * Arrays are per mesh: d_X[i] (3 x nnp[i] doubles), d_IEN[i] (3 x nel[i]
  * 1-based int64), aabb_min[3i..], h[i], N[3i..], d_sdf_out[i] (prod(N+1)
  * doubles); all device pointers stay valid and in place for the batch's
- * life.  At most 256 meshes. */
+ * life.  At most 4096 meshes. */
 typedef struct sdf_batch sdf_batch;
 sdf_status sdf_batch_create(sdf_ctx *ctx, int32_t nmesh, const double *const *d_X,
                             const int64_t *nnp, const int64_t *const *d_IEN,
                             const int64_t *nel, const double *aabb_min, const double *h,
                             const int64_t *N, double *const *d_sdf_out, int32_t autotune,
                             sdf_batch **out);
-/* One replay: recomputes every mesh from its current device arrays and
- * synchronises (lists that outgrow their capacities are re-planned). */
+/* One run: recomputes every mesh from its current device arrays and
+ * synchronises (a mesh whose lists outgrew their capacities is computed
+ * again on its own). */
 sdf_status sdf_batch_run(sdf_batch *batch);
-/* meshes, grid points, last run ms, then the chosen tunables (work-item
- * triangles, ray-stack depth, distance queue blocks and grab, sign queue
- * blocks and grab), the number of timed candidates and (id, ms) pairs. */
+/* meshes, grid points, last run ms (wall time of issue, passes and status
+ * read), then the chosen tunables (work-item triangles, ray-stack depth,
+ * lanes), the number of timed candidates and their (id, ms) pairs (id:
+ * lanes, 100 + work-item triangles, 1000 + ray-stack depth). */
 sdf_status sdf_batch_info(sdf_batch *batch, double *out, int32_t cap, int32_t *n);
 sdf_status sdf_batch_destroy(sdf_batch *batch);
 

@@ -812,41 +812,79 @@ double now_ms() {
 }
 
 
+// Context set-up and release shared by sdf_ctx_create / sdf_ctx_destroy
+// and the batch's worker contexts (own stream, two helper streams, events,
+// status words).
+void ctx_init(sdf_ctx *c, int32_t device) {
+  c->device = device;
+  SDF_HIP(hipSetDevice(device));
+  SDF_HIP(hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking));
+  c->stream = c->own;
+  for (int q = 0; q < 2; q++) SDF_HIP(hipStreamCreateWithFlags(&c->aux[q], hipStreamNonBlocking));
+  for (int q = 0; q < 4; q++) SDF_HIP(hipEventCreateWithFlags(&c->join[q], hipEventDisableTiming));
+  SDF_HIP(hipEventCreateWithFlags(&c->ev_plan, hipEventDisableTiming));
+  SDF_HIP(hipMalloc(&c->d_words, 256));
+  SDF_HIP(hipMalloc(&c->d_status, 16 * sizeof(uint64_t)));
+  SDF_HIP(hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device));
+  c->scratch = scratch_address(device);  // with `device` current
+}
+
+void ctx_free(sdf_ctx *c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  c->ar.release();
+  for (auto e : c->ev) (void)hipEventDestroy(e);
+  for (int q = 0; q < 2; q++)
+    if (c->aux[q]) {
+      (void)hipStreamSynchronize(c->aux[q]);
+      (void)hipStreamDestroy(c->aux[q]);
+    }
+  for (int q = 0; q < 4; q++)
+    if (c->join[q]) (void)hipEventDestroy(c->join[q]);
+  if (c->ev_plan) (void)hipEventDestroy(c->ev_plan);
+  if (c->d_words) (void)hipFree(c->d_words);
+  if (c->d_status && c->own_status) (void)hipFree(c->d_status);
+  if (c->own) (void)hipStreamDestroy(c->own);
+  delete c;
+}
+
 // ---------------------------------------------------------------- batches
-// Many meshes and grids in one submission.  Each mesh has a child context
-// (own buffers, capacities, LBVH) whose per-mesh chains run on one of four
-// capture streams; the distance and sign stages of all meshes are one
-// persistent work-queue launch each.  After planning passes that fix every
-// capacity, the whole pass is captured into one hipGraph and replayed.
+// Many meshes and grids in one submission.  The meshes are dealt round-robin
+// to `lanes` worker contexts (each set up as sdf_ctx_create sets up a
+// context: own stream, helper streams, buffers); every mesh keeps its own
+// list capacities, which its lane takes on before the mesh's pass.  A run
+// issues every mesh's lattice pass with no host wait (capacities known from
+// earlier runs), so the lanes' passes overlap on the GPU, then reads every
+// mesh's status block once; a mesh whose lists outgrew their capacities is
+// computed again on its own.  Results equal sdf_signed_distance_device's
+// bit for bit.
 struct BatchTune {
+  int lanes = 4;            // worker contexts
   int chunk_len = 64;       // triangles per distance work item
   int ray_stack = 1024;     // LDS ray-stack depth of the vote stage
-  int dist_blocks = 2048, dist_grab = 4;
-  int fused_blocks = 2048, fused_grab = 16;
 };
+
+struct BatchMesh {
+  const double *dX = nullptr;
+  const int64_t *dIEN = nullptr;
+  int64_t nnp = 0, nel = 0;
+  sdf::GridP g;
+  double *out = nullptr;
+  sdf_ctx::Caps caps;  // this mesh's list capacities
+};
+
+constexpr int kMaxLanes = 8;
+constexpr int kMaxBatchMeshes = 4096;
 
 }  // namespace
 
 struct sdf_batch {
   sdf_ctx *parent = nullptr;
-  int n = 0;
-  std::vector<sdf_ctx *> kids;
+  std::vector<sdf_ctx *> lanes;  // kMaxLanes worker contexts
+  std::vector<BatchMesh> mesh;
   std::vector<LatticeRun> runs;
-  std::vector<const double *> dX;
-  std::vector<const int64_t *> dIEN;
-  std::vector<int64_t> nnp, nel;
-  hipStream_t s[4] = {};
-  hipStream_t aux[4][2] = {};
-  hipEvent_t ev_fork = nullptr, ev_fused = nullptr, ev_t0 = nullptr, ev_t1 = nullptr;
-  hipEvent_t ev_join[4] = {}, ev_join2[4] = {};
-  sdf::Arena ar;
-  sdf::DistJob *d_djobs = nullptr;
-  sdf::FusedJob *d_fjobs = nullptr;
-  uint64_t *d_dpref = nullptr, *d_fpref = nullptr;
-  unsigned long long *d_cnt = nullptr;
-  uint64_t *d_status = nullptr;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  uint64_t *d_status = nullptr;  // 16 words per mesh
   BatchTune tune;
   std::vector<double> tune_log;  // (candidate id, ms) pairs
   int64_t total_points = 0;
@@ -855,233 +893,116 @@ struct sdf_batch {
 
 namespace {
 
-sdf_ctx *batch_child(sdf_batch *b, int i) {
-  sdf_ctx *k = new sdf_ctx();
-  k->device = b->parent->device;
-  k->stream = b->s[i % 4];
-  k->aux[0] = b->aux[i % 4][0];
-  k->aux[1] = b->aux[i % 4][1];
-  for (int q = 0; q < 4; q++) SDF_HIP(hipEventCreateWithFlags(&k->join[q], hipEventDisableTiming));
-  SDF_HIP(hipEventCreateWithFlags(&k->ev_plan, hipEventDisableTiming));
-  SDF_HIP(hipMalloc(&k->d_words, 256));
-  k->d_status = b->d_status + 16 * i;
-  k->own_status = false;
-  return k;
-}
-
 void batch_free(sdf_batch *b) {
   if (!b) return;
-  (void)hipSetDevice(b->parent->device);
-  for (int q = 0; q < 4; q++)
-    if (b->s[q]) (void)hipStreamSynchronize(b->s[q]);
-  if (b->exec) (void)hipGraphExecDestroy(b->exec);
-  if (b->graph) (void)hipGraphDestroy(b->graph);
-  for (sdf_ctx *k : b->kids) {
-    if (!k) continue;
-    k->ar.release();
-    for (int q = 0; q < 4; q++)
-      if (k->join[q]) (void)hipEventDestroy(k->join[q]);
-    if (k->ev_plan) (void)hipEventDestroy(k->ev_plan);
-    if (k->d_words) (void)hipFree(k->d_words);
-    delete k;
-  }
-  b->ar.release();
-  for (int q = 0; q < 4; q++) {
-    if (q > 0 && b->s[q]) (void)hipStreamDestroy(b->s[q]);
-    for (int a = 0; a < 2; a++)
-      if (b->aux[q][a]) (void)hipStreamDestroy(b->aux[q][a]);
-    if (b->ev_join[q]) (void)hipEventDestroy(b->ev_join[q]);
-    if (b->ev_join2[q]) (void)hipEventDestroy(b->ev_join2[q]);
-  }
-  for (hipEvent_t e : {b->ev_fork, b->ev_fused, b->ev_t0, b->ev_t1})
-    if (e) (void)hipEventDestroy(e);
+  for (sdf_ctx *k : b->lanes) ctx_free(k);
+  if (b->d_status) (void)hipFree(b->d_status);
   delete b;
 }
 
-// One pass over every mesh (capturable when all capacities are known).
-// upload_tables: (re)build the job tables from this pass's buffers.
-void batch_pass(sdf_batch *b, bool upload_tables) {
-  hipStream_t s0 = b->s[0];
-  SDF_HIP(hipEventRecord(b->ev_fork, s0));
-  for (int q = 1; q < 4; q++) SDF_HIP(hipStreamWaitEvent(b->s[q], b->ev_fork, 0));
-  for (int i = 0; i < b->n; i++) {
-    sdf_ctx *k = b->kids[i];
-    k->chunk_len = b->tune.chunk_len;
-    k->ray_stack = b->tune.ray_stack;
-    reset_words(k);
-    set_mesh_device(k, b->dX[i], b->nnp[i], b->dIEN[i], b->nel[i]);
-    stage_bins(k, b->runs[i]);
-  }
-  for (int q = 1; q < 4; q++) {
-    SDF_HIP(hipEventRecord(b->ev_join[q], b->s[q]));
-    SDF_HIP(hipStreamWaitEvent(s0, b->ev_join[q], 0));
-  }
-  if (upload_tables) {
-    std::vector<sdf::DistJob> dj;
-    std::vector<sdf::FusedJob> fj;
-    std::vector<uint64_t> fpref(1, 0);
-    for (int i = 0; i < b->n; i++) {
-      sdf_ctx *k = b->kids[i];
-      LatticeRun &L = b->runs[i];
-      dj.push_back(sdf::make_dist_job(L.g, k->tri, L.bins, L.d2buf, nullptr));
-      fj.push_back(sdf::make_fused_job(L.g, L.bins, L.d2buf, true, L.out, L.pend,
-                                       k->d_words + 1, L.cap, L.pend0, k->d_words + 2, L.cap0,
-                                       k->d_words));
-      fpref.push_back(fpref.back() + (uint64_t)fj.back().nbricks);
-    }
-    SDF_HIP(hipMemcpyAsync(b->d_djobs, dj.data(), dj.size() * sizeof(sdf::DistJob),
-                           hipMemcpyHostToDevice, s0));
-    SDF_HIP(hipMemcpyAsync(b->d_fjobs, fj.data(), fj.size() * sizeof(sdf::FusedJob),
-                           hipMemcpyHostToDevice, s0));
-    SDF_HIP(hipMemcpyAsync(b->d_fpref, fpref.data(), fpref.size() * sizeof(uint64_t),
-                           hipMemcpyHostToDevice, s0));
-    SDF_HIP(hipStreamSynchronize(s0));
-  }
-  sdf::launch_dist_queue(s0, b->d_djobs, b->n, b->d_dpref, b->d_cnt, b->tune.dist_blocks,
-                         b->tune.dist_grab);
-  for (int i = 0; i < b->n; i++) SDF_HIP(hipStreamWaitEvent(s0, b->kids[i]->join[1], 0));
-  sdf::launch_fill_u64(s0, (uint64_t *)b->d_cnt + 1, 1, 0ull);
-  sdf::launch_fused_queue(s0, b->d_fjobs, b->n, b->d_fpref, b->d_cnt + 1, b->tune.fused_blocks,
-                          b->tune.fused_grab);
-  SDF_HIP(hipEventRecord(b->ev_fused, s0));
-  for (int q = 1; q < 4; q++) SDF_HIP(hipStreamWaitEvent(b->s[q], b->ev_fused, 0));
-  for (int i = 0; i < b->n; i++) stage_resolve(b->kids[i], b->runs[i]);
-  for (int q = 1; q < 4; q++) {
-    SDF_HIP(hipEventRecord(b->ev_join2[q], b->s[q]));
-    SDF_HIP(hipStreamWaitEvent(s0, b->ev_join2[q], 0));
-  }
+// Issue mesh i's lattice pass on its lane, status into the batch's block.
+void batch_issue(sdf_batch *b, int i) {
+  sdf_ctx *k = b->lanes[i % b->tune.lanes];
+  BatchMesh &m = b->mesh[(size_t)i];
+  k->caps = m.caps;
+  k->chunk_len = b->tune.chunk_len;
+  k->ray_stack = b->tune.ray_stack;
+  k->d_status = b->d_status + 16 * (int64_t)i;
+  k->nev = 0;
+  reset_words(k);
+  set_mesh_device(k, m.dX, m.nnp, m.dIEN, m.nel);
+  LatticeRun &L = b->runs[(size_t)i];
+  L = LatticeRun();
+  L.g = m.g;
+  L.do_dist = true;
+  L.do_sign = true;
+  L.rp = ray_params(6, 30, 0.6, 1);
+  L.out.sdf = m.out;
+  lattice_pass(k, L);
+  m.caps = k->caps;  // capacities a first pass sized exactly
 }
 
-// Reads every mesh's status; grows overflowed capacities (true: repeat).
-bool batch_settle(sdf_batch *b) {
-  std::vector<uint64_t> st((size_t)16 * b->n);
-  SDF_HIP(hipMemcpyAsync(st.data(), b->d_status, st.size() * sizeof(uint64_t),
-                         hipMemcpyDeviceToHost, b->s[0]));
-  SDF_HIP(hipStreamSynchronize(b->s[0]));
-  bool again = false;
-  for (int i = 0; i < b->n; i++) {
+// Mesh i alone, until its capacities hold (after an overflow).
+void batch_redo(sdf_batch *b, int i) {
+  sdf_ctx *k = b->lanes[i % b->tune.lanes];
+  BatchMesh &m = b->mesh[(size_t)i];
+  k->caps = m.caps;
+  k->chunk_len = b->tune.chunk_len;
+  k->ray_stack = b->tune.ray_stack;
+  k->d_status = b->d_status + 16 * (int64_t)i;
+  k->nev = 0;
+  reset_words(k);
+  set_mesh_device(k, m.dX, m.nnp, m.dIEN, m.nel);
+  sdf::FusedOut out;
+  out.sdf = m.out;
+  run_lattice(k, m.g, true, true, ray_params(6, 30, 0.6, 1), out);
+  m.caps = k->caps;
+}
+
+// Every mesh alone on its lane, waiting for each: sizes every mesh's lists
+// and grows each lane's buffers to the largest of its meshes, so that the
+// overlapped runs after it allocate nothing while other lanes' passes are
+// in flight.  Needed whenever the mesh-to-lane assignment changes.
+void batch_plan(sdf_batch *b) {
+  for (int i = 0; i < (int)b->mesh.size(); i++) batch_redo(b, i);
+}
+
+// One run over every mesh; returns its wall time in ms (the issue, the
+// passes and the status read; re-runs of overflowed meshes excluded).
+double batch_run(sdf_batch *b) {
+  const auto t0 = std::chrono::steady_clock::now();
+  const int n = (int)b->mesh.size();
+  for (int i = 0; i < n; i++) batch_issue(b, i);
+  for (int l = 0; l < b->tune.lanes && l < n; l++)
+    SDF_HIP(hipStreamSynchronize(b->lanes[(size_t)l]->stream));
+  std::vector<uint64_t> st((size_t)16 * n);
+  SDF_HIP(hipMemcpy(st.data(), b->d_status, st.size() * sizeof(uint64_t),
+                    hipMemcpyDeviceToHost));
+  const double ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  for (int i = 0; i < n; i++) {
+    sdf_ctx *k = b->lanes[i % b->tune.lanes];
+    BatchMesh &m = b->mesh[(size_t)i];
+    k->caps = m.caps;
+    bool again;
     try {
-      again |= lattice_settle(b->kids[i], b->runs[i], st.data() + 16 * i);
+      again = lattice_settle(k, b->runs[(size_t)i], st.data() + 16 * (int64_t)i);
     } catch (const Error &e) {
       throw Error{e.code, "mesh " + std::to_string(i) + ": " + e.msg};
     }
+    m.caps = k->caps;
+    if (again) batch_redo(b, i);
   }
-  return again;
+  return ms;
 }
 
-// Non-captured passes until every capacity holds; returns the last pass's
-// time in ms (events on the origin stream).
-double batch_plan(sdf_batch *b) {
-  float ms = 0.f;
-  for (int attempt = 0;; attempt++) {
-    // a pass that sized some list exactly (first pass of a mesh) is
-    // followed by one with the recorded capacities, the sizes a capture uses
-    bool exact = false;
-    for (sdf_ctx *k : b->kids)
-      exact |= k->caps.pairs < 0 || k->caps.chunks < 0 || k->caps.hits < 0;
-    SDF_HIP(hipEventRecord(b->ev_t0, b->s[0]));
-    batch_pass(b, true);
-    SDF_HIP(hipEventRecord(b->ev_t1, b->s[0]));
-    const bool again = batch_settle(b);
-    if (!again && !exact) break;
-    if (attempt >= 5) fail(SDF_ERR_INTERNAL, "batch capacities did not settle");
-  }
-  SDF_HIP(hipEventElapsedTime(&ms, b->ev_t0, b->ev_t1));
-  return (double)ms;
-}
-
-void batch_capture(sdf_batch *b) {
-  if (b->exec) SDF_HIP(hipGraphExecDestroy(b->exec));
-  if (b->graph) SDF_HIP(hipGraphDestroy(b->graph));
-  b->exec = nullptr;
-  b->graph = nullptr;
-  for (sdf_ctx *k : b->kids) k->capturing = true;
-  SDF_HIP(hipStreamBeginCapture(b->s[0], hipStreamCaptureModeRelaxed));
-  try {
-    batch_pass(b, false);
-  } catch (...) {
-    hipGraph_t g = nullptr;
-    (void)hipStreamEndCapture(b->s[0], &g);
-    if (g) (void)hipGraphDestroy(g);
-    for (sdf_ctx *k : b->kids) k->capturing = false;
-    throw;
-  }
-  SDF_HIP(hipStreamEndCapture(b->s[0], &b->graph));
-  for (sdf_ctx *k : b->kids) k->capturing = false;
-  SDF_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
-}
-
-// Times a few candidates of each tunable (best of two planned passes each)
-// and keeps the fastest; ties keep the earlier candidate.
+// Times candidate lane counts, work-item lengths and ray-stack depths (best
+// of two runs each, after a planning pass for the candidate) and keeps the
+// fastest; ties keep the earlier candidate.
 void batch_autotune(sdf_batch *b) {
   auto measure = [&](int id) {
     batch_plan(b);
-    const double ms = batch_plan(b);
+    const double ms = std::min(batch_run(b), batch_run(b));
     b->tune_log.push_back((double)id);
     b->tune_log.push_back(ms);
     return ms;
   };
-  int id = 0;
-  {  // triangles per distance work item
-    const int cand[3] = {64, 32, 16};
+  auto pick = [&](int *field, std::initializer_list<int> cand, int base) {
     double best = 1e30;
-    int pick = b->tune.chunk_len;
-    for (int c : cand) {
-      b->tune.chunk_len = c;
-      const double ms = measure(100 + c);
+    int keep = *field;
+    for (int v : cand) {
+      *field = v;
+      const double ms = measure(base + v);
       if (ms < best) {
         best = ms;
-        pick = c;
+        keep = v;
       }
     }
-    b->tune.chunk_len = pick;
-  }
-  {  // distance queue: persistent blocks x units per grab
-    const int blocks[3] = {1024, 2048, 4096}, grabs[3] = {1, 4, 16};
-    double best = 1e30;
-    BatchTune pick = b->tune;
-    for (int bl : blocks)
-      for (int gr : grabs) {
-        b->tune.dist_blocks = bl;
-        b->tune.dist_grab = gr;
-        const double ms = measure(1000 + bl / 256 * 10 + gr);
-        if (ms < best) {
-          best = ms;
-          pick = b->tune;
-        }
-      }
-    b->tune = pick;
-  }
-  {  // sign queue
-    const int grabs[3] = {4, 16, 64};
-    double best = 1e30;
-    int pick = b->tune.fused_grab;
-    for (int gr : grabs) {
-      b->tune.fused_grab = gr;
-      const double ms = measure(2000 + gr);
-      if (ms < best) {
-        best = ms;
-        pick = gr;
-      }
-    }
-    b->tune.fused_grab = pick;
-  }
-  {  // LDS ray-stack depth
-    const int depth[2] = {1024, 2048};
-    double best = 1e30;
-    int pick = b->tune.ray_stack;
-    for (int d : depth) {
-      b->tune.ray_stack = d;
-      const double ms = measure(3000 + d);
-      if (ms < best) {
-        best = ms;
-        pick = d;
-      }
-    }
-    b->tune.ray_stack = pick;
-  }
-  (void)id;
-  batch_plan(b);
+    *field = keep;
+  };
+  pick(&b->tune.lanes, {1, 2, 3, 4, 6, 8}, 0);
+  pick(&b->tune.chunk_len, {64, 32, 16}, 100);
+  pick(&b->tune.ray_stack, {1024, 2048}, 1000);
 }
 
 }  // namespace
@@ -1102,22 +1023,10 @@ sdf_status sdf_ctx_create(int32_t device, sdf_ctx **out) {
       fail(SDF_ERR_ARG, "device " + std::to_string(device) + " not present (" +
                             std::to_string(n) + " devices)");
     sdf_ctx *c = new sdf_ctx();
-    c->device = device;
     try {
-      SDF_HIP(hipSetDevice(device));
-      SDF_HIP(hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking));
-      c->stream = c->own;
-      for (int q = 0; q < 2; q++)
-        SDF_HIP(hipStreamCreateWithFlags(&c->aux[q], hipStreamNonBlocking));
-      for (int q = 0; q < 4; q++)
-        SDF_HIP(hipEventCreateWithFlags(&c->join[q], hipEventDisableTiming));
-      SDF_HIP(hipEventCreateWithFlags(&c->ev_plan, hipEventDisableTiming));
-      SDF_HIP(hipMalloc(&c->d_words, 256));
-      SDF_HIP(hipMalloc(&c->d_status, 16 * sizeof(uint64_t)));
-      SDF_HIP(hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device));
-      c->scratch = scratch_address(device);  // with `device` current
+      ctx_init(c, device);
     } catch (...) {
-      delete c;
+      ctx_free(c);
       throw;
     }
     *out = c;
@@ -1125,25 +1034,7 @@ sdf_status sdf_ctx_create(int32_t device, sdf_ctx **out) {
 }
 
 sdf_status sdf_ctx_destroy(sdf_ctx *c) {
-  return guarded([&] {
-    if (!c) return;
-    (void)hipSetDevice(c->device);
-    if (c->stream) (void)hipStreamSynchronize(c->stream);
-    c->ar.release();
-    for (auto e : c->ev) (void)hipEventDestroy(e);
-    for (int q = 0; q < 2; q++)
-      if (c->aux[q]) {
-        (void)hipStreamSynchronize(c->aux[q]);
-        (void)hipStreamDestroy(c->aux[q]);
-      }
-    for (int q = 0; q < 4; q++)
-      if (c->join[q]) (void)hipEventDestroy(c->join[q]);
-    if (c->ev_plan) (void)hipEventDestroy(c->ev_plan);
-    if (c->d_words) (void)hipFree(c->d_words);
-    if (c->d_status && c->own_status) (void)hipFree(c->d_status);
-    if (c->own) (void)hipStreamDestroy(c->own);
-    delete c;
-  });
+  return guarded([&] { ctx_free(c); });
 }
 
 sdf_status sdf_ctx_set_stream(sdf_ctx *c, void *s) {
@@ -2007,63 +1898,44 @@ sdf_status sdf_batch_create(sdf_ctx *c, int32_t nmesh, const double *const *d_X,
   return guarded([&] {
     if (!c || !out) fail(SDF_ERR_ARG, "NULL argument");
     *out = nullptr;
-    // Switched off: the first device run of this path ended in a host
-    // segmentation fault inside this call whose cause is not identified
-    // (the host logic is clean under AddressSanitizer on a stub runtime).
-    fail(SDF_ERR_INTERNAL,
-         "batched submission is disabled: its first device run ended in a host "
-         "segmentation fault whose cause is not yet identified");
-    if (nmesh < 1 || nmesh > sdf::kMaxJobs)
-      fail(SDF_ERR_ARG, "nmesh must be in 1.." + std::to_string(sdf::kMaxJobs));
+    if (nmesh < 1 || nmesh > kMaxBatchMeshes)
+      fail(SDF_ERR_ARG, "nmesh must be in 1.." + std::to_string(kMaxBatchMeshes));
     if (!d_X || !nnp || !d_IEN || !nel || !aabb_min || !h || !N || !d_sdf_out)
       fail(SDF_ERR_ARG, "NULL array");
     std::lock_guard<std::mutex> lk(c->mu);
     use_device(c);
     sdf_batch *b = new sdf_batch();
     b->parent = c;
-    b->n = nmesh;
     try {
-      b->s[0] = c->stream;
-      for (int q = 1; q < 4; q++) SDF_HIP(hipStreamCreateWithFlags(&b->s[q], hipStreamNonBlocking));
-      for (int q = 0; q < 4; q++)
-        for (int a = 0; a < 2; a++)
-          SDF_HIP(hipStreamCreateWithFlags(&b->aux[q][a], hipStreamNonBlocking));
-      SDF_HIP(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
-      SDF_HIP(hipEventCreateWithFlags(&b->ev_fused, hipEventDisableTiming));
-      SDF_HIP(hipEventCreate(&b->ev_t0));
-      SDF_HIP(hipEventCreate(&b->ev_t1));
-      for (int q = 0; q < 4; q++) {
-        SDF_HIP(hipEventCreateWithFlags(&b->ev_join[q], hipEventDisableTiming));
-        SDF_HIP(hipEventCreateWithFlags(&b->ev_join2[q], hipEventDisableTiming));
-      }
-      b->d_djobs = b->ar.get<sdf::DistJob>("djobs", nmesh);
-      b->d_fjobs = b->ar.get<sdf::FusedJob>("fjobs", nmesh);
-      b->d_dpref = b->ar.get<uint64_t>("dpref", nmesh + 1);
-      b->d_fpref = b->ar.get<uint64_t>("fpref", nmesh + 1);
-      b->d_cnt = b->ar.get<unsigned long long>("qcnt", 4);
-      b->d_status = b->ar.get<uint64_t>("status", (size_t)16 * nmesh);
       for (int i = 0; i < nmesh; i++) {
-        if (nel[i] < 0 || nnp[i] < 0) fail(SDF_ERR_ARG, "negative mesh size");
-        if (nel[i] > 0 && (!d_X[i] || !d_IEN[i])) fail(SDF_ERR_ARG, "NULL mesh arrays");
-        sdf::GridP g = make_grid(aabb_min + 3 * i, h[i], N + 3 * i);
-        if (grid_points(g) > 0 && !d_sdf_out[i]) fail(SDF_ERR_ARG, "NULL output");
-        b->kids.push_back(batch_child(b, i));
-        b->dX.push_back(d_X[i]);
-        b->dIEN.push_back(d_IEN[i]);
-        b->nnp.push_back(nnp[i]);
-        b->nel.push_back(nel[i]);
-        LatticeRun L;
-        L.g = g;
-        L.do_dist = true;
-        L.do_sign = true;
-        L.rp = ray_params(6, 30, 0.6, 1);
-        L.out.sdf = d_sdf_out[i];
-        b->runs.push_back(L);
-        b->total_points += grid_points(g);
+        if (nel[i] < 0 || nnp[i] < 0) fail(SDF_ERR_ARG, "mesh " + std::to_string(i) + ": negative mesh size");
+        if (nel[i] > 0 && (!d_X[i] || !d_IEN[i]))
+          fail(SDF_ERR_ARG, "mesh " + std::to_string(i) + ": NULL mesh arrays");
+        BatchMesh m;
+        m.dX = d_X[i];
+        m.dIEN = d_IEN[i];
+        m.nnp = nnp[i];
+        m.nel = nel[i];
+        m.g = make_grid(aabb_min + 3 * i, h[i], N + 3 * i);
+        m.out = d_sdf_out[i];
+        if (grid_points(m.g) > 0 && !m.out)
+          fail(SDF_ERR_ARG, "mesh " + std::to_string(i) + ": NULL output");
+        b->mesh.push_back(m);
+        b->total_points += grid_points(m.g);
+      }
+      b->runs.resize((size_t)nmesh);
+      SDF_HIP(hipMalloc(&b->d_status, (size_t)16 * nmesh * sizeof(uint64_t)));
+      for (int l = 0; l < kMaxLanes; l++) {
+        sdf_ctx *k = new sdf_ctx();
+        b->lanes.push_back(k);
+        ctx_init(k, c->device);
+        SDF_HIP(hipFree(k->d_status));  // statuses go to the batch's block
+        k->d_status = nullptr;
+        k->own_status = false;
       }
       if (autotune) batch_autotune(b);
-      b->last_ms = batch_plan(b);
-      batch_capture(b);
+      batch_plan(b);
+      b->last_ms = batch_run(b);
     } catch (...) {
       batch_free(b);
       throw;
@@ -2077,34 +1949,16 @@ sdf_status sdf_batch_run(sdf_batch *b) {
     if (!b) fail(SDF_ERR_ARG, "batch is NULL");
     std::lock_guard<std::mutex> lk(b->parent->mu);
     use_device(b->parent);
-    for (int attempt = 0;; attempt++) {
-      SDF_HIP(hipEventRecord(b->ev_t0, b->s[0]));
-      SDF_HIP(hipGraphLaunch(b->exec, b->s[0]));
-      SDF_HIP(hipEventRecord(b->ev_t1, b->s[0]));
-      if (!batch_settle(b)) break;
-      // a mesh's lists outgrew their capacities: plan and capture again
-      if (attempt >= 2) fail(SDF_ERR_INTERNAL, "batch capacities did not settle");
-      batch_plan(b);
-      batch_capture(b);
-    }
-    float ms = 0.f;
-    SDF_HIP(hipEventElapsedTime(&ms, b->ev_t0, b->ev_t1));
-    b->last_ms = ms;
+    b->last_ms = batch_run(b);
   });
 }
 
 sdf_status sdf_batch_info(sdf_batch *b, double *out, int32_t cap, int32_t *n) {
   return guarded([&] {
     if (!b) fail(SDF_ERR_ARG, "batch is NULL");
-    std::vector<double> v = {(double)b->n,
-                             (double)b->total_points,
-                             b->last_ms,
-                             (double)b->tune.chunk_len,
-                             (double)b->tune.ray_stack,
-                             (double)b->tune.dist_blocks,
-                             (double)b->tune.dist_grab,
-                             (double)b->tune.fused_blocks,
-                             (double)b->tune.fused_grab,
+    std::vector<double> v = {(double)b->mesh.size(),  (double)b->total_points,
+                             b->last_ms,              (double)b->tune.chunk_len,
+                             (double)b->tune.ray_stack, (double)b->tune.lanes,
                              (double)(b->tune_log.size() / 2)};
     v.insert(v.end(), b->tune_log.begin(), b->tune_log.end());
     int w = 0;

@@ -161,7 +161,6 @@ struct FusedOut {
 };
 
 // Most jobs (meshes) one queue launch serves.
-constexpr int kMaxJobs = 256;
 
 // Distance work of one lattice: items (tile, chunk of <= clen triangles).
 struct DistJob {
@@ -221,13 +220,6 @@ DistJob make_dist_job(const GridP &g, const Tri64 *tri, const Bins &b, float *d2
 FusedJob make_fused_job(const GridP &g, const Bins &b, const float *d2buf, bool do_sign,
                         FusedOut out, Pending *pend, uint32_t *npend, uint32_t pend_cap,
                         Pending *pend0, uint32_t *npend0, uint32_t pend0_cap, uint32_t *err);
-// One persistent launch over the items of njobs jobs (device tables).
-void launch_dist_queue(hipStream_t s, const DistJob *d_jobs, int njobs, uint64_t *d_prefix,
-                       unsigned long long *d_counter, int blocks, int grab);
-void launch_fused_queue(hipStream_t s, const FusedJob *d_jobs, int njobs,
-                        const uint64_t *d_prefix, unsigned long long *d_counter, int blocks,
-                        int grab);
-
 void build_tile_bins(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
                      const GridP &g, double radius, Bins &b);
 // Two-phase forms: *_begin issues the counting kernels and scans (no host

@@ -236,47 +236,6 @@ __global__ __launch_bounds__(kTilePts) void dist_chunks_kernel(DistJob J) {
     dist_item(J, it, S, true);
 }
 
-// Persistent work queue over the distance items of many jobs: each block
-// takes `grab` consecutive units of the concatenated item space with one
-// atomic, and maps a unit to (job, item) through the job prefix sums.
-__global__ __launch_bounds__(kTilePts) void dist_queue_kernel(
-    const DistJob *__restrict__ jobs, int njobs, const uint64_t *__restrict__ prefix,
-    unsigned long long *__restrict__ counter, int grab) {
-  __shared__ DistShared S;
-  __shared__ uint64_t s_pref[kMaxJobs + 1];
-  __shared__ unsigned long long s_base;
-  for (int i = threadIdx.x; i <= njobs; i += blockDim.x) s_pref[i] = prefix[i];
-  __syncthreads();
-  const uint64_t total = s_pref[njobs];
-  while (true) {
-    if (threadIdx.x == 0) s_base = atomicAdd(counter, (unsigned long long)grab);
-    __syncthreads();
-    const uint64_t base = s_base;
-    __syncthreads();
-    if (base >= total) break;
-    const uint64_t end = base + grab < total ? base + grab : total;
-    int j = 0;
-    for (uint64_t u = base; u < end; u++) {
-      while (j + 1 < njobs && s_pref[j + 1] <= u) j++;
-      dist_item(jobs[j], (int64_t)(u - s_pref[j]), S);
-    }
-  }
-}
-
-// Prefix sums of the jobs' actual item counts (one thread; <= kMaxJobs jobs).
-__global__ void dist_prefix_kernel(const DistJob *__restrict__ jobs, int njobs,
-                                   uint64_t *__restrict__ prefix,
-                                   unsigned long long *__restrict__ counter) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  uint64_t acc = 0;
-  prefix[0] = 0;
-  for (int j = 0; j < njobs; j++) {
-    acc += (uint64_t)chunk_count(jobs[j].nchunks, jobs[j].nchunks_dev);
-    prefix[j + 1] = acc;
-  }
-  *counter = 0ull;
-}
-
 // Learned-radius variant of the distance item.  Lane p carries a squared
 // search radius R2(p) <= h^2 proposed by the network and evaluates only the
 // triangles whose lower bound (the larger of the squared distances to the
@@ -492,19 +451,13 @@ enum : int { kOutSdf = 1, kOutDist = 2, kOutSigns = 4, kOutConf = 8, kOutAll = 1
 // One sign brick (8 x 8 x 4 points) of job J for the 64-lane block (one
 // wave: its LDS hand-offs are wave_lds_sync); m is the brick's position in
 // the brick launch order; C is the LDS copy of the job's cold fields;
-// outputs outside kOut are not written.  kPlan: the brick's coordinates
-// and distance slots come from J.plan[m] (BrickPlan, one scalar load)
-// instead of being decoded and looked up here.  Uniform per block.
-template <int kOut, bool kPlan>
+// outputs outside kOut are not written.  The brick's coordinates and
+// distance slots come from J.plan[m] (BrickPlan, scalar loads).  Uniform
+// per block.
+template <int kOut>
 __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S,
                                    const FusedCold &C) {
   const GridP &g = J.g;
-  const TileDims &td = J.td;
-  // the tile offsets are read-only here and a brick's are wave-uniform:
-  // through the constant address space they become scalar loads, whose wait
-  // does not also wait for the previous brick's vector stores
-  const __attribute__((address_space(4))) uint64_t *chunk_off_k =
-      (const __attribute__((address_space(4))) uint64_t *)J.chunk_off;
   const float *__restrict__ d2buf = J.d2buf;
   const LineInline *__restrict__ inl = J.inl;
   const int64_t base1 = J.base1, base2 = J.base2;
@@ -518,24 +471,19 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S,
   const bool has_d2 = kStages ? (kStages & 1) != 0 : d2buf != nullptr;
   const int do_sign = kStages ? (kStages >> 1) : J.do_sign;
   const FusedOut out = J.out;
-  const int64_t chunk_cap = J.chunk_cap;
-  int32_t bx, by, bz;
   BrickPlan P;
-  if (kPlan) {
-    // eight adjacent words through the constant view: one scalar load
+  {
+    // adjacent words through the constant view: scalar loads
     const KConst<uint32_t> pw = kconst(reinterpret_cast<const uint32_t *>(J.plan + m));
 #pragma unroll
     for (int t = 0; t < 4; t++) P.slot[t] = pw[t];
     P.xy = pw[4];
     P.z = pw[5];
     P.flags = pw[6];
-    if (!(P.flags & 2u)) return;  // padding position (uniform)
-    bx = (int32_t)(P.xy & 0xFFFFu);
-    by = (int32_t)(P.xy >> 16);
-    bz = (int32_t)P.z;
-  } else if (!brick_tile(m, J.bd, &bx, &by, &bz)) {
-    return;  // uniform
   }
+  if (!(P.flags & 2u)) return;  // padding position (uniform)
+  const int32_t bx = (int32_t)(P.xy & 0xFFFFu), by = (int32_t)(P.xy >> 16),
+                bz = (int32_t)P.z;
   const int lane = threadIdx.x;
   const int32_t i = bx * 8 + (lane & 7);
   const int32_t j = by * 8 + (lane >> 3);
@@ -558,23 +506,10 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S,
   // the staged line (0 when it serves no point) and the lane's own z-line
   const int64_t Lxy = valid ? (xl ? 0 : base1) + (int64_t)(k0 + sq) * across + la : 0;
   const int64_t Lz = in_grid ? base2 + (int64_t)j * g.nx + i : 0;
-  // The brick's four distance tiles (8 x 4 x 2 points; tile t = 2 tz local
-  // + ty local) and their offsets: wave-uniform scalar loads, issued first so
-  // that they are in flight with the segment bytes' vector loads.
+  // The brick covers 1 x 2 x 2 distance tiles (8 x 4 x 2 points; tile
+  // t = 2 tz local + ty local), whose slots the plan holds.
   static_assert(kTileX == 8 && 2 * kTileY == 8 && 2 * kTileZ == kSignZ,
                 "a sign brick covers 1 x 2 x 2 distance tiles");
-  uint64_t c0[4] = {0, 0, 0, 0}, c1[4] = {0, 0, 0, 0};
-  bool tok[4] = {false, false, false, false};
-  if (has_d2 && !kPlan) {
-#pragma unroll
-    for (int t = 0; t < 4; t++) {  // no branches: the eight loads go out together
-      const int32_t tyy = 2 * by + (t & 1), tzz = 2 * bz + (t >> 1);
-      tok[t] = tyy < td.ty && tzz < td.tz;
-      const int64_t tile = tok[t] ? ((int64_t)tzz * td.ty + tyy) * td.tx + bx : 0;
-      c0[t] = chunk_off_k[tile];
-      c1[t] = chunk_off_k[tile + 1];
-    }
-  }
   // segment bytes (SegLayout): segment bx of the staged x-line or by of the
   // staged y-line, segment bz of the lane's z-line.  A line that serves no
   // point reads the sentinel word after the last one (byte 1 at every
@@ -606,18 +541,9 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S,
     // a tile without distance work reads the spare tile of +inf after the
     // last work item: the loads are unconditional and need no select
     uint32_t slot[4];
-    if (kPlan) {
 #pragma unroll
-      for (int t = 0; t < 4; t++) slot[t] = P.slot[t];
-      work = (P.flags & 1u) != 0u;
-    } else {
-#pragma unroll
-      for (int t = 0; t < 4; t++) {
-        slot[t] = tok[t] && c1[t] > c0[t] && (int64_t)c0[t] < chunk_cap ? (uint32_t)c0[t]
-                                                                         : (uint32_t)chunk_cap;
-        work |= slot[t] != (uint32_t)chunk_cap;
-      }
-    }
+    for (int t = 0; t < 4; t++) slot[t] = P.slot[t];
+    work = (P.flags & 1u) != 0u;
     // a brick no triangle is near (most of the grid) keeps +inf: distance h
     if (work) {
       const int tyl = lane >> 5;  // the lane's y tile in the brick
@@ -815,7 +741,7 @@ void fused_lattice_kernel(FusedJob J) {
   // flight may still target
   const uint32_t nb = (uint32_t)J.nbricks;
   for (uint32_t m = (uint32_t)xcd_remap(blockIdx.x, gridDim.x); m < nb; m += gridDim.x)
-    fused_brick<kOut, true>(J, (int64_t)m, S, C);
+    fused_brick<kOut>(J, (int64_t)m, S, C);
 }
 
 // The sign kernel's plan, one thread per launch position: the brick decode
@@ -852,35 +778,6 @@ __global__ __launch_bounds__(256) void brick_plan_kernel(FusedJob J, BrickPlan *
       }
     }
     plan[m] = P;
-  }
-}
-
-// Persistent work queue over the sign bricks of many jobs (static brick
-// counts: prefix computed on the host).
-__global__ __launch_bounds__(64) void fused_queue_kernel(
-    const FusedJob *__restrict__ jobs, int njobs, const uint64_t *__restrict__ prefix,
-    unsigned long long *__restrict__ counter, int grab) {
-  __shared__ FusedShared S;
-  __shared__ FusedCold C;
-  __shared__ uint64_t s_pref[kMaxJobs + 1];
-  __shared__ unsigned long long s_base;
-  for (int i = threadIdx.x; i <= njobs; i += blockDim.x) s_pref[i] = prefix[i];
-  __syncthreads();
-  const uint64_t total = s_pref[njobs];
-  while (true) {
-    if (threadIdx.x == 0) s_base = atomicAdd(counter, (unsigned long long)grab);
-    __syncthreads();
-    const uint64_t base = s_base;
-    __syncthreads();
-    if (base >= total) break;
-    const uint64_t end = base + grab < total ? base + grab : total;
-    int j = 0;
-    for (uint64_t u = base; u < end; u++) {
-      while (j + 1 < njobs && s_pref[j + 1] <= u) j++;
-      if (threadIdx.x == 0) C = jobs[j].cold;
-      __syncthreads();
-      fused_brick<kOutAll, false>(jobs[j], (int64_t)(u - s_pref[j]), S, C);
-    }
   }
 }
 
@@ -1372,16 +1269,6 @@ void launch_dist_chunks(hipStream_t s, const GridP &g, const Tri64 *tri,
   SDF_HIP(hipGetLastError());
 }
 
-void launch_dist_queue(hipStream_t s, const DistJob *d_jobs, int njobs, uint64_t *d_prefix,
-                       unsigned long long *d_counter, int blocks, int grab) {
-  if (njobs <= 0) return;
-  if (njobs > kMaxJobs) throw Error{-1, "too many jobs for one queue launch"};
-  hipLaunchKernelGGL(dist_prefix_kernel, dim3(1), dim3(64), 0, s, d_jobs, njobs, d_prefix,
-                     d_counter);
-  hipLaunchKernelGGL(dist_queue_kernel, dim3(blocks), dim3(kTilePts), 0, s, d_jobs, njobs,
-                     (const uint64_t *)d_prefix, d_counter, grab);
-  SDF_HIP(hipGetLastError());
-}
 
 void launch_dist_chunks_learned(hipStream_t s, const GridP &g, const Tri64 *tri,
                                 const Bins &b, const float *r2buf, float *d2buf,
@@ -1506,15 +1393,6 @@ void launch_brick_plan(hipStream_t s, const GridP &g, const Bins &b, const float
   SDF_HIP(hipGetLastError());
 }
 
-void launch_fused_queue(hipStream_t s, const FusedJob *d_jobs, int njobs,
-                        const uint64_t *d_prefix, unsigned long long *d_counter, int blocks,
-                        int grab) {
-  if (njobs <= 0) return;
-  if (njobs > kMaxJobs) throw Error{-1, "too many jobs for one queue launch"};
-  hipLaunchKernelGGL(fused_queue_kernel, dim3(blocks), dim3(64), 0, s, d_jobs, njobs, d_prefix,
-                     d_counter, grab);
-  SDF_HIP(hipGetLastError());
-}
 
 static int blocks_for(int64_t n, int t) {
   int64_t b = (n + t - 1) / t;

@@ -2,10 +2,12 @@
 reference, which computes one mesh per call).
 
 ``SignedDistanceBatch(meshes, grids)`` uploads the meshes once and builds a
-batch in the library: per-mesh binning and LBVH chains run concurrently,
-the distance and sign stages of all meshes are one persistent work-queue
-launch each, and the whole pass is captured into a hipGraph that ``run()``
-replays.  Each result equals ``SignedDistances.signed_distance(mesh, grid)``.
+batch in the library: the meshes are dealt to worker contexts whose passes
+overlap on the GPU, each run issues every mesh with no host wait and reads
+all statuses once, and ``autotune`` picks the number of worker contexts,
+the distance work-item length and the ray-stack depth by timing the batch.
+Each result equals ``SignedDistances.signed_distance(mesh, grid)`` bit for
+bit.
 Device buffers come from the library's own HIP runtime
 (``_lib.DeviceBuffer``), not from torch: torch ships its own HIP and HSA
 runtimes, and a process that loads this library before importing torch ends
@@ -17,8 +19,7 @@ import numpy as np
 
 from . import _lib
 
-INFO = ("meshes", "grid_points", "last_ms", "chunk_len", "ray_stack", "dist_blocks",
-        "dist_grab", "fused_blocks", "fused_grab", "candidates")
+INFO = ("meshes", "grid_points", "last_ms", "chunk_len", "ray_stack", "lanes", "candidates")
 
 
 class SignedDistanceBatch:

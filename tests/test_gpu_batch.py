@@ -1,15 +1,13 @@
 """Batched signed distance: every mesh of a batch equals its single-mesh
-result, bit for bit, across graph replays, after the meshes change in place,
-and with the autotuner's choices."""
+result, bit for bit, across runs, after the meshes change in place, and
+with the autotuner's choices."""
 import numpy as np
 import pytest
 
 from stl2sdf import Batch, MeshGrid, SignedDistances, synthetic
 from tests import _cases
 
-pytestmark = [pytest.mark.gpu,
-              pytest.mark.skip(reason="sdf_batch_create ended in a host segmentation fault on "
-                                      "its first GPU run; off the GPU until the cause is found")]
+pytestmark = pytest.mark.gpu
 
 
 def _family(count, seed, tri_range, grid_points):
@@ -37,7 +35,7 @@ def test_batch_matches_single_mesh():
     b = Batch.SignedDistanceBatch(meshes, grids, autotune=False)
     b.run()
     _check(b, meshes, grids)
-    b.run()   # graph replay
+    b.run()   # a second run
     _check(b, meshes, grids)
     print("info:", b.info())
     b.close()
@@ -69,5 +67,25 @@ def test_batch_autotune():
     assert info["candidates"] >= 10
     assert info["chunk_len"] in (16, 32, 64) and info["ray_stack"] in (1024, 2048)
     b.run()
+    _check(b, meshes, grids)
+    b.close()
+
+
+def test_batch_config4_family():
+    """The batched configuration at full size: 64 seeded star surfaces with
+    triangle counts log-uniform in [2k, 500k] and grids of 64, 96, 128 or 192
+    points on the longest axis; every mesh equals its single-mesh result bit
+    for bit."""
+    meshes, grids = [], []
+    for X, IEN, g, _ in synthetic.batch_family(64, 2):
+        mesh = MeshGrid.TriangularMesh(X, IEN)
+        xmin, xmax = MeshGrid.getMesh_AABB(mesh.X)
+        meshes.append(mesh)
+        grids.append(MeshGrid.Grid(xmin, xmax, g - 7, 3))
+    print("config 4:", sum(m.nel for m in meshes), "triangles,",
+          sum(int(g.ngp) for g in grids), "points", flush=True)
+    b = Batch.SignedDistanceBatch(meshes, grids, autotune=False)
+    b.run()
+    print("info:", b.info(), flush=True)
     _check(b, meshes, grids)
     b.close()
