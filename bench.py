@@ -214,6 +214,7 @@ def batch_bench(L, ctx, torch, runs=5):
             "meshes": len(meshes), "triangles": int(sum(m.nel for m in meshes)),
             "points": pts, "sequential_ms": t_seq * 1e3, "vs_sequential": t_seq / t_batch,
             "identical_to_single_mesh": identical,
+            "issue_ms": info["issue_ms"],
             "tune": {k: info[k] for k in ("lanes", "chunk_len", "ray_stack")},
             "candidates": info["tuning"]}
 

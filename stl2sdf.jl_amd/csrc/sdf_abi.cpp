@@ -889,6 +889,7 @@ struct sdf_batch {
   std::vector<double> tune_log;  // (candidate id, ms) pairs
   int64_t total_points = 0;
   double last_ms = 0.0;
+  double last_issue_ms = 0.0;  // host time to issue the last run's passes
 };
 
 namespace {
@@ -953,6 +954,8 @@ double batch_run(sdf_batch *b) {
   const auto t0 = std::chrono::steady_clock::now();
   const int n = (int)b->mesh.size();
   for (int i = 0; i < n; i++) batch_issue(b, i);
+  b->last_issue_ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   for (int l = 0; l < b->tune.lanes && l < n; l++)
     SDF_HIP(hipStreamSynchronize(b->lanes[(size_t)l]->stream));
   std::vector<uint64_t> st((size_t)16 * n);
@@ -1959,7 +1962,7 @@ sdf_status sdf_batch_info(sdf_batch *b, double *out, int32_t cap, int32_t *n) {
     std::vector<double> v = {(double)b->mesh.size(),  (double)b->total_points,
                              b->last_ms,              (double)b->tune.chunk_len,
                              (double)b->tune.ray_stack, (double)b->tune.lanes,
-                             (double)(b->tune_log.size() / 2)};
+                             b->last_issue_ms, (double)(b->tune_log.size() / 2)};
     v.insert(v.end(), b->tune_log.begin(), b->tune_log.end());
     int w = 0;
     for (; w < (int)v.size() && w < cap && out; w++) out[w] = v[w];

@@ -19,7 +19,8 @@ import numpy as np
 
 from . import _lib
 
-INFO = ("meshes", "grid_points", "last_ms", "chunk_len", "ray_stack", "lanes", "candidates")
+INFO = ("meshes", "grid_points", "last_ms", "chunk_len", "ray_stack", "lanes", "issue_ms",
+        "candidates")
 
 
 class SignedDistanceBatch:
