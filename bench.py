@@ -370,7 +370,8 @@ def main():
                    "cell_size": h, "tune": args.tune},
         "stages_ms": stage_ms,
         "counters": {"pending_points": cnt[0], "tile_pairs": cnt[1], "line_hits": cnt[2],
-                     "device_bytes": cnt[3]},
+                     "device_bytes": cnt[3], "vote_second_phase": cnt[4],
+                     "winding_points": cnt[5], "margin_points": cnt[6]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "kernel": "fused_lattice_kernel", "alg_bytes_per_launch": alg_bytes,

@@ -81,7 +81,10 @@ sdf_status sdf_ctx_stage_times(sdf_ctx *ctx, char *names, int32_t names_cap,
 /* Counters of the last call: [0] points handed to the adaptive vote (left
  * undecided by the six axis rays, or decided inside a rounding margin),
  * [1] triangles listed in distance tiles, [2] axis-line hits,
- * [3] device bytes held by the context. */
+ * [3] device bytes held by the context, [4] vote points still open after
+ * the first vote phase (10 rays), [5] points settled by the winding
+ * number, [6] of the points of [0], those decided inside a rounding margin
+ * (they vote from ray 1). */
 sdf_status sdf_ctx_counters(sdf_ctx *ctx, int64_t *out, int32_t cap,
                             int32_t *n);
 

@@ -137,6 +137,8 @@ struct sdf_ctx {
   // [0] pass start, [1] line hits done, [2] LBVH done, [3] tile chain done
   hipEvent_t join[4] = {nullptr, nullptr, nullptr, nullptr};
   hipEvent_t ev_plan = nullptr;  // the sign kernel's brick plan is written
+  // the vote's last phase: its item list is known, its winding sums are done
+  hipEvent_t ev_wind[2] = {nullptr, nullptr};
   std::mutex mu;
   sdf::Arena ar;
   void *scratch = nullptr;  // this device's static scratch array
@@ -179,7 +181,7 @@ struct sdf_ctx {
   std::vector<hipEvent_t> ev;
   std::vector<std::string> ev_name;
   int nev = 0;
-  int64_t counters[4] = {0, 0, 0, 0};
+  int64_t counters[7] = {0, 0, 0, 0, 0, 0, 0};
   // the pending lists of the last lattice call (points handed to the vote)
   const sdf::Pending *last_pend[2] = {nullptr, nullptr};
   uint64_t last_npend[2] = {0, 0};
@@ -317,6 +319,10 @@ void bvh_async(sdf_ctx *c, int q, const sdf::GridP &g, bool full, int after = 3)
 // cannot finish to the next, so later (and longer) phases trace rays only
 // for the survivors.  cnt: 4 device words per phase (at most 12); winding
 // counts land in word 2 of each phase.
+// Slots of the last vote phase whose winding sums are computed ahead (rows
+// of at most 1024 chunk sums: at most 64 MB).
+constexpr uint32_t kWindAheadSlots = 8192;
+
 void resolve_items(sdf_ctx *c, const sdf::GridP &g, const double *d_pts,
                    const sdf::RayParams &rp, const sdf::PendLists &first, sdf::FusedOut out,
                    uint32_t *cnt) {
@@ -342,9 +348,26 @@ void resolve_items(sdf_ctx *c, const sdf::GridP &g, const double *d_pts,
   for (int p = 0; p < np; p++) {
     const bool last = p + 1 == np;
     sdf::Pending *next = last ? nullptr : c->ar.get<sdf::Pending>(names[p], cap);
+    // The last phase's items are the few the earlier phases left open: the
+    // winding sums of those in its first kWindAheadSlots slots (the fallback
+    // some of them need after the last rays) are computed on helper stream 0
+    // beside that phase's rays; the vote's winding pass takes the rest.
+    double *ahead = nullptr;
+    const uint32_t rows = cap < kWindAheadSlots ? cap : kWindAheadSlots;
+    if (last && p > 0 && rp.use_winding && c->caps.wind > 0) {
+      int64_t chunk = 0;
+      const int chunks = sdf::winding_chunks(c->nel, &chunk);
+      ahead = c->ar.get<double>("res_wahead", (size_t)rows * chunks);
+      SDF_HIP(hipEventRecord(c->ev_wind[0], c->stream));
+      SDF_HIP(hipStreamWaitEvent(c->aux[0], c->ev_wind[0], 0));
+      sdf::launch_winding_ahead(c->aux[0], g, d_pts, c->tri, c->nel, cur, rows,
+                                cnt + 4 * (p - 1) + 1, ahead);
+      SDF_HIP(hipEventRecord(c->ev_wind[1], c->aux[0]));
+    }
     sdf::resolve_pending(c->stream, c->ar, g, d_pts, c->tri, c->nel, bvh, rp, cur, ph[p],
                          last ? 1 : 0, out, next, nullptr, cap, cnt + 4 * p, c->caps.wind,
-                         c->d_words, c->ray_stack);
+                         c->d_words, c->ray_stack, ahead, ahead ? rows : 0,
+                         ahead ? c->ev_wind[1] : nullptr);
     cur = sdf::PendLists();
     cur.add(next, cnt + 4 * p + 1, cap, ph[p]);
   }
@@ -402,7 +425,7 @@ struct LatticeRun {
 
 enum : int {
   kStErr = 0, kStPend, kStPend0, kStPairs, kStHits, kStChunks, kStWindA1, kStWindA2,
-  kStWindB1, kStWindB2, kStMeshErr, kStCount
+  kStWindB1, kStWindB2, kStMeshErr, kStNext0, kStCount
 };
 
 // Stage 1: the build chains.  Tile bins and distance work items run on the
@@ -573,6 +596,7 @@ void stage_resolve(sdf_ctx *c, LatticeRun &L) {
   src.set(kStHits, L.bins.nhits_dev, true);
   src.set(kStChunks, L.bins.nchunks_dev, true);
   if (L.do_sign) {
+    src.set(kStNext0, cnt + 1, false);
     src.set(kStWindA1, cnt + 2, false);
     src.set(kStWindA2, cnt + 6, false);
     src.set(kStWindB1, cnt + 10, false);
@@ -661,6 +685,11 @@ void run_lattice(sdf_ctx *c, const sdf::GridP &g, bool do_dist, bool do_sign,
       c->counters[1] = (int64_t)st[kStPairs];
       c->counters[2] = (int64_t)st[kStHits];
       c->counters[3] = (int64_t)c->ar.bytes();
+      c->counters[4] = L.do_sign ? (int64_t)st[kStNext0] : 0;
+      c->counters[5] = L.do_sign ? (int64_t)(st[kStWindA1] + st[kStWindA2] + st[kStWindB1] +
+                                             st[kStWindB2])
+                                 : 0;
+      c->counters[6] = L.do_sign ? (int64_t)st[kStPend0] : 0;
       c->lstat_pairs = (int64_t)st[kStPairs];
       return;
     }
@@ -823,6 +852,7 @@ void ctx_init(sdf_ctx *c, int32_t device) {
   for (int q = 0; q < 2; q++) SDF_HIP(hipStreamCreateWithFlags(&c->aux[q], hipStreamNonBlocking));
   for (int q = 0; q < 4; q++) SDF_HIP(hipEventCreateWithFlags(&c->join[q], hipEventDisableTiming));
   SDF_HIP(hipEventCreateWithFlags(&c->ev_plan, hipEventDisableTiming));
+  for (int q = 0; q < 2; q++) SDF_HIP(hipEventCreateWithFlags(&c->ev_wind[q], hipEventDisableTiming));
   SDF_HIP(hipMalloc(&c->d_words, 256));
   SDF_HIP(hipMalloc(&c->d_status, 16 * sizeof(uint64_t)));
   SDF_HIP(hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device));
@@ -843,6 +873,8 @@ void ctx_free(sdf_ctx *c) {
   for (int q = 0; q < 4; q++)
     if (c->join[q]) (void)hipEventDestroy(c->join[q]);
   if (c->ev_plan) (void)hipEventDestroy(c->ev_plan);
+  for (int q = 0; q < 2; q++)
+    if (c->ev_wind[q]) (void)hipEventDestroy(c->ev_wind[q]);
   if (c->d_words) (void)hipFree(c->d_words);
   if (c->d_status && c->own_status) (void)hipFree(c->d_status);
   if (c->own) (void)hipStreamDestroy(c->own);
@@ -1120,7 +1152,7 @@ sdf_status sdf_ctx_counters(sdf_ctx *c, int64_t *out, int32_t cap, int32_t *n) {
   return guarded([&] {
     if (!c) fail(SDF_ERR_ARG, "ctx is NULL");
     int w = 0;
-    for (int i = 0; i < 4 && i < cap; i++, w++) out[i] = c->counters[i];
+    for (int i = 0; i < 7 && i < cap; i++, w++) out[i] = c->counters[i];
     if (n) *n = w;
   });
 }

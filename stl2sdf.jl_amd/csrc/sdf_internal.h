@@ -340,7 +340,19 @@ void resolve_pending(hipStream_t s, Arena &ar, const GridP &g, const double *pts
                      const Tri64 *tri, int64_t nel, const Bvh &bvh, const RayParams &rp,
                      const PendLists &lists, int r1, int final_phase, FusedOut out,
                      Pending *next, uint32_t *nnext, uint32_t next_cap, uint32_t *d_counts,
-                     uint32_t wind_cap, uint32_t *err, int stack = 1024);
+                     uint32_t wind_cap, uint32_t *err, int stack = 1024,
+                     const double *wind_ahead = nullptr, uint64_t ahead_rows = 0,
+                     hipEvent_t wind_ahead_done = nullptr);
+// Chunks of the winding-number sums over nel triangles (chunk: triangles
+// per chunk).
+int winding_chunks(int64_t nel, int64_t *chunk);
+// The winding-number partial sums of the items of `lists` (one list, the
+// count on the device at *count_dev) with a slot below items_cap, by slot:
+// partial[slot * chunks + c], ahead of the final vote that needs some of
+// them (resolve_pending's wind_ahead, ahead_rows = items_cap).
+void launch_winding_ahead(hipStream_t s, const GridP &g, const double *pts, const Tri64 *tri,
+                          int64_t nel, const PendLists &lists, uint32_t items_cap,
+                          const uint32_t *count_dev, double *partial);
 
 // xp of evalDistancesOnTriMesh: exact FP64 closest point on the mesh for
 // every lattice point (pts == nullptr) or caller point, 3 doubles per point.

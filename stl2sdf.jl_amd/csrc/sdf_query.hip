@@ -1130,7 +1130,7 @@ __global__ __launch_bounds__(256) void winding_partial_kernel(
     GridP g, const double *__restrict__ pts, const Tri64 *__restrict__ tri,
     int64_t nel, PendLists lists,
     const uint64_t *__restrict__ wind, uint32_t nwind, const uint32_t *__restrict__ nwind_dev,
-    int chunks, int64_t chunk, double *__restrict__ partial) {
+    int chunks, int64_t chunk, double *__restrict__ partial, uint64_t skip_below) {
 #pragma clang fp contract(off)
   __shared__ double s_red[256];
   if (nwind_dev && *nwind_dev < nwind) nwind = *nwind_dev;
@@ -1138,7 +1138,9 @@ __global__ __launch_bounds__(256) void winding_partial_kernel(
   for (int64_t q = blockIdx.x; q < nblk; q += gridDim.x) {
     const uint32_t w = (uint32_t)(q / chunks);
     const int64_t c = q % chunks;
-    const Pending P = slot_item(lists, wind[w]);
+    const uint64_t slot = wind ? wind[w] : (uint64_t)w;
+    if (slot < skip_below) continue;  // computed ahead (uniform per block)
+    const Pending P = slot_item(lists, slot);
     double o[3];
     item_point(g, pts, P.idx, o);
     const int64_t e0 = c * chunk, e1 = min(nel, e0 + chunk);
@@ -1180,12 +1182,18 @@ __global__ __launch_bounds__(64) void winding_final_kernel(PendLists lists,
                                                            const uint32_t *__restrict__ nwind_dev,
                                                            int chunks,
                                                            const double *__restrict__ partial,
-                                                           FusedOut out) {
+                                                           const double *__restrict__ ahead,
+                                                           uint64_t ahead_rows, FusedOut out) {
 #pragma clang fp contract(off)
   if (nwind_dev && *nwind_dev < nwind) nwind = *nwind_dev;
   for (uint32_t w = blockIdx.x; w < nwind; w += gridDim.x) {
+    // the partial sums of an item with a slot below ahead_rows were computed
+    // ahead, by slot; the others by position in the winding list
+    const uint64_t slot = wind[w];
+    const double *row = slot < ahead_rows ? ahead + (int64_t)slot * chunks
+                                          : partial + (int64_t)w * chunks;
     double tot = 0.0;
-    for (int c = threadIdx.x; c < chunks; c += 64) tot += partial[(int64_t)w * chunks + c];
+    for (int c = threadIdx.x; c < chunks; c += 64) tot += row[c];
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
     if (threadIdx.x == 0) {
@@ -1407,7 +1415,8 @@ void resolve_pending(hipStream_t s, Arena &ar, const GridP &g, const double *pts
                      const Tri64 *tri, int64_t nel, const Bvh &bvh, const RayParams &rp,
                      const PendLists &lists, int r1, int final_phase, FusedOut out,
                      Pending *next, uint32_t *nnext, uint32_t next_cap, uint32_t *d_counts,
-                     uint32_t wind_cap, uint32_t *err, int stack) {
+                     uint32_t wind_cap, uint32_t *err, int stack, const double *wind_ahead,
+                     uint64_t ahead_rows, hipEvent_t wind_ahead_done) {
   // d_counts: [0] fallback rays, [1] next items (unless nnext), [2] winding items
   const uint64_t slots = lists.slots();
   if (slots == 0) return;
@@ -1441,23 +1450,47 @@ void resolve_pending(hipStream_t s, Arena &ar, const GridP &g, const double *pts
   // a non-final phase decides only items with confidence >= 0.85, which
   // never need the winding fallback
   if (final_phase && rp.use_winding && wind_cap > 0) {
-    int64_t chunk = 4096;
-    int chunks = (int)((nel + chunk - 1) / chunk);
-    if (chunks < 1) chunks = 1;
-    if (chunks > 1024) {
-      chunks = 1024;
-      chunk = (nel + chunks - 1) / chunks;
-    }
+    int64_t chunk = 0;
+    const int chunks = winding_chunks(nel, &chunk);
+    if (!wind_ahead) ahead_rows = 0;
+    // the items whose sums were not computed ahead (slot >= ahead_rows)
     double *partial = ar.get<double>(ar.tmp("res_wpartial"), (size_t)wind_cap * chunks);
     hipLaunchKernelGGL(winding_partial_kernel, dim3(1024), dim3(256), 0, s, g, pts, tri, nel,
                        lists, (const uint64_t *)wind, wind_cap,
-                       (const uint32_t *)(d_counts + 2), chunks, chunk, partial);
+                       (const uint32_t *)(d_counts + 2), chunks, chunk, partial, ahead_rows);
+    // the others: computed beside this phase's rays
+    if (wind_ahead) SDF_HIP(hipStreamWaitEvent(s, wind_ahead_done, 0));
     hipLaunchKernelGGL(winding_final_kernel, dim3(wind_cap < 1024u ? wind_cap : 1024u), dim3(64),
                        0, s, lists,
                        (const uint64_t *)wind, wind_cap, (const uint32_t *)(d_counts + 2),
-                       chunks, (const double *)partial, out);
+                       chunks, (const double *)partial, wind_ahead, ahead_rows, out);
     SDF_HIP(hipGetLastError());
   }
+}
+
+int winding_chunks(int64_t nel, int64_t *chunk) {
+  int64_t ch = 4096;
+  int chunks = (int)((nel + ch - 1) / ch);
+  if (chunks < 1) chunks = 1;
+  if (chunks > 1024) {
+    chunks = 1024;
+    ch = (nel + chunks - 1) / chunks;
+  }
+  *chunk = ch;
+  return chunks;
+}
+
+void launch_winding_ahead(hipStream_t s, const GridP &g, const double *pts, const Tri64 *tri,
+                          int64_t nel, const PendLists &lists, uint32_t items_cap,
+                          const uint32_t *count_dev, double *partial) {
+  int64_t chunk = 0;
+  const int chunks = winding_chunks(nel, &chunk);
+  // a quarter of the post-vote pass's grid: it runs beside the last
+  // phase's rays, which it should not crowd
+  hipLaunchKernelGGL(winding_partial_kernel, dim3(256), dim3(256), 0, s, g, pts, tri, nel,
+                     lists, (const uint64_t *)nullptr, items_cap, count_dev, chunks, chunk,
+                     partial, (uint64_t)0);
+  SDF_HIP(hipGetLastError());
 }
 
 void launch_dist_points_bvh(hipStream_t s, const double *pts, int64_t npts,
