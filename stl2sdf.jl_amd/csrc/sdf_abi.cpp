@@ -162,7 +162,6 @@ struct sdf_ctx {
   } caps;
   uint64_t *d_status = nullptr;  // sizes of the last lattice call (16 words)
   bool own_status = true;        // d_status allocated by this context
-  bool capturing = false;        // inside stream capture: no host waits allowed
   // tunables: triangles per distance work item, LDS ray-stack depth,
   // workgroups per CU of the distance and sign kernels (0: one per item).
   // The distance kernel's LDS allows 16 workgroups per CU; at 8 the line
@@ -444,9 +443,6 @@ void stage_bins_tiles(sdf_ctx *c, LatticeRun &L) {
   const int64_t npts = grid_points(g);
   sdf::Bins &bins = L.bins;
   bins = sdf::Bins();
-  if (c->capturing && ((L.do_dist && (c->caps.pairs < 0 || c->caps.chunks < 0)) ||
-                       (L.do_sign && c->caps.hits < 0)))
-    fail(SDF_ERR_INTERNAL, "capture needs known capacities");
   SDF_HIP(hipEventRecord(c->join[0], c->stream));
   L.bvh_was_ready = c->bvh_ready;
   if (c->lines_early == 1) stage_bins_lines(c, L, 3);
