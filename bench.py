@@ -215,7 +215,7 @@ def batch_bench(L, ctx, torch, runs=5):
             "points": pts, "sequential_ms": t_seq * 1e3, "vs_sequential": t_seq / t_batch,
             "identical_to_single_mesh": identical,
             "issue_ms": info["issue_ms"],
-            "tune": {k: info[k] for k in ("lanes", "chunk_len", "ray_stack")},
+            "tune": {k: info[k] for k in ("lanes", "chunk_len", "ray_stack", "threads")},
             "candidates": info["tuning"]}
 
 

@@ -347,9 +347,10 @@ sdf_status sdf_batch_create(sdf_ctx *ctx, int32_t nmesh, const double *const *d_
 sdf_status sdf_batch_run(sdf_batch *batch);
 /* meshes, grid points, last run ms (wall time of issue, passes and status
  * read), then the chosen tunables (work-item triangles, ray-stack depth,
- * lanes), the last run's host issue ms, the number of timed candidates and
+ * lanes), the last run's host issue ms, whether each lane has its own
+ * issuing host thread (1) or not (0), the number of timed candidates and
  * their (id, ms) pairs (id: lanes, 100 + work-item triangles, 1000 +
- * ray-stack depth). */
+ * ray-stack depth, 10000 + threads). */
 sdf_status sdf_batch_info(sdf_batch *batch, double *out, int32_t cap, int32_t *n);
 sdf_status sdf_batch_destroy(sdf_batch *batch);
 

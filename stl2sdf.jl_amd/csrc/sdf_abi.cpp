@@ -14,6 +14,7 @@
 #include <exception>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "sdf_internal.h"
@@ -889,6 +890,7 @@ void ctx_free(sdf_ctx *c) {
 // bit for bit.
 struct BatchTune {
   int lanes = 4;            // worker contexts
+  int threads = 0;          // 1: each lane's passes issued by a host thread of its own
   int chunk_len = 64;       // triangles per distance work item
   int ray_stack = 1024;     // LDS ray-stack depth of the vote stage
 };
@@ -981,7 +983,27 @@ void batch_plan(sdf_batch *b) {
 double batch_run(sdf_batch *b) {
   const auto t0 = std::chrono::steady_clock::now();
   const int n = (int)b->mesh.size();
-  for (int i = 0; i < n; i++) batch_issue(b, i);
+  const int nl = b->tune.lanes < n ? b->tune.lanes : n;
+  if (b->tune.threads && nl > 1) {
+    // lane l issues meshes l, l + lanes, ...: lanes share no host state
+    std::vector<std::thread> th;
+    std::vector<std::exception_ptr> errs((size_t)nl);
+    const int dev = b->parent->device;
+    for (int l = 0; l < nl; l++)
+      th.emplace_back([&, l] {
+        try {
+          SDF_HIP(hipSetDevice(dev));
+          for (int i = l; i < n; i += b->tune.lanes) batch_issue(b, i);
+        } catch (...) {
+          errs[(size_t)l] = std::current_exception();
+        }
+      });
+    for (auto &t : th) t.join();
+    for (auto &e : errs)
+      if (e) std::rethrow_exception(e);
+  } else {
+    for (int i = 0; i < n; i++) batch_issue(b, i);
+  }
   b->last_issue_ms =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   for (int l = 0; l < b->tune.lanes && l < n; l++)
@@ -1007,9 +1029,10 @@ double batch_run(sdf_batch *b) {
   return ms;
 }
 
-// Times candidate lane counts, work-item lengths and ray-stack depths (best
-// of two runs each, after a planning pass for the candidate) and keeps the
-// fastest; ties keep the earlier candidate.
+// Times candidate issue modes (one host thread, or one per lane), lane
+// counts, work-item lengths and ray-stack depths, in that order (best of
+// two runs each, after a planning pass for the candidate), and keeps the
+// fastest of each; ties keep the earlier candidate.
 void batch_autotune(sdf_batch *b) {
   auto measure = [&](int id) {
     batch_plan(b);
@@ -1031,6 +1054,7 @@ void batch_autotune(sdf_batch *b) {
     }
     *field = keep;
   };
+  pick(&b->tune.threads, {0, 1}, 10000);
   pick(&b->tune.lanes, {1, 2, 3, 4, 6, 8}, 0);
   pick(&b->tune.chunk_len, {64, 32, 16}, 100);
   pick(&b->tune.ray_stack, {1024, 2048}, 1000);
@@ -1990,7 +2014,8 @@ sdf_status sdf_batch_info(sdf_batch *b, double *out, int32_t cap, int32_t *n) {
     std::vector<double> v = {(double)b->mesh.size(),  (double)b->total_points,
                              b->last_ms,              (double)b->tune.chunk_len,
                              (double)b->tune.ray_stack, (double)b->tune.lanes,
-                             b->last_issue_ms, (double)(b->tune_log.size() / 2)};
+                             b->last_issue_ms, (double)b->tune.threads,
+                             (double)(b->tune_log.size() / 2)};
     v.insert(v.end(), b->tune_log.begin(), b->tune_log.end());
     int w = 0;
     for (; w < (int)v.size() && w < cap && out; w++) out[w] = v[w];

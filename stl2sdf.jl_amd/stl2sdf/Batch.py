@@ -20,7 +20,7 @@ import numpy as np
 from . import _lib
 
 INFO = ("meshes", "grid_points", "last_ms", "chunk_len", "ray_stack", "lanes", "issue_ms",
-        "candidates")
+        "threads", "candidates")
 
 
 class SignedDistanceBatch:
