@@ -219,6 +219,23 @@ def batch_bench(L, ctx, torch, runs=5):
             "candidates": info["tuning"]}
 
 
+def batch_child(timeout_s=600):
+    """batch_bench in a child process (its own library context and runtime
+    state), so that a failure there costs the batched figure only."""
+    import subprocess
+    cmd = [sys.executable, "-u", os.path.abspath(__file__), "--batch-only"]
+    try:
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, timeout=timeout_s, text=True)
+    except subprocess.TimeoutExpired:
+        log(f"[bench] batch child timed out after {timeout_s} s")
+        return {"error": f"timed out after {timeout_s} s"}
+    lines = [s for s in r.stdout.splitlines() if s.startswith("{")]
+    if r.returncode != 0 or not lines:
+        log(f"[bench] batch child exited with {r.returncode}")
+        return {"error": f"child exited with status {r.returncode}"}
+    return json.loads(lines[-1])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -231,6 +248,9 @@ def main():
     ap.add_argument("--fit-steps", type=int, default=1500)
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched 64-mesh family (the batched configuration)")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only the batched configuration, its result as one JSON line (the "
+                         "default run takes it from such a child process)")
     ap.add_argument("--tune", action="append", default=[], metavar="NAME=VALUE",
                     help="launch tunable of the library (sdf_ctx_tune); results do not change")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
@@ -254,6 +274,14 @@ def main():
 
     import torch
     from stl2sdf import _lib
+
+    if args.batch_only:
+        L = _lib.lib()
+        ctx = _lib.ctx(0)
+        stream = torch.cuda.current_stream(torch.device("cuda:0"))
+        _lib.check(L.sdf_ctx_set_stream(ctx, C.c_void_p(stream.cuda_stream)))
+        print(json.dumps(batch_bench(L, ctx, torch)))
+        return
 
     log(f"[bench] building workload {args.workload}")
     mesh, grid, desc, info = make_workload(args.workload)
@@ -344,7 +372,7 @@ def main():
         }
         log(f"[bench] learned: {lval:.4g} pts/s, identical={identical}")
 
-    batched = None if args.no_batch else batch_bench(L, ctx, torch)
+    batched = None if args.no_batch else batch_child()
 
     ms_step = dt / args.steps * 1e3
     value = grid.ngp / (dt / args.steps)
