@@ -151,15 +151,17 @@ struct sdf_ctx {
   sdf::LineBuild lb;  // the line chain between its counting and emit passes
   bool bvh_ready = false;  // the ray hierarchy of the current mesh is built
   bool bvh_full = false;   // and the per-triangle one too
-  // small device words: [0] error flags, [1] pending count, [2] pending
-  // count of ambiguous axis decisions, [4..19] resolve-phase counters
+  // small device words: [0] error flags, [1] pending count (5:1 splits), [2]
+  // pending count of ambiguous axis decisions, [3] the points path's item
+  // count, [4..15] resolve-phase counters, [16] pending count (4:2, 3:3)
   uint32_t *d_words = nullptr;
   // Capacities of the lattice path's lists, from earlier calls (grow-only;
   // < 0: size exactly, reading the counts back).  With all set, a call
   // issues no host waits until its final status read.
   struct Caps {
     int64_t pairs = -1, hits = -1, chunks = -1;
-    uint32_t pend = 0, pend0 = 0, wind = 1024;
+    uint32_t pend[sdf::kPendLists] = {0, 0, 0};  // PendOut order
+    uint32_t wind = 1024;
   } caps;
   uint64_t *d_status = nullptr;  // sizes of the last lattice call (16 words)
   bool own_status = true;        // d_status allocated by this context
@@ -183,8 +185,8 @@ struct sdf_ctx {
   int nev = 0;
   int64_t counters[7] = {0, 0, 0, 0, 0, 0, 0};
   // the pending lists of the last lattice call (points handed to the vote)
-  const sdf::Pending *last_pend[2] = {nullptr, nullptr};
-  uint64_t last_npend[2] = {0, 0};
+  const sdf::Pending *last_pend[sdf::kPendLists] = {nullptr, nullptr, nullptr};
+  uint64_t last_npend[sdf::kPendLists] = {0, 0, 0};
   // learned search radius
   sdf::MlpDev mlp;
   bool mlp_alloc = false;
@@ -311,25 +313,29 @@ void bvh_async(sdf_ctx *c, int q, const sdf::GridP &g, bool full, int after = 3)
   SDF_HIP(hipEventRecord(c->join[2], c->aux[q]));
 }
 
-// Adaptive vote for the items of one or two lists (PendLists: list k's
-// rays below r0[k] are already counted in its items' state).  Phases, from
-// a = the larger r0: the first min_rays rays (when a < min_rays), the next
-// batch of 4, then the rest; every list takes part in the first phase,
-// tracing its own rays from its r0, and each phase hands the items it
-// cannot finish to the next, so later (and longer) phases trace rays only
-// for the survivors.  cnt: 4 device words per phase (at most 12); winding
-// counts land in word 2 of each phase.
-// Slots of the last vote phase whose winding sums are computed ahead (rows
-// of at most 1024 chunk sums: at most 64 MB).
+// Adaptive vote for the items of the first lists (PendLists, r0 set).  List
+// `spec` (-1: none) holds items that cannot finish by the first phase's end:
+// its rays up to max_rays are traced in the first phase, so none of its
+// items is handed on.  Phases, from a = the largest r0: the first min_rays
+// rays (when a < min_rays), the next batch of 4, then the rest; every list
+// takes part in the first phase, tracing its own rays from its r0, and each
+// phase hands the items it cannot finish to the next, so later (and longer)
+// phases trace rays only for the survivors.  Items below the confidence
+// threshold after their last ray take the winding fallback after the last
+// phase; the chunk sums of the spec list's items and of the last phase's
+// items (the few the earlier phases left open) are computed on helper
+// stream 0 beside the rays, rows of at most kWindAheadSlots items each.
+// cnt: 4 device words per phase (at most 12), zero at the call's start:
+// [0] rays redone by brute force, [1] items handed on; cnt[2] counts the
+// winding items.
 constexpr uint32_t kWindAheadSlots = 8192;
 
 void resolve_items(sdf_ctx *c, const sdf::GridP &g, const double *d_pts,
-                   const sdf::RayParams &rp, const sdf::PendLists &first, sdf::FusedOut out,
-                   uint32_t *cnt) {
+                   const sdf::RayParams &rp, const sdf::PendLists &first, int spec,
+                   sdf::FusedOut out, uint32_t *cnt) {
   const uint64_t slots = first.slots();
   if (slots == 0) return;
   const uint32_t cap = slots < 0xFFFFFFF0ull ? (uint32_t)slots : 0xFFFFFFF0u;
-  const sdf::Bvh &bvh = c->bvh;
   const int rmax = rp.max_rays < sdf::kNumDirs ? rp.max_rays : sdf::kNumDirs;
   int a = first.r0[0];
   for (int k = 1; k < first.count; k++) a = first.r0[k] > a ? first.r0[k] : a;
@@ -343,33 +349,61 @@ void resolve_items(sdf_ctx *c, const sdf::GridP &g, const double *d_pts,
     a += 4;
   }
   ph[np++] = rmax;
+  const bool wind = rp.use_winding != 0 && c->caps.wind > 0;
+  // at most one winding item per item of the first phase
+  sdf::WindItem *wl = c->ar.get<sdf::WindItem>("res_wind", cap);
+  uint32_t *nwind = cnt + 2;
+  int64_t chunk = 0;
+  const int chunks = sdf::winding_chunks(c->nel, &chunk);
+  const uint32_t rows_spec =
+      wind && spec >= 0 ? std::min<uint32_t>(first.cap[spec], kWindAheadSlots) : 0u;
+  const uint32_t rows_last = wind && np > 1 ? std::min<uint32_t>(cap, kWindAheadSlots) : 0u;
+  double *ahead = rows_spec + rows_last > 0
+                      ? c->ar.get<double>("res_wahead", (size_t)(rows_spec + rows_last) * chunks)
+                      : nullptr;
+  bool ahead_issued = false;
+  auto issue_ahead = [&](const sdf::Pending *items, const uint32_t *n, uint32_t rows,
+                         uint32_t base) {
+    SDF_HIP(hipEventRecord(c->ev_wind[0], c->stream));
+    SDF_HIP(hipStreamWaitEvent(c->aux[0], c->ev_wind[0], 0));
+    sdf::launch_winding_ahead(c->aux[0], g, d_pts, c->tri, c->nel, items, n, rows, base, ahead);
+    ahead_issued = true;
+  };
   static const char *names[2] = {"res_next0", "res_next1"};
   sdf::PendLists cur = first;
   for (int p = 0; p < np; p++) {
     const bool last = p + 1 == np;
-    sdf::Pending *next = last ? nullptr : c->ar.get<sdf::Pending>(names[p], cap);
-    // The last phase's items are the few the earlier phases left open: the
-    // winding sums of those in its first kWindAheadSlots slots (the fallback
-    // some of them need after the last rays) are computed on helper stream 0
-    // beside that phase's rays; the vote's winding pass takes the rest.
-    double *ahead = nullptr;
-    const uint32_t rows = cap < kWindAheadSlots ? cap : kWindAheadSlots;
-    if (last && p > 0 && rp.use_winding && c->caps.wind > 0) {
-      int64_t chunk = 0;
-      const int chunks = sdf::winding_chunks(c->nel, &chunk);
-      ahead = c->ar.get<double>("res_wahead", (size_t)rows * chunks);
-      SDF_HIP(hipEventRecord(c->ev_wind[0], c->stream));
-      SDF_HIP(hipStreamWaitEvent(c->aux[0], c->ev_wind[0], 0));
-      sdf::launch_winding_ahead(c->aux[0], g, d_pts, c->tri, c->nel, cur, rows,
-                                cnt + 4 * (p - 1) + 1, ahead);
-      SDF_HIP(hipEventRecord(c->ev_wind[1], c->aux[0]));
+    for (int k = 0; k < cur.count; k++) {
+      const bool all = p == 0 && k == spec;
+      cur.r1[k] = all ? rmax : ph[p];
+      cur.avail[k] = last || all ? sdf::kNumDirs : ph[p];
     }
-    sdf::resolve_pending(c->stream, c->ar, g, d_pts, c->tri, c->nel, bvh, rp, cur, ph[p],
-                         last ? 1 : 0, out, next, nullptr, cap, cnt + 4 * p, c->caps.wind,
-                         c->d_words, c->ray_stack, ahead, ahead ? rows : 0,
-                         ahead ? c->ev_wind[1] : nullptr);
+    if (p == 0 && rows_spec > 0) {
+      issue_ahead(first.items[spec], first.n_dev[spec], rows_spec, 0u);
+      cur.ahead_list = spec;
+      cur.ahead_rows = rows_spec;
+      cur.ahead_base = 0u;
+    }
+    if (last && p > 0 && rows_last > 0) {
+      issue_ahead(cur.items[0], cur.n_dev[0], rows_last, rows_spec);
+      cur.ahead_list = 0;
+      cur.ahead_rows = rows_last;
+      cur.ahead_base = rows_spec;
+    }
+    sdf::Pending *next = last ? nullptr : c->ar.get<sdf::Pending>(names[p], cap);
+    sdf::resolve_pending(c->stream, c->ar, g, d_pts, c->tri, c->nel, c->bvh, rp, cur, out, next,
+                         cnt + 4 * p + 1, cap, wl, nwind, cap, cnt + 4 * p, c->d_words,
+                         c->ray_stack);
     cur = sdf::PendLists();
     cur.add(next, cnt + 4 * p + 1, cap, ph[p]);
+  }
+  if (wind) {
+    if (ahead_issued) {
+      SDF_HIP(hipEventRecord(c->ev_wind[1], c->aux[0]));
+      SDF_HIP(hipStreamWaitEvent(c->stream, c->ev_wind[1], 0));
+    }
+    sdf::resolve_winding(c->stream, c->ar, g, d_pts, c->tri, c->nel, wl, nwind, c->caps.wind,
+                         ahead, out);
   }
 }
 
@@ -418,15 +452,19 @@ struct LatticeRun {
   sdf::Bins bins;
   float *d2buf = nullptr;
   bool bvh_was_ready = false;
-  uint32_t cap = 0, cap0 = 0;
-  sdf::Pending *pend = nullptr, *pend0 = nullptr;
+  // the sign kernel's pending lists (PendOut): margin points, 5:1, 4:2 / 3:3
+  uint32_t cap[sdf::kPendLists] = {0, 0, 0};
+  sdf::Pending *pend[sdf::kPendLists] = {nullptr, nullptr, nullptr};
   sdf::BrickPlan *plan = nullptr;  // the sign kernel's per-brick plan
 };
 
 enum : int {
-  kStErr = 0, kStPend, kStPend0, kStPairs, kStHits, kStChunks, kStWindA1, kStWindA2,
-  kStWindB1, kStWindB2, kStMeshErr, kStNext0, kStCount
+  kStErr = 0, kStPend, kStPend0, kStPairs, kStHits, kStChunks, kStWind, kStPend2,
+  kStMeshErr, kStNext0, kStCount
 };
+
+// device words of the pending lists' counts (PendOut order)
+constexpr int kPendWord[sdf::kPendLists] = {2, 1, 16};
 
 // Stage 1: the build chains.  Tile bins and distance work items run on the
 // main stream; the axis-line hits (needed by the sign stage) on helper
@@ -479,12 +517,13 @@ void stage_bins_tiles(sdf_ctx *c, LatticeRun &L) {
   // issued after the tile chain (so that chain's launches are not delayed)
   // but free to start beside it
   if (L.do_sign && c->lbvh_early) bvh_async(c, 1, L.g, L.learned, 0);
-  L.cap = c->caps.pend ? c->caps.pend
-                       : (uint32_t)(npts < (1 << 22) ? (npts > 0 ? npts : 1) : (1 << 22));
-  L.cap0 = c->caps.pend0 ? c->caps.pend0
-                         : (uint32_t)(npts < (1 << 20) ? (npts > 0 ? npts : 1) : (1 << 20));
-  L.pend = c->ar.get<sdf::Pending>("pend", L.cap);
-  L.pend0 = c->ar.get<sdf::Pending>("pend0", L.cap0);
+  const uint32_t dflt[sdf::kPendLists] = {1u << 20, 1u << 22, 1u << 20};
+  static const char *pnames[sdf::kPendLists] = {"pend0", "pend", "pend2"};
+  for (int k = 0; k < sdf::kPendLists; k++) {
+    const uint32_t d = (uint32_t)(npts < (int64_t)dflt[k] ? (npts > 0 ? npts : 1) : dflt[k]);
+    L.cap[k] = c->caps.pend[k] ? c->caps.pend[k] : d;
+    L.pend[k] = c->ar.get<sdf::Pending>(pnames[k], L.cap[k]);
+  }
 }
 
 // The axis-line hits on helper stream 0, joined through join[1].
@@ -559,9 +598,14 @@ void stage_sign(sdf_ctx *c, LatticeRun &L) {
     SDF_HIP(hipStreamWaitEvent(c->stream, c->ev_plan, 0));
   }
   c->mark("fused");
-  sdf::launch_fused_lattice(c->stream, L.g, c->tri, L.bins, L.d2buf, L.do_sign, L.out, L.pend,
-                            c->d_words + 1, L.cap, L.pend0, c->d_words + 2, L.cap0, c->d_words,
-                            L.plan, (int64_t)c->fused_per_cu * c->ncu);
+  sdf::PendOut po{};
+  for (int k = 0; k < sdf::kPendLists; k++) {
+    po.p[k] = L.pend[k];
+    po.n[k] = c->d_words + kPendWord[k];
+    po.cap[k] = L.cap[k];
+  }
+  sdf::launch_fused_lattice(c->stream, L.g, c->tri, L.bins, L.d2buf, L.do_sign, L.out, po,
+                            c->d_words, L.plan, (int64_t)c->fused_per_cu * c->ncu);
 }
 
 // Stage 4: adaptive vote for the pending points (waits for the LBVH), then
@@ -576,28 +620,27 @@ void stage_resolve(sdf_ctx *c, LatticeRun &L) {
     c->mark("resolve");
     // points whose axis-ray decisions fell inside a rounding margin vote
     // from ray 1, in the same launches as the points the six axis rays left
-    // undecided (which vote from ray min_rays + 1)
+    // undecided (which vote from ray min_rays + 1); those split 4:2 or 3:3
+    // take all their rays in the first phase (list 2)
     sdf::PendLists lists;
-    if (L.cap0 > 0) lists.add(L.pend0, c->d_words + 2, L.cap0, 0);
-    lists.add(L.pend, c->d_words + 1, L.cap, L.rp.min_rays);
-    resolve_items(c, L.g, nullptr, L.rp, lists, L.out, cnt);
+    for (int k = 0; k < sdf::kPendLists; k++)
+      lists.add(L.pend[k], c->d_words + kPendWord[k], L.cap[k], k == 0 ? 0 : L.rp.min_rays);
+    resolve_items(c, L.g, nullptr, L.rp, lists, 2, L.out, cnt);
   }
   c->mark("end");
   sdf::StatusSrc src;
   src.n = kStCount;
   src.set(kStErr, c->d_words, false);
   src.set(kStMeshErr, c->d_words + kMeshErrWord, false);
-  src.set(kStPend, c->d_words + 1, false);
-  src.set(kStPend0, c->d_words + 2, false);
+  src.set(kStPend, c->d_words + kPendWord[1], false);
+  src.set(kStPend0, c->d_words + kPendWord[0], false);
+  src.set(kStPend2, c->d_words + kPendWord[2], false);
   src.set(kStPairs, L.bins.npairs_dev, true);
   src.set(kStHits, L.bins.nhits_dev, true);
   src.set(kStChunks, L.bins.nchunks_dev, true);
   if (L.do_sign) {
     src.set(kStNext0, cnt + 1, false);
-    src.set(kStWindA1, cnt + 2, false);
-    src.set(kStWindA2, cnt + 6, false);
-    src.set(kStWindB1, cnt + 10, false);
-    src.set(kStWindB2, cnt + 14, false);
+    src.set(kStWind, cnt + 2, false);
   }
   sdf::launch_status(c->stream, src, c->d_status);
 }
@@ -624,16 +667,15 @@ bool lattice_settle(sdf_ctx *c, LatticeRun &L, const uint64_t *st) {
   grow64(c->caps.pairs, st[kStPairs], L.do_dist && c->nel > 0);
   grow64(c->caps.chunks, st[kStChunks], L.do_dist);
   grow64(c->caps.hits, st[kStHits], L.do_sign);
-  if (st[kStPend] > L.cap) {
-    c->caps.pend = (uint32_t)std::min<uint64_t>(st[kStPend] + st[kStPend] / 8 + 1024, 0xFFFFFFF0u);
-    again = true;
+  const int pst[sdf::kPendLists] = {kStPend0, kStPend, kStPend2};
+  for (int k = 0; k < sdf::kPendLists; k++) {
+    const uint64_t n = st[pst[k]];
+    if (n > L.cap[k]) {
+      c->caps.pend[k] = (uint32_t)std::min<uint64_t>(n + n / 8 + 1024, 0xFFFFFFF0u);
+      again = true;
+    }
   }
-  if (st[kStPend0] > L.cap0) {
-    c->caps.pend0 = (uint32_t)std::min<uint64_t>(st[kStPend0] + st[kStPend0] / 8 + 1024, 0xFFFFFFF0u);
-    again = true;
-  }
-  uint64_t w = std::max(std::max(st[kStWindA1], st[kStWindA2]),
-                        std::max(st[kStWindB1], st[kStWindB2]));
+  const uint64_t w = st[kStWind];
   if (w > c->caps.wind) {
     c->caps.wind = (uint32_t)std::min<uint64_t>(w + w / 8 + 64, 0xFFFFFFF0u);
     again = true;
@@ -674,18 +716,17 @@ void run_lattice(sdf_ctx *c, const sdf::GridP &g, bool do_dist, bool do_sign,
     uint64_t st[16];
     read_status(c, st);
     if (!lattice_settle(c, L, st)) {
-      c->counters[0] = (int64_t)(st[kStPend] + st[kStPend0]);
-      c->last_pend[0] = L.do_sign ? L.pend0 : nullptr;
-      c->last_pend[1] = L.do_sign ? L.pend : nullptr;
-      c->last_npend[0] = L.do_sign ? std::min<uint64_t>(st[kStPend0], L.cap0) : 0;
-      c->last_npend[1] = L.do_sign ? std::min<uint64_t>(st[kStPend], L.cap) : 0;
+      c->counters[0] = (int64_t)(st[kStPend] + st[kStPend0] + st[kStPend2]);
+      const uint64_t pn[sdf::kPendLists] = {st[kStPend0], st[kStPend], st[kStPend2]};
+      for (int k = 0; k < sdf::kPendLists; k++) {
+        c->last_pend[k] = L.do_sign ? L.pend[k] : nullptr;
+        c->last_npend[k] = L.do_sign ? std::min<uint64_t>(pn[k], L.cap[k]) : 0;
+      }
       c->counters[1] = (int64_t)st[kStPairs];
       c->counters[2] = (int64_t)st[kStHits];
       c->counters[3] = (int64_t)c->ar.bytes();
       c->counters[4] = L.do_sign ? (int64_t)st[kStNext0] : 0;
-      c->counters[5] = L.do_sign ? (int64_t)(st[kStWindA1] + st[kStWindA2] + st[kStWindB1] +
-                                             st[kStWindB2])
-                                 : 0;
+      c->counters[5] = L.do_sign ? (int64_t)st[kStWind] : 0;
       c->counters[6] = L.do_sign ? (int64_t)st[kStPend0] : 0;
       c->lstat_pairs = (int64_t)st[kStPairs];
       return;
@@ -1184,7 +1225,7 @@ sdf_status sdf_ctx_vote_points(sdf_ctx *c, int64_t *idx_out, int64_t cap, int64_
     std::lock_guard<std::mutex> lk(c->mu);
     use_device(c);
     std::vector<int64_t> all;
-    for (int k = 0; k < 2; k++) {
+    for (int k = 0; k < sdf::kPendLists; k++) {
       if (!c->last_pend[k] || c->last_npend[k] == 0) continue;
       std::vector<sdf::Pending> h((size_t)c->last_npend[k]);
       SDF_HIP(hipMemcpyAsync(h.data(), c->last_pend[k], h.size() * sizeof(sdf::Pending),
@@ -1621,12 +1662,12 @@ sdf_status sdf_raycast_signs(sdf_ctx *c, const double *X, int64_t nnp,
         c->mark("resolve");
         sdf::PendLists lists;
         lists.add(pend, nd, (uint32_t)npts, 0);
-        resolve_items(c, g, d_pts, rp, lists, out, cnt);
+        resolve_items(c, g, d_pts, rp, lists, -1, out, cnt);
         c->mark("end");
         uint32_t hc[12];
         SDF_HIP(hipMemcpyAsync(hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, c->stream));
         SDF_HIP(hipStreamSynchronize(c->stream));
-        const uint32_t w = std::max(std::max(hc[2], hc[6]), hc[10]);
+        const uint32_t w = hc[2];
         if (w <= c->caps.wind || attempt >= 2) break;
         c->caps.wind = w + w / 8 + 64;
       }

@@ -70,12 +70,37 @@ class Arena {
 };
 
 // Per-point state that the fused kernel could not settle with the six axis
-// rays; finished by the adaptive-vote kernel.
+// rays; finished by the adaptive-vote kernel.  bits: parities of the rays
+// the current vote phase traces (bit r for ray r), zero when the item is
+// written, set by the ray stage.
 struct Pending {
   int64_t idx;
   double dist;
   int32_t inside;
   int32_t used;
+  unsigned long long bits;
+};
+
+// The sign kernel's pending lists: [0] points whose axis-ray decision fell
+// inside a rounding margin (they vote from ray 1), [1] six votes split 5:1,
+// [2] six votes split 4:2 or 3:3, which cannot reach the 0.85 confidence by
+// ray 10 and take all their rays in the vote's first phase.
+constexpr int kPendLists = 3;
+// (no member initialisers: the sign kernel keeps a copy in LDS; value-
+// initialise it, PendOut{}, on the host)
+struct PendOut {
+  Pending *p[kPendLists];
+  uint32_t *n[kPendLists];
+  uint32_t cap[kPendLists];
+};
+
+// An item of the winding-number fallback: the point, its distance and the
+// row of its chunk sums when they were computed ahead (else -1: computed
+// after the vote, at the item's position in the list).
+struct WindItem {
+  int64_t idx;
+  double dist;
+  int64_t row;
 };
 
 // Error flags written by kernels (bit mask).
@@ -188,12 +213,8 @@ struct FusedCold {
   const uint64_t *line_off;
   const LineFast *fast;
   const uint32_t *meta;
-  Pending *pend;
-  uint32_t *npend;
-  Pending *pend0;
-  uint32_t *npend0;
+  PendOut pend;
   uint32_t *err;
-  uint32_t cap, cap0;
 };
 
 struct FusedJob {
@@ -218,8 +239,7 @@ struct FusedJob {
 DistJob make_dist_job(const GridP &g, const Tri64 *tri, const Bins &b, float *d2buf,
                       const uint32_t *only);
 FusedJob make_fused_job(const GridP &g, const Bins &b, const float *d2buf, bool do_sign,
-                        FusedOut out, Pending *pend, uint32_t *npend, uint32_t pend_cap,
-                        Pending *pend0, uint32_t *npend0, uint32_t pend0_cap, uint32_t *err);
+                        FusedOut out, const PendOut &pend, uint32_t *err);
 void build_tile_bins(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
                      const GridP &g, double radius, Bins &b);
 // Two-phase forms: *_begin issues the counting kernels and scans (no host
@@ -283,8 +303,9 @@ void launch_learned_fallback(hipStream_t s, const GridP &g, const Tri64 *tri, co
                              const unsigned long long *stats, float slack, float *d2buf,
                              uint32_t *err);
 // Per tile: distance from d2buf (or h), six axis rays, combine, outputs.
-// Points with a vote inside its rounding margin go to pend0 (restart from
-// ray 1 on the LBVH); non-unanimous votes go to pend (continue at ray 7).
+// Points with a vote inside its rounding margin go to pend.p[0] (restart
+// from ray 1 on the LBVH); non-unanimous votes go to pend.p[1] (5:1) or
+// pend.p[2] (4:2, 3:3) and continue at ray 7.
 // Launch positions of the sign kernel (BrickPlan entries it reads).
 int64_t brick_plan_size(const GridP &g);
 // The sign kernel's per-brick plan (coordinates, distance slots) from the
@@ -293,10 +314,8 @@ void launch_brick_plan(hipStream_t s, const GridP &g, const Bins &b, const float
                        BrickPlan *plan);
 void launch_fused_lattice(hipStream_t s, const GridP &g, const Tri64 *tri,
                           const Bins &b, const float *d2buf, bool do_sign,
-                          FusedOut out, Pending *pend, uint32_t *npend,
-                          uint32_t pend_cap, Pending *pend0, uint32_t *npend0,
-                          uint32_t pend0_cap, uint32_t *err, const BrickPlan *plan,
-                          int64_t max_blocks = 0);
+                          FusedOut out, const PendOut &pend, uint32_t *err,
+                          const BrickPlan *plan, int64_t max_blocks = 0);
 
 struct RayParams {
   double dirs[kNumDirs * 3];
@@ -307,52 +326,73 @@ struct RayParams {
 
 struct Bvh;
 
-// One or two pending lists voted in one phase.  List k holds *n_dev[k]
-// items (capacity cap[k]) whose rays below r0[k] are already counted in
-// their state; an item's position over both lists (its slot) is its index
-// in list 0, or cap[0] + its index in list 1.
+// The pending lists voted in one phase (at most three).  List k holds
+// *n_dev[k] items (capacity cap[k]) whose rays below r0[k] are counted in
+// their state; the phase traces rays [r0[k], r1[k]) of its items (an item
+// whose state already counts a ray skips it) and the vote replay may use
+// rays below avail[k] (kNumDirs: every ray the vote can use, so no item is
+// carried).  An item's slot over the lists is its index in list 0, cap[0] +
+// its index in list 1, and so on.  Items of list ahead_list with an index
+// below ahead_rows have their winding chunk sums at row ahead_base + index
+// of the ahead buffer.
 struct PendLists {
-  const Pending *items[2] = {nullptr, nullptr};
-  const uint32_t *n_dev[2] = {nullptr, nullptr};
-  uint32_t cap[2] = {0, 0};
-  int32_t r0[2] = {0, 0};
+  Pending *items[kPendLists] = {nullptr, nullptr, nullptr};
+  const uint32_t *n_dev[kPendLists] = {nullptr, nullptr, nullptr};
+  uint32_t cap[kPendLists] = {0, 0, 0};
+  int32_t r0[kPendLists] = {0, 0, 0};
+  int32_t r1[kPendLists] = {0, 0, 0};
+  int32_t avail[kPendLists] = {0, 0, 0};
   int32_t count = 0;
-  void add(const Pending *p, const uint32_t *n, uint32_t c, int r) {
+  int32_t ahead_list = -1;
+  uint32_t ahead_rows = 0;
+  uint32_t ahead_base = 0;
+  void add(Pending *p, const uint32_t *n, uint32_t c, int r) {
     items[count] = p;
     n_dev[count] = n;
     cap[count] = c;
     r0[count] = r;
+    r1[count] = r;
+    avail[count] = r;
     count++;
   }
-  uint64_t slots() const { return (uint64_t)cap[0] + (count > 1 ? cap[1] : 0u); }
+  __host__ __device__ uint64_t base(int k) const {
+    uint64_t b = 0;
+    for (int i = 0; i < k; i++) b += cap[i];
+    return b;
+  }
+  __host__ __device__ uint64_t slots() const { return base(count); }
 };
 
-// Adaptive vote for pending items, with no host waits (grids are fixed,
-// counts are read on the device): each list traces its rays [r0[k], r1)
-// through the LBVH (overflowing rays by brute force), then the vote replay.
-// final_phase 0 carries items whose vote needs a ray >= r1 to `next`
-// (count *nnext, or d_counts[1] when nnext is null; capacity next_cap);
-// otherwise every item is finished.  Items that still need the winding
-// fallback (count d_counts[2]) are finished if it is <= wind_cap (else the
-// caller repeats the call with a larger wind_cap).  d_counts: 4 words.
-// pts == nullptr: lattice points of g.
+// One phase of the adaptive vote, with no host waits (grids are fixed,
+// counts are read on the device): the rays of every list (PendLists) through
+// the LBVH (overflowing rays by brute force), then the vote replay.  Items
+// whose vote needs a ray at or above their list's avail go to `next` (count
+// *nnext, capacity next_cap) with their state and cleared ray bits; items
+// whose final confidence is below the threshold go to the winding list
+// `wind` (count *nwind, capacity wind_cap).  d_counts[0]: rays redone by
+// brute force.  pts == nullptr: lattice points of g.
 void resolve_pending(hipStream_t s, Arena &ar, const GridP &g, const double *pts,
                      const Tri64 *tri, int64_t nel, const Bvh &bvh, const RayParams &rp,
-                     const PendLists &lists, int r1, int final_phase, FusedOut out,
-                     Pending *next, uint32_t *nnext, uint32_t next_cap, uint32_t *d_counts,
-                     uint32_t wind_cap, uint32_t *err, int stack = 1024,
-                     const double *wind_ahead = nullptr, uint64_t ahead_rows = 0,
-                     hipEvent_t wind_ahead_done = nullptr);
+                     const PendLists &lists, FusedOut out, Pending *next, uint32_t *nnext,
+                     uint32_t next_cap, WindItem *wind, uint32_t *nwind, uint32_t wind_cap,
+                     uint32_t *d_counts, uint32_t *err, int stack = 1024);
+// The winding fallback of the items in `wind` (count *nwind, at most
+// wind_cap used): chunk sums of the items without an ahead row, then the
+// totals, signs and outputs (ahead rows from `ahead`, complete once the
+// stream reaches this call).
+void resolve_winding(hipStream_t s, Arena &ar, const GridP &g, const double *pts,
+                     const Tri64 *tri, int64_t nel, const WindItem *wind,
+                     const uint32_t *nwind, uint32_t wind_cap, const double *ahead,
+                     FusedOut out);
 // Chunks of the winding-number sums over nel triangles (chunk: triangles
 // per chunk).
 int winding_chunks(int64_t nel, int64_t *chunk);
-// The winding-number partial sums of the items of `lists` (one list, the
-// count on the device at *count_dev) with a slot below items_cap, by slot:
-// partial[slot * chunks + c], ahead of the final vote that needs some of
-// them (resolve_pending's wind_ahead, ahead_rows = items_cap).
+// The winding-number chunk sums of the first min(*count_dev, rows) items of
+// one pending list, item i into ahead[(base + i) * chunks ...], ahead of the
+// vote that sends some of them to the fallback.
 void launch_winding_ahead(hipStream_t s, const GridP &g, const double *pts, const Tri64 *tri,
-                          int64_t nel, const PendLists &lists, uint32_t items_cap,
-                          const uint32_t *count_dev, double *partial);
+                          int64_t nel, const Pending *items, const uint32_t *count_dev,
+                          uint32_t rows, uint32_t base, double *ahead);
 
 // xp of evalDistancesOnTriMesh: exact FP64 closest point on the mesh for
 // every lattice point (pts == nullptr) or caller point, 3 doubles per point.

@@ -606,9 +606,10 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S,
     return dist;
   };
   // pending entry when a decision sat inside its rounding margin (amb: the
-  // point restarts the vote from ray 1 with the reference's arithmetic) or
-  // the six votes disagree; the vote stage writes such a point's outputs
-  // later, after the provisional ones written here
+  // point restarts the vote from ray 1 with the reference's arithmetic, list
+  // 0) or the six votes disagree (list 1 for a 5:1 split, list 2 for 4:2 or
+  // 3:3); the vote stage writes such a point's outputs later, after the
+  // provisional ones written here
   auto finish = [&](int64_t idx, double dist, int inside, bool amb) {
     if (do_sign && (amb | ((inside != 0) & (inside != 6)))) {
       Pending pr;
@@ -616,11 +617,11 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S,
       pr.dist = dist;
       pr.inside = amb ? 0 : inside;
       pr.used = amb ? 0 : 6;
-      uint32_t *cnt = amb ? C.npend0 : C.npend;
-      Pending *list = amb ? C.pend0 : C.pend;
-      const uint32_t slot = atomicAdd(cnt, 1u);
-      if (slot < (amb ? C.cap0 : C.cap))
-        list[slot] = pr;
+      pr.bits = 0ull;
+      const int li = amb ? 0 : ((inside == 1) | (inside == 5)) ? 1 : 2;
+      const uint32_t slot = atomicAdd(C.pend.n[li], 1u);
+      if (slot < C.pend.cap[li])
+        C.pend.p[li][slot] = pr;
       else
         atomicOr(C.err, kErrPendingOverflow);
     }
@@ -867,9 +868,27 @@ __device__ inline uint32_t list_count(const PendLists &L, int k) {
   return n;
 }
 
-// The item in a slot over both lists.
-__device__ inline const Pending &slot_item(const PendLists &L, uint64_t slot) {
-  return slot < L.cap[0] ? L.items[0][slot] : L.items[1][slot - L.cap[0]];
+// The list of a slot over the lists (k) and its index in that list.
+__device__ inline int slot_list(const PendLists &L, uint64_t slot, uint64_t *index) {
+  int k = 0;
+  while (k + 1 < L.count && slot >= L.cap[k]) {
+    slot -= L.cap[k];
+    k++;
+  }
+  *index = slot;
+  return k;
+}
+
+// The item in a slot over the lists.
+__device__ inline Pending &slot_item(const PendLists &L, uint64_t slot) {
+  uint64_t i;
+  const int k = slot_list(L, slot, &i);
+  return L.items[k][i];
+}
+
+// Rays [r0, r1) of list k (none when r1 <= r0).
+__device__ inline int list_rays(const PendLists &L, int k) {
+  return L.r1[k] > L.r0[k] ? L.r1[k] - L.r0[k] : 0;
 }
 
 // Wavefront-cooperative ray parity: one wavefront per (item, ray) over the
@@ -894,33 +913,32 @@ constexpr int kRunQueue = 512;   // queued runs of one ray (LDS); more: brute fo
 template <int kWaveStack>
 __global__ __launch_bounds__(64) void ray_wave_kernel(
     GridP g, const double *__restrict__ pts, const Tri64 *__restrict__ tri, Bvh bvh,
-    RayParams rp, PendLists lists, int r1, unsigned long long *__restrict__ bits,
-    uint64_t *__restrict__ fb, uint32_t *nfb, uint32_t fb_cap, uint32_t *err) {
+    RayParams rp, PendLists lists, uint64_t *__restrict__ fb, uint32_t *nfb, uint32_t fb_cap,
+    uint32_t *err) {
   __shared__ int32_t stack[kWaveStack];
   __shared__ int32_t runq[kRunQueue];  // runs reached, waiting for their triangle tests
   __shared__ double hit_t[kWaveHits];
   __shared__ int s_top, s_nq, s_nh, s_ovf, s_cnt;
   const int lane = threadIdx.x;
   const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-  // list 0 contributes n0 * nr0 (item, ray) pairs, then list 1
-  const uint32_t n0 = list_count(lists, 0);
-  const uint32_t n1 = lists.count > 1 ? list_count(lists, 1) : 0u;
-  const int nr0 = r1 > lists.r0[0] ? r1 - lists.r0[0] : 0;
-  const int nr1 = lists.count > 1 && r1 > lists.r0[1] ? r1 - lists.r0[1] : 0;
-  const int64_t w0 = (int64_t)n0 * nr0;
-  const int64_t total = w0 + (int64_t)n1 * nr1;
+  // list k contributes n_k * nr_k (item, ray) pairs, after the lists before it
+  int64_t wend[kPendLists];
+  int64_t total = 0;
+  for (int k = 0; k < kPendLists; k++) {
+    if (k < lists.count) total += (int64_t)list_count(lists, k) * list_rays(lists, k);
+    wend[k] = total;
+  }
   for (int64_t w = blockIdx.x; w < total; w += gridDim.x) {
-    uint64_t slot;
-    int r;
-    if (w < w0) {
-      slot = (uint64_t)(w / nr0);
-      r = lists.r0[0] + (int)(w % nr0);
-    } else {
-      slot = (uint64_t)lists.cap[0] + (uint64_t)((w - w0) / nr1);
-      r = lists.r0[1] + (int)((w - w0) % nr1);
-    }
+    int k = 0;
+    while (w >= wend[k]) k++;
+    const int64_t wk = w - (k ? wend[k - 1] : 0);
+    const int nr = list_rays(lists, k);
+    const uint64_t slot = lists.base(k) + (uint64_t)(wk / nr);
+    const int r = lists.r0[k] + (int)(wk % nr);
+    Pending &item = slot_item(lists, slot);
+    if (r < item.used) continue;  // counted in the item's state (uniform per wave)
     double o[3];
-    item_point(g, pts, slot_item(lists, slot).idx, o);
+    item_point(g, pts, item.idx, o);
     const double *d = rp.dirs + 3 * r;
     double inv[3];
     for (int a = 0; a < 3; a++) inv[a] = d[a] != 0.0 ? 1.0 / d[a] : 0.0;
@@ -1022,7 +1040,7 @@ __global__ __launch_bounds__(64) void ray_wave_kernel(
         else
           atomicOr(err, kErrHitOverflow);
       } else if (s_cnt & 1) {
-        atomicOr(&bits[slot], 1ull << r);
+        atomicOr(&item.bits, 1ull << r);
       }
     }
     __syncthreads();
@@ -1032,38 +1050,47 @@ __global__ __launch_bounds__(64) void ray_wave_kernel(
 __global__ __launch_bounds__(256) void ray_bits_brute_kernel(
     GridP g, const double *__restrict__ pts, const Tri64 *__restrict__ tri,
     int64_t nel, RayParams rp, PendLists lists, const uint64_t *__restrict__ fb,
-    uint32_t nfb, const uint32_t *__restrict__ nfb_dev, unsigned long long *__restrict__ bits,
-    uint32_t *err) {
+    uint32_t nfb, const uint32_t *__restrict__ nfb_dev, uint32_t *err) {
   __shared__ double s_t[kHitCap];
   __shared__ int s_n, s_cnt;
   if (nfb_dev && *nfb_dev < nfb) nfb = *nfb_dev;
   for (uint32_t q = blockIdx.x; q < nfb; q += gridDim.x) {
     const uint64_t slot = fb[q] / 64u;
     const int r = (int)(fb[q] % 64u);
+    Pending &item = slot_item(lists, slot);
     double o[3];
-    item_point(g, pts, slot_item(lists, slot).idx, o);
+    item_point(g, pts, item.idx, o);
     int par = block_ray_parity(o, rp.dirs + 3 * r, tri, nel, s_t, &s_n, &s_cnt, err);
-    if (threadIdx.x == 0 && par) atomicOr(&bits[slot], 1ull << r);
+    if (threadIdx.x == 0 && par) atomicOr(&item.bits, 1ull << r);
     __syncthreads();
   }
 }
 
-// Replay of adaptive_point_in_mesh_raycast (:234-288) on precomputed ray
-// parities.  Only rays below `avail` are available: an item whose vote
-// needs a later ray (the first min_rays, or the next batch of 4) is
-// appended to `next` (count *nnext, capacity next_cap) with its state.
-__global__ void vote_kernel(RayParams rp, PendLists lists,
-                            const unsigned long long *__restrict__ bits, int avail,
-                            FusedOut out, Pending *__restrict__ next, uint32_t *nnext,
-                            uint32_t next_cap, uint64_t *__restrict__ wind, uint32_t *nwind,
+// Replay of adaptive_point_in_mesh_raycast (:234-288) on the traced ray
+// parities.  An item of list k may use rays below lists.avail[k]: one whose
+// vote needs a later ray (the first min_rays, or the next batch of 4) is
+// appended to `next` (count *nnext, capacity next_cap) with its state and
+// cleared ray bits; one whose final confidence is below the threshold goes
+// to the winding list.
+__global__ void vote_kernel(RayParams rp, PendLists lists, FusedOut out,
+                            Pending *__restrict__ next, uint32_t *nnext, uint32_t next_cap,
+                            WindItem *__restrict__ wind, uint32_t *nwind, uint32_t wind_cap,
                             uint32_t *err) {
-  const uint64_t n0 = list_count(lists, 0);
-  const uint64_t n = n0 + (lists.count > 1 ? list_count(lists, 1) : 0u);
+  // live items: list k's first n_k, after the lists before it
+  uint64_t tend[kPendLists];
+  uint64_t n = 0;
+  for (int k = 0; k < kPendLists; k++) {
+    if (k < lists.count) n += list_count(lists, k);
+    tend[k] = n;
+  }
   for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < n;
        t += (uint64_t)gridDim.x * blockDim.x) {
-    const uint64_t slot = t < n0 ? t : (uint64_t)lists.cap[0] + (t - n0);
-    Pending P = slot_item(lists, slot);
-    const unsigned long long m = bits[slot];
+    int k = 0;
+    while (t >= tend[k]) k++;
+    const uint64_t index = t - (k ? tend[k - 1] : 0u);
+    Pending P = lists.items[k][index];
+    const int avail = lists.avail[k];
+    const unsigned long long m = P.bits;
     int inside = P.inside, total = P.used, used = P.used;
     bool carry = false;
     if (used < rp.min_rays) {
@@ -1099,13 +1126,10 @@ __global__ void vote_kernel(RayParams rp, PendLists lists,
     if (carry) {
       P.inside = inside;
       P.used = used;
-      if (!next) {  // the last phase has no list to hand items to
-        atomicOr(err, kErrPendingOverflow);
-        continue;
-      }
-      const uint32_t slot = atomicAdd(nnext, 1u);
-      if (slot < next_cap)
-        next[slot] = P;
+      P.bits = 0ull;
+      const uint32_t q = next ? atomicAdd(nnext, 1u) : next_cap;
+      if (q < next_cap)
+        next[q] = P;
       else
         atomicOr(err, kErrPendingOverflow);
       continue;
@@ -1113,7 +1137,15 @@ __global__ void vote_kernel(RayParams rp, PendLists lists,
     double sgn = ratio > 0.5 ? 1.0 : -1.0;
     if (rp.use_winding && conf < rp.conf_thr) {
       const uint32_t q = atomicAdd(nwind, 1u);
-      wind[q] = slot;
+      if (q < wind_cap) {
+        WindItem wi;
+        wi.idx = P.idx;
+        wi.dist = P.dist;
+        wi.row = k == lists.ahead_list && index < lists.ahead_rows
+                     ? (int64_t)(lists.ahead_base + index)
+                     : (int64_t)-1;
+        wind[q] = wi;
+      }
       continue;
     }
     if (out.sdf) out.sdf[P.idx] = P.dist * sgn;
@@ -1122,76 +1154,96 @@ __global__ void vote_kernel(RayParams rp, PendLists lists,
   }
 }
 
-// generalized_winding_number (:295-328) split over many workgroups per
-// point: each block sums the solid angles of one triangle chunk (fixed
-// reduction tree), the final pass adds the chunk sums in chunk order, so the
-// result is deterministic.
-__global__ __launch_bounds__(256) void winding_partial_kernel(
-    GridP g, const double *__restrict__ pts, const Tri64 *__restrict__ tri,
-    int64_t nel, PendLists lists,
-    const uint64_t *__restrict__ wind, uint32_t nwind, const uint32_t *__restrict__ nwind_dev,
-    int chunks, int64_t chunk, double *__restrict__ partial, uint64_t skip_below) {
+// generalized_winding_number (:295-328) of point o over triangles [e0, e1),
+// one block: the block's threads take triangles at stride blockDim.x and a
+// fixed reduction tree adds their sums (deterministic).  Every thread
+// returns the chunk's sum.
+__device__ double winding_chunk_sum(const double *o, const Tri64 *__restrict__ tri,
+                                    int64_t e0, int64_t e1, double *s_red) {
 #pragma clang fp contract(off)
+  double acc = 0.0;
+  for (int64_t el = e0 + threadIdx.x; el < e1; el += blockDim.x) {
+    const Tri64 t = tri[el];
+    double r[3], a[3], b[3], cc[3], bc[3];
+    for (int k = 0; k < 3; k++) r[k] = t.v[k] - o[k];
+    normalize_ref(r, a);
+    for (int k = 0; k < 3; k++) r[k] = t.v[3 + k] - o[k];
+    normalize_ref(r, b);
+    for (int k = 0; k < 3; k++) r[k] = t.v[6 + k] - o[k];
+    normalize_ref(r, cc);
+    cross3_ref(b, cc, bc);
+    double num = dot3_ref(a, bc);
+    double den = ((1.0 + dot3_ref(a, b)) + dot3_ref(b, cc)) + dot3_ref(cc, a);
+    if (fabs(den) > 1e-10) {
+      double sa = 2.0 * atan2(fabs(num), den);
+      if (num < 0) sa = -sa;
+      acc += sa;
+    }
+  }
+  s_red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) s_red[threadIdx.x] += s_red[threadIdx.x + off];
+    __syncthreads();
+  }
+  const double sum = s_red[0];
+  __syncthreads();
+  return sum;
+}
+
+// Chunk sums ahead of the vote: the first min(*count_dev, rows) items of a
+// pending list, item i's chunk c into ahead[(base + i) * chunks + c].
+__global__ __launch_bounds__(256) void winding_ahead_kernel(
+    GridP g, const double *__restrict__ pts, const Tri64 *__restrict__ tri, int64_t nel,
+    const Pending *__restrict__ items, const uint32_t *__restrict__ count_dev, uint32_t rows,
+    uint32_t base, int chunks, int64_t chunk, double *__restrict__ ahead) {
   __shared__ double s_red[256];
-  if (nwind_dev && *nwind_dev < nwind) nwind = *nwind_dev;
-  const int64_t nblk = (int64_t)nwind * chunks;
+  const uint32_t n = *count_dev < rows ? *count_dev : rows;
+  const int64_t nblk = (int64_t)n * chunks;
   for (int64_t q = blockIdx.x; q < nblk; q += gridDim.x) {
-    const uint32_t w = (uint32_t)(q / chunks);
+    const uint32_t i = (uint32_t)(q / chunks);
     const int64_t c = q % chunks;
-    const uint64_t slot = wind ? wind[w] : (uint64_t)w;
-    if (slot < skip_below) continue;  // computed ahead (uniform per block)
-    const Pending P = slot_item(lists, slot);
     double o[3];
-    item_point(g, pts, P.idx, o);
+    item_point(g, pts, items[i].idx, o);
     const int64_t e0 = c * chunk, e1 = min(nel, e0 + chunk);
-    double acc = 0.0;
-    for (int64_t el = e0 + threadIdx.x; el < e1; el += blockDim.x) {
-      const Tri64 t = tri[el];
-      double r[3], a[3], b[3], cc[3], bc[3];
-      for (int k = 0; k < 3; k++) r[k] = t.v[k] - o[k];
-      normalize_ref(r, a);
-      for (int k = 0; k < 3; k++) r[k] = t.v[3 + k] - o[k];
-      normalize_ref(r, b);
-      for (int k = 0; k < 3; k++) r[k] = t.v[6 + k] - o[k];
-      normalize_ref(r, cc);
-      cross3_ref(b, cc, bc);
-      double num = dot3_ref(a, bc);
-      double den = ((1.0 + dot3_ref(a, b)) + dot3_ref(b, cc)) + dot3_ref(cc, a);
-      if (fabs(den) > 1e-10) {
-        double sa = 2.0 * atan2(fabs(num), den);
-        if (num < 0) sa = -sa;
-        acc += sa;
-      }
-    }
-    s_red[threadIdx.x] = acc;
-    __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
-      if ((int)threadIdx.x < off) s_red[threadIdx.x] += s_red[threadIdx.x + off];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) partial[q] = s_red[0];
-    __syncthreads();
+    const double sum = winding_chunk_sum(o, tri, e0, e1, s_red);
+    if (threadIdx.x == 0) ahead[((int64_t)base + i) * chunks + c] = sum;
   }
 }
 
-// One wave per point: lanes add the chunk sums at stride 64 in chunk order,
+// Chunk sums of the winding items without an ahead row: item w's chunk c
+// into partial[w * chunks + c].
+__global__ __launch_bounds__(256) void winding_partial_kernel(
+    GridP g, const double *__restrict__ pts, const Tri64 *__restrict__ tri, int64_t nel,
+    const WindItem *__restrict__ wind, const uint32_t *__restrict__ nwind_dev, uint32_t cap,
+    int chunks, int64_t chunk, double *__restrict__ partial) {
+  __shared__ double s_red[256];
+  const uint32_t n = *nwind_dev < cap ? *nwind_dev : cap;
+  const int64_t nblk = (int64_t)n * chunks;
+  for (int64_t q = blockIdx.x; q < nblk; q += gridDim.x) {
+    const uint32_t w = (uint32_t)(q / chunks);
+    const int64_t c = q % chunks;
+    const WindItem wi = wind[w];
+    if (wi.row >= 0) continue;  // computed ahead (uniform per block)
+    double o[3];
+    item_point(g, pts, wi.idx, o);
+    const int64_t e0 = c * chunk, e1 = min(nel, e0 + chunk);
+    const double sum = winding_chunk_sum(o, tri, e0, e1, s_red);
+    if (threadIdx.x == 0) partial[q] = sum;
+  }
+}
+
+// One wave per item: lanes add the chunk sums at stride 64 in chunk order,
 // then a fixed butterfly; every lane holds the same total.
-__global__ __launch_bounds__(64) void winding_final_kernel(PendLists lists,
-                                                           const uint64_t *__restrict__ wind,
-                                                           uint32_t nwind,
-                                                           const uint32_t *__restrict__ nwind_dev,
-                                                           int chunks,
-                                                           const double *__restrict__ partial,
-                                                           const double *__restrict__ ahead,
-                                                           uint64_t ahead_rows, FusedOut out) {
+__global__ __launch_bounds__(64) void winding_final_kernel(
+    const WindItem *__restrict__ wind, const uint32_t *__restrict__ nwind_dev, uint32_t cap,
+    int chunks, const double *__restrict__ partial, const double *__restrict__ ahead,
+    FusedOut out) {
 #pragma clang fp contract(off)
-  if (nwind_dev && *nwind_dev < nwind) nwind = *nwind_dev;
-  for (uint32_t w = blockIdx.x; w < nwind; w += gridDim.x) {
-    // the partial sums of an item with a slot below ahead_rows were computed
-    // ahead, by slot; the others by position in the winding list
-    const uint64_t slot = wind[w];
-    const double *row = slot < ahead_rows ? ahead + (int64_t)slot * chunks
-                                          : partial + (int64_t)w * chunks;
+  const uint32_t n = *nwind_dev < cap ? *nwind_dev : cap;
+  for (uint32_t w = blockIdx.x; w < n; w += gridDim.x) {
+    const WindItem wi = wind[w];
+    const double *row = wi.row >= 0 ? ahead + wi.row * chunks : partial + (int64_t)w * chunks;
     double tot = 0.0;
     for (int c = threadIdx.x; c < chunks; c += 64) tot += row[c];
 #pragma unroll
@@ -1199,10 +1251,9 @@ __global__ __launch_bounds__(64) void winding_final_kernel(PendLists lists,
     if (threadIdx.x == 0) {
       tot = tot / 12.566370614359172;  // Float64(4pi)
       const double sgn = fabs(tot) > 0.5 ? 1.0 : -1.0;
-      const Pending P = slot_item(lists, wind[w]);
-      if (out.sdf) out.sdf[P.idx] = P.dist * sgn;
-      if (out.signs) out.signs[P.idx] = sgn;
-      if (out.conf) out.conf[P.idx] = 0.8;
+      if (out.sdf) out.sdf[wi.idx] = wi.dist * sgn;
+      if (out.signs) out.signs[wi.idx] = sgn;
+      if (out.conf) out.conf[wi.idx] = 0.8;
     }
   }
 }
@@ -1221,19 +1272,6 @@ __global__ void dist_points_bvh_kernel(const double *__restrict__ pts,
   }
 }
 
-// Clears the ray bits of the live slots of both lists and the phase's four
-// counters.
-__global__ void fill_bits_kernel(unsigned long long *p, PendLists lists, uint32_t *counts) {
-  if (blockIdx.x == 0 && threadIdx.x < 4) counts[threadIdx.x] = 0u;
-  for (int k = 0; k < lists.count; k++) {
-    const uint32_t n = list_count(lists, k);
-    unsigned long long *q = p + (k ? lists.cap[0] : 0u);
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += gridDim.x * blockDim.x)
-      q[i] = 0ull;
-  }
-}
-
 __global__ void init_pending_all_kernel(Pending *pend, int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
@@ -1242,6 +1280,7 @@ __global__ void init_pending_all_kernel(Pending *pend, int64_t n) {
     p.dist = 0.0;
     p.inside = 0;
     p.used = 0;
+    p.bits = 0ull;
     pend[i] = p;
   }
 }
@@ -1305,8 +1344,7 @@ void launch_learned_fallback(hipStream_t s, const GridP &g, const Tri64 *tri, co
 }
 
 FusedJob make_fused_job(const GridP &g, const Bins &b, const float *d2buf, bool do_sign,
-                        FusedOut out, Pending *pend, uint32_t *npend, uint32_t pend_cap,
-                        Pending *pend0, uint32_t *npend0, uint32_t pend0_cap, uint32_t *err) {
+                        FusedOut out, const PendOut &pend, uint32_t *err) {
   FusedJob J;
   J.g = g;
   J.td = tile_dims(g);
@@ -1337,24 +1375,16 @@ FusedJob make_fused_job(const GridP &g, const Bins &b, const float *d2buf, bool 
   J.cold.fast = b.line_fast;
   J.cold.meta = b.line_meta;
   J.cold.pend = pend;
-  J.cold.npend = npend;
-  J.cold.pend0 = pend0;
-  J.cold.npend0 = npend0;
   J.cold.err = err;
-  J.cold.cap = pend_cap;
-  J.cold.cap0 = pend0_cap;
   return J;
 }
 
 void launch_fused_lattice(hipStream_t s, const GridP &g, const Tri64 *tri,
                           const Bins &b, const float *d2buf, bool do_sign,
-                          FusedOut out, Pending *pend, uint32_t *npend,
-                          uint32_t pend_cap, Pending *pend0, uint32_t *npend0,
-                          uint32_t pend0_cap, uint32_t *err, const BrickPlan *plan,
-                          int64_t max_blocks) {
+                          FusedOut out, const PendOut &pend, uint32_t *err,
+                          const BrickPlan *plan, int64_t max_blocks) {
   (void)tri;
-  FusedJob J = make_fused_job(g, b, d2buf, do_sign, out, pend, npend, pend_cap, pend0,
-                              npend0, pend0_cap, err);
+  FusedJob J = make_fused_job(g, b, d2buf, do_sign, out, pend, err);
   if (J.nbricks <= 0) return;
   if (!plan) throw Error{-1, "the sign kernel needs its brick plan"};
   J.plan = plan;
@@ -1389,8 +1419,7 @@ int64_t brick_plan_size(const GridP &g) {
 
 void launch_brick_plan(hipStream_t s, const GridP &g, const Bins &b, const float *d2buf,
                        BrickPlan *plan) {
-  const FusedJob J = make_fused_job(g, b, d2buf, false, FusedOut(), nullptr, nullptr, 0,
-                                    nullptr, nullptr, 0, nullptr);
+  const FusedJob J = make_fused_job(g, b, d2buf, false, FusedOut(), PendOut{}, nullptr);
   if (J.nbricks <= 0) return;
   if (J.nbricks >= ((int64_t)1 << 31)) throw Error{-1, "too many sign bricks for one launch"};
   if (J.bd.tx >= 65536 || J.bd.ty >= 65536)
@@ -1409,63 +1438,51 @@ static int blocks_for(int64_t n, int t) {
   return (int)b;
 }
 
-// Decide pending items with the adaptive vote: each list's rays [r0, r1)
-// by LBVH traversal, overflowing rays by brute force, then the vote replay.
+// One phase of the adaptive vote: each list's rays [r0, r1) by LBVH
+// traversal, overflowing rays by brute force, then the vote replay.
 void resolve_pending(hipStream_t s, Arena &ar, const GridP &g, const double *pts,
                      const Tri64 *tri, int64_t nel, const Bvh &bvh, const RayParams &rp,
-                     const PendLists &lists, int r1, int final_phase, FusedOut out,
-                     Pending *next, uint32_t *nnext, uint32_t next_cap, uint32_t *d_counts,
-                     uint32_t wind_cap, uint32_t *err, int stack, const double *wind_ahead,
-                     uint64_t ahead_rows, hipEvent_t wind_ahead_done) {
-  // d_counts: [0] fallback rays, [1] next items (unless nnext), [2] winding items
+                     const PendLists &lists, FusedOut out, Pending *next, uint32_t *nnext,
+                     uint32_t next_cap, WindItem *wind, uint32_t *nwind, uint32_t wind_cap,
+                     uint32_t *d_counts, uint32_t *err, int stack) {
   const uint64_t slots = lists.slots();
   if (slots == 0) return;
-  if (r1 > kNumDirs) r1 = kNumDirs;
-  unsigned long long *bits = ar.get<unsigned long long>(ar.tmp("res_bits"), slots);
-  hipLaunchKernelGGL(fill_bits_kernel, dim3(128), dim3(256), 0, s, bits, lists, d_counts);
   const uint64_t fb_want = slots * 4u + 1024u;
   const uint32_t fb_cap = fb_want < 0xFFFFFFF0ull ? (uint32_t)fb_want : 0xFFFFFFF0u;
   uint64_t *fb = ar.get<uint64_t>(ar.tmp("res_fb"), fb_cap);
   bool rays = false;
-  for (int k = 0; k < lists.count; k++) rays = rays || r1 > lists.r0[k];
+  for (int k = 0; k < lists.count; k++) rays = rays || lists.r1[k] > lists.r0[k];
   if (rays) {
     // LDS traversal stack depth (entries); a ray that overflows it is redone
     // by the brute-force kernel, so the depth only trades occupancy
     if (stack <= 1024)
       hipLaunchKernelGGL(ray_wave_kernel<1024>, dim3(kRayWaves), dim3(64), 0, s, g, pts, tri,
-                         bvh, rp, lists, r1, bits, fb, d_counts, fb_cap, err);
+                         bvh, rp, lists, fb, d_counts, fb_cap, err);
     else
       hipLaunchKernelGGL(ray_wave_kernel<2048>, dim3(kRayWaves), dim3(64), 0, s, g, pts, tri,
-                         bvh, rp, lists, r1, bits, fb, d_counts, fb_cap, err);
+                         bvh, rp, lists, fb, d_counts, fb_cap, err);
     hipLaunchKernelGGL(ray_bits_brute_kernel, dim3(256), dim3(256), 0, s, g, pts, tri, nel, rp,
-                       lists, (const uint64_t *)fb, fb_cap, (const uint32_t *)d_counts, bits,
-                       err);
+                       lists, (const uint64_t *)fb, fb_cap, (const uint32_t *)d_counts, err);
     SDF_HIP(hipGetLastError());
   }
-  uint64_t *wind = ar.get<uint64_t>(ar.tmp("res_wind"), slots);
-  hipLaunchKernelGGL(vote_kernel, dim3(256), dim3(256), 0, s, rp, lists, bits,
-                     final_phase ? kNumDirs : r1, out, next, nnext ? nnext : d_counts + 1,
-                     next_cap, wind, d_counts + 2, err);
+  hipLaunchKernelGGL(vote_kernel, dim3(256), dim3(256), 0, s, rp, lists, out, next, nnext,
+                     next_cap, wind, nwind, wind_cap, err);
   SDF_HIP(hipGetLastError());
-  // a non-final phase decides only items with confidence >= 0.85, which
-  // never need the winding fallback
-  if (final_phase && rp.use_winding && wind_cap > 0) {
-    int64_t chunk = 0;
-    const int chunks = winding_chunks(nel, &chunk);
-    if (!wind_ahead) ahead_rows = 0;
-    // the items whose sums were not computed ahead (slot >= ahead_rows)
-    double *partial = ar.get<double>(ar.tmp("res_wpartial"), (size_t)wind_cap * chunks);
-    hipLaunchKernelGGL(winding_partial_kernel, dim3(1024), dim3(256), 0, s, g, pts, tri, nel,
-                       lists, (const uint64_t *)wind, wind_cap,
-                       (const uint32_t *)(d_counts + 2), chunks, chunk, partial, ahead_rows);
-    // the others: computed beside this phase's rays
-    if (wind_ahead) SDF_HIP(hipStreamWaitEvent(s, wind_ahead_done, 0));
-    hipLaunchKernelGGL(winding_final_kernel, dim3(wind_cap < 1024u ? wind_cap : 1024u), dim3(64),
-                       0, s, lists,
-                       (const uint64_t *)wind, wind_cap, (const uint32_t *)(d_counts + 2),
-                       chunks, (const double *)partial, wind_ahead, ahead_rows, out);
-    SDF_HIP(hipGetLastError());
-  }
+}
+
+void resolve_winding(hipStream_t s, Arena &ar, const GridP &g, const double *pts,
+                     const Tri64 *tri, int64_t nel, const WindItem *wind,
+                     const uint32_t *nwind, uint32_t wind_cap, const double *ahead,
+                     FusedOut out) {
+  if (wind_cap == 0) return;
+  int64_t chunk = 0;
+  const int chunks = winding_chunks(nel, &chunk);
+  double *partial = ar.get<double>(ar.tmp("res_wpartial"), (size_t)wind_cap * chunks);
+  hipLaunchKernelGGL(winding_partial_kernel, dim3(1024), dim3(256), 0, s, g, pts, tri, nel,
+                     wind, nwind, wind_cap, chunks, chunk, partial);
+  hipLaunchKernelGGL(winding_final_kernel, dim3(wind_cap < 1024u ? wind_cap : 1024u), dim3(64),
+                     0, s, wind, nwind, wind_cap, chunks, (const double *)partial, ahead, out);
+  SDF_HIP(hipGetLastError());
 }
 
 int winding_chunks(int64_t nel, int64_t *chunk) {
@@ -1481,15 +1498,15 @@ int winding_chunks(int64_t nel, int64_t *chunk) {
 }
 
 void launch_winding_ahead(hipStream_t s, const GridP &g, const double *pts, const Tri64 *tri,
-                          int64_t nel, const PendLists &lists, uint32_t items_cap,
-                          const uint32_t *count_dev, double *partial) {
+                          int64_t nel, const Pending *items, const uint32_t *count_dev,
+                          uint32_t rows, uint32_t base, double *ahead) {
+  if (rows == 0) return;
   int64_t chunk = 0;
   const int chunks = winding_chunks(nel, &chunk);
-  // a quarter of the post-vote pass's grid: it runs beside the last
-  // phase's rays, which it should not crowd
-  hipLaunchKernelGGL(winding_partial_kernel, dim3(256), dim3(256), 0, s, g, pts, tri, nel,
-                     lists, (const uint64_t *)nullptr, items_cap, count_dev, chunks, chunk,
-                     partial, (uint64_t)0);
+  // a quarter of the post-vote pass's grid: it runs beside the vote's rays,
+  // which it should not crowd
+  hipLaunchKernelGGL(winding_ahead_kernel, dim3(256), dim3(256), 0, s, g, pts, tri, nel, items,
+                     count_dev, rows, base, chunks, chunk, ahead);
   SDF_HIP(hipGetLastError());
 }
 
