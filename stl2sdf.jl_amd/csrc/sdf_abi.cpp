@@ -147,6 +147,7 @@ struct sdf_ctx {
   sdf::Tri64 *tri = nullptr;
   int64_t nel = 0, nnp = 0;
   bool mesh_ready = false;
+  bool mesh_word_zero = false;  // the mesh error word was cleared by the last reset
   sdf::Bvh bvh;
   sdf::LineBuild lb;  // the line chain between its counting and emit passes
   bool bvh_ready = false;  // the ray hierarchy of the current mesh is built
@@ -164,7 +165,10 @@ struct sdf_ctx {
     uint32_t wind = 1024;
   } caps;
   uint64_t *d_status = nullptr;  // sizes of the last lattice call (16 words)
-  bool own_status = true;        // d_status allocated by this context
+  bool own_status = true;        // d_status is this context's own h_status
+  // pinned, coherent host block the status kernel writes directly (its
+  // device view is d_status while own_status)
+  uint64_t *h_status = nullptr;
   // tunables: triangles per distance work item, LDS ray-stack depth,
   // workgroups per CU of the distance and sign kernels (0: one per item).
   // The distance kernel's LDS allows 16 workgroups per CU; at 8 the line
@@ -244,7 +248,12 @@ int64_t grid_points(const sdf::GridP &g) {
 // reports it, and uploading a device mesh needs no host wait.
 constexpr int kMeshErrWord = 31;
 
-void reset_words(sdf_ctx *c) { sdf::launch_fill_u32(c->stream, c->d_words, kMeshErrWord, 0u); }
+// Clears the call's device words; with mesh, also the mesh upload's word
+// (a mesh upload that follows then issues no fill of its own).
+void reset_words(sdf_ctx *c, bool mesh = false) {
+  sdf::launch_fill_u32(c->stream, c->d_words, mesh ? kMeshErrWord + 1 : kMeshErrWord, 0u);
+  c->mesh_word_zero = mesh;
+}
 
 void check_err_flags(sdf_ctx *c) {
   uint32_t w[kMeshErrWord + 1];
@@ -263,16 +272,25 @@ void check_err_flags(sdf_ctx *c) {
 
 // Mesh from device X / IEN (layouts of TriangularMesh) into per-triangle
 // FP64 vertices held by the context.
+// A mesh upload refused after the call's reset cleared the mesh error word:
+// the current mesh's error record is gone with it, so that mesh is dropped.
+[[noreturn]] void mesh_arg_fail(sdf_ctx *c, const char *msg) {
+  if (c->mesh_word_zero) c->mesh_ready = false;
+  c->mesh_word_zero = false;
+  fail(SDF_ERR_ARG, msg);
+}
+
 void set_mesh_device(sdf_ctx *c, const double *dX, int64_t nnp,
                      const int64_t *dIEN, int64_t nel) {
-  if (nel < 0 || nnp < 0) fail(SDF_ERR_ARG, "negative mesh size");
-  if (nel > 0 && (!dX || !dIEN)) fail(SDF_ERR_ARG, "NULL mesh arrays");
-  if (nel > 0x7FFFFFFFll) fail(SDF_ERR_ARG, "too many triangles");
+  if (nel < 0 || nnp < 0) mesh_arg_fail(c, "negative mesh size");
+  if (nel > 0 && (!dX || !dIEN)) mesh_arg_fail(c, "NULL mesh arrays");
+  if (nel > 0x7FFFFFFFll) mesh_arg_fail(c, "too many triangles");
   c->tri = c->ar.get<sdf::Tri64>("tri", nel + 1);
   c->nel = nel;
   c->nnp = nnp;
   c->mark("gather");
-  sdf::launch_fill_u32(c->stream, c->d_words + kMeshErrWord, 1, 0u);
+  if (!c->mesh_word_zero) sdf::launch_fill_u32(c->stream, c->d_words + kMeshErrWord, 1, 0u);
+  c->mesh_word_zero = false;
   sdf::launch_gather_tris(c->stream, dX, nnp, dIEN, nel, c->tri, c->d_words + kMeshErrWord);
   c->mesh_ready = true;
   c->bvh_ready = false;
@@ -409,8 +427,8 @@ void resolve_items(sdf_ctx *c, const sdf::GridP &g, const double *d_pts,
 
 void set_mesh_host(sdf_ctx *c, const double *X, int64_t nnp, const int64_t *IEN,
                    int64_t nel) {
-  if (nel < 0 || nnp < 0) fail(SDF_ERR_ARG, "negative mesh size");
-  if (nel > 0 && (!X || !IEN)) fail(SDF_ERR_ARG, "NULL mesh arrays");
+  if (nel < 0 || nnp < 0) mesh_arg_fail(c, "negative mesh size");
+  if (nel > 0 && (!X || !IEN)) mesh_arg_fail(c, "NULL mesh arrays");
   double *dX = c->ar.get<double>("in_X", 3 * nnp + 3);
   int64_t *dI = c->ar.get<int64_t>("in_IEN", 3 * nel + 3);
   if (nnp > 0)
@@ -691,7 +709,16 @@ bool lattice_settle(sdf_ctx *c, LatticeRun &L, const uint64_t *st) {
   return false;
 }
 
+// The status block of the last pass: the context's own block is pinned host
+// memory the status kernel wrote directly (no copy); a batch lane's lives in
+// the batch's device block.
 void read_status(sdf_ctx *c, uint64_t *st) {
+  if (c->own_status) {
+    SDF_HIP(hipStreamSynchronize(c->stream));
+    const volatile uint64_t *h = c->h_status;
+    for (int i = 0; i < 16; i++) st[i] = h[i];
+    return;
+  }
   SDF_HIP(hipMemcpyAsync(st, c->d_status, 16 * sizeof(uint64_t), hipMemcpyDeviceToHost,
                          c->stream));
   SDF_HIP(hipStreamSynchronize(c->stream));
@@ -753,10 +780,11 @@ bool is_lattice(const sdf::GridP &g, const double *pts, int64_t npts) {
   return true;
 }
 
-void begin_call(sdf_ctx *c) {
+// mesh: the call uploads a mesh next (its error word is cleared here too)
+void begin_call(sdf_ctx *c, bool mesh = false) {
   use_device(c);
   c->nev = 0;
-  reset_words(c);
+  reset_words(c, mesh);
 }
 
 
@@ -892,7 +920,9 @@ void ctx_init(sdf_ctx *c, int32_t device) {
   SDF_HIP(hipEventCreateWithFlags(&c->ev_plan, hipEventDisableTiming));
   for (int q = 0; q < 2; q++) SDF_HIP(hipEventCreateWithFlags(&c->ev_wind[q], hipEventDisableTiming));
   SDF_HIP(hipMalloc(&c->d_words, 256));
-  SDF_HIP(hipMalloc(&c->d_status, 16 * sizeof(uint64_t)));
+  SDF_HIP(hipHostMalloc((void **)&c->h_status, 16 * sizeof(uint64_t),
+                        hipHostMallocMapped | hipHostMallocCoherent));
+  SDF_HIP(hipHostGetDevicePointer((void **)&c->d_status, c->h_status, 0));
   SDF_HIP(hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, device));
   c->scratch = scratch_address(device);  // with `device` current
 }
@@ -914,7 +944,7 @@ void ctx_free(sdf_ctx *c) {
   for (int q = 0; q < 2; q++)
     if (c->ev_wind[q]) (void)hipEventDestroy(c->ev_wind[q]);
   if (c->d_words) (void)hipFree(c->d_words);
-  if (c->d_status && c->own_status) (void)hipFree(c->d_status);
+  if (c->h_status) (void)hipHostFree(c->h_status);
   if (c->own) (void)hipStreamDestroy(c->own);
   delete c;
 }
@@ -981,7 +1011,7 @@ void batch_issue(sdf_batch *b, int i) {
   k->ray_stack = b->tune.ray_stack;
   k->d_status = b->d_status + 16 * (int64_t)i;
   k->nev = 0;
-  reset_words(k);
+  reset_words(k, true);
   set_mesh_device(k, m.dX, m.nnp, m.dIEN, m.nel);
   LatticeRun &L = b->runs[(size_t)i];
   L = LatticeRun();
@@ -1003,7 +1033,7 @@ void batch_redo(sdf_batch *b, int i) {
   k->ray_stack = b->tune.ray_stack;
   k->d_status = b->d_status + 16 * (int64_t)i;
   k->nev = 0;
-  reset_words(k);
+  reset_words(k, true);
   set_mesh_device(k, m.dX, m.nnp, m.dIEN, m.nel);
   sdf::FusedOut out;
   out.sdf = m.out;
@@ -1548,7 +1578,7 @@ sdf_status eval_distances(sdf_ctx *c, const double *X, int64_t nnp, const int64_
     if (npts < 0) fail(SDF_ERR_ARG, "npts < 0");
     if (npts > 0 && !dist_out) fail(SDF_ERR_ARG, "dist_out is NULL");
     if (want_xp && npts > 0 && !xp_out) fail(SDF_ERR_ARG, "xp_out is NULL");
-    begin_call(c);
+    begin_call(c, true);
     set_mesh_host(c, X, nnp, IEN, nel);
     double *d_dist = c->ar.get<double>("out_dist", npts + 1);
     const bool lattice = is_lattice(g, points, npts);
@@ -1632,7 +1662,7 @@ sdf_status sdf_raycast_signs(sdf_ctx *c, const double *X, int64_t nnp,
     if (npts > 0 && (!signs_out || !conf_out))
       fail(SDF_ERR_ARG, "signs_out / conf_out is NULL");
     sdf::RayParams rp = ray_params(min_rays, max_rays, conf_thr, use_winding);
-    begin_call(c);
+    begin_call(c, true);
     set_mesh_host(c, X, nnp, IEN, nel);
     double *d_s = c->ar.get<double>("out_signs", npts + 1);
     double *d_c = c->ar.get<double>("out_conf", npts + 1);
@@ -1693,7 +1723,7 @@ sdf_status sdf_signed_distance(sdf_ctx *c, const double *X, int64_t nnp,
     sdf::GridP g = make_grid(aabb_min, h, N);
     int64_t npts = grid_points(g);
     if (npts > 0 && !sdf_out) fail(SDF_ERR_ARG, "sdf_out is NULL");
-    begin_call(c);
+    begin_call(c, true);
     set_mesh_host(c, X, nnp, IEN, nel);
     double *d_out = c->ar.get<double>("out_sdf", npts + 1);
     sdf::FusedOut out;
@@ -1711,7 +1741,7 @@ sdf_status sdf_mesh_set_device(sdf_ctx *c, const double *d_X, int64_t nnp,
   return guarded([&] {
     if (!c) fail(SDF_ERR_ARG, "ctx is NULL");
     std::lock_guard<std::mutex> lk(c->mu);
-    begin_call(c);
+    begin_call(c, true);
     set_mesh_device(c, d_X, nnp, d_IEN, nel);
   });
 }
@@ -1931,7 +1961,7 @@ sdf_status sdf_learned_signed_distance(sdf_ctx *c, const double *X, int64_t nnp,
     sdf::GridP g = make_grid(aabb_min, h, N);
     int64_t npts = grid_points(g);
     if (npts > 0 && !sdf_out) fail(SDF_ERR_ARG, "sdf_out is NULL");
-    begin_call(c);
+    begin_call(c, true);
     set_mesh_host(c, X, nnp, IEN, nel);
     double *d_out = c->ar.get<double>("out_sdf", npts + 1);
     sdf::FusedOut out;
@@ -1979,7 +2009,7 @@ sdf_status sdf_mesh_set(sdf_ctx *c, const double *X, int64_t nnp, const int64_t 
   return guarded([&] {
     if (!c) fail(SDF_ERR_ARG, "ctx is NULL");
     std::lock_guard<std::mutex> lk(c->mu);
-    begin_call(c);
+    begin_call(c, true);
     set_mesh_host(c, X, nnp, IEN, nel);
     SDF_HIP(hipStreamSynchronize(c->stream));
     check_err_flags(c);
@@ -2025,8 +2055,7 @@ sdf_status sdf_batch_create(sdf_ctx *c, int32_t nmesh, const double *const *d_X,
         sdf_ctx *k = new sdf_ctx();
         b->lanes.push_back(k);
         ctx_init(k, c->device);
-        SDF_HIP(hipFree(k->d_status));  // statuses go to the batch's block
-        k->d_status = nullptr;
+        k->d_status = nullptr;  // statuses go to the batch's block
         k->own_status = false;
       }
       if (autotune) batch_autotune(b);
