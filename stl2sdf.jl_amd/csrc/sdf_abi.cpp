@@ -171,13 +171,15 @@ struct sdf_ctx {
   uint64_t *h_status = nullptr;
   // tunables: triangles per distance work item, LDS ray-stack depth,
   // workgroups per CU of the distance and sign kernels (0: one per item).
-  // The distance kernel's LDS allows 16 workgroups per CU; at 8 the line
-  // hits and the LBVH build beside it get room and finish before the sign
-  // kernel needs them (measured best of 6..14 on the 1M-triangle star),
-  // while the scalar-bound sign kernel needs all its waves.
+  // The pair-window distance kernel's LDS allows 12 workgroups per CU; at 6
+  // the line hits and the LBVH build beside it get room and finish when the
+  // distances do (1M-triangle star: 4..10 measured; at 10 the distances take
+  // 0.30 ms and the sign stage waits 0.16 ms more for the line hits), while
+  // the scalar-bound sign kernel needs all its waves.  chunk_len applies to
+  // the learned radius's work items.
   int chunk_len = 64;
   int ray_stack = 1024;
-  int dist_per_cu = 10, fused_per_cu = 0;
+  int dist_per_cu = 6, fused_per_cu = 0;
   int lines_early = 0;  // 1: the line chain starts with the tile chain; 2: its count pass
   int line_per_cu = 0;  // workgroups per CU of the line kernels (0: no cap)
   int lbvh_early = 0;   // 1: the LBVH build starts with the tile chain
@@ -509,12 +511,19 @@ void stage_bins_tiles(sdf_ctx *c, LatticeRun &L) {
   if (L.do_dist) {
     sdf::TileBuild tb;
     sdf::tile_bins_begin(c->stream, c->ar, c->tri, c->nel, g, g.h, tb);
-    sdf::tile_bins_finish(c->stream, c->ar, c->tri, c->nel, g, tb, bins, c->caps.pairs);
-    // the distance buffer, with a spare slot of +inf after the last work item
-    // (the sign kernel reads it for tiles without distance work, so its
-    // loads are unconditional)
-    sdf::build_dist_chunks(c->stream, c->ar, g, bins, c->caps.chunks, c->chunk_len,
-                           &L.d2buf);
+    // the distance buffer, with a spare slot of +inf (the sign kernel reads
+    // it for tiles without distance work, so its loads are unconditional):
+    // by tile for the pair windows of the plain path, by work item for the
+    // learned radius's chunks
+    float *d2tile = nullptr;
+    if (!L.learned)
+      d2tile = c->ar.get<float>("d2tile", (size_t)(sdf::tile_dims(g).ntiles + 1) * sdf::kTilePts);
+    sdf::tile_bins_finish(c->stream, c->ar, c->tri, c->nel, g, tb, bins, c->caps.pairs, d2tile);
+    if (L.learned)
+      sdf::build_dist_chunks(c->stream, c->ar, g, bins, c->caps.chunks, c->chunk_len,
+                             &L.d2buf);
+    else
+      L.d2buf = d2tile;
   }
   // the helper chains start when the tile chain is done: beside the
   // distance kernel (whose grid leaves room on every CU) they cost little,
@@ -603,8 +612,7 @@ void stage_dist(sdf_ctx *c, LatticeRun &L) {
     sdf::launch_learned_fallback(c->stream, g, c->tri, c->bvh, bins, list, c->lstats, slack,
                                  L.d2buf, c->d_words);
   } else {
-    sdf::launch_dist_chunks(c->stream, g, c->tri, bins, L.d2buf, nullptr,
-                            (int64_t)c->dist_per_cu * c->ncu);
+    sdf::launch_dist_pairs(c->stream, g, c->tri, bins, L.d2buf, c->dist_per_cu, c->ncu);
   }
 }
 
@@ -683,7 +691,7 @@ bool lattice_settle(sdf_ctx *c, LatticeRun &L, const uint64_t *st) {
     if (cap < want && ((int64_t)need > cap || cap < 0)) cap = want;
   };
   grow64(c->caps.pairs, st[kStPairs], L.do_dist && c->nel > 0);
-  grow64(c->caps.chunks, st[kStChunks], L.do_dist);
+  grow64(c->caps.chunks, st[kStChunks], L.do_dist && L.learned);
   grow64(c->caps.hits, st[kStHits], L.do_sign);
   const int pst[sdf::kPendLists] = {kStPend0, kStPend, kStPend2};
   for (int k = 0; k < sdf::kPendLists; k++) {

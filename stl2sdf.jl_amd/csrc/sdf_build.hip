@@ -384,15 +384,31 @@ __global__ __launch_bounds__(256) void tile_big_kernel(
 
 // Run bounds of every non-empty tile from the sorted tile keys: the first
 // pair of a run writes its begin, the last its end (one coalesced pass; the
-// bounds of empty tiles stay 0).
+// bounds of empty tiles stay 0).  With d2 (distances by tile, ntiles + 1
+// slots): the spare slot and the slot of every tile whose run reaches past
+// the next window boundary (a multiple of kTilePts pairs), which several
+// windows combine by atomicMin, are set to +inf.
 __global__ void tile_bounds_kernel(const uint32_t *__restrict__ keys, int64_t n,
                                    const uint64_t *__restrict__ n_dev,
-                                   uint64_t *__restrict__ begin, uint64_t *__restrict__ end) {
+                                   uint64_t *__restrict__ begin, uint64_t *__restrict__ end,
+                                   float *__restrict__ d2, int64_t ntiles) {
+  const float4 inf4 = make_float4(__builtin_huge_valf(), __builtin_huge_valf(),
+                                  __builtin_huge_valf(), __builtin_huge_valf());
+  if (d2 && blockIdx.x == 0 && threadIdx.x < kTilePts / 4)
+    reinterpret_cast<float4 *>(d2 + ntiles * kTilePts)[threadIdx.x] = inf4;
   if (n_dev && (int64_t)*n_dev < n) n = (int64_t)*n_dev;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     const uint32_t k = keys[i];
-    if (i == 0 || keys[i - 1] != k) begin[k] = (uint64_t)i;
+    if (i == 0 || keys[i - 1] != k) {
+      begin[k] = (uint64_t)i;
+      const int64_t wb = (i | (kTilePts - 1)) + 1;  // the next window boundary
+      if (d2 && wb < n && keys[wb] == k) {
+        float4 *slot = reinterpret_cast<float4 *>(d2 + (int64_t)k * kTilePts);
+#pragma unroll
+        for (int q = 0; q < kTilePts / 4; q++) slot[q] = inf4;
+      }
+    }
     if (i == n - 1 || keys[i + 1] != k) end[k] = (uint64_t)(i + 1);
   }
 }
@@ -997,7 +1013,7 @@ void tile_bins_begin(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
 }
 
 void tile_bins_finish(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
-                      const GridP &g, TileBuild &tb, Bins &b, int64_t cap) {
+                      const GridP &g, TileBuild &tb, Bins &b, int64_t cap, float *d2tile) {
   TileDims td = tile_dims(g);
   uint64_t *tile_off = tb.bounds, *tile_end = tb.bounds + td.ntiles + 1;
   const uint64_t *n_dev = nel > 0 ? tb.toff + nel : nullptr;
@@ -1020,14 +1036,19 @@ void tile_bins_finish(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
     radix_sort_u32_pairs(s, ar, keys, tile_tris, cap, bits, n_dev, &keys, &tile_tris);
     ar.scope = scope;
     hipLaunchKernelGGL(tile_bounds_kernel, dim3(grid_for(cap, 256)), dim3(256), 0, s,
-                       (const uint32_t *)keys, cap, n_dev, tile_off, tile_end);
+                       (const uint32_t *)keys, cap, n_dev, tile_off, tile_end, d2tile,
+                       td.ntiles);
     SDF_HIP(hipGetLastError());
-  } else if (nel <= 0) {
-    launch_fill_u64(s, tb.bounds, 2 * (td.ntiles + 1), 0ull);  // no counting pass ran
+  } else {
+    if (nel <= 0) launch_fill_u64(s, tb.bounds, 2 * (td.ntiles + 1), 0ull);  // no counting pass
+    if (d2tile)  // the spare slot
+      launch_fill_u32(s, (uint32_t *)d2tile + td.ntiles * kTilePts, kTilePts, 0x7F800000u);
   }
   b.tile_off = tile_off;
   b.tile_end = tile_end;
   b.tile_tris = tile_tris;
+  b.tile_keys = keys;
+  b.d2_by_tile = d2tile != nullptr;
   b.ntile_pairs = cap;
   b.npairs_dev = n_dev;
 }

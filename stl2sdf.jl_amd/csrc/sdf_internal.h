@@ -156,6 +156,10 @@ struct Bins {
   uint64_t *tile_off = nullptr;  // per tile: first pair of its run (0 when empty)
   uint64_t *tile_end = nullptr;  // per tile: end of its run (0 when empty)
   uint32_t *tile_tris = nullptr;
+  const uint32_t *tile_keys = nullptr;  // the sorted pairs' tile ids
+  // distances by tile (d2buf slot = tile id, spare slot ntiles; the pair
+  // windows of launch_dist_pairs) rather than by work item
+  bool d2_by_tile = false;
   // axis-line hits (signs)
   uint64_t *line_off = nullptr;  // nlines + 1
   LineHit *line_hits = nullptr;
@@ -203,6 +207,10 @@ struct DistJob {
   float *d2buf;
   int32_t clen;
   int32_t pad_;
+  // pair windows: the sorted pairs (tile_keys, tile_tris), npairs capacity
+  const uint32_t *keys;
+  int64_t npairs;
+  const uint64_t *npairs_dev;
 };
 
 // Sign / combine work of one lattice: 8 x 8 x kSignZ bricks.
@@ -222,6 +230,7 @@ struct FusedJob {
   TileDims td;
   TileDims bd;  // brick counts
   const uint64_t *chunk_off;
+  const uint64_t *tile_end;  // distances by tile (non-null): a tile has work when its run ends > 0
   const float *d2buf;
   const LineInline *inl;
   const BrickPlan *plan;     // per launch position (null: decoded in the kernel)
@@ -264,8 +273,12 @@ struct LineBuild {
 void tile_bins_begin(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
                      const GridP &g, double radius, TileBuild &tb);
 // cap < 0: size the lists exactly (reads the count back: host wait).
+// d2tile (nullable): the distance buffer by tile ((ntiles + 1) * kTilePts
+// floats); its spare slot and the slots of tiles whose runs cross a pair
+// window boundary are set to +inf (launch_dist_pairs).
 void tile_bins_finish(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
-                      const GridP &g, TileBuild &tb, Bins &b, int64_t cap = -1);
+                      const GridP &g, TileBuild &tb, Bins &b, int64_t cap = -1,
+                      float *d2tile = nullptr);
 void line_hits_begin(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
                      const GridP &g, LineBuild &lb, Bins &b);
 void line_hits_finish(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
@@ -295,6 +308,12 @@ void launch_dist_chunks(hipStream_t s, const GridP &g, const Tri64 *tri,
 void launch_dist_chunks_learned(hipStream_t s, const GridP &g, const Tri64 *tri,
                                 const Bins &b, const float *r2buf, float *d2buf,
                                 uint64_t *list, unsigned long long *stats);
+// Squared distances of every non-empty tile's points from the sorted pairs
+// in windows of kTilePts consecutive pairs (d2buf by tile, prepared by
+// tile_bins_finish).  Grid: per_cu workgroups per CU (at most what the LDS
+// allows; <= 0: that many) on ncu CUs.
+void launch_dist_pairs(hipStream_t s, const GridP &g, const Tri64 *tri, const Bins &b,
+                       float *d2buf, int per_cu, int ncu);
 struct Bvh;
 // Exact distances of the listed nodes (count in stats[0]) over the LBVH with
 // the tile loop's arithmetic; overwrites their d2buf slots.

@@ -236,6 +236,179 @@ __global__ __launch_bounds__(kTilePts) void dist_chunks_kernel(DistJob J) {
     dist_item(J, it, S, true);
 }
 
+// Pair windows.  The sorted (tile, triangle) pairs are cut into windows of
+// kTilePts consecutive pairs, one per workgroup round.  Phase 1 stages every
+// pair of the window at once, one per lane (a tile's list averages a few
+// triangles, so a wave per tile would leave most lanes idle here): the
+// triangle record relative to the pair's own tile origin and the row test of
+// dist_item.  The window holds the runs of one or more tiles; phase 2 and
+// the minima then take one run at a time, with dist_item's jobs and
+// arithmetic, so a point's value is the one dist_item gives.  A run inside
+// the window stores its tile's 64 minima; a run cut by a window boundary
+// combines them by atomicMin into the slot tile_bins_finish set to +inf.
+struct PairShared {
+  float pre[kTriPreFloats][kTilePts];
+  uint16_t job[kTileRows * kTilePts];  // row << 8 | lane, run by run
+  uint32_t best[kTilePts];             // squared distances as ordered bits
+  // per run, from its first lane: the points' coordinates relative to the
+  // tile origin (x of the 8 columns, y of the 4 rows, z of the 2 layers)
+  float pc[kTilePts][kTileX + kTileY + kTileZ];
+};
+
+__device__ inline void dist_window(const DistJob &J, int64_t w, int64_t n, PairShared &S) {
+  const GridP &g = J.g;
+  const TileDims &td = J.td;
+  const int lane = threadIdx.x;
+  const int64_t q0 = w * kTilePts;
+  const int64_t q = q0 + lane;
+  const bool valid = q < n;
+  const uint32_t tile = valid ? J.keys[q] : 0xFFFFFFFFu;
+  const uint32_t tid = valid ? J.tile_tris[q] : 0u;
+  // runs cut by the window's ends (wave-uniform, scalar loads: the keys are
+  // read-only here)
+  const auto kc = kconst(J.keys);
+  const bool first_split = w > 0 && kc[q0 - 1] == kc[q0];
+  const int64_t qe = q0 + kTilePts;
+  const bool last_split = qe < n && kc[qe] == kc[qe - 1];
+  const uint32_t prev = __shfl_up(tile, 1, kTilePts);
+  const bool head = valid && (lane == 0 || prev != tile);
+  const uint64_t hm = __ballot(head);
+  const uint64_t vm = __ballot(valid);
+  const float hf = (float)g.h;
+  const float H2 = hf * hf * (1.0f + 0x1p-10f);
+  uint32_t rowbits = 0;
+  if (valid) {
+    int32_t tx, ty, tz;
+    tile_coords(tile, td, &tx, &ty, &tz);
+    const int32_t i0 = tx * kTileX, j0 = ty * kTileY, k0 = tz * kTileZ;
+    const Vec3d T0 = lattice_point(g, i0, j0, k0);
+    const Tri64 t = J.tri[tid];
+    float v[9];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      v[3 * k + 0] = (float)(t.v[3 * k + 0] - T0.x);
+      v[3 * k + 1] = (float)(t.v[3 * k + 1] - T0.y);
+      v[3 * k + 2] = (float)(t.v[3 * k + 2] - T0.z);
+    }
+    float rec[kTriPreFloats];
+    tri_pre_make(v, v + 3, v + 6, rec);
+#pragma unroll
+    for (int f = 0; f < kTriPreFloats; f++) S.pre[f][lane] = rec[f];
+    float bmin[3], bmax[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      bmin[a] = fminf(fminf(v[a], v[3 + a]), v[6 + a]);
+      bmax[a] = fmaxf(fmaxf(v[a], v[3 + a]), v[6 + a]);
+    }
+    const float x1 = (float)(lattice_coord(g.ox, g.h, i0 + kTileX - 1) - T0.x);
+    const float ex = fmaxf(fmaxf(bmin[0] - x1, -bmax[0]), 0.f);
+    const float nx = rec[21], ny = rec[22], nz = rec[23], inn = rec[24];
+    const float s0x = (0.f - rec[0]) * nx, s1x = (x1 - rec[0]) * nx;
+#pragma unroll
+    for (int zz = 0; zz < kTileZ; zz++) {
+      const float pz = (float)(lattice_coord(g.oz, g.h, k0 + zz) - T0.z);
+      const float ez = fmaxf(fmaxf(bmin[2] - pz, pz - bmax[2]), 0.f);
+      const float cz = (pz - rec[2]) * nz;
+#pragma unroll
+      for (int yy = 0; yy < kTileY; yy++) {
+        const float py = (float)(lattice_coord(g.oy, g.h, j0 + yy) - T0.y);
+        const float ey = fmaxf(fmaxf(bmin[1] - py, py - bmax[1]), 0.f);
+        const float box2 = ex * ex + ey * ey + ez * ez;
+        const float c = (py - rec[1]) * ny + cz;
+        const float s0 = s0x + c, s1 = s1x + c;
+        const float sm = ((s0 > 0.f) == (s1 > 0.f)) ? fminf(fabsf(s0), fabsf(s1)) : 0.f;
+        if (fmaxf(box2, sm * sm * inn) <= H2) rowbits |= 1u << (yy + kTileY * zz);
+      }
+    }
+    if (head) {  // the run's coordinates, formed as the row test formed them
+      float *pc = S.pc[lane];
+#pragma unroll
+      for (int x = 0; x < kTileX; x++)
+        pc[x] = (float)(lattice_coord(g.ox, g.h, i0 + x) - T0.x);
+#pragma unroll
+      for (int y = 0; y < kTileY; y++)
+        pc[kTileX + y] = (float)(lattice_coord(g.oy, g.h, j0 + y) - T0.y);
+#pragma unroll
+      for (int z = 0; z < kTileZ; z++)
+        pc[kTileX + kTileY + z] = (float)(lattice_coord(g.oz, g.h, k0 + z) - T0.z);
+    }
+  }
+  const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  // the window's job list, run by run, each run's row 0 candidates in lane
+  // order, then row 1's, ...: job (lane, r) sits after every job of the runs
+  // before (all lanes below its run's first lane la), then after its run's
+  // jobs of the rows before r, then after its run's row-r jobs of lower lanes
+  const uint64_t upto = lane == 63 ? ~0ull : (~0ull >> (63 - lane));
+  const int la_own = 63 - __builtin_clzll(hm & upto | 1ull);  // the lane's run's first lane
+  const uint64_t before_run = la_own == 0 ? 0ull : (~0ull >> (64 - la_own));
+  const uint64_t hm_above = hm & ~upto;
+  const uint64_t run_mask = (hm_above ? (hm_above & (0ull - hm_above)) - 1ull : ~0ull) & ~before_run;
+  unsigned long long mr[kTileRows];
+#pragma unroll
+  for (int r = 0; r < kTileRows; r++) mr[r] = __ballot((rowbits >> r) & 1u);
+  {
+    int pos = 0;
+#pragma unroll
+    for (int r = 0; r < kTileRows; r++) pos += __popcll(mr[r] & before_run);
+#pragma unroll
+    for (int r = 0; r < kTileRows; r++) {
+      if ((rowbits >> r) & 1u)
+        S.job[pos + __popcll(mr[r] & run_mask & below)] = (uint16_t)((r << 8) | lane);
+      pos += __popcll(mr[r] & run_mask);
+    }
+  }
+  // one run (local tile) at a time: lanes [la, lb) of the window
+  uint64_t rest = hm;
+  int jstart = 0;
+  while (rest != 0ull) {
+    const int la = __builtin_ctzll(rest);
+    rest &= rest - 1ull;
+    const int lb = rest ? __builtin_ctzll(rest) : __popcll(vm);
+    const uint64_t m_run = (lb == 64 ? ~0ull : ((1ull << lb) - 1ull)) & ~((1ull << la) - 1ull);
+    int njobs = 0;
+#pragma unroll
+    for (int r = 0; r < kTileRows; r++) njobs += __popcll(mr[r] & m_run);
+    const uint32_t rt = __shfl(tile, la, kTilePts);  // the run's tile
+    S.best[lane] = 0x7F800000u;  // +inf
+    __syncthreads();
+    const float *pc = S.pc[la];
+    const float px = pc[lane & (kTileX - 1)];
+    const float py0 = pc[kTileX], py1 = pc[kTileX + 1], py2 = pc[kTileX + 2],
+                py3 = pc[kTileX + 3];
+    const float pz0 = pc[kTileX + kTileY], pz1 = pc[kTileX + kTileY + 1];
+    for (int jq = jstart + lane / kTileX; jq < jstart + njobs; jq += kTilePts / kTileX) {
+      const uint32_t e = S.job[jq];
+      const int r = (int)(e >> 8), c = (int)(e & 63u);
+      const int ry = r & (kTileY - 1);
+      const float py = ry < 2 ? (ry == 0 ? py0 : py1) : (ry == 2 ? py2 : py3);
+      const float pz = r < kTileY ? pz0 : pz1;
+      const float d2 = tri_pre_dist2(px, py, pz, [&](int f) { return S.pre[f][c]; });
+      // squared distances are >= +0, so their bits order like the values
+      atomicMin(&S.best[r * kTileX + (lane & (kTileX - 1))], __float_as_uint(d2));
+    }
+    __syncthreads();
+    const uint32_t bv = S.best[lane];
+    float *slot = J.d2buf + (uint64_t)rt * kTilePts + lane;
+    if ((la == 0 && first_split) || (rest == 0ull && last_split))
+      atomicMin((unsigned int *)slot, bv);
+    else
+      *slot = __uint_as_float(bv);
+    jstart += njobs;
+    __syncthreads();  // S.best is reused by the next run
+  }
+  __syncthreads();  // S.pre is reused by the next window
+}
+
+// Windows of the sorted pairs, grid-stride.
+__global__ __launch_bounds__(kTilePts) void dist_pairs_kernel(DistJob J) {
+  __shared__ PairShared S;
+  int64_t n = J.npairs;
+  if (J.npairs_dev && (int64_t)*kconst(J.npairs_dev) < n) n = (int64_t)*kconst(J.npairs_dev);
+  const int64_t nw = (n + kTilePts - 1) / kTilePts;
+  for (int64_t w = xcd_remap(blockIdx.x, gridDim.x); w < nw; w += gridDim.x)
+    dist_window(J, w, n, S);
+}
+
 // Learned-radius variant of the distance item.  Lane p carries a squared
 // search radius R2(p) <= h^2 proposed by the network and evaluates only the
 // triangles whose lower bound (the larger of the squared distances to the
@@ -763,16 +936,23 @@ __global__ __launch_bounds__(256) void brick_plan_kernel(FusedJob J, BrickPlan *
       P.xy = (uint32_t)bx | (uint32_t)by << 16;
       P.z = (uint32_t)bz;
       P.flags = 2u;
-      if (J.d2buf && J.chunk_off) {
+      if (J.d2buf && (J.chunk_off || J.tile_end)) {
 #pragma unroll
         for (int t = 0; t < 4; t++) {
           const int32_t tyy = 2 * by + (t & 1), tzz = 2 * bz + (t >> 1);
           if (tyy < td.ty && tzz < td.tz) {
             const int64_t tile = ((int64_t)tzz * td.ty + tyy) * td.tx + bx;
-            const uint64_t c0 = J.chunk_off[tile], c1 = J.chunk_off[tile + 1];
-            if (c1 > c0 && (int64_t)c0 < chunk_cap) {
-              P.slot[t] = (uint32_t)c0;
-              P.flags |= 1u;
+            if (J.tile_end) {  // distances by tile: a tile with a run has its slot
+              if (J.tile_end[tile] != 0u) {
+                P.slot[t] = (uint32_t)tile;
+                P.flags |= 1u;
+              }
+            } else {
+              const uint64_t c0 = J.chunk_off[tile], c1 = J.chunk_off[tile + 1];
+              if (c1 > c0 && (int64_t)c0 < chunk_cap) {
+                P.slot[t] = (uint32_t)c0;
+                P.flags |= 1u;
+              }
             }
           }
         }
@@ -1301,6 +1481,9 @@ DistJob make_dist_job(const GridP &g, const Tri64 *tri, const Bins &b, float *d2
   J.nchunks = b.nchunks;
   J.nchunks_dev = b.nchunks_dev;
   J.only = only;
+  J.keys = b.tile_keys;
+  J.npairs = b.ntile_pairs;
+  J.npairs_dev = b.npairs_dev;
   J.d2buf = d2buf;
   J.clen = b.chunk_len;
   J.pad_ = 0;
@@ -1316,6 +1499,21 @@ void launch_dist_chunks(hipStream_t s, const GridP &g, const Tri64 *tri,
   SDF_HIP(hipGetLastError());
 }
 
+
+void launch_dist_pairs(hipStream_t s, const GridP &g, const Tri64 *tri, const Bins &b,
+                       float *d2buf, int per_cu, int ncu) {
+  if (b.ntile_pairs <= 0) return;
+  DistJob J = make_dist_job(g, tri, b, d2buf, nullptr);
+  const int64_t nw = (b.ntile_pairs + kTilePts - 1) / kTilePts;
+  // at most the workgroups the LDS lets each CU hold (in a persistent grid,
+  // workgroups that wait for a slot would start their share late)
+  const int fit = (int)(160 * 1024 / sizeof(PairShared));
+  if (per_cu <= 0 || per_cu > fit) per_cu = fit;
+  const int64_t cap = (int64_t)per_cu * (ncu > 0 ? ncu : 1);
+  const int64_t grid = nw < cap ? nw : cap;
+  hipLaunchKernelGGL(dist_pairs_kernel, dim3((unsigned)grid), dim3(kTilePts), 0, s, J);
+  SDF_HIP(hipGetLastError());
+}
 
 void launch_dist_chunks_learned(hipStream_t s, const GridP &g, const Tri64 *tri,
                                 const Bins &b, const float *r2buf, float *d2buf,
@@ -1353,7 +1551,8 @@ FusedJob make_fused_job(const GridP &g, const Bins &b, const float *d2buf, bool 
   J.bd.ty = sd.sy;
   J.bd.tz = sd.sz;
   J.bd.ntiles = (int64_t)sd.sx * sd.sy * sd.sz;
-  J.chunk_off = b.chunk_off;
+  J.chunk_off = b.d2_by_tile ? nullptr : b.chunk_off;
+  J.tile_end = b.d2_by_tile ? b.tile_end : nullptr;
   J.d2buf = d2buf;
   J.inl = b.line_inline;
   J.segw = b.seg_words;
@@ -1367,7 +1566,8 @@ FusedJob make_fused_job(const GridP &g, const Bins &b, const float *d2buf, bool 
   J.base1 = b.line_base[1];
   J.base2 = b.line_base[2];
   J.out = out;
-  J.chunk_cap = b.nchunks;
+  // the spare slot of +inf: after the last work item, or after the last tile
+  J.chunk_cap = b.d2_by_tile ? J.td.ntiles : b.nchunks;
   J.nbricks = J.td.ntiles > 0 ? brick_launch_size(J.bd) : 0;
   J.do_sign = do_sign ? 1 : 0;
   J.pad_ = 0;
