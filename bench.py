@@ -343,34 +343,45 @@ def main():
             _lib.check(L.sdf_learned_signed_distance_device(
                 ctx, _lib.dptr(amin), h, _lib.i64ptr(N), C.c_void_p(out_l.data_ptr())))
 
-        for w in range(args.warmup):
-            lstep()
+        run_names = ("fallback_nodes", "exact_evals", "tile_pairs", "node_pairs_radius",
+                     "node_pairs_h", "delta", "hc", "exact_evals_plain", "network_skipped")
+
+        def learned_run(force):
+            # force: evaluate the network even where its radius is h everywhere
+            _lib.check(L.sdf_ctx_tune(ctx, b"learned_force", int(force)))
+            for w in range(args.warmup):
+                lstep()
+                torch.cuda.synchronize()
             torch.cuda.synchronize()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            lstep()
-        torch.cuda.synchronize()
-        ldt = time.perf_counter() - t0
-        lstages = staged(L, ctx, lstep, torch)
-        lst = np.zeros(8)
-        nst = C.c_int32()
-        _lib.check(L.sdf_learned_stats(ctx, _lib.dptr(lst), 8, C.byref(nst)))
-        step()
-        torch.cuda.synchronize()
-        identical = bool(_t.equal(out, out_l))
-        lval = grid.ngp / (ldt / args.steps)
-        learned = {
-            "value": lval, "ms_per_step": ldt / args.steps * 1e3,
-            "vs_plain": lval / (grid.ngp / (dt / args.steps)),
-            "identical_to_plain": identical,
-            "stages_ms": {k: float(np.mean(v)) for k, v in lstages.items()},
-            "fit": fit,
-            "run": dict(zip(("fallback_nodes", "exact_evals", "tile_pairs",
-                             "node_pairs_radius", "node_pairs_h", "delta", "hc",
-                             "exact_evals_plain"), lst.tolist())),
-        }
-        log(f"[bench] learned: {lval:.4g} pts/s, identical={identical}")
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                lstep()
+            torch.cuda.synchronize()
+            ldt = time.perf_counter() - t0
+            lstages = staged(L, ctx, lstep, torch)
+            lst = np.zeros(len(run_names))
+            nst = C.c_int32()
+            _lib.check(L.sdf_learned_stats(ctx, _lib.dptr(lst), len(run_names), C.byref(nst)))
+            step()
+            torch.cuda.synchronize()
+            identical = bool(_t.equal(out, out_l))
+            lval = grid.ngp / (ldt / args.steps)
+            return {
+                "value": lval, "ms_per_step": ldt / args.steps * 1e3,
+                "vs_plain": lval / (grid.ngp / (dt / args.steps)),
+                "identical_to_plain": identical,
+                "stages_ms": {k: float(np.mean(v)) for k, v in lstages.items()},
+                "run": dict(zip(run_names, lst.tolist())),
+            }
+
+        learned = learned_run(False)
+        learned["fit"] = fit
+        forced = learned_run(True)
+        _lib.check(L.sdf_ctx_tune(ctx, b"learned_force", 0))
+        learned["network_evaluated"] = forced
+        log(f"[bench] learned: {learned['value']:.4g} pts/s, identical="
+            f"{learned['identical_to_plain']}, skipped={learned['run']['network_skipped']}; "
+            f"with the network {forced['value']:.4g} pts/s")
 
     batched = None if args.no_batch else batch_child()
 

@@ -295,8 +295,8 @@ sdf_status sdf_mlp_train(sdf_ctx *ctx, const float *u, const float *t, int64_t n
  * on the longest axis (spacing hc), clamped at hc, are the targets (d/hc);
  * training set = every node with d < hc plus about as many farther nodes.
  * The radius offset delta is the given quantile (<= 0: 0.99) of
- * (target - output) over the nodes with d < hc; a fine node's radius is
- * min(max((output + delta) * hc, h / 100), h).  stats_out (nullable, up to
+ * (target - max(output, 0)) over the nodes with d < hc; a fine node's radius
+ * is min(max((max(output, 0) + delta) * hc, h / 100), h).  stats_out (nullable, up to
  * 9): final loss, mean absolute error (units of hc), delta, samples, samples
  * with d < hc, training ms, coarse-distance ms, hc, steps. */
 sdf_status sdf_learned_fit(sdf_ctx *ctx, const double aabb_min[3], double h,
@@ -304,7 +304,9 @@ sdf_status sdf_learned_fit(sdf_ctx *ctx, const double aabb_min[3], double h,
                            int32_t batch, float lr, uint64_t seed, double quantile,
                            double *stats_out, int32_t stats_cap);
 /* sdf_signed_distance / sdf_signed_distance_device with the learned radius
- * (identical outputs). */
+ * (identical outputs).  When delta * hc >= h every radius is h whatever the
+ * network outputs: it could prune nothing, and the call runs the plain
+ * distance loop without evaluating it. */
 sdf_status sdf_learned_signed_distance(sdf_ctx *ctx, const double *X, int64_t nnp,
                                        const int64_t *IEN, int64_t nel,
                                        const double aabb_min[3], double h,
@@ -312,12 +314,14 @@ sdf_status sdf_learned_signed_distance(sdf_ctx *ctx, const double *X, int64_t nn
 sdf_status sdf_learned_signed_distance_device(sdf_ctx *ctx, const double aabb_min[3],
                                               double h, const int64_t N[3],
                                               double *d_sdf_out);
-/* Counters of the last learned call (synchronises), up to 8: nodes whose
+/* Counters of the last learned call (synchronises), up to 9: nodes whose
  * radius held no triangle (settled by exact LBVH queries), exact
  * evaluations (per wave: its longest candidate list, summed), (tile,
  * triangle) pairs, (node, triangle) pairs within the learned radius and
- * within h (box/plane lower bound), delta, hc, and the exact evaluations the
- * plain loop makes on the same tiles. */
+ * within h (box/plane lower bound), delta, hc, the exact evaluations the
+ * plain loop makes on the same tiles, and 1 when the call ran without the
+ * network (delta * hc >= h; the counters before it are then 0 but for the
+ * tile pairs, delta and hc). */
 sdf_status sdf_learned_stats(sdf_ctx *ctx, double *out, int32_t cap, int32_t *n);
 
 /* ------------------------------------------------------------------

@@ -201,6 +201,8 @@ struct sdf_ctx {
   double mlp_hc = 0.0;        // radius = (y + delta) * hc
   double mlp_delta = 0.0;
   unsigned long long *lstats = nullptr;  // counters of the last learned run
+  bool learned_skipped = false;  // the last learned call ran without the network
+  bool learned_force = false;    // tunable: evaluate the network even then
   int64_t lstat_pairs = 0;
 
   void mark(const char *name) {
@@ -735,6 +737,17 @@ void read_status(sdf_ctx *c, uint64_t *st) {
 void run_lattice(sdf_ctx *c, const sdf::GridP &g, bool do_dist, bool do_sign,
                  const sdf::RayParams &rp, sdf::FusedOut out, bool learned = false) {
   if (!c->mesh_ready) fail(SDF_ERR_ARG, "no mesh");
+  if (learned) {
+    // the learned radius is min(max((max(y, 0) + delta) * hc, h / 100), h):
+    // when delta * hc >= h it is h at every point whatever the network
+    // outputs, so it would prune nothing and the plain distance loop (whose
+    // radius is h) runs instead, without the network
+    c->learned_skipped = !c->learned_force && c->mlp_delta * c->mlp_hc >= g.h;
+    if (c->learned_skipped) {
+      learned = false;
+      c->lstats = nullptr;  // no counters for this call
+    }
+  }
   LatticeRun L;
   L.g = g;
   L.do_dist = do_dist;
@@ -1204,6 +1217,9 @@ sdf_status sdf_ctx_tune(sdf_ctx *c, const char *name, int64_t value) {
     } else if (n == "lines_early") {
       if (value < 0 || value > 2) fail(SDF_ERR_ARG, "lines_early must be 0, 1 or 2");
       c->lines_early = (int)value;
+    } else if (n == "learned_force") {
+      if (value != 0 && value != 1) fail(SDF_ERR_ARG, "learned_force must be 0 or 1");
+      c->learned_force = value != 0;
     } else if (n == "dist_per_cu" || n == "fused_per_cu" || n == "line_per_cu") {
       if (value < 0 || value > 64) fail(SDF_ERR_ARG, n + " must be in 0..64");
       (n == "dist_per_cu" ? c->dist_per_cu
@@ -1910,7 +1926,8 @@ sdf_status sdf_learned_fit(sdf_ctx *c, const double aabb_min[3], double h, const
     mlp_init_params(seed, p);
     mlp_upload(c, p.data());
     const double loss = mlp_train(c, smp, ns, steps, batch, lr, seed, nullptr);
-    // calibration: delta = `quantile` of (target - output) over the band
+    // calibration: delta = `quantile` of (target - output) over the band,
+    // the output clamped at 0 as the radius kernel clamps it
     float *dy = c->ar.get<float>("fit_y", ns);
     sdf::mlp_forward(c->stream, c->mlp, (const float *)smp, 4, ns, dy);
     std::vector<float> y((size_t)ns), hs((size_t)(4 * ns));
@@ -1923,7 +1940,7 @@ sdf_status sdf_learned_fit(sdf_ctx *c, const double aabb_min[3], double h, const
     double mae = 0.0;
     for (int64_t i = 0; i < ns; i++)
       if (hs[4 * i + 3] < 1.f) {
-        const double e = (double)hs[4 * i + 3] - (double)y[i];
+        const double e = (double)hs[4 * i + 3] - (double)(y[i] > 0.f ? y[i] : 0.f);
         res.push_back(e);
         mae += fabs(e);
       }
@@ -1987,7 +2004,8 @@ sdf_status sdf_learned_stats(sdf_ctx *c, double *out, int32_t cap, int32_t *n) {
     if (!c) fail(SDF_ERR_ARG, "ctx is NULL");
     std::lock_guard<std::mutex> lk(c->mu);
     use_device(c);
-    double st[8] = {0, 0, (double)c->lstat_pairs, 0, 0, c->mlp_delta, c->mlp_hc, 0};
+    double st[9] = {0, 0, (double)c->lstat_pairs, 0, 0, c->mlp_delta, c->mlp_hc, 0,
+                    c->learned_skipped ? 1.0 : 0.0};
     if (c->lstats) {
       std::vector<unsigned long long> h(kLearnedStats);
       SDF_HIP(hipMemcpyAsync(h.data(), c->lstats, sizeof(unsigned long long) * kLearnedStats,
@@ -2007,7 +2025,7 @@ sdf_status sdf_learned_stats(sdf_ctx *c, double *out, int32_t cap, int32_t *n) {
       st[7] = (double)vp;
     }
     int w = 0;
-    for (; w < 8 && w < cap && out; w++) out[w] = st[w];
+    for (; w < 9 && w < cap && out; w++) out[w] = st[w];
     if (n) *n = w;
   });
 }

@@ -264,7 +264,8 @@ __global__ __launch_bounds__(WAVES * 64) void mlp_tile_radius_kernel(
 #pragma unroll
       for (int t = 0; t < NT; t++) {
         if (q == t) {  // lane group t writes point tile pass*NT + t
-          const float r = fminf(fmaxf((y[t] + delta) * scale, 0.01f * h), h);
+          // the prediction of a distance, clamped at 0, plus the calibration
+          const float r = fminf(fmaxf((fmaxf(y[t], 0.f) + delta) * scale, 0.01f * h), h);
           r2out[first * kTilePts + 16 * (pass * NT + t) + (lane & 15)] = r * r;
         }
       }

@@ -84,7 +84,7 @@ def mlp_train(u, t, steps, batch, lr, seed=0, device=0):
 FIT_STATS = ("loss", "mae_band", "delta", "samples", "samples_band", "train_ms",
              "coarse_ms", "hc", "steps")
 RUN_STATS = ("fallback_nodes", "evals", "tile_pairs", "lane_pairs_radius",
-             "lane_pairs_h", "delta", "hc", "evals_plain")
+             "lane_pairs_h", "delta", "hc", "evals_plain", "network_skipped")
 
 
 class LearnedRadius:

@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
-from stl2sdf import Learned, SignedDistances
+from stl2sdf import Learned, SignedDistances, _lib
 from tests import _cases
 from tests.test_oracle_mlp import he_params
 
@@ -145,3 +145,30 @@ def test_learned_radius_gain_star():
     assert lane_ratio < 0.99
     assert pair_ratio <= 1.0
     assert st["fallback_nodes"] * 3 < su["fallback_nodes"]
+
+
+def test_network_skipped_when_every_radius_is_h():
+    """With delta * hc >= h every learned radius is h whatever the network
+    outputs (its prediction is clamped at 0), so the call runs the plain
+    distance loop without the network; forcing the network gives the same
+    output.  A one-step fit leaves delta large."""
+    mesh, grid = _cases.star_case(60, 45)
+    plain = SignedDistances.signed_distance(mesh, grid)
+    lr_path = Learned.LearnedRadius()
+    fs = lr_path.fit(mesh, grid, n_coarse=12, steps=1, batch=64)
+    print("fit:", fs)
+    assert fs["delta"] * fs["hc"] >= grid.cell_size
+    skipped = lr_path.signed_distance(mesh, grid)
+    st = lr_path.stats()
+    assert st["network_skipped"] == 1.0 and st["evals"] == 0.0
+    assert np.array_equal(skipped, plain)
+    ctx = _lib.ctx(0)
+    _lib.check(_lib.lib().sdf_ctx_tune(ctx, b"learned_force", 1))
+    try:
+        forced = lr_path.signed_distance(mesh, grid)
+        sf = lr_path.stats()
+    finally:
+        _lib.check(_lib.lib().sdf_ctx_tune(ctx, b"learned_force", 0))
+    print("forced:", sf)
+    assert sf["network_skipped"] == 0.0 and sf["evals"] > 0
+    assert np.array_equal(forced, plain)
