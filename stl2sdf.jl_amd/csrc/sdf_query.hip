@@ -240,15 +240,22 @@ __global__ __launch_bounds__(kTilePts) void dist_chunks_kernel(DistJob J) {
 // kTilePts consecutive pairs, one per workgroup round.  Phase 1 stages every
 // pair of the window at once, one per lane (a tile's list averages a few
 // triangles, so a wave per tile would leave most lanes idle here): the
-// triangle record relative to the pair's own tile origin and the row test of
-// dist_item.  The window holds the runs of one or more tiles; phase 2 and
-// the minima then take one run at a time, with dist_item's jobs and
-// arithmetic, so a point's value is the one dist_item gives.  A run inside
-// the window stores its tile's 64 minima; a run cut by a window boundary
-// combines them by atomicMin into the slot tile_bins_finish set to +inf.
+// triangle record relative to the pair's own tile origin and dist_item's row
+// test, applied to each half row (4 points): a triangle near a tile is
+// usually within h of only a few consecutive points of a row, so half-row
+// jobs evaluate fewer points beyond h.  The window holds the runs of one or
+// more tiles; phase 2 and the minima then take one run at a time, with
+// dist_item's arithmetic, so a point's value is the one dist_item gives
+// wherever that is below h^2 (a half row the test passes over has no point
+// within h of the triangle; at and above h^2 the output is h either way).
+// A run inside the window stores its tile's 64 minima; a run cut by a
+// window boundary combines them by atomicMin into the slot tile_bins_finish
+// set to +inf.
+constexpr int kHalfX = kTileX / 2;
+constexpr int kHalfRows = 2 * kTileRows;  // half row hr = 2 row + (x >= kHalfX)
 struct PairShared {
   float pre[kTriPreFloats][kTilePts];
-  uint16_t job[kTileRows * kTilePts];  // row << 8 | lane, run by run
+  uint16_t job[kHalfRows * kTilePts];  // half row << 8 | lane, run by run
   uint32_t best[kTilePts];             // squared distances as ordered bits
   // per run, from its first lane: the points' coordinates relative to the
   // tile origin (x of the 8 columns, y of the 4 rows, z of the 2 layers)
@@ -300,10 +307,16 @@ __device__ inline void dist_window(const DistJob &J, int64_t w, int64_t n, PairS
       bmin[a] = fminf(fminf(v[a], v[3 + a]), v[6 + a]);
       bmax[a] = fmaxf(fmaxf(v[a], v[3 + a]), v[6 + a]);
     }
+    // the half rows span x in [0, xa] and [xb, x1] (relative to the tile
+    // origin); per half, the row test's two bounds on that x range
+    const float xa = (float)(lattice_coord(g.ox, g.h, i0 + kHalfX - 1) - T0.x);
+    const float xb = (float)(lattice_coord(g.ox, g.h, i0 + kHalfX) - T0.x);
     const float x1 = (float)(lattice_coord(g.ox, g.h, i0 + kTileX - 1) - T0.x);
-    const float ex = fmaxf(fmaxf(bmin[0] - x1, -bmax[0]), 0.f);
+    const float ex0 = fmaxf(fmaxf(bmin[0] - xa, -bmax[0]), 0.f);
+    const float ex1 = fmaxf(fmaxf(bmin[0] - x1, xb - bmax[0]), 0.f);
     const float nx = rec[21], ny = rec[22], nz = rec[23], inn = rec[24];
-    const float s0x = (0.f - rec[0]) * nx, s1x = (x1 - rec[0]) * nx;
+    const float s0x = (0.f - rec[0]) * nx, sax = (xa - rec[0]) * nx;
+    const float sbx = (xb - rec[0]) * nx, s1x = (x1 - rec[0]) * nx;
 #pragma unroll
     for (int zz = 0; zz < kTileZ; zz++) {
       const float pz = (float)(lattice_coord(g.oz, g.h, k0 + zz) - T0.z);
@@ -313,11 +326,19 @@ __device__ inline void dist_window(const DistJob &J, int64_t w, int64_t n, PairS
       for (int yy = 0; yy < kTileY; yy++) {
         const float py = (float)(lattice_coord(g.oy, g.h, j0 + yy) - T0.y);
         const float ey = fmaxf(fmaxf(bmin[1] - py, py - bmax[1]), 0.f);
-        const float box2 = ex * ex + ey * ey + ez * ez;
+        const float eyz = ey * ey + ez * ez;
         const float c = (py - rec[1]) * ny + cz;
-        const float s0 = s0x + c, s1 = s1x + c;
-        const float sm = ((s0 > 0.f) == (s1 > 0.f)) ? fminf(fabsf(s0), fabsf(s1)) : 0.f;
-        if (fmaxf(box2, sm * sm * inn) <= H2) rowbits |= 1u << (yy + kTileY * zz);
+        const int r = yy + kTileY * zz;
+        {
+          const float s0 = s0x + c, s1 = sax + c;
+          const float sm = ((s0 > 0.f) == (s1 > 0.f)) ? fminf(fabsf(s0), fabsf(s1)) : 0.f;
+          if (fmaxf(ex0 * ex0 + eyz, sm * sm * inn) <= H2) rowbits |= 1u << (2 * r);
+        }
+        {
+          const float s0 = sbx + c, s1 = s1x + c;
+          const float sm = ((s0 > 0.f) == (s1 > 0.f)) ? fminf(fabsf(s0), fabsf(s1)) : 0.f;
+          if (fmaxf(ex1 * ex1 + eyz, sm * sm * inn) <= H2) rowbits |= 1u << (2 * r + 1);
+        }
       }
     }
     if (head) {  // the run's coordinates, formed as the row test formed them
@@ -343,15 +364,15 @@ __device__ inline void dist_window(const DistJob &J, int64_t w, int64_t n, PairS
   const uint64_t before_run = la_own == 0 ? 0ull : (~0ull >> (64 - la_own));
   const uint64_t hm_above = hm & ~upto;
   const uint64_t run_mask = (hm_above ? (hm_above & (0ull - hm_above)) - 1ull : ~0ull) & ~before_run;
-  unsigned long long mr[kTileRows];
+  unsigned long long mr[kHalfRows];
 #pragma unroll
-  for (int r = 0; r < kTileRows; r++) mr[r] = __ballot((rowbits >> r) & 1u);
+  for (int r = 0; r < kHalfRows; r++) mr[r] = __ballot((rowbits >> r) & 1u);
   {
     int pos = 0;
 #pragma unroll
-    for (int r = 0; r < kTileRows; r++) pos += __popcll(mr[r] & before_run);
+    for (int r = 0; r < kHalfRows; r++) pos += __popcll(mr[r] & before_run);
 #pragma unroll
-    for (int r = 0; r < kTileRows; r++) {
+    for (int r = 0; r < kHalfRows; r++) {
       if ((rowbits >> r) & 1u)
         S.job[pos + __popcll(mr[r] & run_mask & below)] = (uint16_t)((r << 8) | lane);
       pos += __popcll(mr[r] & run_mask);
@@ -367,24 +388,28 @@ __device__ inline void dist_window(const DistJob &J, int64_t w, int64_t n, PairS
     const uint64_t m_run = (lb == 64 ? ~0ull : ((1ull << lb) - 1ull)) & ~((1ull << la) - 1ull);
     int njobs = 0;
 #pragma unroll
-    for (int r = 0; r < kTileRows; r++) njobs += __popcll(mr[r] & m_run);
+    for (int r = 0; r < kHalfRows; r++) njobs += __popcll(mr[r] & m_run);
     const uint32_t rt = __shfl(tile, la, kTilePts);  // the run's tile
     S.best[lane] = 0x7F800000u;  // +inf
     __syncthreads();
     const float *pc = S.pc[la];
-    const float px = pc[lane & (kTileX - 1)];
+    const int xl = lane & (kHalfX - 1);  // the lane's point in its job's half row
+    const float px0 = pc[xl], px1 = pc[kHalfX + xl];
     const float py0 = pc[kTileX], py1 = pc[kTileX + 1], py2 = pc[kTileX + 2],
                 py3 = pc[kTileX + 3];
     const float pz0 = pc[kTileX + kTileY], pz1 = pc[kTileX + kTileY + 1];
-    for (int jq = jstart + lane / kTileX; jq < jstart + njobs; jq += kTilePts / kTileX) {
+    // job q = lane / 4 + 16 t: 4 lanes (the half row's points) per job
+    for (int jq = jstart + lane / kHalfX; jq < jstart + njobs; jq += kTilePts / kHalfX) {
       const uint32_t e = S.job[jq];
-      const int r = (int)(e >> 8), c = (int)(e & 63u);
+      const int hr = (int)(e >> 8), c = (int)(e & 63u);
+      const int r = hr >> 1, half = hr & 1;
       const int ry = r & (kTileY - 1);
+      const float px = half ? px1 : px0;
       const float py = ry < 2 ? (ry == 0 ? py0 : py1) : (ry == 2 ? py2 : py3);
       const float pz = r < kTileY ? pz0 : pz1;
       const float d2 = tri_pre_dist2(px, py, pz, [&](int f) { return S.pre[f][c]; });
       // squared distances are >= +0, so their bits order like the values
-      atomicMin(&S.best[r * kTileX + (lane & (kTileX - 1))], __float_as_uint(d2));
+      atomicMin(&S.best[r * kTileX + half * kHalfX + xl], __float_as_uint(d2));
     }
     __syncthreads();
     const uint32_t bv = S.best[lane];
