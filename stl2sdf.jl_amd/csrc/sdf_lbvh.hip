@@ -597,7 +597,9 @@ Bvh build_lbvh(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t n, const QBox
                        b.leaf_box, n, cb, codes, leaf_tri);
     SDF_HIP(hipGetLastError());
   }
-  radix_sort_pairs(s, ar, codes, leaf_tri, n, 3 * bits, nullptr);
+  // the sorted pairs stay in whichever buffers the last pass wrote (the
+  // arena scope keeps the sort's scratch pair this hierarchy's own)
+  radix_sort_pairs(s, ar, codes, leaf_tri, n, 3 * bits, nullptr, &codes, &leaf_tri);
   b.leaf_tri = leaf_tri;
   b.nc = (n + kRayLeaf - 1) / kRayLeaf;
   if (n == 1) return b;
