@@ -171,18 +171,21 @@ struct sdf_ctx {
   uint64_t *h_status = nullptr;
   // tunables: triangles per distance work item, LDS ray-stack depth,
   // workgroups per CU of the distance and sign kernels (0: one per item).
-  // The pair-window distance kernel's LDS allows 12 workgroups per CU; at 6
-  // the line hits and the LBVH build beside it get room and finish when the
-  // distances do (1M-triangle star: 4..10 measured; at 10 the distances take
-  // 0.30 ms and the sign stage waits 0.16 ms more for the line hits), while
-  // the scalar-bound sign kernel needs all its waves.  chunk_len applies to
-  // the learned radius's work items.
+  // The pair-window distance kernel's LDS allows 11 workgroups per CU; at 7
+  // the line hits beside it get room and finish about when the distances do
+  // (1M-triangle star, half-row jobs, LBVH built after the line hits: 6..8
+  // measured, 1.090 ms per step at 7 against 1.10 at 6 and 8), while the
+  // scalar-bound sign kernel needs all its waves.  chunk_len applies to the
+  // learned radius's work items.
   int chunk_len = 64;
   int ray_stack = 1024;
-  int dist_per_cu = 6, fused_per_cu = 0;
+  int dist_per_cu = 7, fused_per_cu = 0;
   int lines_early = 0;  // 1: the line chain starts with the tile chain; 2: its count pass
   int line_per_cu = 0;  // workgroups per CU of the line kernels (0: no cap)
-  int lbvh_early = 0;   // 1: the LBVH build starts with the tile chain
+  // LBVH start: 0 with the helper chains (after the tile chain), 1 with the
+  // tile chain, 2 after the line hits (beside the sign stage, which is
+  // bound by its stores: measured 10 us per step better than 0)
+  int lbvh_early = 2;
   int ncu = 256;
   // timing
   bool timing = false;
@@ -520,7 +523,10 @@ void stage_bins_tiles(sdf_ctx *c, LatticeRun &L) {
     float *d2tile = nullptr;
     if (!L.learned)
       d2tile = c->ar.get<float>("d2tile", (size_t)(sdf::tile_dims(g).ntiles + 1) * sdf::kTilePts);
-    sdf::tile_bins_finish(c->stream, c->ar, c->tri, c->nel, g, tb, bins, c->caps.pairs, d2tile);
+    // (plain path: the run bounds go to the brick plan's stream below, off
+    // the distance kernel's path)
+    sdf::tile_bins_finish(c->stream, c->ar, c->tri, c->nel, g, tb, bins, c->caps.pairs, d2tile,
+                          !L.learned);
     if (L.learned)
       sdf::build_dist_chunks(c->stream, c->ar, g, bins, c->caps.chunks, c->chunk_len,
                              &L.d2buf);
@@ -538,9 +544,11 @@ void stage_bins_tiles(sdf_ctx *c, LatticeRun &L) {
   L.plan = c->ar.get<sdf::BrickPlan>("sign_plan", sdf::brick_plan_size(g) + 1);
   if (L.do_sign) {
     SDF_HIP(hipStreamWaitEvent(c->aux[1], c->join[3], 0));
+    sdf::tile_bins_bounds(c->aux[1], g, bins);
     sdf::launch_brick_plan(c->aux[1], g, bins, L.d2buf, L.plan);
     SDF_HIP(hipEventRecord(c->ev_plan, c->aux[1]));
   } else {
+    sdf::tile_bins_bounds(c->stream, g, bins);
     sdf::launch_brick_plan(c->stream, g, bins, L.d2buf, L.plan);
   }
   // issued after the tile chain (so that chain's launches are not delayed)
@@ -577,7 +585,9 @@ void stage_bins_lines(sdf_ctx *c, LatticeRun &L, int part) {
 }
 
 void stage_bins_bvh(sdf_ctx *c, LatticeRun &L) {
-  if (L.do_sign) bvh_async(c, 1, L.g, false);  // the vote stage needs only the rays' one
+  // the vote stage needs only the rays' one; lbvh_early 2: built after the
+  // axis-line chain (beside the sign stage rather than the distance kernel)
+  if (L.do_sign) bvh_async(c, 1, L.g, false, c->lbvh_early == 2 ? 1 : 3);
 }
 
 void stage_bins(sdf_ctx *c, LatticeRun &L) {
@@ -1212,7 +1222,7 @@ sdf_status sdf_ctx_tune(sdf_ctx *c, const char *name, int64_t value) {
       if (value != 1024 && value != 2048) fail(SDF_ERR_ARG, "ray_stack must be 1024 or 2048");
       c->ray_stack = (int)value;
     } else if (n == "lbvh_early") {
-      if (value != 0 && value != 1) fail(SDF_ERR_ARG, "lbvh_early must be 0 or 1");
+      if (value < 0 || value > 2) fail(SDF_ERR_ARG, "lbvh_early must be 0, 1 or 2");
       c->lbvh_early = (int)value;
     } else if (n == "lines_early") {
       if (value < 0 || value > 2) fail(SDF_ERR_ARG, "lines_early must be 0, 1 or 2");

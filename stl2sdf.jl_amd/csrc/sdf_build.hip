@@ -413,6 +413,29 @@ __global__ void tile_bounds_kernel(const uint32_t *__restrict__ keys, int64_t n,
   }
 }
 
+// With the run bounds deferred (tile_bins_bounds on another stream): the
+// spare slot of d2 and the slots tile_bounds_kernel would set to +inf, one
+// thread per window boundary wb (a run holding pairs wb - 1 and wb).
+__global__ void split_slots_kernel(const uint32_t *__restrict__ keys, int64_t n,
+                                   const uint64_t *__restrict__ n_dev, float *__restrict__ d2,
+                                   int64_t ntiles) {
+  const float4 inf4 = make_float4(__builtin_huge_valf(), __builtin_huge_valf(),
+                                  __builtin_huge_valf(), __builtin_huge_valf());
+  if (blockIdx.x == 0 && threadIdx.x < kTilePts / 4)
+    reinterpret_cast<float4 *>(d2 + ntiles * kTilePts)[threadIdx.x] = inf4;
+  if (n_dev && (int64_t)*n_dev < n) n = (int64_t)*n_dev;
+  for (int64_t m = blockIdx.x * (int64_t)blockDim.x + threadIdx.x + 1; m * kTilePts < n;
+       m += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t wb = m * kTilePts;
+    const uint32_t k = keys[wb];
+    if (keys[wb - 1] == k) {
+      float4 *slot = reinterpret_cast<float4 *>(d2 + (int64_t)k * kTilePts);
+#pragma unroll
+      for (int q = 0; q < kTilePts / 4; q++) slot[q] = inf4;
+    }
+  }
+}
+
 // ---- axis-line hits ----
 
 struct LineRange {
@@ -1013,7 +1036,8 @@ void tile_bins_begin(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
 }
 
 void tile_bins_finish(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
-                      const GridP &g, TileBuild &tb, Bins &b, int64_t cap, float *d2tile) {
+                      const GridP &g, TileBuild &tb, Bins &b, int64_t cap, float *d2tile,
+                      bool defer_bounds) {
   TileDims td = tile_dims(g);
   uint64_t *tile_off = tb.bounds, *tile_end = tb.bounds + td.ntiles + 1;
   const uint64_t *n_dev = nel > 0 ? tb.toff + nel : nullptr;
@@ -1035,9 +1059,15 @@ void tile_bins_finish(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
     ar.scope = scope + "tile/";
     radix_sort_u32_pairs(s, ar, keys, tile_tris, cap, bits, n_dev, &keys, &tile_tris);
     ar.scope = scope;
-    hipLaunchKernelGGL(tile_bounds_kernel, dim3(grid_for(cap, 256)), dim3(256), 0, s,
-                       (const uint32_t *)keys, cap, n_dev, tile_off, tile_end, d2tile,
-                       td.ntiles);
+    if (defer_bounds && d2tile) {
+      const int64_t nwin = (cap + kTilePts - 1) / kTilePts;
+      hipLaunchKernelGGL(split_slots_kernel, dim3(grid_for(nwin, 256)), dim3(256), 0, s,
+                         (const uint32_t *)keys, cap, n_dev, d2tile, td.ntiles);
+    } else {
+      hipLaunchKernelGGL(tile_bounds_kernel, dim3(grid_for(cap, 256)), dim3(256), 0, s,
+                         (const uint32_t *)keys, cap, n_dev, tile_off, tile_end, d2tile,
+                         td.ntiles);
+    }
     SDF_HIP(hipGetLastError());
   } else {
     if (nel <= 0) launch_fill_u64(s, tb.bounds, 2 * (td.ntiles + 1), 0ull);  // no counting pass
@@ -1051,6 +1081,15 @@ void tile_bins_finish(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
   b.d2_by_tile = d2tile != nullptr;
   b.ntile_pairs = cap;
   b.npairs_dev = n_dev;
+  b.bounds_deferred = defer_bounds && d2tile && cap > 0 && nel > 0;
+}
+
+void tile_bins_bounds(hipStream_t s, const GridP &g, const Bins &b) {
+  if (!b.bounds_deferred) return;
+  hipLaunchKernelGGL(tile_bounds_kernel, dim3(grid_for(b.ntile_pairs, 256)), dim3(256), 0, s,
+                     b.tile_keys, b.ntile_pairs, b.npairs_dev, b.tile_off, b.tile_end,
+                     (float *)nullptr, tile_dims(g).ntiles);
+  SDF_HIP(hipGetLastError());
 }
 
 void build_tile_bins(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,

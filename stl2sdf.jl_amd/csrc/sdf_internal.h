@@ -160,6 +160,8 @@ struct Bins {
   // distances by tile (d2buf slot = tile id, spare slot ntiles; the pair
   // windows of launch_dist_pairs) rather than by work item
   bool d2_by_tile = false;
+  // tile_off / tile_end are left to tile_bins_bounds (d2's slots are set)
+  bool bounds_deferred = false;
   // axis-line hits (signs)
   uint64_t *line_off = nullptr;  // nlines + 1
   LineHit *line_hits = nullptr;
@@ -275,10 +277,15 @@ void tile_bins_begin(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
 // cap < 0: size the lists exactly (reads the count back: host wait).
 // d2tile (nullable): the distance buffer by tile ((ntiles + 1) * kTilePts
 // floats); its spare slot and the slots of tiles whose runs cross a pair
-// window boundary are set to +inf (launch_dist_pairs).
+// window boundary are set to +inf (launch_dist_pairs).  defer_bounds (with
+// d2tile): the run bounds (tile_off, tile_end), which the pair windows do
+// not read, are left to tile_bins_bounds, which the caller issues on a
+// stream of its choice after this one's work.
 void tile_bins_finish(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
                       const GridP &g, TileBuild &tb, Bins &b, int64_t cap = -1,
-                      float *d2tile = nullptr);
+                      float *d2tile = nullptr, bool defer_bounds = false);
+// The run bounds tile_bins_finish deferred (no-op otherwise).
+void tile_bins_bounds(hipStream_t s, const GridP &g, const Bins &b);
 void line_hits_begin(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
                      const GridP &g, LineBuild &lb, Bins &b);
 void line_hits_finish(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
