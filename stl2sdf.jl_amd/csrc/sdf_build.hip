@@ -490,23 +490,21 @@ __device__ inline int64_t line_origin(const GridP &g, int ax, int32_t a, int32_t
   return base2 + (int64_t)b * g.nx + a;
 }
 
-__device__ inline void line_visit(const Tri64 &tr, const GridP &g, int ax,
+__device__ inline void line_visit(const Tri64 &tr, uint32_t tid, const GridP &g, int ax,
                                   int32_t a, int32_t b, int64_t base1,
                                   int64_t base2, int mode,
                                   uint32_t *__restrict__ cnt,
                                   const uint64_t *__restrict__ off,
-                                  LineHit *__restrict__ hits, uint64_t cap) {
+                                  uint32_t *__restrict__ hits, uint64_t cap) {
   double o[3];
   const int64_t line = line_origin(g, ax, a, b, base1, base2, o);
-  LineHit rec;
-  if (!line_hit_test(o, ax, tr.v, tr.v + 3, tr.v + 6, mode == 0 ? nullptr : &rec))
-    return;
+  if (!line_hit_test(o, ax, tr.v, tr.v + 3, tr.v + 6, nullptr)) return;
   if (mode == 0) {
     atomicAdd(&cnt[line], 1u);
   } else {
     const uint32_t slot = atomicAdd(&cnt[line], 1u);  // cnt is the cursor here
     const uint64_t pos = off[line] + slot;
-    if (pos < cap) hits[pos] = rec;  // beyond: overflow, reported by the sizes
+    if (pos < cap) hits[pos] = tid;  // beyond: overflow, reported by the sizes
   }
 }
 
@@ -515,8 +513,9 @@ __device__ inline void line_visit(const Tri64 &tr, const GridP &g, int ax,
 // (in candidate order) keep their line ids and take their slots in `cnt` by
 // one batch of returning atomics at the end (hitn[t] = hits, kHitsMany for
 // more; lines and slots at hitl / hslot[kEmitBatch t ..]), further hits are
-// counted in `cnt2`.  The emit pass then writes the first hits' records at
-// their slots without atomics; for a triangle with more hits it tests again,
+// counted in `cnt2`.  The emit pass then writes the triangle's id at the
+// first hits' slots without atomics (line_prep_kernel forms each hit's
+// record from its triangle); for a triangle with more hits it tests again,
 // skips the first kEmitBatch hits and takes cursor slots in `cnt`, which
 // continue after every slot the counting pass gave.
 constexpr int kEmitBatch = 4;
@@ -572,25 +571,22 @@ __global__ void line_emit_small_kernel(const Tri64 *__restrict__ tri, int64_t ne
                                        const uint32_t *__restrict__ hslot,
                                        uint32_t *__restrict__ cursor,
                                        const uint64_t *__restrict__ off,
-                                       LineHit *__restrict__ hits, uint64_t cap) {
+                                       uint32_t *__restrict__ hits, uint64_t cap) {
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nel;
        t += (int64_t)gridDim.x * blockDim.x) {
     const int nh = hitn[t];
     if (nh == 0) continue;
-    const Tri64 tr = tri[t];
     const int nf = nh < kEmitBatch ? nh : kEmitBatch;  // hits with a slot
 #pragma unroll
     for (int k = 0; k < kEmitBatch; k++) {
       if (k < nf) {
         const uint32_t ln = hitl[kEmitBatch * t + k];
-        const int ax = ln < base1 ? 0 : (ln < base2 ? 1 : 2);
-        LineHit rec;
-        line_hit_record(ax, tr.v, tr.v + 3, tr.v + 6, &rec);
         const uint64_t pos = off[ln] + hslot[kEmitBatch * t + k];
-        if (pos < cap) hits[pos] = rec;  // beyond: overflow, reported by the sizes
+        if (pos < cap) hits[pos] = (uint32_t)t;  // beyond: overflow, reported by the sizes
       }
     }
     if (nh == kHitsMany) {  // the hits after the first kEmitBatch, in the same order
+      const Tri64 tr = tri[t];
       const LineRange r = tri_line_range(tr, g);
       int h = 0;
       for (int ax = 0; ax < 3; ax++)
@@ -598,11 +594,10 @@ __global__ void line_emit_small_kernel(const Tri64 *__restrict__ tri, int64_t ne
           for (int32_t a = r.lo[ax][0]; a <= r.hi[ax][0]; a++) {
             double o[3];
             const int64_t line = line_origin(g, ax, a, b, base1, base2, o);
-            LineHit rec;
-            if (!line_hit_test(o, ax, tr.v, tr.v + 3, tr.v + 6, &rec)) continue;
+            if (!line_hit_test(o, ax, tr.v, tr.v + 3, tr.v + 6, nullptr)) continue;
             if (h++ < kEmitBatch) continue;
             const uint64_t pos = off[line] + atomicAdd(&cursor[line], 1u);
-            if (pos < cap) hits[pos] = rec;
+            if (pos < cap) hits[pos] = (uint32_t)t;
           }
     }
   }
@@ -612,12 +607,13 @@ __global__ void line_bin_big_kernel(const Tri64 *__restrict__ tri, GridP g,
                                     int64_t base1, int64_t base2, int mode,
                                     uint32_t *__restrict__ cnt,
                                     const uint64_t *__restrict__ off,
-                                    LineHit *__restrict__ hits, uint64_t cap,
+                                    uint32_t *__restrict__ hits, uint64_t cap,
                                     const uint32_t *__restrict__ big,
                                     const uint32_t *nbig) {
   const uint32_t nb = *nbig;
   for (uint32_t q = blockIdx.x; q < nb; q += gridDim.x) {
-    const Tri64 tr = tri[big[q]];
+    const uint32_t tid = big[q];
+    const Tri64 tr = tri[tid];
     const LineRange r = tri_line_range(tr, g);
     const int64_t total = r.cnt[0] + r.cnt[1] + r.cnt[2];
     for (int64_t c = threadIdx.x; c < total; c += blockDim.x) {
@@ -628,7 +624,7 @@ __global__ void line_bin_big_kernel(const Tri64 *__restrict__ tri, GridP g,
         ax++;
       }
       const int64_t w0 = r.hi[ax][0] - r.lo[ax][0] + 1;
-      line_visit(tr, g, ax, r.lo[ax][0] + (int32_t)(local % w0),
+      line_visit(tr, tid, g, ax, r.lo[ax][0] + (int32_t)(local % w0),
                  r.lo[ax][1] + (int32_t)(local / w0), base1, base2, mode, cnt, off,
                  hits, cap);
     }
@@ -738,7 +734,8 @@ __device__ inline void write_seg_words(const GridP &g, const LineInline &li, int
 // eps + 2 Dmax (Dmax the line's largest delta); any gap in between makes
 // the whole line take the exact path.
 __global__ void line_prep_kernel(GridP g, const uint64_t *__restrict__ line_off,
-                                 const LineHit *__restrict__ hits, uint64_t cap,
+                                 const Tri64 *__restrict__ tri,
+                                 const uint32_t *__restrict__ hits, uint64_t cap,
                                  int64_t base1, int64_t base2, int64_t nlines,
                                  LineFast *__restrict__ fast,
                                  uint32_t *__restrict__ meta,
@@ -815,7 +812,13 @@ __global__ void line_prep_kernel(GridP g, const uint64_t *__restrict__ line_off,
     const int j = (ax + 1) % 3, k = (ax + 2) % 3;
     // crossing coordinate and certified bound of hit q
     auto summarize = [&](int64_t q, double *c_out, float *delta_out) {
-      const LineHit r = hits[hb + q];
+      // the hit's record, by line_hit_test's arithmetic (the triangle's
+      // id is all the emit pass stores)
+      LineHit r;
+      {
+        const Tri64 &tr = tri[hits[hb + q]];
+        line_hit_record(ax, tr.v, tr.v + 3, tr.v + 6, &r);
+      }
       const double t0 = line_hit_t(r, o0);
       const double c = o0[ax] + t0;
       double S[3];
@@ -1165,7 +1168,7 @@ void line_hits_begin(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
                        lb.big, lb.nbig, lb.hitn, lb.hitl, lb.hslot);
     hipLaunchKernelGGL(line_bin_big_kernel, dim3(1024), dim3(256), 0, s, tri, g, lb.nlx,
                        lb.nlx + lb.nly, 0, lb.lcnt2, (const uint64_t *)nullptr,
-                       (LineHit *)nullptr, (uint64_t)0, (const uint32_t *)lb.big,
+                       (uint32_t *)nullptr, (uint64_t)0, (const uint32_t *)lb.big,
                        (const uint32_t *)lb.nbig);
     SDF_HIP(hipGetLastError());
   }
@@ -1177,7 +1180,7 @@ void line_hits_finish(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
                       const GridP &g, LineBuild &lb, Bins &b, int64_t cap) {
   const uint64_t *n_dev = lb.loff + lb.nlines;
   if (cap < 0) cap = (int64_t)read_u64(s, lb.loff + lb.nlines);
-  LineHit *hits = ar.get<LineHit>("line_hits", cap + 1);
+  uint32_t *hits = ar.get<uint32_t>("line_hit_tri", cap + 1);
   if (cap > 0 && nel > 0) {
     // lcnt continues as the cursor of the hits without a slot
     hipLaunchKernelGGL(line_emit_small_kernel, dim3(capped_grid(nel, 256, lb.max_blocks)),
@@ -1198,14 +1201,14 @@ void line_hits_finish(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
   uint32_t *segw = ar.get<uint32_t>("line_segw", (size_t)sl.total + 1);
   hipLaunchKernelGGL(line_prep_kernel, dim3(capped_grid(lb.nlines, 256, lb.max_blocks)),
                      dim3(256), 0, s, g,
-                     (const uint64_t *)lb.loff, (const LineHit *)hits, (uint64_t)cap, lb.nlx,
+                     (const uint64_t *)lb.loff, tri, (const uint32_t *)hits, (uint64_t)cap, lb.nlx,
                      lb.nlx + lb.nly, lb.nlines, fast, meta, inl, sl, segw);
   // the sentinel read by lanes whose line serves no point (byte 1 everywhere)
   launch_fill_u32(s, segw + sl.total, 1, 0x01010101u);
   SDF_HIP(hipGetLastError());
   b.line_off = lb.loff;
   b.seg_words = segw;
-  b.line_hits = hits;
+  b.line_hit_tri = hits;
   b.line_fast = fast;
   b.line_meta = meta;
   b.line_inline = inl;

@@ -164,7 +164,7 @@ struct Bins {
   bool bounds_deferred = false;
   // axis-line hits (signs)
   uint64_t *line_off = nullptr;  // nlines + 1
-  LineHit *line_hits = nullptr;
+  uint32_t *line_hit_tri = nullptr;  // per hit: its triangle (hits grouped by line)
   int64_t line_base[4] = {0, 0, 0, 0};
   LineFast *line_fast = nullptr;   // per hit, sorted by c within each line
   uint32_t *line_meta = nullptr;   // per line: cluster count | kLineExact
