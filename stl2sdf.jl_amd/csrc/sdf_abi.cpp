@@ -182,10 +182,12 @@ struct sdf_ctx {
   int dist_per_cu = 7, fused_per_cu = 0;
   int lines_early = 0;  // 1: the line chain starts with the tile chain; 2: its count pass
   int line_per_cu = 0;  // workgroups per CU of the line kernels (0: no cap)
-  // LBVH start: 0 with the helper chains (after the tile chain), 1 with the
-  // tile chain, 2 after the line hits (beside the sign stage, which is
-  // bound by its stores: measured 10 us per step better than 0)
-  int lbvh_early = 2;
+  // LBVH build on helper stream 1, after the tile runs' bounds and the brick
+  // plan there: 0 issued after the distance kernel and the line hits, 1
+  // issued before them (its launches reach the GPU first; no change), 2 after
+  // the line hits end (beside the sign stage: not done when the vote needs
+  // it, 0.14 ms wait)
+  int lbvh_early = 0;
   int ncu = 256;
   // timing
   bool timing = false;
@@ -553,7 +555,7 @@ void stage_bins_tiles(sdf_ctx *c, LatticeRun &L) {
   }
   // issued after the tile chain (so that chain's launches are not delayed)
   // but free to start beside it
-  if (L.do_sign && c->lbvh_early) bvh_async(c, 1, L.g, L.learned, 0);
+  if (L.do_sign && c->lbvh_early == 1) bvh_async(c, 1, L.g, L.learned, 0);
   const uint32_t dflt[sdf::kPendLists] = {1u << 20, 1u << 22, 1u << 20};
   static const char *pnames[sdf::kPendLists] = {"pend0", "pend", "pend2"};
   for (int k = 0; k < sdf::kPendLists; k++) {
