@@ -610,37 +610,89 @@ __device__ inline void tri_pre_make(const float *a, const float *b,
 // Squared distance from p to the triangle: the closest point lies in the
 // face interior (all three edge functions (p - v) . (n x edge) >= 0, then
 // the plane distance) or on one of the three edges (clamped projection);
-// the minimum of those four candidates, without branches.  T(i) reads
-// field i of the precomputed record.
+// the minimum of those four candidates, without branches.  Split so that
+// points sharing y and z (a row) share the work that does not depend on x:
+// tri_pre_yz forms each dot product's y and z terms, tri_pre_dist2_x adds
+// the x term (every caller goes through these, so all kernels round alike).
+// T(i) reads field i of the precomputed record.
+struct TriYZ {
+  float pay, paz, pby, pbz, pcy, pcz;  // p - a, p - b, p - c (y, z)
+  float kwc, kwa, kwb, kpn;            // edge functions and plane: y + z terms
+  float k0, k1, k2;                    // edge projections: y + z terms
+};
+struct TriX {  // the record's fields the x part reads
+  float ax, bx, cx, wcx, wax, wbx, nx, inn;
+  float e0[3], il0, e1[3], il1, e2[3], il2;
+};
+template <typename Get>
+__device__ inline TriX tri_pre_x(Get T) {
+  TriX r;
+  r.ax = T(0);
+  r.bx = T(3);
+  r.cx = T(6);
+  r.wcx = T(25);
+  r.wax = T(28);
+  r.wbx = T(31);
+  r.nx = T(21);
+  r.inn = T(24);
+  for (int k = 0; k < 3; k++) {
+    r.e0[k] = T(9 + k);
+    r.e1[k] = T(12 + k);
+    r.e2[k] = T(15 + k);
+  }
+  r.il0 = T(18);
+  r.il1 = T(19);
+  r.il2 = T(20);
+  return r;
+}
+// (explicit fused multiply-adds, no contraction left to the compiler: every
+// kernel that inlines these rounds exactly alike)
+template <typename Get>
+__device__ inline TriYZ tri_pre_yz(float py, float pz, Get T) {
+#pragma clang fp contract(off)
+  TriYZ q;
+  q.pay = py - T(1);
+  q.paz = pz - T(2);
+  q.pby = py - T(4);
+  q.pbz = pz - T(5);
+  q.pcy = py - T(7);
+  q.pcz = pz - T(8);
+  q.kwc = __builtin_fmaf(q.pay, T(26), q.paz * T(27));
+  q.kwa = __builtin_fmaf(q.pby, T(29), q.pbz * T(30));
+  q.kwb = __builtin_fmaf(q.pcy, T(32), q.pcz * T(33));
+  q.kpn = __builtin_fmaf(q.pay, T(22), q.paz * T(23));
+  q.k0 = __builtin_fmaf(q.pay, T(10), q.paz * T(11));
+  q.k1 = __builtin_fmaf(q.pby, T(13), q.pbz * T(14));
+  q.k2 = __builtin_fmaf(q.pcy, T(16), q.pcz * T(17));
+  return q;
+}
+// distance to the segment from v (p - v given) along e, 1/|e|^2 = il
+__device__ inline float seg_dist2_x(float pvx, float pvy, float pvz, float k, const float *e,
+                                    float il) {
+#pragma clang fp contract(off)
+  const float t = fminf(fmaxf(__builtin_fmaf(pvx, e[0], k) * il, 0.f), 1.f);
+  const float rx = __builtin_fmaf(-t, e[0], pvx);
+  const float ry = __builtin_fmaf(-t, e[1], pvy);
+  const float rz = __builtin_fmaf(-t, e[2], pvz);
+  return __builtin_fmaf(rx, rx, __builtin_fmaf(ry, ry, rz * rz));
+}
+__device__ inline float tri_pre_dist2_x(float px, const TriX &r, const TriYZ &q) {
+#pragma clang fp contract(off)
+  const float pax = px - r.ax, pbx = px - r.bx, pcx = px - r.cx;
+  const float wc = __builtin_fmaf(pax, r.wcx, q.kwc);
+  const float wa = __builtin_fmaf(pbx, r.wax, q.kwa);
+  const float wb = __builtin_fmaf(pcx, r.wbx, q.kwb);
+  const float pn = __builtin_fmaf(pax, r.nx, q.kpn);
+  const bool inside = (wa >= 0.f) & (wb >= 0.f) & (wc >= 0.f) & (r.inn > 0.f);
+  const float face = inside ? (pn * pn) * r.inn : __builtin_huge_valf();
+  const float d0 = seg_dist2_x(pax, q.pay, q.paz, q.k0, r.e0, r.il0);
+  const float d1 = seg_dist2_x(pbx, q.pby, q.pbz, q.k1, r.e1, r.il1);
+  const float d2 = seg_dist2_x(pcx, q.pcy, q.pcz, q.k2, r.e2, r.il2);
+  return fminf(fminf(face, d0), fminf(d1, d2));
+}
 template <typename Get>
 __device__ inline float tri_pre_dist2(float px, float py, float pz, Get T) {
-  const float pax = px - T(0), pay = py - T(1), paz = pz - T(2);
-  const float pbx = px - T(3), pby = py - T(4), pbz = pz - T(5);
-  const float pcx = px - T(6), pcy = py - T(7), pcz = pz - T(8);
-  const float wc = pax * T(25) + pay * T(26) + paz * T(27);
-  const float wa = pbx * T(28) + pby * T(29) + pbz * T(30);
-  const float wb = pcx * T(31) + pcy * T(32) + pcz * T(33);
-  const float pn = pax * T(21) + pay * T(22) + paz * T(23);
-  const float inn = T(24);
-  const bool inside = (wa >= 0.f) & (wb >= 0.f) & (wc >= 0.f) & (inn > 0.f);
-  const float face = inside ? pn * pn * inn : __builtin_huge_valf();
-  float t, rx, ry, rz;
-  t = fminf(fmaxf((pax * T(9) + pay * T(10) + paz * T(11)) * T(18), 0.f), 1.f);
-  rx = pax - t * T(9);
-  ry = pay - t * T(10);
-  rz = paz - t * T(11);
-  const float d0 = rx * rx + ry * ry + rz * rz;
-  t = fminf(fmaxf((pbx * T(12) + pby * T(13) + pbz * T(14)) * T(19), 0.f), 1.f);
-  rx = pbx - t * T(12);
-  ry = pby - t * T(13);
-  rz = pbz - t * T(14);
-  const float d1 = rx * rx + ry * ry + rz * rz;
-  t = fminf(fmaxf((pcx * T(15) + pcy * T(16) + pcz * T(17)) * T(20), 0.f), 1.f);
-  rx = pcx - t * T(15);
-  ry = pcy - t * T(16);
-  rz = pcz - t * T(17);
-  const float d2 = rx * rx + ry * ry + rz * rz;
-  return fminf(fminf(face, d0), fminf(d1, d2));
+  return tri_pre_dist2_x(px, tri_pre_x(T), tri_pre_yz(py, pz, T));
 }
 
 // Number of work items when the host holds only a capacity and the device

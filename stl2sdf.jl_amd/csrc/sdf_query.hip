@@ -393,23 +393,28 @@ __device__ inline void dist_window(const DistJob &J, int64_t w, int64_t n, PairS
     S.best[lane] = 0x7F800000u;  // +inf
     __syncthreads();
     const float *pc = S.pc[la];
-    const int xl = lane & (kHalfX - 1);  // the lane's point in its job's half row
-    const float px0 = pc[xl], px1 = pc[kHalfX + xl];
     const float py0 = pc[kTileX], py1 = pc[kTileX + 1], py2 = pc[kTileX + 2],
                 py3 = pc[kTileX + 3];
     const float pz0 = pc[kTileX + kTileY], pz1 = pc[kTileX + kTileY + 1];
-    // job q = lane / 4 + 16 t: 4 lanes (the half row's points) per job
-    for (int jq = jstart + lane / kHalfX; jq < jstart + njobs; jq += kTilePts / kHalfX) {
+    // one job per lane: the triangle's record and the half row's y and z
+    // terms once, then its 4 points
+    for (int jq = jstart + lane; jq < jstart + njobs; jq += kTilePts) {
       const uint32_t e = S.job[jq];
       const int hr = (int)(e >> 8), c = (int)(e & 63u);
       const int r = hr >> 1, half = hr & 1;
       const int ry = r & (kTileY - 1);
-      const float px = half ? px1 : px0;
       const float py = ry < 2 ? (ry == 0 ? py0 : py1) : (ry == 2 ? py2 : py3);
       const float pz = r < kTileY ? pz0 : pz1;
-      const float d2 = tri_pre_dist2(px, py, pz, [&](int f) { return S.pre[f][c]; });
-      // squared distances are >= +0, so their bits order like the values
-      atomicMin(&S.best[r * kTileX + half * kHalfX + xl], __float_as_uint(d2));
+      auto T = [&](int f) { return S.pre[f][c]; };
+      const TriX tx = tri_pre_x(T);
+      const TriYZ yz = tri_pre_yz(py, pz, T);
+#pragma unroll
+      for (int xi = 0; xi < kHalfX; xi++) {
+        const float px = pc[half * kHalfX + xi];
+        const float d2 = tri_pre_dist2_x(px, tx, yz);
+        // squared distances are >= +0, so their bits order like the values
+        atomicMin(&S.best[r * kTileX + half * kHalfX + xi], __float_as_uint(d2));
+      }
     }
     __syncthreads();
     const uint32_t bv = S.best[lane];
