@@ -67,8 +67,9 @@ sdf_status sdf_ctx_synchronize(sdf_ctx *ctx);
  * item (1..64, default 64), "ray_stack" LDS ray-stack depth of the vote
  * stage (1024 or 2048), "dist_per_cu" / "fused_per_cu" workgroups per CU
  * of the distance / sign kernels (0: one per work item; defaults 7 / 0),
- * "lbvh_early" when the LBVH build is issued: 0 after the distance and
- * axis-line work, 1 before it, 2 after the axis-line hits end (default 0),
+ * "lbvh_early" when the LBVH build runs: 0 after the tile binning, issued
+ * after the distance and axis-line work, 1 the same, issued before it, 2
+ * after the axis-line hits end, 3 beside the tile binning (default 3),
  * "lines_early" 1: the axis-line hits start beside the tile binning
  * instead of after it, 2: only their counting pass does (default 0), "line_per_cu" workgroups per CU of the
  * axis-line kernels (0: one per triangle or line, the default). */

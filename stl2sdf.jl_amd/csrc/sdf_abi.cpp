@@ -134,7 +134,7 @@ struct sdf_ctx {
   hipStream_t stream = nullptr;
   // helper streams for independent build chains (line hits, LBVH); default
   // priority, joined to the main stream with events
-  hipStream_t aux[2] = {nullptr, nullptr};
+  hipStream_t aux[3] = {nullptr, nullptr, nullptr};
   // [0] pass start, [1] line hits done, [2] LBVH done, [3] tile chain done
   hipEvent_t join[4] = {nullptr, nullptr, nullptr, nullptr};
   hipEvent_t ev_plan = nullptr;  // the sign kernel's brick plan is written
@@ -182,12 +182,15 @@ struct sdf_ctx {
   int dist_per_cu = 7, fused_per_cu = 0;
   int lines_early = 0;  // 1: the line chain starts with the tile chain; 2: its count pass
   int line_per_cu = 0;  // workgroups per CU of the line kernels (0: no cap)
-  // LBVH build on helper stream 1, after the tile runs' bounds and the brick
-  // plan there: 0 issued after the distance kernel and the line hits, 1
-  // issued before them (its launches reach the GPU first; no change), 2 after
-  // the line hits end (beside the sign stage: not done when the vote needs
-  // it, 0.14 ms wait)
-  int lbvh_early = 0;
+  // LBVH build: 3 on helper stream 2 from the start of the pass, beside the
+  // latency-bound tile chain (which it stretches by 0.03 ms), so the
+  // distance kernel and the line hits later share the GPU with nothing
+  // else (1.045-1.056 against 1.065-1.077 ms per step for 0); on helper
+  // stream 1 after the tiles' run bounds and the brick plan: 0 issued after
+  // the distance kernel and the line hits, 1 issued before them (no change
+  // against 0), 2 after the line hits end (beside the sign stage: not done
+  // when the vote needs it, 0.14 ms wait)
+  int lbvh_early = 3;
   int ncu = 256;
   // timing
   bool timing = false;
@@ -323,6 +326,7 @@ const sdf::Bvh &need_bvh(sdf_ctx *c) {
 // grid was built around; any box gives a valid hierarchy).
 void bvh_async(sdf_ctx *c, int q, const sdf::GridP &g, bool full, int after = 3) {
   if (c->bvh_ready && (c->bvh_full || !full)) return;
+  if (!c->aux[q]) SDF_HIP(hipStreamCreateWithFlags(&c->aux[q], hipStreamNonBlocking));
   SDF_HIP(hipStreamWaitEvent(c->aux[q], c->join[after], 0));
   c->ar.scope = "bvh/";
   sdf::QBox qb;
@@ -556,6 +560,8 @@ void stage_bins_tiles(sdf_ctx *c, LatticeRun &L) {
   // issued after the tile chain (so that chain's launches are not delayed)
   // but free to start beside it
   if (L.do_sign && c->lbvh_early == 1) bvh_async(c, 1, L.g, L.learned, 0);
+  // (issued after the tile chain's launches, which it would delay)
+  if (L.do_sign && c->lbvh_early == 3) bvh_async(c, 2, L.g, L.learned, 0);
   const uint32_t dflt[sdf::kPendLists] = {1u << 20, 1u << 22, 1u << 20};
   static const char *pnames[sdf::kPendLists] = {"pend0", "pend", "pend2"};
   for (int k = 0; k < sdf::kPendLists; k++) {
@@ -948,6 +954,9 @@ void ctx_init(sdf_ctx *c, int32_t device) {
   SDF_HIP(hipSetDevice(device));
   SDF_HIP(hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking));
   c->stream = c->own;
+  // (helper stream 2, for the LBVH beside the tile chain, is created on
+  // first use: batch lanes never use it, and every stream shares the
+  // device's few hardware queues)
   for (int q = 0; q < 2; q++) SDF_HIP(hipStreamCreateWithFlags(&c->aux[q], hipStreamNonBlocking));
   for (int q = 0; q < 4; q++) SDF_HIP(hipEventCreateWithFlags(&c->join[q], hipEventDisableTiming));
   SDF_HIP(hipEventCreateWithFlags(&c->ev_plan, hipEventDisableTiming));
@@ -964,13 +973,13 @@ void ctx_free(sdf_ctx *c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  // the helper streams' work is done before the buffers it uses go
+  for (int q = 0; q < 3; q++)
+    if (c->aux[q]) (void)hipStreamSynchronize(c->aux[q]);
   c->ar.release();
   for (auto e : c->ev) (void)hipEventDestroy(e);
-  for (int q = 0; q < 2; q++)
-    if (c->aux[q]) {
-      (void)hipStreamSynchronize(c->aux[q]);
-      (void)hipStreamDestroy(c->aux[q]);
-    }
+  for (int q = 0; q < 3; q++)
+    if (c->aux[q]) (void)hipStreamDestroy(c->aux[q]);
   for (int q = 0; q < 4; q++)
     if (c->join[q]) (void)hipEventDestroy(c->join[q]);
   if (c->ev_plan) (void)hipEventDestroy(c->ev_plan);
@@ -1035,6 +1044,12 @@ void batch_free(sdf_batch *b) {
   delete b;
 }
 
+// LBVH start of the batch's lanes: after the tile chain, on helper stream 1
+// (the small meshes' passes already overlap across lanes, and a third
+// helper stream per lane crowds the device's hardware queues: with one
+// created in every lane, 7.5-7.6 against 9.2 G points/s).
+constexpr int kBatchLbvh = 0;
+
 // Issue mesh i's lattice pass on its lane, status into the batch's block.
 void batch_issue(sdf_batch *b, int i) {
   sdf_ctx *k = b->lanes[i % b->tune.lanes];
@@ -1042,6 +1057,7 @@ void batch_issue(sdf_batch *b, int i) {
   k->caps = m.caps;
   k->chunk_len = b->tune.chunk_len;
   k->ray_stack = b->tune.ray_stack;
+  k->lbvh_early = kBatchLbvh;
   k->d_status = b->d_status + 16 * (int64_t)i;
   k->nev = 0;
   reset_words(k, true);
@@ -1064,6 +1080,7 @@ void batch_redo(sdf_batch *b, int i) {
   k->caps = m.caps;
   k->chunk_len = b->tune.chunk_len;
   k->ray_stack = b->tune.ray_stack;
+  k->lbvh_early = kBatchLbvh;
   k->d_status = b->d_status + 16 * (int64_t)i;
   k->nev = 0;
   reset_words(k, true);
@@ -1224,7 +1241,7 @@ sdf_status sdf_ctx_tune(sdf_ctx *c, const char *name, int64_t value) {
       if (value != 1024 && value != 2048) fail(SDF_ERR_ARG, "ray_stack must be 1024 or 2048");
       c->ray_stack = (int)value;
     } else if (n == "lbvh_early") {
-      if (value < 0 || value > 2) fail(SDF_ERR_ARG, "lbvh_early must be 0, 1 or 2");
+      if (value < 0 || value > 3) fail(SDF_ERR_ARG, "lbvh_early must be in 0..3");
       c->lbvh_early = (int)value;
     } else if (n == "lines_early") {
       if (value < 0 || value > 2) fail(SDF_ERR_ARG, "lines_early must be 0, 1 or 2");
