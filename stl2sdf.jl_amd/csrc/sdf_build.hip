@@ -495,7 +495,8 @@ __device__ inline void line_visit(const Tri64 &tr, uint32_t tid, const GridP &g,
                                   int64_t base2, int mode,
                                   uint32_t *__restrict__ cnt,
                                   const uint64_t *__restrict__ off,
-                                  uint32_t *__restrict__ hits, uint64_t cap) {
+                                  uint32_t *__restrict__ hits, uint32_t *__restrict__ hline,
+                                  uint64_t cap) {
   double o[3];
   const int64_t line = line_origin(g, ax, a, b, base1, base2, o);
   if (!line_hit_test(o, ax, tr.v, tr.v + 3, tr.v + 6, nullptr)) return;
@@ -504,7 +505,10 @@ __device__ inline void line_visit(const Tri64 &tr, uint32_t tid, const GridP &g,
   } else {
     const uint32_t slot = atomicAdd(&cnt[line], 1u);  // cnt is the cursor here
     const uint64_t pos = off[line] + slot;
-    if (pos < cap) hits[pos] = tid;  // beyond: overflow, reported by the sizes
+    if (pos < cap) {  // beyond: overflow, reported by the sizes
+      hits[pos] = tid;
+      hline[pos] = (uint32_t)line;
+    }
   }
 }
 
@@ -571,7 +575,8 @@ __global__ void line_emit_small_kernel(const Tri64 *__restrict__ tri, int64_t ne
                                        const uint32_t *__restrict__ hslot,
                                        uint32_t *__restrict__ cursor,
                                        const uint64_t *__restrict__ off,
-                                       uint32_t *__restrict__ hits, uint64_t cap) {
+                                       uint32_t *__restrict__ hits,
+                                       uint32_t *__restrict__ hline, uint64_t cap) {
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < nel;
        t += (int64_t)gridDim.x * blockDim.x) {
     const int nh = hitn[t];
@@ -582,7 +587,10 @@ __global__ void line_emit_small_kernel(const Tri64 *__restrict__ tri, int64_t ne
       if (k < nf) {
         const uint32_t ln = hitl[kEmitBatch * t + k];
         const uint64_t pos = off[ln] + hslot[kEmitBatch * t + k];
-        if (pos < cap) hits[pos] = (uint32_t)t;  // beyond: overflow, reported by the sizes
+        if (pos < cap) {  // beyond: overflow, reported by the sizes
+          hits[pos] = (uint32_t)t;
+          hline[pos] = ln;
+        }
       }
     }
     if (nh == kHitsMany) {  // the hits after the first kEmitBatch, in the same order
@@ -597,7 +605,10 @@ __global__ void line_emit_small_kernel(const Tri64 *__restrict__ tri, int64_t ne
             if (!line_hit_test(o, ax, tr.v, tr.v + 3, tr.v + 6, nullptr)) continue;
             if (h++ < kEmitBatch) continue;
             const uint64_t pos = off[line] + atomicAdd(&cursor[line], 1u);
-            if (pos < cap) hits[pos] = (uint32_t)t;
+            if (pos < cap) {
+              hits[pos] = (uint32_t)t;
+              hline[pos] = (uint32_t)line;
+            }
           }
     }
   }
@@ -607,8 +618,8 @@ __global__ void line_bin_big_kernel(const Tri64 *__restrict__ tri, GridP g,
                                     int64_t base1, int64_t base2, int mode,
                                     uint32_t *__restrict__ cnt,
                                     const uint64_t *__restrict__ off,
-                                    uint32_t *__restrict__ hits, uint64_t cap,
-                                    const uint32_t *__restrict__ big,
+                                    uint32_t *__restrict__ hits, uint32_t *__restrict__ hline,
+                                    uint64_t cap, const uint32_t *__restrict__ big,
                                     const uint32_t *nbig) {
   const uint32_t nb = *nbig;
   for (uint32_t q = blockIdx.x; q < nb; q += gridDim.x) {
@@ -626,7 +637,7 @@ __global__ void line_bin_big_kernel(const Tri64 *__restrict__ tri, GridP g,
       const int64_t w0 = r.hi[ax][0] - r.lo[ax][0] + 1;
       line_visit(tr, tid, g, ax, r.lo[ax][0] + (int32_t)(local % w0),
                  r.lo[ax][1] + (int32_t)(local / w0), base1, base2, mode, cnt, off,
-                 hits, cap);
+                 hits, hline, cap);
     }
   }
 }
@@ -733,9 +744,148 @@ __device__ inline void write_seg_words(const GridP &g, const LineInline &li, int
 // c_k+1 - c_k < eps - 2 Dmax and start a new one when the gap exceeds
 // eps + 2 Dmax (Dmax the line's largest delta); any gap in between makes
 // the whole line take the exact path.
-__global__ void line_prep_kernel(GridP g, const uint64_t *__restrict__ line_off,
-                                 const Tri64 *__restrict__ tri,
-                                 const uint32_t *__restrict__ hits, uint64_t cap,
+// Geometry of axis line L: its axis, first and last lattice points, length
+// and the sign bricks' segments along it (kSegLen points each).
+struct LineGeom {
+  int ax, seglen;
+  int32_t nax;
+  int64_t nseg, lidx;
+  double o0[3], on[3], len, oa;
+  double o0a, ona;  // o0[ax], on[ax]
+};
+// v[i] by selects (a private array indexed at run time would live in scratch)
+__device__ inline double sel3(const double *v, int i) {
+  return i == 0 ? v[0] : (i == 1 ? v[1] : v[2]);
+}
+__device__ inline LineGeom line_geom(const GridP &g, int64_t L, int64_t base1, int64_t base2) {
+  LineGeom G;
+  int64_t a, b;
+  if (L < base1) {
+    G.ax = 0;
+    G.lidx = L;
+    a = L % g.ny;
+    b = L / g.ny;
+  } else if (L < base2) {
+    G.ax = 1;
+    G.lidx = L - base1;
+    a = G.lidx % g.nx;
+    b = G.lidx / g.nx;
+  } else {
+    G.ax = 2;
+    G.lidx = L - base2;
+    a = G.lidx % g.nx;
+    b = G.lidx / g.nx;
+  }
+  G.nax = G.ax == 0 ? g.nx : (G.ax == 1 ? g.ny : g.nz);
+  G.seglen = kSegLen[G.ax];
+  G.nseg = ((int64_t)G.nax + G.seglen - 1) / G.seglen;
+  if (G.ax == 0) {
+    G.o0[1] = G.on[1] = lattice_coord(g.oy, g.h, a);
+    G.o0[2] = G.on[2] = lattice_coord(g.oz, g.h, b);
+    G.o0[0] = lattice_coord(g.ox, g.h, 0);
+    G.on[0] = lattice_coord(g.ox, g.h, G.nax - 1);
+  } else if (G.ax == 1) {
+    G.o0[0] = G.on[0] = lattice_coord(g.ox, g.h, a);
+    G.o0[2] = G.on[2] = lattice_coord(g.oz, g.h, b);
+    G.o0[1] = lattice_coord(g.oy, g.h, 0);
+    G.on[1] = lattice_coord(g.oy, g.h, G.nax - 1);
+  } else {
+    G.o0[0] = G.on[0] = lattice_coord(g.ox, g.h, a);
+    G.o0[1] = G.on[1] = lattice_coord(g.oy, g.h, b);
+    G.o0[2] = lattice_coord(g.oz, g.h, 0);
+    G.on[2] = lattice_coord(g.oz, g.h, G.nax - 1);
+  }
+  G.o0a = sel3(G.o0, G.ax);
+  G.ona = sel3(G.on, G.ax);
+  G.len = G.ona - G.o0a;
+  G.oa = G.ax == 0 ? g.ox : (G.ax == 1 ? g.oy : g.oz);
+  return G;
+}
+
+// Per hit (one thread each, hits in their lines' slots): the crossing
+// coordinate c, its certified bound delta and, on lines of at most 255
+// segments, the segments the crossing leaves uniform (below a: crossing
+// above them; from b on: crossing below them, by the comparisons the sign
+// kernel makes, found by bisection as they are monotone in the segment),
+// written as fast[i] = {c, delta, 0, a | b << 8}; line_prep_kernel puts each
+// line's hits in order.  The hit's record is formed from its triangle by
+// line_hit_test's arithmetic (the emit pass stores only the triangle's id).
+__global__ void hit_prep_kernel(GridP g, const uint32_t *__restrict__ hit_tri,
+                                const uint32_t *__restrict__ hit_line,
+                                const uint64_t *__restrict__ n_dev, uint64_t cap,
+                                const Tri64 *__restrict__ tri, int64_t base1, int64_t base2,
+                                LineFast *__restrict__ fast) {
+#pragma clang fp contract(off)
+  const double u = 1.1102230246251565e-16;
+  const uint64_t n = *n_dev < cap ? *n_dev : cap;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const LineGeom G = line_geom(g, (int64_t)hit_line[i], base1, base2);
+    const int ax = G.ax, j = (ax + 1) % 3, k = (ax + 2) % 3;
+    LineHit r;
+    {
+      const Tri64 &tr = tri[hit_tri[i]];
+      line_hit_record(ax, tr.v, tr.v + 3, tr.v + 6, &r);
+    }
+    const double t0 = line_hit_t(r, G.o0);
+    const double c = G.o0a + t0;
+    const double v1a = sel3(r.v1, ax);
+    const double sa = fmax(fabs(G.o0a - v1a), fabs(G.ona - v1a));
+    double S[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) S[d] = d == ax ? sa : fabs(G.o0[d] - r.v1[d]);
+    double Mt = 0.0;
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      const int q1 = (q + 1) % 3, q2 = (q + 2) % 3;
+      Mt += fabs(r.e2[q]) * (S[q1] * fabs(r.e1[q2]) + S[q2] * fabs(r.e1[q1]));
+    }
+    const double af = fabs(r.f);
+    const double Ma = fabs(sel3(r.e1, j) * sel3(r.e2, k)) + fabs(sel3(r.e1, k) * sel3(r.e2, j));
+    const double beta = 2.0 * u * (Ma * af + 1.0);
+    const double E = 8.0 * u * (af * Mt + fabs(t0) + 2.0 * G.len);
+    double delta = 2.0 * (beta * G.len + 2.0 * E + u * (fabs(c) + fabs(t0) + 2.0 * G.len)) +
+                   1e-300;
+    delta *= 1.0 + 1e-6;
+    LineFast lf;
+    lf.c = c;
+    lf.delta = __double2float_ru(delta);
+    lf.cid = 0;
+    lf.pad_ = 0;
+    if (G.nseg <= 255) {
+      const double D = (double)lf.delta;
+      auto seg_lo = [&](int64_t q) { return lattice_coord(G.oa, g.h, q * G.seglen); };
+      auto seg_hi = [&](int64_t q) {
+        const int64_t e = q * G.seglen + G.seglen - 1;
+        return lattice_coord(G.oa, g.h, e < G.nax - 1 ? e : G.nax - 1);
+      };
+      int64_t lo = 0, hi = G.nseg;
+      while (lo < hi) {
+        const int64_t mid = (lo + hi) / 2;
+        if (c - seg_hi(mid) > kRayEps + D)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      const int64_t sa = lo;
+      lo = 0;
+      hi = G.nseg;
+      while (lo < hi) {
+        const int64_t mid = (lo + hi) / 2;
+        if (c - seg_lo(mid) < -kRayEps - D)
+          hi = mid;
+        else
+          lo = mid + 1;
+      }
+      lf.pad_ = (uint16_t)(sa | (lo << 8));
+    }
+    fast[i] = lf;
+  }
+}
+
+// Per line (one thread each): its hits (hit_prep_kernel's summaries) in
+// order of c, their clusters, the inline summary and the segment bytes.
+__global__ void line_prep_kernel(GridP g, const uint64_t *__restrict__ line_off, uint64_t cap,
                                  int64_t base1, int64_t base2, int64_t nlines,
                                  LineFast *__restrict__ fast,
                                  uint32_t *__restrict__ meta,
@@ -775,70 +925,6 @@ __global__ void line_prep_kernel(GridP g, const uint64_t *__restrict__ line_off,
       write_seg_words(g, li, lax, lidx, sl, segw);
       continue;
     }
-    int ax;
-    int64_t a, b;
-    if (L < base1) {
-      ax = 0;
-      a = L % g.ny;
-      b = L / g.ny;
-    } else if (L < base2) {
-      ax = 1;
-      a = (L - base1) % g.nx;
-      b = (L - base1) / g.nx;
-    } else {
-      ax = 2;
-      a = (L - base2) % g.nx;
-      b = (L - base2) / g.nx;
-    }
-    double o0[3], on[3];
-    const int32_t nax = ax == 0 ? g.nx : (ax == 1 ? g.ny : g.nz);
-    if (ax == 0) {
-      o0[1] = on[1] = lattice_coord(g.oy, g.h, a);
-      o0[2] = on[2] = lattice_coord(g.oz, g.h, b);
-      o0[0] = lattice_coord(g.ox, g.h, 0);
-      on[0] = lattice_coord(g.ox, g.h, nax - 1);
-    } else if (ax == 1) {
-      o0[0] = on[0] = lattice_coord(g.ox, g.h, a);
-      o0[2] = on[2] = lattice_coord(g.oz, g.h, b);
-      o0[1] = lattice_coord(g.oy, g.h, 0);
-      on[1] = lattice_coord(g.oy, g.h, nax - 1);
-    } else {
-      o0[0] = on[0] = lattice_coord(g.ox, g.h, a);
-      o0[1] = on[1] = lattice_coord(g.oy, g.h, b);
-      o0[2] = lattice_coord(g.oz, g.h, 0);
-      on[2] = lattice_coord(g.oz, g.h, nax - 1);
-    }
-    const double len = on[ax] - o0[ax];
-    const int j = (ax + 1) % 3, k = (ax + 2) % 3;
-    // crossing coordinate and certified bound of hit q
-    auto summarize = [&](int64_t q, double *c_out, float *delta_out) {
-      // the hit's record, by line_hit_test's arithmetic (the triangle's
-      // id is all the emit pass stores)
-      LineHit r;
-      {
-        const Tri64 &tr = tri[hits[hb + q]];
-        line_hit_record(ax, tr.v, tr.v + 3, tr.v + 6, &r);
-      }
-      const double t0 = line_hit_t(r, o0);
-      const double c = o0[ax] + t0;
-      double S[3];
-      for (int d = 0; d < 3; d++) S[d] = fabs(o0[d] - r.v1[d]);
-      S[ax] = fmax(fabs(o0[ax] - r.v1[ax]), fabs(on[ax] - r.v1[ax]));
-      double Mt = 0.0;
-      for (int i = 0; i < 3; i++) {
-        const int i1 = (i + 1) % 3, i2 = (i + 2) % 3;
-        Mt += fabs(r.e2[i]) * (S[i1] * fabs(r.e1[i2]) + S[i2] * fabs(r.e1[i1]));
-      }
-      const double af = fabs(r.f);
-      const double Ma = fabs(r.e1[j] * r.e2[k]) + fabs(r.e1[k] * r.e2[j]);
-      const double beta = 2.0 * u * (Ma * af + 1.0);
-      const double E = 8.0 * u * (af * Mt + fabs(t0) + 2.0 * len);
-      double delta = 2.0 * (beta * len + 2.0 * E + u * (fabs(c) + fabs(t0) + 2.0 * len)) +
-                     1e-300;
-      delta *= 1.0 + 1e-6;
-      *c_out = c;
-      *delta_out = __double2float_ru(delta);
-    };
     // clusters of the sorted hits: a gap beyond eps by more than the bounds
     // starts a new cluster; a gap within the bounds of eps has no fast path
     auto gap_class = [&](double gap, double dmax) -> int {
@@ -850,16 +936,21 @@ __global__ void line_prep_kernel(GridP g, const uint64_t *__restrict__ line_off,
     bool ok = true;
     if (n <= kInlineHits) {
       // short line: sorted in registers (stable bubble passes), no global
-      // round trips
+      // round trips beyond the one read
       double cs[kInlineHits];
       float ds[kInlineHits];
+      uint16_t ss[kInlineHits];
 #pragma unroll
       for (int q = 0; q < kInlineHits; q++) {
         if (q < n) {
-          summarize(q, &cs[q], &ds[q]);
+          const LineFast lf = fast[hb + q];
+          cs[q] = lf.c;
+          ds[q] = lf.delta;
+          ss[q] = lf.pad_;
         } else {
           cs[q] = __builtin_huge_val();
           ds[q] = 0.f;
+          ss[q] = 0;
         }
       }
 #pragma unroll
@@ -873,6 +964,9 @@ __global__ void line_prep_kernel(GridP g, const uint64_t *__restrict__ line_off,
             const float td = ds[q];
             ds[q] = ds[q + 1];
             ds[q + 1] = td;
+            const uint16_t ts = ss[q];
+            ss[q] = ss[q + 1];
+            ss[q + 1] = ts;
           }
       double dmax = 0.0;
 #pragma unroll
@@ -887,15 +981,6 @@ __global__ void line_prep_kernel(GridP g, const uint64_t *__restrict__ line_off,
           cid[q] = (uint8_t)cl;
         }
       }
-      // the segments each crossing leaves uniform (below a: crossing above
-      // them; from b on: crossing below them), by the comparisons the sign
-      // kernel would make, found by bisection (they are monotone in s)
-      const double oa = lax == 0 ? g.ox : (lax == 1 ? g.oy : g.oz);
-      auto seg_lo = [&](int64_t q) { return lattice_coord(oa, g.h, q * seglen); };
-      auto seg_hi = [&](int64_t q) {
-        const int64_t e = q * seglen + seglen - 1;
-        return lattice_coord(oa, g.h, e < lnax - 1 ? e : lnax - 1);
-      };
 #pragma unroll
       for (int q = 0; q < kInlineHits; q++) {
         if (q < n) {
@@ -909,39 +994,17 @@ __global__ void line_prep_kernel(GridP g, const uint64_t *__restrict__ line_off,
             li.c[q] = cs[q];
             li.delta[q] = ds[q];
             li.cid[q] = cid[q];
-            if (segflag) {
-              const double D = (double)ds[q];
-              int64_t lo = 0, hi = nseg;
-              while (lo < hi) {
-                const int64_t mid = (lo + hi) / 2;
-                if (cs[q] - seg_hi(mid) > kRayEps + D)
-                  lo = mid + 1;
-                else
-                  hi = mid;
-              }
-              const int64_t sa = lo;
-              lo = 0;
-              hi = nseg;
-              while (lo < hi) {
-                const int64_t mid = (lo + hi) / 2;
-                if (cs[q] - seg_lo(mid) < -kRayEps - D)
-                  hi = mid;
-                else
-                  lo = mid + 1;
-              }
-              li.seg[q] = (uint16_t)(sa | (lo << 8));
-            }
+            if (segflag) li.seg[q] = ss[q];
           }
         }
       }
     } else {
+      // longer line: insertion sort in place (global memory)
       double dmax = 0.0;
       for (int64_t q = 0; q < n; q++) {
-        LineFast lf;
-        summarize(q, &lf.c, &lf.delta);
+        LineFast lf = fast[hb + q];
         lf.cid = 0;
         lf.pad_ = 0;
-        // insertion into the sorted prefix (global memory)
         int64_t p = q;
         while (p > 0 && fast[hb + p - 1].c > lf.c) {
           fast[hb + p] = fast[hb + p - 1];
@@ -1168,7 +1231,8 @@ void line_hits_begin(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
                        lb.big, lb.nbig, lb.hitn, lb.hitl, lb.hslot);
     hipLaunchKernelGGL(line_bin_big_kernel, dim3(1024), dim3(256), 0, s, tri, g, lb.nlx,
                        lb.nlx + lb.nly, 0, lb.lcnt2, (const uint64_t *)nullptr,
-                       (uint32_t *)nullptr, (uint64_t)0, (const uint32_t *)lb.big,
+                       (uint32_t *)nullptr, (uint32_t *)nullptr, (uint64_t)0,
+                       (const uint32_t *)lb.big,
                        (const uint32_t *)lb.nbig);
     SDF_HIP(hipGetLastError());
   }
@@ -1181,15 +1245,16 @@ void line_hits_finish(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
   const uint64_t *n_dev = lb.loff + lb.nlines;
   if (cap < 0) cap = (int64_t)read_u64(s, lb.loff + lb.nlines);
   uint32_t *hits = ar.get<uint32_t>("line_hit_tri", cap + 1);
+  uint32_t *hline = ar.get<uint32_t>("line_hit_line", cap + 1);
   if (cap > 0 && nel > 0) {
     // lcnt continues as the cursor of the hits without a slot
     hipLaunchKernelGGL(line_emit_small_kernel, dim3(capped_grid(nel, 256, lb.max_blocks)),
                        dim3(256), 0, s, tri, nel, g, lb.nlx, lb.nlx + lb.nly,
                        (const uint8_t *)lb.hitn, (const uint32_t *)lb.hitl,
                        (const uint32_t *)lb.hslot, lb.lcnt, (const uint64_t *)lb.loff, hits,
-                       (uint64_t)cap);
+                       hline, (uint64_t)cap);
     hipLaunchKernelGGL(line_bin_big_kernel, dim3(1024), dim3(256), 0, s, tri, g, lb.nlx,
-                       lb.nlx + lb.nly, 1, lb.lcnt, (const uint64_t *)lb.loff, hits,
+                       lb.nlx + lb.nly, 1, lb.lcnt, (const uint64_t *)lb.loff, hits, hline,
                        (uint64_t)cap, (const uint32_t *)lb.big, (const uint32_t *)lb.nbig);
     SDF_HIP(hipGetLastError());
   }
@@ -1199,9 +1264,12 @@ void line_hits_finish(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
   // the summaries, then each line's segment bytes (SegLayout) from them
   const SegLayout sl = seg_layout(g);
   uint32_t *segw = ar.get<uint32_t>("line_segw", (size_t)sl.total + 1);
+  if (cap > 0 && nel > 0)
+    hipLaunchKernelGGL(hit_prep_kernel, dim3(capped_grid(cap, 256, lb.max_blocks)), dim3(256), 0,
+                       s, g, (const uint32_t *)hits, (const uint32_t *)hline, n_dev,
+                       (uint64_t)cap, tri, lb.nlx, lb.nlx + lb.nly, fast);
   hipLaunchKernelGGL(line_prep_kernel, dim3(capped_grid(lb.nlines, 256, lb.max_blocks)),
-                     dim3(256), 0, s, g,
-                     (const uint64_t *)lb.loff, tri, (const uint32_t *)hits, (uint64_t)cap, lb.nlx,
+                     dim3(256), 0, s, g, (const uint64_t *)lb.loff, (uint64_t)cap, lb.nlx,
                      lb.nlx + lb.nly, lb.nlines, fast, meta, inl, sl, segw);
   // the sentinel read by lanes whose line serves no point (byte 1 everywhere)
   launch_fill_u32(s, segw + sl.total, 1, 0x01010101u);
