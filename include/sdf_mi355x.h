@@ -67,6 +67,9 @@ sdf_status sdf_ctx_synchronize(sdf_ctx *ctx);
  * item (1..64, default 64), "ray_stack" LDS ray-stack depth of the vote
  * stage (1024 or 2048), "dist_per_cu" / "fused_per_cu" workgroups per CU
  * of the distance / sign kernels (0: one per work item; defaults 7 / 0),
+ * "sign_split" 1: the sign stage's bricks whose lines all agree run on a
+ * helper stream beside the vote (default 1), "same_per_cu" workgroups per CU
+ * of that pass (0: one per brick, the default),
  * "lbvh_early" when the LBVH build runs: 0 after the tile binning, issued
  * after the distance and axis-line work, 1 the same, issued before it, 2
  * after the axis-line hits end, 3 beside the tile binning (default 3),

@@ -138,6 +138,10 @@ struct sdf_ctx {
   // [0] pass start, [1] line hits done, [2] LBVH done, [3] tile chain done
   hipEvent_t join[4] = {nullptr, nullptr, nullptr, nullptr};
   hipEvent_t ev_plan = nullptr;  // the sign kernel's brick plan is written
+  // split sign stage: the distances are written (main stream), the marked
+  // bricks are written (helper stream 1), and that pass's timing events
+  hipEvent_t ev_dist = nullptr, ev_same = nullptr, ev_same_t[2] = {nullptr, nullptr};
+  bool same_timed = false;
   // the vote's last phase: its item list is known, its winding sums are done
   hipEvent_t ev_wind[2] = {nullptr, nullptr};
   std::mutex mu;
@@ -191,6 +195,11 @@ struct sdf_ctx {
   // against 0), 2 after the line hits end (beside the sign stage: not done
   // when the vote needs it, 0.14 ms wait)
   int lbvh_early = 3;
+  int sign_split = 1;  // split sign stage when the bricks are marked (0: one pass)
+  // workgroups per CU of its marked-brick pass (0: one per brick; capping it
+  // to leave wave slots for the vote beside it was slower: 8, 16, 24 per CU
+  // gave 1.04, 1.01, 1.03 against 0.99 ms per step)
+  int same_per_cu = 0;
   int ncu = 256;
   // timing
   bool timing = false;
@@ -487,6 +496,10 @@ struct LatticeRun {
   uint32_t cap[sdf::kPendLists] = {0, 0, 0};
   sdf::Pending *pend[sdf::kPendLists] = {nullptr, nullptr, nullptr};
   sdf::BrickPlan *plan = nullptr;  // the sign kernel's per-brick plan
+  // split sign stage: the bricks whose lines all agree on helper stream 1
+  // (beside the others and the vote), the others listed in rest
+  bool split = false;
+  uint32_t *rest = nullptr;
 };
 
 enum : int {
@@ -494,8 +507,10 @@ enum : int {
   kStMeshErr, kStNext0, kStCount
 };
 
-// device words of the pending lists' counts (PendOut order)
+// device words of the pending lists' counts (PendOut order) and of the
+// split sign stage's list of other bricks
 constexpr int kPendWord[sdf::kPendLists] = {2, 1, 16};
+constexpr int kRestWord = 17;
 
 // Stage 1: the build chains.  Tile bins and distance work items run on the
 // main stream; the axis-line hits (needed by the sign stage) on helper
@@ -587,6 +602,17 @@ void stage_bins_lines(sdf_ctx *c, LatticeRun &L, int part) {
   if (part & 2) {
     if (c->lines_early == 2) SDF_HIP(hipStreamWaitEvent(c->aux[0], c->join[3], 0));
     sdf::line_hits_finish(c->aux[0], c->ar, c->tri, c->nel, L.g, c->lb, L.bins, c->caps.hits);
+    // the bricks whose lines all agree: marked in the brick plan (written on
+    // helper stream 1), the others listed, for a split sign stage (sdf
+    // output with distances by tile only)
+    SDF_HIP(hipStreamWaitEvent(c->aux[0], c->ev_plan, 0));
+    const bool splittable = c->sign_split && L.d2buf && L.bins.d2_by_tile && L.out.sdf &&
+                            !L.out.dist && !L.out.signs && !L.out.conf;
+    if (splittable)
+      L.rest = c->ar.get<uint32_t>("sign_rest", (size_t)sdf::brick_plan_size(L.g) + 1);
+    L.split = sdf::launch_brick_same(c->aux[0], L.g, L.bins, L.plan,
+                                     splittable ? L.rest : nullptr, c->d_words + kRestWord) &&
+              splittable;
     SDF_HIP(hipEventRecord(c->join[1], c->aux[0]));
   }
   c->ar.scope.clear();
@@ -643,13 +669,32 @@ void stage_sign(sdf_ctx *c, LatticeRun &L) {
     SDF_HIP(hipStreamWaitEvent(c->stream, c->join[1], 0));
     SDF_HIP(hipStreamWaitEvent(c->stream, c->ev_plan, 0));
   }
-  c->mark("fused");
   sdf::PendOut po{};
   for (int k = 0; k < sdf::kPendLists; k++) {
     po.p[k] = L.pend[k];
     po.n[k] = c->d_words + kPendWord[k];
     po.cap[k] = L.cap[k];
   }
+  c->same_timed = false;
+  if (L.split) {
+    // the marked bricks (store-bound, most of the grid) on helper stream 1
+    // once the distances and the marks are done; the others, which alone
+    // hand points to the vote, here, so the vote runs beside the former
+    SDF_HIP(hipEventRecord(c->ev_dist, c->stream));
+    SDF_HIP(hipStreamWaitEvent(c->aux[1], c->ev_dist, 0));
+    SDF_HIP(hipStreamWaitEvent(c->aux[1], c->join[1], 0));
+    if (c->timing) SDF_HIP(hipEventRecord(c->ev_same_t[0], c->aux[1]));
+    sdf::launch_fused_lattice(c->aux[1], L.g, c->tri, L.bins, L.d2buf, L.do_sign, L.out, po,
+                              c->d_words, L.plan, (int64_t)c->same_per_cu * c->ncu, 1);
+    if (c->timing) SDF_HIP(hipEventRecord(c->ev_same_t[1], c->aux[1]));
+    c->same_timed = c->timing;
+    SDF_HIP(hipEventRecord(c->ev_same, c->aux[1]));
+    c->mark("fused");
+    sdf::launch_fused_lattice(c->stream, L.g, c->tri, L.bins, L.d2buf, L.do_sign, L.out, po,
+                              c->d_words, L.plan, 0, 2, L.rest, c->d_words + kRestWord);
+    return;
+  }
+  c->mark("fused");
   sdf::launch_fused_lattice(c->stream, L.g, c->tri, L.bins, L.d2buf, L.do_sign, L.out, po,
                             c->d_words, L.plan, (int64_t)c->fused_per_cu * c->ncu);
 }
@@ -672,6 +717,10 @@ void stage_resolve(sdf_ctx *c, LatticeRun &L) {
     for (int k = 0; k < sdf::kPendLists; k++)
       lists.add(L.pend[k], c->d_words + kPendWord[k], L.cap[k], k == 0 ? 0 : L.rp.min_rays);
     resolve_items(c, L.g, nullptr, L.rp, lists, 2, L.out, cnt);
+  }
+  if (L.split) {  // the marked bricks' pass on helper stream 1
+    c->mark("same_wait");
+    SDF_HIP(hipStreamWaitEvent(c->stream, c->ev_same, 0));
   }
   c->mark("end");
   sdf::StatusSrc src;
@@ -960,6 +1009,9 @@ void ctx_init(sdf_ctx *c, int32_t device) {
   for (int q = 0; q < 2; q++) SDF_HIP(hipStreamCreateWithFlags(&c->aux[q], hipStreamNonBlocking));
   for (int q = 0; q < 4; q++) SDF_HIP(hipEventCreateWithFlags(&c->join[q], hipEventDisableTiming));
   SDF_HIP(hipEventCreateWithFlags(&c->ev_plan, hipEventDisableTiming));
+  SDF_HIP(hipEventCreateWithFlags(&c->ev_dist, hipEventDisableTiming));
+  SDF_HIP(hipEventCreateWithFlags(&c->ev_same, hipEventDisableTiming));
+  for (int q = 0; q < 2; q++) SDF_HIP(hipEventCreate(&c->ev_same_t[q]));
   for (int q = 0; q < 2; q++) SDF_HIP(hipEventCreateWithFlags(&c->ev_wind[q], hipEventDisableTiming));
   SDF_HIP(hipMalloc(&c->d_words, 256));
   SDF_HIP(hipHostMalloc((void **)&c->h_status, 16 * sizeof(uint64_t),
@@ -983,6 +1035,10 @@ void ctx_free(sdf_ctx *c) {
   for (int q = 0; q < 4; q++)
     if (c->join[q]) (void)hipEventDestroy(c->join[q]);
   if (c->ev_plan) (void)hipEventDestroy(c->ev_plan);
+  if (c->ev_dist) (void)hipEventDestroy(c->ev_dist);
+  if (c->ev_same) (void)hipEventDestroy(c->ev_same);
+  for (int q = 0; q < 2; q++)
+    if (c->ev_same_t[q]) (void)hipEventDestroy(c->ev_same_t[q]);
   for (int q = 0; q < 2; q++)
     if (c->ev_wind[q]) (void)hipEventDestroy(c->ev_wind[q]);
   if (c->d_words) (void)hipFree(c->d_words);
@@ -1246,14 +1302,19 @@ sdf_status sdf_ctx_tune(sdf_ctx *c, const char *name, int64_t value) {
     } else if (n == "lines_early") {
       if (value < 0 || value > 2) fail(SDF_ERR_ARG, "lines_early must be 0, 1 or 2");
       c->lines_early = (int)value;
+    } else if (n == "sign_split") {
+      if (value != 0 && value != 1) fail(SDF_ERR_ARG, "sign_split must be 0 or 1");
+      c->sign_split = (int)value;
     } else if (n == "learned_force") {
       if (value != 0 && value != 1) fail(SDF_ERR_ARG, "learned_force must be 0 or 1");
       c->learned_force = value != 0;
-    } else if (n == "dist_per_cu" || n == "fused_per_cu" || n == "line_per_cu") {
+    } else if (n == "dist_per_cu" || n == "fused_per_cu" || n == "line_per_cu" ||
+               n == "same_per_cu") {
       if (value < 0 || value > 64) fail(SDF_ERR_ARG, n + " must be in 0..64");
-      (n == "dist_per_cu" ? c->dist_per_cu
-                          : (n == "fused_per_cu" ? c->fused_per_cu : c->line_per_cu)) =
-          (int)value;
+      (n == "dist_per_cu"    ? c->dist_per_cu
+       : n == "fused_per_cu" ? c->fused_per_cu
+       : n == "line_per_cu"  ? c->line_per_cu
+                             : c->same_per_cu) = (int)value;
     } else {
       fail(SDF_ERR_ARG, "unknown tunable " + n);
     }
@@ -1281,6 +1342,15 @@ sdf_status sdf_ctx_stage_times(sdf_ctx *c, char *names, int32_t names_cap,
       if (w < cap && ms) ms[w] = t;
       if (!all.empty()) all += ";";
       all += c->ev_name[i];
+      w++;
+    }
+    if (c->same_timed) {  // the split sign stage's pass on helper stream 1
+      float t = 0.f;
+      SDF_HIP(hipEventSynchronize(c->ev_same_t[1]));
+      SDF_HIP(hipEventElapsedTime(&t, c->ev_same_t[0], c->ev_same_t[1]));
+      if (w < cap && ms) ms[w] = t;
+      if (!all.empty()) all += ";";
+      all += "fused_same";
       w++;
     }
     if (names && names_cap > 0) {

@@ -692,15 +692,41 @@ __global__ void chunk_items_kernel(const uint64_t *__restrict__ chunk_off,
 // segment's coordinate range when the line has none), so the sign kernel
 // reads one byte per line and brick.  With segment ranges and at most 128
 // segments the test runs on all segments at once (seg_words_masks).
+// Segments of word w (bytes) whose byte is 1 (uniform, both rays even) or 7
+// (uniform, both odd), into 128-bit masks m1 = (m[0], m[1]), m7 = (m[2], m[3]).
+__device__ inline void word_same_masks(uint32_t word, int w, uint64_t m[4]) {
+#pragma unroll
+  for (int b = 0; b < 4; b++) {
+    const uint32_t byte = (word >> (8 * b)) & 0xFFu;
+    const int s = 4 * w + b;
+    const uint64_t bit = s < 128 ? 1ull << (s & 63) : 0ull;
+    const bool hi = s >= 64;
+    if (byte == 1u) {
+      m[0] |= hi ? 0ull : bit;
+      m[1] |= hi ? bit : 0ull;
+    }
+    if (byte == 7u) {
+      m[2] |= hi ? 0ull : bit;
+      m[3] |= hi ? bit : 0ull;
+    }
+  }
+}
+
+// lmask (nullable): the line's byte-1 and byte-7 segment masks (4 words),
+// zero beyond 128 segments.
 __device__ inline void write_seg_words(const GridP &g, const LineInline &li, int ax, int64_t l,
-                                       const SegLayout &sl, uint32_t *__restrict__ segw) {
+                                       const SegLayout &sl, uint32_t *__restrict__ segw,
+                                       uint64_t *__restrict__ lmask) {
   LineTail r;
   memcpy(&r, reinterpret_cast<const char *>(&li) + 48, sizeof(r));
   uint32_t *out = segw + sl.off[ax] + l;
   const int64_t stride = sl.nl[ax];
   const int ns = sl.ns[ax];
+  uint64_t m[4] = {0ull, 0ull, 0ull, 0ull};
   if ((r.info & kLineSeg) && ns <= 128) {
-    seg_words_masks(r, ns, [&](int w, uint32_t word) { out[(int64_t)w * stride] = word; });
+    seg_words_masks(r, ns, [&](int w, uint32_t word) { out[(int64_t)w * stride] = word; }, m);
+    if (lmask)
+      for (int q = 0; q < 4; q++) lmask[q] = m[q];
     return;
   }
   const double oa = ax == 0 ? g.ox : (ax == 1 ? g.oy : g.oz);
@@ -722,9 +748,12 @@ __device__ inline void write_seg_words(const GridP &g, const LineInline &li, int
     word |= (u ? (uint32_t)(1 | (p << 1) | (q << 2)) : 0u) << (8 * (sgi & 3));
     if ((sgi & 3) == 3 || sgi == ns - 1) {
       out[(int64_t)(sgi >> 2) * stride] = word;
+      word_same_masks(word, sgi >> 2, m);
       word = 0u;
     }
   }
+  if (lmask)
+    for (int q = 0; q < 4; q++) lmask[q] = m[q];
 }
 
 // Fast-path summaries of each axis line (one thread per line).
@@ -890,7 +919,7 @@ __global__ void line_prep_kernel(GridP g, const uint64_t *__restrict__ line_off,
                                  LineFast *__restrict__ fast,
                                  uint32_t *__restrict__ meta,
                                  LineInline *__restrict__ inl, SegLayout sl,
-                                 uint32_t *__restrict__ segw) {
+                                 uint32_t *__restrict__ segw, uint64_t *__restrict__ lmask) {
 #pragma clang fp contract(off)
   const double u = 1.1102230246251565e-16;
   for (int64_t L = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; L < nlines;
@@ -915,14 +944,14 @@ __global__ void line_prep_kernel(GridP g, const uint64_t *__restrict__ line_off,
       meta[L] = 0u;
       li.info = segflag;
       inl[L] = li;
-      write_seg_words(g, li, lax, lidx, sl, segw);
+      write_seg_words(g, li, lax, lidx, sl, segw, lmask ? lmask + 4 * L : nullptr);
       continue;
     }
     if (n > kLineFastMax || he > cap) {  // he > cap: overflow, the call is re-run
       meta[L] = kLineExact;
       li.info = kLineExact | 255u;
       inl[L] = li;
-      write_seg_words(g, li, lax, lidx, sl, segw);
+      write_seg_words(g, li, lax, lidx, sl, segw, lmask ? lmask + 4 * L : nullptr);
       continue;
     }
     // clusters of the sorted hits: a gap beyond eps by more than the bounds
@@ -1027,7 +1056,102 @@ __global__ void line_prep_kernel(GridP g, const uint64_t *__restrict__ line_off,
     li.info = (uint32_t)(n < 255 ? n : 255) | ((cl + 1u) & 0xFFu) << 8 | (ok ? 0u : kLineExact) |
               segflag;
     inl[L] = li;
-    write_seg_words(g, li, lax, lidx, sl, segw);
+    write_seg_words(g, li, lax, lidx, sl, segw, lmask ? lmask + 4 * L : nullptr);
+  }
+}
+
+// Per brick row of each axis, the AND over the brick's lines of that axis
+// (inside the grid) of their byte-1 and byte-7 segment masks, one wave per
+// row: part 0 the x-lines of bricks (., by, bz), a mask over bx; part 1 the
+// y-lines of (bx, ., bz) over by; part 2 the z-lines of (bx, by, .) over bz.
+__global__ __launch_bounds__(64) void brick_same_reduce_kernel(GridP g,
+                                                               const uint64_t *__restrict__ lmask,
+                                                               int64_t base1, int64_t base2,
+                                                               SameRows rows, int32_t nbz) {
+  const int lane = threadIdx.x;
+  const int32_t nbx = rows.nbx, nby = rows.nby;
+  const int64_t n0 = (int64_t)nby * nbz, n1 = (int64_t)nbx * nbz, n2 = (int64_t)nbx * nby;
+  for (int64_t w = blockIdx.x; w < n0 + n1 + n2; w += gridDim.x) {
+    int64_t L = 0;
+    bool valid;
+    uint64_t *dst;
+    if (w < n0) {
+      const int32_t by = (int32_t)(w % nby), bz = (int32_t)(w / nby);
+      const int32_t j = by * 8 + (lane & 7), k = bz * kSignZ + (lane >> 3);
+      valid = lane < 8 * kSignZ && j < g.ny && k < g.nz;
+      L = (int64_t)k * g.ny + j;
+      dst = rows.x + 4 * w;
+    } else if (w < n0 + n1) {
+      const int64_t r = w - n0;
+      const int32_t bx = (int32_t)(r % nbx), bz = (int32_t)(r / nbx);
+      const int32_t i = bx * 8 + (lane & 7), k = bz * kSignZ + (lane >> 3);
+      valid = lane < 8 * kSignZ && i < g.nx && k < g.nz;
+      L = base1 + (int64_t)k * g.nx + i;
+      dst = rows.y + 4 * r;
+    } else {
+      const int64_t r = w - n0 - n1;
+      const int32_t bx = (int32_t)(r % nbx), by = (int32_t)(r / nbx);
+      const int32_t i = bx * 8 + (lane & 7), j = by * 8 + (lane >> 3);
+      valid = i < g.nx && j < g.ny;
+      L = base2 + (int64_t)j * g.nx + i;
+      dst = rows.z + 4 * r;
+    }
+    uint64_t v[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) v[q] = valid ? lmask[4 * L + q] : ~0ull;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1)
+#pragma unroll
+      for (int q = 0; q < 4; q++) v[q] &= __shfl_xor(v[q], off);
+    if (lane == 0)
+      for (int q = 0; q < 4; q++) dst[q] = v[q];
+  }
+}
+
+// Bits 2 (every line byte 1: outside) and 3 (every line byte 7: inside) of
+// each brick's plan flags, from the three row masks.
+// rest (nullable): the other bricks' launch positions, in rest[0, *nrest)
+// (*nrest zero on entry; one atomic per wave).
+__global__ __launch_bounds__(256) void brick_same_flags_kernel(
+    BrickPlan *__restrict__ plan, int64_t nbricks, const uint64_t *__restrict__ xm,
+    const uint64_t *__restrict__ ym, const uint64_t *__restrict__ zm, int32_t nbx, int32_t nby,
+    uint32_t *__restrict__ rest, uint32_t *nrest) {
+  auto bit = [](const uint64_t *M, int32_t s, int which) -> uint32_t {
+    const uint64_t w = s < 64 ? M[2 * which] : M[2 * which + 1];
+    return (uint32_t)((w >> (s & 63)) & 1ull);
+  };
+  const int lane = threadIdx.x & 63;
+  for (int64_t m0 = (int64_t)blockIdx.x * blockDim.x; m0 < nbricks;
+       m0 += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t m = m0 + threadIdx.x;
+    bool other = false;
+    if (m < nbricks) {
+      // xy, z, flags, pad in one 16-byte load
+      const uint4 t = reinterpret_cast<const uint4 *>(plan + m)[1];
+      const uint32_t flags = t.z;
+      if (flags & 2u) {  // not a padding position
+        const int32_t bx = (int32_t)(t.x & 0xFFFFu), by = (int32_t)(t.x >> 16),
+                      bz = (int32_t)t.y;
+        const uint64_t *X = xm + 4 * ((int64_t)bz * nby + by);
+        const uint64_t *Y = ym + 4 * ((int64_t)bz * nbx + bx);
+        const uint64_t *Z = zm + 4 * ((int64_t)by * nbx + bx);
+        const uint32_t s1 = bit(X, bx, 0) & bit(Y, by, 0) & bit(Z, bz, 0);
+        const uint32_t s7 = bit(X, bx, 1) & bit(Y, by, 1) & bit(Z, bz, 1);
+        if (s1 | s7)
+          plan[m].flags = flags | (s1 << 2) | (s7 << 3);
+        else
+          other = true;
+      }
+    }
+    if (rest) {
+      const uint64_t bal = __ballot(other);
+      if (bal) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(nrest, (uint32_t)__popcll(bal));
+        base = __shfl(base, 0);
+        if (other) rest[base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] = (uint32_t)m;
+      }
+    }
   }
 }
 
@@ -1263,16 +1387,38 @@ void line_hits_finish(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
   LineInline *inl = ar.get<LineInline>("line_inline", lb.nlines + 1);
   // the summaries, then each line's segment bytes (SegLayout) from them
   const SegLayout sl = seg_layout(g);
-  uint32_t *segw = ar.get<uint32_t>("line_segw", (size_t)sl.total + 1);
+  uint32_t *segw = ar.get<uint32_t>("line_segw", (size_t)sl.total + 2);
   if (cap > 0 && nel > 0)
     hipLaunchKernelGGL(hit_prep_kernel, dim3(capped_grid(cap, 256, lb.max_blocks)), dim3(256), 0,
                        s, g, (const uint32_t *)hits, (const uint32_t *)hline, n_dev,
                        (uint64_t)cap, tri, lb.nlx, lb.nlx + lb.nly, fast);
+  // the lines' byte-1 / byte-7 segment masks (launch_brick_same; lines of
+  // at most 128 segments)
+  const bool masks = sl.ns[0] <= 128 && sl.ns[1] <= 128 && sl.ns[2] <= 128;
+  uint64_t *lmask = masks ? ar.get<uint64_t>("line_same_mask", 4 * (size_t)lb.nlines + 4) : nullptr;
   hipLaunchKernelGGL(line_prep_kernel, dim3(capped_grid(lb.nlines, 256, lb.max_blocks)),
                      dim3(256), 0, s, g, (const uint64_t *)lb.loff, (uint64_t)cap, lb.nlx,
-                     lb.nlx + lb.nly, lb.nlines, fast, meta, inl, sl, segw);
-  // the sentinel read by lanes whose line serves no point (byte 1 everywhere)
+                     lb.nlx + lb.nly, lb.nlines, fast, meta, inl, sl, segw, lmask);
+  SameRows rows;
+  if (masks) {
+    const SignDims sd = sign_dims(g);
+    rows.nbx = sd.sx;
+    rows.nby = sd.sy;
+    const int64_t nx4 = 4 * (int64_t)sd.sy * sd.sz, ny4 = 4 * (int64_t)sd.sx * sd.sz,
+                  nz4 = 4 * (int64_t)sd.sx * sd.sy;
+    rows.x = ar.get<uint64_t>("brick_same_rows", (size_t)(nx4 + ny4 + nz4));
+    rows.y = rows.x + nx4;
+    rows.z = rows.y + ny4;
+    const int64_t nrows = (nx4 + ny4 + nz4) / 4;
+    hipLaunchKernelGGL(brick_same_reduce_kernel, dim3((unsigned)(nrows < 65536 ? nrows : 65536)),
+                       dim3(64), 0, s, g, (const uint64_t *)lmask, lb.nlx, lb.nlx + lb.nly, rows,
+                       sd.sz);
+  }
+  b.same_rows = rows;
+  // the sentinel read by lanes whose line serves no point (byte 1 everywhere),
+  // and the one of byte 7 (bricks whose lines all carry byte 7)
   launch_fill_u32(s, segw + sl.total, 1, 0x01010101u);
+  launch_fill_u32(s, segw + sl.total + 1, 1, 0x07070707u);
   SDF_HIP(hipGetLastError());
   b.line_off = lb.loff;
   b.seg_words = segw;
@@ -1282,6 +1428,19 @@ void line_hits_finish(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
   b.line_inline = inl;
   b.nline_hits = cap;
   b.nhits_dev = n_dev;
+}
+
+bool launch_brick_same(hipStream_t s, const GridP &g, const Bins &b, BrickPlan *plan,
+                       uint32_t *rest, uint32_t *nrest) {
+  if (!b.same_rows.x || !plan) return false;
+  const int64_t nbricks = brick_plan_size(g);
+  if (nbricks <= 0) return false;
+  hipLaunchKernelGGL(brick_same_flags_kernel, dim3(grid_for(nbricks, 256)), dim3(256), 0, s, plan,
+                     nbricks, (const uint64_t *)b.same_rows.x, (const uint64_t *)b.same_rows.y,
+                     (const uint64_t *)b.same_rows.z, b.same_rows.nbx, b.same_rows.nby, rest,
+                     nrest);
+  SDF_HIP(hipGetLastError());
+  return true;
 }
 
 void build_line_hits(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,

@@ -346,7 +346,8 @@ __host__ __device__ inline uint32_t spread_nibble(uint32_t x) {
 // (A_h minus the earlier A's), its -ray parity from the last one below it
 // (B_h minus the later B's).  put(w, word) receives words 0 .. (ns+3)/4-1.
 template <typename Put>
-__host__ __device__ inline void seg_words_masks(const LineTail &r, int ns, Put put) {
+__host__ __device__ inline void seg_words_masks(const LineTail &r, int ns, Put put,
+                                                uint64_t *same = nullptr) {
   const uint32_t n = r.info & 0xFFu;
   const bool usable = !(r.info & kLineExact) && n <= (uint32_t)kInlineHits;
   const int ncl = (int)((r.info >> 8) & 0xFFu);
@@ -390,6 +391,12 @@ __host__ __device__ inline void seg_words_masks(const LineTail &r, int ns, Put p
     }
     sblo |= blo[h2];
     sbhi |= bhi[h2];
+  }
+  if (same) {  // segments whose byte is 1 (u, neither ray odd) and 7 (u, both odd)
+    same[0] = ulo & ~plo & ~qlo;
+    same[1] = uhi & ~phi & ~qhi;
+    same[2] = ulo & plo & qlo;
+    same[3] = uhi & phi & qhi;
   }
   const int nw = (ns + 3) / 4;
   for (int w = 0; w < nw; w++) {

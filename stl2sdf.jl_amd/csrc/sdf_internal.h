@@ -151,6 +151,15 @@ struct DistItem {
 };
 constexpr uint64_t kItemBeginMask = (1ull << 48) - 1;
 
+// Brick rows' byte-1 / byte-7 segment masks (4 words each: m1 lo, hi, m7 lo,
+// hi), the AND over a row's lines of theirs: x rows (by, bz) over bx, y rows
+// (bx, bz) over by, z rows (bx, by) over bz; nbx, nby the sign-brick counts
+// along x and y.
+struct SameRows {
+  uint64_t *x = nullptr, *y = nullptr, *z = nullptr;
+  int32_t nbx = 0, nby = 0;
+};
+
 struct Bins {
   // tile binning (distance)
   uint64_t *tile_off = nullptr;  // per tile: first pair of its run (0 when empty)
@@ -169,6 +178,9 @@ struct Bins {
   LineFast *line_fast = nullptr;   // per hit, sorted by c within each line
   uint32_t *line_meta = nullptr;   // per line: cluster count | kLineExact
   LineInline *line_inline = nullptr;  // per line: inline summary (<= 4 hits)
+  // per brick row of each axis: the AND over its lines of their byte-1 and
+  // byte-7 segment masks (null when a line has more than 128 segments)
+  SameRows same_rows;
   uint32_t *seg_words = nullptr;      // sign-brick segment bytes, 4 per word (SegLayout),
                                       // then the sentinel word 0x01010101
   int64_t ntile_pairs = 0, nline_hits = 0;
@@ -286,6 +298,14 @@ void tile_bins_finish(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
                       float *d2tile = nullptr, bool defer_bounds = false);
 // The run bounds tile_bins_finish deferred (no-op otherwise).
 void tile_bins_bounds(hipStream_t s, const GridP &g, const Bins &b);
+// The sign bricks whose lines all carry segment byte 1, or all byte 7, at
+// the brick: bits 2 and 3 of their plan flags (after launch_brick_plan and
+// line_hits_finish; no-op without the lines' masks).  The sign kernel then
+// writes such a brick without reading its lines.
+// rest (nullable, one entry per launch position, *nrest zero): the launch
+// positions of the other bricks.  False when nothing was marked.
+bool launch_brick_same(hipStream_t s, const GridP &g, const Bins &b, BrickPlan *plan,
+                       uint32_t *rest = nullptr, uint32_t *nrest = nullptr);
 void line_hits_begin(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
                      const GridP &g, LineBuild &lb, Bins &b);
 void line_hits_finish(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
@@ -328,6 +348,9 @@ void launch_learned_fallback(hipStream_t s, const GridP &g, const Tri64 *tri, co
                              const Bins &b, const uint64_t *list,
                              const unsigned long long *stats, float slack, float *d2buf,
                              uint32_t *err);
+// select (sdf output with signs and distances by tile only): 1 the bricks
+// launch_brick_same marked, 2 the bricks in list[0, *nlist) (the others);
+// 0 every brick.
 // Per tile: distance from d2buf (or h), six axis rays, combine, outputs.
 // Points with a vote inside its rounding margin go to pend.p[0] (restart
 // from ray 1 on the LBVH); non-unanimous votes go to pend.p[1] (5:1) or
@@ -341,7 +364,8 @@ void launch_brick_plan(hipStream_t s, const GridP &g, const Bins &b, const float
 void launch_fused_lattice(hipStream_t s, const GridP &g, const Tri64 *tri,
                           const Bins &b, const float *d2buf, bool do_sign,
                           FusedOut out, const PendOut &pend, uint32_t *err,
-                          const BrickPlan *plan, int64_t max_blocks = 0);
+                          const BrickPlan *plan, int64_t max_blocks = 0, int select = 0,
+                          const uint32_t *list = nullptr, const uint32_t *nlist = nullptr);
 
 struct RayParams {
   double dirs[kNumDirs * 3];

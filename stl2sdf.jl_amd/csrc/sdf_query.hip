@@ -723,12 +723,19 @@ __device__ inline void fused_brick(const FusedJob &J, int64_t m, FusedShared &S,
   if (do_sign) {
     const int32_t sxy = xl ? bx : by;
     const int64_t nl = (int64_t)across * g.nz;
-    const int64_t oxy = valid ? (xl ? 0 : J.segoff1) + (int64_t)(sxy >> 2) * nl +
-                                    (int64_t)(k0 + sq) * across + la
-                              : J.segtotal;
-    const int64_t oz = in_grid ? J.segoff2 + (int64_t)(bz >> 2) * ((int64_t)g.nx * g.ny) +
-                                     (int64_t)j * g.nx + i
-                               : J.segtotal;
+    // a brick whose lines all carry byte 1, or all byte 7, at the brick
+    // (plan flag bits 2, 3: brick_same_flags_kernel) reads the matching
+    // sentinel word instead (one cached address), so its loads return at
+    // once and it takes the shared-byte path below with the same bytes
+    const uint32_t same = P.flags & 12u;
+    const int64_t sent = J.segtotal + (same == 8u ? 1 : 0);
+    const int64_t oxy = valid && !same ? (xl ? 0 : J.segoff1) + (int64_t)(sxy >> 2) * nl +
+                                             (int64_t)(k0 + sq) * across + la
+                                       : sent;
+    const int64_t oz = in_grid && !same
+                           ? J.segoff2 + (int64_t)(bz >> 2) * ((int64_t)g.nx * g.ny) +
+                                 (int64_t)j * g.nx + i
+                           : sent;
     const uint32_t wxy = J.segw[oxy], wz = J.segw[oz];
     bxy = (wxy >> (8 * (sxy & 3))) & 0xFFu;
     bzl = (wz >> (8 * (bz & 3))) & 0xFFu;
@@ -946,6 +953,33 @@ void fused_lattice_kernel(FusedJob J) {
   const uint32_t nb = (uint32_t)J.nbricks;
   for (uint32_t m = (uint32_t)xcd_remap(blockIdx.x, gridDim.x); m < nb; m += gridDim.x)
     fused_brick<kOut>(J, (int64_t)m, S, C);
+}
+
+// The bricks launch_brick_same marked (every line byte 1 or every line byte
+// 7), one per workgroup; the others return after their plan's flags.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8)))
+void fused_same_kernel(FusedJob J) {
+  __shared__ FusedShared S;
+  __shared__ FusedCold C;
+  if (threadIdx.x == 0) C = J.cold;
+  wave_lds_sync();
+  const uint32_t nb = (uint32_t)J.nbricks;
+  for (uint32_t m = (uint32_t)xcd_remap(blockIdx.x, gridDim.x); m < nb; m += gridDim.x) {
+    const uint32_t flags = kconst(reinterpret_cast<const uint32_t *>(J.plan + m))[6];
+    if (flags & 12u) fused_brick<kOutSdf>(J, (int64_t)m, S, C);
+  }
+}
+
+// The bricks in list[0, *n) (launch positions).
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8)))
+void fused_list_kernel(FusedJob J, const uint32_t *__restrict__ list, const uint32_t *n) {
+  __shared__ FusedShared S;
+  __shared__ FusedCold C;
+  if (threadIdx.x == 0) C = J.cold;
+  wave_lds_sync();
+  const uint32_t cnt = *kconst(n);
+  for (uint32_t t = blockIdx.x; t < cnt; t += gridDim.x)
+    fused_brick<kOutSdf>(J, (int64_t)kconst(list)[t], S, C);
 }
 
 // The sign kernel's plan, one thread per launch position: the brick decode
@@ -1612,7 +1646,8 @@ FusedJob make_fused_job(const GridP &g, const Bins &b, const float *d2buf, bool 
 void launch_fused_lattice(hipStream_t s, const GridP &g, const Tri64 *tri,
                           const Bins &b, const float *d2buf, bool do_sign,
                           FusedOut out, const PendOut &pend, uint32_t *err,
-                          const BrickPlan *plan, int64_t max_blocks) {
+                          const BrickPlan *plan, int64_t max_blocks, int select,
+                          const uint32_t *list, const uint32_t *nlist) {
   (void)tri;
   FusedJob J = make_fused_job(g, b, d2buf, do_sign, out, pend, err);
   if (J.nbricks <= 0) return;
@@ -1625,7 +1660,16 @@ void launch_fused_lattice(hipStream_t s, const GridP &g, const Tri64 *tri,
   // the specialised sets fix their stages (fused_brick): sdf = distances +
   // signs, dist = distances, signs + conf = signs; anything else runs the
   // generic instantiation
-  if (set == kOutSdf && do_sign && d2buf)
+  if (select != 0 && !(set == kOutSdf && do_sign && d2buf && b.d2_by_tile))
+    throw Error{-1, "brick selection needs the sdf output with distances by tile"};
+  if (select == 1) {
+    // (max_blocks caps the grid: wave slots left for the vote beside it)
+    hipLaunchKernelGGL(fused_same_kernel, dim3((unsigned)grid), dim3(64), 0, s, J);
+  } else if (select == 2) {
+    if (!list || !nlist) throw Error{-1, "brick list missing"};
+    const int64_t g2 = J.nbricks < 16384 ? J.nbricks : 16384;
+    hipLaunchKernelGGL(fused_list_kernel, dim3((unsigned)g2), dim3(64), 0, s, J, list, nlist);
+  } else if (set == kOutSdf && do_sign && d2buf)
     hipLaunchKernelGGL(fused_lattice_kernel<kOutSdf>, dim3((unsigned)grid), dim3(64), 0, s, J);
   else if (set == kOutDist && !do_sign && d2buf)
     hipLaunchKernelGGL(fused_lattice_kernel<kOutDist>, dim3((unsigned)grid), dim3(64), 0, s, J);
