@@ -601,18 +601,19 @@ void stage_bins_lines(sdf_ctx *c, LatticeRun &L, int part) {
   }
   if (part & 2) {
     if (c->lines_early == 2) SDF_HIP(hipStreamWaitEvent(c->aux[0], c->join[3], 0));
+    c->lb.same_masks = c->sign_split != 0;
     sdf::line_hits_finish(c->aux[0], c->ar, c->tri, c->nel, L.g, c->lb, L.bins, c->caps.hits);
     // the bricks whose lines all agree: marked in the brick plan (written on
     // helper stream 1), the others listed, for a split sign stage (sdf
     // output with distances by tile only)
-    SDF_HIP(hipStreamWaitEvent(c->aux[0], c->ev_plan, 0));
     const bool splittable = c->sign_split && L.d2buf && L.bins.d2_by_tile && L.out.sdf &&
                             !L.out.dist && !L.out.signs && !L.out.conf;
-    if (splittable)
+    if (splittable) {
+      SDF_HIP(hipStreamWaitEvent(c->aux[0], c->ev_plan, 0));
       L.rest = c->ar.get<uint32_t>("sign_rest", (size_t)sdf::brick_plan_size(L.g) + 1);
-    L.split = sdf::launch_brick_same(c->aux[0], L.g, L.bins, L.plan,
-                                     splittable ? L.rest : nullptr, c->d_words + kRestWord) &&
-              splittable;
+      L.split = sdf::launch_brick_same(c->aux[0], L.g, L.bins, L.plan, L.rest,
+                                       c->d_words + kRestWord);
+    }
     SDF_HIP(hipEventRecord(c->join[1], c->aux[0]));
   }
   c->ar.scope.clear();
@@ -1114,6 +1115,7 @@ void batch_issue(sdf_batch *b, int i) {
   k->chunk_len = b->tune.chunk_len;
   k->ray_stack = b->tune.ray_stack;
   k->lbvh_early = kBatchLbvh;
+  k->sign_split = 0;  // (the batch's lanes overlap their passes already)
   k->d_status = b->d_status + 16 * (int64_t)i;
   k->nev = 0;
   reset_words(k, true);
@@ -1137,6 +1139,7 @@ void batch_redo(sdf_batch *b, int i) {
   k->chunk_len = b->tune.chunk_len;
   k->ray_stack = b->tune.ray_stack;
   k->lbvh_early = kBatchLbvh;
+  k->sign_split = 0;  // (the batch's lanes overlap their passes already)
   k->d_status = b->d_status + 16 * (int64_t)i;
   k->nev = 0;
   reset_words(k, true);

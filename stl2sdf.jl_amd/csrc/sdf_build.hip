@@ -922,6 +922,12 @@ __global__ void line_prep_kernel(GridP g, const uint64_t *__restrict__ line_off,
                                  uint32_t *__restrict__ segw, uint64_t *__restrict__ lmask) {
 #pragma clang fp contract(off)
   const double u = 1.1102230246251565e-16;
+  // the sentinel read by lanes whose line serves no point (byte 1
+  // everywhere), and the one of byte 7 (bricks whose lines all carry 7)
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    segw[sl.total] = 0x01010101u;
+    segw[sl.total + 1] = 0x07070707u;
+  }
   for (int64_t L = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; L < nlines;
        L += (int64_t)gridDim.x * blockDim.x) {
     const uint64_t hb = line_off[L], he = line_off[L + 1];
@@ -1394,7 +1400,7 @@ void line_hits_finish(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
                        (uint64_t)cap, tri, lb.nlx, lb.nlx + lb.nly, fast);
   // the lines' byte-1 / byte-7 segment masks (launch_brick_same; lines of
   // at most 128 segments)
-  const bool masks = sl.ns[0] <= 128 && sl.ns[1] <= 128 && sl.ns[2] <= 128;
+  const bool masks = lb.same_masks && sl.ns[0] <= 128 && sl.ns[1] <= 128 && sl.ns[2] <= 128;
   uint64_t *lmask = masks ? ar.get<uint64_t>("line_same_mask", 4 * (size_t)lb.nlines + 4) : nullptr;
   hipLaunchKernelGGL(line_prep_kernel, dim3(capped_grid(lb.nlines, 256, lb.max_blocks)),
                      dim3(256), 0, s, g, (const uint64_t *)lb.loff, (uint64_t)cap, lb.nlx,
@@ -1415,10 +1421,7 @@ void line_hits_finish(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
                        sd.sz);
   }
   b.same_rows = rows;
-  // the sentinel read by lanes whose line serves no point (byte 1 everywhere),
-  // and the one of byte 7 (bricks whose lines all carry byte 7)
-  launch_fill_u32(s, segw + sl.total, 1, 0x01010101u);
-  launch_fill_u32(s, segw + sl.total + 1, 1, 0x07070707u);
+
   SDF_HIP(hipGetLastError());
   b.line_off = lb.loff;
   b.seg_words = segw;

@@ -283,6 +283,7 @@ struct LineBuild {
   int64_t max_blocks = 0;  // grid cap of the per-triangle and per-line kernels (0: none)
   uint8_t *hitn = nullptr;   // per triangle: hits found by the counting pass (255: many)
   uint32_t *hitl = nullptr;  // their line ids, 4 per triangle
+  bool same_masks = false;   // the brick rows' byte-1 / byte-7 masks (launch_brick_same)
 };
 void tile_bins_begin(hipStream_t s, Arena &ar, const Tri64 *tri, int64_t nel,
                      const GridP &g, double radius, TileBuild &tb);
