@@ -33,10 +33,11 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_PROFILE = os.path.join(REPO, "profiles", "fused_lattice_star512_hbm_traffic.md")
 
 
-# the sources the fused kernel is compiled from; the traffic summary records
-# their hash, and a summary measured on other sources is not reported
+# the sources of the sign stage's two kernels and of the brick marks that
+# split the bricks between them; the traffic summary records their hash, and a
+# summary measured on other sources is not reported
 KERNEL_SOURCES = ["stl2sdf.jl_amd/csrc/sdf_query.hip", "stl2sdf.jl_amd/csrc/sdf_common.h",
-                  "stl2sdf.jl_amd/csrc/sdf_internal.h"]
+                  "stl2sdf.jl_amd/csrc/sdf_internal.h", "stl2sdf.jl_amd/csrc/sdf_build.hip"]
 
 
 def kernel_source_hash():
@@ -49,8 +50,8 @@ def kernel_source_hash():
 
 
 def measured_traffic(workload):
-    """HBM bytes per launch of the fused kernel from the committed counter
-    summary (FETCH_SIZE / WRITE_SIZE passes, copy-calibrated), with the
+    """HBM bytes per step of the sign stage's two kernels (fused_list_kernel,
+    fused_same_kernel) from the committed counter summary (FETCH_SIZE / WRITE_SIZE passes, copy-calibrated), with the
     summary's source hash and commit.  traffic is None when the summary does
     not cover this workload or was measured on other kernel sources."""
     info = {"traffic": None, "traffic_profile": os.path.relpath(TRAFFIC_PROFILE, REPO),
@@ -60,7 +61,7 @@ def measured_traffic(workload):
         return info
     value = None
     for line in open(TRAFFIC_PROFILE):
-        if line.startswith("fused_lattice_kernel traffic_bytes_per_launch:"):
+        if line.startswith("fused_lattice traffic_bytes_per_step:"):
             value = float(line.split(":")[1])
         elif line.startswith("kernel_source_hash:"):
             info["traffic_source_hash"] = line.split(":")[1].strip()
@@ -416,7 +417,8 @@ def main():
                      "winding_points": cnt[5], "margin_points": cnt[6]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "kernel": "fused_lattice_kernel", "alg_bytes_per_launch": alg_bytes,
+                     "kernel": "fused_list_kernel + fused_same_kernel (sign stage)",
+                     "alg_bytes_per_step": alg_bytes,
                      "fused_ms": fused_ms, **measured_traffic(args.workload)},
     }
     res["paths"] = {"value": "plain GPU path (the drop-in default)",

@@ -956,17 +956,43 @@ void fused_lattice_kernel(FusedJob J) {
 }
 
 // The bricks launch_brick_same marked (every line byte 1 or every line byte
-// 7), one per workgroup; the others return after their plan's flags.
+// 7), one per workgroup; the others return after their plan's flags.  A
+// marked brick without distance work (plan flag bit 0 clear: no triangle
+// near any of its points) is h from the surface at every point, outside
+// (byte 1) or inside (byte 7): it only stores, with no load after its plan.
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(8)))
 void fused_same_kernel(FusedJob J) {
   __shared__ FusedShared S;
   __shared__ FusedCold C;
   if (threadIdx.x == 0) C = J.cold;
   wave_lds_sync();
+  const GridP &g = J.g;
+  const int lane = threadIdx.x;
+  const int64_t plane = (int64_t)g.nx * g.ny;
   const uint32_t nb = (uint32_t)J.nbricks;
   for (uint32_t m = (uint32_t)xcd_remap(blockIdx.x, gridDim.x); m < nb; m += gridDim.x) {
-    const uint32_t flags = kconst(reinterpret_cast<const uint32_t *>(J.plan + m))[6];
-    if (flags & 12u) fused_brick<kOutSdf>(J, (int64_t)m, S, C);
+    // BrickPlan words 4-6: xy, z, flags (scalar loads)
+    const KConst<uint32_t> pw = kconst(reinterpret_cast<const uint32_t *>(J.plan + m));
+    const uint32_t flags = pw[6];
+    if (!(flags & 12u)) continue;
+    if (flags & 1u) {
+      fused_brick<kOutSdf>(J, (int64_t)m, S, C);
+      continue;
+    }
+    const uint32_t xy = pw[4], bz = pw[5];
+    const int32_t i = (int32_t)(xy & 0xFFFFu) * 8 + (lane & 7);
+    const int32_t j = (int32_t)(xy >> 16) * 8 + (lane >> 3);
+    const int32_t k0 = (int32_t)bz * kSignZ;
+    const int nk = min(kSignZ, g.nz - k0);
+    const double v = (flags & 8u) ? g.h : -g.h;
+    if (i < g.nx && j < g.ny) {
+      const int64_t idx0 = ((int64_t)k0 * g.ny + j) * g.nx + i;
+#pragma unroll
+      for (int q = 0; q < kSignZ; q++) {
+        if (q >= nk) break;  // uniform per wave
+        J.out.sdf[idx0 + q * plane] = v;
+      }
+    }
   }
 }
 
