@@ -69,7 +69,9 @@ sdf_status sdf_ctx_synchronize(sdf_ctx *ctx);
  * of the distance / sign kernels (0: one per work item; defaults 7 / 0),
  * "sign_split" 1: the sign stage's bricks whose lines all agree run on a
  * helper stream beside the vote (default 1), "same_per_cu" workgroups per CU
- * of that pass (0: one per brick, the default),
+ * of that pass (0: one per brick, the default), "wind_ahead" 1: the
+ * winding sums of the vote's likely winding items run beside its ray phases
+ * (0: only after the vote, for the items it leaves; default 0),
  * "lbvh_early" when the LBVH build runs: 0 after the tile binning, issued
  * after the distance and axis-line work, 1 the same, issued before it, 2
  * after the axis-line hits end, 3 beside the tile binning (default 3),

@@ -196,6 +196,13 @@ struct sdf_ctx {
   // when the vote needs it, 0.14 ms wait)
   int lbvh_early = 3;
   int sign_split = 1;  // split sign stage when the bricks are marked (0: one pass)
+  // winding sums of the vote's likely winding items issued on helper stream
+  // 0 beside the ray phases (0: only after the vote, for the items it
+  // leaves).  On a closed surface the vote leaves next to none (one point of
+  // 82.6 M on the star surface), and the sums issued ahead slow the ray
+  // phases and the marked-brick pass beside them: 0.959-0.963 ms per step
+  // without against 0.969-0.973 ms with
+  int wind_ahead = 0;
   // workgroups per CU of its marked-brick pass (0: one per brick; capping it
   // to leave wave slots for the vote beside it was slower: 8, 16, 24 per CU
   // gave 1.04, 1.01, 1.03 against 0.99 ms per step)
@@ -395,9 +402,10 @@ void resolve_items(sdf_ctx *c, const sdf::GridP &g, const double *d_pts,
   uint32_t *nwind = cnt + 2;
   int64_t chunk = 0;
   const int chunks = sdf::winding_chunks(c->nel, &chunk);
+  const bool early = wind && c->wind_ahead;
   const uint32_t rows_spec =
-      wind && spec >= 0 ? std::min<uint32_t>(first.cap[spec], kWindAheadSlots) : 0u;
-  const uint32_t rows_last = wind && np > 1 ? std::min<uint32_t>(cap, kWindAheadSlots) : 0u;
+      early && spec >= 0 ? std::min<uint32_t>(first.cap[spec], kWindAheadSlots) : 0u;
+  const uint32_t rows_last = early && np > 1 ? std::min<uint32_t>(cap, kWindAheadSlots) : 0u;
   double *ahead = rows_spec + rows_last > 0
                       ? c->ar.get<double>("res_wahead", (size_t)(rows_spec + rows_last) * chunks)
                       : nullptr;
@@ -1308,6 +1316,9 @@ sdf_status sdf_ctx_tune(sdf_ctx *c, const char *name, int64_t value) {
     } else if (n == "sign_split") {
       if (value != 0 && value != 1) fail(SDF_ERR_ARG, "sign_split must be 0 or 1");
       c->sign_split = (int)value;
+    } else if (n == "wind_ahead") {
+      if (value != 0 && value != 1) fail(SDF_ERR_ARG, "wind_ahead must be 0 or 1");
+      c->wind_ahead = (int)value;
     } else if (n == "learned_force") {
       if (value != 0 && value != 1) fail(SDF_ERR_ARG, "learned_force must be 0 or 1");
       c->learned_force = value != 0;

@@ -29,7 +29,7 @@ def _run(h, mesh, grid):
                                         ("lines_early", 1), ("lines_early", 2),
                                         ("line_per_cu", 1), ("lbvh_early", 0),
                                         ("lbvh_early", 1), ("lbvh_early", 2),
-                                        ("lbvh_early", 3), ("sign_split", 0),
+                                        ("lbvh_early", 3), ("sign_split", 0), ("wind_ahead", 1),
                                         ("same_per_cu", 4)])
 def test_tunable_keeps_results(name, value):
     mesh, grid = _cases.star_case(n=60, n_max=45)
@@ -51,7 +51,7 @@ def test_tunable_errors():
     L = _lib.lib()
     c = _lib.ctx(0)
     for name, value in (("chunk_len", 0), ("chunk_len", 65), ("ray_stack", 512),
-                        ("dist_per_cu", -1), ("lines_early", 3), ("lbvh_early", 4), ("sign_split", 2), ("same_per_cu", 65),
+                        ("dist_per_cu", -1), ("lines_early", 3), ("lbvh_early", 4), ("sign_split", 2), ("wind_ahead", 2), ("same_per_cu", 65),
                         ("no_such_knob", 1)):
         with pytest.raises(_lib.SdfError):
             _lib.check(L.sdf_ctx_tune(c, name.encode(), value))
