@@ -1122,41 +1122,68 @@ __global__ __launch_bounds__(256) void brick_same_flags_kernel(
     BrickPlan *__restrict__ plan, int64_t nbricks, const uint64_t *__restrict__ xm,
     const uint64_t *__restrict__ ym, const uint64_t *__restrict__ zm, int32_t nbx, int32_t nby,
     uint32_t *__restrict__ rest, uint32_t *nrest) {
+  // kPer bricks per thread (m0 + r * 256 + thread), and the unmarked ones
+  // listed with one counter atomic per workgroup: a wave's atomic on the one
+  // counter each (5 K of them on the star surface's 322 K bricks) took most
+  // of this pass's 20 us
+  constexpr int kPer = 4;
+  __shared__ uint32_t wcnt[4], bbase;
   auto bit = [](const uint64_t *M, int32_t s, int which) -> uint32_t {
     const uint64_t w = s < 64 ? M[2 * which] : M[2 * which + 1];
     return (uint32_t)((w >> (s & 63)) & 1ull);
   };
-  const int lane = threadIdx.x & 63;
-  for (int64_t m0 = (int64_t)blockIdx.x * blockDim.x; m0 < nbricks;
-       m0 += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t m = m0 + threadIdx.x;
-    bool other = false;
-    if (m < nbricks) {
-      // xy, z, flags, pad in one 16-byte load
-      const uint4 t = reinterpret_cast<const uint4 *>(plan + m)[1];
-      const uint32_t flags = t.z;
-      if (flags & 2u) {  // not a padding position
-        const int32_t bx = (int32_t)(t.x & 0xFFFFu), by = (int32_t)(t.x >> 16),
-                      bz = (int32_t)t.y;
-        const uint64_t *X = xm + 4 * ((int64_t)bz * nby + by);
-        const uint64_t *Y = ym + 4 * ((int64_t)bz * nbx + bx);
-        const uint64_t *Z = zm + 4 * ((int64_t)by * nbx + bx);
-        const uint32_t s1 = bit(X, bx, 0) & bit(Y, by, 0) & bit(Z, bz, 0);
-        const uint32_t s7 = bit(X, bx, 1) & bit(Y, by, 1) & bit(Z, bz, 1);
-        if (s1 | s7)
-          plan[m].flags = flags | (s1 << 2) | (s7 << 3);
-        else
-          other = true;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int64_t m0 = (int64_t)blockIdx.x * (256 * kPer); m0 < nbricks;
+       m0 += (int64_t)gridDim.x * (256 * kPer)) {
+    bool other[kPer];
+#pragma unroll
+    for (int r = 0; r < kPer; r++) {
+      const int64_t m = m0 + r * 256 + threadIdx.x;
+      other[r] = false;
+      if (m < nbricks) {
+        // xy, z, flags, pad in one 16-byte load
+        const uint4 t = reinterpret_cast<const uint4 *>(plan + m)[1];
+        const uint32_t flags = t.z;
+        if (flags & 2u) {  // not a padding position
+          const int32_t bx = (int32_t)(t.x & 0xFFFFu), by = (int32_t)(t.x >> 16),
+                        bz = (int32_t)t.y;
+          const uint64_t *X = xm + 4 * ((int64_t)bz * nby + by);
+          const uint64_t *Y = ym + 4 * ((int64_t)bz * nbx + bx);
+          const uint64_t *Z = zm + 4 * ((int64_t)by * nbx + bx);
+          const uint32_t s1 = bit(X, bx, 0) & bit(Y, by, 0) & bit(Z, bz, 0);
+          const uint32_t s7 = bit(X, bx, 1) & bit(Y, by, 1) & bit(Z, bz, 1);
+          if (s1 | s7)
+            plan[m].flags = flags | (s1 << 2) | (s7 << 3);
+          else
+            other[r] = true;
+        }
       }
     }
-    if (rest) {
-      const uint64_t bal = __ballot(other);
-      if (bal) {
-        uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(nrest, (uint32_t)__popcll(bal));
-        base = __shfl(base, 0);
-        if (other) rest[base + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull))] = (uint32_t)m;
+    if (rest) {  // uniform
+      uint64_t bal[kPer];
+      uint32_t tot = 0;
+#pragma unroll
+      for (int r = 0; r < kPer; r++) {
+        bal[r] = __ballot(other[r]);
+        tot += (uint32_t)__popcll(bal[r]);
       }
+      if (lane == 0) wcnt[wv] = tot;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        const uint32_t sum = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+        bbase = sum ? atomicAdd(nrest, sum) : 0u;
+      }
+      __syncthreads();
+      uint32_t base = bbase;
+      for (int k = 0; k < wv; k++) base += wcnt[k];
+      const uint64_t below = (1ull << lane) - 1ull;
+#pragma unroll
+      for (int r = 0; r < kPer; r++) {
+        if (other[r])
+          rest[base + (uint32_t)__popcll(bal[r] & below)] = (uint32_t)(m0 + r * 256 + threadIdx.x);
+        base += (uint32_t)__popcll(bal[r]);
+      }
+      __syncthreads();  // wcnt and bbase are rewritten by the next round
     }
   }
 }
@@ -1438,7 +1465,8 @@ bool launch_brick_same(hipStream_t s, const GridP &g, const Bins &b, BrickPlan *
   if (!b.same_rows.x || !plan) return false;
   const int64_t nbricks = brick_plan_size(g);
   if (nbricks <= 0) return false;
-  hipLaunchKernelGGL(brick_same_flags_kernel, dim3(grid_for(nbricks, 256)), dim3(256), 0, s, plan,
+  hipLaunchKernelGGL(brick_same_flags_kernel, dim3(grid_for((nbricks + 3) / 4, 256)), dim3(256), 0,
+                     s, plan,
                      nbricks, (const uint64_t *)b.same_rows.x, (const uint64_t *)b.same_rows.y,
                      (const uint64_t *)b.same_rows.z, b.same_rows.nbx, b.same_rows.nby, rest,
                      nrest);
